@@ -1,0 +1,189 @@
+/*
+ * rbmdx.h - C ABI of the MI355X-native RBMD force path (librbmdx.so).
+ *
+ * One context owns the device state of one periodic system on one GPU:
+ * cell-sorted copies of positions/charges/types, neighbour candidate lists,
+ * k-space scratch and the RBE k-sample stream.  Caller arrays are never
+ * reordered: inputs and outputs are in the caller's atom order, device
+ * pointers, SoA, exactly like RBMD's DeviceData (_d_px/_d_py/_d_pz,
+ * _d_fx/_d_fy/_d_fz).
+ *
+ * Every call that takes a `stream` is stream-ordered and performs no host
+ * synchronisation unless RBMDX_FLAG_SYNC is given.  A context is not
+ * thread-safe; use one per host thread.  No function calls exit(): errors are
+ * returned as status codes and described by rbmdx_last_error().
+ *
+ * Reference interfaces replaced (SOG-AITech/rbmd source tree):
+ *   rbmdx_create        Force plugin construction from JSON
+ *                       (src/force/include/force.h:10-23, factory map at
+ *                        src/simulate_pipeline/src/nve_ensemble.cpp:19-26,
+ *                        config keys of docs/json-configuration.en.md:75-167)
+ *   rbmdx_set_system    DeviceData upload (src/data_manager/src/memory_scheduler.cpp:16-220,
+ *                        src/data_manager/src/lj_memory_scheduler.cpp:3-36) and
+ *                       Force::Init (src/force/src/lj_cut_coul_kspace.cpp:424-462,
+ *                        src/force/src/kspace_calculator.cpp:159-355)
+ *   rbmdx_force         Force::Execute (src/force/src/lj_cut_coul_kspace.cpp:464-471,
+ *                        src/force/src/lj.cpp:54-69) including the neighbour
+ *                        builders it calls and the k-space calculator
+ *   rbmdx_thermo        Force::EvaluatePotentialEnergy + virial read-back
+ *                       (src/force/src/lj_cut_coul_kspace.cpp:867-897,
+ *                        src/force/src/kspace_calculator.cpp:648-669)
+ */
+#ifndef RBMDX_H
+#define RBMDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rbmdx_ctx rbmdx_ctx;
+
+/* status codes */
+enum {
+  RBMDX_OK = 0,
+  RBMDX_E_CONFIG = 1,    /* bad or unsupported JSON configuration          */
+  RBMDX_E_CAPACITY = 2,  /* a list outgrew its capacity (auto-grown; the
+                            forces of the call are still correct)          */
+  RBMDX_E_HIP = 3,       /* HIP runtime / launch failure                   */
+  RBMDX_E_NONFINITE = 4, /* non-finite force or energy detected            */
+  RBMDX_E_ARG = 5,       /* invalid argument                               */
+  RBMDX_E_STATE = 6      /* call out of order (e.g. force before system)   */
+};
+
+/* rbmdx_force flags */
+enum {
+  RBMDX_FLAG_ENERGY = 1,   /* also accumulate energies and the virial      */
+  RBMDX_FLAG_SYNC = 2,     /* synchronise and validate before returning    */
+  RBMDX_FLAG_REBUILD = 4   /* force a neighbour rebuild this call          */
+};
+
+/* Thermo record filled by rbmdx_thermo (all FP64, extensive totals). */
+typedef struct rbmdx_thermo_t {
+  double e_vdwl;      /* LJ pair energy (full list within cut_off)          */
+  double e_coul;      /* real-space erfc Coulomb energy                     */
+  double e_kspace;    /* k-space energy incl. self/background terms         */
+  double virial[6];   /* pair + k-space virial: xx yy zz xy xz yz           */
+  double virial_kspace[6];
+  double mean_force[3]; /* RBL mean pair force that was removed             */
+  int64_t steps;      /* force evaluations performed                        */
+  int64_t rebuilds;   /* neighbour rebuilds performed                       */
+  int32_t rbl_frequency; /* RBL shell selection frequency (pice_num)        */
+  int32_t num_k;      /* k-vectors used by the last evaluation              */
+  double alpha;       /* Ewald splitting parameter (RBMD "alpha" = g^2)    */
+  double rbe_S;       /* RBE normalisation S = sum_{k!=0} exp(-k^2/4alpha)  */
+} rbmdx_thermo_t;
+
+/*
+ * Create a context from an RBMD JSON configuration (the whole document, or
+ * just its "hyper_parameters" object).  Recognised keys:
+ *   init_configuration.read_data.unit   "LJ" | "REAL" | "METAL"
+ *   hyper_parameters.force_field.type   "LJ/CUT" | "LJ/CUT/COUL/LONG"
+ *   hyper_parameters.neighbor.{type,cut_off,skin,r_core,neighbor_sample_num,
+ *                              energy_rbl_flag}
+ *       type: "RBL" | "Verlet-List"
+ *   hyper_parameters.neighbor.lj_cut_off   (extension: separate LJ cutoff,
+ *                                           default cut_off)
+ *   hyper_parameters.coulomb.{type,alpha,accuracy,coulomb_sample_num,kmax,
+ *                             energy_rbe_flag,random_seed}
+ *       type: "RBE" | "EWALD"
+ *   hyper_parameters.precision             (extension) "fp32" | "fp64"
+ * `device` is the HIP device ordinal.
+ */
+int rbmdx_create(const char* json_text, int device, rbmdx_ctx** out);
+void rbmdx_destroy(rbmdx_ctx* ctx);
+const char* rbmdx_last_error(const rbmdx_ctx* ctx);
+
+/*
+ * Describe the system (host arrays, copied).  Box is orthogonal and periodic:
+ * [lo_d, lo_d + L_d).  type is 0-based (< ntypes); id is the 0-based global
+ * atom id used by the RBL shell hash; eps/sigma are per type (mixing:
+ * eps_ij = sqrt(eps_i eps_j), sigma_ij = (sigma_i + sigma_j)/2).
+ */
+int rbmdx_set_system(rbmdx_ctx* ctx, int64_t n, const double* box_lo,
+                     const double* box_len, const int32_t* type,
+                     const int32_t* id, const double* charge, int32_t ntypes,
+                     const double* eps, const double* sigma);
+
+/* Replace the charges (host array, n entries); keeps lists. */
+int rbmdx_set_charges(rbmdx_ctx* ctx, const double* charge);
+
+/*
+ * One force evaluation.  px/py/pz are device arrays of float (precision
+ * fp32) or double (precision fp64); fx/fy/fz are device arrays of the same
+ * type and receive the total force (pair + k-space).  `step` is the RBL
+ * sampling epoch (RBMD's test_current_step).  `m_samples` is NULL (draw the
+ * next RBE k-sample from the seeded stream) or a device array of num_k*3
+ * int32 integer k-vectors m (k = 2 pi m / L) used for this call only.
+ * `stream` is a hipStream_t (NULL = default stream).
+ */
+int rbmdx_force(rbmdx_ctx* ctx, int64_t step, const void* px, const void* py,
+                const void* pz, void* fx, void* fy, void* fz,
+                const int32_t* m_samples, int32_t num_k, uint32_t flags,
+                void* stream);
+
+/* Synchronise the context's last stream and read energies/virial. */
+int rbmdx_thermo(rbmdx_ctx* ctx, rbmdx_thermo_t* out);
+
+/*
+ * Host copy of the RBE k-sample the seeded stream yields for force evaluation
+ * `call_index` (0 = first rbmdx_force call).  m_out holds P*3 int32.
+ */
+int rbmdx_rbe_sample(rbmdx_ctx* ctx, int64_t call_index, int32_t* m_out);
+
+/* Number of RBE samples P per evaluation (0 if no RBE). */
+int32_t rbmdx_num_samples(const rbmdx_ctx* ctx);
+
+/*
+ * Parity export: rebuild the lists for the given positions (device arrays)
+ * and write, for every directed pair the RBL classification keeps at `step`,
+ * (gid_i, gid_j, cls) with cls = 1 core, 2 selected shell (Verlet mode: 1
+ * for every pair inside cut_off).  Host arrays of capacity `cap`; the return
+ * value is the pair count (negative status on error; if the count exceeds
+ * cap only cap pairs are written).
+ */
+int64_t rbmdx_export_pairs(rbmdx_ctx* ctx, int64_t step, const void* px,
+                           const void* py, const void* pz, int32_t* gi,
+                           int32_t* gj, int32_t* cls, int64_t cap);
+
+/*
+ * Parity export of the neighbour candidate lists (every directed pair with
+ * d^2 < (cut + skin)^2 as stored on the device) for the positions of the
+ * last rebuild.  Same conventions as rbmdx_export_pairs; cls = 1 for pairs
+ * of the all-atom list, 2 for pairs of the charged-pair list.
+ */
+int64_t rbmdx_export_candidates(rbmdx_ctx* ctx, int32_t* gi, int32_t* gj,
+                                int32_t* cls, int64_t cap);
+
+/*
+ * Host-only helpers (no device needed).
+ *
+ * rbmdx_rbe_sample_host: the k-sample the seeded stream yields for force
+ * evaluation `call_index` of a system with Ewald parameter `alpha` (RBMD
+ * "alpha" = g_ewald^2) and box lengths L[3]; replaces
+ * RbmdParallelUntil::GetRbeRadomM (src/parallel/src/rbmd_parallel_until.cpp:557-724)
+ * together with its seed chain (:305-325, :484-501).  m_out holds P*3 int32.
+ *
+ * rbmdx_check_config: parse and validate a JSON configuration the way
+ * rbmdx_create does (the checks of LJCutCoulKspace::Init and
+ * KSpaceCalculator::Init, src/force/src/lj_cut_coul_kspace.cpp:424-462 and
+ * src/force/src/kspace_calculator.cpp:159-355); writes the message of a
+ * failure into err (errlen bytes).  Returns a status code.
+ */
+int rbmdx_rbe_sample_host(double alpha, const double* L, int32_t P,
+                          uint32_t base_seed, int64_t call_index,
+                          int32_t* m_out);
+int rbmdx_check_config(const char* json_text, char* err, int32_t errlen);
+
+/* Device-kernel timing of the last call: fills up to n entries of
+ * (name, microseconds) when the context was created with timing enabled
+ * (RBMDX_TIMING=1 in the environment).  Returns the number of kernels. */
+int rbmdx_kernel_times(rbmdx_ctx* ctx, const char** names, double* usec,
+                       int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBMDX_H */

@@ -1,0 +1,264 @@
+"""rbmd_amd: MI355X-native RBMD force path (cell list + RBL pair + RBE k-space).
+
+The compute lives in ``librbmdx.so`` (hand-written HIP kernels for gfx950 and a
+C++ host engine) behind the C ABI declared in ``include/rbmdx.h``.  This module
+is a thin ctypes binding over that ABI; device arrays are passed as raw
+pointers (torch tensors are used only for device memory and streams).
+
+There is no CPU fallback: if the library is missing, :func:`load_library`
+raises.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+__all__ = ["load_library", "ForceContext", "RbmdxError", "LIB_PATH"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librbmdx.so")
+
+RBMDX_FLAG_ENERGY = 1
+RBMDX_FLAG_SYNC = 2
+RBMDX_FLAG_REBUILD = 4
+
+_STATUS = {
+    0: "RBMDX_OK",
+    1: "RBMDX_E_CONFIG",
+    2: "RBMDX_E_CAPACITY",
+    3: "RBMDX_E_HIP",
+    4: "RBMDX_E_NONFINITE",
+    5: "RBMDX_E_ARG",
+    6: "RBMDX_E_STATE",
+}
+
+
+class RbmdxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Thermo(ctypes.Structure):
+    _fields_ = [
+        ("e_vdwl", ctypes.c_double),
+        ("e_coul", ctypes.c_double),
+        ("e_kspace", ctypes.c_double),
+        ("virial", ctypes.c_double * 6),
+        ("virial_kspace", ctypes.c_double * 6),
+        ("mean_force", ctypes.c_double * 3),
+        ("steps", ctypes.c_int64),
+        ("rebuilds", ctypes.c_int64),
+        ("rbl_frequency", ctypes.c_int32),
+        ("num_k", ctypes.c_int32),
+        ("alpha", ctypes.c_double),
+        ("rbe_S", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {
+            "e_vdwl": self.e_vdwl,
+            "e_coul": self.e_coul,
+            "e_kspace": self.e_kspace,
+            "virial": list(self.virial),
+            "virial_kspace": list(self.virial_kspace),
+            "mean_force": list(self.mean_force),
+            "steps": self.steps,
+            "rebuilds": self.rebuilds,
+            "rbl_frequency": self.rbl_frequency,
+            "num_k": self.num_k,
+            "alpha": self.alpha,
+            "rbe_S": self.rbe_S,
+        }
+
+
+# Every symbol include/rbmdx.h declares, with its ctypes signature.
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int32)
+SIGNATURES = {
+    "rbmdx_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_P)]),
+    "rbmdx_destroy": (None, [_P]),
+    "rbmdx_last_error": (ctypes.c_char_p, [_P]),
+    "rbmdx_set_system": (ctypes.c_int, [_P, _I64, _DP, _DP, _IP, _IP, _DP, _I32, _DP, _DP]),
+    "rbmdx_set_charges": (ctypes.c_int, [_P, _DP]),
+    "rbmdx_force": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _I32,
+                                   ctypes.c_uint32, _P]),
+    "rbmdx_thermo": (ctypes.c_int, [_P, ctypes.POINTER(Thermo)]),
+    "rbmdx_rbe_sample": (ctypes.c_int, [_P, _I64, _IP]),
+    "rbmdx_num_samples": (_I32, [_P]),
+    "rbmdx_export_pairs": (_I64, [_P, _I64, _P, _P, _P, _IP, _IP, _IP, _I64]),
+    "rbmdx_export_candidates": (_I64, [_P, _IP, _IP, _IP, _I64]),
+    "rbmdx_kernel_times": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_char_p), _DP,
+                                          ctypes.c_int]),
+    "rbmdx_rbe_sample_host": (ctypes.c_int, [ctypes.c_double, _DP, _I32, ctypes.c_uint32,
+                                             _I64, _IP]),
+    "rbmdx_check_config": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _I32]),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load librbmdx.so and bind every ABI symbol (raises if anything is missing)."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: run __graft_entry__.build() first")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)  # AttributeError if the symbol is absent
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def rbe_sample_host(alpha, L, P, base_seed, call_index):
+    """Host k-sample of evaluation `call_index` (P x 3 int32), no device needed."""
+    lib = load_library()
+    Ld = np.ascontiguousarray(L, dtype=np.float64)
+    m = np.zeros(3 * P, dtype=np.int32)
+    rc = lib.rbmdx_rbe_sample_host(float(alpha), _dptr(Ld), int(P), int(base_seed) & 0xFFFFFFFF,
+                                   int(call_index), _iptr(m))
+    if rc != 0:
+        raise RbmdxError(rc, "rbmdx_rbe_sample_host failed")
+    return m.reshape(P, 3)
+
+
+def check_config(config):
+    """Validate a JSON configuration without a device: (status, message)."""
+    lib = load_library()
+    text = config if isinstance(config, str) else json.dumps(config)
+    buf = ctypes.create_string_buffer(512)
+    rc = lib.rbmdx_check_config(text.encode(), buf, 512)
+    return rc, buf.value.decode()
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_IP)
+
+
+def _tensor_ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class ForceContext:
+    """One periodic system on one GPU (wraps an ``rbmdx_ctx``)."""
+
+    def __init__(self, config, device=0):
+        self.lib = load_library()
+        text = config if isinstance(config, str) else json.dumps(config)
+        h = _P()
+        rc = self.lib.rbmdx_create(text.encode(), int(device), ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self.lib.rbmdx_last_error(h).decode() if h else ""
+            self.lib.rbmdx_destroy(h)
+            self._h = None
+            raise RbmdxError(rc, msg)
+        self.device = device
+        self.config = json.loads(text)
+        hp = self.config.get("hyper_parameters", self.config)
+        self.fp64 = hp.get("precision", "fp32") == "fp64"
+        self.n = 0
+        self.calls = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.rbmdx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RbmdxError(rc, self.lib.rbmdx_last_error(self._h).decode())
+
+    def set_system(self, box_lo, box_len, types, ids, charges, eps, sigma):
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        charges = np.ascontiguousarray(charges, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        lo = np.ascontiguousarray(box_lo, dtype=np.float64)
+        L = np.ascontiguousarray(box_len, dtype=np.float64)
+        self.n = len(types)
+        self._check(self.lib.rbmdx_set_system(
+            self._h, self.n, _dptr(lo), _dptr(L), _iptr(types), _iptr(ids),
+            _dptr(charges), len(eps), _dptr(eps), _dptr(sigma)))
+
+    def set_charges(self, charges):
+        q = np.ascontiguousarray(charges, dtype=np.float64)
+        self._check(self.lib.rbmdx_set_charges(self._h, _dptr(q)))
+
+    def force(self, step, px, py, pz, fx, fy, fz, m_samples=None, energy=False,
+              sync=False, rebuild=False, stream=None):
+        """Total force into fx/fy/fz (device tensors, caller order)."""
+        flags = ((RBMDX_FLAG_ENERGY if energy else 0) | (RBMDX_FLAG_SYNC if sync else 0)
+                 | (RBMDX_FLAG_REBUILD if rebuild else 0))
+        nk = 0 if m_samples is None else int(m_samples.numel() // 3)
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(px.device).cuda_stream
+        self._check(self.lib.rbmdx_force(
+            self._h, int(step), _tensor_ptr(px), _tensor_ptr(py), _tensor_ptr(pz),
+            _tensor_ptr(fx), _tensor_ptr(fy), _tensor_ptr(fz),
+            _tensor_ptr(m_samples), nk, flags, ctypes.c_void_p(stream)))
+        self.calls += 1
+
+    def thermo(self):
+        t = Thermo()
+        self._check(self.lib.rbmdx_thermo(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def num_samples(self):
+        return int(self.lib.rbmdx_num_samples(self._h))
+
+    def rbe_sample(self, call_index):
+        P = self.num_samples()
+        m = np.zeros(3 * P, dtype=np.int32)
+        self._check(self.lib.rbmdx_rbe_sample(self._h, int(call_index), _iptr(m)))
+        return m.reshape(P, 3)
+
+    def _export(self, fn, *args, cap):
+        gi = np.zeros(cap, dtype=np.int32)
+        gj = np.zeros(cap, dtype=np.int32)
+        cls = np.zeros(cap, dtype=np.int32)
+        n = fn(self._h, *args, _iptr(gi), _iptr(gj), _iptr(cls), cap)
+        if n < 0:
+            self._check(int(-n))
+        if n > cap:
+            return self._export(fn, *args, cap=int(n))
+        return gi[:n], gj[:n], cls[:n]
+
+    def export_pairs(self, step, px, py, pz, cap=1 << 20):
+        """(gid_i, gid_j, cls) of every directed pair the RBL classification keeps."""
+        return self._export(self.lib.rbmdx_export_pairs, int(step), _tensor_ptr(px),
+                            _tensor_ptr(py), _tensor_ptr(pz), cap=cap)
+
+    def export_candidates(self, cap=1 << 20):
+        return self._export(self.lib.rbmdx_export_candidates, cap=cap)
+
+    def kernel_times(self):
+        names = (ctypes.c_char_p * 8)()
+        us = np.zeros(8, dtype=np.float64)
+        n = self.lib.rbmdx_kernel_times(self._h, names, _dptr(us), 8)
+        if n < 0:
+            self._check(-n)
+        return {names[k].decode(): float(us[k]) for k in range(n)}

@@ -1,0 +1,1028 @@
+// Host engine behind the C ABI of include/rbmdx.h.
+//
+// A context parses RBMD's JSON configuration (the keys RBMD's force modules
+// read: src/force/src/lj_cut_coul_kspace.cpp:424-462,
+// src/force/src/kspace_calculator.cpp:159-355,
+// src/neighbor_list/src/neighbor_list_builder/rbl_full_neighbor_list_builder.cpp:43-77),
+// owns every device buffer of one system, and launches one stream-ordered
+// force evaluation per rbmdx_force call.  Nothing is allocated inside a force
+// call except on the first call (list capacity planning) or when a list
+// outgrows its capacity (growth, which the kernels survive by falling back to
+// a direct stencil scan for the affected atoms).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "json.hpp"
+#include "rbmdx.h"
+#include "sampler.hpp"
+
+using namespace rbmdx;
+
+namespace {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      throw Error(RBMDX_E_HIP, std::string(#expr) + ": " +                   \
+                                   hipGetErrorString(e_));                   \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Configuration (RBMD JSON surface; docs/json-configuration.en.md:75-167)
+// ---------------------------------------------------------------------------
+struct Config {
+  std::string force_type;     // "LJ/CUT" | "LJ/CUT/COUL/LONG"
+  std::string neighbor_type;  // "RBL" | "Verlet-List"
+  double cut_off = 0, skin = 0, r_core = 0, lj_cut_off = 0;
+  int sample_num = 0;         // neighbor_sample_num
+  bool energy_rbl = false;
+  std::string coulomb_type;   // "" | "RBE" | "EWALD"
+  bool has_alpha = false;
+  double alpha = 0, accuracy = 0;
+  int coulomb_sample_num = 0;
+  bool has_kmax = false;
+  int kmax[3] = {0, 0, 0};
+  bool energy_rbe = false;
+  bool has_seed = false;
+  uint32_t seed = 0;
+  double qqr2e = 1.0;
+  bool fp64 = false;
+};
+
+double num_of(const Json* j, const char* what) {
+  if (!j || j->kind != Json::Number)
+    throw Error(RBMDX_E_CONFIG, std::string("missing or non-numeric ") + what);
+  return j->num;
+}
+std::string str_of(const Json* j, const char* what) {
+  if (!j || j->kind != Json::String)
+    throw Error(RBMDX_E_CONFIG, std::string("missing or non-string ") + what);
+  return j->str;
+}
+
+Config parse_config(const std::string& text) {
+  Json root;
+  try {
+    root = parse_json(text);
+  } catch (const std::exception& e) {
+    throw Error(RBMDX_E_CONFIG, e.what());
+  }
+  const Json* hp = root.get("hyper_parameters");
+  if (!hp) hp = &root;  // a bare hyper_parameters object
+  Config c;
+  if (const Json* u = root.at({"init_configuration", "read_data", "unit"})) {
+    const std::string unit = str_of(u, "unit");
+    if (unit == "LJ")
+      c.qqr2e = 1.0;
+    else if (unit == "REAL")
+      c.qqr2e = 332.06371;
+    else if (unit == "METAL")
+      c.qqr2e = 14.399645;
+    else
+      throw Error(RBMDX_E_CONFIG, "unsupported unit style: " + unit);
+  }
+  c.force_type = str_of(hp->at({"force_field", "type"}), "force_field.type");
+  if (c.force_type != "LJ/CUT" && c.force_type != "LJ/CUT/COUL/LONG")
+    throw Error(RBMDX_E_CONFIG,
+                "force_field.type not supported by this build: " + c.force_type);
+  const Json* nb = hp->get("neighbor");
+  if (!nb) throw Error(RBMDX_E_CONFIG, "hyper_parameters.neighbor missing");
+  c.neighbor_type = str_of(nb->get("type"), "neighbor.type");
+  if (c.neighbor_type != "RBL" && c.neighbor_type != "Verlet-List")
+    throw Error(RBMDX_E_CONFIG,
+                "neighbor.type not supported by this build: " + c.neighbor_type);
+  c.cut_off = num_of(nb->get("cut_off"), "neighbor.cut_off");
+  if (!(c.cut_off > 0)) throw Error(RBMDX_E_CONFIG, "neighbor.cut_off must be > 0");
+  if (const Json* s = nb->get("skin")) c.skin = num_of(s, "neighbor.skin");
+  if (!(c.skin >= 0) || !std::isfinite(c.skin))
+    throw Error(RBMDX_E_CONFIG, "neighbor.skin must be finite and >= 0");
+  c.lj_cut_off = c.cut_off;
+  if (const Json* l = nb->get("lj_cut_off"))
+    c.lj_cut_off = num_of(l, "neighbor.lj_cut_off");
+  if (!(c.lj_cut_off > 0))
+    throw Error(RBMDX_E_CONFIG, "neighbor.lj_cut_off must be > 0");
+  if (c.neighbor_type == "RBL") {
+    c.r_core = num_of(nb->get("r_core"), "neighbor.r_core");
+    const double p = num_of(nb->get("neighbor_sample_num"),
+                            "neighbor.neighbor_sample_num");
+    if (!(p > 0))
+      throw Error(RBMDX_E_CONFIG, "neighbor_sample_num must be larger than 0");
+    c.sample_num = (int)p;
+    if (!(c.r_core > 0 && c.r_core < c.cut_off))
+      throw Error(RBMDX_E_CONFIG, "r_core must be > 0 and less than the cutoff");
+    const Json* ef = nb->get("energy_rbl_flag");
+    if (!ef)
+      throw Error(RBMDX_E_CONFIG,
+                  "when using RBL for the neighbor type, the key "
+                  "'energy_rbl_flag' must be defined");
+    c.energy_rbl = str_of(ef, "energy_rbl_flag") == "yes";
+  }
+  if (const Json* pr = hp->get("precision")) {
+    const std::string p = str_of(pr, "precision");
+    if (p == "fp64")
+      c.fp64 = true;
+    else if (p != "fp32")
+      throw Error(RBMDX_E_CONFIG, "precision must be fp32 or fp64");
+  }
+  const Json* cl = hp->get("coulomb");
+  if (c.force_type == "LJ/CUT/COUL/LONG") {
+    if (!cl) throw Error(RBMDX_E_CONFIG, "LJ/CUT/COUL/LONG needs hyper_parameters.coulomb");
+    c.coulomb_type = str_of(cl->get("type"), "coulomb.type");
+    if (c.coulomb_type != "RBE" && c.coulomb_type != "EWALD")
+      throw Error(RBMDX_E_CONFIG,
+                  "coulomb.type not supported by this build: " + c.coulomb_type);
+    if (const Json* a = cl->get("alpha")) {
+      c.has_alpha = true;
+      c.alpha = num_of(a, "coulomb.alpha");
+      if (!(c.alpha > 0))
+        throw Error(RBMDX_E_CONFIG, "coulomb alpha override must be > 0");
+    } else {
+      const Json* acc = cl->get("accuracy");
+      if (!acc)
+        throw Error(RBMDX_E_CONFIG,
+                    "coulomb accuracy must be defined when alpha is derived");
+      c.accuracy = num_of(acc, "coulomb.accuracy");
+      if (!(c.accuracy > 0))
+        throw Error(RBMDX_E_CONFIG, "coulomb accuracy must be > 0");
+    }
+    if (const Json* k = cl->get("kmax")) {
+      if (k->kind != Json::Array || k->arr.size() != 3)
+        throw Error(RBMDX_E_CONFIG, "'kmax' must contain exactly 3 integers");
+      c.has_kmax = true;
+      for (int d = 0; d < 3; ++d) c.kmax[d] = (int)num_of(&k->arr[d], "kmax");
+    }
+    if (c.coulomb_type == "RBE") {
+      c.coulomb_sample_num =
+          (int)num_of(cl->get("coulomb_sample_num"), "coulomb.coulomb_sample_num");
+      if (c.coulomb_sample_num <= 0)
+        throw Error(RBMDX_E_CONFIG, "the RBE sample number must be greater than 0");
+      const Json* ef = cl->get("energy_rbe_flag");
+      if (!ef)
+        throw Error(RBMDX_E_CONFIG,
+                    "when using RBE for the coulomb type, the key "
+                    "'energy_rbe_flag' must be defined");
+      c.energy_rbe = str_of(ef, "energy_rbe_flag") == "yes";
+    }
+    if (const Json* s = cl->get("random_seed")) {
+      const double v = num_of(s, "coulomb.random_seed");
+      if (v < 0 || v > 4294967295.0)
+        throw Error(RBMDX_E_CONFIG, "coulomb.random_seed exceeds UINT_MAX");
+      c.has_seed = true;
+      c.seed = (uint32_t)v;
+    }
+  }
+  return c;
+}
+
+// RblFullNeighborListBuilder::GetRblParams
+// (rbl_full_neighbor_list_builder.cpp:193-228), same float/double mix.
+int rbl_frequency(long n_global, const double L[3], double r_core, double rc,
+                  int sample_num) {
+  const float volume = (float)L[0] * (float)L[1] * (float)L[2];
+  const float rho = (float)n_global / volume;
+  const float coeff = (float)(1.0 + (0.05 / rho - 0.05));
+  const int icoeff = (int)std::round((double)coeff);
+  const float rsf = (float)r_core, rcf = (float)rc;
+  const int rs_num =
+      (int)(icoeff * rho * std::ceil(4.0 / 3.0 * M_PI * std::pow((double)rsf, 3)) + 1);
+  const int rc_num =
+      (int)(icoeff * rho * std::ceil(4.0 / 3.0 * M_PI * std::pow((double)rcf, 3)) + 1);
+  const int shell = rc_num - rs_num;
+  if (shell <= sample_num) return 1;
+  return (int)std::ceil((float)shell / (float)sample_num);
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+struct EngineBase {
+  virtual ~EngineBase() = default;
+  virtual void force(int64_t step, const void* px, const void* py,
+                     const void* pz, void* fx, void* fy, void* fz,
+                     const int32_t* m, int32_t nk, uint32_t flags,
+                     hipStream_t s) = 0;
+  virtual void thermo(rbmdx_thermo_t* out) = 0;
+  virtual int64_t export_pairs(int64_t step, const void* px, const void* py,
+                               const void* pz, int candidates, int32_t* gi,
+                               int32_t* gj, int32_t* cls, int64_t cap) = 0;
+  virtual void set_charges(const double* q) = 0;
+  virtual int kernel_times(const char** names, double* usec, int n) = 0;
+  virtual void rbe_sample(int64_t call, int32_t* m) = 0;
+  virtual int num_samples() const = 0;
+};
+
+template <typename T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc(&p, n * sizeof(T)));
+  return (T*)p;
+}
+
+template <typename R>
+class Engine : public EngineBase {
+ public:
+  using R4 = typename Dev<R>::R4;
+
+  Engine(const Config& c, int device) : cfg_(c), device_(device) {}
+
+  ~Engine() override {
+    hipSetDevice(device_);
+    if (last_stream_set_) hipStreamSynchronize(last_stream_);
+    pipeline_.reset();
+    for (void* p : allocs_) hipFree(p);
+    for (auto& ev : timing_events_)
+      for (auto e : ev) hipEventDestroy(e);
+    for (auto e : slot_events_) hipEventDestroy(e);
+    if (stage_) hipHostFree(stage_);
+    if (ctrl_host_) hipHostFree(ctrl_host_);
+  }
+
+  void set_system(int64_t n, const double* lo, const double* L,
+                  const int32_t* type, const int32_t* id, const double* q,
+                  int32_t ntypes, const double* eps, const double* sig) {
+    if (n <= 0 || n > (1LL << 30)) throw Error(RBMDX_E_ARG, "bad atom count");
+    if (ntypes <= 0 || ntypes > kMaxTypes)
+      throw Error(RBMDX_E_ARG, "ntypes must be in [1, 32]");
+    for (int d = 0; d < 3; ++d)
+      if (!(L[d] > 0) || !std::isfinite(L[d]) || !std::isfinite(lo[d]))
+        throw Error(RBMDX_E_ARG, "box lengths must be finite and > 0");
+    n_ = (int)n;
+    ntypes_ = ntypes;
+    for (int d = 0; d < 3; ++d) {
+      lo_[d] = lo[d];
+      L_[d] = L[d];
+    }
+    type_.assign(type, type + n);
+    id_.assign(id, id + n);
+    q_.assign(q, q + n);
+    for (int64_t i = 0; i < n; ++i)
+      if (type_[i] < 0 || type_[i] >= ntypes)
+        throw Error(RBMDX_E_ARG, "atom type out of range");
+    eps_.assign(eps, eps + ntypes);
+    sig_.assign(sig, sig + ntypes);
+    setup();
+  }
+
+  void set_charges(const double* q) override {
+    if (!ready_) throw Error(RBMDX_E_STATE, "set_system first");
+    const bool was_split = split_;
+    std::vector<double> nq(q, q + n_);
+    bool any = false;
+    for (double v : nq) any |= v != 0.0;
+    // The charged/neutral partition is fixed at set-up; allow value changes
+    // that keep it (zero stays zero).
+    for (int i = 0; i < n_; ++i)
+      if (((R)nq[i] != R(0)) != ((R)q_[i] != R(0)) && was_split)
+        throw Error(RBMDX_E_ARG,
+                    "set_charges may not change which atoms are charged");
+    (void)any;
+    q_ = nq;
+    upload_charges();
+    charge_sums();
+    hipDeviceSynchronize();
+  }
+
+  int num_samples() const override {
+    return cfg_.coulomb_type == "RBE" ? cfg_.coulomb_sample_num : 0;
+  }
+
+  void rbe_sample(int64_t call, int32_t* m) override {
+    if (cfg_.coulomb_type != "RBE") throw Error(RBMDX_E_STATE, "no RBE k-space");
+    sampler_->draw(rbe_seed_of_call(base_seed_, call), m);
+  }
+
+  void force(int64_t step, const void* px, const void* py, const void* pz,
+             void* fx, void* fy, void* fz, const int32_t* m, int32_t nk,
+             uint32_t flags, hipStream_t s) override {
+    if (!ready_) throw Error(RBMDX_E_STATE, "rbmdx_set_system first");
+    if (!px || !py || !pz || !fx || !fy || !fz)
+      throw Error(RBMDX_E_ARG, "null position or force pointer");
+    HIPCHK(hipSetDevice(device_));
+    bind_positions(px, py, pz);
+    d_.fx = (R*)fx;
+    d_.fy = (R*)fy;
+    d_.fz = (R*)fz;
+    int force_rebuild = (flags & RBMDX_FLAG_REBUILD) ? 1 : 0;
+    if (d_.cap_a == 0) {
+      plan_capacity(s);
+      force_rebuild = 1;
+    } else if (grow_if_needed(s)) {
+      force_rebuild = 1;
+    }
+    StepArgs<R> a = args(step, flags);
+    a.force_rebuild = force_rebuild;
+    // k-sample of this evaluation
+    if (kspace_ == 1) {
+      if (m) {
+        if (nk <= 0 || nk > d_.kcap)
+          throw Error(RBMDX_E_ARG, "num_k out of range for explicit samples");
+        a.m_use = m;
+        a.p.num_k = nk;
+      } else {
+        const int slot = (int)(calls_ % kSlots);
+        if (slot_used_[slot]) HIPCHK(hipEventSynchronize(slot_events_[slot]));
+        int32_t* dst = stage_ + (size_t)slot * 3 * d_.kcap;
+        const int32_t* dst_dev = stage_dev_ + (size_t)slot * 3 * d_.kcap;
+        if (!pipeline_) {
+          const char* th = std::getenv("RBMDX_SAMPLER_THREADS");
+          const int nth = th ? std::max(1, std::atoi(th)) : 4;
+          pipeline_.reset(new SamplePipeline(sampler_.get(), base_seed_, nth, 64));
+          pipeline_call_ = 0;
+        }
+        pipeline_->take(pipeline_call_++, dst);
+        a.m_host = dst_dev;
+        a.m_use = d_.m_dev;
+        a.p.num_k = num_samples();
+        stage_slot_ = slot;
+      }
+    } else if (kspace_ == 2) {
+      a.m_use = d_.m_dev;
+    }
+    if (timing_) {
+      timing_events_.emplace_back();
+      auto& ev = timing_events_.back();
+      for (int k = 0; k < 6; ++k) {
+        HIPCHK(hipEventCreate(&ev[k]));
+        a.ev[k] = ev[k];
+      }
+    }
+    HIPCHK(launch_force<R>(d_, a, s));
+    if (a.m_host) {
+      HIPCHK(hipEventRecord(slot_events_[stage_slot_], s));
+      slot_used_[stage_slot_] = true;
+    }
+    last_stream_ = s;
+    last_stream_set_ = true;
+    ++calls_;
+    if (force_rebuild) ++forced_;
+    last_num_k_ = (kspace_ ? a.p.num_k : 0);
+    if (flags & RBMDX_FLAG_SYNC) {
+      HIPCHK(hipStreamSynchronize(s));
+      if (grow_if_needed(s)) {
+        // lists were outgrown: the result is still exact (stencil fallback);
+        // capacity is now larger for the next call.
+      }
+    }
+  }
+
+  void thermo(rbmdx_thermo_t* out) override {
+    if (!ready_) throw Error(RBMDX_E_STATE, "rbmdx_set_system first");
+    HIPCHK(hipSetDevice(device_));
+    if (last_stream_set_) HIPCHK(hipStreamSynchronize(last_stream_));
+    double t[32];
+    HIPCHK(hipMemcpy(t, d_.thermo, sizeof(double) * 18, hipMemcpyDeviceToHost));
+    Ctrl c;
+    HIPCHK(hipMemcpy(&c, d_.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->e_vdwl = t[0];
+    out->e_coul = t[1];
+    out->e_kspace = t[2];
+    for (int k = 0; k < 6; ++k) {
+      out->virial[k] = t[3 + k];
+      out->virial_kspace[k] = t[9 + k];
+    }
+    for (int k = 0; k < 3; ++k) out->mean_force[k] = t[15 + k];
+    out->steps = (int64_t)c.step_count;
+    out->rebuilds = (int64_t)c.rebuild_count;
+    out->rbl_frequency = freq_;
+    out->num_k = last_num_k_;
+    out->alpha = alpha_;
+    out->rbe_S = rbe_S_;
+  }
+
+  int64_t export_pairs(int64_t step, const void* px, const void* py,
+                       const void* pz, int candidates, int32_t* gi,
+                       int32_t* gj, int32_t* cls, int64_t cap) override {
+    if (!ready_) throw Error(RBMDX_E_STATE, "rbmdx_set_system first");
+    HIPCHK(hipSetDevice(device_));
+    hipStream_t s = nullptr;
+    if (!candidates) {
+      bind_positions(px, py, pz);
+      if (d_.cap_a == 0) plan_capacity(s);
+      StepArgs<R> a = args(step, 0);
+      HIPCHK(launch_build<R>(d_, a, s));
+      HIPCHK(hipStreamSynchronize(s));
+      Ctrl c;
+      HIPCHK(hipMemcpy(&c, d_.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+      if (c.max_count_a > d_.cap_a || c.max_count_b > d_.cap_b) {
+        alloc_lists(std::max(c.max_count_a, d_.cap_a),
+                    std::max(c.max_count_b, d_.cap_b));
+        a = args(step, 0);
+        HIPCHK(launch_build<R>(d_, a, s));
+      }
+    } else if (d_.cap_a == 0) {
+      throw Error(RBMDX_E_STATE, "no lists built yet");
+    }
+    StepArgs<R> a = args(step, 0);
+    const int ecap = (int)std::min<int64_t>(cap, 1LL << 30);
+    int* buf = dalloc<int>((size_t)3 * std::max(ecap, 1));
+    d_.export_buf = buf;
+    HIPCHK(hipMemsetAsync(d_.export_count, 0, sizeof(int), s));
+    HIPCHK(launch_export<R>(d_, a, candidates, ecap, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int count = 0;
+    HIPCHK(hipMemcpy(&count, d_.export_count, sizeof(int), hipMemcpyDeviceToHost));
+    const int ncopy = std::min(count, ecap);
+    std::vector<int> h((size_t)3 * std::max(ncopy, 1));
+    if (ncopy > 0)
+      HIPCHK(hipMemcpy(h.data(), buf, sizeof(int) * 3 * ncopy, hipMemcpyDeviceToHost));
+    hipFree(buf);
+    d_.export_buf = nullptr;
+    for (int k = 0; k < ncopy; ++k) {
+      gi[k] = h[3 * k];
+      gj[k] = h[3 * k + 1];
+      cls[k] = h[3 * k + 2];
+    }
+    return count;
+  }
+
+  int kernel_times(const char** names, double* usec, int n) override {
+    static const char* kNames[5] = {"rebuild", "pair_a", "pair_b", "kspace",
+                                    "assemble"};
+    if (!timing_) return 0;
+    if (last_stream_set_) HIPCHK(hipStreamSynchronize(last_stream_));
+    double acc[5] = {0, 0, 0, 0, 0};
+    const size_t calls = timing_events_.size();
+    for (auto& ev : timing_events_) {
+      for (int k = 0; k < 5; ++k) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        acc[k] += ms * 1000.0;
+      }
+      for (auto e : ev) hipEventDestroy(e);
+    }
+    timing_events_.clear();
+    const int m = std::min(n, 5);
+    for (int k = 0; k < m; ++k) {
+      if (names) names[k] = kNames[k];
+      usec[k] = calls ? acc[k] / (double)calls : 0.0;
+    }
+    return m;
+  }
+
+ private:
+  static constexpr int kSlots = 16;
+  Config cfg_;
+  int device_;
+  bool ready_ = false;
+  int n_ = 0, ntypes_ = 0;
+  double lo_[3], L_[3];
+  std::vector<int32_t> type_, id_;
+  std::vector<double> q_, eps_, sig_;
+  bool split_ = false;
+  int kspace_ = 0;  // 0 none, 1 RBE, 2 EWALD
+  double alpha_ = 0, rbe_S_ = 0, sum_q_ = 0, sum_q2_ = 0, e_self_ = 0;
+  int freq_ = 1;
+  Grid grid_{};
+  Box<R> box_{};
+  PairConst<R> pc_{};
+  Params params_{};
+  Dev<R> d_{};
+  R* lj_tab_ = nullptr;
+  std::vector<void*> allocs_;
+  std::unique_ptr<RbeSampler> sampler_;
+  std::unique_ptr<SamplePipeline> pipeline_;
+  int64_t pipeline_call_ = 0;
+  uint32_t base_seed_ = 0;
+  int32_t* stage_ = nullptr;       // host view of the k-sample staging ring
+  int32_t* stage_dev_ = nullptr;   // device view of the same memory
+  Ctrl* ctrl_host_ = nullptr;      // host view of the control mirror
+  hipEvent_t slot_events_[kSlots] = {};
+  bool slot_used_[kSlots] = {};
+  int stage_slot_ = 0;
+  hipStream_t last_stream_ = nullptr;
+  bool last_stream_set_ = false;
+  int64_t calls_ = 0, forced_ = 0;
+  int last_num_k_ = 0;
+  bool timing_ = false;
+  std::vector<std::array<hipEvent_t, 6>> timing_events_;
+  unsigned rebuild_thr_ = 0;
+
+  template <typename T>
+  T* alloc(size_t n) {
+    T* p = dalloc<T>(n);
+    allocs_.push_back(p);
+    return p;
+  }
+
+
+  void bind_positions(const void* px, const void* py, const void* pz) {
+    d_.px = (const R*)px;
+    d_.py = (const R*)py;
+    d_.pz = (const R*)pz;
+  }
+
+  void charge_sums() {
+    sum_q_ = 0;
+    sum_q2_ = 0;
+    for (double v : q_) {
+      sum_q_ += v;
+      sum_q2_ += v * v;
+    }
+    params_.sum_q = sum_q_;
+    params_.sum_q2 = sum_q2_;
+    const double V = L_[0] * L_[1] * L_[2];
+    if (kspace_)
+      e_self_ = -cfg_.qqr2e * (std::sqrt(alpha_ / M_PI) * sum_q2_ +
+                               0.5 * M_PI * sum_q_ * sum_q_ / (alpha_ * V));
+  }
+
+  void upload_charges() {
+    std::vector<R> qr(n_);
+    for (int i = 0; i < n_; ++i) qr[i] = (R)q_[i];
+    HIPCHK(hipMemcpy(d_.charge, qr.data(), sizeof(R) * n_, hipMemcpyHostToDevice));
+  }
+
+  void setup() {
+    HIPCHK(hipSetDevice(device_));
+    const Config& c = cfg_;
+    const bool coulomb = c.force_type == "LJ/CUT/COUL/LONG";
+    kspace_ = coulomb ? (c.coulomb_type == "RBE" ? 1 : 2) : 0;
+    // charged = nonzero in the device precision (the kernels test (R)q != 0)
+    int nc = 0;
+    if (coulomb)
+      for (double v : q_) nc += (R)v != R(0);
+    split_ = coulomb && nc > 0;
+    if (!split_) nc = 0;
+    const double V = L_[0] * L_[1] * L_[2];
+    charge_sums();
+    // alpha: manual override wins, else DeriveAlphaFromAccuracy
+    // (kspace_calculator.cpp:105-157)
+    if (coulomb) {
+      if (c.has_alpha) {
+        alpha_ = c.alpha;
+      } else {
+        const double q2 = c.qqr2e * sum_q2_;
+        if (!(q2 > 0))
+          throw Error(RBMDX_E_CONFIG,
+                      "automatic alpha derivation requires a charged system");
+        const double acc = c.accuracy * c.qqr2e;
+        double g = acc * std::sqrt((double)n_ * c.cut_off * V) / (2.0 * q2);
+        g = g >= 1.0 ? (1.35 - 0.15 * std::log(acc)) / c.cut_off
+                     : std::sqrt(-std::log(g)) / c.cut_off;
+        alpha_ = g * g;
+      }
+    }
+    // interaction radii and grid
+    const double rc = c.cut_off;
+    const double rclj = c.lj_cut_off;
+    const double ra = std::min(rclj, rc) + c.skin;  // all-atom list radius
+    const double rb = rc + c.skin;                    // charged-pair radius
+    const double rmax = split_ ? std::max(ra, rb) : ra;
+    for (int d = 0; d < 3; ++d)
+      if (!(rmax < 0.5 * L_[d]))
+        throw Error(RBMDX_E_CONFIG,
+                    "neighbor cutoff + skin must be below half the shortest box edge");
+    for (int d = 0; d < 3; ++d) {
+      int ncell = (int)std::floor(L_[d] / (ra * (1.0 + 1e-5)));
+      if (ncell < 1) ncell = 1;
+      grid_.nc[d] = ncell;
+      const double side = L_[d] / ncell;
+      grid_.ext_a[d] = std::max(1, (int)std::ceil(ra * (1.0 + 1e-5) / side));
+      grid_.ext_b[d] = std::max(1, (int)std::ceil(rb * (1.0 + 1e-5) / side));
+      grid_.scale[d] = (float)(ncell / L_[d]);
+    }
+    grid_.ncell = grid_.nc[0] * grid_.nc[1] * grid_.nc[2];
+    for (int d = 0; d < 3; ++d) {
+      box_.lo[d] = (R)lo_[d];
+      box_.L[d] = (R)L_[d];
+      box_.hi[d] = (R)(lo_[d] + L_[d]);
+      box_.half[d] = (R)0.5 * box_.L[d];
+      box_.invL[d] = (R)(1.0 / L_[d]);
+    }
+    // RBL frequency (pice_num)
+    freq_ = c.neighbor_type == "RBL"
+                ? rbl_frequency(n_, L_, c.r_core, rc, c.sample_num)
+                : 1;
+    // pair constants, thresholds formed in R as the reference forms them
+    {
+      const R rs = (R)c.r_core, rcr = (R)rc, rl = (R)rclj, eps = (R)1e-4;
+      pc_.rs2 = rs * rs;
+      pc_.rc2 = rcr * rcr;
+      pc_.rclj2 = rl * rl;
+      pc_.t_core = pc_.rs2 - eps;
+      pc_.t_shell = pc_.rc2 - eps;
+      pc_.eps_min = eps;
+      pc_.r2_list_a = (R)(ra * ra);
+      pc_.r2_list_b = (R)(rb * rb);
+      pc_.alpha = (R)alpha_;
+      pc_.g = (R)std::sqrt(alpha_);
+      pc_.qqr2e = (R)c.qqr2e;
+      pc_.two_g_over_sqrtpi = (R)(2.0 * std::sqrt(alpha_) / std::sqrt(M_PI));
+      pc_.freq = (R)freq_;
+      pc_.ifreq = freq_;
+      pc_.ntypes = ntypes_;
+    }
+    params_.rbl = c.neighbor_type == "RBL" ? 1 : 0;
+    params_.coulomb = coulomb ? 1 : 0;
+    params_.kspace = kspace_;
+    params_.mean_fix = params_.rbl;
+    params_.split = split_ ? 1 : 0;
+    params_.qqr2e = c.qqr2e;
+    params_.alpha = alpha_;
+    params_.volume = V;
+    params_.n_global = (double)n_;
+    {
+      const float h = (float)(0.5 * c.skin);
+      const float thr = h * h;
+      std::memcpy(&rebuild_thr_, &thr, sizeof(thr));
+    }
+    // ---- device buffers
+    d_.n = n_;
+    d_.nc = nc;
+    d_.type = alloc<int>(n_);
+    d_.gid = alloc<int>(n_);
+    d_.charge = alloc<R>(n_);
+    d_.ref = alloc<R4>(n_);
+    d_.key = alloc<int>(n_);
+    d_.slot = alloc<int>(n_);
+    d_.key_count = alloc<int>(2 * (size_t)grid_.ncell);
+    d_.key_start = alloc<int>(2 * (size_t)grid_.ncell + 1);
+    d_.cstart = alloc<int>((size_t)grid_.ncell + 1);
+    d_.perm = alloc<int>(n_);
+    d_.cell_s = alloc<int>(n_);
+    d_.gid_s = alloc<int>(n_);
+    d_.s2c = alloc<int>(n_);
+    d_.xs = alloc<R4>(n_);
+    d_.fa = alloc<R4>(n_);
+    d_.c2s = alloc<int>(nc);
+    d_.gidc = alloc<int>(nc);
+    d_.typec = alloc<int>(nc);
+    d_.xqc = alloc<R4>(nc);
+    d_.fb = alloc<R4>(nc);
+    d_.fk = alloc<R4>(nc);
+    d_.cnt_a = alloc<int>(n_);
+    d_.cnt_b = alloc<int>(nc);
+    const int na = (n_ + kBlock - 1) / kBlock;
+    const int nbb = (nc + kBlock - 1) / kBlock;
+    d_.pair_part_blocks = na + nbb;
+    d_.pair_part = alloc<double>((size_t)11 * (na + nbb));
+    d_.thermo = alloc<double>(32);
+    d_.mean = alloc<double>(3);
+    d_.ctrl = alloc<Ctrl>(1);
+    d_.export_count = alloc<int>(1);
+    HIPCHK(hipMemset(d_.key_count, 0, sizeof(int) * 2 * (size_t)grid_.ncell));
+    HIPCHK(hipMemset(d_.ctrl, 0, sizeof(Ctrl)));
+    HIPCHK(hipMemset(d_.thermo, 0, sizeof(double) * 32));
+    HIPCHK(hipMemset(d_.mean, 0, sizeof(double) * 3));
+    HIPCHK(hipMemset(d_.ref, 0, sizeof(R4) * n_));
+    HIPCHK(hipMemset(d_.fb, 0, sizeof(R4) * std::max(nc, 1)));
+    HIPCHK(hipMemset(d_.fk, 0, sizeof(R4) * std::max(nc, 1)));
+    {
+      void* hp = nullptr;
+      HIPCHK(hipHostMalloc(&hp, sizeof(Ctrl), hipHostMallocMapped));
+      std::memset(hp, 0, sizeof(Ctrl));
+      ctrl_host_ = (Ctrl*)hp;
+      void* dp = nullptr;
+      HIPCHK(hipHostGetDevicePointer(&dp, hp, 0));
+      d_.ctrl_host = (Ctrl*)dp;
+    }
+    HIPCHK(hipMemcpy(d_.type, type_.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_.gid, id_.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
+    upload_charges();
+    // LJ table: (eps_ij, sigma_ij^2), Lorentz-Berthelot (lj_op.hip.cu:130-131)
+    {
+      std::vector<R> t(2 * (size_t)ntypes_ * ntypes_);
+      for (int a = 0; a < ntypes_; ++a)
+        for (int b = 0; b < ntypes_; ++b) {
+          const double e = std::sqrt(eps_[a] * eps_[b]);
+          const double s = (sig_[a] + sig_[b]) / 2.0;
+          t[2 * (a * ntypes_ + b)] = (R)e;
+          t[2 * (a * ntypes_ + b) + 1] = (R)(s * s);
+        }
+      lj_tab_ = alloc<R>(t.size());
+      HIPCHK(hipMemcpy(lj_tab_, t.data(), sizeof(R) * t.size(), hipMemcpyHostToDevice));
+    }
+    // ---- k-space
+    if (kspace_ == 1) {
+      const int P = c.coulomb_sample_num;
+      d_.kcap = std::max(P, 1);
+      // S = prod_d H_d - 1 (kspace_calculator.cpp:68-78), in FP64
+      double h[3];
+      for (int d = 0; d < 3; ++d) {
+        const double f = -(alpha_ * L_[d] * L_[d]);
+        h[d] = 0;
+        for (int mm = -10; mm <= 10; ++mm) h[d] += std::exp(mm * mm * f);
+        h[d] *= std::sqrt(-f / M_PI);
+      }
+      rbe_S_ = h[0] * h[1] * h[2] - 1.0;
+      const float Lf[3] = {(float)L_[0], (float)L_[1], (float)L_[2]};
+      sampler_.reset(new RbeSampler((float)alpha_, Lf, P));
+      if (c.has_seed) {
+        base_seed_ = c.seed;
+      } else {
+        std::random_device rd;
+        base_seed_ = rd();
+      }
+      std::vector<double> w(d_.kcap, rbe_S_ / P);
+      alloc_kspace(w, nullptr);
+      void* hp = nullptr;
+      HIPCHK(hipHostMalloc(&hp, sizeof(int32_t) * 3 * (size_t)d_.kcap * kSlots,
+                           hipHostMallocMapped));
+      stage_ = (int32_t*)hp;
+      void* dp = nullptr;
+      HIPCHK(hipHostGetDevicePointer(&dp, hp, 0));
+      stage_dev_ = (int32_t*)dp;
+      for (int k = 0; k < kSlots; ++k)
+        HIPCHK(hipEventCreateWithFlags(&slot_events_[k], hipEventDisableTiming));
+    } else if (kspace_ == 2) {
+      int km[3];
+      if (c.has_kmax) {
+        for (int d = 0; d < 3; ++d) km[d] = c.kmax[d];
+      } else {
+        // SetKspacePara (kspace_calculator.cpp:920-986)
+        const double g = std::sqrt(alpha_);
+        const double acc = c.has_alpha ? 1e-5 * c.qqr2e : c.accuracy * c.qqr2e;
+        const double q2 = c.qqr2e * sum_q2_;
+        for (int d = 0; d < 3; ++d) {
+          int k = 1;
+          auto rms = [&](int kk) {
+            return 2.0 * q2 * g / L_[d] * std::sqrt(1.0 / (M_PI * kk * n_)) *
+                   std::exp(-M_PI * M_PI * kk * kk / (g * g * L_[d] * L_[d]));
+          };
+          while (rms(k) > acc && k < 512) ++k;
+          km[d] = k;
+        }
+      }
+      std::vector<int> mv;
+      std::vector<double> w;
+      for (int a = -km[0]; a <= km[0]; ++a)
+        for (int b = -km[1]; b <= km[1]; ++b)
+          for (int e = -km[2]; e <= km[2]; ++e) {
+            if (!a && !b && !e) continue;
+            mv.push_back(a);
+            mv.push_back(b);
+            mv.push_back(e);
+            const double kx = 2 * M_PI * a / L_[0], ky = 2 * M_PI * b / L_[1],
+                         kz = 2 * M_PI * e / L_[2];
+            w.push_back(std::exp(-(kx * kx + ky * ky + kz * kz) / (4 * alpha_)));
+          }
+      if ((int)w.size() > kMaxK)
+        throw Error(RBMDX_E_CONFIG, "EWALD kmax grid too large");
+      d_.kcap = (int)w.size();
+      alloc_kspace(w, &mv);
+      params_.num_k = d_.kcap;
+    }
+    params_.num_k = kspace_ == 1 ? c.coulomb_sample_num : params_.num_k;
+    timing_ = std::getenv("RBMDX_TIMING") && std::atoi(std::getenv("RBMDX_TIMING"));
+    HIPCHK(hipDeviceSynchronize());
+    ready_ = true;
+  }
+
+  void alloc_kspace(const std::vector<double>& w, const std::vector<int>* mv) {
+    const int nchunks = std::max(1, (d_.nc + kSkChunk - 1) / kSkChunk);
+    d_.m_dev = alloc<int>(3 * (size_t)d_.kcap);
+    d_.kw = alloc<double>(d_.kcap);
+    d_.sk_part = alloc<double2>((size_t)nchunks * d_.kcap);
+    d_.kf0 = alloc<float4>(d_.kcap);
+    d_.kf1 = alloc<float4>(d_.kcap);
+    d_.rho = alloc<double2>(d_.kcap);
+    HIPCHK(hipMemcpy(d_.kw, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    if (mv)
+      HIPCHK(hipMemcpy(d_.m_dev, mv->data(), sizeof(int) * mv->size(),
+                       hipMemcpyHostToDevice));
+    else
+      HIPCHK(hipMemset(d_.m_dev, 0, sizeof(int) * 3 * (size_t)d_.kcap));
+  }
+
+  StepArgs<R> args(int64_t step, uint32_t flags) {
+    StepArgs<R> a;
+    a.box = box_;
+    a.grid = grid_;
+    a.pc = pc_;
+    a.pc.epoch = (long long)step;
+    a.p = params_;
+    a.p.eflag = (flags & RBMDX_FLAG_ENERGY) ? 1 : 0;
+    a.lj_tab = lj_tab_;
+    a.rebuild_thr = rebuild_thr_;
+    a.e_self = e_self_;
+    a.m_use = d_.m_dev;
+    return a;
+  }
+
+  // First call: count-only build to size the candidate lists.
+  void plan_capacity(hipStream_t s) {
+    StepArgs<R> a = args(0, 0);
+    a.count_only = 1;
+    HIPCHK(hipMemset(&d_.ctrl->max_count_a, 0, 2 * sizeof(int)));
+    HIPCHK(launch_build<R>(d_, a, s));
+    HIPCHK(hipStreamSynchronize(s));
+    Ctrl c;
+    HIPCHK(hipMemcpy(&c, d_.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    alloc_lists(c.max_count_a, c.max_count_b);
+  }
+
+  void alloc_lists(int max_a, int max_b) {
+    const int cap_a = (int)std::ceil(max_a * 1.2) + 16;
+    const int cap_b = (int)std::ceil(max_b * 1.2) + 16;
+    if (d_.list_a) release(d_.list_a);
+    if (d_.list_b) release(d_.list_b);
+    d_.list_a = alloc<int>((size_t)cap_a * n_);
+    d_.cap_a = cap_a;
+    if (d_.nc > 0) {
+      d_.list_b = alloc<int>((size_t)cap_b * d_.nc);
+      d_.cap_b = cap_b;
+    } else {
+      d_.cap_b = 0;
+    }
+  }
+
+  void release(void* p) {
+    auto it = std::find(allocs_.begin(), allocs_.end(), p);
+    if (it != allocs_.end()) allocs_.erase(it);
+    hipFree(p);
+  }
+
+  // Reads the host-mapped counters written by the last completed call.
+  bool grow_if_needed(hipStream_t s) {
+    const volatile Ctrl* h = ctrl_host_;
+    const int ma = h->max_count_a, mb = h->max_count_b;
+    if (ma <= d_.cap_a && mb <= d_.cap_b) return false;
+    HIPCHK(hipStreamSynchronize(s));
+    alloc_lists(std::max(ma, d_.cap_a), std::max(mb, d_.cap_b));
+    return true;
+  }
+};
+
+}  // namespace
+
+struct rbmdx_ctx {
+  std::string err;
+  int device = 0;
+  Config cfg;
+  std::unique_ptr<EngineBase> eng;
+  Engine<float>* ef = nullptr;
+  Engine<double>* ed = nullptr;
+};
+
+namespace {
+template <typename F>
+int guarded(rbmdx_ctx* c, F&& f) {
+  try {
+    f();
+    return RBMDX_OK;
+  } catch (const Error& e) {
+    if (c) c->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    if (c) c->err = e.what();
+    return RBMDX_E_HIP;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int rbmdx_create(const char* json_text, int device, rbmdx_ctx** out) {
+  if (!out) return RBMDX_E_ARG;
+  *out = nullptr;
+  auto* c = new rbmdx_ctx();
+  const int rc = guarded(c, [&] {
+    if (!json_text) throw Error(RBMDX_E_ARG, "null JSON text");
+    c->cfg = parse_config(json_text);
+    c->device = device;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+      throw Error(RBMDX_E_ARG, "device ordinal out of range");
+    HIPCHK(hipSetDevice(device));
+    if (c->cfg.fp64) {
+      c->ed = new Engine<double>(c->cfg, device);
+      c->eng.reset(c->ed);
+    } else {
+      c->ef = new Engine<float>(c->cfg, device);
+      c->eng.reset(c->ef);
+    }
+  });
+  *out = c;  // returned even on error so rbmdx_last_error can explain
+  return rc;
+}
+
+void rbmdx_destroy(rbmdx_ctx* ctx) { delete ctx; }
+
+const char* rbmdx_last_error(const rbmdx_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+int rbmdx_set_system(rbmdx_ctx* ctx, int64_t n, const double* box_lo,
+                     const double* box_len, const int32_t* type,
+                     const int32_t* id, const double* charge, int32_t ntypes,
+                     const double* eps, const double* sigma) {
+  if (!ctx) return RBMDX_E_ARG;
+  return guarded(ctx, [&] {
+    if (!ctx->eng) throw Error(RBMDX_E_STATE, "context was not created");
+    if (!box_lo || !box_len || !type || !id || !charge || !eps || !sigma)
+      throw Error(RBMDX_E_ARG, "null argument");
+    // a new system replaces the old engine wholesale
+    if (ctx->cfg.fp64) {
+      ctx->ed = new Engine<double>(ctx->cfg, ctx->device);
+      ctx->eng.reset(ctx->ed);
+      ctx->ed->set_system(n, box_lo, box_len, type, id, charge, ntypes, eps, sigma);
+    } else {
+      ctx->ef = new Engine<float>(ctx->cfg, ctx->device);
+      ctx->eng.reset(ctx->ef);
+      ctx->ef->set_system(n, box_lo, box_len, type, id, charge, ntypes, eps, sigma);
+    }
+  });
+}
+
+int rbmdx_set_charges(rbmdx_ctx* ctx, const double* charge) {
+  if (!ctx || !charge) return RBMDX_E_ARG;
+  return guarded(ctx, [&] { ctx->eng->set_charges(charge); });
+}
+
+int rbmdx_force(rbmdx_ctx* ctx, int64_t step, const void* px, const void* py,
+                const void* pz, void* fx, void* fy, void* fz,
+                const int32_t* m_samples, int32_t num_k, uint32_t flags,
+                void* stream) {
+  if (!ctx) return RBMDX_E_ARG;
+  return guarded(ctx, [&] {
+    ctx->eng->force(step, px, py, pz, fx, fy, fz, m_samples, num_k, flags,
+                    (hipStream_t)stream);
+  });
+}
+
+int rbmdx_thermo(rbmdx_ctx* ctx, rbmdx_thermo_t* out) {
+  if (!ctx || !out) return RBMDX_E_ARG;
+  return guarded(ctx, [&] { ctx->eng->thermo(out); });
+}
+
+int rbmdx_rbe_sample(rbmdx_ctx* ctx, int64_t call_index, int32_t* m_out) {
+  if (!ctx || !m_out || call_index < 0) return RBMDX_E_ARG;
+  return guarded(ctx, [&] { ctx->eng->rbe_sample(call_index, m_out); });
+}
+
+int32_t rbmdx_num_samples(const rbmdx_ctx* ctx) {
+  return (ctx && ctx->eng) ? ctx->eng->num_samples() : 0;
+}
+
+int64_t rbmdx_export_pairs(rbmdx_ctx* ctx, int64_t step, const void* px,
+                           const void* py, const void* pz, int32_t* gi,
+                           int32_t* gj, int32_t* cls, int64_t cap) {
+  if (!ctx || !gi || !gj || !cls || cap < 0) return -RBMDX_E_ARG;
+  int64_t r = 0;
+  const int rc = guarded(ctx, [&] {
+    r = ctx->eng->export_pairs(step, px, py, pz, 0, gi, gj, cls, cap);
+  });
+  return rc ? -rc : r;
+}
+
+int64_t rbmdx_export_candidates(rbmdx_ctx* ctx, int32_t* gi, int32_t* gj,
+                                int32_t* cls, int64_t cap) {
+  if (!ctx || !gi || !gj || !cls || cap < 0) return -RBMDX_E_ARG;
+  int64_t r = 0;
+  const int rc = guarded(ctx, [&] {
+    r = ctx->eng->export_pairs(0, nullptr, nullptr, nullptr, 1, gi, gj, cls, cap);
+  });
+  return rc ? -rc : r;
+}
+
+int rbmdx_rbe_sample_host(double alpha, const double* L, int32_t P,
+                          uint32_t base_seed, int64_t call_index,
+                          int32_t* m_out) {
+  if (!L || !m_out || P <= 0 || call_index < 0 || !(alpha > 0)) return RBMDX_E_ARG;
+  return guarded(nullptr, [&] {
+    const float Lf[3] = {(float)L[0], (float)L[1], (float)L[2]};
+    RbeSampler s((float)alpha, Lf, P);
+    s.draw(rbe_seed_of_call(base_seed, call_index), m_out);
+  });
+}
+
+int rbmdx_check_config(const char* json_text, char* err, int32_t errlen) {
+  if (!json_text) return RBMDX_E_ARG;
+  rbmdx_ctx tmp;
+  const int rc = guarded(&tmp, [&] { parse_config(json_text); });
+  if (err && errlen > 0) {
+    std::strncpy(err, tmp.err.c_str(), (size_t)errlen - 1);
+    err[errlen - 1] = 0;
+  }
+  return rc;
+}
+
+int rbmdx_kernel_times(rbmdx_ctx* ctx, const char** names, double* usec,
+                       int n) {
+  if (!ctx || !usec) return -RBMDX_E_ARG;
+  int r = 0;
+  const int rc = guarded(ctx, [&] { r = ctx->eng->kernel_times(names, usec, n); });
+  return rc ? -rc : r;
+}
+
+}  // extern "C"

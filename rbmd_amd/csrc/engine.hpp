@@ -1,0 +1,167 @@
+// Internal interface between the host engine (api.cpp) and the HIP kernels
+// (kernels.hip).  Plain structs of device pointers and scalars; every launch
+// wrapper takes the stream it runs on.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace rbmdx {
+
+constexpr int kBlock = 256;      // threads per block for per-atom kernels
+constexpr int kMaxTypes = 32;    // LJ type table held in LDS
+constexpr int kSkTile = 8;       // k-vectors per S(k) block
+constexpr int kSkChunk = 1024;   // charged atoms per S(k) block
+constexpr int kRbeWaves = 8;     // waves per RBE-force block (k split)
+constexpr int kMaxK = 1 << 20;   // k-vector capacity bound
+
+// Device control block: rebuild decision, overflow bookkeeping.
+struct Ctrl {
+  unsigned int maxdisp_bits;  // max |dr|^2 since the last rebuild (float bits)
+  int rebuild;                // unused by kernels; host mirror
+  int max_count_a;            // largest all-atom candidate count seen
+  int max_count_b;            // largest charged-pair candidate count seen
+  unsigned long long rebuild_count;
+  unsigned long long step_count;
+  int nonfinite;
+  int pad;
+};
+
+template <typename R>
+struct Box {
+  R lo[3];
+  R hi[3];  // lo + L as stored by the caller's box (upper bound, exclusive)
+  R L[3];
+  R half[3];
+  R invL[3];
+};
+
+// Cell grid shared by the list builders and the overflow fallback.
+struct Grid {
+  int nc[3];       // cells per dimension
+  int ncell;
+  int ext_a[3];    // stencil half-width for the all-atom list
+  int ext_b[3];    // stencil half-width for the charged-pair list
+  float scale[3];  // nc / L (for cell index)
+};
+
+template <typename R>
+struct PairConst {
+  R t_core, t_shell;  // RBL classification thresholds r_core^2-eps, rc^2-eps
+  R rs2, rc2, rclj2;  // force windows
+  R eps_min;          // coincident-pair guard (EPSILON)
+  R r2_list_a, r2_list_b;  // (cut + skin)^2 of the two lists
+  R alpha, g, qqr2e, two_g_over_sqrtpi;
+  R freq;             // RBL selection frequency as a multiplier
+  int ifreq;
+  long long epoch;
+  int ntypes;
+};
+
+// All device buffers of one context (sorted order unless noted).
+template <typename R>
+struct Dev {
+  using R4 = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
+  int n = 0, nc = 0;  // atoms, charged atoms
+  // caller order
+  const R* px = nullptr;
+  const R* py = nullptr;
+  const R* pz = nullptr;
+  R* fx = nullptr;
+  R* fy = nullptr;
+  R* fz = nullptr;
+  int* type = nullptr;     // [n] caller order
+  int* gid = nullptr;      // [n] caller order
+  R* charge = nullptr;     // [n] caller order
+  R4* ref = nullptr;       // [n] positions at the last rebuild (caller order)
+  int* key = nullptr;      // [n] cell key of the last rebuild (caller order)
+  int* slot = nullptr;     // [n] arrival slot inside the key
+  // grid
+  int* key_count = nullptr;  // [2*ncell]
+  int* key_start = nullptr;  // [2*ncell+1]
+  int* cstart = nullptr;     // [ncell+1] charged atoms before the cell
+  // sorted order
+  int* perm = nullptr;     // [n] sorted -> caller
+  int* cell_s = nullptr;   // [n]
+  int* gid_s = nullptr;    // [n]
+  int* s2c = nullptr;      // [n] sorted -> charged compact, -1 if neutral
+  R4* xs = nullptr;        // [n] (x, y, z, type)
+  R4* fa = nullptr;        // [n] all-atom list pair force
+  // charged compact
+  int* c2s = nullptr;      // [nc]
+  int* gidc = nullptr;     // [nc]
+  int* typec = nullptr;    // [nc]
+  R4* xqc = nullptr;       // [nc] (x, y, z, q)
+  R4* fb = nullptr;        // [nc] charged-pair force
+  R4* fk = nullptr;        // [nc] k-space force
+  // lists (column-major: list[k * rows + i])
+  int cap_a = 0, cap_b = 0;
+  int* list_a = nullptr;
+  int* cnt_a = nullptr;
+  int* list_b = nullptr;
+  int* cnt_b = nullptr;
+  // pair partial sums per block: 3 force + 2 energy + 6 virial
+  double* pair_part = nullptr;
+  int pair_part_blocks = 0;
+  // k-space
+  int kcap = 0;
+  int* m_dev = nullptr;          // [kcap*3]
+  double* kw = nullptr;          // [kcap] per-k weight
+  double2* sk_part = nullptr;    // [chunks * kcap]
+  float4* kf0 = nullptr;         // [kcap] (mx, my, mz, A Re rho)
+  float4* kf1 = nullptr;         // [kcap] (kx, ky, kz, A Im rho)
+  double2* rho = nullptr;        // [kcap]
+  double* thermo = nullptr;      // [32] device thermo record
+  double* mean = nullptr;        // [3]
+  Ctrl* ctrl = nullptr;
+  Ctrl* ctrl_host = nullptr;     // host-mapped mirror written each call
+  int* export_count = nullptr;
+  int* export_buf = nullptr;     // [3*cap]
+};
+
+struct Params {
+  int rbl = 1;          // RBL split (else full Verlet list)
+  int coulomb = 0;      // real-space erfc + k-space present
+  int kspace = 0;       // 0 none, 1 RBE, 2 EWALD
+  int eflag = 0;
+  int mean_fix = 0;
+  int split = 0;        // charged/neutral partition of the pair work
+  int num_k = 0;
+  double qqr2e = 1.0;
+  double alpha = 0.0;
+  double volume = 0.0;
+  double sum_q2 = 0.0, sum_q = 0.0;
+  double n_global = 0.0;
+  float skin_half2 = 0.f;  // (skin/2)^2
+};
+
+// Everything one force evaluation needs besides the device buffers.
+template <typename R>
+struct StepArgs {
+  Box<R> box;
+  Grid grid;
+  PairConst<R> pc;
+  Params p;
+  const R* lj_tab = nullptr;  // [ntypes^2 * 2] (eps_ij, sigma_ij^2), device
+  const int* m_use = nullptr; // [num_k * 3] k-vectors for this call, device
+  const int* m_host = nullptr;// host-mapped staging copied into m_dev, or null
+  unsigned int rebuild_thr;   // float bits of (skin/2)^2
+  int force_rebuild = 0;
+  int count_only = 0;         // list builders only count (capacity planning)
+  double e_self = 0.0;        // k-space self + background energy
+  hipEvent_t ev[6] = {};      // optional section timing events
+};
+
+// Launch one complete force evaluation (kernels.hip).
+template <typename R>
+hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s);
+// Launch only the (forced) rebuild: skin, binning, sort, gather, lists.
+template <typename R>
+hipError_t launch_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s);
+// Parity export of classified pairs (candidates = 0) or raw candidate lists.
+template <typename R>
+hipError_t launch_export(const Dev<R>& d, const StepArgs<R>& a, int candidates,
+                         int cap, hipStream_t s);
+
+}  // namespace rbmdx

@@ -1,0 +1,150 @@
+"""Seeded synthetic systems for the benchmark configurations (no data files).
+
+* ``lj_fcc``        plain LJ fluid: fcc lattice, uniform jitter, Maxwell velocities
+                    (4,000 atoms = 10x10x10 fcc cells at rho* = 0.8442).
+* ``electrolyte``   LJ electrolyte: 5 % +1, 5 % -1, 90 % neutral LJ solvent,
+                    uniform positions with overlap rejection (random sequential
+                    addition with a minimum distance d_min) at rho* = 0.8.
+* ``ljsalt_config`` the RBMD JSON configuration used with ``electrolyte``.
+
+Atom ids are 0-based, as RBMD makes them when it reads a LAMMPS data file
+(src/reader/src/format/lammps/lammps_full_reader.cpp:1543).
+"""
+import json
+import math
+
+import numpy as np
+
+
+class System:
+    def __init__(self, pos, box_lo, box_len, types, ids, charges, eps, sigma, vel=None,
+                 mass=None, name=""):
+        self.pos = np.asarray(pos, dtype=np.float64)
+        self.box_lo = np.asarray(box_lo, dtype=np.float64)
+        self.box_len = np.asarray(box_len, dtype=np.float64)
+        self.types = np.asarray(types, dtype=np.int32)
+        self.ids = np.asarray(ids, dtype=np.int32)
+        self.charges = np.asarray(charges, dtype=np.float64)
+        self.eps = np.asarray(eps, dtype=np.float64)
+        self.sigma = np.asarray(sigma, dtype=np.float64)
+        self.vel = None if vel is None else np.asarray(vel, dtype=np.float64)
+        self.mass = np.ones(len(eps)) if mass is None else np.asarray(mass, dtype=np.float64)
+        self.name = name
+
+    @property
+    def n(self):
+        return len(self.types)
+
+
+def lj_fcc(cells=10, rho=0.8442, jitter=0.05, temperature=1.0, seed=1):
+    """fcc LJ fluid: 4 * cells^3 atoms, uniform +-jitter, Maxwell velocities at T*."""
+    rng = np.random.default_rng(seed)
+    a = (4.0 / rho) ** (1.0 / 3.0)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(np.arange(cells), np.arange(cells), np.arange(cells),
+                             indexing="ij"), axis=-1).reshape(-1, 3)
+    pos = ((g[:, None, :] + basis[None, :, :]).reshape(-1, 3)) * a
+    L = cells * a
+    pos = pos + rng.uniform(-jitter, jitter, size=pos.shape)
+    pos = np.mod(pos, L)
+    n = len(pos)
+    vel = rng.normal(0.0, math.sqrt(temperature), size=(n, 3))
+    vel -= vel.mean(axis=0)
+    ke_t = (vel ** 2).sum() / (3 * n - 3)
+    vel *= math.sqrt(temperature / ke_t)
+    return System(pos, [0, 0, 0], [L, L, L], np.zeros(n), np.arange(n), np.zeros(n),
+                  [1.0], [1.0], vel=vel, name=f"lj_fcc_{n}")
+
+
+def _rsa(n, L, d_min, rng, batch=None):
+    """Random sequential addition in a periodic cube: n points, pairwise >= d_min."""
+    from scipy.spatial import cKDTree
+
+    pts = np.zeros((0, 3))
+    batch = batch or max(1000, n // 4)
+    tries = 0
+    while len(pts) < n:
+        tries += 1
+        if tries > 10000:
+            raise RuntimeError("RSA did not converge; lower d_min")
+        need = n - len(pts)
+        cand = rng.uniform(0.0, L, size=(max(batch, 2 * need), 3))
+        cand = np.minimum(cand, np.nextafter(L, 0))
+        if len(pts):
+            tree = cKDTree(pts, boxsize=L)
+            dist, _ = tree.query(cand, k=1, distance_upper_bound=d_min)
+            cand = cand[~(dist < d_min)]
+        if len(cand) == 0:
+            continue
+        # inside the batch keep the first of every conflicting pair (in order)
+        ct = cKDTree(cand, boxsize=L)
+        pairs = ct.query_pairs(d_min, output_type="ndarray")
+        keep = np.ones(len(cand), dtype=bool)
+        if len(pairs):
+            keep[np.maximum(pairs[:, 0], pairs[:, 1])] = False
+        cand = cand[keep][:need]
+        pts = np.concatenate([pts, cand], axis=0)
+    return pts
+
+
+def electrolyte(n=200000, rho=0.8, frac_ion=0.05, d_min=0.85, seed=2, temperature=1.0):
+    """LJ electrolyte: n*frac_ion cations (+1), as many anions (-1), rest neutral.
+
+    Types: 0 = cation, 1 = anion, 2 = solvent; eps = sigma = 1 for all.
+    """
+    rng = np.random.default_rng(seed)
+    L = (n / rho) ** (1.0 / 3.0)
+    pos = _rsa(n, L, d_min, rng)
+    nion = int(round(n * frac_ion))
+    types = np.full(n, 2, dtype=np.int32)
+    perm = rng.permutation(n)
+    types[perm[:nion]] = 0
+    types[perm[nion:2 * nion]] = 1
+    q = np.where(types == 0, 1.0, np.where(types == 1, -1.0, 0.0))
+    vel = rng.normal(0.0, math.sqrt(temperature), size=(n, 3))
+    vel -= vel.mean(axis=0)
+    return System(pos, [0, 0, 0], [L, L, L], types, np.arange(n), q, [1.0, 1.0, 1.0],
+                  [1.0, 1.0, 1.0], vel=vel, name=f"electrolyte_{n}")
+
+
+def ljsalt_config(neighbor="RBL", coulomb="RBE", cut_off=3.5, lj_cut_off=2.5, r_core=2.5,
+                  sample_num=30, skin=0.3, alpha=0.09, P=200, seed=20261202,
+                  precision="fp32", kmax=None):
+    """RBMD JSON for the LJ electrolyte: LJ cutoff 2.5, Coulomb (real-space) cutoff
+    3.5, Ewald alpha = g^2 with g = 0.3 / sigma, P = 200 RBE samples."""
+    nb = {"type": neighbor, "cut_off": cut_off, "skin": skin}
+    if lj_cut_off is not None:
+        nb["lj_cut_off"] = lj_cut_off
+    if neighbor == "RBL":
+        nb.update({"r_core": r_core, "neighbor_sample_num": sample_num,
+                   "energy_rbl_flag": "no"})
+    cl = {"type": coulomb, "alpha": alpha, "random_seed": seed}
+    if coulomb == "RBE":
+        cl.update({"coulomb_sample_num": P, "energy_rbe_flag": "no"})
+    if kmax is not None:
+        cl["kmax"] = list(kmax)
+    return {
+        "init_configuration": {"read_data": {"unit": "LJ", "atom_style": "charge"}},
+        "hyper_parameters": {
+            "force_field": {"type": "LJ/CUT/COUL/LONG"},
+            "neighbor": nb,
+            "coulomb": cl,
+            "precision": precision,
+        },
+    }
+
+
+def lj_config(neighbor="Verlet-List", cut_off=2.5, skin=0.3, precision="fp64"):
+    """RBMD JSON for the plain LJ fluid (LJ/CUT, full list)."""
+    return {
+        "init_configuration": {"read_data": {"unit": "LJ", "atom_style": "atomic"}},
+        "hyper_parameters": {
+            "force_field": {"type": "LJ/CUT"},
+            "neighbor": {"type": neighbor, "cut_off": cut_off, "skin": skin},
+            "precision": precision,
+        },
+    }
+
+
+def dumps(cfg):
+    return json.dumps(cfg)
