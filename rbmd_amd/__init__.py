@@ -109,6 +109,14 @@ def load_library(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise OSError(f"{path} is missing: run __graft_entry__.build() first")
+    # torch bundles its own HIP runtime with the same soname as ROCm's
+    # (libamdhip64.so.7).  Loading torch first makes librbmdx.so bind to that
+    # already-loaded runtime; otherwise the process would hold two HIP runtimes
+    # and torch's handles would be invalid in ours (and vice versa).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)  # AttributeError if the symbol is absent

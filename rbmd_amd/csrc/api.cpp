@@ -244,15 +244,20 @@ class Engine : public EngineBase {
   Engine(const Config& c, int device) : cfg_(c), device_(device) {}
 
   ~Engine() override {
+    if (!ready_ && allocs_.empty() && !stage_ && !ctrl_host_) return;
     hipSetDevice(device_);
     if (last_stream_set_) hipStreamSynchronize(last_stream_);
     pipeline_.reset();
     for (void* p : allocs_) hipFree(p);
     for (auto& ev : timing_events_)
-      for (auto e : ev) hipEventDestroy(e);
-    for (auto e : slot_events_) hipEventDestroy(e);
+      for (auto e : ev)
+        if (e) hipEventDestroy(e);
+    for (auto e : slot_events_)
+      if (e) hipEventDestroy(e);
     if (stage_) hipHostFree(stage_);
     if (ctrl_host_) hipHostFree(ctrl_host_);
+    // leave no sticky error behind for the caller's next HIP call
+    (void)hipGetLastError();
   }
 
   void set_system(int64_t n, const double* lo, const double* L,
@@ -581,6 +586,7 @@ class Engine : public EngineBase {
         alpha_ = g * g;
       }
     }
+    charge_sums();  // self energy needs alpha
     // interaction radii and grid
     const double rc = c.cut_off;
     const double rclj = c.lj_cut_off;
@@ -878,6 +884,9 @@ template <typename F>
 int guarded(rbmdx_ctx* c, F&& f) {
   try {
     f();
+    // Every HIP status this call produced was checked; do not leak one into
+    // the caller's hipGetLastError().
+    (void)hipGetLastError();
     return RBMDX_OK;
   } catch (const Error& e) {
     if (c) c->err = e.what();
