@@ -1,0 +1,182 @@
+"""Benchmark: RBE + LJ force path on the 200,000-particle LJ electrolyte, one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 200,000
+particles at rho* = 0.8 (5 % +1, 5 % -1, 90 % neutral LJ solvent), uniform
+positions with overlap rejection (d_min = 0.85 sigma), LJ cutoff 2.5 sigma,
+real-space Coulomb cutoff 3.5 sigma, skin 0.3 sigma, RBL short range
+(r_core = 2.5, neighbor_sample_num = 30), RBE k-space with alpha = 0.09
+(g_ewald = 0.3/sigma) and P = 200 importance-sampled k-vectors per step drawn
+from the seeded stream, FP32 pair and e^{ik.r} terms, FP64 reductions.
+
+A step is one MD step: one full force evaluation (skin check, conditional
+neighbour rebuild, RBL pair forces, RBE k-space, assembly) plus a
+semi-implicit Euler update of velocities and positions, so rebuilds happen at
+their natural rate.  Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line.  --gpus above 1 exits: this build measures one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-steps/s, RBE+LJ force, 200k LJ-salt; %HBM peak; force err vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--atoms", type=int, default=200000)
+    ap.add_argument("--dt", type=float, default=0.002)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the "
+              "multi-GPU decomposition is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    os.environ.setdefault("RBMDX_TIMING", "1")
+    import torch
+
+    from rbmd_amd import ForceContext, systems
+
+    dev = torch.device("cuda:0")
+    sys_ = systems.electrolyte(args.atoms, seed=2)
+    cfg = systems.ljsalt_config()
+    ctx = ForceContext(cfg, device=0)
+    ctx.set_system(sys_.box_lo, sys_.box_len, sys_.types, sys_.ids, sys_.charges,
+                   sys_.eps, sys_.sigma)
+    n = sys_.n
+    pos = torch.tensor(sys_.pos.T.copy(), dtype=torch.float32, device=dev)  # (3, N)
+    vel = torch.tensor(sys_.vel.T.copy(), dtype=torch.float32, device=dev) * 0.5
+    frc = torch.zeros((3, n), dtype=torch.float32, device=dev)
+    dt = args.dt
+    vmax = 0.05 / dt  # cap during warm-up only: relaxes overlaps of the random start
+
+    def step(k, warm):
+        ctx.force(k, pos[0], pos[1], pos[2], frc[0], frc[1], frc[2])
+        vel.add_(frc, alpha=dt)
+        if warm:
+            vel.clamp_(-vmax, vmax)
+        pos.add_(vel, alpha=dt)
+
+    for k in range(args.warmup):
+        step(k, True)
+    torch.cuda.synchronize()
+    ctx.kernel_times()  # drop warm-up timings
+    th0 = ctx.thermo()
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        step(k, False)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = ctx.kernel_times()
+    th1 = ctx.thermo()
+    value = n * args.steps / elapsed
+    rebuilds = th1["rebuilds"] - th0["rebuilds"]
+
+    # dominant kernel: the all-atom pair kernel.  Algorithmic bytes per launch:
+    # 4 B per stored list entry + per atom 16 B position tile + 4 B count +
+    # 4 B id + 16 B force tile (neighbour tiles assumed cache-resident).
+    pairs_a = th1["pairs_a"]
+    bytes_pair_a = 4.0 * pairs_a + 40.0 * n
+    t_pair_a = kt.get("pair_a", 0.0) * 1e-6
+    achieved = bytes_pair_a / t_pair_a / 1e9 if t_pair_a > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_pair_a", "kernel_us": round(kt.get("pair_a", 0.0), 2),
+                "algorithmic_bytes": int(bytes_pair_a)}
+
+    # force error vs FP64 and the CPU baseline (oracle restatement, one core)
+    cpu = None
+    ferr = None
+    if not args.no_cpu:
+        cpu, ferr = cpu_leg(ctx, sys_, cfg, pos, args.cpu_steps)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "LJ electrolyte RBL+RBE, 200k atoms, P=200", "atoms": n,
+                   "charged": th1["charged"], "rho": 0.8, "lj_cut": 2.5, "coul_cut": 3.5,
+                   "r_core": 2.5, "neighbor_sample_num": 30,
+                   "rbl_frequency": th1["rbl_frequency"], "alpha": 0.09, "P": 200,
+                   "skin": 0.3, "dt": dt, "rebuilds_in_timed_region": rebuilds,
+                   "parallelism": "single GPU"},
+        "kernel_us": {k: round(v, 2) for k, v in kt.items()},
+        "force_rel_rms_vs_fp64": ferr,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def cpu_leg(ctx, sys_, cfg, pos, cpu_steps):
+    """Oracle CPU restatement timed on a bounded sample, and the GPU force error
+    against the oracle's FP64 evaluation at the same positions and k-sample."""
+    import torch
+
+    import oracle
+    from rbmd_amd import rbe_sample_host
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refs import reference_force, rel_rms
+
+    pos32 = pos.T.contiguous().cpu().numpy().astype(np.float32)
+    pos32 = np.mod(pos32, sys_.box_len.astype(np.float32)).astype(np.float32)
+    L = sys_.box_len
+    n = sys_.n
+    P = 200
+    step = 10 ** 6
+    m = rbe_sample_host(0.09, L, P, 777, 0)
+    dev = pos.device
+    p = torch.tensor(pos32.T.copy(), device=dev)
+    f = torch.zeros_like(p)
+    mt = torch.tensor(m.reshape(-1), dtype=torch.int32, device=dev)
+    ctx.force(step, p[0], p[1], p[2], f[0], f[1], f[2], m_samples=mt, sync=True)
+    got = f.T.double().cpu().numpy()
+    ref = reference_force(sys_, pos32, cfg, step, m, prec=0)[0]
+    ferr = rel_rms(got, ref)
+
+    nb = cfg["hyper_parameters"]["neighbor"]
+    freq = oracle.rbl_frequency(n, L, nb["r_core"], nb["cut_off"], nb["neighbor_sample_num"])
+    params = oracle.pair_params(nb["r_core"], nb["cut_off"], nb["lj_cut_off"], 0.09, 1.0, freq,
+                                step, 1, 1, 1)
+    w = oracle.rbe_S(0.09, L) / P
+    t0 = time.perf_counter()
+    for k in range(cpu_steps):
+        oracle.force_step(pos32, sys_.charges.astype(np.float32), sys_.types, sys_.ids,
+                          sys_.eps, sys_.sigma, L, params, m, w)
+    el = time.perf_counter() - t0
+    cpu = {"value": round(n * cpu_steps / el, 1), "unit": "atom-steps/s", "cores": 1,
+           "kind": "port",
+           "sample": f"{cpu_steps} force evaluations of the same 200k-atom configuration "
+                     "(FP32 pair + FP64 k-space oracle, one thread, neighbour search each "
+                     "evaluation)",
+           "seconds": round(el, 2)}
+    return cpu, ferr
+
+
+if __name__ == "__main__":
+    main()

@@ -73,6 +73,10 @@ typedef struct rbmdx_thermo_t {
   int32_t num_k;      /* k-vectors used by the last evaluation              */
   double alpha;       /* Ewald splitting parameter (RBMD "alpha" = g^2)    */
   double rbe_S;       /* RBE normalisation S = sum_{k!=0} exp(-k^2/4alpha)  */
+  int64_t pairs_a;    /* stored all-atom candidate entries (last rebuild)  */
+  int64_t pairs_b;    /* stored charged-pair candidate entries              */
+  int64_t atoms;      /* atoms in the system                                */
+  int64_t charged;    /* charged atoms handled by the k-space kernels       */
 } rbmdx_thermo_t;
 
 /*

@@ -54,6 +54,10 @@ class Thermo(ctypes.Structure):
         ("num_k", ctypes.c_int32),
         ("alpha", ctypes.c_double),
         ("rbe_S", ctypes.c_double),
+        ("pairs_a", ctypes.c_int64),
+        ("pairs_b", ctypes.c_int64),
+        ("atoms", ctypes.c_int64),
+        ("charged", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -70,6 +74,10 @@ class Thermo(ctypes.Structure):
             "num_k": self.num_k,
             "alpha": self.alpha,
             "rbe_S": self.rbe_S,
+            "pairs_a": self.pairs_a,
+            "pairs_b": self.pairs_b,
+            "atoms": self.atoms,
+            "charged": self.charged,
         }
 
 

@@ -362,12 +362,12 @@ class Engine : public EngineBase {
       a.m_use = d_.m_dev;
     }
     if (timing_) {
-      timing_events_.emplace_back();
-      auto& ev = timing_events_.back();
-      for (int k = 0; k < 6; ++k) {
-        HIPCHK(hipEventCreate(&ev[k]));
-        a.ev[k] = ev[k];
+      if (timing_used_ == timing_events_.size()) {
+        timing_events_.emplace_back();
+        for (auto& e : timing_events_.back()) HIPCHK(hipEventCreate(&e));
       }
+      auto& ev = timing_events_[timing_used_++];
+      for (int k = 0; k < 6; ++k) a.ev[k] = ev[k];
     }
     HIPCHK(launch_force<R>(d_, a, s));
     if (a.m_host) {
@@ -411,6 +411,10 @@ class Engine : public EngineBase {
     out->num_k = last_num_k_;
     out->alpha = alpha_;
     out->rbe_S = rbe_S_;
+    out->pairs_a = (int64_t)c.sum_count_a;
+    out->pairs_b = (int64_t)c.sum_count_b;
+    out->atoms = n_;
+    out->charged = d_.nc;
   }
 
   int64_t export_pairs(int64_t step, const void* px, const void* py,
@@ -465,16 +469,16 @@ class Engine : public EngineBase {
     if (!timing_) return 0;
     if (last_stream_set_) HIPCHK(hipStreamSynchronize(last_stream_));
     double acc[5] = {0, 0, 0, 0, 0};
-    const size_t calls = timing_events_.size();
-    for (auto& ev : timing_events_) {
+    const size_t calls = timing_used_;
+    for (size_t c = 0; c < calls; ++c) {
+      auto& ev = timing_events_[c];
       for (int k = 0; k < 5; ++k) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
         acc[k] += ms * 1000.0;
       }
-      for (auto e : ev) hipEventDestroy(e);
     }
-    timing_events_.clear();
+    timing_used_ = 0;
     const int m = std::min(n, 5);
     for (int k = 0; k < m; ++k) {
       if (names) names[k] = kNames[k];
@@ -518,7 +522,8 @@ class Engine : public EngineBase {
   int64_t calls_ = 0, forced_ = 0;
   int last_num_k_ = 0;
   bool timing_ = false;
-  std::vector<std::array<hipEvent_t, 6>> timing_events_;
+  std::vector<std::array<hipEvent_t, 6>> timing_events_;  // reusable pool
+  size_t timing_used_ = 0;
   unsigned rebuild_thr_ = 0;
 
   template <typename T>

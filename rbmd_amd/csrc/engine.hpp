@@ -24,8 +24,8 @@ struct Ctrl {
   int max_count_b;            // largest charged-pair candidate count seen
   unsigned long long rebuild_count;
   unsigned long long step_count;
-  int nonfinite;
-  int pad;
+  unsigned long long sum_count_a;  // stored candidates of the last rebuild
+  unsigned long long sum_count_b;
 };
 
 template <typename R>

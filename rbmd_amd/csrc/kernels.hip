@@ -230,7 +230,11 @@ __global__ __launch_bounds__(1024) void k_scan(int* key_count, int* key_start,
                                                int* cstart, int ncell,
                                                Ctrl* ctrl, unsigned thr) {
   if (!rebuild_now(ctrl, thr)) return;
-  if (threadIdx.x == 0) ctrl->rebuild_count += 1;
+  if (threadIdx.x == 0) {
+    ctrl->rebuild_count += 1;
+    ctrl->sum_count_a = 0ull;
+    ctrl->sum_count_b = 0ull;
+  }
   __shared__ int s[1024];
   const int nkeys = 2 * ncell;
   const int t = threadIdx.x;
@@ -375,7 +379,11 @@ __global__ __launch_bounds__(256) void k_list_a(Dev<R> d, Box<R> b, Grid g,
     d.cnt_a[i] = count;
   }
   const int m = wave_max_i(count);
-  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(&d.ctrl->max_count_a, m);
+  const int tot = wave_sum(count);
+  if ((threadIdx.x & 63) == 0 && m > 0) {
+    atomicMax(&d.ctrl->max_count_a, m);
+    atomicAdd(&d.ctrl->sum_count_a, (unsigned long long)tot);
+  }
 }
 
 // Charged-pair candidate list (charged compact indices), stencil ext_b.
@@ -416,7 +424,11 @@ __global__ __launch_bounds__(256) void k_list_b(Dev<R> d, Box<R> b, Grid g,
     d.cnt_b[c] = count;
   }
   const int m = wave_max_i(count);
-  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(&d.ctrl->max_count_b, m);
+  const int tot = wave_sum(count);
+  if ((threadIdx.x & 63) == 0 && m > 0) {
+    atomicMax(&d.ctrl->max_count_b, m);
+    atomicAdd(&d.ctrl->sum_count_b, (unsigned long long)tot);
+  }
 }
 
 // ---------------------------------------------------------------------------
