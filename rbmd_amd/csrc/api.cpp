@@ -602,16 +602,67 @@ class Engine : public EngineBase {
       if (!(rmax < 0.5 * L_[d]))
         throw Error(RBMDX_E_CONFIG,
                     "neighbor cutoff + skin must be below half the shortest box edge");
+    // cells of side >= ra (the 27-cell stencil covers the all-atom list)
     for (int d = 0; d < 3; ++d) {
       int ncell = (int)std::floor(L_[d] / (ra * (1.0 + 1e-5)));
       if (ncell < 1) ncell = 1;
+      if (ncell > 1023) ncell = 1023;
       grid_.nc[d] = ncell;
       const double side = L_[d] / ncell;
-      grid_.ext_a[d] = std::max(1, (int)std::ceil(ra * (1.0 + 1e-5) / side));
       grid_.ext_b[d] = std::max(1, (int)std::ceil(rb * (1.0 + 1e-5) / side));
       grid_.scale[d] = (float)(ncell / L_[d]);
     }
     grid_.ncell = grid_.nc[0] * grid_.nc[1] * grid_.nc[2];
+    // bricks: 2x2x2 cells when the 4x4x4-cell neighbourhood fits the LDS
+    // stage with margin, else single cells
+    {
+      const double apc = (double)n_ / grid_.ncell;
+      const int stage_cap = (32 * 1024) / (int)sizeof(R4);
+      grid_.bw = (64.0 * apc * 1.3 <= stage_cap) ? 2 : 1;
+      int nb = 1;
+      for (int d = 0; d < 3; ++d) {
+        grid_.nb[d] = (grid_.nc[d] + grid_.bw - 1) / grid_.bw;
+        nb *= grid_.nb[d];
+      }
+      grid_.nbrick = nb;
+      const int per = (int)std::ceil(1.25 * apc * grid_.bw * grid_.bw * grid_.bw);
+      grid_.block = std::min(512, std::max(64, (per + 63) / 64 * 64));
+      // brick-major cell numbering
+      std::vector<int> c_of_xyz(grid_.ncell), xyz_of_c(grid_.ncell),
+          b0(nb + 1);
+      int next = 0;
+      const int* ncd = grid_.nc;
+      for (int bz = 0; bz < grid_.nb[2]; ++bz)
+        for (int by = 0; by < grid_.nb[1]; ++by)
+          for (int bx = 0; bx < grid_.nb[0]; ++bx) {
+            const int b = (bz * grid_.nb[1] + by) * grid_.nb[0] + bx;
+            b0[b] = next;
+            const int o[3] = {bx * grid_.bw, by * grid_.bw, bz * grid_.bw};
+            const int w[3] = {std::min(grid_.bw, ncd[0] - o[0]),
+                              std::min(grid_.bw, ncd[1] - o[1]),
+                              std::min(grid_.bw, ncd[2] - o[2])};
+            for (int lz = 0; lz < w[2]; ++lz)
+              for (int ly = 0; ly < w[1]; ++ly)
+                for (int lx = 0; lx < w[0]; ++lx) {
+                  const int cx = o[0] + lx, cy = o[1] + ly, cz = o[2] + lz;
+                  c_of_xyz[(cz * ncd[1] + cy) * ncd[0] + cx] = next;
+                  xyz_of_c[next] = cx | (cy << 10) | (cz << 20);
+                  ++next;
+                }
+          }
+      b0[nb] = next;
+      int* t0 = alloc<int>(grid_.ncell);
+      int* t1 = alloc<int>(grid_.ncell);
+      int* t2 = alloc<int>(nb + 1);
+      HIPCHK(hipMemcpy(t0, c_of_xyz.data(), sizeof(int) * grid_.ncell,
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(t1, xyz_of_c.data(), sizeof(int) * grid_.ncell,
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(t2, b0.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+      grid_.cell_of_coord = t0;
+      grid_.coord_of_cell = t1;
+      grid_.brick_cell0 = t2;
+    }
     for (int d = 0; d < 3; ++d) {
       box_.lo[d] = (R)lo_[d];
       box_.L[d] = (R)L_[d];
@@ -665,6 +716,8 @@ class Engine : public EngineBase {
     d_.ref = alloc<R4>(n_);
     d_.key = alloc<int>(n_);
     d_.slot = alloc<int>(n_);
+    d_.inv_perm = alloc<int>(n_);
+    d_.c_of = alloc<int>(n_);
     d_.key_count = alloc<int>(2 * (size_t)grid_.ncell);
     d_.key_start = alloc<int>(2 * (size_t)grid_.ncell + 1);
     d_.cstart = alloc<int>((size_t)grid_.ncell + 1);
@@ -678,14 +731,11 @@ class Engine : public EngineBase {
     d_.gidc = alloc<int>(nc);
     d_.typec = alloc<int>(nc);
     d_.xqc = alloc<R4>(nc);
-    d_.fb = alloc<R4>(nc);
     d_.fk = alloc<R4>(nc);
     d_.cnt_a = alloc<int>(n_);
     d_.cnt_b = alloc<int>(nc);
-    const int na = (n_ + kBlock - 1) / kBlock;
-    const int nbb = (nc + kBlock - 1) / kBlock;
-    d_.pair_part_blocks = na + nbb;
-    d_.pair_part = alloc<double>((size_t)11 * (na + nbb));
+    d_.pair_part_blocks = grid_.nbrick;
+    d_.pair_part = alloc<double>((size_t)11 * grid_.nbrick);
     d_.thermo = alloc<double>(32);
     d_.mean = alloc<double>(3);
     d_.ctrl = alloc<Ctrl>(1);
@@ -695,7 +745,6 @@ class Engine : public EngineBase {
     HIPCHK(hipMemset(d_.thermo, 0, sizeof(double) * 32));
     HIPCHK(hipMemset(d_.mean, 0, sizeof(double) * 3));
     HIPCHK(hipMemset(d_.ref, 0, sizeof(R4) * n_));
-    HIPCHK(hipMemset(d_.fb, 0, sizeof(R4) * std::max(nc, 1)));
     HIPCHK(hipMemset(d_.fk, 0, sizeof(R4) * std::max(nc, 1)));
     {
       void* hp = nullptr;
@@ -846,7 +895,7 @@ class Engine : public EngineBase {
     const int cap_b = (int)std::ceil(max_b * 1.2) + 16;
     if (d_.list_a) release(d_.list_a);
     if (d_.list_b) release(d_.list_b);
-    d_.list_a = alloc<int>((size_t)cap_a * n_);
+    d_.list_a = alloc<uint16_t>((size_t)cap_a * n_);
     d_.cap_a = cap_a;
     if (d_.nc > 0) {
       d_.list_b = alloc<int>((size_t)cap_b * d_.nc);

@@ -15,6 +15,10 @@ constexpr int kSkTile = 8;       // k-vectors per S(k) block
 constexpr int kSkChunk = 1024;   // charged atoms per S(k) block
 constexpr int kRbeWaves = 8;     // waves per RBE-force block (k split)
 constexpr int kMaxK = 1 << 20;   // k-vector capacity bound
+constexpr int kMaxSlots = 64;    // neighbourhood cells staged per brick
+constexpr int kStageCap = 2048;  // neighbourhood atoms staged in LDS
+constexpr int kChargedLanes = 16;  // lanes per charged atom (Coulomb pairs)
+constexpr int kScanPer = 32;     // keys per thread in the one-block scan
 
 // Device control block: rebuild decision, overflow bookkeeping.
 struct Ctrl {
@@ -26,6 +30,8 @@ struct Ctrl {
   unsigned long long step_count;
   unsigned long long sum_count_a;  // stored candidates of the last rebuild
   unsigned long long sum_count_b;
+  int max_stage;              // largest brick neighbourhood seen
+  int unstaged_bricks;        // bricks whose neighbourhood exceeded LDS
 };
 
 template <typename R>
@@ -37,13 +43,21 @@ struct Box {
   R invL[3];
 };
 
-// Cell grid shared by the list builders and the overflow fallback.
+// Cell grid.  Cells are numbered brick-major: the cells of one brick (a
+// bw x bw x bw block of cells, ragged at the upper edges) are consecutive, so
+// the cell-sorted atoms of a brick form one contiguous range.
 struct Grid {
   int nc[3];       // cells per dimension
   int ncell;
-  int ext_a[3];    // stencil half-width for the all-atom list
   int ext_b[3];    // stencil half-width for the charged-pair list
   float scale[3];  // nc / L (for cell index)
+  int bw;          // brick width in cells
+  int nb[3];       // bricks per dimension
+  int nbrick;
+  int block;       // threads per brick block
+  const int* cell_of_coord = nullptr;  // [ncell] (z*ny+y)*nx+x -> cell id
+  const int* coord_of_cell = nullptr;  // [ncell] cell id -> packed x|y<<10|z<<20
+  const int* brick_cell0 = nullptr;    // [nbrick+1] first cell id of a brick
 };
 
 template <typename R>
@@ -71,14 +85,16 @@ struct Dev {
   R* fx = nullptr;
   R* fy = nullptr;
   R* fz = nullptr;
-  int* type = nullptr;     // [n] caller order
-  int* gid = nullptr;      // [n] caller order
-  R* charge = nullptr;     // [n] caller order
-  R4* ref = nullptr;       // [n] positions at the last rebuild (caller order)
-  int* key = nullptr;      // [n] cell key of the last rebuild (caller order)
+  int* type = nullptr;     // [n]
+  int* gid = nullptr;      // [n]
+  R* charge = nullptr;     // [n]
+  R4* ref = nullptr;       // [n] positions at the last rebuild
+  int* key = nullptr;      // [n] cell key of the last rebuild
   int* slot = nullptr;     // [n] arrival slot inside the key
+  int* inv_perm = nullptr; // [n] caller -> sorted
+  int* c_of = nullptr;     // [n] caller -> charged compact, -1 if neutral
   // grid
-  int* key_count = nullptr;  // [2*ncell]
+  int* key_count = nullptr;  // [2*ncell padded]
   int* key_start = nullptr;  // [2*ncell+1]
   int* cstart = nullptr;     // [ncell+1] charged atoms before the cell
   // sorted order
@@ -87,19 +103,18 @@ struct Dev {
   int* gid_s = nullptr;    // [n]
   int* s2c = nullptr;      // [n] sorted -> charged compact, -1 if neutral
   R4* xs = nullptr;        // [n] (x, y, z, type)
-  R4* fa = nullptr;        // [n] all-atom list pair force
+  R4* fa = nullptr;        // [n] total pair force
   // charged compact
   int* c2s = nullptr;      // [nc]
   int* gidc = nullptr;     // [nc]
   int* typec = nullptr;    // [nc]
   R4* xqc = nullptr;       // [nc] (x, y, z, q)
-  R4* fb = nullptr;        // [nc] charged-pair force
   R4* fk = nullptr;        // [nc] k-space force
-  // lists (column-major: list[k * rows + i])
+  // lists
   int cap_a = 0, cap_b = 0;
-  int* list_a = nullptr;
+  uint16_t* list_a = nullptr;  // [cap_a * n] column-major, brick-local index
   int* cnt_a = nullptr;
-  int* list_b = nullptr;
+  int* list_b = nullptr;       // [nc * cap_b] row-major, charged compact index
   int* cnt_b = nullptr;
   // pair partial sums per block: 3 force + 2 energy + 6 virial
   double* pair_part = nullptr;

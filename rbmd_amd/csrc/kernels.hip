@@ -1,19 +1,24 @@
 // HIP kernels of the RBMD force path for gfx950 (CDNA4, wave64).
 //
 // Per force evaluation (one stream, no host synchronisation):
-//   k_skin       max displacement since the last rebuild -> rebuild decision
-//                (also stages the host k-sample into device memory)
+//   k_skin        max displacement since the last rebuild -> rebuild decision
+//                 (also stages the host k-sample into device memory)
 //   k_bin .. k_list_b   (early exit unless the decision says rebuild)
-//                cell keys, counting sort with a deterministic in-cell order,
-//                all-atom and charged-pair candidate lists (cut + skin)
-//   k_gather     caller-order positions -> cell-sorted float4 tiles
-//   k_pair_a     LJ over the all-atom list, RBL classification on the fly
-//   k_pair_b     LJ + erfc Coulomb over the charged-pair list
-//   k_sk         S(k) partial sums over the charged atoms (k tiles x atoms)
-//   k_finalize   one block: FP64 reductions of S(k), energy, virial, mean
-//                pair force; per-k force coefficients
-//   k_rbe_force  per charged atom, k split over the waves of a block
-//   k_assemble   F = F_pair - <F_pair> + F_kspace, scattered to caller order
+//                 cell keys, counting sort with a deterministic in-cell order
+//                 (wave bitonic), brick-local all-atom candidate lists built
+//                 from LDS-staged neighbourhoods, charged-pair lists
+//   k_gather      caller-order positions -> cell-sorted tiles
+//   k_pair        one workgroup per brick: the brick's 4x4x4-cell
+//                 neighbourhood is staged in LDS; LJ pairs gather from LDS
+//                 through 16-bit brick-local lists with the RBL
+//                 classification applied on the fly; charged atoms then add
+//                 LJ + erfc Coulomb over their charged-pair lists, 16 lanes
+//                 per atom
+//   k_sk          S(k) partial sums over the charged atoms (k tiles x atoms)
+//   k_finalize    one block: FP64 reductions of S(k), energy, virial, mean
+//                 pair force; per-k force coefficients
+//   k_rbe_force   per charged atom, k split over the waves of a block
+//   k_assemble    F = F_pair - <F_pair> + F_kspace in caller order
 #include <hip/hip_runtime.h>
 
 #include "engine.hpp"
@@ -23,6 +28,12 @@ namespace {
 
 template <typename R>
 using R4T = typename Dev<R>::R4;
+
+// LDS budget of one brick stage (32 KB of positions).
+template <typename R>
+struct StageCap {
+  static constexpr int value = (32 * 1024) / (int)sizeof(R4T<R>);
+};
 
 __device__ __forceinline__ float mul_rn(float a, float b) {
   return __fmul_rn(a, b);
@@ -108,15 +119,16 @@ __device__ __forceinline__ int cell_coord(float u, float scale, int n) {
   return c < 0 ? 0 : (c >= n ? n - 1 : c);
 }
 
+__device__ __forceinline__ int wrapi(int c, int n) {
+  c %= n;
+  return c < 0 ? c + n : c;
+}
+
 // Stencil along one dimension: `ext` cells either side, or the whole
 // dimension once when the stencil would wrap onto itself.
 struct Span {
   int start, count, n;
-  __device__ __forceinline__ int at(int k) const {
-    int c = start + k;
-    c %= n;
-    return c < 0 ? c + n : c;
-  }
+  __device__ __forceinline__ int at(int k) const { return wrapi(start + k, n); }
 };
 __device__ __forceinline__ Span make_span(int c, int ext, int n) {
   Span s;
@@ -144,6 +156,15 @@ __device__ __forceinline__ int wave_max_i(int v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
   return v;
 }
+// Inclusive prefix sum across the wave.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
 
 // Block sum of NV doubles (blockDim.x multiple of 64, <= 1024).  Result valid
 // in thread 0.  Fixed reduction tree: deterministic.
@@ -168,14 +189,168 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* smem) {
   __syncthreads();
 }
 
+// Exclusive prefix over the block (blockDim <= 1024, multiple of 64); the
+// block total is returned in `total`.
+__device__ __forceinline__ int block_excl_scan(int v, int& total, int* smem) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  const int inc = wave_incl_scan(v);
+  if (lane == 63) smem[wid] = inc;
+  __syncthreads();
+  if (wid == 0) {
+    const int wv = lane < nw ? smem[lane] : 0;
+    const int wi = wave_incl_scan(wv);
+    __builtin_amdgcn_wave_barrier();
+    if (lane < nw) smem[lane] = wi - wv;  // exclusive wave offsets
+    if (lane == nw - 1) smem[32] = wi;
+  }
+  __syncthreads();
+  const int r = smem[wid] + inc - v;
+  total = smem[32];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Brick neighbourhood.  A brick is a bw^3 block of cells; its neighbourhood
+// adds one cell on every side (the whole dimension when that would wrap onto
+// itself).  Slots are enumerated x-fastest; every slot holds one cell whose
+// atoms are contiguous in sorted order (charged first).
+// ---------------------------------------------------------------------------
+struct Neigh {
+  int start[3], cnt[3], full[3];
+  int nslots;
+  int o[3];          // brick origin (cell coords)
+  int cell0, cell1;  // cell id range of the brick
+};
+
+__device__ __forceinline__ Neigh brick_neigh(const Grid& g, int b) {
+  Neigh h;
+  const int bx = b % g.nb[0], by = (b / g.nb[0]) % g.nb[1],
+            bz = b / (g.nb[0] * g.nb[1]);
+  const int bc[3] = {bx, by, bz};
+  for (int d = 0; d < 3; ++d) {
+    h.o[d] = bc[d] * g.bw;
+    const int w = min(g.bw, g.nc[d] - h.o[d]);
+    if (w + 2 >= g.nc[d]) {
+      h.start[d] = 0;
+      h.cnt[d] = g.nc[d];
+      h.full[d] = 1;
+    } else {
+      h.start[d] = h.o[d] - 1;
+      h.cnt[d] = w + 2;
+      h.full[d] = 0;
+    }
+  }
+  h.nslots = h.cnt[0] * h.cnt[1] * h.cnt[2];
+  h.cell0 = g.brick_cell0[b];
+  h.cell1 = g.brick_cell0[b + 1];
+  return h;
+}
+
+// Slot tables in LDS: off (prefix of atom counts), first (sorted index of the
+// slot cell's first atom), nq (charged atoms of the slot cell).
+struct SlotTables {
+  int off[kMaxSlots + 1];
+  int first[kMaxSlots];
+  int nq[kMaxSlots];
+  int scan[33];
+};
+
+__device__ __forceinline__ void build_slots(const Grid& g, const Neigh& h,
+                                            const int* key_start,
+                                            SlotTables& t) {
+  int cnt = 0;
+  if ((int)threadIdx.x < h.nslots) {
+    const int s = threadIdx.x;
+    const int kx = s % h.cnt[0], ky = (s / h.cnt[0]) % h.cnt[1],
+              kz = s / (h.cnt[0] * h.cnt[1]);
+    const int cx = wrapi(h.start[0] + kx, g.nc[0]);
+    const int cy = wrapi(h.start[1] + ky, g.nc[1]);
+    const int cz = wrapi(h.start[2] + kz, g.nc[2]);
+    const int cell = g.cell_of_coord[(cz * g.nc[1] + cy) * g.nc[0] + cx];
+    const int a = key_start[2 * cell];
+    t.first[s] = a;
+    t.nq[s] = key_start[2 * cell + 1] - a;
+    cnt = key_start[2 * cell + 2] - a;
+  }
+  int total;
+  const int ex = block_excl_scan(cnt, total, t.scan);
+  if ((int)threadIdx.x < h.nslots) t.off[threadIdx.x] = ex;
+  if (threadIdx.x == 0) t.off[h.nslots] = total;
+  __syncthreads();
+}
+
+// Slot holding stage position l (binary search over the prefix table).
+__device__ __forceinline__ int slot_of_pos(const SlotTables& t, int nslots,
+                                           int l) {
+  int lo = 0, hi = nslots - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t.off[mid] <= l)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename R>
+__device__ __forceinline__ void stage_atoms(const Neigh& h,
+                                            const SlotTables& t,
+                                            const R4T<R>* xs, R4T<R>* stage) {
+  const int total = t.off[h.nslots];
+  for (int l = threadIdx.x; l < total; l += blockDim.x) {
+    const int s = slot_of_pos(t, h.nslots, l);
+    stage[l] = xs[t.first[s] + (l - t.off[s])];
+  }
+  __syncthreads();
+}
+
+// The 27-cell stencil of an atom in cell coords c, as slot indices.
+struct Stencil {
+  int n[3];
+  int sl[3][3];
+};
+__device__ __forceinline__ Stencil atom_stencil(const Grid& g, const Neigh& h,
+                                                const int c[3]) {
+  Stencil st;
+  for (int d = 0; d < 3; ++d) {
+    if (h.full[d] && g.nc[d] < 3) {
+      st.n[d] = g.nc[d];
+      for (int k = 0; k < 3; ++k) st.sl[d][k] = k < g.nc[d] ? k : 0;
+    } else if (h.full[d]) {
+      st.n[d] = 3;
+      for (int k = 0; k < 3; ++k) st.sl[d][k] = wrapi(c[d] - 1 + k, g.nc[d]);
+    } else {
+      st.n[d] = 3;
+      for (int k = 0; k < 3; ++k) st.sl[d][k] = c[d] - 1 + k - h.start[d];
+    }
+  }
+  return st;
+}
+
+__device__ __forceinline__ int own_slot(const Neigh& h, const int c[3]) {
+  int k[3];
+  for (int d = 0; d < 3; ++d) k[d] = h.full[d] ? c[d] : c[d] - h.start[d];
+  return (k[2] * h.cnt[1] + k[1]) * h.cnt[0] + k[0];
+}
+
+__device__ __forceinline__ void unpack_coord(int packed, int c[3]) {
+  c[0] = packed & 1023;
+  c[1] = (packed >> 10) & 1023;
+  c[2] = (packed >> 20) & 1023;
+}
+
 // ---------------------------------------------------------------------------
 // Rebuild decision.  Max |r - r_ref|^2 (minimum image), as float bits in
-// ctrl->maxdisp_bits; +inf when a rebuild is forced.  Block 0 also copies the
-// k-sample from host-mapped staging memory.
+// ctrl->maxdisp_bits; +inf when a rebuild is forced.  One atomic per block.
+// Block 0 also copies the k-sample from host-mapped staging memory.
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force,
                                               const int* m_host, int nk3) {
+  __shared__ float s_max[4];
   if (m_host && blockIdx.x == 0)
     for (int t = threadIdx.x; t < nk3; t += blockDim.x) d.m_dev[t] = m_host[t];
   if (force) {
@@ -197,13 +372,18 @@ __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force,
     mx = fmaxf(mx, d2 != d2 ? __int_as_float(0x7f800000) : d2);
   }
   mx = wave_max(mx);
-  if ((threadIdx.x & 63) == 0 && mx > 0.f)
-    atomicMax(&d.ctrl->maxdisp_bits, __float_as_uint(mx));
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = s_max[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, s_max[w]);
+    if (m > 0.f) atomicMax(&d.ctrl->maxdisp_bits, __float_as_uint(m));
+  }
 }
 
 // Cell key per atom (caller order): key = 2 * cell + neutral, so that inside
 // every cell the charged atoms come first.  Arrival slots from atomics; the
-// in-cell order is made deterministic by k_segsort.
+// in-cell order is made deterministic by k_cellsort.
 template <typename R>
 __global__ __launch_bounds__(256) void k_bin(Dev<R> d, Box<R> b, Grid g,
                                              int split, unsigned thr) {
@@ -216,7 +396,7 @@ __global__ __launch_bounds__(256) void k_bin(Dev<R> d, Box<R> b, Grid g,
     const int cx = cell_coord((float)(x - b.lo[0]), g.scale[0], g.nc[0]);
     const int cy = cell_coord((float)(y - b.lo[1]), g.scale[1], g.nc[1]);
     const int cz = cell_coord((float)(z - b.lo[2]), g.scale[2], g.nc[2]);
-    const int cell = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+    const int cell = g.cell_of_coord[(cz * g.nc[1] + cy) * g.nc[0] + cx];
     const int neutral = (!split || d.charge[i] == R(0)) ? 1 : 0;
     const int key = 2 * cell + neutral;
     d.key[i] = key;
@@ -224,52 +404,48 @@ __global__ __launch_bounds__(256) void k_bin(Dev<R> d, Box<R> b, Grid g,
   }
 }
 
-// Exclusive scans (one block of 1024): key_start over all keys, cstart over
-// the charged counts of each cell.  Clears key_count for the next rebuild.
+// Exclusive scans in one block of 1024: key_start over all keys and cstart
+// over the charged count of every cell, kScanPer consecutive values per
+// thread loaded together; clears key_count for the next rebuild.
 __global__ __launch_bounds__(1024) void k_scan(int* key_count, int* key_start,
                                                int* cstart, int ncell,
                                                Ctrl* ctrl, unsigned thr) {
   if (!rebuild_now(ctrl, thr)) return;
-  if (threadIdx.x == 0) {
+  __shared__ int smem[33];
+  const int t = threadIdx.x;
+  if (t == 0) {
     ctrl->rebuild_count += 1;
     ctrl->sum_count_a = 0ull;
     ctrl->sum_count_b = 0ull;
   }
-  __shared__ int s[1024];
-  const int nkeys = 2 * ncell;
-  const int t = threadIdx.x;
+  const int chunk = 1024 * kScanPer;
   for (int pass = 0; pass < 2; ++pass) {
-    const int n = pass == 0 ? nkeys : ncell;
-    const int per = (n + 1023) / 1024;
-    const int a = t * per, e = min(n, a + per);
-    int sum = 0;
-    for (int k = a; k < e; ++k) sum += pass == 0 ? key_count[k] : key_count[2 * k];
-    s[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const int v = t >= off ? s[t - off] : 0;
-      __syncthreads();
-      s[t] += v;
-      __syncthreads();
+    const int n = pass == 0 ? 2 * ncell : ncell;
+    const int stride = pass == 0 ? 1 : 2;
+    int* out = pass == 0 ? key_start : cstart;
+    int carry = 0;
+    for (int base = 0; base < n; base += chunk) {
+      int v[kScanPer];
+      const int a = base + t * kScanPer;
+#pragma unroll
+      for (int k = 0; k < kScanPer; ++k)
+        v[k] = (a + k < n) ? key_count[(size_t)(a + k) * stride] : 0;
+      int sum = 0;
+#pragma unroll
+      for (int k = 0; k < kScanPer; ++k) sum += v[k];
+      int total;
+      int run = carry + block_excl_scan(sum, total, smem);
+#pragma unroll
+      for (int k = 0; k < kScanPer; ++k) {
+        if (a + k < n) out[a + k] = run;
+        run += v[k];
+      }
+      carry += total;
     }
-    int run = s[t] - sum;  // exclusive prefix of this thread's segment
-    for (int k = a; k < e; ++k) {
-      const int c = pass == 0 ? key_count[k] : key_count[2 * k];
-      if (pass == 0)
-        key_start[k] = run;
-      else
-        cstart[k] = run;
-      run += c;
-    }
-    if (t == 1023) {
-      if (pass == 0)
-        key_start[nkeys] = s[1023];
-      else
-        cstart[ncell] = s[1023];
-    }
-    __syncthreads();
+    if (t == 0) out[n] = carry;
   }
-  for (int k = t; k < nkeys; k += 1024) key_count[k] = 0;
+  __syncthreads();
+  for (int k = t; k < 2 * ncell; k += 1024) key_count[k] = 0;
 }
 
 template <typename R>
@@ -280,169 +456,217 @@ __global__ __launch_bounds__(256) void k_scatter(Dev<R> d, unsigned thr) {
     d.perm[d.key_start[d.key[i]] + d.slot[i]] = i;
 }
 
-// Sort every key segment of perm by caller index: a stable counting sort.
-__global__ __launch_bounds__(256) void k_segsort(int* perm,
-                                                 const int* key_start,
-                                                 int nkeys, const Ctrl* ctrl,
-                                                 unsigned thr) {
+// One wave per cell: both key segments are sorted by caller index (bitonic
+// across the wave; serial by lane 0 beyond 64 atoms), then inv_perm is set.
+__global__ __launch_bounds__(256) void k_cellsort(int* perm, int* inv_perm,
+                                                  const int* key_start,
+                                                  int ncell, const Ctrl* ctrl,
+                                                  unsigned thr) {
   if (!rebuild_now(ctrl, thr)) return;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nkeys;
-       k += gridDim.x * blockDim.x) {
-    const int a = key_start[k], e = key_start[k + 1];
-    for (int p = a + 1; p < e; ++p) {
-      const int v = perm[p];
-      int q = p - 1;
-      while (q >= a && perm[q] > v) {
-        perm[q + 1] = perm[q];
-        --q;
+  const int lane = threadIdx.x & 63;
+  const int cell = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (cell >= ncell) return;
+  for (int seg = 0; seg < 2; ++seg) {
+    const int a = key_start[2 * cell + seg], e = key_start[2 * cell + seg + 1];
+    const int len = e - a;
+    if (len <= 1) continue;
+    if (len <= 64) {
+      int v = lane < len ? perm[a + lane] : 0x7fffffff;
+      for (int k = 2; k <= 64; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          const int p = __shfl_xor(v, j, 64);
+          const bool up = (lane & k) == 0;
+          const bool lower = (lane & j) == 0;
+          v = (lower == up) ? min(v, p) : max(v, p);
+        }
+      if (lane < len) perm[a + lane] = v;
+    } else if (lane == 0) {
+      for (int p = a + 1; p < e; ++p) {
+        const int v = perm[p];
+        int q = p - 1;
+        while (q >= a && perm[q] > v) {
+          perm[q + 1] = perm[q];
+          --q;
+        }
+        perm[q + 1] = v;
       }
-      perm[q + 1] = v;
     }
   }
+  __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+  const int a = key_start[2 * cell], e = key_start[2 * cell + 2];
+  for (int s = a + lane; s < e; s += 64) inv_perm[perm[s]] = s;
 }
 
-// Caller-order positions -> sorted tiles (every evaluation).  On rebuild
-// also records reference positions, sorted ids/cells and the charged map.
+// Caller-order positions -> sorted tiles (every evaluation, one thread per
+// caller atom: coalesced reads, scattered writes).  On rebuild also records
+// reference positions, sorted ids/cells and the charged map.
 template <typename R>
 __global__ __launch_bounds__(256) void k_gather(Dev<R> d, Box<R> b,
                                                 unsigned thr) {
   const bool rb = rebuild_now(d.ctrl, thr);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
        i += gridDim.x * blockDim.x) {
-    const int src = d.perm[i];
-    const R x = wrap<R>(d.px[src], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
-    const R y = wrap<R>(d.py[src], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
-    const R z = wrap<R>(d.pz[src], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
-    const int t = d.type[src];
-    d.xs[i] = make4<R>(x, y, z, (R)t);
+    const int s = d.inv_perm[i];
+    const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
+    const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
+    const R z = wrap<R>(d.pz[i], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
+    const int t = d.type[i];
+    d.xs[s] = make4<R>(x, y, z, (R)t);
     int c;
     if (rb) {
-      d.ref[src] = make4<R>(x, y, z, R(0));
-      d.gid_s[i] = d.gid[src];
-      const int key = d.key[src];
-      d.cell_s[i] = key >> 1;
+      d.ref[i] = make4<R>(x, y, z, R(0));
+      d.gid_s[s] = d.gid[i];
+      const int key = d.key[i];
+      d.cell_s[s] = key >> 1;
       if (!(key & 1)) {
-        c = d.cstart[key >> 1] + (i - d.key_start[key]);
-        d.c2s[c] = i;
-        d.gidc[c] = d.gid[src];
+        c = d.cstart[key >> 1] + (s - d.key_start[key]);
+        d.c2s[c] = s;
+        d.gidc[c] = d.gid[i];
         d.typec[c] = t;
       } else {
         c = -1;
       }
-      d.s2c[i] = c;
+      d.s2c[s] = c;
+      d.c_of[i] = c;
     } else {
-      c = d.s2c[i];
+      c = d.c_of[i];
     }
-    if (c >= 0) d.xqc[c] = make4<R>(x, y, z, d.charge[src]);
+    if (c >= 0) d.xqc[c] = make4<R>(x, y, z, d.charge[i]);
   }
 }
 
-// All-atom candidate list: every j != i within r2 in the stencil, except
-// charged-charged pairs when the work is split (those go to list B).
-// Column-major storage list[k * n + i].  count_only: counts, no writes.
+// All-atom candidate list of one brick: every j != i within r2 among the 27
+// cells around i, except charged-charged pairs when the work is split (those
+// go to list B).  Entries are 16-bit positions in the brick's staged
+// neighbourhood, column-major list[k * n + i].  count_only: no writes.
 template <typename R>
-__global__ __launch_bounds__(256) void k_list_a(Dev<R> d, Box<R> b, Grid g,
+__global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
                                                 R r2, int split, int cap,
                                                 unsigned thr) {
   if (!rebuild_now(d.ctrl, thr)) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int count = 0;
-  if (i < d.n) {
+  __shared__ R4T<R> stage[StageCap<R>::value];
+  __shared__ SlotTables t;
+  const Neigh h = brick_neigh(g, blockIdx.x);
+  build_slots(g, h, d.key_start, t);
+  const int total = t.off[h.nslots];
+  const bool staged = total <= StageCap<R>::value;
+  if (staged) stage_atoms<R>(h, t, d.xs, stage);
+  const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
+  int mx = 0, sum = 0;
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    int c[3];
+    unpack_coord(g.coord_of_cell[d.cell_s[i]], c);
+    const Stencil st = atom_stencil(g, h, c);
+    const int os = own_slot(h, c);
+    const int self = t.off[os] + (i - t.first[os]);
+    const bool qi = split && (i - t.first[os]) < t.nq[os];
     const R4T<R> xi = d.xs[i];
-    const bool qi = split && d.s2c[i] >= 0;
-    const int cell = d.cell_s[i];
-    const int cx = cell % g.nc[0], cy = (cell / g.nc[0]) % g.nc[1],
-              cz = cell / (g.nc[0] * g.nc[1]);
-    const Span sx = make_span(cx, g.ext_a[0], g.nc[0]);
-    const Span sy = make_span(cy, g.ext_a[1], g.nc[1]);
-    const Span sz = make_span(cz, g.ext_a[2], g.nc[2]);
-    for (int kz = 0; kz < sz.count; ++kz)
-      for (int ky = 0; ky < sy.count; ++ky) {
-        const int row = (sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0];
-        for (int kx = 0; kx < sx.count; ++kx) {
-          const int nb = row + sx.at(kx);
-          const int js = qi ? d.key_start[2 * nb + 1] : d.key_start[2 * nb];
-          const int je = d.key_start[2 * nb + 2];
-          for (int j = js; j < je; ++j) {
-            if (j == i) continue;
-            const R4T<R> xj = d.xs[j];
+    int count = 0;
+    // neighbourhoods beyond 16-bit positions are left to the direct scan
+    const int lcap = total > 65535 ? 0 : cap;
+    for (int kz = 0; kz < st.n[2]; ++kz)
+      for (int ky = 0; ky < st.n[1]; ++ky)
+        for (int kx = 0; kx < st.n[0]; ++kx) {
+          const int sl =
+              (st.sl[2][kz] * h.cnt[1] + st.sl[1][ky]) * h.cnt[0] + st.sl[0][kx];
+          const int base = t.off[sl];
+          const int len = t.off[sl + 1] - base;
+          const int js = qi ? t.nq[sl] : 0;
+          for (int k = js; k < len; ++k) {
+            const int l = base + k;
+            if (l == self) continue;
+            const R4T<R> xj = staged ? stage[l] : d.xs[t.first[sl] + k];
             R dx, dy, dz;
-            const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z,
-                                    dx, dy, dz);
+            const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx,
+                                    dy, dz);
             if (d2 < r2) {
-              if (count < cap) d.list_a[(size_t)count * d.n + i] = j;
+              if (count < lcap) d.list_a[(size_t)count * d.n + i] = (uint16_t)l;
               ++count;
             }
           }
         }
-      }
-    d.cnt_a[i] = count;
+    d.cnt_a[i] = (lcap == 0 && cap > 0) ? cap + 1 + count : count;
+    mx = max(mx, count);
+    sum += count;
   }
-  const int m = wave_max_i(count);
-  const int tot = wave_sum(count);
-  if ((threadIdx.x & 63) == 0 && m > 0) {
-    atomicMax(&d.ctrl->max_count_a, m);
-    atomicAdd(&d.ctrl->sum_count_a, (unsigned long long)tot);
+  mx = wave_max_i(mx);
+  sum = wave_sum(sum);
+  if ((threadIdx.x & 63) == 0 && mx > 0) {
+    atomicMax(&d.ctrl->max_count_a, mx);
+    atomicAdd(&d.ctrl->sum_count_a, (unsigned long long)sum);
+  }
+  if (threadIdx.x == 0) {
+    atomicMax(&d.ctrl->max_stage, total);
+    if (!staged) atomicAdd(&d.ctrl->unstaged_bricks, 1);
   }
 }
 
-// Charged-pair candidate list (charged compact indices), stencil ext_b.
+// Charged-pair candidate list: one wave per charged atom; lanes split the
+// ext_b stencil cells, count, prefix-sum and write in a fixed order.
+// Row-major list_b[c * cap + k] of charged compact indices.
 template <typename R>
 __global__ __launch_bounds__(256) void k_list_b(Dev<R> d, Box<R> b, Grid g,
                                                 R r2, int cap, unsigned thr) {
   if (!rebuild_now(d.ctrl, thr)) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (c >= d.nc) return;
+  const int i = d.c2s[c];
+  const R4T<R> xi = d.xs[i];
+  int cc[3];
+  unpack_coord(g.coord_of_cell[d.cell_s[i]], cc);
+  const Span sx = make_span(cc[0], g.ext_b[0], g.nc[0]);
+  const Span sy = make_span(cc[1], g.ext_b[1], g.nc[1]);
+  const Span sz = make_span(cc[2], g.ext_b[2], g.nc[2]);
+  const int nst = sx.count * sy.count * sz.count;
   int count = 0;
-  if (c < d.nc) {
-    const int i = d.c2s[c];
-    const R4T<R> xi = d.xs[i];
-    const int cell = d.cell_s[i];
-    const int cx = cell % g.nc[0], cy = (cell / g.nc[0]) % g.nc[1],
-              cz = cell / (g.nc[0] * g.nc[1]);
-    const Span sx = make_span(cx, g.ext_b[0], g.nc[0]);
-    const Span sy = make_span(cy, g.ext_b[1], g.nc[1]);
-    const Span sz = make_span(cz, g.ext_b[2], g.nc[2]);
-    for (int kz = 0; kz < sz.count; ++kz)
-      for (int ky = 0; ky < sy.count; ++ky) {
-        const int row = (sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0];
-        for (int kx = 0; kx < sx.count; ++kx) {
-          const int nb = row + sx.at(kx);
-          const int js = d.key_start[2 * nb], je = d.key_start[2 * nb + 1];
-          for (int j = js; j < je; ++j) {
-            if (j == i) continue;
-            const R4T<R> xj = d.xs[j];
-            R dx, dy, dz;
-            const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z,
-                                    dx, dy, dz);
-            if (d2 < r2) {
-              if (count < cap) d.list_b[(size_t)count * d.nc + c] = d.s2c[j];
-              ++count;
-            }
+  int pos = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1) pos = wave_incl_scan(count) - count;
+    for (int s = lane; s < nst; s += 64) {
+      const int kx = s % sx.count, ky = (s / sx.count) % sy.count,
+                kz = s / (sx.count * sy.count);
+      const int cell =
+          g.cell_of_coord[(sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0] + sx.at(kx)];
+      const int js = d.key_start[2 * cell], je = d.key_start[2 * cell + 1];
+      for (int j = js; j < je; ++j) {
+        if (j == i) continue;
+        const R4T<R> xj = d.xs[j];
+        R dx, dy, dz;
+        const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+        if (d2 < r2) {
+          if (pass == 0) {
+            ++count;
+          } else {
+            if (pos < cap) d.list_b[(size_t)c * cap + pos] = d.s2c[j];
+            ++pos;
           }
         }
       }
-    d.cnt_b[c] = count;
+    }
   }
-  const int m = wave_max_i(count);
   const int tot = wave_sum(count);
-  if ((threadIdx.x & 63) == 0 && m > 0) {
-    atomicMax(&d.ctrl->max_count_b, m);
-    atomicAdd(&d.ctrl->sum_count_b, (unsigned long long)tot);
+  if (lane == 0) {
+    d.cnt_b[c] = tot;
+    if (tot > 0) {
+      atomicMax(&d.ctrl->max_count_b, tot);
+      atomicAdd(&d.ctrl->sum_count_b, (unsigned long long)tot);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Pair interaction of one directed pair.  Returns the weight w applied to the
-// force (RBL: 1 core, f selected shell, 0 otherwise; Verlet: 1 inside rc).
+// Pair interaction of one directed pair.  Weight applied to the force (RBL:
+// 1 core, f selected shell, 0 otherwise; Verlet: 1 inside rc).
 // ---------------------------------------------------------------------------
-template <typename R, bool RBL>
-__device__ __forceinline__ R pair_weight(const PairConst<R>& pc, R d2,
-                                         bool nonzero, int gi,
-                                         const int* gid_j_ptr) {
+template <typename R, bool RBL, typename GidJ>
+__device__ __forceinline__ R pair_weight(const PairConst<R>& pc, R d2, int gi,
+                                         GidJ gid_j) {
   if (!RBL) return R(1);
   if (d2 < pc.t_core) return (d2 < pc.rs2 && d2 > pc.eps_min) ? R(1) : R(0);
-  if (nonzero && d2 < pc.t_shell && d2 > pc.rs2 &&
-      rbl_selected(pc.ifreq, pc.epoch, gi, *gid_j_ptr))
+  if (d2 < pc.t_shell && d2 > pc.rs2 &&
+      rbl_selected(pc.ifreq, pc.epoch, gi, gid_j()))
     return pc.freq;
   return R(0);
 }
@@ -466,12 +690,13 @@ __device__ __forceinline__ void add_virial(Acc<R>& a, R dx, R dy, R dz, R f) {
   }
 }
 
-// LJ-only pair of list A.
-template <typename R, bool RBL, bool EFLAG>
+// LJ-only pair of list A.  gidj() yields the global id of j (evaluated only
+// for RBL shell pairs).
+template <typename R, bool RBL, bool EFLAG, typename GidJ>
 __device__ __forceinline__ void interact_a(const PairConst<R>& pc,
                                            const Box<R>& b, const R* s_lj,
                                            int ti, const R4T<R>& xi, int gi,
-                                           const R4T<R>& xj, const int* gidj,
+                                           const R4T<R>& xj, GidJ gidj,
                                            Acc<R>& a) {
   R dx, dy, dz;
   const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
@@ -483,7 +708,7 @@ __device__ __forceinline__ void interact_a(const PairConst<R>& pc,
   const R s2 = sig2 * inv;
   const R s6 = s2 * s2 * s2;
   const R flj = R(-24) * eps * (R(2) * s6 - R(1)) * s6 * inv;
-  const R w = pair_weight<R, RBL>(pc, d2, true, gi, gidj);
+  const R w = pair_weight<R, RBL>(pc, d2, gi, gidj);
   const R f = w * flj;
   a.fx += f * dx;
   a.fy += f * dy;
@@ -527,7 +752,7 @@ __device__ __forceinline__ void interact_b(const PairConst<R>& pc,
   }
   R fc, ec;
   coul<R>(pc, d2, xi.w * xj.w, fc, ec);
-  const R w = pair_weight<R, RBL>(pc, d2, true, gi, gidj);
+  const R w = pair_weight<R, RBL>(pc, d2, gi, [&] { return *gidj; });
   const R f = w * (flj + fc);
   a.fx += f * dx;
   a.fy += f * dy;
@@ -543,7 +768,6 @@ template <typename R>
 __device__ __forceinline__ void load_lj(const R* tab, int ntypes, R* s) {
   for (int t = threadIdx.x; t < 2 * ntypes * ntypes; t += blockDim.x)
     s[t] = tab[t];
-  __syncthreads();
 }
 
 template <typename R, bool EFLAG>
@@ -566,101 +790,142 @@ __device__ __forceinline__ void write_partials(const Acc<R>& a, double* part,
   }
 }
 
+// Pair forces of one brick.  Phase 1: LJ over list A for every atom of the
+// brick (one lane per atom, neighbours from the LDS stage).  Phase 2: the
+// brick's charged atoms add LJ + Coulomb over list B, kChargedLanes lanes per
+// atom.  fa[i] receives the total pair force of atom i.
 template <typename R, bool RBL, bool EFLAG>
-__global__ __launch_bounds__(256) void k_pair_a(Dev<R> d, Box<R> b, Grid g,
-                                                PairConst<R> pc,
-                                                const R* lj_tab, int split) {
-  __shared__ R s_lj[2 * kMaxTypes * kMaxTypes];
-  __shared__ double smem[4 * 11];
+__global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
+                                              PairConst<R> pc, const R* lj_tab,
+                                              int split) {
+  __shared__ R4T<R> stage[StageCap<R>::value];
+  __shared__ SlotTables t;
+  __shared__ double smem[8 * 11];
+  extern __shared__ char dyn_lds[];  // LJ table: 2 * ntypes^2 values of R
+  R* s_lj = reinterpret_cast<R*>(dyn_lds);
   load_lj<R>(lj_tab, pc.ntypes, s_lj);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  Acc<R> a;
-  if (i < d.n) {
+  const Neigh h = brick_neigh(g, blockIdx.x);
+  build_slots(g, h, d.key_start, t);
+  const int total = t.off[h.nslots];
+  const bool staged = total <= StageCap<R>::value;
+  if (staged) stage_atoms<R>(h, t, d.xs, stage);
+  const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
+  Acc<R> acc;
+  // phase 1: LJ over list A
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    Acc<R> a;
     const R4T<R> xi = d.xs[i];
     const int ti = (int)xi.w;
     const int gi = d.gid_s[i];
     const int cnt = d.cnt_a[i];
     if (cnt <= d.cap_a) {
-      const int* lp = d.list_a + i;
+      const uint16_t* lp = d.list_a + i;
       for (int k = 0; k < cnt; ++k) {
-        const int j = lp[(size_t)k * d.n];
-        interact_a<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, d.xs[j],
-                                  d.gid_s + j, a);
+        const int l = lp[(size_t)k * d.n];
+        auto global_j = [&]() {
+          const int s = slot_of_pos(t, h.nslots, l);
+          return t.first[s] + (l - t.off[s]);
+        };
+        const R4T<R> xj = staged ? stage[l] : d.xs[global_j()];
+        interact_a<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, xj,
+                                  [&] { return d.gid_s[global_j()]; }, a);
       }
     } else {
-      // list overflow since the last rebuild: scan the stencil directly
-      const bool qi = split && d.s2c[i] >= 0;
-      const int cell = d.cell_s[i];
-      const int cx = cell % g.nc[0], cy = (cell / g.nc[0]) % g.nc[1],
-                cz = cell / (g.nc[0] * g.nc[1]);
-      const Span sx = make_span(cx, g.ext_a[0], g.nc[0]);
-      const Span sy = make_span(cy, g.ext_a[1], g.nc[1]);
-      const Span sz = make_span(cz, g.ext_a[2], g.nc[2]);
-      for (int kz = 0; kz < sz.count; ++kz)
-        for (int ky = 0; ky < sy.count; ++ky)
-          for (int kx = 0; kx < sx.count; ++kx) {
-            const int nb =
-                (sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0] + sx.at(kx);
-            const int js = qi ? d.key_start[2 * nb + 1] : d.key_start[2 * nb];
-            const int je = d.key_start[2 * nb + 2];
-            for (int j = js; j < je; ++j)
-              if (j != i)
-                interact_a<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, d.xs[j],
-                                          d.gid_s + j, a);
-          }
-    }
-    d.fa[i] = make4<R>(a.fx, a.fy, a.fz, R(0));
-  }
-  write_partials<R, EFLAG>(a, d.pair_part, smem);
-}
-
-template <typename R, bool RBL, bool EFLAG>
-__global__ __launch_bounds__(256) void k_pair_b(Dev<R> d, Box<R> b, Grid g,
-                                                PairConst<R> pc,
-                                                const R* lj_tab,
-                                                int part_offset) {
-  __shared__ R s_lj[2 * kMaxTypes * kMaxTypes];
-  __shared__ double smem[4 * 11];
-  load_lj<R>(lj_tab, pc.ntypes, s_lj);
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  Acc<R> a;
-  if (c < d.nc) {
-    const R4T<R> xi = d.xqc[c];
-    const int ti = d.typec[c];
-    const int gi = d.gidc[c];
-    const int cnt = d.cnt_b[c];
-    if (cnt <= d.cap_b) {
-      const int* lp = d.list_b + c;
-      for (int k = 0; k < cnt; ++k) {
-        const int e = lp[(size_t)k * d.nc];
-        interact_b<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, d.xqc[e],
-                                  d.typec[e], d.gidc + e, a);
-      }
-    } else {
-      const int i = d.c2s[c];
-      const int cell = d.cell_s[i];
-      const int cx = cell % g.nc[0], cy = (cell / g.nc[0]) % g.nc[1],
-                cz = cell / (g.nc[0] * g.nc[1]);
-      const Span sx = make_span(cx, g.ext_b[0], g.nc[0]);
-      const Span sy = make_span(cy, g.ext_b[1], g.nc[1]);
-      const Span sz = make_span(cz, g.ext_b[2], g.nc[2]);
-      for (int kz = 0; kz < sz.count; ++kz)
-        for (int ky = 0; ky < sy.count; ++ky)
-          for (int kx = 0; kx < sx.count; ++kx) {
-            const int nb =
-                (sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0] + sx.at(kx);
-            const int js = d.key_start[2 * nb], je = d.key_start[2 * nb + 1];
-            for (int j = js; j < je; ++j) {
-              if (j == i) continue;
-              const int e = d.s2c[j];
-              interact_b<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, d.xqc[e],
-                                        d.typec[e], d.gidc + e, a);
+      // list overflow since the last rebuild: scan the 27 cells directly
+      int c[3];
+      unpack_coord(g.coord_of_cell[d.cell_s[i]], c);
+      const Stencil st = atom_stencil(g, h, c);
+      const int os = own_slot(h, c);
+      const bool qi = split && (i - t.first[os]) < t.nq[os];
+      for (int kz = 0; kz < st.n[2]; ++kz)
+        for (int ky = 0; ky < st.n[1]; ++ky)
+          for (int kx = 0; kx < st.n[0]; ++kx) {
+            const int sl = (st.sl[2][kz] * h.cnt[1] + st.sl[1][ky]) * h.cnt[0] +
+                           st.sl[0][kx];
+            const int len = t.off[sl + 1] - t.off[sl];
+            for (int k = qi ? t.nq[sl] : 0; k < len; ++k) {
+              const int jg = t.first[sl] + k;
+              if (jg == i) continue;
+              interact_a<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, d.xs[jg],
+                                        [&] { return d.gid_s[jg]; }, a);
             }
           }
     }
-    d.fb[c] = make4<R>(a.fx, a.fy, a.fz, R(0));
+    d.fa[i] = make4<R>(a.fx, a.fy, a.fz, R(0));
+    acc.fx += a.fx;
+    acc.fy += a.fy;
+    acc.fz += a.fz;
+    if (EFLAG) {
+      acc.e_lj += a.e_lj;
+      for (int k = 0; k < 6; ++k) acc.v[k] += a.v[k];
+    }
   }
-  write_partials<R, EFLAG>(a, d.pair_part + 11 * part_offset, smem);
+  // phase 2: charged atoms of the brick over list B
+  if (split) {
+    __syncthreads();
+    const int c0 = d.cstart[h.cell0], c1 = d.cstart[h.cell1];
+    const int grp = threadIdx.x / kChargedLanes, gl = threadIdx.x % kChargedLanes;
+    const int ngrp = blockDim.x / kChargedLanes;
+    for (int c = c0 + grp; c < c1; c += ngrp) {
+      Acc<R> a;
+      const R4T<R> xi = d.xqc[c];
+      const int ti = d.typec[c], gi = d.gidc[c];
+      const int cnt = d.cnt_b[c];
+      if (cnt <= d.cap_b) {
+        const int* lp = d.list_b + (size_t)c * d.cap_b;
+        for (int k = gl; k < cnt; k += kChargedLanes) {
+          const int e = lp[k];
+          interact_b<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, d.xqc[e], d.typec[e],
+                                    d.gidc + e, a);
+        }
+      } else {
+        const int i = d.c2s[c];
+        int cc[3];
+        unpack_coord(g.coord_of_cell[d.cell_s[i]], cc);
+        const Span sx = make_span(cc[0], g.ext_b[0], g.nc[0]);
+        const Span sy = make_span(cc[1], g.ext_b[1], g.nc[1]);
+        const Span sz = make_span(cc[2], g.ext_b[2], g.nc[2]);
+        const int nst = sx.count * sy.count * sz.count;
+        for (int s = gl; s < nst; s += kChargedLanes) {
+          const int kx = s % sx.count, ky = (s / sx.count) % sy.count,
+                    kz = s / (sx.count * sy.count);
+          const int cell =
+              g.cell_of_coord[(sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0] + sx.at(kx)];
+          const int js = d.key_start[2 * cell], je = d.key_start[2 * cell + 1];
+          for (int j = js; j < je; ++j) {
+            if (j == i) continue;
+            const int e = d.s2c[j];
+            interact_b<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, d.xqc[e],
+                                      d.typec[e], d.gidc + e, a);
+          }
+        }
+      }
+      // reduce over the group's lanes (fixed tree)
+      R fx = a.fx, fy = a.fy, fz = a.fz;
+      for (int o = kChargedLanes / 2; o > 0; o >>= 1) {
+        fx += __shfl_xor(fx, o, kChargedLanes);
+        fy += __shfl_xor(fy, o, kChargedLanes);
+        fz += __shfl_xor(fz, o, kChargedLanes);
+      }
+      if (gl == 0) {
+        const int i = d.c2s[c];
+        R4T<R> f = d.fa[i];
+        f.x += fx;
+        f.y += fy;
+        f.z += fz;
+        d.fa[i] = f;
+      }
+      acc.fx += a.fx;
+      acc.fy += a.fy;
+      acc.fz += a.fz;
+      if (EFLAG) {
+        acc.e_lj += a.e_lj;
+        acc.e_c += a.e_c;
+        for (int k = 0; k < 6; ++k) acc.v[k] += a.v[k];
+      }
+    }
+  }
+  write_partials<R, EFLAG>(acc, d.pair_part, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -752,7 +1017,22 @@ __global__ __launch_bounds__(1024) void k_finalize(Dev<R> d, Box<R> b,
     const double two_pi = 6.283185307179586476925;
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
       double re = 0.0, im = 0.0;
-      for (int c = 0; c < nchunks; ++c) {
+      int c = 0;
+      for (; c + 4 <= nchunks; c += 4) {
+        const double2 s0 = d.sk_part[(size_t)c * d.kcap + k];
+        const double2 s1 = d.sk_part[(size_t)(c + 1) * d.kcap + k];
+        const double2 s2 = d.sk_part[(size_t)(c + 2) * d.kcap + k];
+        const double2 s3 = d.sk_part[(size_t)(c + 3) * d.kcap + k];
+        re += s0.x;
+        im += s0.y;
+        re += s1.x;
+        im += s1.y;
+        re += s2.x;
+        im += s2.y;
+        re += s3.x;
+        im += s3.y;
+      }
+      for (; c < nchunks; ++c) {
         const double2 s = d.sk_part[(size_t)c * d.kcap + k];
         re += s.x;
         im += s.y;
@@ -842,10 +1122,11 @@ __global__ __launch_bounds__(64 * kRbeWaves) void k_rbe_force(Dev<R> d,
   }
 }
 
-// Total force in caller order; resets the skin accumulator for the next call.
+// Total force in caller order (one thread per caller atom); resets the skin
+// accumulator for the next call and mirrors the counters to the host.
 template <typename R>
 __global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix,
-                                                  int has_b, int has_k) {
+                                                  int has_k) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     d.ctrl->maxdisp_bits = 0u;
@@ -854,42 +1135,39 @@ __global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix,
       d.ctrl_host->max_count_a = d.ctrl->max_count_a;
       d.ctrl_host->max_count_b = d.ctrl->max_count_b;
       d.ctrl_host->step_count = d.ctrl->step_count;
+      d.ctrl_host->max_stage = d.ctrl->max_stage;
+      d.ctrl_host->unstaged_bricks = d.ctrl->unstaged_bricks;
     }
   }
   if (i >= d.n) return;
-  R4T<R> f = d.fa[i];
+  const int s = d.inv_perm[i];
+  R4T<R> f = d.fa[s];
   if (mean_fix) {
     f.x -= (R)d.mean[0];
     f.y -= (R)d.mean[1];
     f.z -= (R)d.mean[2];
   }
-  const int c = d.s2c[i];
-  if (c >= 0) {
-    if (has_b) {
-      const R4T<R> g = d.fb[c];
-      f.x += g.x;
-      f.y += g.y;
-      f.z += g.z;
-    }
-    if (has_k) {
+  if (has_k) {
+    const int c = d.c_of[i];
+    if (c >= 0) {
       const R4T<R> g = d.fk[c];
       f.x += g.x;
       f.y += g.y;
       f.z += g.z;
     }
   }
-  const int o = d.perm[i];
-  d.fx[o] = f.x;
-  d.fy[o] = f.y;
-  d.fz[o] = f.z;
+  d.fx[i] = f.x;
+  d.fy[i] = f.y;
+  d.fz[i] = f.z;
 }
 
-// Parity export: classified pairs (or raw candidates) as (gid_i, gid_j, cls).
+// Parity export (block per brick): classified pairs (or raw candidates) as
+// (gid_i, gid_j, cls).
 template <typename R, bool RBL>
-__global__ __launch_bounds__(256) void k_export(Dev<R> d, Box<R> b,
+__global__ __launch_bounds__(512) void k_export(Dev<R> d, Box<R> b, Grid g,
                                                 PairConst<R> pc,
                                                 int candidates, int cap) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ SlotTables t;
   auto emit = [&](int gi, int gj, int cls) {
     const int pos = atomicAdd(d.export_count, 1);
     if (pos < cap) {
@@ -904,13 +1182,17 @@ __global__ __launch_bounds__(256) void k_export(Dev<R> d, Box<R> b,
     if (d2 < pc.t_shell && rbl_selected(pc.ifreq, pc.epoch, gi, gj)) return 2;
     return 0;
   };
-  if (t < d.n) {
-    const int i = t;
+  const Neigh h = brick_neigh(g, blockIdx.x);
+  build_slots(g, h, d.key_start, t);
+  const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     const R4T<R> xi = d.xs[i];
     const int gi = d.gid_s[i];
     const int cnt = min(d.cnt_a[i], d.cap_a);
     for (int k = 0; k < cnt; ++k) {
-      const int j = d.list_a[(size_t)k * d.n + i];
+      const int l = d.list_a[(size_t)k * d.n + i];
+      const int s = slot_of_pos(t, h.nslots, l);
+      const int j = t.first[s] + (l - t.off[s]);
       const int gj = d.gid_s[j];
       if (candidates) {
         emit(gi, gj, 1);
@@ -923,13 +1205,13 @@ __global__ __launch_bounds__(256) void k_export(Dev<R> d, Box<R> b,
       if (cls) emit(gi, gj, cls);
     }
   }
-  if (t < d.nc) {
-    const int c = t;
+  const int c0 = d.cstart[h.cell0], c1 = d.cstart[h.cell1];
+  for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
     const R4T<R> xi = d.xqc[c];
     const int gi = d.gidc[c];
     const int cnt = min(d.cnt_b[c], d.cap_b);
     for (int k = 0; k < cnt; ++k) {
-      const int e = d.list_b[(size_t)k * d.nc + c];
+      const int e = d.list_b[(size_t)c * d.cap_b + k];
       const int gj = d.gidc[e];
       if (candidates) {
         emit(gi, gj, 2);
@@ -954,7 +1236,6 @@ template <typename R>
 hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   const Grid& g = a.grid;
   const unsigned thr = a.rebuild_thr;
-  const int nb = blocks_for(d.n);
   const int gb = blocks_for(d.n, kBlock, 2048);
   const int cap_a = a.count_only ? 0 : d.cap_a;
   const int cap_b = a.count_only ? 0 : d.cap_b;
@@ -963,16 +1244,17 @@ hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, d.key_count,
                      d.key_start, d.cstart, g.ncell, d.ctrl, thr);
   hipLaunchKernelGGL((k_scatter<R>), dim3(gb), dim3(kBlock), 0, s, d, thr);
-  hipLaunchKernelGGL(k_segsort, dim3(blocks_for(2 * g.ncell, kBlock, 2048)),
-                     dim3(kBlock), 0, s, d.perm, d.key_start, 2 * g.ncell,
-                     d.ctrl, thr);
+  hipLaunchKernelGGL(k_cellsort, dim3(blocks_for(g.ncell * 64, kBlock)),
+                     dim3(kBlock), 0, s, d.perm, d.inv_perm, d.key_start,
+                     g.ncell, d.ctrl, thr);
   hipLaunchKernelGGL((k_gather<R>), dim3(gb), dim3(kBlock), 0, s, d, a.box,
                      thr);
-  hipLaunchKernelGGL((k_list_a<R>), dim3(nb), dim3(kBlock), 0, s, d, a.box, g,
-                     a.pc.r2_list_a, a.p.split, cap_a, thr);
+  hipLaunchKernelGGL((k_list_a<R>), dim3(g.nbrick), dim3(g.block), 0, s, d,
+                     a.box, g, a.pc.r2_list_a, a.p.split, cap_a, thr);
   if (a.p.split && d.nc > 0)
-    hipLaunchKernelGGL((k_list_b<R>), dim3(blocks_for(d.nc)), dim3(kBlock), 0,
-                       s, d, a.box, g, a.pc.r2_list_b, cap_b, thr);
+    hipLaunchKernelGGL((k_list_b<R>), dim3(blocks_for(d.nc * 64, kBlock)),
+                       dim3(kBlock), 0, s, d, a.box, g, a.pc.r2_list_b, cap_b,
+                       thr);
   return hipGetLastError();
 }
 
@@ -988,24 +1270,18 @@ hipError_t launch_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
 template <typename R>
 hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   const Params& p = a.p;
-  const int gb = blocks_for(d.n, kBlock, 2048);
+  const int sb = blocks_for(d.n, kBlock, 256);
   if (a.ev[0]) hipEventRecord(a.ev[0], s);
-  hipLaunchKernelGGL((k_skin<R>), dim3(gb), dim3(kBlock), 0, s, d, a.box,
+  hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(kBlock), 0, s, d, a.box,
                      a.force_rebuild, a.m_host, a.m_host ? 3 * p.num_k : 0);
   hipError_t e = build_part<R>(d, a, s);
   if (e != hipSuccess) return e;
   if (a.ev[1]) hipEventRecord(a.ev[1], s);
-  const int na = blocks_for(d.n);
-  const int nbb = (p.split && d.nc > 0) ? blocks_for(d.nc) : 0;
-#define RBMDX_PAIR(RBLv, EFv)                                                  \
-  do {                                                                         \
-    hipLaunchKernelGGL((k_pair_a<R, RBLv, EFv>), dim3(na), dim3(kBlock), 0, s, \
-                       d, a.box, a.grid, a.pc, a.lj_tab, p.split);             \
-    if (a.ev[2]) hipEventRecord(a.ev[2], s);                                   \
-    if (nbb)                                                                   \
-      hipLaunchKernelGGL((k_pair_b<R, RBLv, EFv>), dim3(nbb), dim3(kBlock), 0, \
-                         s, d, a.box, a.grid, a.pc, a.lj_tab, na);             \
-  } while (0)
+  const Grid& g = a.grid;
+  const size_t lj_bytes = 2 * sizeof(R) * (size_t)a.pc.ntypes * a.pc.ntypes;
+#define RBMDX_PAIR(RBLv, EFv)                                         \
+  hipLaunchKernelGGL((k_pair<R, RBLv, EFv>), dim3(g.nbrick), dim3(g.block), \
+                     lj_bytes, s, d, a.box, g, a.pc, a.lj_tab, p.split)
   if (p.rbl) {
     if (p.eflag)
       RBMDX_PAIR(true, true);
@@ -1018,6 +1294,7 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
       RBMDX_PAIR(false, false);
   }
 #undef RBMDX_PAIR
+  if (a.ev[2]) hipEventRecord(a.ev[2], s);
   if (a.ev[3]) hipEventRecord(a.ev[3], s);
   const int K = p.kspace ? p.num_k : 0;
   const int nchunks = (d.nc + kSkChunk - 1) / kSkChunk;
@@ -1027,13 +1304,13 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   Params pf = p;
   if (d.nc == 0) pf.kspace = 0;
   hipLaunchKernelGGL((k_finalize<R>), dim3(1), dim3(1024), 0, s, d, a.box,
-                     a.m_use, K, nchunks, na + nbb, pf, a.e_self);
+                     a.m_use, K, nchunks, g.nbrick, pf, a.e_self);
   if (K > 0 && d.nc > 0)
     hipLaunchKernelGGL((k_rbe_force<R>), dim3(blocks_for(d.nc, 64)),
                        dim3(64 * kRbeWaves), 0, s, d, a.box, K);
   if (a.ev[4]) hipEventRecord(a.ev[4], s);
-  hipLaunchKernelGGL((k_assemble<R>), dim3(na), dim3(kBlock), 0, s, d,
-                     p.mean_fix, nbb > 0 ? 1 : 0, (K > 0 && d.nc > 0) ? 1 : 0);
+  hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,
+                     s, d, p.mean_fix, (K > 0 && d.nc > 0) ? 1 : 0);
   if (a.ev[5]) hipEventRecord(a.ev[5], s);
   return hipGetLastError();
 }
@@ -1041,13 +1318,13 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
 template <typename R>
 hipError_t launch_export(const Dev<R>& d, const StepArgs<R>& a, int candidates,
                          int cap, hipStream_t s) {
-  const int nmax = d.n > d.nc ? d.n : d.nc;
+  const Grid& g = a.grid;
   if (a.p.rbl)
-    hipLaunchKernelGGL((k_export<R, true>), dim3(blocks_for(nmax)),
-                       dim3(kBlock), 0, s, d, a.box, a.pc, candidates, cap);
+    hipLaunchKernelGGL((k_export<R, true>), dim3(g.nbrick), dim3(g.block), 0, s,
+                       d, a.box, g, a.pc, candidates, cap);
   else
-    hipLaunchKernelGGL((k_export<R, false>), dim3(blocks_for(nmax)),
-                       dim3(kBlock), 0, s, d, a.box, a.pc, candidates, cap);
+    hipLaunchKernelGGL((k_export<R, false>), dim3(g.nbrick), dim3(g.block), 0,
+                       s, d, a.box, g, a.pc, candidates, cap);
   return hipGetLastError();
 }
 
