@@ -625,6 +625,16 @@ class Engine : public EngineBase {
         nb *= grid_.nb[d];
       }
       grid_.nbrick = nb;
+      // the charged-pair neighbourhood of a brick must fit its slot table
+      if (split_) {
+        int slots = 1;
+        for (int d = 0; d < 3; ++d)
+          slots *= std::min(grid_.nc[d], grid_.bw + 2 * grid_.ext_b[d]);
+        if (slots > kMaxSlotsB)
+          throw Error(RBMDX_E_CONFIG,
+                      "Coulomb cut_off too large relative to the LJ list radius "
+                      "(charged-pair neighbourhood exceeds the brick slot table)");
+      }
       const int per = (int)std::ceil(1.25 * apc * grid_.bw * grid_.bw * grid_.bw);
       grid_.block = std::min(512, std::max(64, (per + 63) / 64 * 64));
       // brick-major cell numbering
@@ -891,7 +901,8 @@ class Engine : public EngineBase {
   }
 
   void alloc_lists(int max_a, int max_b) {
-    const int cap_a = (int)std::ceil(max_a * 1.2) + 16;
+    // list A is read 8 entries at a time: capacity is a multiple of 8
+    const int cap_a = ((int)std::ceil(max_a * 1.2) + 16 + 7) / 8 * 8;
     const int cap_b = (int)std::ceil(max_b * 1.2) + 16;
     if (d_.list_a) release(d_.list_a);
     if (d_.list_b) release(d_.list_b);

@@ -29,11 +29,18 @@ namespace {
 template <typename R>
 using R4T = typename Dev<R>::R4;
 
-// LDS budget of one brick stage (32 KB of positions).
+// LDS budget of one brick stage (24 KB of positions).
 template <typename R>
 struct StageCap {
-  static constexpr int value = (32 * 1024) / (int)sizeof(R4T<R>);
+  static constexpr int value = (24 * 1024) / (int)sizeof(R4T<R>);
 };
+
+// List A layout: entries interleaved in groups of 8 so that one 16-byte load
+// fetches 8 consecutive entries of an atom: entry k of sorted atom i lives
+// at ((k / 8) * n + i) * 8 + k % 8.
+__device__ __forceinline__ size_t list_a_index(int k, int i, int n) {
+  return ((size_t)(k >> 3) * n + i) * 8 + (k & 7);
+}
 
 __device__ __forceinline__ float mul_rn(float a, float b) {
   return __fmul_rn(a, b);
@@ -213,32 +220,34 @@ __device__ __forceinline__ int block_excl_scan(int v, int& total, int* smem) {
 
 // ---------------------------------------------------------------------------
 // Brick neighbourhood.  A brick is a bw^3 block of cells; its neighbourhood
-// adds one cell on every side (the whole dimension when that would wrap onto
-// itself).  Slots are enumerated x-fastest; every slot holds one cell whose
-// atoms are contiguous in sorted order (charged first).
+// with extension e adds e cells on every side (the whole dimension when that
+// would wrap onto itself).  Slots are enumerated x-fastest; every slot holds
+// one cell whose atoms are contiguous in sorted order (charged first).
 // ---------------------------------------------------------------------------
 struct Neigh {
-  int start[3], cnt[3], full[3];
+  int start[3], cnt[3], full[3], e[3];
   int nslots;
-  int o[3];          // brick origin (cell coords)
   int cell0, cell1;  // cell id range of the brick
 };
 
-__device__ __forceinline__ Neigh brick_neigh(const Grid& g, int b) {
+__device__ __forceinline__ Neigh brick_neigh(const Grid& g, int b,
+                                             const int e[3]) {
   Neigh h;
   const int bx = b % g.nb[0], by = (b / g.nb[0]) % g.nb[1],
             bz = b / (g.nb[0] * g.nb[1]);
   const int bc[3] = {bx, by, bz};
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
-    h.o[d] = bc[d] * g.bw;
-    const int w = min(g.bw, g.nc[d] - h.o[d]);
-    if (w + 2 >= g.nc[d]) {
+    const int o = bc[d] * g.bw;
+    const int w = min(g.bw, g.nc[d] - o);
+    h.e[d] = e[d];
+    if (w + 2 * e[d] >= g.nc[d]) {
       h.start[d] = 0;
       h.cnt[d] = g.nc[d];
       h.full[d] = 1;
     } else {
-      h.start[d] = h.o[d] - 1;
-      h.cnt[d] = w + 2;
+      h.start[d] = o - e[d];
+      h.cnt[d] = w + 2 * e[d];
       h.full[d] = 0;
     }
   }
@@ -248,41 +257,62 @@ __device__ __forceinline__ Neigh brick_neigh(const Grid& g, int b) {
   return h;
 }
 
-// Slot tables in LDS: off (prefix of atom counts), first (sorted index of the
-// slot cell's first atom), nq (charged atoms of the slot cell).
+__device__ __forceinline__ int slot_cell(const Grid& g, const Neigh& h, int s) {
+  const int kx = s % h.cnt[0], ky = (s / h.cnt[0]) % h.cnt[1],
+            kz = s / (h.cnt[0] * h.cnt[1]);
+  const int cx = wrapi(h.start[0] + kx, g.nc[0]);
+  const int cy = wrapi(h.start[1] + ky, g.nc[1]);
+  const int cz = wrapi(h.start[2] + kz, g.nc[2]);
+  return g.cell_of_coord[(cz * g.nc[1] + cy) * g.nc[0] + cx];
+}
+
+// Slot tables in LDS: off (prefix of staged atom counts), first (sorted index
+// of the slot cell's first atom), nq (charged atoms of the slot cell), cfirst
+// (charged compact index of the slot cell's first charged atom).
+template <int NS>
 struct SlotTables {
-  int off[kMaxSlots + 1];
-  int first[kMaxSlots];
-  int nq[kMaxSlots];
+  int off[NS + 1];
+  int first[NS];
+  int nq[NS];
+  int cfirst[NS];
   int scan[33];
 };
 
+// charged_only: stage only the charged atoms of every slot cell.
+template <int NS>
 __device__ __forceinline__ void build_slots(const Grid& g, const Neigh& h,
                                             const int* key_start,
-                                            SlotTables& t) {
-  int cnt = 0;
-  if ((int)threadIdx.x < h.nslots) {
-    const int s = threadIdx.x;
-    const int kx = s % h.cnt[0], ky = (s / h.cnt[0]) % h.cnt[1],
-              kz = s / (h.cnt[0] * h.cnt[1]);
-    const int cx = wrapi(h.start[0] + kx, g.nc[0]);
-    const int cy = wrapi(h.start[1] + ky, g.nc[1]);
-    const int cz = wrapi(h.start[2] + kz, g.nc[2]);
-    const int cell = g.cell_of_coord[(cz * g.nc[1] + cy) * g.nc[0] + cx];
+                                            const int* cstart, bool charged_only,
+                                            SlotTables<NS>& t) {
+  const int per = (h.nslots + blockDim.x - 1) / blockDim.x;
+  const int s0 = threadIdx.x * per;
+  int sum = 0;
+  for (int s = s0; s < min(h.nslots, s0 + per); ++s) {
+    const int cell = slot_cell(g, h, s);
     const int a = key_start[2 * cell];
+    const int nq = key_start[2 * cell + 1] - a;
     t.first[s] = a;
-    t.nq[s] = key_start[2 * cell + 1] - a;
-    cnt = key_start[2 * cell + 2] - a;
+    t.nq[s] = nq;
+    t.cfirst[s] = cstart ? cstart[cell] : 0;
+    const int cnt = charged_only ? nq : key_start[2 * cell + 2] - a;
+    t.off[s] = cnt;  // count, turned into an offset below
+    sum += cnt;
   }
+  __syncthreads();
   int total;
-  const int ex = block_excl_scan(cnt, total, t.scan);
-  if ((int)threadIdx.x < h.nslots) t.off[threadIdx.x] = ex;
+  int run = block_excl_scan(sum, total, t.scan);
+  for (int s = s0; s < min(h.nslots, s0 + per); ++s) {
+    const int c = t.off[s];
+    t.off[s] = run;
+    run += c;
+  }
   if (threadIdx.x == 0) t.off[h.nslots] = total;
   __syncthreads();
 }
 
 // Slot holding stage position l (binary search over the prefix table).
-__device__ __forceinline__ int slot_of_pos(const SlotTables& t, int nslots,
+template <int NS>
+__device__ __forceinline__ int slot_of_pos(const SlotTables<NS>& t, int nslots,
                                            int l) {
   int lo = 0, hi = nslots - 1;
   while (lo < hi) {
@@ -295,9 +325,9 @@ __device__ __forceinline__ int slot_of_pos(const SlotTables& t, int nslots,
   return lo;
 }
 
-template <typename R>
+template <typename R, int NS>
 __device__ __forceinline__ void stage_atoms(const Neigh& h,
-                                            const SlotTables& t,
+                                            const SlotTables<NS>& t,
                                             const R4T<R>* xs, R4T<R>* stage) {
   const int total = t.off[h.nslots];
   for (int l = threadIdx.x; l < total; l += blockDim.x) {
@@ -307,24 +337,38 @@ __device__ __forceinline__ void stage_atoms(const Neigh& h,
   __syncthreads();
 }
 
-// The 27-cell stencil of an atom in cell coords c, as slot indices.
+// The (2e+1)^3 stencil of an atom in neighbourhood slot coordinates: per
+// dimension n[d] distinct cells, the k-th at slot coordinate at(d, k).
 struct Stencil {
-  int n[3];
-  int sl[3][3];
+  int n[3], base[3], mode[3], nc[3], e[3];
+  __device__ __forceinline__ int at(int d, int k) const {
+    // mode 0: window inside the neighbourhood; 1: wrapped window over a full
+    // dimension; 2: the whole dimension once
+    return mode[d] == 0 ? base[d] - e[d] + k
+                        : (mode[d] == 1 ? wrapi(base[d] - e[d] + k, nc[d]) : k);
+  }
+  __device__ __forceinline__ int count() const { return n[0] * n[1] * n[2]; }
 };
 __device__ __forceinline__ Stencil atom_stencil(const Grid& g, const Neigh& h,
                                                 const int c[3]) {
   Stencil st;
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
-    if (h.full[d] && g.nc[d] < 3) {
+    st.nc[d] = g.nc[d];
+    st.e[d] = h.e[d];
+    const int w = 2 * h.e[d] + 1;
+    if (h.full[d] && w > g.nc[d]) {
       st.n[d] = g.nc[d];
-      for (int k = 0; k < 3; ++k) st.sl[d][k] = k < g.nc[d] ? k : 0;
+      st.mode[d] = 2;
+      st.base[d] = 0;
     } else if (h.full[d]) {
-      st.n[d] = 3;
-      for (int k = 0; k < 3; ++k) st.sl[d][k] = wrapi(c[d] - 1 + k, g.nc[d]);
+      st.n[d] = w;
+      st.mode[d] = 1;
+      st.base[d] = c[d];
     } else {
-      st.n[d] = 3;
-      for (int k = 0; k < 3; ++k) st.sl[d][k] = c[d] - 1 + k - h.start[d];
+      st.n[d] = w;
+      st.mode[d] = 0;
+      st.base[d] = c[d] - h.start[d];
     }
   }
   return st;
@@ -332,6 +376,7 @@ __device__ __forceinline__ Stencil atom_stencil(const Grid& g, const Neigh& h,
 
 __device__ __forceinline__ int own_slot(const Neigh& h, const int c[3]) {
   int k[3];
+#pragma unroll
   for (int d = 0; d < 3; ++d) k[d] = h.full[d] ? c[d] : c[d] - h.start[d];
   return (k[2] * h.cnt[1] + k[1]) * h.cnt[0] + k[0];
 }
@@ -348,9 +393,9 @@ __device__ __forceinline__ void unpack_coord(int packed, int c[3]) {
 // Block 0 also copies the k-sample from host-mapped staging memory.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force,
-                                              const int* m_host, int nk3) {
-  __shared__ float s_max[4];
+__global__ __launch_bounds__(1024) void k_skin(Dev<R> d, Box<R> b, int force,
+                                               const int* m_host, int nk3) {
+  __shared__ float s_max[16];
   if (m_host && blockIdx.x == 0)
     for (int t = threadIdx.x; t < nk3; t += blockDim.x) d.m_dev[t] = m_host[t];
   if (force) {
@@ -546,9 +591,10 @@ __global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
                                                 unsigned thr) {
   if (!rebuild_now(d.ctrl, thr)) return;
   __shared__ R4T<R> stage[StageCap<R>::value];
-  __shared__ SlotTables t;
-  const Neigh h = brick_neigh(g, blockIdx.x);
-  build_slots(g, h, d.key_start, t);
+  __shared__ SlotTables<kMaxSlots> t;
+  const int e1[3] = {1, 1, 1};
+  const Neigh h = brick_neigh(g, blockIdx.x, e1);
+  build_slots(g, h, d.key_start, nullptr, false, t);
   const int total = t.off[h.nslots];
   const bool staged = total <= StageCap<R>::value;
   if (staged) stage_atoms<R>(h, t, d.xs, stage);
@@ -569,7 +615,7 @@ __global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
       for (int ky = 0; ky < st.n[1]; ++ky)
         for (int kx = 0; kx < st.n[0]; ++kx) {
           const int sl =
-              (st.sl[2][kz] * h.cnt[1] + st.sl[1][ky]) * h.cnt[0] + st.sl[0][kx];
+              (st.at(2, kz) * h.cnt[1] + st.at(1, ky)) * h.cnt[0] + st.at(0, kx);
           const int base = t.off[sl];
           const int len = t.off[sl + 1] - base;
           const int js = qi ? t.nq[sl] : 0;
@@ -581,7 +627,7 @@ __global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
             const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx,
                                     dy, dz);
             if (d2 < r2) {
-              if (count < lcap) d.list_a[(size_t)count * d.n + i] = (uint16_t)l;
+              if (count < lcap) d.list_a[list_a_index(count, i, d.n)] = (uint16_t)l;
               ++count;
             }
           }
@@ -602,57 +648,81 @@ __global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
   }
 }
 
-// Charged-pair candidate list: one wave per charged atom; lanes split the
-// ext_b stencil cells, count, prefix-sum and write in a fixed order.
+// Charged-pair candidate list of one brick: the charged atoms of the brick's
+// ext_b neighbourhood are staged in LDS; kChargedLanes lanes per charged atom
+// split its stencil cells, count, prefix-sum and write in a fixed order.
 // Row-major list_b[c * cap + k] of charged compact indices.
 template <typename R>
-__global__ __launch_bounds__(256) void k_list_b(Dev<R> d, Box<R> b, Grid g,
+__global__ __launch_bounds__(512) void k_list_b(Dev<R> d, Box<R> b, Grid g,
                                                 R r2, int cap, unsigned thr) {
   if (!rebuild_now(d.ctrl, thr)) return;
-  const int lane = threadIdx.x & 63;
-  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (c >= d.nc) return;
-  const int i = d.c2s[c];
-  const R4T<R> xi = d.xs[i];
-  int cc[3];
-  unpack_coord(g.coord_of_cell[d.cell_s[i]], cc);
-  const Span sx = make_span(cc[0], g.ext_b[0], g.nc[0]);
-  const Span sy = make_span(cc[1], g.ext_b[1], g.nc[1]);
-  const Span sz = make_span(cc[2], g.ext_b[2], g.nc[2]);
-  const int nst = sx.count * sy.count * sz.count;
-  int count = 0;
-  int pos = 0;
-  for (int pass = 0; pass < 2; ++pass) {
-    if (pass == 1) pos = wave_incl_scan(count) - count;
-    for (int s = lane; s < nst; s += 64) {
-      const int kx = s % sx.count, ky = (s / sx.count) % sy.count,
-                kz = s / (sx.count * sy.count);
-      const int cell =
-          g.cell_of_coord[(sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0] + sx.at(kx)];
-      const int js = d.key_start[2 * cell], je = d.key_start[2 * cell + 1];
-      for (int j = js; j < je; ++j) {
-        if (j == i) continue;
-        const R4T<R> xj = d.xs[j];
-        R dx, dy, dz;
-        const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
-        if (d2 < r2) {
-          if (pass == 0) {
-            ++count;
-          } else {
-            if (pos < cap) d.list_b[(size_t)c * cap + pos] = d.s2c[j];
-            ++pos;
+  __shared__ R4T<R> stage[kStageCapB];
+  __shared__ SlotTables<kMaxSlotsB> t;
+  const Neigh h = brick_neigh(g, blockIdx.x, g.ext_b);
+  const int c0 = d.cstart[h.cell0], c1 = d.cstart[h.cell1];
+  if (c0 == c1) return;
+  build_slots(g, h, d.key_start, d.cstart, true, t);
+  const int total = t.off[h.nslots];
+  const bool staged = total <= kStageCapB;
+  if (staged) stage_atoms<R>(h, t, d.xs, stage);
+  const int grp = threadIdx.x / kChargedLanes, gl = threadIdx.x % kChargedLanes;
+  const int ngrp = blockDim.x / kChargedLanes;
+  int mx = 0;
+  unsigned long long sum = 0;
+  for (int c = c0 + grp; c < c1; c += ngrp) {
+    const int i = d.c2s[c];
+    const R4T<R> xi = d.xs[i];
+    int cc[3];
+    unpack_coord(g.coord_of_cell[d.cell_s[i]], cc);
+    const Stencil st = atom_stencil(g, h, cc);
+    const int os = own_slot(h, cc);
+    const int self = t.off[os] + (i - t.first[os]);
+    const int nst = st.count();
+    int count = 0, pos = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      if (pass == 1) {
+        int inc = count;
+        for (int o = 1; o < kChargedLanes; o <<= 1) {
+          const int u = __shfl_up(inc, o, kChargedLanes);
+          if (gl >= o) inc += u;
+        }
+        pos = inc - count;
+      }
+      for (int s = gl; s < nst; s += kChargedLanes) {
+        const int kx = s % st.n[0], ky = (s / st.n[0]) % st.n[1],
+                  kz = s / (st.n[0] * st.n[1]);
+        const int sl = (st.at(2, kz) * h.cnt[1] + st.at(1, ky)) * h.cnt[0] +
+                       st.at(0, kx);
+        const int base = t.off[sl], len = t.off[sl + 1] - base;
+        for (int k = 0; k < len; ++k) {
+          const int l = base + k;
+          if (l == self) continue;
+          const R4T<R> xj = staged ? stage[l] : d.xs[t.first[sl] + k];
+          R dx, dy, dz;
+          const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+          if (d2 < r2) {
+            if (pass == 0) {
+              ++count;
+            } else {
+              if (pos < cap) d.list_b[(size_t)c * cap + pos] = t.cfirst[sl] + k;
+              ++pos;
+            }
           }
         }
       }
     }
+    int tot = count;
+    for (int o = kChargedLanes / 2; o > 0; o >>= 1)
+      tot += __shfl_xor(tot, o, kChargedLanes);
+    if (gl == 0) d.cnt_b[c] = tot;
+    mx = max(mx, tot);
+    sum += (gl == 0) ? (unsigned long long)tot : 0ull;
   }
-  const int tot = wave_sum(count);
-  if (lane == 0) {
-    d.cnt_b[c] = tot;
-    if (tot > 0) {
-      atomicMax(&d.ctrl->max_count_b, tot);
-      atomicAdd(&d.ctrl->sum_count_b, (unsigned long long)tot);
-    }
+  mx = wave_max_i(mx);
+  sum = wave_sum(sum);
+  if ((threadIdx.x & 63) == 0 && mx > 0) {
+    atomicMax(&d.ctrl->max_count_b, mx);
+    atomicAdd(&d.ctrl->sum_count_b, sum);
   }
 }
 
@@ -799,13 +869,14 @@ __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
                                               PairConst<R> pc, const R* lj_tab,
                                               int split) {
   __shared__ R4T<R> stage[StageCap<R>::value];
-  __shared__ SlotTables t;
+  __shared__ SlotTables<kMaxSlots> t;
   __shared__ double smem[8 * 11];
   extern __shared__ char dyn_lds[];  // LJ table: 2 * ntypes^2 values of R
   R* s_lj = reinterpret_cast<R*>(dyn_lds);
   load_lj<R>(lj_tab, pc.ntypes, s_lj);
-  const Neigh h = brick_neigh(g, blockIdx.x);
-  build_slots(g, h, d.key_start, t);
+  const int e1[3] = {1, 1, 1};
+  const Neigh h = brick_neigh(g, blockIdx.x, e1);
+  build_slots(g, h, d.key_start, nullptr, false, t);
   const int total = t.off[h.nslots];
   const bool staged = total <= StageCap<R>::value;
   if (staged) stage_atoms<R>(h, t, d.xs, stage);
@@ -819,16 +890,24 @@ __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
     const int gi = d.gid_s[i];
     const int cnt = d.cnt_a[i];
     if (cnt <= d.cap_a) {
-      const uint16_t* lp = d.list_a + i;
-      for (int k = 0; k < cnt; ++k) {
-        const int l = lp[(size_t)k * d.n];
-        auto global_j = [&]() {
-          const int s = slot_of_pos(t, h.nslots, l);
-          return t.first[s] + (l - t.off[s]);
-        };
-        const R4T<R> xj = staged ? stage[l] : d.xs[global_j()];
-        interact_a<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, xj,
-                                  [&] { return d.gid_s[global_j()]; }, a);
+      const uint4* lp = reinterpret_cast<const uint4*>(d.list_a) + i;
+      for (int k0 = 0; k0 < cnt; k0 += 8) {
+        const uint4 q8 = lp[(size_t)(k0 >> 3) * d.n];
+        const unsigned w[4] = {q8.x, q8.y, q8.z, q8.w};
+        const int kn = min(8, cnt - k0);
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+          if (kk < kn) {
+            const int l = (int)((w[kk >> 1] >> (16 * (kk & 1))) & 0xffffu);
+            auto global_j = [&]() {
+              const int s = slot_of_pos(t, h.nslots, l);
+              return t.first[s] + (l - t.off[s]);
+            };
+            const R4T<R> xj = staged ? stage[l] : d.xs[global_j()];
+            interact_a<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, xj,
+                                      [&] { return d.gid_s[global_j()]; }, a);
+          }
+        }
       }
     } else {
       // list overflow since the last rebuild: scan the 27 cells directly
@@ -840,8 +919,8 @@ __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
       for (int kz = 0; kz < st.n[2]; ++kz)
         for (int ky = 0; ky < st.n[1]; ++ky)
           for (int kx = 0; kx < st.n[0]; ++kx) {
-            const int sl = (st.sl[2][kz] * h.cnt[1] + st.sl[1][ky]) * h.cnt[0] +
-                           st.sl[0][kx];
+            const int sl = (st.at(2, kz) * h.cnt[1] + st.at(1, ky)) * h.cnt[0] +
+                           st.at(0, kx);
             const int len = t.off[sl + 1] - t.off[sl];
             for (int k = qi ? t.nq[sl] : 0; k < len; ++k) {
               const int jg = t.first[sl] + k;
@@ -934,9 +1013,12 @@ __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
 // from the reference's e^{ik.r} by the constant phase e^{ik.lo}, which cancels
 // in every force, energy and virial term.
 // ---------------------------------------------------------------------------
+// sin and cos of 2 pi t for t already reduced to [-1/2, 1/2]: the hardware
+// transcendental unit takes its argument in revolutions.
 __device__ __forceinline__ void sincos_rev(float t, float& s, float& c) {
   t -= rintf(t);
-  sincospif(2.0f * t, &s, &c);
+  s = __builtin_amdgcn_sinf(t);
+  c = __builtin_amdgcn_cosf(t);
 }
 
 template <typename R>
@@ -1015,28 +1097,23 @@ __global__ __launch_bounds__(1024) void k_finalize(Dev<R> d, Box<R> b,
   if (p.kspace) {
     const double V = p.volume;
     const double two_pi = 6.283185307179586476925;
-    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    // four threads per k sum interleaved chunks, then combine (fixed order)
+    const int sub = threadIdx.x & 3;
+    for (int kb = threadIdx.x >> 2; kb < ((K + 255) / 256) * 256; kb += blockDim.x >> 2) {
+      const int k = kb;
       double re = 0.0, im = 0.0;
-      int c = 0;
-      for (; c + 4 <= nchunks; c += 4) {
-        const double2 s0 = d.sk_part[(size_t)c * d.kcap + k];
-        const double2 s1 = d.sk_part[(size_t)(c + 1) * d.kcap + k];
-        const double2 s2 = d.sk_part[(size_t)(c + 2) * d.kcap + k];
-        const double2 s3 = d.sk_part[(size_t)(c + 3) * d.kcap + k];
-        re += s0.x;
-        im += s0.y;
-        re += s1.x;
-        im += s1.y;
-        re += s2.x;
-        im += s2.y;
-        re += s3.x;
-        im += s3.y;
+      if (k < K) {
+        for (int c = sub; c < nchunks; c += 4) {
+          const double2 s = d.sk_part[(size_t)c * d.kcap + k];
+          re += s.x;
+          im += s.y;
+        }
       }
-      for (; c < nchunks; ++c) {
-        const double2 s = d.sk_part[(size_t)c * d.kcap + k];
-        re += s.x;
-        im += s.y;
-      }
+      re += __shfl_xor(re, 1, 64);
+      im += __shfl_xor(im, 1, 64);
+      re += __shfl_xor(re, 2, 64);
+      im += __shfl_xor(im, 2, 64);
+      if (k >= K || sub != 0) continue;
       d.rho[k] = make_double2(re, im);
       const double kx = two_pi * m[3 * k] / (double)b.L[0];
       const double ky = two_pi * m[3 * k + 1] / (double)b.L[1];
@@ -1167,7 +1244,7 @@ template <typename R, bool RBL>
 __global__ __launch_bounds__(512) void k_export(Dev<R> d, Box<R> b, Grid g,
                                                 PairConst<R> pc,
                                                 int candidates, int cap) {
-  __shared__ SlotTables t;
+  __shared__ SlotTables<kMaxSlots> t;
   auto emit = [&](int gi, int gj, int cls) {
     const int pos = atomicAdd(d.export_count, 1);
     if (pos < cap) {
@@ -1182,15 +1259,16 @@ __global__ __launch_bounds__(512) void k_export(Dev<R> d, Box<R> b, Grid g,
     if (d2 < pc.t_shell && rbl_selected(pc.ifreq, pc.epoch, gi, gj)) return 2;
     return 0;
   };
-  const Neigh h = brick_neigh(g, blockIdx.x);
-  build_slots(g, h, d.key_start, t);
+  const int e1[3] = {1, 1, 1};
+  const Neigh h = brick_neigh(g, blockIdx.x, e1);
+  build_slots(g, h, d.key_start, nullptr, false, t);
   const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
   for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     const R4T<R> xi = d.xs[i];
     const int gi = d.gid_s[i];
     const int cnt = min(d.cnt_a[i], d.cap_a);
     for (int k = 0; k < cnt; ++k) {
-      const int l = d.list_a[(size_t)k * d.n + i];
+      const int l = d.list_a[list_a_index(k, i, d.n)];
       const int s = slot_of_pos(t, h.nslots, l);
       const int j = t.first[s] + (l - t.off[s]);
       const int gj = d.gid_s[j];
@@ -1252,9 +1330,8 @@ hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   hipLaunchKernelGGL((k_list_a<R>), dim3(g.nbrick), dim3(g.block), 0, s, d,
                      a.box, g, a.pc.r2_list_a, a.p.split, cap_a, thr);
   if (a.p.split && d.nc > 0)
-    hipLaunchKernelGGL((k_list_b<R>), dim3(blocks_for(d.nc * 64, kBlock)),
-                       dim3(kBlock), 0, s, d, a.box, g, a.pc.r2_list_b, cap_b,
-                       thr);
+    hipLaunchKernelGGL((k_list_b<R>), dim3(g.nbrick), dim3(g.block), 0, s, d,
+                       a.box, g, a.pc.r2_list_b, cap_b, thr);
   return hipGetLastError();
 }
 
@@ -1270,9 +1347,9 @@ hipError_t launch_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
 template <typename R>
 hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   const Params& p = a.p;
-  const int sb = blocks_for(d.n, kBlock, 256);
+  const int sb = blocks_for(d.n, 1024, 64);
   if (a.ev[0]) hipEventRecord(a.ev[0], s);
-  hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(kBlock), 0, s, d, a.box,
+  hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(1024), 0, s, d, a.box,
                      a.force_rebuild, a.m_host, a.m_host ? 3 * p.num_k : 0);
   hipError_t e = build_part<R>(d, a, s);
   if (e != hipSuccess) return e;
