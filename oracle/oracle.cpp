@@ -238,10 +238,19 @@ void pair_force_f32pos(int n, const float* x, const float* y, const float* z,
     if (!(d2f < t.rc2)) return;
     const int cls = p.rbl ? rbl_class(d2f, t, p, gid[i], gid[j]) : 1;
     const double w = pair_weight(d2f, cls, t, p);
-    // force: RBL-selected/weighted pairs
-    const T dx = (T)min_image_1<float>(x[j] - x[i], L[0]);
-    const T dy = (T)min_image_1<float>(y[j] - y[i], L[1]);
-    const T dz = (T)min_image_1<float>(z[j] - z[i], L[2]);
+    // force: RBL-selected/weighted pairs.  The FP32 mirror reuses the float
+    // displacement; the FP64 reference recomputes it in double from the same
+    // (float) positions, so no FP32 rounding leaks into the error reference.
+    T dx, dy, dz;
+    if (std::is_same<T, float>::value) {
+      dx = (T)min_image_1<float>(x[j] - x[i], L[0]);
+      dy = (T)min_image_1<float>(y[j] - y[i], L[1]);
+      dz = (T)min_image_1<float>(z[j] - z[i], L[2]);
+    } else {
+      dx = (T)min_image_1<double>((double)x[j] - (double)x[i], L64[0]);
+      dy = (T)min_image_1<double>((double)y[j] - (double)y[i], L64[1]);
+      dz = (T)min_image_1<double>((double)z[j] - (double)z[i], L64[2]);
+    }
     const T d2 = std::is_same<T, float>::value ? (T)d2f : dx * dx + dy * dy + dz * dz;
     const int ti = type[i], tj = type[j];
     const T eij = (T)std::sqrt(eps[ti] * eps[tj]);

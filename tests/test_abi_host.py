@@ -1,0 +1,122 @@
+"""Host-side checks of the C ABI (no GPU): exported symbols, configuration
+validation with the reference's error behaviour, and the k-sampler stream."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import rbmd_amd
+from rbmd_amd import systems
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "rbmdx.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rbmdx_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_symbols()
+    assert len(names) >= 14
+    lib = ctypes.CDLL(rbmd_amd.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+    # the Python binding binds exactly the declared set
+    assert set(rbmd_amd.SIGNATURES) == set(names)
+    out = subprocess.run(["nm", "-D", "--defined-only", rbmd_amd.LIB_PATH],
+                         capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (rbmdx_\w+)", out))
+    assert set(names) <= exported
+
+
+def test_binding_loads_and_binds():
+    lib = rbmd_amd.load_library()
+    assert lib.rbmdx_create.restype == ctypes.c_int
+
+
+def test_product_never_imports_the_oracle():
+    for dirpath, _, files in os.walk(os.path.join(ROOT, "rbmd_amd")):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
+                src = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in src and "liboracle" not in src, f
+    for f in ("rbmdx.h",):
+        assert "oracle" not in open(os.path.join(ROOT, "include", f)).read()
+
+
+GOOD = systems.ljsalt_config()
+
+
+def _cfg(**edits):
+    import copy
+
+    c = copy.deepcopy(GOOD)
+    for path, val in edits.items():
+        node = c
+        keys = path.split(".")
+        for k in keys[:-1]:
+            node = node[k]
+        if val is None:
+            node.pop(keys[-1], None)
+        else:
+            node[keys[-1]] = val
+    return c
+
+
+@pytest.mark.parametrize("edit,msg", [
+    ({}, None),
+    ({"hyper_parameters.neighbor.energy_rbl_flag": None}, "energy_rbl_flag"),
+    ({"hyper_parameters.neighbor.r_core": 3.5}, "r_core"),
+    ({"hyper_parameters.neighbor.neighbor_sample_num": 0}, "neighbor_sample_num"),
+    ({"hyper_parameters.coulomb.energy_rbe_flag": None}, "energy_rbe_flag"),
+    ({"hyper_parameters.coulomb.coulomb_sample_num": 0}, "sample number"),
+    ({"hyper_parameters.coulomb.alpha": -1.0}, "alpha"),
+    ({"hyper_parameters.coulomb.alpha": None}, "accuracy"),
+    ({"hyper_parameters.coulomb.kmax": [2, 2]}, "kmax"),
+    ({"hyper_parameters.coulomb.random_seed": 2 ** 33}, "UINT_MAX"),
+    ({"hyper_parameters.coulomb.type": "PPPM"}, "coulomb.type"),
+    ({"hyper_parameters.force_field.type": "EAM"}, "force_field.type"),
+    ({"hyper_parameters.neighbor.type": "VERLET-SOG"}, "neighbor.type"),
+    ({"hyper_parameters.neighbor.skin": -0.1}, "skin"),
+    ({"init_configuration.read_data.unit": "SI"}, "unit"),
+])
+def test_config_validation(edit, msg):
+    rc, err = rbmd_amd.check_config(_cfg(**edit))
+    if msg is None:
+        assert rc == 0, err
+    else:
+        assert rc == 1 and msg in err, (rc, err)
+
+
+def test_config_accepts_comments_and_bare_hyper_parameters():
+    text = """{
+      // RBMD allows comments
+      "force_field": {"type": "LJ/CUT"}, /* block comment */
+      "neighbor": {"type": "Verlet-List", "cut_off": 2.5, "skin": 0.3}
+    }"""
+    assert rbmd_amd.check_config(text)[0] == 0
+    assert rbmd_amd.check_config("{ bad json")[0] == 1
+
+
+@pytest.mark.parametrize("seed", [0, 1, 12345, 20261202, 4294967295])
+def test_sampler_stream_equals_oracle(seed):
+    L = [62.99605249474366] * 3
+    for call in (0, 1, 2, 17, 999):
+        a = rbmd_amd.rbe_sample_host(0.09, L, 200, seed, call)
+        b = oracle.rbe_sample(seed, call, 200, 0.09, L)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_sampler_other_boxes():
+    for alpha, L, P in [(0.3, [12.0, 14.5, 9.25], 64), (0.01, [90.0] * 3, 500),
+                        (1.5, [5.0] * 3, 17)]:
+        a = rbmd_amd.rbe_sample_host(alpha, L, P, 77, 3)
+        b = oracle.rbe_sample(77, 3, P, alpha, L)
+        np.testing.assert_array_equal(a, b)
+        assert not (a == 0).all(1).any()
