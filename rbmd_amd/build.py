@@ -56,6 +56,13 @@ def build(verbose=False):
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
          + ["-lpthread"])
     os.replace(tmp, LIB)
+    # C++ driver of the force-module layer, linked against the library
+    drv = os.path.join(HERE, "rbmdx_plugin_driver")
+    _run(["g++"] + CXX_FLAGS + ["-o", drv + ".tmp", os.path.join(CSRC, "plugin_driver.cpp"),
+                                "-L", HERE, "-lrbmdx", "-Wl,-rpath,$ORIGIN",
+                                "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+                                "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(drv + ".tmp", drv)
     if verbose:
         print("built", LIB)
     return LIB

@@ -1,0 +1,104 @@
+// Minimal C++ driver of the force-module layer: reads an RBMD JSON file and
+// a plain-text system file, runs `steps` evaluations through
+// rbmdx::plugin::MakeForce(force_field.type, ...) and prints the forces of
+// the last evaluation and the potential energy, one value per line.
+//
+// System file: n ntypes / lo[3] / L[3] / ntypes lines "eps sigma" /
+// n lines "id type q x y z".  Positions are uploaded once; precision follows
+// the JSON ("fp32" default, "fp64").
+#include <hip/hip_runtime_api.h>
+
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "force_plugin.hpp"
+
+using rbmdx::plugin::DeviceData;
+
+static std::string slurp(const char* path) {
+  std::ifstream f(path);
+  std::stringstream ss;
+  ss << f.rdbuf();
+  return ss.str();
+}
+
+template <typename R>
+static int run(const std::string& json, const std::string& fftype, DeviceData& d,
+               const std::vector<double>& pos, int steps) {
+  const size_t n = (size_t)d.num_atoms;
+  std::vector<R> h(n);
+  R* dev[6];
+  for (int k = 0; k < 6; ++k)
+    if (hipMalloc((void**)&dev[k], n * sizeof(R)) != hipSuccess) return 3;
+  for (int a = 0; a < 3; ++a) {
+    for (size_t i = 0; i < n; ++i) h[i] = (R)pos[3 * i + a];
+    hipMemcpy(dev[a], h.data(), n * sizeof(R), hipMemcpyHostToDevice);
+  }
+  d.d_px = dev[0];
+  d.d_py = dev[1];
+  d.d_pz = dev[2];
+  d.d_fx = dev[3];
+  d.d_fy = dev[4];
+  d.d_fz = dev[5];
+  auto force = rbmdx::plugin::MakeForce(fftype, json, &d, 0);
+  force->Init();
+  force->set_energy_every_call(true);
+  for (int s = 0; s < steps; ++s) force->Execute();
+  force->EvaluatePotentialEnergy();
+  const auto& pe = force->potential_energy();
+  std::printf("energy %.17g %.17g %.17g\n", pe.vdwl, pe.coul, pe.kspace);
+  std::printf("virial %.17g %.17g %.17g %.17g %.17g %.17g\n", pe.virial[0],
+              pe.virial[1], pe.virial[2], pe.virial[3], pe.virial[4],
+              pe.virial[5]);
+  std::vector<R> f(3 * n);
+  for (int a = 0; a < 3; ++a) {
+    hipMemcpy(h.data(), dev[3 + a], n * sizeof(R), hipMemcpyDeviceToHost);
+    for (size_t i = 0; i < n; ++i) f[3 * i + a] = h[i];
+  }
+  for (size_t i = 0; i < n; ++i)
+    std::printf("f %.9g %.9g %.9g\n", (double)f[3 * i], (double)f[3 * i + 1],
+                (double)f[3 * i + 2]);
+  force.reset();
+  for (int k = 0; k < 6; ++k) hipFree(dev[k]);
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    std::fprintf(stderr, "usage: %s config.json system.txt force_field_type [steps]\n",
+                 argv[0]);
+    return 2;
+  }
+  const std::string json = slurp(argv[1]);
+  std::ifstream sf(argv[2]);
+  int64_t n;
+  int nt;
+  sf >> n >> nt;
+  DeviceData d;
+  d.num_atoms = n;
+  for (double& v : d.box_lo) sf >> v;
+  for (double& v : d.box_len) sf >> v;
+  d.eps.resize(nt);
+  d.sigma.resize(nt);
+  for (int t = 0; t < nt; ++t) sf >> d.eps[t] >> d.sigma[t];
+  d.id.resize(n);
+  d.type.resize(n);
+  d.charge.resize(n);
+  std::vector<double> pos(3 * n);
+  for (int64_t i = 0; i < n; ++i)
+    sf >> d.id[i] >> d.type[i] >> d.charge[i] >> pos[3 * i] >> pos[3 * i + 1] >>
+        pos[3 * i + 2];
+  const int steps = argc > 4 ? std::atoi(argv[4]) : 1;
+  const bool fp64 = json.find("\"fp64\"") != std::string::npos;
+  try {
+    return fp64 ? run<double>(json, argv[3], d, pos, steps)
+                : run<float>(json, argv[3], d, pos, steps);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+}
