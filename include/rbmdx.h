@@ -127,6 +127,29 @@ int rbmdx_force(rbmdx_ctx* ctx, int64_t step, const void* px, const void* py,
                 const int32_t* m_samples, int32_t num_k, uint32_t flags,
                 void* stream);
 
+/*
+ * Per-type masses (ntypes entries, default 1) for rbmdx_force_leapfrog.
+ * Replaces the mass table RBMD's integrator reads (DeviceData _d_mass).
+ */
+int rbmdx_set_masses(rbmdx_ctx* ctx, const double* mass);
+
+/*
+ * One force evaluation fused with the leapfrog update of the caller's arrays,
+ * in the same kernel that assembles the force:
+ *   F = force(x);  v += dt * ftm2v * F / m;  x += dt * v
+ * (positions and velocities updated in place; forces written to fx..fz).
+ * This is velocity Verlet with the second half-kick of one step and the first
+ * half-kick of the next merged (RBMD's UpdateVelocity / position update,
+ * src/simulate_pipeline/src/controller/velocity_controller/
+ * velocity_controller_op/rocm/update_velocity_op.hip.cu:5-27); ftm2v follows
+ * the unit style (src/common/unit_factor.cpp:30-50).  The k-sample comes from
+ * the seeded stream.
+ */
+int rbmdx_force_leapfrog(rbmdx_ctx* ctx, int64_t step, void* px, void* py,
+                         void* pz, void* vx, void* vy, void* vz, void* fx,
+                         void* fy, void* fz, double dt, uint32_t flags,
+                         void* stream);
+
 /* Synchronise the context's last stream and read energies/virial. */
 int rbmdx_thermo(rbmdx_ctx* ctx, rbmdx_thermo_t* out);
 

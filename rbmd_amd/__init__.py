@@ -102,6 +102,9 @@ SIGNATURES = {
     "rbmdx_export_candidates": (_I64, [_P, _IP, _IP, _IP, _I64]),
     "rbmdx_kernel_times": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_char_p), _DP,
                                           ctypes.c_int]),
+    "rbmdx_set_masses": (ctypes.c_int, [_P, _DP]),
+    "rbmdx_force_leapfrog": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                            ctypes.c_double, ctypes.c_uint32, _P]),
     "rbmdx_rbe_sample_host": (ctypes.c_int, [ctypes.c_double, _DP, _I32, ctypes.c_uint32,
                                              _I64, _IP]),
     "rbmdx_check_config": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _I32]),
@@ -236,6 +239,22 @@ class ForceContext:
             self._h, int(step), _tensor_ptr(px), _tensor_ptr(py), _tensor_ptr(pz),
             _tensor_ptr(fx), _tensor_ptr(fy), _tensor_ptr(fz),
             _tensor_ptr(m_samples), nk, flags, ctypes.c_void_p(stream)))
+        self.calls += 1
+
+    def set_masses(self, masses):
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        self._check(self.lib.rbmdx_set_masses(self._h, _dptr(m)))
+
+    def force_leapfrog(self, step, px, py, pz, vx, vy, vz, fx, fy, fz, dt, energy=False,
+                       sync=False, stream=None):
+        """Force evaluation fused with v += dt F/m, x += dt v (in place)."""
+        flags = (RBMDX_FLAG_ENERGY if energy else 0) | (RBMDX_FLAG_SYNC if sync else 0)
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(px.device).cuda_stream
+        args = [_tensor_ptr(t) for t in (px, py, pz, vx, vy, vz, fx, fy, fz)]
+        self._check(self.lib.rbmdx_force_leapfrog(self._h, int(step), *args, float(dt), flags,
+                                                  ctypes.c_void_p(stream)))
         self.calls += 1
 
     def thermo(self):

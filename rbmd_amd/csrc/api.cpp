@@ -63,6 +63,7 @@ struct Config {
   bool has_seed = false;
   uint32_t seed = 0;
   double qqr2e = 1.0;
+  double ftm2v = 1.0;  // force/mass -> velocity/time (unit_factor.cpp:30-50)
   bool fp64 = false;
 };
 
@@ -89,12 +90,16 @@ Config parse_config(const std::string& text) {
   Config c;
   if (const Json* u = root.at({"init_configuration", "read_data", "unit"})) {
     const std::string unit = str_of(u, "unit");
-    if (unit == "LJ")
+    if (unit == "LJ") {
       c.qqr2e = 1.0;
-    else if (unit == "REAL")
+      c.ftm2v = 1.0;
+    } else if (unit == "REAL") {
       c.qqr2e = 332.06371;
-    else if (unit == "METAL")
+      c.ftm2v = 4.186e-4;
+    } else if (unit == "METAL") {
       c.qqr2e = 14.399645;
+      c.ftm2v = 1.0 / 1.0364269e-4;
+    }
     else
       throw Error(RBMDX_E_CONFIG, "unsupported unit style: " + unit);
   }
@@ -218,6 +223,11 @@ struct EngineBase {
                      const void* pz, void* fx, void* fy, void* fz,
                      const int32_t* m, int32_t nk, uint32_t flags,
                      hipStream_t s) = 0;
+  virtual void force_leapfrog(int64_t step, void* px, void* py, void* pz,
+                              void* vx, void* vy, void* vz, void* fx, void* fy,
+                              void* fz, double dt, uint32_t flags,
+                              hipStream_t s) = 0;
+  virtual void set_masses(const double* m) = 0;
   virtual void thermo(rbmdx_thermo_t* out) = 0;
   virtual int64_t export_pairs(int64_t step, const void* px, const void* py,
                                const void* pz, int candidates, int32_t* gi,
@@ -314,6 +324,39 @@ class Engine : public EngineBase {
     sampler_->draw(rbe_seed_of_call(base_seed_, call), m);
   }
 
+  void set_masses(const double* m) override {
+    if (!ready_) throw Error(RBMDX_E_STATE, "rbmdx_set_system first");
+    std::vector<R> inv(ntypes_);
+    for (int t = 0; t < ntypes_; ++t) {
+      if (!(m[t] > 0)) throw Error(RBMDX_E_ARG, "masses must be > 0");
+      inv[t] = (R)(1.0 / m[t]);
+    }
+    HIPCHK(hipMemcpy(d_.inv_mass, inv.data(), sizeof(R) * ntypes_,
+                     hipMemcpyHostToDevice));
+  }
+
+  void force_leapfrog(int64_t step, void* px, void* py, void* pz, void* vx,
+                      void* vy, void* vz, void* fx, void* fy, void* fz,
+                      double dt, uint32_t flags, hipStream_t s) override {
+    if (!vx || !vy || !vz) throw Error(RBMDX_E_ARG, "null velocity pointer");
+    d_.vx = (R*)vx;
+    d_.vy = (R*)vy;
+    d_.vz = (R*)vz;
+    d_.xw = (R*)px;
+    d_.yw = (R*)py;
+    d_.zw = (R*)pz;
+    leap_dt_ = dt;
+    try {
+      force(step, px, py, pz, fx, fy, fz, nullptr, 0, flags, s);
+    } catch (...) {
+      d_.vx = d_.vy = d_.vz = nullptr;
+      leap_dt_ = 0;
+      throw;
+    }
+    d_.vx = d_.vy = d_.vz = nullptr;
+    leap_dt_ = 0;
+  }
+
   void force(int64_t step, const void* px, const void* py, const void* pz,
              void* fx, void* fy, void* fz, const int32_t* m, int32_t nk,
              uint32_t flags, hipStream_t s) override {
@@ -328,10 +371,12 @@ class Engine : public EngineBase {
     int force_rebuild = (flags & RBMDX_FLAG_REBUILD) ? 1 : 0;
     if (d_.cap_a == 0) {
       plan_capacity(s);
-      force_rebuild = 1;
-    } else if (grow_if_needed(s)) {
-      force_rebuild = 1;
+    } else {
+      grow_if_needed(s);
     }
+    // freshly (re)allocated lists hold no entries yet: rebuild before use
+    if (lists_stale_) force_rebuild = 1;
+    lists_stale_ = false;
     StepArgs<R> a = args(step, flags);
     a.force_rebuild = force_rebuild;
     // k-sample of this evaluation
@@ -440,6 +485,8 @@ class Engine : public EngineBase {
     } else if (d_.cap_a == 0) {
       throw Error(RBMDX_E_STATE, "no lists built yet");
     }
+    if (!candidates) lists_stale_ = false;
+    else if (lists_stale_) throw Error(RBMDX_E_STATE, "no lists built yet");
     StepArgs<R> a = args(step, 0);
     const int ecap = (int)std::min<int64_t>(cap, 1LL << 30);
     int* buf = dalloc<int>((size_t)3 * std::max(ecap, 1));
@@ -525,6 +572,8 @@ class Engine : public EngineBase {
   std::vector<std::array<hipEvent_t, 6>> timing_events_;  // reusable pool
   size_t timing_used_ = 0;
   unsigned rebuild_thr_ = 0;
+  bool lists_stale_ = true;  // lists (re)allocated since the last rebuild
+  double leap_dt_ = 0;        // fused leapfrog time step of the current call
 
   template <typename T>
   T* alloc(size_t n) {
@@ -744,6 +793,12 @@ class Engine : public EngineBase {
     d_.fk = alloc<R4>(nc);
     d_.cnt_a = alloc<int>(n_);
     d_.cnt_b = alloc<int>(nc);
+    d_.inv_mass = alloc<R>(ntypes_);
+    {
+      std::vector<R> one(ntypes_, (R)1);
+      HIPCHK(hipMemcpy(d_.inv_mass, one.data(), sizeof(R) * ntypes_,
+                       hipMemcpyHostToDevice));
+    }
     d_.pair_part_blocks = grid_.nbrick;
     d_.pair_part = alloc<double>((size_t)11 * grid_.nbrick);
     d_.thermo = alloc<double>(32);
@@ -883,6 +938,8 @@ class Engine : public EngineBase {
     a.p.eflag = (flags & RBMDX_FLAG_ENERGY) ? 1 : 0;
     a.lj_tab = lj_tab_;
     a.rebuild_thr = rebuild_thr_;
+    a.dt = leap_dt_;
+    a.dt_ftm = leap_dt_ * cfg_.ftm2v;
     a.e_self = e_self_;
     a.m_use = d_.m_dev;
     return a;
@@ -901,6 +958,7 @@ class Engine : public EngineBase {
   }
 
   void alloc_lists(int max_a, int max_b) {
+    lists_stale_ = true;
     // list A is read 8 entries at a time: capacity is a multiple of 8
     const int cap_a = ((int)std::ceil(max_a * 1.2) + 16 + 7) / 8 * 8;
     const int cap_b = (int)std::ceil(max_b * 1.2) + 16;
@@ -1031,6 +1089,22 @@ int rbmdx_force(rbmdx_ctx* ctx, int64_t step, const void* px, const void* py,
   return guarded(ctx, [&] {
     ctx->eng->force(step, px, py, pz, fx, fy, fz, m_samples, num_k, flags,
                     (hipStream_t)stream);
+  });
+}
+
+int rbmdx_set_masses(rbmdx_ctx* ctx, const double* mass) {
+  if (!ctx || !mass) return RBMDX_E_ARG;
+  return guarded(ctx, [&] { ctx->eng->set_masses(mass); });
+}
+
+int rbmdx_force_leapfrog(rbmdx_ctx* ctx, int64_t step, void* px, void* py,
+                         void* pz, void* vx, void* vy, void* vz, void* fx,
+                         void* fy, void* fz, double dt, uint32_t flags,
+                         void* stream) {
+  if (!ctx) return RBMDX_E_ARG;
+  return guarded(ctx, [&] {
+    ctx->eng->force_leapfrog(step, px, py, pz, vx, vy, vz, fx, fy, fz, dt, flags,
+                             (hipStream_t)stream);
   });
 }
 

@@ -86,6 +86,13 @@ struct Dev {
   R* fx = nullptr;
   R* fy = nullptr;
   R* fz = nullptr;
+  R* vx = nullptr;         // [n] velocities, fused leapfrog update only
+  R* vy = nullptr;
+  R* vz = nullptr;
+  R* xw = nullptr;         // [n] writable positions (same arrays as px..pz)
+  R* yw = nullptr;
+  R* zw = nullptr;
+  R* inv_mass = nullptr;   // [ntypes]
   int* type = nullptr;     // [n]
   int* gid = nullptr;      // [n]
   R* charge = nullptr;     // [n]
@@ -166,6 +173,8 @@ struct StepArgs {
   int force_rebuild = 0;
   int count_only = 0;         // list builders only count (capacity planning)
   double e_self = 0.0;        // k-space self + background energy
+  double dt = 0.0;            // fused leapfrog: x += dt v
+  double dt_ftm = 0.0;        //                 v += dt ftm2v F / m
   hipEvent_t ev[6] = {};      // optional section timing events
 };
 

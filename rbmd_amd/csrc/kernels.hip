@@ -1200,10 +1200,14 @@ __global__ __launch_bounds__(64 * kRbeWaves) void k_rbe_force(Dev<R> d,
 }
 
 // Total force in caller order (one thread per caller atom); resets the skin
-// accumulator for the next call and mirrors the counters to the host.
+// accumulator for the next call and mirrors the counters to the host.  With
+// velocities bound it also applies the leapfrog update
+// v += dt ftm2v F / m, x += dt v (RBMD's UpdateVelocity half-kicks of two
+// consecutive steps merged, src/simulate_pipeline/src/controller/
+// velocity_controller/velocity_controller_op/rocm/update_velocity_op.hip.cu:5-27).
 template <typename R>
 __global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix,
-                                                  int has_k) {
+                                                  int has_k, R dt, R dt_ftm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     d.ctrl->maxdisp_bits = 0u;
@@ -1236,6 +1240,16 @@ __global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix,
   d.fx[i] = f.x;
   d.fy[i] = f.y;
   d.fz[i] = f.z;
+  if (d.vx) {
+    const R s = dt_ftm * d.inv_mass[d.type[i]];
+    const R vx = d.vx[i] + s * f.x, vy = d.vy[i] + s * f.y, vz = d.vz[i] + s * f.z;
+    d.vx[i] = vx;
+    d.vy[i] = vy;
+    d.vz[i] = vz;
+    d.xw[i] = d.px[i] + dt * vx;
+    d.yw[i] = d.py[i] + dt * vy;
+    d.zw[i] = d.pz[i] + dt * vz;
+  }
 }
 
 // Parity export (block per brick): classified pairs (or raw candidates) as
@@ -1387,7 +1401,8 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
                        dim3(64 * kRbeWaves), 0, s, d, a.box, K);
   if (a.ev[4]) hipEventRecord(a.ev[4], s);
   hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,
-                     s, d, p.mean_fix, (K > 0 && d.nc > 0) ? 1 : 0);
+                     s, d, p.mean_fix, (K > 0 && d.nc > 0) ? 1 : 0, (R)a.dt,
+                     (R)a.dt_ftm);
   if (a.ev[5]) hipEventRecord(a.ev[5], s);
   return hipGetLastError();
 }

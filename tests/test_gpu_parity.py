@@ -193,3 +193,140 @@ def test_lj_fcc_fp64_verlet(device):
     th = ctx.thermo()
     assert th["e_vdwl"] == pytest.approx(en[0], rel=1e-11)
     np.testing.assert_allclose(th["virial"], vir, rtol=1e-10, atol=1e-10)
+
+
+def _kgrid(kmax):
+    r = [np.arange(-k, k + 1) for k in kmax]
+    m = np.stack(np.meshgrid(*r, indexing="ij"), -1).reshape(-1, 3)
+    return m[(m != 0).any(1)].astype(np.int32)
+
+
+def test_ewald_exact_matches_oracle(device):
+    import torch
+
+    s = systems.electrolyte(1500, seed=5)
+    kmax = (7, 7, 7)
+    cfg = systems.ljsalt_config(neighbor="Verlet-List", coulomb="EWALD", kmax=kmax,
+                                alpha=0.25)
+    ctx = _ctx(s, cfg)
+    (px, py, pz), f = _tensors(s, device, torch.float32)
+    ctx.force(0, px, py, pz, *f, energy=True, sync=True)
+    pos32 = s.pos.astype(np.float32)
+    ref, en, vir, _, _ = reference_force(s, pos32, cfg, 0, None, with_kspace=False)
+    ci = np.nonzero(s.charges)[0]
+    m = _kgrid(kmax)
+    k2 = ((2 * np.pi * m / s.box_len) ** 2).sum(1)
+    fk, _, ek, vk = oracle.kspace_f64(pos32.astype(np.float64)[ci], s.charges[ci], s.box_len,
+                                      m, np.exp(-k2 / (4 * 0.25)), 1.0, 0.25)
+    ref[ci] += fk
+    V = float(np.prod(s.box_len))
+    ek += oracle.kspace_self((s.charges ** 2).sum(), s.charges.sum(), 0.25, V, 1.0)
+    err = rel_rms(_stack(f), ref)
+    print(f"EWALD force rel RMS vs FP64 = {err:.3e}")
+    assert err < 2e-5
+    th = ctx.thermo()
+    assert th["num_k"] == len(m)
+    assert th["e_kspace"] == pytest.approx(ek, rel=1e-5)
+    np.testing.assert_allclose(th["virial_kspace"], vk, rtol=1e-5, atol=1e-6 * abs(vk).max())
+
+
+def test_small_box_whole_dimension_stencil(device):
+    import torch
+
+    # 2 cells per dimension: every dimension is scanned once, no duplicate pairs
+    s = systems.electrolyte(400, seed=8, d_min=0.8)
+    cfg = systems.ljsalt_config(cut_off=3.0, lj_cut_off=3.0, r_core=2.0, skin=0.2, P=50)
+    assert s.box_len[0] < 8.0
+    ctx = _ctx(s, cfg)
+    (px, py, pz), f = _tensors(s, device, torch.float32)
+    m = rbe_sample_host(0.09, s.box_len, 50, 3, 0)
+    mt = torch.tensor(m.reshape(-1), dtype=torch.int32, device=device)
+    ctx.force(5, px, py, pz, *f, m_samples=mt, sync=True)
+    ref = reference_force(s, s.pos.astype(np.float32), cfg, 5, m)[0]
+    assert rel_rms(_stack(f), ref) < 2e-5
+    gi, gj, cls = ctx.export_candidates()
+    pairs = list(zip(gi.tolist(), gj.tolist()))
+    assert len(pairs) == len(set(pairs))
+
+
+def test_list_overflow_fallback_is_exact(device):
+    import torch
+
+    s = systems.electrolyte(4000, seed=13)
+    cfg = systems.ljsalt_config(P=64)
+    ctx = _ctx(s, cfg)
+    (px, py, pz), f = _tensors(s, device, torch.float32)
+    m = rbe_sample_host(0.09, s.box_len, 64, 9, 0)
+    mt = torch.tensor(m.reshape(-1), dtype=torch.int32, device=device)
+    ctx.force(0, px, py, pz, *f, m_samples=mt, sync=True)
+    # squeeze a slab of atoms towards a plane: local density (and list counts)
+    # jump beyond the planned capacity; the same call must still be exact
+    pos = s.pos.copy()
+    L = s.box_len[0]
+    sel = (pos[:, 2] > 0.4 * L) & (pos[:, 2] < 0.6 * L)
+    pos[sel, 2] = 0.5 * L + 0.75 * (pos[sel, 2] - 0.5 * L)
+    pos32 = pos.astype(np.float32)
+    px.copy_(torch.tensor(pos32[:, 0], device=device))
+    py.copy_(torch.tensor(pos32[:, 1], device=device))
+    pz.copy_(torch.tensor(pos32[:, 2], device=device))
+    ctx.force(1, px, py, pz, *f, m_samples=mt, sync=True)
+    ref = reference_force(s, pos32, cfg, 1, m)[0]
+    assert rel_rms(_stack(f), ref) < 2e-5
+    ctx.force(2, px, py, pz, *f, m_samples=mt, sync=True)  # after growth
+    ref = reference_force(s, pos32, cfg, 2, m)[0]
+    assert rel_rms(_stack(f), ref) < 2e-5
+
+
+def test_cpp_force_module_driver(device, tmp_path):
+    import json
+    import os
+    import subprocess
+
+    import rbmd_amd
+
+    drv = os.path.join(os.path.dirname(rbmd_amd.LIB_PATH), "rbmdx_plugin_driver")
+    s = systems.lj_fcc(cells=6, seed=4)
+    cfg = systems.lj_config(precision="fp64")
+    (tmp_path / "cfg.json").write_text(json.dumps(cfg))
+    lines = [f"{s.n} 1", " ".join(map(str, s.box_lo)), " ".join(map(str, s.box_len)),
+             "1.0 1.0"]
+    lines += [f"{i} 0 0.0 {x!r} {y!r} {z!r}" for i, (x, y, z) in enumerate(s.pos.tolist())]
+    (tmp_path / "sys.txt").write_text("\n".join(lines) + "\n")
+    out = subprocess.run([drv, str(tmp_path / "cfg.json"), str(tmp_path / "sys.txt"), "LJ/CUT",
+                          "2"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = out.stdout.split("\n")
+    e = [float(v) for v in rows[0].split()[1:]]
+    fr = np.array([[float(v) for v in r.split()[1:]] for r in rows if r.startswith("f ")])
+    ref, en, vir = oracle.verlet_force_f64(s.pos, s.charges, s.types, s.eps, s.sigma,
+                                           s.box_len, 2.5)
+    assert rel_rms(fr, ref) < 1e-8  # printed with 9 significant digits
+    assert e[0] == pytest.approx(en[0], rel=1e-11)
+
+
+def test_fused_leapfrog_matches_separate_update(device):
+    import torch
+
+    s = systems.electrolyte(4000, seed=21)
+    cfg = systems.ljsalt_config(P=32)
+    ctx_a = _ctx(s, cfg)
+    ctx_b = _ctx(s, cfg)
+    masses = np.array([2.0, 3.0, 1.5])
+    ctx_a.set_masses(masses)
+    p0 = torch.tensor(s.pos.T.copy(), dtype=torch.float32, device=device)
+    v0 = torch.tensor(s.vel.T.copy(), dtype=torch.float32, device=device)
+    pa, va = p0.clone(), v0.clone()
+    pb, vb = p0.clone(), v0.clone()
+    fa = torch.zeros_like(p0)
+    fb = torch.zeros_like(p0)
+    inv_m = torch.tensor(1.0 / masses[s.types], dtype=torch.float32, device=device)
+    dt = 0.002
+    for k in range(3):
+        ctx_a.force_leapfrog(k, pa[0], pa[1], pa[2], va[0], va[1], va[2], fa[0], fa[1], fa[2],
+                             dt, sync=True)
+        ctx_b.force(k, pb[0], pb[1], pb[2], fb[0], fb[1], fb[2], sync=True)
+        vb += dt * fb * inv_m
+        pb += dt * vb
+        np.testing.assert_allclose(fa.cpu().numpy(), fb.cpu().numpy(), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(pa.cpu().numpy(), pb.cpu().numpy(), rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(va.cpu().numpy(), vb.cpu().numpy(), rtol=1e-5, atol=1e-5)
