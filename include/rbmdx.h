@@ -56,7 +56,10 @@ enum {
 enum {
   RBMDX_FLAG_ENERGY = 1,   /* also accumulate energies and the virial      */
   RBMDX_FLAG_SYNC = 2,     /* synchronise and validate before returning    */
-  RBMDX_FLAG_REBUILD = 4   /* force a neighbour rebuild this call          */
+  RBMDX_FLAG_REBUILD = 4,  /* force a neighbour rebuild this call          */
+  RBMDX_FLAG_TIMING = 8    /* record per-phase GPU timing events this call
+                              (read with rbmdx_kernel_times; the events
+                              themselves cost a few microseconds per call) */
 };
 
 /* Thermo record filled by rbmdx_thermo (all FP64, extensive totals). */
@@ -203,9 +206,11 @@ int rbmdx_rbe_sample_host(double alpha, const double* L, int32_t P,
                           int32_t* m_out);
 int rbmdx_check_config(const char* json_text, char* err, int32_t errlen);
 
-/* Device-kernel timing of the last call: fills up to n entries of
- * (name, microseconds) when the context was created with timing enabled
- * (RBMDX_TIMING=1 in the environment).  Returns the number of kernels. */
+/* Per-phase GPU times ("rebuild", "pair", "kspace", "assemble"), averaged
+ * over the calls timed since the previous rbmdx_kernel_times: calls made with
+ * RBMDX_FLAG_TIMING, or every call when the context was created with
+ * RBMDX_TIMING=1 in the environment.  Fills up to n (name, microseconds)
+ * entries and returns their number (synchronises). */
 int rbmdx_kernel_times(rbmdx_ctx* ctx, const char** names, double* usec,
                        int n);
 

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "librbmdx.so")
 RBMDX_FLAG_ENERGY = 1
 RBMDX_FLAG_SYNC = 2
 RBMDX_FLAG_REBUILD = 4
+RBMDX_FLAG_TIMING = 8
 
 _STATUS = {
     0: "RBMDX_OK",
@@ -227,10 +228,11 @@ class ForceContext:
         self._check(self.lib.rbmdx_set_charges(self._h, _dptr(q)))
 
     def force(self, step, px, py, pz, fx, fy, fz, m_samples=None, energy=False,
-              sync=False, rebuild=False, stream=None):
+              sync=False, rebuild=False, timing=False, stream=None):
         """Total force into fx/fy/fz (device tensors, caller order)."""
         flags = ((RBMDX_FLAG_ENERGY if energy else 0) | (RBMDX_FLAG_SYNC if sync else 0)
-                 | (RBMDX_FLAG_REBUILD if rebuild else 0))
+                 | (RBMDX_FLAG_REBUILD if rebuild else 0)
+                 | (RBMDX_FLAG_TIMING if timing else 0))
         nk = 0 if m_samples is None else int(m_samples.numel() // 3)
         if stream is None:
             import torch
@@ -246,9 +248,10 @@ class ForceContext:
         self._check(self.lib.rbmdx_set_masses(self._h, _dptr(m)))
 
     def force_leapfrog(self, step, px, py, pz, vx, vy, vz, fx, fy, fz, dt, energy=False,
-                       sync=False, stream=None):
+                       sync=False, timing=False, stream=None):
         """Force evaluation fused with v += dt F/m, x += dt v (in place)."""
-        flags = (RBMDX_FLAG_ENERGY if energy else 0) | (RBMDX_FLAG_SYNC if sync else 0)
+        flags = ((RBMDX_FLAG_ENERGY if energy else 0) | (RBMDX_FLAG_SYNC if sync else 0)
+                 | (RBMDX_FLAG_TIMING if timing else 0))
         if stream is None:
             import torch
             stream = torch.cuda.current_stream(px.device).cuda_stream

@@ -264,6 +264,9 @@ class Engine : public EngineBase {
         if (e) hipEventDestroy(e);
     for (auto e : slot_events_)
       if (e) hipEventDestroy(e);
+    if (ev_fork_) hipEventDestroy(ev_fork_);
+    if (ev_join_) hipEventDestroy(ev_join_);
+    if (ks_) hipStreamDestroy(ks_);
     if (stage_) hipHostFree(stage_);
     if (ctrl_host_) hipHostFree(ctrl_host_);
     // leave no sticky error behind for the caller's next HIP call
@@ -406,7 +409,7 @@ class Engine : public EngineBase {
     } else if (kspace_ == 2) {
       a.m_use = d_.m_dev;
     }
-    if (timing_) {
+    if (timing_ || (flags & RBMDX_FLAG_TIMING)) {
       if (timing_used_ == timing_events_.size()) {
         timing_events_.emplace_back();
         for (auto& e : timing_events_.back()) HIPCHK(hipEventCreate(&e));
@@ -437,19 +440,24 @@ class Engine : public EngineBase {
     if (!ready_) throw Error(RBMDX_E_STATE, "rbmdx_set_system first");
     HIPCHK(hipSetDevice(device_));
     if (last_stream_set_) HIPCHK(hipStreamSynchronize(last_stream_));
+    // device layout: [0..2] sum of pair forces, [3] E_lj, [4] E_coul,
+    // [5..10] pair virial, [16] k-space energy without the self terms,
+    // [17..22] k-space virial
     double t[32];
-    HIPCHK(hipMemcpy(t, d_.thermo, sizeof(double) * 18, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(t, d_.thermo, sizeof(double) * 32, hipMemcpyDeviceToHost));
     Ctrl c;
     HIPCHK(hipMemcpy(&c, d_.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
     std::memset(out, 0, sizeof(*out));
-    out->e_vdwl = t[0];
-    out->e_coul = t[1];
-    out->e_kspace = t[2];
+    const bool has_k = kspace_ != 0 && d_.nc > 0 && last_num_k_ > 0;
+    out->e_vdwl = t[3];
+    out->e_coul = t[4];
+    out->e_kspace = has_k ? t[16] + e_self_ : 0.0;
     for (int k = 0; k < 6; ++k) {
-      out->virial[k] = t[3 + k];
-      out->virial_kspace[k] = t[9 + k];
+      out->virial_kspace[k] = has_k ? t[17 + k] : 0.0;
+      out->virial[k] = t[5 + k] + out->virial_kspace[k];
     }
-    for (int k = 0; k < 3; ++k) out->mean_force[k] = t[15 + k];
+    if (params_.mean_fix)
+      for (int k = 0; k < 3; ++k) out->mean_force[k] = t[k] / params_.n_global;
     out->steps = (int64_t)c.step_count;
     out->rebuilds = (int64_t)c.rebuild_count;
     out->rbl_frequency = freq_;
@@ -511,22 +519,22 @@ class Engine : public EngineBase {
   }
 
   int kernel_times(const char** names, double* usec, int n) override {
-    static const char* kNames[5] = {"rebuild", "pair_a", "pair_b", "kspace",
-                                    "assemble"};
-    if (!timing_) return 0;
+    // phase -> (start event, end event); kspace runs on the second stream
+    static const char* kNames[4] = {"rebuild", "pair", "kspace", "assemble"};
+    static const int kFrom[4] = {0, 1, 3, 2}, kTo[4] = {1, 2, 4, 5};
     if (last_stream_set_) HIPCHK(hipStreamSynchronize(last_stream_));
-    double acc[5] = {0, 0, 0, 0, 0};
+    double acc[4] = {0, 0, 0, 0};
     const size_t calls = timing_used_;
     for (size_t c = 0; c < calls; ++c) {
       auto& ev = timing_events_[c];
-      for (int k = 0; k < 5; ++k) {
+      for (int k = 0; k < 4; ++k) {
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        HIPCHK(hipEventElapsedTime(&ms, ev[kFrom[k]], ev[kTo[k]]));
         acc[k] += ms * 1000.0;
       }
     }
     timing_used_ = 0;
-    const int m = std::min(n, 5);
+    const int m = std::min(n, 4);
     for (int k = 0; k < m; ++k) {
       if (names) names[k] = kNames[k];
       usec[k] = calls ? acc[k] / (double)calls : 0.0;
@@ -569,6 +577,8 @@ class Engine : public EngineBase {
   int64_t calls_ = 0, forced_ = 0;
   int last_num_k_ = 0;
   bool timing_ = false;
+  hipStream_t ks_ = nullptr;  // k-space stream (default priority)
+  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
   std::vector<std::array<hipEvent_t, 6>> timing_events_;  // reusable pool
   size_t timing_used_ = 0;
   unsigned rebuild_thr_ = 0;
@@ -908,6 +918,14 @@ class Engine : public EngineBase {
     }
     params_.num_k = kspace_ == 1 ? c.coulomb_sample_num : params_.num_k;
     timing_ = std::getenv("RBMDX_TIMING") && std::atoi(std::getenv("RBMDX_TIMING"));
+    // k-space beside the pair kernel on a second stream (RBMDX_KSTREAM=0:
+    // same stream, in order)
+    const char* kst = std::getenv("RBMDX_KSTREAM");
+    if (kspace_ && !(kst && std::atoi(kst) == 0) && !ks_) {
+      HIPCHK(hipStreamCreateWithFlags(&ks_, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+    }
     HIPCHK(hipDeviceSynchronize());
     ready_ = true;
   }
@@ -942,6 +960,9 @@ class Engine : public EngineBase {
     a.dt_ftm = leap_dt_ * cfg_.ftm2v;
     a.e_self = e_self_;
     a.m_use = d_.m_dev;
+    a.ks = ks_;
+    a.ev_fork = ev_fork_;
+    a.ev_join = ev_join_;
     return a;
   }
 

@@ -33,6 +33,8 @@ struct Ctrl {
   unsigned long long sum_count_b;
   int max_stage;              // largest brick neighbourhood seen
   int unstaged_bricks;        // bricks whose neighbourhood exceeded LDS
+  unsigned int pair_ticket;   // blocks of k_pair done (last block reduces)
+  unsigned int sk_ticket;     // blocks of k_sk done (last block reduces)
 };
 
 template <typename R>
@@ -135,7 +137,8 @@ struct Dev {
   float4* kf0 = nullptr;         // [kcap] (mx, my, mz, A Re rho)
   float4* kf1 = nullptr;         // [kcap] (kx, ky, kz, A Im rho)
   double2* rho = nullptr;        // [kcap]
-  double* thermo = nullptr;      // [32] device thermo record
+  double* thermo = nullptr;      // [32]: [0..10] pair sums (F, E_lj, E_coul,
+                                 // virial), [16..22] k-space (E, virial)
   double* mean = nullptr;        // [3]
   Ctrl* ctrl = nullptr;
   Ctrl* ctrl_host = nullptr;     // host-mapped mirror written each call
@@ -173,6 +176,8 @@ struct StepArgs {
   int force_rebuild = 0;
   int count_only = 0;         // list builders only count (capacity planning)
   double e_self = 0.0;        // k-space self + background energy
+  hipStream_t ks = nullptr;   // k-space stream (runs beside the pair kernel)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double dt = 0.0;            // fused leapfrog: x += dt v
   double dt_ftm = 0.0;        //                 v += dt ftm2v F / m
   hipEvent_t ev[6] = {};      // optional section timing events

@@ -14,11 +14,14 @@
 //                 classification applied on the fly; charged atoms then add
 //                 LJ + erfc Coulomb over their charged-pair lists, 16 lanes
 //                 per atom
-//   k_sk          S(k) partial sums over the charged atoms (k tiles x atoms)
-//   k_finalize    one block: FP64 reductions of S(k), energy, virial, mean
-//                 pair force; per-k force coefficients
+//                 (its last block reduces the per-brick FP64 partial sums:
+//                 mean pair force, energies, virial)
+//   k_sk          S(k) partial sums over the charged atoms (k tiles x atoms);
+//                 its last block reduces them in FP64 to rho_k, the k-space
+//                 energy and virial, and the per-k force coefficients
 //   k_rbe_force   per charged atom, k split over the waves of a block
 //   k_assemble    F = F_pair - <F_pair> + F_kspace in caller order
+// k_sk and k_rbe_force run on a second stream beside k_pair.
 #include <hip/hip_runtime.h>
 
 #include "engine.hpp"
@@ -834,6 +837,97 @@ __device__ __forceinline__ void interact_b(const PairConst<R>& pc,
   }
 }
 
+// Reciprocal of d^2 in the force terms: the hardware reciprocal (1 ulp) in
+// FP32, IEEE division in FP64.
+__device__ __forceinline__ float pair_rcp(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+__device__ __forceinline__ double pair_rcp(double x) { return 1.0 / x; }
+
+// min_image without a branch: the same single shift, d - sign(d) L when
+// |d| > L/2, so results are identical bit for bit.
+template <typename R>
+__device__ __forceinline__ R min_image_sel(R d, R L, R half) {
+  return d - (fabs(d) > half ? copysign(L, d) : R(0));
+}
+
+template <typename R>
+struct R2Of {
+  using type = float2;
+};
+template <>
+struct R2Of<double> {
+  using type = double2;
+};
+
+// All list-A entries of sorted atom i (LJ only).  Entries are read 8 at a
+// time (one 16-byte load of the interleaved list); their 8 neighbour
+// positions are fetched together (from the LDS stage when STAGED, else
+// through the slot tables from the sorted arrays), then evaluated without
+// branches: entries past the count and pairs outside the cutoffs are masked
+// by selects.  Only RBL shell pairs, rare in list A, branch to the hash.
+template <typename R, bool RBL, bool EFLAG, bool STAGED>
+__device__ __forceinline__ void list_a_atom(
+    const Dev<R>& d, const Box<R>& b, const PairConst<R>& pc, const R* s_lj,
+    const SlotTables<kMaxSlots>& t, int nslots, const R4T<R>* stage, int i,
+    int cnt, const R4T<R>& xi, int ti, int gi, Acc<R>& a) {
+  using R2 = typename R2Of<R>::type;
+  const uint4* lp = reinterpret_cast<const uint4*>(d.list_a) + i;
+  const int tbase = 2 * ti * pc.ntypes;
+  auto global_of = [&](int l) {
+    const int s = slot_of_pos(t, nslots, l);
+    return t.first[s] + (l - t.off[s]);
+  };
+  for (int k0 = 0; k0 < cnt; k0 += 8) {
+    const uint4 q8 = lp[(size_t)(k0 >> 3) * d.n];
+    const int kn = cnt - k0;
+    const unsigned wv[4] = {q8.x, q8.y, q8.z, q8.w};
+    int lj_[8];
+    R4T<R> xj[8];
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const int l = (int)((wv[kk >> 1] >> (16 * (kk & 1))) & 0xffffu);
+      if (STAGED) {
+        lj_[kk] = kk < kn ? l : 0;  // stage[0] exists whenever i does
+        xj[kk] = stage[lj_[kk]];
+      } else {
+        lj_[kk] = kk < kn ? global_of(l) : i;
+        xj[kk] = d.xs[lj_[kk]];
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const R dx = min_image_sel<R>(xj[kk].x - xi.x, b.L[0], b.half[0]);
+      const R dy = min_image_sel<R>(xj[kk].y - xi.y, b.L[1], b.half[1]);
+      const R dz = min_image_sel<R>(xj[kk].z - xi.z, b.L[2], b.half[2]);
+      const R d2 = fma_r(dz, dz, fma_r(dy, dy, mul_rn(dx, dx)));
+      const R2 lj = *reinterpret_cast<const R2*>(s_lj + tbase + 2 * (int)xj[kk].w);
+      const R inv = pair_rcp(d2);
+      const R s2 = lj.y * inv;
+      const R s6 = s2 * s2 * s2;
+      const R flj = R(-24) * lj.x * (R(2) * s6 - R(1)) * s6 * inv;
+      const bool in = kk < kn && d2 < pc.rclj2 && d2 < pc.rc2;
+      R w = R(1);
+      if (RBL) {
+        const bool core = d2 < pc.t_core;
+        w = (core && d2 < pc.rs2 && d2 > pc.eps_min) ? R(1) : R(0);
+        if (in && !core && d2 < pc.t_shell && d2 > pc.rs2) {
+          const int jg = STAGED ? global_of(lj_[kk]) : lj_[kk];
+          if (rbl_selected(pc.ifreq, pc.epoch, gi, d.gid_s[jg])) w = pc.freq;
+        }
+      }
+      const R f = in ? w * flj : R(0);
+      a.fx = fma_r(f, dx, a.fx);
+      a.fy = fma_r(f, dy, a.fy);
+      a.fz = fma_r(f, dz, a.fz);
+      if (EFLAG) {
+        a.e_lj += in ? (double)(R(2) * lj.x * (s6 - R(1)) * s6) : 0.0;
+        add_virial<R, EFLAG>(a, dx, dy, dz, in ? flj : R(0));
+      }
+    }
+  }
+}
+
 template <typename R>
 __device__ __forceinline__ void load_lj(const R* tab, int ntypes, R* s) {
   for (int t = threadIdx.x; t < 2 * ntypes * ntypes; t += blockDim.x)
@@ -867,7 +961,8 @@ __device__ __forceinline__ void write_partials(const Acc<R>& a, double* part,
 template <typename R, bool RBL, bool EFLAG>
 __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
                                               PairConst<R> pc, const R* lj_tab,
-                                              int split) {
+                                              int split, int mean_fix,
+                                              double n_global) {
   __shared__ R4T<R> stage[StageCap<R>::value];
   __shared__ SlotTables<kMaxSlots> t;
   __shared__ double smem[8 * 11];
@@ -890,25 +985,12 @@ __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
     const int gi = d.gid_s[i];
     const int cnt = d.cnt_a[i];
     if (cnt <= d.cap_a) {
-      const uint4* lp = reinterpret_cast<const uint4*>(d.list_a) + i;
-      for (int k0 = 0; k0 < cnt; k0 += 8) {
-        const uint4 q8 = lp[(size_t)(k0 >> 3) * d.n];
-        const unsigned w[4] = {q8.x, q8.y, q8.z, q8.w};
-        const int kn = min(8, cnt - k0);
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) {
-          if (kk < kn) {
-            const int l = (int)((w[kk >> 1] >> (16 * (kk & 1))) & 0xffffu);
-            auto global_j = [&]() {
-              const int s = slot_of_pos(t, h.nslots, l);
-              return t.first[s] + (l - t.off[s]);
-            };
-            const R4T<R> xj = staged ? stage[l] : d.xs[global_j()];
-            interact_a<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, xj,
-                                      [&] { return d.gid_s[global_j()]; }, a);
-          }
-        }
-      }
+      if (staged)
+        list_a_atom<R, RBL, EFLAG, true>(d, b, pc, s_lj, t, h.nslots, stage, i,
+                                         cnt, xi, ti, gi, a);
+      else
+        list_a_atom<R, RBL, EFLAG, false>(d, b, pc, s_lj, t, h.nslots, stage,
+                                          i, cnt, xi, ti, gi, a);
     } else {
       // list overflow since the last rebuild: scan the 27 cells directly
       int c[3];
@@ -1005,6 +1087,26 @@ __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
     }
   }
   write_partials<R, EFLAG>(acc, d.pair_part, smem);
+  // the last block to finish reduces all brick partials in a fixed order
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(&d.ctrl->pair_ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last) {
+    __threadfence();
+    double v[11];
+    for (int k = 0; k < 11; ++k) v[k] = 0.0;
+    for (int blk = threadIdx.x; blk < (int)gridDim.x; blk += blockDim.x)
+      for (int k = 0; k < 11; ++k) v[k] += d.pair_part[11 * blk + k];
+    block_sum<11>(v, smem);
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < 11; ++k) d.thermo[k] = v[k];
+      for (int k = 0; k < 3; ++k) d.mean[k] = mean_fix ? v[k] / n_global : 0.0;
+      d.ctrl->pair_ticket = 0u;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1030,9 +1132,16 @@ __device__ __forceinline__ void frac_coords(const Box<R>& b, const R4T<R>& p,
 }
 
 // S(k) partial sums: block (chunk of kSkChunk charged atoms) x (kSkTile k's).
+// The last block to finish sums the chunk partials of every k in FP64 (fixed
+// order), writes rho_k and the per-k force coefficients
+//   A_k = w_k (-4 pi qqr2e) / (V k^2),  kf0 = (m, A Re rho), kf1 = (k, A Im rho)
+// and the k-space energy and virial (without the self terms):
+//   E = qqr2e (2 pi / V) sum_k w_k |rho|^2 / k^2,
+//   W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/(4 alpha) + 1/k^2)).
 template <typename R>
 __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
-                                            const int* __restrict__ m, int K) {
+                                            const int* __restrict__ m, int K,
+                                            Params p) {
   __shared__ double smem[4 * 2 * kSkTile];
   const int chunk = blockIdx.x, k0 = blockIdx.y * kSkTile;
   const int kn = min(kSkTile, K - k0);
@@ -1049,10 +1158,10 @@ __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
   for (int kk = 0; kk < kSkTile; ++kk) re[kk] = im[kk] = 0.f;
   const int a0 = chunk * kSkChunk, a1 = min(d.nc, a0 + kSkChunk);
   for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
-    const R4T<R> p = d.xqc[a];
+    const R4T<R> pp = d.xqc[a];
     float sx, sy, sz;
-    frac_coords<R>(b, p, sx, sy, sz);
-    const float q = (float)p.w;
+    frac_coords<R>(b, pp, sx, sy, sz);
+    const float q = (float)pp.w;
 #pragma unroll
     for (int kk = 0; kk < kSkTile; ++kk) {
       const float t = fma_r(mx[kk], sx, fma_r(my[kk], sy, mz[kk] * sz));
@@ -1073,83 +1182,51 @@ __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
     for (int kk = 0; kk < kn; ++kk)
       d.sk_part[(size_t)chunk * d.kcap + k0 + kk] =
           make_double2(v[2 * kk], v[2 * kk + 1]);
-}
-
-// One block of 1024: pair partial sums -> mean force / energies / virial;
-// S(k) partials -> rho, k-space energy and virial, per-k force coefficients.
-template <typename R>
-__global__ __launch_bounds__(1024) void k_finalize(Dev<R> d, Box<R> b,
-                                                   const int* __restrict__ m,
-                                                   int K, int nchunks,
-                                                   int npart, Params p,
-                                                   double e_self) {
-  __shared__ double smem[16 * 11];
-  double v[11];
-  for (int k = 0; k < 11; ++k) v[k] = 0.0;
-  for (int blk = threadIdx.x; blk < npart; blk += blockDim.x)
-    for (int k = 0; k < 11; ++k) v[k] += d.pair_part[11 * blk + k];
-  block_sum<11>(v, smem);
-  double pair[11];
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 11; ++k) pair[k] = v[k];
-
-  double kv[7] = {0, 0, 0, 0, 0, 0, 0};
-  if (p.kspace) {
-    const double V = p.volume;
-    const double two_pi = 6.283185307179586476925;
-    // four threads per k sum interleaved chunks, then combine (fixed order)
-    const int sub = threadIdx.x & 3;
-    for (int kb = threadIdx.x >> 2; kb < ((K + 255) / 256) * 256; kb += blockDim.x >> 2) {
-      const int k = kb;
-      double re = 0.0, im = 0.0;
-      if (k < K) {
-        for (int c = sub; c < nchunks; c += 4) {
-          const double2 s = d.sk_part[(size_t)c * d.kcap + k];
-          re += s.x;
-          im += s.y;
-        }
-      }
-      re += __shfl_xor(re, 1, 64);
-      im += __shfl_xor(im, 1, 64);
-      re += __shfl_xor(re, 2, 64);
-      im += __shfl_xor(im, 2, 64);
-      if (k >= K || sub != 0) continue;
-      d.rho[k] = make_double2(re, im);
-      const double kx = two_pi * m[3 * k] / (double)b.L[0];
-      const double ky = two_pi * m[3 * k + 1] / (double)b.L[1];
-      const double kz = two_pi * m[3 * k + 2] / (double)b.L[2];
-      const double k2 = kx * kx + ky * ky + kz * kz;
-      const double w = d.kw[k];
-      const double A = w * (-4.0 * 3.141592653589793238463 * p.qqr2e) / (V * k2);
-      d.kf0[k] = make_float4((float)m[3 * k], (float)m[3 * k + 1],
-                             (float)m[3 * k + 2], (float)(A * re));
-      d.kf1[k] = make_float4((float)kx, (float)ky, (float)kz, (float)(A * im));
-      const double base = p.qqr2e * two_pi / V * w * (re * re + im * im) / k2;
-      const double cc = 2.0 * (1.0 / (4.0 * p.alpha) + 1.0 / k2);
-      kv[0] += base;
-      kv[1] += base * (1.0 - cc * kx * kx);
-      kv[2] += base * (1.0 - cc * ky * ky);
-      kv[3] += base * (1.0 - cc * kz * kz);
-      kv[4] += base * (-cc * kx * ky);
-      kv[5] += base * (-cc * kx * kz);
-      kv[6] += base * (-cc * ky * kz);
-    }
-    block_sum<7>(kv, smem);
-  }
+  // last block: FP64 reduction over chunks and the k-space energy / virial
+  __shared__ int s_last;
   if (threadIdx.x == 0) {
-    double mean[3] = {0, 0, 0};
-    if (p.mean_fix)
-      for (int k = 0; k < 3; ++k) mean[k] = pair[k] / p.n_global;
-    for (int k = 0; k < 3; ++k) d.mean[k] = mean[k];
-    double* t = d.thermo;
-    t[0] = pair[3];
-    t[1] = pair[4];
-    t[2] = p.kspace ? kv[0] + e_self : 0.0;
-    for (int k = 0; k < 6; ++k) {
-      t[3 + k] = pair[5 + k] + (p.kspace ? kv[1 + k] : 0.0);
-      t[9 + k] = p.kspace ? kv[1 + k] : 0.0;
+    __threadfence();
+    s_last = atomicAdd(&d.ctrl->sk_ticket, 1u) == gridDim.x * gridDim.y - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  const int nchunks = gridDim.x;
+  const double V = p.volume;
+  const double two_pi = 6.283185307179586476925;
+  double kv[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    double sre = 0.0, sim = 0.0;
+    for (int c = 0; c < nchunks; ++c) {
+      const double2 s = d.sk_part[(size_t)c * d.kcap + k];
+      sre += s.x;
+      sim += s.y;
     }
-    for (int k = 0; k < 3; ++k) t[15 + k] = mean[k];
+    d.rho[k] = make_double2(sre, sim);
+    const double kx = two_pi * m[3 * k] / (double)b.L[0];
+    const double ky = two_pi * m[3 * k + 1] / (double)b.L[1];
+    const double kz = two_pi * m[3 * k + 2] / (double)b.L[2];
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double w = d.kw[k];
+    const double A = w * (-4.0 * 3.141592653589793238463 * p.qqr2e) / (V * k2);
+    d.kf0[k] = make_float4((float)m[3 * k], (float)m[3 * k + 1],
+                           (float)m[3 * k + 2], (float)(A * sre));
+    d.kf1[k] = make_float4((float)kx, (float)ky, (float)kz, (float)(A * sim));
+    const double base = p.qqr2e * two_pi / V * w * (sre * sre + sim * sim) / k2;
+    const double cc = 2.0 * (1.0 / (4.0 * p.alpha) + 1.0 / k2);
+    kv[0] += base;
+    kv[1] += base * (1.0 - cc * kx * kx);
+    kv[2] += base * (1.0 - cc * ky * ky);
+    kv[3] += base * (1.0 - cc * kz * kz);
+    kv[4] += base * (-cc * kx * ky);
+    kv[5] += base * (-cc * kx * kz);
+    kv[6] += base * (-cc * ky * kz);
+  }
+  __shared__ double smem7[4 * 7];
+  block_sum<7>(kv, smem7);
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 7; ++q) d.thermo[16 + q] = kv[q];
+    d.ctrl->sk_ticket = 0u;
   }
 }
 
@@ -1369,10 +1446,32 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   if (e != hipSuccess) return e;
   if (a.ev[1]) hipEventRecord(a.ev[1], s);
   const Grid& g = a.grid;
+  // k-space on the second stream, beside the pair kernel
+  const int K = p.kspace ? p.num_k : 0;
+  const bool ks_on = K > 0 && d.nc > 0;
+  hipStream_t ks = a.ks ? a.ks : s;
+  if (ks_on) {
+    if (ks != s) {
+      hipEventRecord(a.ev_fork, s);
+      hipStreamWaitEvent(ks, a.ev_fork, 0);
+    }
+    if (a.ev[3]) hipEventRecord(a.ev[3], ks);
+    const int nchunks = (d.nc + kSkChunk - 1) / kSkChunk;
+    hipLaunchKernelGGL((k_sk<R>), dim3(nchunks, (K + kSkTile - 1) / kSkTile),
+                       dim3(kBlock), 0, ks, d, a.box, a.m_use, K, p);
+    hipLaunchKernelGGL((k_rbe_force<R>), dim3(blocks_for(d.nc, 64)),
+                       dim3(64 * kRbeWaves), 0, ks, d, a.box, K);
+    if (a.ev[4]) hipEventRecord(a.ev[4], ks);
+    if (ks != s) hipEventRecord(a.ev_join, ks);
+  } else if (a.ev[3]) {
+    hipEventRecord(a.ev[3], s);
+    hipEventRecord(a.ev[4], s);
+  }
   const size_t lj_bytes = 2 * sizeof(R) * (size_t)a.pc.ntypes * a.pc.ntypes;
-#define RBMDX_PAIR(RBLv, EFv)                                         \
+#define RBMDX_PAIR(RBLv, EFv)                                                \
   hipLaunchKernelGGL((k_pair<R, RBLv, EFv>), dim3(g.nbrick), dim3(g.block), \
-                     lj_bytes, s, d, a.box, g, a.pc, a.lj_tab, p.split)
+                     lj_bytes, s, d, a.box, g, a.pc, a.lj_tab, p.split,      \
+                     p.mean_fix, p.n_global)
   if (p.rbl) {
     if (p.eflag)
       RBMDX_PAIR(true, true);
@@ -1386,23 +1485,9 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   }
 #undef RBMDX_PAIR
   if (a.ev[2]) hipEventRecord(a.ev[2], s);
-  if (a.ev[3]) hipEventRecord(a.ev[3], s);
-  const int K = p.kspace ? p.num_k : 0;
-  const int nchunks = (d.nc + kSkChunk - 1) / kSkChunk;
-  if (K > 0 && d.nc > 0)
-    hipLaunchKernelGGL((k_sk<R>), dim3(nchunks, (K + kSkTile - 1) / kSkTile),
-                       dim3(kBlock), 0, s, d, a.box, a.m_use, K);
-  Params pf = p;
-  if (d.nc == 0) pf.kspace = 0;
-  hipLaunchKernelGGL((k_finalize<R>), dim3(1), dim3(1024), 0, s, d, a.box,
-                     a.m_use, K, nchunks, g.nbrick, pf, a.e_self);
-  if (K > 0 && d.nc > 0)
-    hipLaunchKernelGGL((k_rbe_force<R>), dim3(blocks_for(d.nc, 64)),
-                       dim3(64 * kRbeWaves), 0, s, d, a.box, K);
-  if (a.ev[4]) hipEventRecord(a.ev[4], s);
+  if (ks_on && ks != s) hipStreamWaitEvent(s, a.ev_join, 0);
   hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,
-                     s, d, p.mean_fix, (K > 0 && d.nc > 0) ? 1 : 0, (R)a.dt,
-                     (R)a.dt_ftm);
+                     s, d, p.mean_fix, ks_on ? 1 : 0, (R)a.dt, (R)a.dt_ftm);
   if (a.ev[5]) hipEventRecord(a.ev[5], s);
   return hipGetLastError();
 }

@@ -330,3 +330,32 @@ def test_fused_leapfrog_matches_separate_update(device):
         np.testing.assert_allclose(fa.cpu().numpy(), fb.cpu().numpy(), rtol=1e-5, atol=1e-5)
     np.testing.assert_allclose(pa.cpu().numpy(), pb.cpu().numpy(), rtol=1e-6, atol=1e-6)
     np.testing.assert_allclose(va.cpu().numpy(), vb.cpu().numpy(), rtol=1e-5, atol=1e-5)
+
+
+def test_kspace_stream_schedules_bitwise_equal(device, small_electrolyte, monkeypatch):
+    """k-space beside the pair kernel on a second stream and k-space in order
+    on one stream run the same kernels with the same reductions: forces,
+    energies and virials agree bit for bit; per-phase timing is reported."""
+    import torch
+
+    s = small_electrolyte
+    cfg = systems.ljsalt_config(P=64)
+    m = rbe_sample_host(0.09, s.box_len, 64, 9, 0)
+    mt = torch.tensor(m.reshape(-1), dtype=torch.int32, device=device)
+    out = []
+    for kst in ("1", "0"):
+        monkeypatch.setenv("RBMDX_KSTREAM", kst)
+        ctx = _ctx(s, cfg)
+        (px, py, pz), f = _tensors(s, device, torch.float32)
+        for step in range(3):
+            ctx.force(step, px, py, pz, *f, m_samples=mt, energy=True, timing=True)
+        kt = ctx.kernel_times()
+        assert set(kt) == {"rebuild", "pair", "kspace", "assemble"}
+        assert kt["pair"] > 0 and kt["kspace"] > 0
+        th = ctx.thermo()
+        out.append((_stack(f), th))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for key in ("e_vdwl", "e_coul", "e_kspace", "virial", "mean_force"):
+        assert out[0][1][key] == out[1][1][key], key
+    ref = reference_force(s, s.pos.astype(np.float32), cfg, 2, m)[0]
+    assert rel_rms(out[0][0], ref) < 2e-5
