@@ -101,14 +101,13 @@ def main():
     kt = ctx.kernel_times()
     th2 = ctx.thermo()
 
-    # dominant kernel: the pair kernel k_pair.  Algorithmic bytes per launch:
-    # 2 B per stored list-A entry (uint16) + 4 B per list-B entry (int32) +
-    # per atom 40 B (16 B position, 4 B count, 4 B id, 16 B force written) +
-    # per charged atom 60 B (16 B position/charge, 4 B type, 4 B id, 4 B
-    # count, 16 B force read and written back); neighbour positions come from
-    # the brick's LDS stage.
-    pairs_a, pairs_b, nch = th2["pairs_a"], th2["pairs_b"], th2["charged"]
-    bytes_pair = 2.0 * pairs_a + 4.0 * pairs_b + 40.0 * n + 60.0 * nch
+    # dominant kernel: the pair kernel k_pair.  Algorithmic bytes per launch,
+    # the compulsory-traffic model of the pair pass: 4 B per pair-list entry
+    # (list A + list B; this build stores list-A entries in 2 B) + 36 B per
+    # atom (16 B position/charge tile, 4 B type, 16 B force); neighbour
+    # positions assumed cache- or LDS-resident.
+    pairs_a, pairs_b = th2["pairs_a"], th2["pairs_b"]
+    bytes_pair = 4.0 * (pairs_a + pairs_b) + 36.0 * n
     t_pair = kt.get("pair", 0.0) * 1e-6
     achieved = bytes_pair / t_pair / 1e9 if t_pair > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -676,24 +676,14 @@ class Engine : public EngineBase {
     // stage with margin, else single cells
     {
       const double apc = (double)n_ / grid_.ncell;
-      const int stage_cap = (32 * 1024) / (int)sizeof(R4);
-      grid_.bw = (64.0 * apc * 1.3 <= stage_cap) ? 2 : 1;
+      const int stage_cap = (24 * 1024) / (int)sizeof(R4);  // the kernels' LDS stage
+      grid_.bw = (64.0 * apc * 1.15 <= stage_cap) ? 2 : 1;
       int nb = 1;
       for (int d = 0; d < 3; ++d) {
         grid_.nb[d] = (grid_.nc[d] + grid_.bw - 1) / grid_.bw;
         nb *= grid_.nb[d];
       }
       grid_.nbrick = nb;
-      // the charged-pair neighbourhood of a brick must fit its slot table
-      if (split_) {
-        int slots = 1;
-        for (int d = 0; d < 3; ++d)
-          slots *= std::min(grid_.nc[d], grid_.bw + 2 * grid_.ext_b[d]);
-        if (slots > kMaxSlotsB)
-          throw Error(RBMDX_E_CONFIG,
-                      "Coulomb cut_off too large relative to the LJ list radius "
-                      "(charged-pair neighbourhood exceeds the brick slot table)");
-      }
       const int per = (int)std::ceil(1.25 * apc * grid_.bw * grid_.bw * grid_.bw);
       grid_.block = std::min(512, std::max(64, (per + 63) / 64 * 64));
       // brick-major cell numbering

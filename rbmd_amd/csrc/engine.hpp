@@ -16,8 +16,6 @@ constexpr int kSkChunk = 1024;   // charged atoms per S(k) block
 constexpr int kRbeWaves = 8;     // waves per RBE-force block (k split)
 constexpr int kMaxK = 1 << 20;   // k-vector capacity bound
 constexpr int kMaxSlots = 64;    // neighbourhood cells staged per brick
-constexpr int kMaxSlotsB = 512;  // charged-pair neighbourhood cells per brick
-constexpr int kStageCapB = 1024; // charged atoms staged per brick
 constexpr int kChargedLanes = 16;  // lanes per charged atom (Coulomb pairs)
 constexpr int kScanPer = 32;     // keys per thread in the one-block scan
 

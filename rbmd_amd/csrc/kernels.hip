@@ -24,6 +24,8 @@
 // k_sk and k_rbe_force run on a second stream beside k_pair.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "engine.hpp"
 
 namespace rbmdx {
@@ -396,9 +398,9 @@ __device__ __forceinline__ void unpack_coord(int packed, int c[3]) {
 // Block 0 also copies the k-sample from host-mapped staging memory.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(1024) void k_skin(Dev<R> d, Box<R> b, int force,
-                                               const int* m_host, int nk3) {
-  __shared__ float s_max[16];
+__global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force,
+                                              const int* m_host, int nk3) {
+  __shared__ float s_max[4];
   if (m_host && blockIdx.x == 0)
     for (int t = threadIdx.x; t < nk3; t += blockDim.x) d.m_dev[t] = m_host[t];
   if (force) {
@@ -587,7 +589,71 @@ __global__ __launch_bounds__(256) void k_gather(Dev<R> d, Box<R> b,
 // All-atom candidate list of one brick: every j != i within r2 among the 27
 // cells around i, except charged-charged pairs when the work is split (those
 // go to list B).  Entries are 16-bit positions in the brick's staged
-// neighbourhood, column-major list[k * n + i].  count_only: no writes.
+// neighbourhood in the interleaved layout of list_a_index; each lane packs 8
+// entries in registers and writes them with one 16-byte store.  count_only
+// (cap 0): no writes.  Neighbourhood positions come from the LDS stage when
+// STAGED, else from the sorted arrays.
+template <typename R, bool STAGED>
+__device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
+                                             const Grid& g, const Neigh& h,
+                                             const SlotTables<kMaxSlots>& t,
+                                             const R4T<R>* stage, R r2,
+                                             int split, int lcap, int cap,
+                                             int& mx, int& sum) {
+  const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    int c[3];
+    unpack_coord(g.coord_of_cell[d.cell_s[i]], c);
+    const Stencil st = atom_stencil(g, h, c);
+    const int os = own_slot(h, c);
+    const int self = t.off[os] + (i - t.first[os]);
+    const bool qi = split && (i - t.first[os]) < t.nq[os];
+    const R4T<R> xi = d.xs[i];
+    int count = 0;
+    unsigned long long lo = 0, hi = 0;  // entries count&~7 .. count&~7 + 7
+    uint4* out = reinterpret_cast<uint4*>(d.list_a) + i;
+    for (int kz = 0; kz < st.n[2]; ++kz)
+      for (int ky = 0; ky < st.n[1]; ++ky)
+        for (int kx = 0; kx < st.n[0]; ++kx) {
+          const int sl =
+              (st.at(2, kz) * h.cnt[1] + st.at(1, ky)) * h.cnt[0] + st.at(0, kx);
+          const int base = t.off[sl];
+          const int len = t.off[sl + 1] - base;
+          const int js = qi ? t.nq[sl] : 0;
+          for (int k = js; k < len; ++k) {
+            const int l = base + k;
+            const R4T<R> xj = STAGED ? stage[l] : d.xs[t.first[sl] + k];
+            R dx, dy, dz;
+            const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx,
+                                    dy, dz);
+            if (d2 < r2 && l != self) {
+              const int s8 = count & 7;
+              const unsigned long long v = (unsigned long long)l;
+              if (s8 < 4)
+                lo |= v << (16 * s8);
+              else
+                hi |= v << (16 * (s8 - 4));
+              if (s8 == 7) {
+                if (count < lcap)
+                  out[(size_t)(count >> 3) * d.n] =
+                      make_uint4((unsigned)lo, (unsigned)(lo >> 32),
+                                 (unsigned)hi, (unsigned)(hi >> 32));
+                lo = hi = 0;
+              }
+              ++count;
+            }
+          }
+        }
+    if ((count & 7) && count <= lcap)
+      out[(size_t)(count >> 3) * d.n] =
+          make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi,
+                     (unsigned)(hi >> 32));
+    d.cnt_a[i] = (lcap == 0 && cap > 0) ? cap + 1 + count : count;
+    mx = max(mx, count);
+    sum += count;
+  }
+}
+
 template <typename R>
 __global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
                                                 R r2, int split, int cap,
@@ -601,44 +667,13 @@ __global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
   const int total = t.off[h.nslots];
   const bool staged = total <= StageCap<R>::value;
   if (staged) stage_atoms<R>(h, t, d.xs, stage);
-  const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
+  // neighbourhoods beyond 16-bit positions are left to the direct scan
+  const int lcap = total > 65535 ? 0 : cap;
   int mx = 0, sum = 0;
-  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    int c[3];
-    unpack_coord(g.coord_of_cell[d.cell_s[i]], c);
-    const Stencil st = atom_stencil(g, h, c);
-    const int os = own_slot(h, c);
-    const int self = t.off[os] + (i - t.first[os]);
-    const bool qi = split && (i - t.first[os]) < t.nq[os];
-    const R4T<R> xi = d.xs[i];
-    int count = 0;
-    // neighbourhoods beyond 16-bit positions are left to the direct scan
-    const int lcap = total > 65535 ? 0 : cap;
-    for (int kz = 0; kz < st.n[2]; ++kz)
-      for (int ky = 0; ky < st.n[1]; ++ky)
-        for (int kx = 0; kx < st.n[0]; ++kx) {
-          const int sl =
-              (st.at(2, kz) * h.cnt[1] + st.at(1, ky)) * h.cnt[0] + st.at(0, kx);
-          const int base = t.off[sl];
-          const int len = t.off[sl + 1] - base;
-          const int js = qi ? t.nq[sl] : 0;
-          for (int k = js; k < len; ++k) {
-            const int l = base + k;
-            if (l == self) continue;
-            const R4T<R> xj = staged ? stage[l] : d.xs[t.first[sl] + k];
-            R dx, dy, dz;
-            const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx,
-                                    dy, dz);
-            if (d2 < r2) {
-              if (count < lcap) d.list_a[list_a_index(count, i, d.n)] = (uint16_t)l;
-              ++count;
-            }
-          }
-        }
-    d.cnt_a[i] = (lcap == 0 && cap > 0) ? cap + 1 + count : count;
-    mx = max(mx, count);
-    sum += count;
-  }
+  if (staged)
+    list_a_atoms<R, true>(d, b, g, h, t, stage, r2, split, lcap, cap, mx, sum);
+  else
+    list_a_atoms<R, false>(d, b, g, h, t, stage, r2, split, lcap, cap, mx, sum);
   mx = wave_max_i(mx);
   sum = wave_sum(sum);
   if ((threadIdx.x & 63) == 0 && mx > 0) {
@@ -651,36 +686,27 @@ __global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
   }
 }
 
-// Charged-pair candidate list of one brick: the charged atoms of the brick's
-// ext_b neighbourhood are staged in LDS; kChargedLanes lanes per charged atom
-// split its stencil cells, count, prefix-sum and write in a fixed order.
-// Row-major list_b[c * cap + k] of charged compact indices.
+// Charged-pair candidate list: kChargedLanes lanes per charged atom scan the
+// charged atoms of the cells within ext_b around it, read from the compact
+// charged array (20 B per charged atom, cache-resident), count, prefix-sum
+// over the group and write in a fixed order (lane-major over the lane's
+// cells).  Row-major list_b[c * cap + k] of charged compact indices.
 template <typename R>
-__global__ __launch_bounds__(512) void k_list_b(Dev<R> d, Box<R> b, Grid g,
+__global__ __launch_bounds__(256) void k_list_b(Dev<R> d, Box<R> b, Grid g,
                                                 R r2, int cap, unsigned thr) {
   if (!rebuild_now(d.ctrl, thr)) return;
-  __shared__ R4T<R> stage[kStageCapB];
-  __shared__ SlotTables<kMaxSlotsB> t;
-  const Neigh h = brick_neigh(g, blockIdx.x, g.ext_b);
-  const int c0 = d.cstart[h.cell0], c1 = d.cstart[h.cell1];
-  if (c0 == c1) return;
-  build_slots(g, h, d.key_start, d.cstart, true, t);
-  const int total = t.off[h.nslots];
-  const bool staged = total <= kStageCapB;
-  if (staged) stage_atoms<R>(h, t, d.xs, stage);
   const int grp = threadIdx.x / kChargedLanes, gl = threadIdx.x % kChargedLanes;
-  const int ngrp = blockDim.x / kChargedLanes;
-  int mx = 0;
-  unsigned long long sum = 0;
-  for (int c = c0 + grp; c < c1; c += ngrp) {
+  const int c = blockIdx.x * (blockDim.x / kChargedLanes) + grp;
+  int tot = 0;
+  if (c < d.nc) {
     const int i = d.c2s[c];
-    const R4T<R> xi = d.xs[i];
+    const R4T<R> xi = d.xqc[c];
     int cc[3];
     unpack_coord(g.coord_of_cell[d.cell_s[i]], cc);
-    const Stencil st = atom_stencil(g, h, cc);
-    const int os = own_slot(h, cc);
-    const int self = t.off[os] + (i - t.first[os]);
-    const int nst = st.count();
+    const Span sx = make_span(cc[0], g.ext_b[0], g.nc[0]);
+    const Span sy = make_span(cc[1], g.ext_b[1], g.nc[1]);
+    const Span sz = make_span(cc[2], g.ext_b[2], g.nc[2]);
+    const int nst = sx.count * sy.count * sz.count;
     int count = 0, pos = 0;
     for (int pass = 0; pass < 2; ++pass) {
       if (pass == 1) {
@@ -692,37 +718,34 @@ __global__ __launch_bounds__(512) void k_list_b(Dev<R> d, Box<R> b, Grid g,
         pos = inc - count;
       }
       for (int s = gl; s < nst; s += kChargedLanes) {
-        const int kx = s % st.n[0], ky = (s / st.n[0]) % st.n[1],
-                  kz = s / (st.n[0] * st.n[1]);
-        const int sl = (st.at(2, kz) * h.cnt[1] + st.at(1, ky)) * h.cnt[0] +
-                       st.at(0, kx);
-        const int base = t.off[sl], len = t.off[sl + 1] - base;
-        for (int k = 0; k < len; ++k) {
-          const int l = base + k;
-          if (l == self) continue;
-          const R4T<R> xj = staged ? stage[l] : d.xs[t.first[sl] + k];
+        const int kx = s % sx.count, ky = (s / sx.count) % sy.count,
+                  kz = s / (sx.count * sy.count);
+        const int cell =
+            g.cell_of_coord[(sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0] + sx.at(kx)];
+        const int j0 = d.cstart[cell], j1 = d.cstart[cell + 1];
+        for (int j = j0; j < j1; ++j) {
+          const R4T<R> xj = d.xqc[j];
           R dx, dy, dz;
           const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
-          if (d2 < r2) {
+          if (d2 < r2 && j != c) {
             if (pass == 0) {
               ++count;
             } else {
-              if (pos < cap) d.list_b[(size_t)c * cap + pos] = t.cfirst[sl] + k;
+              if (pos < cap) d.list_b[(size_t)c * cap + pos] = j;
               ++pos;
             }
           }
         }
       }
     }
-    int tot = count;
+    tot = count;
     for (int o = kChargedLanes / 2; o > 0; o >>= 1)
       tot += __shfl_xor(tot, o, kChargedLanes);
     if (gl == 0) d.cnt_b[c] = tot;
-    mx = max(mx, tot);
-    sum += (gl == 0) ? (unsigned long long)tot : 0ull;
   }
-  mx = wave_max_i(mx);
-  sum = wave_sum(sum);
+  const int mx = wave_max_i(tot);
+  const unsigned long long sum =
+      wave_sum((gl == 0) ? (unsigned long long)tot : 0ull);
   if ((threadIdx.x & 63) == 0 && mx > 0) {
     atomicMax(&d.ctrl->max_count_b, mx);
     atomicAdd(&d.ctrl->sum_count_b, sum);
@@ -1421,8 +1444,10 @@ hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   hipLaunchKernelGGL((k_list_a<R>), dim3(g.nbrick), dim3(g.block), 0, s, d,
                      a.box, g, a.pc.r2_list_a, a.p.split, cap_a, thr);
   if (a.p.split && d.nc > 0)
-    hipLaunchKernelGGL((k_list_b<R>), dim3(g.nbrick), dim3(g.block), 0, s, d,
-                       a.box, g, a.pc.r2_list_b, cap_b, thr);
+    hipLaunchKernelGGL((k_list_b<R>),
+                       dim3(blocks_for(d.nc, kBlock / kChargedLanes)),
+                       dim3(kBlock), 0, s, d, a.box, g, a.pc.r2_list_b, cap_b,
+                       thr);
   return hipGetLastError();
 }
 
@@ -1438,9 +1463,11 @@ hipError_t launch_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
 template <typename R>
 hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   const Params& p = a.p;
-  const int sb = blocks_for(d.n, 1024, 64);
+  // one atom per thread for the first 2048 * 256 atoms (HBM-bound read of
+  // positions and reference positions), one atomic per block
+  const int sb = blocks_for(d.n, kBlock, 2048);
   if (a.ev[0]) hipEventRecord(a.ev[0], s);
-  hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(1024), 0, s, d, a.box,
+  hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(kBlock), 0, s, d, a.box,
                      a.force_rebuild, a.m_host, a.m_host ? 3 * p.num_k : 0);
   hipError_t e = build_part<R>(d, a, s);
   if (e != hipSuccess) return e;
