@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--dt", type=float, default=0.002)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB three times (a "
+                         "plain copy kernel of known size, to scale HBM counters)")
     return ap.parse_args()
 
 
@@ -115,6 +118,16 @@ def main():
                 "kernel": "k_pair", "kernel_us": round(kt.get("pair", 0.0), 2),
                 "algorithmic_bytes": int(bytes_pair), "pairs_a": int(pairs_a),
                 "pairs_b": int(pairs_b)}
+
+    if args.calib_copy_mb > 0:
+        src = torch.empty(args.calib_copy_mb << 18, dtype=torch.float32, device=dev)
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # force error vs FP64 and the CPU baseline (oracle restatement, one core)
     cpu = None

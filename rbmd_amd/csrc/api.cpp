@@ -750,6 +750,8 @@ class Engine : public EngineBase {
       pc_.two_g_over_sqrtpi = (R)(2.0 * std::sqrt(alpha_) / std::sqrt(M_PI));
       pc_.freq = (R)freq_;
       pc_.ifreq = freq_;
+      pc_.fmask = (freq_ >= 2 && (freq_ & (freq_ - 1)) == 0) ? (unsigned)(freq_ - 1) : 0u;
+      pc_.c32 = freq_ >= 2 ? (unsigned)((1ULL << 32) % (unsigned long long)freq_) : 0u;
       pc_.ntypes = ntypes_;
     }
     params_.rbl = c.neighbor_type == "RBL" ? 1 : 0;
@@ -783,11 +785,13 @@ class Engine : public EngineBase {
     d_.perm = alloc<int>(n_);
     d_.cell_s = alloc<int>(n_);
     d_.gid_s = alloc<int>(n_);
+    d_.hj_s = alloc<unsigned long long>(n_);
     d_.s2c = alloc<int>(n_);
     d_.xs = alloc<R4>(n_);
     d_.fa = alloc<R4>(n_);
     d_.c2s = alloc<int>(nc);
     d_.gidc = alloc<int>(nc);
+    d_.hjc = alloc<unsigned long long>(nc);
     d_.typec = alloc<int>(nc);
     d_.xqc = alloc<R4>(nc);
     d_.fk = alloc<R4>(nc);
@@ -801,6 +805,9 @@ class Engine : public EngineBase {
     }
     d_.pair_part_blocks = grid_.nbrick;
     d_.pair_part = alloc<double>((size_t)11 * grid_.nbrick);
+    d_.skin_part = alloc<float>(kSkinBlocksMax);
+    d_.lista_part = alloc<int4>(grid_.nbrick);
+    d_.listb_part = alloc<int2>(std::max(1, list_b_blocks(nc)));
     d_.thermo = alloc<double>(32);
     d_.mean = alloc<double>(3);
     d_.ctrl = alloc<Ctrl>(1);

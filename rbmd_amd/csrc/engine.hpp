@@ -10,6 +10,7 @@
 namespace rbmdx {
 
 constexpr int kBlock = 256;      // threads per block for per-atom kernels
+constexpr int kSkinBlocksMax = 256;  // skin-check grid (grid-stride beyond)
 constexpr int kMaxTypes = 32;    // LJ type table held in LDS
 constexpr int kSkTile = 8;       // k-vectors per S(k) block
 constexpr int kSkChunk = 1024;   // charged atoms per S(k) block
@@ -31,9 +32,13 @@ struct Ctrl {
   unsigned long long sum_count_b;
   int max_stage;              // largest brick neighbourhood seen
   int unstaged_bricks;        // bricks whose neighbourhood exceeded LDS
-  unsigned int pair_ticket;   // blocks of k_pair done (last block reduces)
-  unsigned int sk_ticket;     // blocks of k_sk done (last block reduces)
 };
+
+// Blocks of the charged-pair list build (kChargedLanes lanes per atom).
+inline int list_b_blocks(int nc) {
+  return (nc + kBlock / kChargedLanes - 1) / (kBlock / kChargedLanes);
+}
+
 
 template <typename R>
 struct Box {
@@ -70,6 +75,8 @@ struct PairConst {
   R alpha, g, qqr2e, two_g_over_sqrtpi;
   R freq;             // RBL selection frequency as a multiplier
   int ifreq;
+  unsigned fmask;     // ifreq - 1 when ifreq is a power of two, else 0
+  unsigned c32;       // 2^32 mod ifreq
   long long epoch;
   int ntypes;
 };
@@ -109,6 +116,8 @@ struct Dev {
   int* perm = nullptr;     // [n] sorted -> caller
   int* cell_s = nullptr;   // [n]
   int* gid_s = nullptr;    // [n]
+  unsigned long long* hj_s = nullptr;  // [n] mix64(gid + 0xd1b54a32d192ed03)
+  unsigned long long* hjc = nullptr;   // [nc] the same, charged order
   int* s2c = nullptr;      // [n] sorted -> charged compact, -1 if neutral
   R4* xs = nullptr;        // [n] (x, y, z, type)
   R4* fa = nullptr;        // [n] total pair force
@@ -125,6 +134,9 @@ struct Dev {
   int* list_b = nullptr;       // [nc * cap_b] row-major, charged compact index
   int* cnt_b = nullptr;
   // pair partial sums per block: 3 force + 2 energy + 6 virial
+  float* skin_part = nullptr;   // [skin blocks] per-block max |dr|^2
+  int4* lista_part = nullptr;   // [nbrick] (max count, sum, stage size, unstaged)
+  int2* listb_part = nullptr;   // [list-B blocks] (max count, sum)
   double* pair_part = nullptr;
   int pair_part_blocks = 0;
   // k-space

@@ -112,14 +112,37 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long v) {
   return v ^ (v >> 31);
 }
 
-// Directed RBL shell selection for sampling epoch `epoch`.
-__device__ __forceinline__ bool rbl_selected(int f, long long epoch, int gi,
-                                             int gj) {
+// Directed RBL shell selection for sampling epoch `epoch`: with
+// hi = mix64(gid_i) and hj = mix64(gid_j + 0xd1b54a32d192ed03), the pair is
+// kept iff mix64(epoch ^ hi ^ hj) mod f == 0.  hj is precomputed per atom at
+// rebuild.  The remainder is exact: a mask for powers of two, else
+// h mod f = ((h_hi mod f)(2^32 mod f) + h_lo mod f) mod f in 32-bit
+// arithmetic (f < 2^16), else the 64-bit remainder.
+__device__ __forceinline__ unsigned long long rbl_hi(int gi) {
+  return mix64((unsigned long long)(long long)gi);
+}
+__device__ __forceinline__ unsigned long long rbl_hj(int gj) {
+  return mix64((unsigned long long)(long long)gj + 0xd1b54a32d192ed03ULL);
+}
+__device__ __forceinline__ bool rbl_keep(int f, unsigned fmask, unsigned c32,
+                                         long long epoch,
+                                         unsigned long long hi,
+                                         unsigned long long hj) {
   if (f <= 1) return true;
-  unsigned long long key = (unsigned long long)epoch;
-  key ^= mix64((unsigned long long)(long long)gi);
-  key ^= mix64((unsigned long long)(long long)gj + 0xd1b54a32d192ed03ULL);
-  return mix64(key) % (unsigned long long)f == 0ULL;
+  const unsigned long long h = mix64((unsigned long long)epoch ^ hi ^ hj);
+  if (fmask) return ((unsigned)h & fmask) == 0u;
+  const unsigned uf = (unsigned)f;
+  if (uf < 65536u) {
+    const unsigned r1 = (unsigned)(h >> 32) % uf;
+    unsigned r = (r1 * c32) % uf + (unsigned)h % uf;
+    if (r >= uf) r -= uf;
+    return r == 0u;
+  }
+  return h % (unsigned long long)f == 0ULL;
+}
+template <typename PC>
+__device__ __forceinline__ bool rbl_selected(const PC& pc, int gi, int gj) {
+  return rbl_keep(pc.ifreq, pc.fmask, pc.c32, pc.epoch, rbl_hi(gi), rbl_hj(gj));
 }
 
 __device__ __forceinline__ bool rebuild_now(const Ctrl* c, unsigned thr) {
@@ -393,33 +416,29 @@ __device__ __forceinline__ void unpack_coord(int packed, int c[3]) {
 }
 
 // ---------------------------------------------------------------------------
-// Rebuild decision.  Max |r - r_ref|^2 (minimum image), as float bits in
-// ctrl->maxdisp_bits; +inf when a rebuild is forced.  One atomic per block.
-// Block 0 also copies the k-sample from host-mapped staging memory.
+// Rebuild decision, part 1: per-block max |r - r_ref|^2 (minimum image);
+// +inf when a rebuild is forced.  k_bin folds the block maxima (the kernel
+// boundary makes them visible to every block) into ctrl->maxdisp_bits.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force,
-                                              const int* m_host, int nk3) {
+__global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force) {
   __shared__ float s_max[4];
-  if (m_host && blockIdx.x == 0)
-    for (int t = threadIdx.x; t < nk3; t += blockDim.x) d.m_dev[t] = m_host[t];
-  if (force) {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-      atomicMax(&d.ctrl->maxdisp_bits, 0x7f800000u);
-    return;
-  }
   float mx = 0.f;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
-       i += gridDim.x * blockDim.x) {
-    const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
-    const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
-    const R z = wrap<R>(d.pz[i], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
-    const R4T<R> r = d.ref[i];
-    const R dx = min_image<R>(x - r.x, b.L[0], b.half[0]);
-    const R dy = min_image<R>(y - r.y, b.L[1], b.half[1]);
-    const R dz = min_image<R>(z - r.z, b.L[2], b.half[2]);
-    const float d2 = (float)(dx * dx + dy * dy + dz * dz);
-    mx = fmaxf(mx, d2 != d2 ? __int_as_float(0x7f800000) : d2);
+  if (!force) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
+         i += gridDim.x * blockDim.x) {
+      const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
+      const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
+      const R z = wrap<R>(d.pz[i], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
+      const R4T<R> r = d.ref[i];
+      const R dx = min_image<R>(x - r.x, b.L[0], b.half[0]);
+      const R dy = min_image<R>(y - r.y, b.L[1], b.half[1]);
+      const R dz = min_image<R>(z - r.z, b.L[2], b.half[2]);
+      const float d2 = (float)(dx * dx + dy * dy + dz * dz);
+      mx = fmaxf(mx, d2 != d2 ? __int_as_float(0x7f800000) : d2);
+    }
+  } else {
+    mx = __int_as_float(0x7f800000);
   }
   mx = wave_max(mx);
   if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = mx;
@@ -427,7 +446,7 @@ __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force,
   if (threadIdx.x == 0) {
     float m = s_max[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, s_max[w]);
-    if (m > 0.f) atomicMax(&d.ctrl->maxdisp_bits, __float_as_uint(m));
+    d.skin_part[blockIdx.x] = m;
   }
 }
 
@@ -436,8 +455,20 @@ __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force,
 // in-cell order is made deterministic by k_cellsort.
 template <typename R>
 __global__ __launch_bounds__(256) void k_bin(Dev<R> d, Box<R> b, Grid g,
-                                             int split, unsigned thr) {
-  if (!rebuild_now(d.ctrl, thr)) return;
+                                             int split, unsigned thr,
+                                             int nskin) {
+  // rebuild decision, part 2: every block folds the skin-check maxima
+  __shared__ float s_max[4];
+  float m = 0.f;
+  for (int k = threadIdx.x; k < nskin; k += blockDim.x)
+    m = fmaxf(m, d.skin_part[k]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+  __syncthreads();
+  m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    d.ctrl->maxdisp_bits = __float_as_uint(m);
+  if (!(__float_as_uint(m) > thr)) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
        i += gridDim.x * blockDim.x) {
     const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
@@ -463,11 +494,7 @@ __global__ __launch_bounds__(1024) void k_scan(int* key_count, int* key_start,
   if (!rebuild_now(ctrl, thr)) return;
   __shared__ int smem[33];
   const int t = threadIdx.x;
-  if (t == 0) {
-    ctrl->rebuild_count += 1;
-    ctrl->sum_count_a = 0ull;
-    ctrl->sum_count_b = 0ull;
-  }
+  if (t == 0) ctrl->rebuild_count += 1;
   const int chunk = 1024 * kScanPer;
   for (int pass = 0; pass < 2; ++pass) {
     const int n = pass == 0 ? 2 * ncell : ncell;
@@ -567,12 +594,15 @@ __global__ __launch_bounds__(256) void k_gather(Dev<R> d, Box<R> b,
     if (rb) {
       d.ref[i] = make4<R>(x, y, z, R(0));
       d.gid_s[s] = d.gid[i];
+      const unsigned long long hj = rbl_hj(d.gid[i]);
+      d.hj_s[s] = hj;
       const int key = d.key[i];
       d.cell_s[s] = key >> 1;
       if (!(key & 1)) {
         c = d.cstart[key >> 1] + (s - d.key_start[key]);
         d.c2s[c] = s;
         d.gidc[c] = d.gid[i];
+        d.hjc[c] = hj;
         d.typec[c] = t;
       } else {
         c = -1;
@@ -674,15 +704,23 @@ __global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
     list_a_atoms<R, true>(d, b, g, h, t, stage, r2, split, lcap, cap, mx, sum);
   else
     list_a_atoms<R, false>(d, b, g, h, t, stage, r2, split, lcap, cap, mx, sum);
+  // per-block statistics (folded by k_reduce)
+  __shared__ int s_red[2 * 8];
   mx = wave_max_i(mx);
   sum = wave_sum(sum);
-  if ((threadIdx.x & 63) == 0 && mx > 0) {
-    atomicMax(&d.ctrl->max_count_a, mx);
-    atomicAdd(&d.ctrl->sum_count_a, (unsigned long long)sum);
+  const int nw = (int)(blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_red[2 * (threadIdx.x >> 6)] = mx;
+    s_red[2 * (threadIdx.x >> 6) + 1] = sum;
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    atomicMax(&d.ctrl->max_stage, total);
-    if (!staged) atomicAdd(&d.ctrl->unstaged_bricks, 1);
+    int m = 0, su = 0;
+    for (int w = 0; w < nw; ++w) {
+      m = max(m, s_red[2 * w]);
+      su += s_red[2 * w + 1];
+    }
+    d.lista_part[blockIdx.x] = make_int4(m, su, total, staged ? 0 : 1);
   }
 }
 
@@ -743,12 +781,21 @@ __global__ __launch_bounds__(256) void k_list_b(Dev<R> d, Box<R> b, Grid g,
       tot += __shfl_xor(tot, o, kChargedLanes);
     if (gl == 0) d.cnt_b[c] = tot;
   }
-  const int mx = wave_max_i(tot);
-  const unsigned long long sum =
-      wave_sum((gl == 0) ? (unsigned long long)tot : 0ull);
-  if ((threadIdx.x & 63) == 0 && mx > 0) {
-    atomicMax(&d.ctrl->max_count_b, mx);
-    atomicAdd(&d.ctrl->sum_count_b, sum);
+  // per-block statistics (folded by k_reduce)
+  __shared__ int s_red[2 * 4];
+  int mx = wave_max_i(tot);
+  int sum = wave_sum((gl == 0) ? tot : 0);
+  if ((threadIdx.x & 63) == 0) {
+    s_red[2 * (threadIdx.x >> 6)] = mx;
+    s_red[2 * (threadIdx.x >> 6) + 1] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      mx = max(mx, s_red[2 * w]);
+      sum += s_red[2 * w + 1];
+    }
+    d.listb_part[blockIdx.x] = make_int2(mx, sum);
   }
 }
 
@@ -756,13 +803,13 @@ __global__ __launch_bounds__(256) void k_list_b(Dev<R> d, Box<R> b, Grid g,
 // Pair interaction of one directed pair.  Weight applied to the force (RBL:
 // 1 core, f selected shell, 0 otherwise; Verlet: 1 inside rc).
 // ---------------------------------------------------------------------------
-template <typename R, bool RBL, typename GidJ>
-__device__ __forceinline__ R pair_weight(const PairConst<R>& pc, R d2, int gi,
-                                         GidJ gid_j) {
+template <typename R, bool RBL, typename HashJ>
+__device__ __forceinline__ R pair_weight(const PairConst<R>& pc, R d2,
+                                         unsigned long long hi, HashJ hash_j) {
   if (!RBL) return R(1);
   if (d2 < pc.t_core) return (d2 < pc.rs2 && d2 > pc.eps_min) ? R(1) : R(0);
   if (d2 < pc.t_shell && d2 > pc.rs2 &&
-      rbl_selected(pc.ifreq, pc.epoch, gi, gid_j()))
+      rbl_keep(pc.ifreq, pc.fmask, pc.c32, pc.epoch, hi, hash_j()))
     return pc.freq;
   return R(0);
 }
@@ -786,13 +833,14 @@ __device__ __forceinline__ void add_virial(Acc<R>& a, R dx, R dy, R dz, R f) {
   }
 }
 
-// LJ-only pair of list A.  gidj() yields the global id of j (evaluated only
+// LJ-only pair of list A.  hj() yields j's selection hash (evaluated only
 // for RBL shell pairs).
 template <typename R, bool RBL, bool EFLAG, typename GidJ>
 __device__ __forceinline__ void interact_a(const PairConst<R>& pc,
                                            const Box<R>& b, const R* s_lj,
-                                           int ti, const R4T<R>& xi, int gi,
-                                           const R4T<R>& xj, GidJ gidj,
+                                           int ti, const R4T<R>& xi,
+                                           unsigned long long hi,
+                                           const R4T<R>& xj, GidJ hj,
                                            Acc<R>& a) {
   R dx, dy, dz;
   const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
@@ -804,7 +852,7 @@ __device__ __forceinline__ void interact_a(const PairConst<R>& pc,
   const R s2 = sig2 * inv;
   const R s6 = s2 * s2 * s2;
   const R flj = R(-24) * eps * (R(2) * s6 - R(1)) * s6 * inv;
-  const R w = pair_weight<R, RBL>(pc, d2, gi, gidj);
+  const R w = pair_weight<R, RBL>(pc, d2, hi, hj);
   const R f = w * flj;
   a.fx += f * dx;
   a.fy += f * dy;
@@ -830,9 +878,11 @@ __device__ __forceinline__ void coul(const PairConst<R>& pc, R d2, R qq, R& f,
 template <typename R, bool RBL, bool EFLAG>
 __device__ __forceinline__ void interact_b(const PairConst<R>& pc,
                                            const Box<R>& b, const R* s_lj,
-                                           int ti, const R4T<R>& xi, int gi,
+                                           int ti, const R4T<R>& xi,
+                                           unsigned long long hi,
                                            const R4T<R>& xj, int tj,
-                                           const int* gidj, Acc<R>& a) {
+                                           const unsigned long long* hjp,
+                                           Acc<R>& a) {
   R dx, dy, dz;
   const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
   if (!(d2 < pc.rc2)) return;
@@ -848,7 +898,7 @@ __device__ __forceinline__ void interact_b(const PairConst<R>& pc,
   }
   R fc, ec;
   coul<R>(pc, d2, xi.w * xj.w, fc, ec);
-  const R w = pair_weight<R, RBL>(pc, d2, gi, [&] { return *gidj; });
+  const R w = pair_weight<R, RBL>(pc, d2, hi, [&] { return *hjp; });
   const R f = w * (flj + fc);
   a.fx += f * dx;
   a.fy += f * dy;
@@ -888,12 +938,13 @@ struct R2Of<double> {
 // positions are fetched together (from the LDS stage when STAGED, else
 // through the slot tables from the sorted arrays), then evaluated without
 // branches: entries past the count and pairs outside the cutoffs are masked
-// by selects.  Only RBL shell pairs, rare in list A, branch to the hash.
-template <typename R, bool RBL, bool EFLAG, bool STAGED>
+// by selects.  Only RBL shell pairs branch to the hash; SHELL is false when
+// the LJ cutoff does not exceed r_core, so that list A holds no shell pairs.
+template <typename R, bool RBL, bool EFLAG, bool SHELL, bool STAGED>
 __device__ __forceinline__ void list_a_atom(
     const Dev<R>& d, const Box<R>& b, const PairConst<R>& pc, const R* s_lj,
     const SlotTables<kMaxSlots>& t, int nslots, const R4T<R>* stage, int i,
-    int cnt, const R4T<R>& xi, int ti, int gi, Acc<R>& a) {
+    int cnt, const R4T<R>& xi, int ti, unsigned long long hi, Acc<R>& a) {
   using R2 = typename R2Of<R>::type;
   const uint4* lp = reinterpret_cast<const uint4*>(d.list_a) + i;
   const int tbase = 2 * ti * pc.ntypes;
@@ -905,10 +956,12 @@ __device__ __forceinline__ void list_a_atom(
     const uint4 q8 = lp[(size_t)(k0 >> 3) * d.n];
     const int kn = cnt - k0;
     const unsigned wv[4] = {q8.x, q8.y, q8.z, q8.w};
+#pragma unroll
+    for (int half = 0; half < 8; half += 4) {
     int lj_[8];
     R4T<R> xj[8];
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
+    for (int kk = half; kk < half + 4; ++kk) {
       const int l = (int)((wv[kk >> 1] >> (16 * (kk & 1))) & 0xffffu);
       if (STAGED) {
         lj_[kk] = kk < kn ? l : 0;  // stage[0] exists whenever i does
@@ -919,7 +972,7 @@ __device__ __forceinline__ void list_a_atom(
       }
     }
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
+    for (int kk = half; kk < half + 4; ++kk) {
       const R dx = min_image_sel<R>(xj[kk].x - xi.x, b.L[0], b.half[0]);
       const R dy = min_image_sel<R>(xj[kk].y - xi.y, b.L[1], b.half[1]);
       const R dz = min_image_sel<R>(xj[kk].z - xi.z, b.L[2], b.half[2]);
@@ -934,9 +987,10 @@ __device__ __forceinline__ void list_a_atom(
       if (RBL) {
         const bool core = d2 < pc.t_core;
         w = (core && d2 < pc.rs2 && d2 > pc.eps_min) ? R(1) : R(0);
-        if (in && !core && d2 < pc.t_shell && d2 > pc.rs2) {
+        if (SHELL && in && !core && d2 < pc.t_shell && d2 > pc.rs2) {
           const int jg = STAGED ? global_of(lj_[kk]) : lj_[kk];
-          if (rbl_selected(pc.ifreq, pc.epoch, gi, d.gid_s[jg])) w = pc.freq;
+          if (rbl_keep(pc.ifreq, pc.fmask, pc.c32, pc.epoch, hi, d.hj_s[jg]))
+            w = pc.freq;
         }
       }
       const R f = in ? w * flj : R(0);
@@ -947,6 +1001,7 @@ __device__ __forceinline__ void list_a_atom(
         a.e_lj += in ? (double)(R(2) * lj.x * (s6 - R(1)) * s6) : 0.0;
         add_virial<R, EFLAG>(a, dx, dy, dz, in ? flj : R(0));
       }
+    }
     }
   }
 }
@@ -977,15 +1032,23 @@ __device__ __forceinline__ void write_partials(const Acc<R>& a, double* part,
   }
 }
 
+// Occupancy target of the pair kernel: 6 waves per SIMD for the FP32 force
+// pass without energies or list-A shell pairs (the per-step hot case), so
+// that every brick of a 200k-atom system is resident at once.
+template <typename R, bool EFLAG, bool SHELL>
+struct kPairWaves {
+  static constexpr int value = (sizeof(R) == 4 && !EFLAG && !SHELL) ? 6 : 1;
+};
+
 // Pair forces of one brick.  Phase 1: LJ over list A for every atom of the
 // brick (one lane per atom, neighbours from the LDS stage).  Phase 2: the
 // brick's charged atoms add LJ + Coulomb over list B, kChargedLanes lanes per
 // atom.  fa[i] receives the total pair force of atom i.
-template <typename R, bool RBL, bool EFLAG>
-__global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
+template <typename R, bool RBL, bool EFLAG, bool SHELL>
+__global__ __launch_bounds__(512)
+__attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_pair(Dev<R> d, Box<R> b, Grid g,
                                               PairConst<R> pc, const R* lj_tab,
-                                              int split, int mean_fix,
-                                              double n_global) {
+                                              int split) {
   __shared__ R4T<R> stage[StageCap<R>::value];
   __shared__ SlotTables<kMaxSlots> t;
   __shared__ double smem[8 * 11];
@@ -1005,15 +1068,15 @@ __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
     Acc<R> a;
     const R4T<R> xi = d.xs[i];
     const int ti = (int)xi.w;
-    const int gi = d.gid_s[i];
+    const unsigned long long hi = RBL ? rbl_hi(d.gid_s[i]) : 0ull;
     const int cnt = d.cnt_a[i];
     if (cnt <= d.cap_a) {
       if (staged)
-        list_a_atom<R, RBL, EFLAG, true>(d, b, pc, s_lj, t, h.nslots, stage, i,
-                                         cnt, xi, ti, gi, a);
+        list_a_atom<R, RBL, EFLAG, SHELL, true>(d, b, pc, s_lj, t, h.nslots, stage, i,
+                                         cnt, xi, ti, hi, a);
       else
-        list_a_atom<R, RBL, EFLAG, false>(d, b, pc, s_lj, t, h.nslots, stage,
-                                          i, cnt, xi, ti, gi, a);
+        list_a_atom<R, RBL, EFLAG, SHELL, false>(d, b, pc, s_lj, t, h.nslots, stage,
+                                          i, cnt, xi, ti, hi, a);
     } else {
       // list overflow since the last rebuild: scan the 27 cells directly
       int c[3];
@@ -1030,8 +1093,8 @@ __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
             for (int k = qi ? t.nq[sl] : 0; k < len; ++k) {
               const int jg = t.first[sl] + k;
               if (jg == i) continue;
-              interact_a<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, d.xs[jg],
-                                        [&] { return d.gid_s[jg]; }, a);
+              interact_a<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, hi, d.xs[jg],
+                                        [&] { return d.hj_s[jg]; }, a);
             }
           }
     }
@@ -1053,14 +1116,15 @@ __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
     for (int c = c0 + grp; c < c1; c += ngrp) {
       Acc<R> a;
       const R4T<R> xi = d.xqc[c];
-      const int ti = d.typec[c], gi = d.gidc[c];
+      const int ti = d.typec[c];
+      const unsigned long long hi = RBL ? rbl_hi(d.gidc[c]) : 0ull;
       const int cnt = d.cnt_b[c];
       if (cnt <= d.cap_b) {
         const int* lp = d.list_b + (size_t)c * d.cap_b;
         for (int k = gl; k < cnt; k += kChargedLanes) {
           const int e = lp[k];
-          interact_b<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, d.xqc[e], d.typec[e],
-                                    d.gidc + e, a);
+          interact_b<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, hi, d.xqc[e], d.typec[e],
+                                    d.hjc + e, a);
         }
       } else {
         const int i = d.c2s[c];
@@ -1079,8 +1143,8 @@ __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
           for (int j = js; j < je; ++j) {
             if (j == i) continue;
             const int e = d.s2c[j];
-            interact_b<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, gi, d.xqc[e],
-                                      d.typec[e], d.gidc + e, a);
+            interact_b<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, hi, d.xqc[e],
+                                      d.typec[e], d.hjc + e, a);
           }
         }
       }
@@ -1110,24 +1174,74 @@ __global__ __launch_bounds__(512) void k_pair(Dev<R> d, Box<R> b, Grid g,
     }
   }
   write_partials<R, EFLAG>(acc, d.pair_part, smem);
-  // the last block to finish reduces all brick partials in a fixed order
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    s_last = atomicAdd(&d.ctrl->pair_ticket, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (s_last) {
-    __threadfence();
+}
+
+// One block after the pair kernel: FP64 sums of the per-brick partials in a
+// fixed order (thermo[0..10]: force sum, E_lj, E_coul, virial; the mean pair
+// force when mean_fix) and, after a rebuild, the list statistics of the
+// per-block partials of k_list_a / k_list_b.
+__global__ __launch_bounds__(1024) void k_reduce(
+    const double* __restrict__ pair_part, int npair, double* thermo,
+    double* mean, int mean_fix, double n_global, const int4* lista_part,
+    int nlista, const int2* listb_part, int nlistb, Ctrl* ctrl,
+    unsigned thr) {
+  __shared__ double smem[16 * 11];
+  if (npair > 0) {
     double v[11];
     for (int k = 0; k < 11; ++k) v[k] = 0.0;
-    for (int blk = threadIdx.x; blk < (int)gridDim.x; blk += blockDim.x)
-      for (int k = 0; k < 11; ++k) v[k] += d.pair_part[11 * blk + k];
+    for (int blk = threadIdx.x; blk < npair; blk += blockDim.x)
+      for (int k = 0; k < 11; ++k) v[k] += pair_part[11 * blk + k];
     block_sum<11>(v, smem);
     if (threadIdx.x == 0) {
-      for (int k = 0; k < 11; ++k) d.thermo[k] = v[k];
-      for (int k = 0; k < 3; ++k) d.mean[k] = mean_fix ? v[k] / n_global : 0.0;
-      d.ctrl->pair_ticket = 0u;
+      for (int k = 0; k < 11; ++k) thermo[k] = v[k];
+      for (int k = 0; k < 3; ++k) mean[k] = mean_fix ? v[k] / n_global : 0.0;
+    }
+  }
+  if (!rebuild_now(ctrl, thr)) return;
+  __syncthreads();
+  double s[6] = {0, 0, 0, 0, 0, 0};  // max_a, sum_a, stage, unstaged, max_b, sum_b
+  for (int k = threadIdx.x; k < nlista; k += blockDim.x) {
+    const int4 v = lista_part[k];
+    s[0] = fmax(s[0], (double)v.x);
+    s[1] += (double)v.y;
+    s[2] = fmax(s[2], (double)v.z);
+    s[3] += (double)v.w;
+  }
+  for (int k = threadIdx.x; k < nlistb; k += blockDim.x) {
+    const int2 v = listb_part[k];
+    s[4] = fmax(s[4], (double)v.x);
+    s[5] += (double)v.y;
+  }
+  // counts are small integers, exact in FP64; maxima and sums by waves, then
+  // across waves
+  for (int o = 32; o > 0; o >>= 1) {
+    s[0] = fmax(s[0], __shfl_xor(s[0], o, 64));
+    s[2] = fmax(s[2], __shfl_xor(s[2], o, 64));
+    s[4] = fmax(s[4], __shfl_xor(s[4], o, 64));
+    s[1] += __shfl_xor(s[1], o, 64);
+    s[3] += __shfl_xor(s[3], o, 64);
+    s[5] += __shfl_xor(s[5], o, 64);
+  }
+  const int nw = blockDim.x >> 6, wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < 6; ++q) smem[6 * wid + q] = s[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nw; ++w) {
+      s[0] = fmax(s[0], smem[6 * w]);
+      s[2] = fmax(s[2], smem[6 * w + 2]);
+      s[4] = fmax(s[4], smem[6 * w + 4]);
+      s[1] += smem[6 * w + 1];
+      s[3] += smem[6 * w + 3];
+      s[5] += smem[6 * w + 5];
+    }
+    ctrl->max_count_a = max(ctrl->max_count_a, (int)s[0]);
+    ctrl->sum_count_a = (unsigned long long)s[1];
+    ctrl->max_stage = max(ctrl->max_stage, (int)s[2]);
+    ctrl->unstaged_bricks = (int)s[3];
+    if (nlistb > 0) {
+      ctrl->max_count_b = max(ctrl->max_count_b, (int)s[4]);
+      ctrl->sum_count_b = (unsigned long long)s[5];
     }
   }
 }
@@ -1155,26 +1269,35 @@ __device__ __forceinline__ void frac_coords(const Box<R>& b, const R4T<R>& p,
 }
 
 // S(k) partial sums: block (chunk of kSkChunk charged atoms) x (kSkTile k's).
-// The last block to finish sums the chunk partials of every k in FP64 (fixed
-// order), writes rho_k and the per-k force coefficients
-//   A_k = w_k (-4 pi qqr2e) / (V k^2),  kf0 = (m, A Re rho), kf1 = (k, A Im rho)
-// and the k-space energy and virial (without the self terms):
-//   E = qqr2e (2 pi / V) sum_k w_k |rho|^2 / k^2,
-//   W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/(4 alpha) + 1/k^2)).
+// m_src holds the K integer k-vectors, in device memory or, for the sampled
+// stream, in host-mapped staging memory; in that case m_copy (device) is
+// given and the chunk-0 block of every k tile copies its tile there for
+// k_sk_fin and later calls.
 template <typename R>
 __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
-                                            const int* __restrict__ m, int K,
-                                            Params p) {
+                                            const int* __restrict__ m_src,
+                                            int* __restrict__ m_copy, int K) {
   __shared__ double smem[4 * 2 * kSkTile];
   const int chunk = blockIdx.x, k0 = blockIdx.y * kSkTile;
   const int kn = min(kSkTile, K - k0);
   float mx[kSkTile], my[kSkTile], mz[kSkTile];
+  int mi[3 * kSkTile];
 #pragma unroll
   for (int kk = 0; kk < kSkTile; ++kk) {
     const int k = k0 + (kk < kn ? kk : 0);
-    mx[kk] = (float)m[3 * k];
-    my[kk] = (float)m[3 * k + 1];
-    mz[kk] = (float)m[3 * k + 2];
+    mi[3 * kk] = m_src[3 * k];
+    mi[3 * kk + 1] = m_src[3 * k + 1];
+    mi[3 * kk + 2] = m_src[3 * k + 2];
+    mx[kk] = (float)mi[3 * kk];
+    my[kk] = (float)mi[3 * kk + 1];
+    mz[kk] = (float)mi[3 * kk + 2];
+  }
+  if (m_copy && chunk == 0 && threadIdx.x == 0) {
+#pragma unroll
+    for (int kk = 0; kk < kSkTile; ++kk)
+      if (kk < kn)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) m_copy[3 * (k0 + kk) + q] = mi[3 * kk + q];
   }
   float re[kSkTile], im[kSkTile];
 #pragma unroll
@@ -1205,16 +1328,19 @@ __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
     for (int kk = 0; kk < kn; ++kk)
       d.sk_part[(size_t)chunk * d.kcap + k0 + kk] =
           make_double2(v[2 * kk], v[2 * kk + 1]);
-  // last block: FP64 reduction over chunks and the k-space energy / virial
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    s_last = atomicAdd(&d.ctrl->sk_ticket, 1u) == gridDim.x * gridDim.y - 1;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  const int nchunks = gridDim.x;
+}
+
+// FP64 reduction of the S(k) chunk partials (fixed order, one thread per k)
+// into rho_k and the per-k force coefficients
+//   A_k = w_k (-4 pi qqr2e) / (V k^2),  kf0 = (m, A Re rho), kf1 = (k, A Im rho)
+// and the k-space energy and virial (without the self terms):
+//   E = qqr2e (2 pi / V) sum_k w_k |rho|^2 / k^2,
+//   W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/(4 alpha) + 1/k^2)).
+// One block; the kernel boundary after k_sk makes every partial visible.
+template <typename R>
+__global__ __launch_bounds__(1024) void k_sk_fin(Dev<R> d, Box<R> b,
+                                                 const int* __restrict__ m,
+                                                 int K, int nchunks, Params p) {
   const double V = p.volume;
   const double two_pi = 6.283185307179586476925;
   double kv[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1245,11 +1371,10 @@ __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
     kv[5] += base * (-cc * kx * kz);
     kv[6] += base * (-cc * ky * kz);
   }
-  __shared__ double smem7[4 * 7];
+  __shared__ double smem7[16 * 7];
   block_sum<7>(kv, smem7);
   if (threadIdx.x == 0) {
     for (int q = 0; q < 7; ++q) d.thermo[16 + q] = kv[q];
-    d.ctrl->sk_ticket = 0u;
   }
 }
 
@@ -1370,7 +1495,7 @@ __global__ __launch_bounds__(512) void k_export(Dev<R> d, Box<R> b, Grid g,
   auto classify = [&](R d2, int gi, int gj) -> int {
     if (!RBL) return d2 < pc.rc2 ? 1 : 0;
     if (d2 < pc.t_core) return 1;
-    if (d2 < pc.t_shell && rbl_selected(pc.ifreq, pc.epoch, gi, gj)) return 2;
+    if (d2 < pc.t_shell && rbl_selected(pc, gi, gj)) return 2;
     return 0;
   };
   const int e1[3] = {1, 1, 1};
@@ -1425,14 +1550,15 @@ inline int blocks_for(int n, int per = kBlock, int cap = 1 << 20) {
 }
 
 template <typename R>
-hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
+hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, int nskin,
+                      hipStream_t s) {
   const Grid& g = a.grid;
   const unsigned thr = a.rebuild_thr;
   const int gb = blocks_for(d.n, kBlock, 2048);
   const int cap_a = a.count_only ? 0 : d.cap_a;
   const int cap_b = a.count_only ? 0 : d.cap_b;
   hipLaunchKernelGGL((k_bin<R>), dim3(gb), dim3(kBlock), 0, s, d, a.box, g,
-                     a.p.split, thr);
+                     a.p.split, thr, nskin);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, d.key_count,
                      d.key_start, d.cstart, g.ncell, d.ctrl, thr);
   hipLaunchKernelGGL((k_scatter<R>), dim3(gb), dim3(kBlock), 0, s, d, thr);
@@ -1451,25 +1577,36 @@ hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+int list_b_grid(const Params& p, int nc) {
+  return (p.split && nc > 0) ? list_b_blocks(nc) : 0;
+}
+
 }  // namespace
 
 template <typename R>
 hipError_t launch_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_skin<R>), dim3(1), dim3(kBlock), 0, s, d, a.box, 1,
-                     nullptr, 0);
-  return build_part<R>(d, a, s);
+  hipLaunchKernelGGL((k_skin<R>), dim3(1), dim3(kBlock), 0, s, d, a.box, 1);
+  hipError_t e = build_part<R>(d, a, 1, s);
+  if (e != hipSuccess) return e;
+  // list statistics only (no pair partials)
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part, 0,
+                     d.thermo, d.mean, 0, 1.0, d.lista_part, a.grid.nbrick,
+                     d.listb_part, list_b_grid(a.p, d.nc), d.ctrl,
+                     a.rebuild_thr);
+  return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   const Params& p = a.p;
-  // one atom per thread for the first 2048 * 256 atoms (HBM-bound read of
-  // positions and reference positions), one atomic per block
-  const int sb = blocks_for(d.n, kBlock, 2048);
+  // skin check: HBM-bound read of positions and reference positions,
+  // grid-stride over at most kSkinBlocksMax blocks (their maxima are folded
+  // by every block of k_bin)
+  const int sb = blocks_for(d.n, kBlock, kSkinBlocksMax);
   if (a.ev[0]) hipEventRecord(a.ev[0], s);
   hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(kBlock), 0, s, d, a.box,
-                     a.force_rebuild, a.m_host, a.m_host ? 3 * p.num_k : 0);
-  hipError_t e = build_part<R>(d, a, s);
+                     a.force_rebuild);
+  hipError_t e = build_part<R>(d, a, sb, s);
   if (e != hipSuccess) return e;
   if (a.ev[1]) hipEventRecord(a.ev[1], s);
   const Grid& g = a.grid;
@@ -1485,7 +1622,11 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
     if (a.ev[3]) hipEventRecord(a.ev[3], ks);
     const int nchunks = (d.nc + kSkChunk - 1) / kSkChunk;
     hipLaunchKernelGGL((k_sk<R>), dim3(nchunks, (K + kSkTile - 1) / kSkTile),
-                       dim3(kBlock), 0, ks, d, a.box, a.m_use, K, p);
+                       dim3(kBlock), 0, ks, d, a.box,
+                       a.m_host ? a.m_host : a.m_use,
+                       a.m_host ? d.m_dev : nullptr, K);
+    hipLaunchKernelGGL((k_sk_fin<R>), dim3(1), dim3(1024), 0, ks, d, a.box,
+                       a.m_host ? (const int*)d.m_dev : a.m_use, K, nchunks, p);
     hipLaunchKernelGGL((k_rbe_force<R>), dim3(blocks_for(d.nc, 64)),
                        dim3(64 * kRbeWaves), 0, ks, d, a.box, K);
     if (a.ev[4]) hipEventRecord(a.ev[4], ks);
@@ -1495,22 +1636,35 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
     hipEventRecord(a.ev[4], s);
   }
   const size_t lj_bytes = 2 * sizeof(R) * (size_t)a.pc.ntypes * a.pc.ntypes;
-#define RBMDX_PAIR(RBLv, EFv)                                                \
-  hipLaunchKernelGGL((k_pair<R, RBLv, EFv>), dim3(g.nbrick), dim3(g.block), \
-                     lj_bytes, s, d, a.box, g, a.pc, a.lj_tab, p.split,      \
-                     p.mean_fix, p.n_global)
+#define RBMDX_PAIR(RBLv, EFv, SHv)                                          \
+  hipLaunchKernelGGL((k_pair<R, RBLv, EFv, SHv>), dim3(g.nbrick),            \
+                     dim3(g.block), lj_bytes, s, d, a.box, g, a.pc, a.lj_tab, \
+                     p.split)
+  // list A holds RBL shell pairs only when the LJ cutoff exceeds r_core
+  const bool shell_a = a.pc.rclj2 > a.pc.rs2;
   if (p.rbl) {
-    if (p.eflag)
-      RBMDX_PAIR(true, true);
-    else
-      RBMDX_PAIR(true, false);
+    if (p.eflag) {
+      if (shell_a)
+        RBMDX_PAIR(true, true, true);
+      else
+        RBMDX_PAIR(true, true, false);
+    } else {
+      if (shell_a)
+        RBMDX_PAIR(true, false, true);
+      else
+        RBMDX_PAIR(true, false, false);
+    }
   } else {
     if (p.eflag)
-      RBMDX_PAIR(false, true);
+      RBMDX_PAIR(false, true, false);
     else
-      RBMDX_PAIR(false, false);
+      RBMDX_PAIR(false, false, false);
   }
 #undef RBMDX_PAIR
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part, g.nbrick,
+                     d.thermo, d.mean, p.mean_fix, p.n_global, d.lista_part,
+                     g.nbrick, d.listb_part, list_b_grid(p, d.nc), d.ctrl,
+                     a.rebuild_thr);
   if (a.ev[2]) hipEventRecord(a.ev[2], s);
   if (ks_on && ks != s) hipStreamWaitEvent(s, a.ev_join, 0);
   hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,

@@ -46,13 +46,16 @@ def _sets(gi, gj, cls):
     return set(zip(gi.tolist(), gj.tolist(), cls.tolist()))
 
 
+# neighbor_sample_num 30, 40, 20 give RBL frequencies 4, 3, 5 on this system:
+# a power of two and two odd frequencies (the two remainder paths)
+@pytest.mark.parametrize("sample_num", [30, 40, 20])
 @pytest.mark.parametrize("lj_cut", [3.5, 2.5])
 @pytest.mark.parametrize("step", [0, 7])
-def test_rbl_pair_sets_bit_exact(device, small_electrolyte, lj_cut, step):
+def test_rbl_pair_sets_bit_exact(device, small_electrolyte, lj_cut, step, sample_num):
     import torch
 
     s = small_electrolyte
-    cfg = systems.ljsalt_config(lj_cut_off=lj_cut, sample_num=30)
+    cfg = systems.ljsalt_config(lj_cut_off=lj_cut, sample_num=sample_num)
     ctx = _ctx(s, cfg)
     (px, py, pz), _ = _tensors(s, device, torch.float32)
     gi, gj, cls = ctx.export_pairs(step, px, py, pz)
@@ -60,6 +63,7 @@ def test_rbl_pair_sets_bit_exact(device, small_electrolyte, lj_cut, step):
     nb = cfg["hyper_parameters"]["neighbor"]
     freq = oracle.rbl_frequency(s.n, s.box_len, nb["r_core"], nb["cut_off"],
                                 nb["neighbor_sample_num"])
+    assert freq == {30: 4, 40: 3, 20: 5}[sample_num]
     assert ctx.thermo()["rbl_frequency"] == freq
     params = oracle.pair_params(nb["r_core"], nb["cut_off"], lj_cut, 0.09, 1.0, freq, step,
                                 1, 1, 1)
@@ -105,7 +109,8 @@ def test_candidate_lists_bit_exact(device, small_electrolyte):
     del R2
 
 
-@pytest.mark.parametrize("lj_cut,sample_num", [(3.5, 30), (2.5, 30), (2.5, 1000)])
+@pytest.mark.parametrize("lj_cut,sample_num", [(3.5, 30), (2.5, 30), (2.5, 1000), (3.5, 40),
+                                               (2.5, 20)])
 def test_rbl_rbe_force_parity(device, small_electrolyte, lj_cut, sample_num):
     import torch
 
@@ -327,9 +332,16 @@ def test_fused_leapfrog_matches_separate_update(device):
         ctx_b.force(k, pb[0], pb[1], pb[2], fb[0], fb[1], fb[2], sync=True)
         vb += dt * fb * inv_m
         pb += dt * vb
-        np.testing.assert_allclose(fa.cpu().numpy(), fb.cpu().numpy(), rtol=1e-5, atol=1e-5)
-    np.testing.assert_allclose(pa.cpu().numpy(), pb.cpu().numpy(), rtol=1e-6, atol=1e-6)
-    np.testing.assert_allclose(va.cpu().numpy(), vb.cpu().numpy(), rtol=1e-5, atol=1e-5)
+        if k == 0:
+            # same positions: the same kernels give the same forces
+            np.testing.assert_array_equal(fa.cpu().numpy(), fb.cpu().numpy())
+        else:
+            # the two updates round v += dt F / m differently (the kernel
+            # contracts to an FMA); close LJ contacts amplify the ulp-level
+            # position difference, so forces are compared as a relative RMS
+            assert rel_rms(fa.T.double().cpu().numpy(), fb.T.double().cpu().numpy()) < 1e-5
+        np.testing.assert_allclose(pa.cpu().numpy(), pb.cpu().numpy(), rtol=1e-6, atol=1e-6)
+        np.testing.assert_allclose(va.cpu().numpy(), vb.cpu().numpy(), rtol=1e-4, atol=1e-4)
 
 
 def test_kspace_stream_schedules_bitwise_equal(device, small_electrolyte, monkeypatch):
