@@ -89,6 +89,7 @@ def main():
     th0 = ctx.thermo()
     t0 = time.perf_counter()
     md_steps(args.warmup, args.steps, False)
+    host_elapsed = time.perf_counter() - t0  # enqueue time on the host
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     th1 = ctx.thermo()
@@ -156,6 +157,7 @@ def main():
                    "parallelism": "single GPU"},
         "kernel_us": {k: round(v, 2) for k, v in kt.items()},
         "ms_per_step_with_timing_events": round(1e3 * elapsed_ev / args.steps, 4),
+        "host_enqueue_ms_per_step": round(1e3 * host_elapsed / args.steps, 4),
         "force_rel_rms_vs_fp64": ferr,
         "roofline": roofline,
         "cpu_baseline": cpu,

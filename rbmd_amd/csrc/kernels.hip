@@ -74,8 +74,8 @@ __device__ __forceinline__ R4T<R> make4(R x, R y, R z, R w) {
 // half the box edge, which set-up enforces).
 template <typename R>
 __device__ __forceinline__ R min_image(R d, R L, R half) {
-  if (fabs(d) > half) d = d > R(0) ? d - L : d + L;
-  return d;
+  // one shift by L towards zero when |d| > L/2 (d - sign(d) L), as a select
+  return d - (fabs(d) > half ? copysign(L, d) : R(0));
 }
 
 // The squared pair distance every classification uses:
@@ -642,38 +642,68 @@ __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
     int count = 0;
     unsigned long long lo = 0, hi = 0;  // entries count&~7 .. count&~7 + 7
     uint4* out = reinterpret_cast<uint4*>(d.list_a) + i;
+    auto accept = [&](int l) {
+      const int s8 = count & 7;
+      const unsigned long long v = (unsigned long long)l;
+      if (s8 < 4)
+        lo |= v << (16 * s8);
+      else
+        hi |= v << (16 * (s8 - 4));
+      if (s8 == 7) {
+        if (count < lcap)
+          out[(size_t)(count >> 3) * d.n] =
+              make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi,
+                         (unsigned)(hi >> 32));
+        lo = hi = 0;
+      }
+      ++count;
+    };
+    // Staged neighbourhoods scan a stencil row (its x cells are consecutive
+    // slots, hence one contiguous stage range) as one run, 4 candidates at a
+    // time; the charged atoms of each cell are skipped for a charged i.
+    const bool rows = STAGED && st.mode[0] != 1;
+    const int runlen = rows ? st.n[0] : 1;
     for (int kz = 0; kz < st.n[2]; ++kz)
-      for (int ky = 0; ky < st.n[1]; ++ky)
-        for (int kx = 0; kx < st.n[0]; ++kx) {
-          const int sl =
-              (st.at(2, kz) * h.cnt[1] + st.at(1, ky)) * h.cnt[0] + st.at(0, kx);
-          const int base = t.off[sl];
-          const int len = t.off[sl + 1] - base;
-          const int js = qi ? t.nq[sl] : 0;
-          for (int k = js; k < len; ++k) {
-            const int l = base + k;
-            const R4T<R> xj = STAGED ? stage[l] : d.xs[t.first[sl] + k];
-            R dx, dy, dz;
-            const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx,
-                                    dy, dz);
-            if (d2 < r2 && l != self) {
-              const int s8 = count & 7;
-              const unsigned long long v = (unsigned long long)l;
-              if (s8 < 4)
-                lo |= v << (16 * s8);
-              else
-                hi |= v << (16 * (s8 - 4));
-              if (s8 == 7) {
-                if (count < lcap)
-                  out[(size_t)(count >> 3) * d.n] =
-                      make_uint4((unsigned)lo, (unsigned)(lo >> 32),
-                                 (unsigned)hi, (unsigned)(hi >> 32));
-                lo = hi = 0;
+      for (int ky = 0; ky < st.n[1]; ++ky) {
+        const int srow = (st.at(2, kz) * h.cnt[1] + st.at(1, ky)) * h.cnt[0];
+        for (int kx = 0; kx < st.n[0]; kx += runlen) {
+          const int s0 = srow + st.at(0, kx);
+          const int l0 = t.off[s0], l1 = t.off[s0 + runlen];
+          int qa[3], qb[3];  // charged windows of the run's cells
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            qa[q] = q < runlen ? t.off[s0 + q] : 0;
+            qb[q] = (q < runlen && qi) ? qa[q] + t.nq[s0 + q] : qa[q];
+          }
+          if (STAGED) {
+            for (int l = l0; l < l1; l += 4) {
+              R4T<R> xj[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) xj[u] = stage[min(l + u, l1 - 1)];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const int lu = l + u;
+                R dx, dy, dz;
+                const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj[u].x, xj[u].y,
+                                        xj[u].z, dx, dy, dz);
+                const bool excl = (lu >= qa[0] && lu < qb[0]) ||
+                                  (lu >= qa[1] && lu < qb[1]) ||
+                                  (lu >= qa[2] && lu < qb[2]);
+                if (lu < l1 && d2 < r2 && lu != self && !excl) accept(lu);
               }
-              ++count;
+            }
+          } else {
+            const int jf = t.first[s0];
+            for (int l = qb[0]; l < l1; ++l) {  // charged i skips the cell's charged
+              const R4T<R> xj = d.xs[jf + (l - l0)];
+              R dx, dy, dz;
+              const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z,
+                                      dx, dy, dz);
+              if (d2 < r2 && l != self) accept(l);
             }
           }
         }
+      }
     if ((count & 7) && count <= lcap)
       out[(size_t)(count >> 3) * d.n] =
           make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi,
@@ -917,13 +947,6 @@ __device__ __forceinline__ float pair_rcp(float x) {
 }
 __device__ __forceinline__ double pair_rcp(double x) { return 1.0 / x; }
 
-// min_image without a branch: the same single shift, d - sign(d) L when
-// |d| > L/2, so results are identical bit for bit.
-template <typename R>
-__device__ __forceinline__ R min_image_sel(R d, R L, R half) {
-  return d - (fabs(d) > half ? copysign(L, d) : R(0));
-}
-
 template <typename R>
 struct R2Of {
   using type = float2;
@@ -952,8 +975,11 @@ __device__ __forceinline__ void list_a_atom(
     const int s = slot_of_pos(t, nslots, l);
     return t.first[s] + (l - t.off[s]);
   };
+  // the next 16-byte group is loaded while the current one is evaluated
+  uint4 q_next = cnt > 0 ? lp[0] : make_uint4(0u, 0u, 0u, 0u);
   for (int k0 = 0; k0 < cnt; k0 += 8) {
-    const uint4 q8 = lp[(size_t)(k0 >> 3) * d.n];
+    const uint4 q8 = q_next;
+    if (k0 + 8 < cnt) q_next = lp[(size_t)((k0 >> 3) + 1) * d.n];
     const int kn = cnt - k0;
     const unsigned wv[4] = {q8.x, q8.y, q8.z, q8.w};
 #pragma unroll
@@ -973,9 +999,9 @@ __device__ __forceinline__ void list_a_atom(
     }
 #pragma unroll
     for (int kk = half; kk < half + 4; ++kk) {
-      const R dx = min_image_sel<R>(xj[kk].x - xi.x, b.L[0], b.half[0]);
-      const R dy = min_image_sel<R>(xj[kk].y - xi.y, b.L[1], b.half[1]);
-      const R dz = min_image_sel<R>(xj[kk].z - xi.z, b.L[2], b.half[2]);
+      const R dx = min_image<R>(xj[kk].x - xi.x, b.L[0], b.half[0]);
+      const R dy = min_image<R>(xj[kk].y - xi.y, b.L[1], b.half[1]);
+      const R dz = min_image<R>(xj[kk].z - xi.z, b.L[2], b.half[2]);
       const R d2 = fma_r(dz, dz, fma_r(dy, dy, mul_rn(dx, dx)));
       const R2 lj = *reinterpret_cast<const R2*>(s_lj + tbase + 2 * (int)xj[kk].w);
       const R inv = pair_rcp(d2);
