@@ -76,6 +76,30 @@ def lib():
         L.orc_force_step.argtypes = [ctypes.c_int, _F, _F, _F, _F, _I, _I, ctypes.c_int,
                                      _D, _D, _D, _D, ctypes.c_int, _I, ctypes.c_double,
                                      _D, _D, _D]
+        c_int, c_dbl = ctypes.c_int, ctypes.c_double
+        L.orc_sog_preset.restype = None
+        L.orc_sog_preset.argtypes = [c_int, _D]
+        L.orc_sog_setup.restype = None
+        L.orc_sog_setup.argtypes = [c_int, c_dbl, _D, c_int, c_int, _D, _D, _D]
+        L.orc_sog_sample.restype = None
+        L.orc_sog_sample.argtypes = [ctypes.c_uint32, ctypes.c_long, c_int, _D, c_dbl,
+                                     c_dbl, c_int, _D, _D, c_int, _I, _I]
+        L.orc_sog_f.restype = c_dbl
+        L.orc_sog_f.argtypes = [c_dbl, c_dbl, c_dbl, c_int, _D]
+        L.orc_sog_kspace_f64.restype = None
+        L.orc_sog_kspace_f64.argtypes = [c_int, _D, _D, _D, _D, _D, c_int, _I, _I, c_dbl,
+                                         c_dbl, c_dbl, c_dbl, c_int, _D, _D, c_int, c_dbl,
+                                         _D, _D, _D, _D, _D]
+        L.orc_sog_self.restype = c_dbl
+        L.orc_sog_self.argtypes = [c_dbl, c_dbl, c_dbl, c_int, c_dbl, c_dbl]
+        L.orc_sog_far_r.restype = c_dbl
+        L.orc_sog_far_r.argtypes = [c_dbl, c_dbl, c_dbl, c_int, c_dbl]
+        L.orc_sog_taylor.restype = None
+        L.orc_sog_taylor.argtypes = [c_dbl, c_dbl, c_int, c_dbl, _D]
+        L.orc_sog_verlet_f64.restype = None
+        L.orc_sog_verlet_f64.argtypes = [c_int, _D, _D, _D, _D, _I, c_int, _D, _D, _D,
+                                         c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_int, c_dbl,
+                                         _D, _D, _D, _D, _D, _D]
         _lib = L
     return _lib
 
@@ -275,3 +299,102 @@ def force_step(pos, q, types, gid, eps, sig, L, params, m, rbe_weight):
     lib().orc_force_step(n, xp, yp, zp, qp, tp, gp, len(eps), ep, sp, Lp, ppp,
                          len(mm) // 3, mp, rbe_weight, *[a.ctypes.data_as(_D) for a in f])
     return np.stack(f, axis=1)
+
+
+# ---- RBSOG / VERLET-SOG (oracle/sog.cpp) -----------------------------------
+def sog_preset(level):
+    out = np.zeros(4)
+    lib().orc_sog_preset(level, out.ctypes.data_as(_D))
+    return {"b": out[0], "sigma_rc10": out[1], "accuracy": out[2], "mmax": int(out[3])}
+
+
+def sog_setup(level, cut_off, L, kcut=1, exact=False):
+    """Coefficients and normalisation: dict(b, sigma, w0, S, S_npt, mmax, coef,
+    coef_npt).  exact=False: the reference's float set-up (with its defects);
+    exact=True: the corrected double set-up."""
+    Ld, Lp = _d(L)
+    scal = np.zeros(6)
+    coef = np.zeros(128)
+    coefn = np.zeros(128)
+    lib().orc_sog_setup(level, float(cut_off), Lp, kcut, 1 if exact else 0,
+                        scal.ctypes.data_as(_D), coef.ctypes.data_as(_D),
+                        coefn.ctypes.data_as(_D))
+    M = int(scal[5])
+    return {"b": scal[0], "sigma": scal[1], "w0": scal[2], "S": scal[3], "S_npt": scal[4],
+            "mmax": M, "coef": coef[:M].copy(), "coef_npt": coefn[:M].copy()}
+
+
+def sog_sample(base_seed, call_index, P, L, s, kcut=1):
+    """(m (P,3), idx (P,)) of force evaluation call_index for setup dict s."""
+    Ld, Lp = _d(L)
+    c, cp = _d(s["coef"])
+    cn, cnp = _d(s["coef_npt"])
+    m = np.zeros(3 * P, dtype=np.int32)
+    idx = np.zeros(P, dtype=np.int32)
+    lib().orc_sog_sample(base_seed & 0xFFFFFFFF, call_index, P, Lp, s["sigma"], s["b"],
+                         s["mmax"], cp, cnp, kcut, m.ctypes.data_as(_I),
+                         idx.ctypes.data_as(_I))
+    return m.reshape(P, 3), idx
+
+
+def sog_f(k2, s, npt=False):
+    c, cp = _d(s["coef_npt"] if npt else s["coef"])
+    return lib().orc_sog_f(float(k2), s["sigma"], s["b"], s["mmax"], cp)
+
+
+def sog_kspace_f64(pos, q, L, m, idx, s, qqr2e=1.0, kcut=1):
+    """(F (N,3), energy without self term, virial[6]) of the RBSOG k-space."""
+    n = len(pos)
+    x, xp = _d(pos[:, 0])
+    y, yp = _d(pos[:, 1])
+    z, zp = _d(pos[:, 2])
+    qq, qp = _d(q)
+    Ld, Lp = _d(L)
+    mm, mp = _i(np.asarray(m).reshape(-1))
+    ii, ip = _i(idx)
+    c, cp = _d(s["coef"])
+    cn, cnp = _d(s["coef_npt"])
+    f = [np.zeros(n) for _ in range(3)]
+    e = np.zeros(1)
+    v = np.zeros(6)
+    lib().orc_sog_kspace_f64(n, xp, yp, zp, qp, Lp, len(ii), mp, ip, s["S"], s["S_npt"],
+                             s["sigma"], s["b"], s["mmax"], cp, cnp, kcut, qqr2e,
+                             *[a.ctypes.data_as(_D) for a in f], e.ctypes.data_as(_D),
+                             v.ctypes.data_as(_D))
+    return np.stack(f, axis=1), e[0], v
+
+
+def sog_self(sum_q2, s, qqr2e=1.0):
+    return lib().orc_sog_self(sum_q2, s["sigma"], s["b"], s["mmax"], s["w0"], qqr2e)
+
+
+def sog_far_r(r, s):
+    return lib().orc_sog_far_r(float(r), s["sigma"], s["b"], s["mmax"], s["w0"])
+
+
+def sog_taylor(s):
+    T = np.zeros(6)
+    lib().orc_sog_taylor(s["sigma"], s["b"], s["mmax"], s["w0"], T.ctypes.data_as(_D))
+    return T
+
+
+def sog_verlet_f64(pos, q, types, eps, sig, L, rc, rc_lj, s, qqr2e=1.0):
+    """(F (N,3), (E_lj, E_coul), virial[6]) of VERLET-SOG pairs, O(N^2)."""
+    n = len(pos)
+    x, xp = _d(pos[:, 0])
+    y, yp = _d(pos[:, 1])
+    z, zp = _d(pos[:, 2])
+    qq, qp = _d(q)
+    t, tp = _i(types)
+    e_, ep = _d(eps)
+    s_, sp = _d(sig)
+    Ld, Lp = _d(L)
+    T, Tp = _d(sog_taylor(s))
+    f = [np.zeros(n) for _ in range(3)]
+    en = np.zeros(2)
+    v = np.zeros(6)
+    lib().orc_sog_verlet_f64(n, xp, yp, zp, qp, tp, len(eps), ep, sp, Lp, rc, rc_lj, qqr2e,
+                             s["sigma"], s["b"], s["mmax"], s["w0"], Tp,
+                             *[a.ctypes.data_as(_D) for a in f], en.ctypes.data_as(_D),
+                             v.ctypes.data_as(_D))
+    return np.stack(f, axis=1), en, v

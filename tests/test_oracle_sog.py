@@ -1,0 +1,239 @@
+"""Pins of the RBSOG / VERLET-SOG oracle (oracle/sog.cpp), checked against what
+the mathematics fixes rather than against a retyped formula:
+
+* the normalisation S (and S_npt) of the exact set against a brute-force 3-D
+  lattice sum, and the reference's float S against an independent double
+  evaluation of its level-0-only analytic series;
+* the far-field Gaussian sum against 1/r beyond the cutoff, to the accuracy the
+  level preset states;
+* the Taylor near-field force against the derivative of the Gaussian sum;
+* k-space force = -grad of k-space energy for a fixed k-set;
+* VERLET-SOG pair force = -grad of its energy;
+* the self energy against the far field at r = 0;
+* the sampler: no |m|^2 <= Kcut, the virial chain's index structure, and a
+  chi-square of the m-marginal against the target f(k) k^2.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+L_BOX = [10.0, 11.0, 12.5]
+
+
+def _k2(m, L):
+    k = 2 * np.pi * np.asarray(m, dtype=float) / np.asarray(L, dtype=float)
+    return float((k * k).sum())
+
+
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_exact_S_equals_brute_force_lattice_sum(level):
+    s = oracle.sog_setup(level, 3.0, L_BOX, exact=True)
+    M = 40
+    S = Sn = 0.0
+    r = np.arange(-M, M + 1)
+    mx, my, mz = np.meshgrid(r, r, r, indexing="ij")
+    k2 = ((2 * np.pi * mx / L_BOX[0]) ** 2 + (2 * np.pi * my / L_BOX[1]) ** 2
+          + (2 * np.pi * mz / L_BOX[2]) ** 2)
+    msq = mx ** 2 + my ** 2 + mz ** 2
+    keep = msq > 1
+    f = np.zeros_like(k2)
+    fn = np.zeros_like(k2)
+    for l in range(s["mmax"]):
+        e = np.exp(-s["b"] ** (2 * l) * s["sigma"] ** 2 * k2 / 2)
+        f += s["coef"][l] * e
+        fn += s["coef_npt"][l] * e
+    S = (f * k2)[keep].sum()
+    Sn = (fn * k2 * k2)[keep].sum()
+    assert s["S"] == pytest.approx(S, rel=1e-10)
+    assert s["S_npt"] == pytest.approx(Sn, rel=1e-10)
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_reference_S_is_level0_series_minus_all_level_low_k(level):
+    """The reference's S: its analytic high-k series runs for level 0 only, the
+    low-k subtraction covers every level.  Recomputed here in double from the
+    reference's own (float) coefficients."""
+    L = [20.0, 20.0, 20.0]
+    s = oracle.sog_setup(level, 3.5, L)
+    sl2 = 2 * s["sigma"] ** 2  # sl[0]^2 = 2 sigma^2
+    M = 60
+    r = np.arange(-M, M + 1)
+    mx, my, mz = np.meshgrid(r, r, r, indexing="ij")
+    k2 = sum((2 * np.pi * m / Ld) ** 2 for m, Ld in zip((mx, my, mz), L))
+    # the series' exponent is (pi j / L)^2 sl^2 = k^2 sigma^2 / 2
+    high = s["coef"][0] * (k2 * np.exp(-k2 * sl2 / 4)).sum()
+    low = 0.0
+    for m in ([1, 0, 0], [0, 1, 0], [0, 0, 1]):
+        kk = _k2(m, L)
+        low += 2 * oracle.sog_f(kk, s) * kk
+    assert s["S"] == pytest.approx(high - low, rel=2e-5)
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3])
+def test_far_field_approximates_coulomb_beyond_cutoff(level):
+    rc = 10.0
+    s = oracle.sog_setup(level, rc, [200.0] * 3, exact=True)
+    acc = oracle.sog_preset(level)["accuracy"]
+    # the truncated series is 1/r plus a near-constant offset (the missing
+    # wide levels, inert for a neutral system): compare potential differences
+    rs = np.linspace(rc, 3 * rc, 200)
+    far_ref = oracle.sog_far_r(3 * rc, s)
+    err = max(abs((oracle.sog_far_r(r, s) - far_ref) - (1 / r - 1 / (3 * rc))) * r
+              for r in rs)
+    assert err < 2.5 * acc, (err, acc)
+    # a dropped w0 or a wrong level weight is far outside that bound
+    bad = dict(s, w0=1.5 * s["w0"])
+    err_bad = max(abs((oracle.sog_far_r(r, bad) - oracle.sog_far_r(3 * rc, bad))
+                      - (1 / r - 1 / (3 * rc))) * r for r in rs)
+    assert err_bad > 10 * acc
+    # inside the cutoff the far field is smooth and bounded (no 1/r growth)
+    assert oracle.sog_far_r(0.0, s) < 2.0 / s["sigma"]
+
+
+@pytest.mark.parametrize("level", [0, 2])
+def test_taylor_is_expansion_of_near_field_force(level):
+    s = oracle.sog_setup(level, 3.5, [40.0] * 3, exact=True)
+    T = oracle.sog_taylor(s)
+    norm = 2 * np.log(s["b"]) / np.sqrt(2 * np.pi * s["sigma"] ** 2)
+
+    def exact_poly(r2):
+        # (1/r) d/dr of the far-field sum, whose r-derivative the near field
+        # force removes: -sum_l omega_l / s_l^2 exp(-r^2 / (2 s_l^2))
+        v = 0.0
+        for l in range(s["mmax"]):
+            w = norm * (s["w0"] if l == 0 else s["b"] ** (-l))
+            sl2 = s["sigma"] ** 2 * s["b"] ** (2 * l)
+            v += -w / sl2 * np.exp(-r2 / (2 * sl2))
+        return v
+
+    for r in [0.0, 0.05 * s["sigma"], 0.2 * s["sigma"]]:
+        r2 = r * r
+        poly = sum(T[n] * r2 ** n for n in range(6))
+        assert poly == pytest.approx(exact_poly(r2), rel=1e-6)
+    # and the far field's derivative agrees with it (finite difference)
+    r, h = 0.3 * s["sigma"], 1e-5
+    d = (oracle.sog_far_r(r + h, s) - oracle.sog_far_r(r - h, s)) / (2 * h)
+    assert d / r == pytest.approx(exact_poly(r * r), rel=1e-6)
+
+
+def _small_system(n=24, L=(9.0, 9.5, 10.0), seed=3):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0, 1, size=(n, 3)) * np.asarray(L)
+    q = rng.choice([-1.0, 1.0], size=n)
+    q[-1] = -q[:-1].sum() if abs(q[:-1].sum()) <= 1 else q[-1]
+    return pos, q, list(L)
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_sog_kspace_force_is_minus_gradient(exact):
+    pos, q, L = _small_system()
+    s = oracle.sog_setup(0, 3.0, L, exact=exact)
+    m, idx = oracle.sog_sample(123, 0, 32, L, s)
+    F, E, _ = oracle.sog_kspace_f64(pos, q, L, m, idx, s)
+    h = 1e-6
+    for i in (0, 5, 17):
+        for d in range(3):
+            p1 = pos.copy()
+            p1[i, d] += h
+            p2 = pos.copy()
+            p2[i, d] -= h
+            e1 = oracle.sog_kspace_f64(p1, q, L, m, idx, s)[1]
+            e2 = oracle.sog_kspace_f64(p2, q, L, m, idx, s)[1]
+            assert F[i, d] == pytest.approx(-(e1 - e2) / (2 * h), rel=1e-5, abs=1e-8)
+
+
+def test_sog_verlet_force_is_minus_gradient():
+    # LJ part over a neutral configuration: exact consistency
+    pos, q, L = _small_system(n=20, L=(8.0, 8.0, 8.0), seed=5)
+    pos = pos * 0.9 + 0.4
+    types = np.zeros(len(pos), dtype=np.int32)
+    eps, sig = np.array([0.2]), np.array([0.8])
+    s = oracle.sog_setup(1, 3.5, L, exact=True)
+    qz = np.zeros(len(pos))
+    F, en, _ = oracle.sog_verlet_f64(pos, qz, types, eps, sig, L, 3.5, 3.5, s)
+    h = 1e-6
+
+    def etot(p, qq, e, sg):
+        return sum(oracle.sog_verlet_f64(p, qq, types[:len(p)], e, sg, L, 3.5, 3.5, s)[1])
+
+    for i in (0, 7):
+        for d in range(3):
+            p1 = pos.copy()
+            p1[i, d] += h
+            p2 = pos.copy()
+            p2[i, d] -= h
+            fd = -(etot(p1, qz, eps, sig) - etot(p2, qz, eps, sig)) / (2 * h)
+            assert F[i, d] == pytest.approx(fd, rel=1e-5, abs=1e-8)
+    # Coulomb part: the near-field force is the Taylor series (6 terms) of the
+    # derivative of the near-field energy, exact where r^2 / (2 sigma^2) is
+    # small; a +-1 pair at r = 0.25 sigma
+    r = 0.25 * s["sigma"]
+    pp = np.array([[1.0, 1.0, 1.0], [1.0 + r, 1.0, 1.0]])
+    qq = np.array([1.0, -1.0])
+    e0 = np.array([0.0])
+    F2, _, _ = oracle.sog_verlet_f64(pp, qq, types[:2], e0, sig, L, 3.5, 3.5, s)
+    p1 = pp.copy()
+    p1[0, 0] += h
+    p2 = pp.copy()
+    p2[0, 0] -= h
+    fd = -(etot(p1, qq, e0, sig) - etot(p2, qq, e0, sig)) / (2 * h)
+    assert F2[0, 0] == pytest.approx(fd, rel=1e-6)
+    # near the cutoff the 6-term series departs from the derivative (the
+    # reference's truncation); the force stays finite and attractive here
+    pp[1, 0] = 1.0 + 3.0
+    F3, _, _ = oracle.sog_verlet_f64(pp, qq, types[:2], e0, sig, L, 3.5, 3.5, s)
+    assert np.isfinite(F3).all()
+
+
+@pytest.mark.parametrize("level", [0, 3])
+def test_self_energy_matches_far_field_at_origin(level):
+    s = oracle.sog_setup(level, 3.5, [30.0] * 3, exact=True)
+    q2 = 7.5
+    norm = 2 * np.log(s["b"]) / np.sqrt(2 * np.pi * s["sigma"] ** 2)
+    # reference formula sums b^-l for l = 1..mmax, one level beyond the far
+    # field's last level (kspace_calculator.cpp:1524-1525)
+    expect = -0.5 * q2 * (oracle.sog_far_r(0.0, s) + norm * s["b"] ** (-s["mmax"]))
+    assert oracle.sog_self(q2, s) == pytest.approx(expect, rel=1e-12)
+
+
+def test_sampler_support_and_virial_chain():
+    L = [20.0, 22.0, 24.0]
+    s = oracle.sog_setup(0, 3.5, L)
+    for call in range(5):
+        m, idx = oracle.sog_sample(77, call, 48, L, s)
+        assert ((m ** 2).sum(1) > 1).all()
+        assert idx[0] == 0
+        for i in range(1, len(idx)):
+            assert idx[i] in (i, idx[i - 1])
+    # determinism and dependence on the evaluation index
+    a = oracle.sog_sample(77, 3, 48, L, s)[0]
+    b = oracle.sog_sample(77, 3, 48, L, s)[0]
+    c = oracle.sog_sample(77, 4, 48, L, s)[0]
+    assert (a == b).all() and not (a == c).all()
+
+
+def test_sampler_marginal_chi_square():
+    L = [12.0, 12.0, 12.0]
+    s = oracle.sog_setup(0, 3.0, L)
+    P = 64
+    samples = np.concatenate([oracle.sog_sample(2024, c, P, L, s)[0] for c in range(150)])
+    # target: p(m) ~ f(k) k^2 on |m|^2 > 1; marginal of m_x by brute force
+    M = 25
+    r = np.arange(-M, M + 1)
+    mx, my, mz = np.meshgrid(r, r, r, indexing="ij")
+    k2 = sum((2 * np.pi * mm / Ld) ** 2 for mm, Ld in zip((mx, my, mz), L))
+    f = np.zeros_like(k2)
+    for l in range(s["mmax"]):
+        f += s["coef"][l] * np.exp(-s["b"] ** (2 * l) * s["sigma"] ** 2 * k2 / 2)
+    w = f * k2 * ((mx ** 2 + my ** 2 + mz ** 2) > 1)
+    px = w.sum(axis=(1, 2))
+    px /= px.sum()
+    vals = np.arange(-M, M + 1)
+    obs = np.array([(samples[:, 0] == v).sum() for v in vals], dtype=float)
+    exp = px * len(samples)
+    sel = exp > 20
+    chi2 = ((obs[sel] - exp[sel]) ** 2 / exp[sel]).sum()
+    dof = sel.sum() - 1
+    # consecutive kept samples are correlated; allow a generous margin
+    assert chi2 < 4 * dof + 40, (chi2, dof)
