@@ -206,6 +206,26 @@ int rbmdx_rbe_sample_host(double alpha, const double* L, int32_t P,
                           int32_t* m_out);
 int rbmdx_check_config(const char* json_text, char* err, int32_t errlen);
 
+/*
+ * rbmdx_sog_setup_host: the RBSOG far-field set-up for coulomb.rbsog_level
+ * `level`, neighbor.cut_off and box lengths L[3] (replaces
+ * KSpaceCalculator::RBSOGInit / RBSOGSetup, src/force/src/kspace_calculator.cpp:
+ * 988-1140): scal = {b, sigma, w0, S, S_npt, mmax}, coef / coef_npt receive
+ * mmax values (at most 128).  exact = 0 reproduces the reference's float
+ * set-up; 1 the corrected one (extension key coulomb.rbsog_exact).
+ *
+ * rbmdx_sog_sample_host: the RBSOG k-sample of force evaluation call_index
+ * (RBSOGSampleKSpace, kspace_calculator.cpp:1142-1329, seeded from the same
+ * per-evaluation seed stream as RBE): m_out holds P*3 int32, idx_out the P
+ * indices of the virial chain.
+ */
+int rbmdx_sog_setup_host(int32_t level, double cut_off, const double* L,
+                         int32_t exact, double* scal, double* coef,
+                         double* coef_npt);
+int rbmdx_sog_sample_host(int32_t level, double cut_off, const double* L,
+                          int32_t exact, int32_t P, uint32_t base_seed,
+                          int64_t call_index, int32_t* m_out, int32_t* idx_out);
+
 /* Per-phase GPU times ("rebuild", "pair", "kspace", "assemble"), averaged
  * over the calls timed since the previous rbmdx_kernel_times: calls made with
  * RBMDX_FLAG_TIMING, or every call when the context was created with

@@ -109,6 +109,10 @@ SIGNATURES = {
     "rbmdx_rbe_sample_host": (ctypes.c_int, [ctypes.c_double, _DP, _I32, ctypes.c_uint32,
                                              _I64, _IP]),
     "rbmdx_check_config": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _I32]),
+    "rbmdx_sog_setup_host": (ctypes.c_int, [_I32, ctypes.c_double, _DP, _I32, _DP, _DP,
+                                            _DP]),
+    "rbmdx_sog_sample_host": (ctypes.c_int, [_I32, ctypes.c_double, _DP, _I32, _I32,
+                                             ctypes.c_uint32, _I64, _IP, _IP]),
 }
 
 _lib = None
@@ -149,6 +153,36 @@ def rbe_sample_host(alpha, L, P, base_seed, call_index):
     if rc != 0:
         raise RbmdxError(rc, "rbmdx_rbe_sample_host failed")
     return m.reshape(P, 3)
+
+
+def sog_setup_host(level, cut_off, L, exact=False):
+    """RBSOG set-up: dict(b, sigma, w0, S, S_npt, mmax, coef, coef_npt)."""
+    lib = load_library()
+    Ld = np.ascontiguousarray(L, dtype=np.float64)
+    scal = np.zeros(6)
+    coef = np.zeros(128)
+    coefn = np.zeros(128)
+    rc = lib.rbmdx_sog_setup_host(int(level), float(cut_off), _dptr(Ld), 1 if exact else 0,
+                                  _dptr(scal), _dptr(coef), _dptr(coefn))
+    if rc != 0:
+        raise RbmdxError(rc, "rbmdx_sog_setup_host failed")
+    M = int(scal[5])
+    return {"b": scal[0], "sigma": scal[1], "w0": scal[2], "S": scal[3], "S_npt": scal[4],
+            "mmax": M, "coef": coef[:M].copy(), "coef_npt": coefn[:M].copy()}
+
+
+def sog_sample_host(level, cut_off, L, P, base_seed, call_index, exact=False):
+    """RBSOG k-sample of evaluation call_index: (m (P,3), idx (P,))."""
+    lib = load_library()
+    Ld = np.ascontiguousarray(L, dtype=np.float64)
+    m = np.zeros(3 * P, dtype=np.int32)
+    idx = np.zeros(P, dtype=np.int32)
+    rc = lib.rbmdx_sog_sample_host(int(level), float(cut_off), _dptr(Ld), 1 if exact else 0,
+                                   int(P), int(base_seed) & 0xFFFFFFFF, int(call_index),
+                                   _iptr(m), _iptr(idx))
+    if rc != 0:
+        raise RbmdxError(rc, "rbmdx_sog_sample_host failed")
+    return m.reshape(P, 3), idx
 
 
 def check_config(config):

@@ -48,10 +48,12 @@ def build(verbose=False):
                                os.path.join(CSRC, "api.cpp")],
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "sampler.o"),
                                os.path.join(CSRC, "sampler.cpp")],
+        ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "sog.o"),
+                               os.path.join(CSRC, "sog.cpp")],
     ]
-    with ThreadPoolExecutor(max_workers=3) as ex:
+    with ThreadPoolExecutor(max_workers=4) as ex:
         list(ex.map(_run, jobs))
-    objs = [os.path.join(BUILD, f) for f in ("kernels.o", "api.o", "sampler.o")]
+    objs = [os.path.join(BUILD, f) for f in ("kernels.o", "api.o", "sampler.o", "sog.o")]
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
          + ["-lpthread"])

@@ -26,6 +26,7 @@
 #include "json.hpp"
 #include "rbmdx.h"
 #include "sampler.hpp"
+#include "sog.hpp"
 
 using namespace rbmdx;
 
@@ -1169,6 +1170,41 @@ int rbmdx_rbe_sample_host(double alpha, const double* L, int32_t P,
     const float Lf[3] = {(float)L[0], (float)L[1], (float)L[2]};
     RbeSampler s((float)alpha, Lf, P);
     s.draw(rbe_seed_of_call(base_seed, call_index), m_out);
+  });
+}
+
+int rbmdx_sog_setup_host(int32_t level, double cut_off, const double* L,
+                         int32_t exact, double* scal, double* coef,
+                         double* coef_npt) {
+  SogPresetValues pv;
+  if (!L || !scal || !coef || !coef_npt || !sog_preset(level, &pv) ||
+      !(cut_off > 0))
+    return RBMDX_E_ARG;
+  return guarded(nullptr, [&] {
+    const SogSet s = sog_set(level, cut_off, L, 1, exact != 0);
+    const double v[6] = {s.b, s.sigma, s.w0, s.S, s.S_npt, (double)s.mmax};
+    for (int q = 0; q < 6; ++q) scal[q] = v[q];
+    for (int l = 0; l < s.mmax; ++l) {
+      coef[l] = s.coef[l];
+      coef_npt[l] = s.coef_npt[l];
+    }
+  });
+}
+
+int rbmdx_sog_sample_host(int32_t level, double cut_off, const double* L,
+                          int32_t exact, int32_t P, uint32_t base_seed,
+                          int64_t call_index, int32_t* m_out, int32_t* idx_out) {
+  SogPresetValues pv;
+  if (!L || !m_out || !idx_out || P <= 0 || call_index < 0 ||
+      !sog_preset(level, &pv) || !(cut_off > 0))
+    return RBMDX_E_ARG;
+  return guarded(nullptr, [&] {
+    const SogSet s = sog_set(level, cut_off, L, 1, exact != 0);
+    SogSampler smp(s, L, P);
+    std::vector<int32_t> out((size_t)smp.out_size());
+    smp.draw(rbe_seed_of_call(base_seed, call_index), out.data());
+    std::copy(out.begin(), out.begin() + 3 * P, m_out);
+    std::copy(out.begin() + 3 * P, out.end(), idx_out);
   });
 }
 

@@ -104,7 +104,7 @@ uint32_t rbe_seed_of_call(uint32_t base_seed, int64_t call_index) {
   return (uint32_t)base();
 }
 
-SamplePipeline::SamplePipeline(const RbeSampler* s, uint32_t base_seed,
+SamplePipeline::SamplePipeline(const KSampler* s, uint32_t base_seed,
                                int threads, int window)
     : sampler_(s), base_(base_seed), window_(window) {
   if (threads < 1) threads = 1;
@@ -121,7 +121,7 @@ SamplePipeline::~SamplePipeline() {
 }
 
 void SamplePipeline::worker() {
-  std::vector<int32_t> buf((size_t)sampler_->P() * 3);
+  std::vector<int32_t> buf((size_t)sampler_->out_size());
   for (;;) {
     int64_t call;
     uint32_t seed;

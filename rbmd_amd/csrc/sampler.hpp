@@ -25,11 +25,20 @@
 
 namespace rbmdx {
 
-class RbeSampler {
+// A per-evaluation sampler: draw(seed) writes out_size() integers.
+class KSampler {
+ public:
+  virtual ~KSampler() = default;
+  virtual void draw(uint32_t seed, int32_t* out) const = 0;
+  virtual int out_size() const = 0;
+};
+
+class RbeSampler : public KSampler {
  public:
   RbeSampler(float alpha, const float L[3], int P);
   // The k-sample of one evaluation started from `seed`; m_out holds P*3.
-  void draw(uint32_t seed, int32_t* m_out) const;
+  void draw(uint32_t seed, int32_t* m_out) const override;
+  int out_size() const override { return 3 * P_; }
   int P() const { return P_; }
 
  private:
@@ -56,7 +65,7 @@ uint32_t rbe_seed_of_call(uint32_t base_seed, int64_t call_index);
 // Draws samples for upcoming evaluations on worker threads.
 class SamplePipeline {
  public:
-  SamplePipeline(const RbeSampler* s, uint32_t base_seed, int threads,
+  SamplePipeline(const KSampler* s, uint32_t base_seed, int threads,
                  int window);
   ~SamplePipeline();
   // Blocks until the sample of evaluation `call` is ready, copies it out.
@@ -64,7 +73,7 @@ class SamplePipeline {
 
  private:
   void worker();
-  const RbeSampler* sampler_;
+  const KSampler* sampler_;
   std::mt19937 base_;
   int64_t next_seeded_ = 0;     // next call index whose seed is drawn
   int64_t consumer_ = 0;        // next call the consumer will take

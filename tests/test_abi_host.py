@@ -120,3 +120,31 @@ def test_sampler_other_boxes():
         b = oracle.rbe_sample(77, 3, P, alpha, L)
         np.testing.assert_array_equal(a, b)
         assert not (a == 0).all(1).any()
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("exact", [False, True])
+def test_sog_setup_equals_oracle(level, exact):
+    L = [62.99605249474366, 60.0, 61.5]
+    a = rbmd_amd.sog_setup_host(level, 3.5, L, exact=exact)
+    b = oracle.sog_setup(level, 3.5, L, exact=exact)
+    assert a["mmax"] == b["mmax"]
+    for key in ("b", "sigma", "w0", "S", "S_npt"):
+        if exact:
+            assert a[key] == pytest.approx(b[key], rel=1e-12), key
+        else:
+            assert a[key] == b[key], key  # the same float arithmetic
+    np.testing.assert_allclose(a["coef"], b["coef"], rtol=0 if not exact else 1e-13)
+    np.testing.assert_allclose(a["coef_npt"], b["coef_npt"], rtol=0 if not exact else 1e-13)
+
+
+@pytest.mark.parametrize("level,P,seed", [(0, 64, 20261202), (1, 512, 7), (3, 48, 0)])
+def test_sog_sampler_stream_equals_oracle(level, P, seed):
+    L = [29.24, 31.0, 27.5]
+    for exact in (False, True):
+        s = oracle.sog_setup(level, 3.5, L, exact=exact)
+        for call in (0, 1, 5):
+            m, idx = rbmd_amd.sog_sample_host(level, 3.5, L, P, seed, call, exact=exact)
+            mo, io = oracle.sog_sample(seed, call, P, L, s)
+            np.testing.assert_array_equal(m, mo)
+            np.testing.assert_array_equal(idx, io)
