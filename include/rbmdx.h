@@ -157,12 +157,16 @@ int rbmdx_force_leapfrog(rbmdx_ctx* ctx, int64_t step, void* px, void* py,
 int rbmdx_thermo(rbmdx_ctx* ctx, rbmdx_thermo_t* out);
 
 /*
- * Host copy of the RBE k-sample the seeded stream yields for force evaluation
- * `call_index` (0 = first rbmdx_force call).  m_out holds P*3 int32.
+ * Host copy of the k-sample the seeded stream yields for force evaluation
+ * `call_index` (0 = first rbmdx_force call).  RBE: m_out holds P*3 int32.
+ * RBSOG: m_out holds P*4 int32, the P m-vectors followed by the P indices of
+ * the virial chain (the same layout rbmdx_force accepts as m_samples with
+ * num_k = P).
  */
 int rbmdx_rbe_sample(rbmdx_ctx* ctx, int64_t call_index, int32_t* m_out);
 
-/* Number of RBE samples P per evaluation (0 if no RBE). */
+/* Number of sampled k-vectors P per evaluation (RBE, RBSOG; 0 otherwise).
+ * For RBSOG this is coulomb_sample_num rounded up to a multiple of 16. */
 int32_t rbmdx_num_samples(const rbmdx_ctx* ctx);
 
 /*

@@ -302,10 +302,14 @@ class ForceContext:
     def num_samples(self):
         return int(self.lib.rbmdx_num_samples(self._h))
 
-    def rbe_sample(self, call_index):
+    def rbe_sample(self, call_index, sog=False):
+        """k-sample of evaluation call_index: (P,3) m-vectors; with sog=True
+        (an RBSOG context) the pair (m (P,3), virial-chain indices (P,))."""
         P = self.num_samples()
-        m = np.zeros(3 * P, dtype=np.int32)
+        m = np.zeros((4 if sog else 3) * P, dtype=np.int32)
         self._check(self.lib.rbmdx_rbe_sample(self._h, int(call_index), _iptr(m)))
+        if sog:
+            return m[:3 * P].reshape(P, 3), m[3 * P:].copy()
         return m.reshape(P, 3)
 
     def _export(self, fn, *args, cap):

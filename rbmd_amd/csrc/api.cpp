@@ -50,11 +50,13 @@ struct Error : std::runtime_error {
 // ---------------------------------------------------------------------------
 struct Config {
   std::string force_type;     // "LJ/CUT" | "LJ/CUT/COUL/LONG"
-  std::string neighbor_type;  // "RBL" | "Verlet-List"
+  std::string neighbor_type;  // "RBL" | "Verlet-List" | "VERLET-SOG"
   double cut_off = 0, skin = 0, r_core = 0, lj_cut_off = 0;
   int sample_num = 0;         // neighbor_sample_num
   bool energy_rbl = false;
-  std::string coulomb_type;   // "" | "RBE" | "EWALD"
+  std::string coulomb_type;   // "" | "RBE" | "EWALD" | "RBSOG"
+  int rbsog_level = -1;       // coulomb.rbsog_level (RBSOG, VERLET-SOG)
+  bool rbsog_exact = false;   // extension: corrected SOG set-up
   bool has_alpha = false;
   double alpha = 0, accuracy = 0;
   int coulomb_sample_num = 0;
@@ -111,7 +113,8 @@ Config parse_config(const std::string& text) {
   const Json* nb = hp->get("neighbor");
   if (!nb) throw Error(RBMDX_E_CONFIG, "hyper_parameters.neighbor missing");
   c.neighbor_type = str_of(nb->get("type"), "neighbor.type");
-  if (c.neighbor_type != "RBL" && c.neighbor_type != "Verlet-List")
+  if (c.neighbor_type != "RBL" && c.neighbor_type != "Verlet-List" &&
+      c.neighbor_type != "VERLET-SOG")
     throw Error(RBMDX_E_CONFIG,
                 "neighbor.type not supported by this build: " + c.neighbor_type);
   c.cut_off = num_of(nb->get("cut_off"), "neighbor.cut_off");
@@ -151,14 +154,42 @@ Config parse_config(const std::string& text) {
   if (c.force_type == "LJ/CUT/COUL/LONG") {
     if (!cl) throw Error(RBMDX_E_CONFIG, "LJ/CUT/COUL/LONG needs hyper_parameters.coulomb");
     c.coulomb_type = str_of(cl->get("type"), "coulomb.type");
-    if (c.coulomb_type != "RBE" && c.coulomb_type != "EWALD")
+    if (c.coulomb_type != "RBE" && c.coulomb_type != "EWALD" &&
+        c.coulomb_type != "RBSOG")
       throw Error(RBMDX_E_CONFIG,
                   "coulomb.type not supported by this build: " + c.coulomb_type);
+    // RBSOG: a level preset replaces accuracy / rbsog_* keys
+    // (kspace_calculator.cpp:254-300); VERLET-SOG reads the same level
+    // (lj_cut_coul_kspace.cpp:651-665)
+    const bool sog = c.coulomb_type == "RBSOG" || c.neighbor_type == "VERLET-SOG";
+    if (sog) {
+      const Json* lv = cl->get("rbsog_level");
+      if (!lv) throw Error(RBMDX_E_CONFIG, "RBSOG requires hyper_parameters.coulomb.rbsog_level");
+      for (const char* legacy : {"accuracy", "rbsog_b", "rbsog_sigma", "rbsog_Mmax",
+                                 "rbsog_Kcut"})
+        if (cl->get(legacy))
+          throw Error(RBMDX_E_CONFIG, std::string("Legacy RBSOG config key '") + legacy +
+                                          "' is not supported. Use "
+                                          "hyper_parameters.coulomb.rbsog_level instead.");
+      const double lvv = num_of(lv, "coulomb.rbsog_level");
+      if (!(lvv >= 0 && lvv <= 4) || lvv != std::floor(lvv))
+        throw Error(RBMDX_E_CONFIG, "RBSOG rbsog_level must be in [0, 4]");
+      c.rbsog_level = (int)lvv;
+      if (const Json* ex = cl->get("rbsog_exact")) {
+        if (ex->kind != Json::Bool)
+          throw Error(RBMDX_E_CONFIG, "coulomb.rbsog_exact must be true or false");
+        c.rbsog_exact = ex->b;
+      }
+    }
     if (const Json* a = cl->get("alpha")) {
       c.has_alpha = true;
       c.alpha = num_of(a, "coulomb.alpha");
       if (!(c.alpha > 0))
         throw Error(RBMDX_E_CONFIG, "coulomb alpha override must be > 0");
+    } else if (sog) {
+      SogPresetValues pv;
+      sog_preset(c.rbsog_level, &pv);
+      c.accuracy = pv.accuracy;  // DeriveAlphaFromAccuracy(target_accuracy)
     } else {
       const Json* acc = cl->get("accuracy");
       if (!acc)
@@ -173,6 +204,13 @@ Config parse_config(const std::string& text) {
         throw Error(RBMDX_E_CONFIG, "'kmax' must contain exactly 3 integers");
       c.has_kmax = true;
       for (int d = 0; d < 3; ++d) c.kmax[d] = (int)num_of(&k->arr[d], "kmax");
+    }
+    if (c.coulomb_type == "RBSOG") {
+      const int P =
+          (int)num_of(cl->get("coulomb_sample_num"), "coulomb.coulomb_sample_num");
+      if (P <= 0)
+        throw Error(RBMDX_E_CONFIG, "the RBSOG sample number must be greater than 0");
+      c.coulomb_sample_num = ((P + 15) / 16) * 16;  // kspace_calculator.cpp:300
     }
     if (c.coulomb_type == "RBE") {
       c.coulomb_sample_num =
@@ -320,11 +358,13 @@ class Engine : public EngineBase {
   }
 
   int num_samples() const override {
-    return cfg_.coulomb_type == "RBE" ? cfg_.coulomb_sample_num : 0;
+    return (cfg_.coulomb_type == "RBE" || cfg_.coulomb_type == "RBSOG")
+               ? cfg_.coulomb_sample_num
+               : 0;
   }
 
   void rbe_sample(int64_t call, int32_t* m) override {
-    if (cfg_.coulomb_type != "RBE") throw Error(RBMDX_E_STATE, "no RBE k-space");
+    if (!sampler_) throw Error(RBMDX_E_STATE, "no sampled k-space");
     sampler_->draw(rbe_seed_of_call(base_seed_, call), m);
   }
 
@@ -384,27 +424,59 @@ class Engine : public EngineBase {
     StepArgs<R> a = args(step, flags);
     a.force_rebuild = force_rebuild;
     // k-sample of this evaluation
-    if (kspace_ == 1) {
+    if (kspace_ == 1 || kspace_ == 3) {
+      const bool sog = kspace_ == 3;
+      const int P = num_samples();
       if (m) {
-        if (nk <= 0 || nk > d_.kcap)
-          throw Error(RBMDX_E_ARG, "num_k out of range for explicit samples");
-        a.m_use = m;
-        a.p.num_k = nk;
+        if (!sog) {
+          if (nk <= 0 || nk > d_.kcap)
+            throw Error(RBMDX_E_ARG, "num_k out of range for explicit samples");
+          a.m_use = m;
+          a.p.num_k = nk;
+        } else {
+          // RBSOG: P m-vectors followed by P virial-chain indices; the direct
+          // vectors stay in place behind them
+          if (nk != P)
+            throw Error(RBMDX_E_ARG, "RBSOG explicit samples must hold exactly P vectors");
+          HIPCHK(hipMemcpyAsync(d_.m_dev, m, sizeof(int32_t) * 3 * P,
+                                hipMemcpyDeviceToDevice, s));
+          HIPCHK(hipMemcpyAsync(d_.src_dev, m + 3 * P, sizeof(int32_t) * P,
+                                hipMemcpyDeviceToDevice, s));
+          a.m_use = d_.m_dev;
+          a.src_use = d_.src_dev;
+          a.p.num_k = d_.kcap;
+        }
       } else {
         const int slot = (int)(calls_ % kSlots);
         if (slot_used_[slot]) HIPCHK(hipEventSynchronize(slot_events_[slot]));
-        int32_t* dst = stage_ + (size_t)slot * 3 * d_.kcap;
-        const int32_t* dst_dev = stage_dev_ + (size_t)slot * 3 * d_.kcap;
+        int32_t* dst = stage_ + (size_t)slot * stage_stride_;
+        const int32_t* dst_dev = stage_dev_ + (size_t)slot * stage_stride_;
         if (!pipeline_) {
           const char* th = std::getenv("RBMDX_SAMPLER_THREADS");
           const int nth = th ? std::max(1, std::atoi(th)) : 4;
           pipeline_.reset(new SamplePipeline(sampler_.get(), base_seed_, nth, 64));
           pipeline_call_ = 0;
         }
-        pipeline_->take(pipeline_call_++, dst);
+        if (!sog) {
+          pipeline_->take(pipeline_call_++, dst);
+          a.p.num_k = P;
+        } else {
+          // slot layout: m of the P samples and the direct vectors, then the
+          // virial-chain source of every k (direct ones are their own)
+          const int K = d_.kcap;
+          sog_buf_.resize((size_t)4 * P);
+          pipeline_->take(pipeline_call_++, sog_buf_.data());
+          std::copy(sog_buf_.begin(), sog_buf_.begin() + 3 * P, dst);
+          std::copy(sog_direct_m_.begin(), sog_direct_m_.end(), dst + 3 * P);
+          int32_t* src = dst + 3 * K;
+          std::copy(sog_buf_.begin() + 3 * P, sog_buf_.end(), src);
+          for (int e = P; e < K; ++e) src[e] = e;
+          a.src_host = dst_dev + 3 * K;
+          a.src_use = d_.src_dev;
+          a.p.num_k = K;
+        }
         a.m_host = dst_dev;
         a.m_use = d_.m_dev;
-        a.p.num_k = num_samples();
         stage_slot_ = slot;
       }
     } else if (kspace_ == 2) {
@@ -563,12 +635,18 @@ class Engine : public EngineBase {
   Dev<R> d_{};
   R* lj_tab_ = nullptr;
   std::vector<void*> allocs_;
-  std::unique_ptr<RbeSampler> sampler_;
+  std::unique_ptr<KSampler> sampler_;
   std::unique_ptr<SamplePipeline> pipeline_;
   int64_t pipeline_call_ = 0;
   uint32_t base_seed_ = 0;
   int32_t* stage_ = nullptr;       // host view of the k-sample staging ring
   int32_t* stage_dev_ = nullptr;   // device view of the same memory
+  size_t stage_stride_ = 0;        // ints per slot
+  SogSet sog_;                     // RBSOG / VERLET-SOG set-up
+  bool sog_ready_ = false;
+  int sog_ndirect_ = 0;
+  std::vector<int> sog_direct_m_;
+  std::vector<int32_t> sog_buf_;
   Ctrl* ctrl_host_ = nullptr;      // host view of the control mirror
   hipEvent_t slot_events_[kSlots] = {};
   bool slot_used_[kSlots] = {};
@@ -610,7 +688,9 @@ class Engine : public EngineBase {
     params_.sum_q = sum_q_;
     params_.sum_q2 = sum_q2_;
     const double V = L_[0] * L_[1] * L_[2];
-    if (kspace_)
+    if (kspace_ == 3)  // RBSOG (kspace_calculator.cpp:1524-1528)
+      e_self_ = sog_ready_ ? sog_self_energy(sog_, sum_q2_, cfg_.qqr2e) : 0.0;
+    else if (kspace_)
       e_self_ = -cfg_.qqr2e * (std::sqrt(alpha_ / M_PI) * sum_q2_ +
                                0.5 * M_PI * sum_q_ * sum_q_ / (alpha_ * V));
   }
@@ -625,7 +705,10 @@ class Engine : public EngineBase {
     HIPCHK(hipSetDevice(device_));
     const Config& c = cfg_;
     const bool coulomb = c.force_type == "LJ/CUT/COUL/LONG";
-    kspace_ = coulomb ? (c.coulomb_type == "RBE" ? 1 : 2) : 0;
+    kspace_ = !coulomb ? 0
+              : c.coulomb_type == "RBE" ? 1
+              : c.coulomb_type == "EWALD" ? 2
+                                          : 3;
     // charged = nonzero in the device precision (the kernels test (R)q != 0)
     int nc = 0;
     if (coulomb)
@@ -754,6 +837,28 @@ class Engine : public EngineBase {
       pc_.fmask = (freq_ >= 2 && (freq_ & (freq_ - 1)) == 0) ? (unsigned)(freq_ - 1) : 0u;
       pc_.c32 = freq_ >= 2 ? (unsigned)((1ULL << 32) % (unsigned long long)freq_) : 0u;
       pc_.ntypes = ntypes_;
+      pc_.sog = 0;
+      for (int q = 0; q < 6; ++q) pc_.T[q] = (R)0;
+    }
+    // SOG far field (RBSOG k-space) and, for VERLET-SOG, its near field
+    if (c.rbsog_level >= 0) {
+      sog_ = sog_set(c.rbsog_level, rc, L_, 1, c.rbsog_exact);
+      sog_ready_ = true;
+      if (c.neighbor_type == "VERLET-SOG") {
+        const SogNear nf = sog_near(sog_);
+        pc_.sog = 1;
+        for (int q = 0; q < 6; ++q) pc_.T[q] = (R)nf.T[q];
+        std::vector<R> tab(2 * nf.omega.size());
+        for (size_t l = 0; l < nf.omega.size(); ++l) {
+          tab[2 * l] = (R)nf.omega[l];
+          tab[2 * l + 1] = (R)nf.inv2s2[l];
+        }
+        R* dt = alloc<R>(tab.size());
+        HIPCHK(hipMemcpy(dt, tab.data(), sizeof(R) * tab.size(), hipMemcpyHostToDevice));
+        pc_.far_tab = dt;
+        pc_.far_m = (int)nf.omega.size();
+      }
+      charge_sums();
     }
     params_.rbl = c.neighbor_type == "RBL" ? 1 : 0;
     params_.coulomb = coulomb ? 1 : 0;
@@ -845,7 +950,60 @@ class Engine : public EngineBase {
       HIPCHK(hipMemcpy(lj_tab_, t.data(), sizeof(R) * t.size(), hipMemcpyHostToDevice));
     }
     // ---- k-space
-    if (kspace_ == 1) {
+    if (kspace_ == 3) {
+      // RBSOG: P sampled vectors (weights S/(4 pi P)) followed by the direct
+      // vectors |m|^2 <= Kcut (weights f(k) k^2 / (4 pi)), so that every k
+      // enters the force and energy in the RBE form
+      // E = qqr2e (2 pi / V) w |rho|^2 / k^2 (kspace_calculator.cpp:1331-1543)
+      const int P = c.coulomb_sample_num;
+      const double qqr2e = c.qqr2e;
+      std::vector<int> dm;
+      for (int i = -1; i <= 1; ++i)
+        for (int j = -1; j <= 1; ++j)
+          for (int kk = -1; kk <= 1; ++kk)
+            if ((i || j || kk) && i * i + j * j + kk * kk <= sog_.kcut) {
+              dm.push_back(i);
+              dm.push_back(j);
+              dm.push_back(kk);
+            }
+      sog_ndirect_ = (int)dm.size() / 3;
+      d_.kcap = P + sog_ndirect_;
+      std::vector<double> w(d_.kcap);
+      std::vector<double2> vir(d_.kcap);
+      for (int k = 0; k < P; ++k) {
+        w[k] = sog_.S / (4 * M_PI * P);
+        vir[k] = make_double2(-(sog_.S_npt / P) * qqr2e / (4.0 * V), 1.0);
+      }
+      sog_direct_m_.assign(dm.begin(), dm.end());
+      for (int e = 0; e < sog_ndirect_; ++e) {
+        double k2 = 0;
+        for (int d = 0; d < 3; ++d) {
+          const double kd = 2 * M_PI * dm[3 * e + d] / L_[d];
+          k2 += kd * kd;
+        }
+        w[P + e] = sog_f(sog_, sog_.coef, k2) * k2 / (4 * M_PI);
+        vir[P + e] = make_double2(-qqr2e / (4.0 * V) * sog_f(sog_, sog_.coef_npt, k2), 0.0);
+      }
+      rbe_S_ = sog_.S;
+      sampler_.reset(new SogSampler(sog_, L_, P));
+      if (c.has_seed) {
+        base_seed_ = c.seed;
+      } else {
+        std::random_device rd;
+        base_seed_ = rd();
+      }
+      alloc_kspace(w, nullptr);
+      d_.kvir = alloc<double2>(d_.kcap);
+      d_.src_dev = alloc<int>(d_.kcap);
+      HIPCHK(hipMemcpy(d_.kvir, vir.data(), sizeof(double2) * d_.kcap, hipMemcpyHostToDevice));
+      // the direct vectors and their (own) virial sources stay resident
+      HIPCHK(hipMemcpy(d_.m_dev + 3 * P, dm.data(), sizeof(int) * dm.size(),
+                       hipMemcpyHostToDevice));
+      std::vector<int> src(d_.kcap);
+      for (int e = 0; e < d_.kcap; ++e) src[e] = e;
+      HIPCHK(hipMemcpy(d_.src_dev, src.data(), sizeof(int) * d_.kcap, hipMemcpyHostToDevice));
+      alloc_stage(4);
+    } else if (kspace_ == 1) {
       const int P = c.coulomb_sample_num;
       d_.kcap = std::max(P, 1);
       // S = prod_d H_d - 1 (kspace_calculator.cpp:68-78), in FP64
@@ -867,15 +1025,7 @@ class Engine : public EngineBase {
       }
       std::vector<double> w(d_.kcap, rbe_S_ / P);
       alloc_kspace(w, nullptr);
-      void* hp = nullptr;
-      HIPCHK(hipHostMalloc(&hp, sizeof(int32_t) * 3 * (size_t)d_.kcap * kSlots,
-                           hipHostMallocMapped));
-      stage_ = (int32_t*)hp;
-      void* dp = nullptr;
-      HIPCHK(hipHostGetDevicePointer(&dp, hp, 0));
-      stage_dev_ = (int32_t*)dp;
-      for (int k = 0; k < kSlots; ++k)
-        HIPCHK(hipEventCreateWithFlags(&slot_events_[k], hipEventDisableTiming));
+      alloc_stage(3);
     } else if (kspace_ == 2) {
       int km[3];
       if (c.has_kmax) {
@@ -914,7 +1064,9 @@ class Engine : public EngineBase {
       alloc_kspace(w, &mv);
       params_.num_k = d_.kcap;
     }
-    params_.num_k = kspace_ == 1 ? c.coulomb_sample_num : params_.num_k;
+    params_.num_k = kspace_ == 1 ? c.coulomb_sample_num
+                    : kspace_ == 3 ? d_.kcap
+                                   : params_.num_k;
     timing_ = std::getenv("RBMDX_TIMING") && std::atoi(std::getenv("RBMDX_TIMING"));
     // k-space beside the pair kernel on a second stream (RBMDX_KSTREAM=0:
     // same stream, in order)
@@ -926,6 +1078,21 @@ class Engine : public EngineBase {
     }
     HIPCHK(hipDeviceSynchronize());
     ready_ = true;
+  }
+
+  // Host-mapped ring of per-evaluation k-samples: ints_per_k values per k
+  // (m, or m then the virial-chain index for RBSOG), kSlots evaluations.
+  void alloc_stage(int ints_per_k) {
+    stage_stride_ = ints_per_k * (size_t)d_.kcap;
+    void* hp = nullptr;
+    HIPCHK(hipHostMalloc(&hp, sizeof(int32_t) * stage_stride_ * kSlots,
+                         hipHostMallocMapped));
+    stage_ = (int32_t*)hp;
+    void* dp = nullptr;
+    HIPCHK(hipHostGetDevicePointer(&dp, hp, 0));
+    stage_dev_ = (int32_t*)dp;
+    for (int k = 0; k < kSlots; ++k)
+      HIPCHK(hipEventCreateWithFlags(&slot_events_[k], hipEventDisableTiming));
   }
 
   void alloc_kspace(const std::vector<double>& w, const std::vector<int>* mv) {

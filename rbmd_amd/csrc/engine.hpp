@@ -75,6 +75,12 @@ struct PairConst {
   R alpha, g, qqr2e, two_g_over_sqrtpi;
   R freq;             // RBL selection frequency as a multiplier
   int ifreq;
+  // SOG near field (VERLET-SOG) instead of erfc: force polynomial T[0..5] in
+  // r^2 and, for energies, far_m Gaussians (omega_l, 1/(2 s_l^2)) in far_tab
+  int sog = 0;
+  R T[6];
+  const R* far_tab = nullptr;
+  int far_m = 0;
   unsigned fmask;     // ifreq - 1 when ifreq is a power of two, else 0
   unsigned c32;       // 2^32 mod ifreq
   long long epoch;
@@ -142,7 +148,9 @@ struct Dev {
   // k-space
   int kcap = 0;
   int* m_dev = nullptr;          // [kcap*3]
+  int* src_dev = nullptr;        // [kcap] virial-tensor source k (RBSOG)
   double* kw = nullptr;          // [kcap] per-k weight
+  double2* kvir = nullptr;       // [kcap] (tau, divide by |k_src|^4) (RBSOG)
   double2* sk_part = nullptr;    // [chunks * kcap]
   float4* kf0 = nullptr;         // [kcap] (mx, my, mz, A Re rho)
   float4* kf1 = nullptr;         // [kcap] (kx, ky, kz, A Im rho)
@@ -159,7 +167,7 @@ struct Dev {
 struct Params {
   int rbl = 1;          // RBL split (else full Verlet list)
   int coulomb = 0;      // real-space erfc + k-space present
-  int kspace = 0;       // 0 none, 1 RBE, 2 EWALD
+  int kspace = 0;       // 0 none, 1 RBE, 2 EWALD, 3 RBSOG
   int eflag = 0;
   int mean_fix = 0;
   int split = 0;        // charged/neutral partition of the pair work
@@ -182,6 +190,8 @@ struct StepArgs {
   const R* lj_tab = nullptr;  // [ntypes^2 * 2] (eps_ij, sigma_ij^2), device
   const int* m_use = nullptr; // [num_k * 3] k-vectors for this call, device
   const int* m_host = nullptr;// host-mapped staging copied into m_dev, or null
+  const int* src_host = nullptr;  // RBSOG: host-mapped virial-chain sources
+  const int* src_use = nullptr;   // RBSOG: device virial-chain sources
   unsigned int rebuild_thr;   // float bits of (skin/2)^2
   int force_rebuild = 0;
   int count_only = 0;         // list builders only count (capacity planning)

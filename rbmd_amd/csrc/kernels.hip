@@ -904,6 +904,27 @@ __device__ __forceinline__ void coul(const PairConst<R>& pc, R d2, R qq, R& f,
   e = R(0.5) * pc.qqr2e * qq * ec / r;
 }
 
+// SOG near-field Coulomb (VERLET-SOG, lj_cut_coul_kspace_op.hip.cu:196-252):
+// f = -qqr2e qq (1/r^3 + sum_n T_n r^{2n}) and, when energies are wanted,
+// e = 1/2 qqr2e qq (1/r - sum_l omega_l exp(-r^2 / (2 s_l^2))).
+template <typename R, bool EFLAG>
+__device__ __forceinline__ void coul_sog(const PairConst<R>& pc, R d2, R qq,
+                                         R& f, R& e) {
+  const R rinv = R(1) / sqrt(d2);
+  const R r3inv = rinv / d2;
+  R poly = pc.T[5];
+#pragma unroll
+  for (int n = 4; n >= 0; --n) poly = fma_r(poly, d2, pc.T[n]);
+  f = -pc.qqr2e * qq * (r3inv + poly);
+  e = R(0);
+  if (EFLAG) {
+    R sume = R(0);
+    for (int l = 0; l < pc.far_m; ++l)
+      sume += pc.far_tab[2 * l] * exp(-d2 * pc.far_tab[2 * l + 1]);
+    e = R(0.5) * pc.qqr2e * qq * (rinv - sume);
+  }
+}
+
 // LJ + Coulomb pair of list B (both atoms charged).
 template <typename R, bool RBL, bool EFLAG>
 __device__ __forceinline__ void interact_b(const PairConst<R>& pc,
@@ -927,7 +948,10 @@ __device__ __forceinline__ void interact_b(const PairConst<R>& pc,
     elj = R(2) * eps * (s6 - R(1)) * s6;
   }
   R fc, ec;
-  coul<R>(pc, d2, xi.w * xj.w, fc, ec);
+  if (pc.sog)
+    coul_sog<R, EFLAG>(pc, d2, xi.w * xj.w, fc, ec);
+  else
+    coul<R>(pc, d2, xi.w * xj.w, fc, ec);
   const R w = pair_weight<R, RBL>(pc, d2, hi, [&] { return *hjp; });
   const R f = w * (flj + fc);
   a.fx += f * dx;
@@ -1298,11 +1322,14 @@ __device__ __forceinline__ void frac_coords(const Box<R>& b, const R4T<R>& p,
 // m_src holds the K integer k-vectors, in device memory or, for the sampled
 // stream, in host-mapped staging memory; in that case m_copy (device) is
 // given and the chunk-0 block of every k tile copies its tile there for
-// k_sk_fin and later calls.
+// k_sk_fin and later calls (likewise src_src -> src_copy, the RBSOG
+// virial-chain sources, when given).
 template <typename R>
 __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
                                             const int* __restrict__ m_src,
-                                            int* __restrict__ m_copy, int K) {
+                                            int* __restrict__ m_copy,
+                                            const int* __restrict__ src_src,
+                                            int* __restrict__ src_copy, int K) {
   __shared__ double smem[4 * 2 * kSkTile];
   const int chunk = blockIdx.x, k0 = blockIdx.y * kSkTile;
   const int kn = min(kSkTile, K - k0);
@@ -1325,6 +1352,8 @@ __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
 #pragma unroll
         for (int q = 0; q < 3; ++q) m_copy[3 * (k0 + kk) + q] = mi[3 * kk + q];
   }
+  if (src_copy && chunk == 0 && threadIdx.x < kn)
+    src_copy[k0 + threadIdx.x] = src_src[k0 + threadIdx.x];
   float re[kSkTile], im[kSkTile];
 #pragma unroll
   for (int kk = 0; kk < kSkTile; ++kk) re[kk] = im[kk] = 0.f;
@@ -1363,10 +1392,16 @@ __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
 //   E = qqr2e (2 pi / V) sum_k w_k |rho|^2 / k^2,
 //   W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/(4 alpha) + 1/k^2)).
 // One block; the kernel boundary after k_sk makes every partial visible.
+// vmode 0 (RBE, EWALD): W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/(4 alpha)
+// + 1/k^2)).  vmode 1 (RBSOG, kspace_calculator_op.hip.cu:1133-1222 and
+// :1308-1352): W_aa += E_k, and W_ab += tau_k |rho_s|^2 k_s,a k_s,b with
+// s = src[k] and tau_k = kvir[k].x, divided by |k_s|^4 when kvir[k].y != 0.
 template <typename R>
 __global__ __launch_bounds__(1024) void k_sk_fin(Dev<R> d, Box<R> b,
                                                  const int* __restrict__ m,
-                                                 int K, int nchunks, Params p) {
+                                                 int K, int nchunks, Params p,
+                                                 int vmode,
+                                                 const int* __restrict__ src) {
   const double V = p.volume;
   const double two_pi = 6.283185307179586476925;
   double kv[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1388,14 +1423,44 @@ __global__ __launch_bounds__(1024) void k_sk_fin(Dev<R> d, Box<R> b,
                            (float)m[3 * k + 2], (float)(A * sre));
     d.kf1[k] = make_float4((float)kx, (float)ky, (float)kz, (float)(A * sim));
     const double base = p.qqr2e * two_pi / V * w * (sre * sre + sim * sim) / k2;
-    const double cc = 2.0 * (1.0 / (4.0 * p.alpha) + 1.0 / k2);
     kv[0] += base;
-    kv[1] += base * (1.0 - cc * kx * kx);
-    kv[2] += base * (1.0 - cc * ky * ky);
-    kv[3] += base * (1.0 - cc * kz * kz);
-    kv[4] += base * (-cc * kx * ky);
-    kv[5] += base * (-cc * kx * kz);
-    kv[6] += base * (-cc * ky * kz);
+    if (vmode == 0) {
+      const double cc = 2.0 * (1.0 / (4.0 * p.alpha) + 1.0 / k2);
+      kv[1] += base * (1.0 - cc * kx * kx);
+      kv[2] += base * (1.0 - cc * ky * ky);
+      kv[3] += base * (1.0 - cc * kz * kz);
+      kv[4] += base * (-cc * kx * ky);
+      kv[5] += base * (-cc * kx * kz);
+      kv[6] += base * (-cc * ky * kz);
+    } else {
+      kv[1] += base;
+      kv[2] += base;
+      kv[3] += base;
+    }
+  }
+  if (vmode == 1) {
+    // the tensor part reads rho of another k, written above by other threads
+    __syncthreads();
+    __threadfence_block();
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+      const int s = src[k];
+      const double kx = two_pi * m[3 * s] / (double)b.L[0];
+      const double ky = two_pi * m[3 * s + 1] / (double)b.L[1];
+      const double kz = two_pi * m[3 * s + 2] / (double)b.L[2];
+      const double2 r = d.rho[s];
+      const double2 tv = d.kvir[k];
+      double t = tv.x * (r.x * r.x + r.y * r.y);
+      if (tv.y != 0.0) {
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        t /= k2 * k2;
+      }
+      kv[1] += t * kx * kx;
+      kv[2] += t * ky * ky;
+      kv[3] += t * kz * kz;
+      kv[4] += t * kx * ky;
+      kv[5] += t * kx * kz;
+      kv[6] += t * ky * kz;
+    }
   }
   __shared__ double smem7[16 * 7];
   block_sum<7>(kv, smem7);
@@ -1647,12 +1712,16 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
     }
     if (a.ev[3]) hipEventRecord(a.ev[3], ks);
     const int nchunks = (d.nc + kSkChunk - 1) / kSkChunk;
+    const bool sog = p.kspace == 3;
     hipLaunchKernelGGL((k_sk<R>), dim3(nchunks, (K + kSkTile - 1) / kSkTile),
                        dim3(kBlock), 0, ks, d, a.box,
                        a.m_host ? a.m_host : a.m_use,
-                       a.m_host ? d.m_dev : nullptr, K);
+                       a.m_host ? d.m_dev : nullptr,
+                       (sog && a.src_host) ? a.src_host : nullptr,
+                       (sog && a.src_host) ? d.src_dev : nullptr, K);
     hipLaunchKernelGGL((k_sk_fin<R>), dim3(1), dim3(1024), 0, ks, d, a.box,
-                       a.m_host ? (const int*)d.m_dev : a.m_use, K, nchunks, p);
+                       a.m_host ? (const int*)d.m_dev : a.m_use, K, nchunks, p,
+                       sog ? 1 : 0, sog ? (const int*)d.src_dev : nullptr);
     hipLaunchKernelGGL((k_rbe_force<R>), dim3(blocks_for(d.nc, 64)),
                        dim3(64 * kRbeWaves), 0, ks, d, a.box, K);
     if (a.ev[4]) hipEventRecord(a.ev[4], ks);

@@ -82,7 +82,19 @@ def _cfg(**edits):
     ({"hyper_parameters.coulomb.random_seed": 2 ** 33}, "UINT_MAX"),
     ({"hyper_parameters.coulomb.type": "PPPM"}, "coulomb.type"),
     ({"hyper_parameters.force_field.type": "EAM"}, "force_field.type"),
-    ({"hyper_parameters.neighbor.type": "VERLET-SOG"}, "neighbor.type"),
+    ({"hyper_parameters.neighbor.type": "HALF"}, "neighbor.type"),
+    ({"hyper_parameters.neighbor.type": "VERLET-SOG"}, "rbsog_level"),
+    ({"hyper_parameters.coulomb.type": "RBSOG"}, "rbsog_level"),
+    ({"hyper_parameters.coulomb.type": "RBSOG", "hyper_parameters.coulomb.rbsog_level": 7},
+     "[0, 4]"),
+    ({"hyper_parameters.coulomb.type": "RBSOG", "hyper_parameters.coulomb.rbsog_level": 1,
+      "hyper_parameters.coulomb.rbsog_b": 2.0}, "Legacy"),
+    ({"hyper_parameters.coulomb.type": "RBSOG", "hyper_parameters.coulomb.rbsog_level": 1,
+      "hyper_parameters.coulomb.accuracy": None, "hyper_parameters.coulomb.alpha": None},
+     None),
+    ({"hyper_parameters.neighbor.type": "VERLET-SOG",
+      "hyper_parameters.coulomb.type": "RBSOG", "hyper_parameters.coulomb.rbsog_level": 2},
+     None),
     ({"hyper_parameters.neighbor.skin": -0.1}, "skin"),
     ({"init_configuration.read_data.unit": "SI"}, "unit"),
 ])
