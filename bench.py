@@ -105,6 +105,22 @@ def main():
     kt = ctx.kernel_times()
     th2 = ctx.thermo()
 
+    # host cost of one call, measured over 10 calls from an idle device (fewer
+    # than the 16 in-flight evaluations the context allows, so no call waits)
+    def host_cost(eager):
+        base = args.warmup + 2 * args.steps + (100 if eager else 0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(base, base + 10):
+            ctx.force_leapfrog(k, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], frc[0],
+                               frc[1], frc[2], dt, eager=eager)
+        h = (time.perf_counter() - t) / 10
+        torch.cuda.synchronize()
+        return h
+
+    host_graph_us = 1e6 * min(host_cost(False) for _ in range(3))
+    host_eager_us = 1e6 * min(host_cost(True) for _ in range(3))
+
     # dominant kernel: the pair kernel k_pair.  Algorithmic bytes per launch,
     # the compulsory-traffic model of the pair pass: 4 B per pair-list entry
     # (list A + list B; this build stores list-A entries in 2 B) + 36 B per
@@ -158,6 +174,7 @@ def main():
         "kernel_us": {k: round(v, 2) for k, v in kt.items()},
         "ms_per_step_with_timing_events": round(1e3 * elapsed_ev / args.steps, 4),
         "host_enqueue_ms_per_step": round(1e3 * host_elapsed / args.steps, 4),
+        "host_us_per_call": {"graph": round(host_graph_us, 1), "eager": round(host_eager_us, 1)},
         "force_rel_rms_vs_fp64": ferr,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -57,9 +57,13 @@ enum {
   RBMDX_FLAG_ENERGY = 1,   /* also accumulate energies and the virial      */
   RBMDX_FLAG_SYNC = 2,     /* synchronise and validate before returning    */
   RBMDX_FLAG_REBUILD = 4,  /* force a neighbour rebuild this call          */
-  RBMDX_FLAG_TIMING = 8    /* record per-phase GPU timing events this call
+  RBMDX_FLAG_TIMING = 8,   /* record per-phase GPU timing events this call
                               (read with rbmdx_kernel_times; the events
-                              themselves cost a few microseconds per call) */
+                              themselves cost a few microseconds per call;
+                              implies RBMDX_FLAG_EAGER) */
+  RBMDX_FLAG_EAGER = 16    /* launch the kernels one by one instead of
+                              replaying the context's captured hipGraph
+                              (also RBMDX_GRAPH=0 in the environment) */
 };
 
 /* Thermo record filled by rbmdx_thermo (all FP64, extensive totals). */
@@ -230,13 +234,18 @@ int rbmdx_sog_sample_host(int32_t level, double cut_off, const double* L,
                           int32_t exact, int32_t P, uint32_t base_seed,
                           int64_t call_index, int32_t* m_out, int32_t* idx_out);
 
-/* Per-phase GPU times ("rebuild", "pair", "kspace", "assemble"), averaged
+/* Per-phase GPU times, averaged
  * over the calls timed since the previous rbmdx_kernel_times: calls made with
  * RBMDX_FLAG_TIMING, or every call when the context was created with
  * RBMDX_TIMING=1 in the environment.  Fills up to n (name, microseconds)
  * entries and returns their number (synchronises). */
 int rbmdx_kernel_times(rbmdx_ctx* ctx, const char** names, double* usec,
                        int n);
+/* The phases: "rebuild" (skin check, the conditional rebuild kernels and the
+ * gather), "pair" (the pair kernel alone), "kspace" (S(k), its reduction and
+ * the k-space force, on the second stream when that runs beside the pair
+ * kernel), "assemble" (the pair-sum reduction, the wait for k-space and the
+ * assembly). */
 
 #ifdef __cplusplus
 }

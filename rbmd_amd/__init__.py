@@ -23,6 +23,7 @@ RBMDX_FLAG_ENERGY = 1
 RBMDX_FLAG_SYNC = 2
 RBMDX_FLAG_REBUILD = 4
 RBMDX_FLAG_TIMING = 8
+RBMDX_FLAG_EAGER = 16
 
 _STATUS = {
     0: "RBMDX_OK",
@@ -262,12 +263,14 @@ class ForceContext:
         self._check(self.lib.rbmdx_set_charges(self._h, _dptr(q)))
 
     def force(self, step, px, py, pz, fx, fy, fz, m_samples=None, energy=False,
-              sync=False, rebuild=False, timing=False, stream=None):
-        """Total force into fx/fy/fz (device tensors, caller order)."""
+              sync=False, rebuild=False, timing=False, stream=None, num_k=None):
+        """Total force into fx/fy/fz (device tensors, caller order).  m_samples:
+        explicit k-sample (int32 device tensor) of num_k vectors (default
+        numel // 3; RBSOG: num_k = P m-vectors followed by P chain indices)."""
         flags = ((RBMDX_FLAG_ENERGY if energy else 0) | (RBMDX_FLAG_SYNC if sync else 0)
                  | (RBMDX_FLAG_REBUILD if rebuild else 0)
                  | (RBMDX_FLAG_TIMING if timing else 0))
-        nk = 0 if m_samples is None else int(m_samples.numel() // 3)
+        nk = 0 if m_samples is None else int(num_k or m_samples.numel() // 3)
         if stream is None:
             import torch
             stream = torch.cuda.current_stream(px.device).cuda_stream
@@ -282,10 +285,10 @@ class ForceContext:
         self._check(self.lib.rbmdx_set_masses(self._h, _dptr(m)))
 
     def force_leapfrog(self, step, px, py, pz, vx, vy, vz, fx, fy, fz, dt, energy=False,
-                       sync=False, timing=False, stream=None):
+                       sync=False, timing=False, eager=False, stream=None):
         """Force evaluation fused with v += dt F/m, x += dt v (in place)."""
         flags = ((RBMDX_FLAG_ENERGY if energy else 0) | (RBMDX_FLAG_SYNC if sync else 0)
-                 | (RBMDX_FLAG_TIMING if timing else 0))
+                 | (RBMDX_FLAG_TIMING if timing else 0) | (RBMDX_FLAG_EAGER if eager else 0))
         if stream is None:
             import torch
             stream = torch.cuda.current_stream(px.device).cuda_stream

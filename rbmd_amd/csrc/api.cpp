@@ -293,7 +293,7 @@ class Engine : public EngineBase {
   Engine(const Config& c, int device) : cfg_(c), device_(device) {}
 
   ~Engine() override {
-    if (!ready_ && allocs_.empty() && !stage_ && !ctrl_host_) return;
+    if (!ready_ && allocs_.empty() && !stage_ && !ctrl_host_ && !hdr_) return;
     hipSetDevice(device_);
     if (last_stream_set_) hipStreamSynchronize(last_stream_);
     pipeline_.reset();
@@ -306,6 +306,9 @@ class Engine : public EngineBase {
     if (ev_fork_) hipEventDestroy(ev_fork_);
     if (ev_join_) hipEventDestroy(ev_join_);
     if (ks_) hipStreamDestroy(ks_);
+    if (graph_exec_) hipGraphExecDestroy(graph_exec_);
+    if (cap_stream_) hipStreamDestroy(cap_stream_);
+    if (hdr_) hipHostFree(hdr_);
     if (stage_) hipHostFree(stage_);
     if (ctrl_host_) hipHostFree(ctrl_host_);
     // leave no sticky error behind for the caller's next HIP call
@@ -423,11 +426,19 @@ class Engine : public EngineBase {
     lists_stale_ = false;
     StepArgs<R> a = args(step, flags);
     a.force_rebuild = force_rebuild;
-    // k-sample of this evaluation
+    // this evaluation's slot of the host-mapped rings: step parameters and,
+    // for sampled k-space, the k-sample (read on the device through
+    // step_count, so one captured graph serves every evaluation)
+    const int slot = (int)(calls_ % kSlots);
+    if (slot_used_[slot]) HIPCHK(hipEventSynchronize(slot_events_[slot]));
+    hdr_[slot].epoch = (long long)step;
+    hdr_[slot].force = force_rebuild;
+    bool explicit_m = false;
     if (kspace_ == 1 || kspace_ == 3) {
       const bool sog = kspace_ == 3;
       const int P = num_samples();
       if (m) {
+        explicit_m = true;
         if (!sog) {
           if (nk <= 0 || nk > d_.kcap)
             throw Error(RBMDX_E_ARG, "num_k out of range for explicit samples");
@@ -443,14 +454,10 @@ class Engine : public EngineBase {
           HIPCHK(hipMemcpyAsync(d_.src_dev, m + 3 * P, sizeof(int32_t) * P,
                                 hipMemcpyDeviceToDevice, s));
           a.m_use = d_.m_dev;
-          a.src_use = d_.src_dev;
           a.p.num_k = d_.kcap;
         }
       } else {
-        const int slot = (int)(calls_ % kSlots);
-        if (slot_used_[slot]) HIPCHK(hipEventSynchronize(slot_events_[slot]));
         int32_t* dst = stage_ + (size_t)slot * stage_stride_;
-        const int32_t* dst_dev = stage_dev_ + (size_t)slot * stage_stride_;
         if (!pipeline_) {
           const char* th = std::getenv("RBMDX_SAMPLER_THREADS");
           const int nth = th ? std::max(1, std::atoi(th)) : 4;
@@ -471,30 +478,33 @@ class Engine : public EngineBase {
           int32_t* src = dst + 3 * K;
           std::copy(sog_buf_.begin() + 3 * P, sog_buf_.end(), src);
           for (int e = P; e < K; ++e) src[e] = e;
-          a.src_host = dst_dev + 3 * K;
-          a.src_use = d_.src_dev;
           a.p.num_k = K;
         }
-        a.m_host = dst_dev;
+        a.m_from_ring = 1;
         a.m_use = d_.m_dev;
-        stage_slot_ = slot;
       }
     } else if (kspace_ == 2) {
       a.m_use = d_.m_dev;
     }
-    if (timing_ || (flags & RBMDX_FLAG_TIMING)) {
-      if (timing_used_ == timing_events_.size()) {
-        timing_events_.emplace_back();
-        for (auto& e : timing_events_.back()) HIPCHK(hipEventCreate(&e));
+    const bool timed = timing_ || (flags & RBMDX_FLAG_TIMING);
+    if (graph_on_ && !timed && !explicit_m && !(flags & RBMDX_FLAG_EAGER)) {
+      const GraphKey key = graph_key(a);
+      if (!graph_exec_ || std::memcmp(&key, &graph_key_, sizeof key) != 0)
+        capture(a, key);
+      HIPCHK(hipGraphLaunch(graph_exec_, s));
+    } else {
+      if (timed) {
+        if (timing_used_ == timing_events_.size()) {
+          timing_events_.emplace_back();
+          for (auto& e : timing_events_.back()) HIPCHK(hipEventCreate(&e));
+        }
+        auto& ev = timing_events_[timing_used_++];
+        for (int q = 0; q < 6; ++q) a.ev[q] = ev[q];
       }
-      auto& ev = timing_events_[timing_used_++];
-      for (int k = 0; k < 6; ++k) a.ev[k] = ev[k];
+      HIPCHK(launch_force<R>(d_, a, s));
     }
-    HIPCHK(launch_force<R>(d_, a, s));
-    if (a.m_host) {
-      HIPCHK(hipEventRecord(slot_events_[stage_slot_], s));
-      slot_used_[stage_slot_] = true;
-    }
+    HIPCHK(hipEventRecord(slot_events_[slot], s));
+    slot_used_[slot] = true;
     last_stream_ = s;
     last_stream_set_ = true;
     ++calls_;
@@ -507,6 +517,56 @@ class Engine : public EngineBase {
         // capacity is now larger for the next call.
       }
     }
+  }
+
+  // Everything a captured evaluation bakes into its kernel arguments besides
+  // the device buffers (whose reallocation bumps gen_).
+  struct GraphKey {
+    const void* ptr[12];
+    int eflag, num_k, m_from_ring, pad;
+    double dt;
+    uint64_t gen;
+  };
+  GraphKey graph_key(const StepArgs<R>& a) const {
+    GraphKey g;
+    std::memset(&g, 0, sizeof g);
+    const void* p[12] = {d_.px, d_.py, d_.pz, d_.fx, d_.fy, d_.fz,
+                         d_.vx, d_.vy, d_.vz, d_.xw, d_.yw, d_.zw};
+    for (int q = 0; q < 12; ++q) g.ptr[q] = p[q];
+    g.eflag = a.p.eflag;
+    g.num_k = a.p.num_k;
+    g.m_from_ring = a.m_from_ring;
+    g.dt = a.dt;
+    g.gen = gen_;
+    return g;
+  }
+
+  // Captures one evaluation (both streams) into a graph and instantiates it.
+  void capture(const StepArgs<R>& a, const GraphKey& key) {
+    if (graph_exec_) {
+      HIPCHK(hipGraphExecDestroy(graph_exec_));
+      graph_exec_ = nullptr;
+    }
+    if (!cap_stream_) HIPCHK(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
+    HIPCHK(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeRelaxed));
+    const hipError_t e1 = launch_force<R>(d_, a, cap_stream_);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(cap_stream_, &g);
+    if (e1 != hipSuccess || e2 != hipSuccess || !g) {
+      if (g) hipGraphDestroy(g);
+      (void)hipGetLastError();
+      throw Error(RBMDX_E_HIP, std::string("graph capture failed: ") +
+                                   hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+    }
+    const hipError_t e3 = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e3 != hipSuccess) {
+      graph_exec_ = nullptr;
+      throw Error(RBMDX_E_HIP, std::string("graph instantiation failed: ") +
+                                   hipGetErrorString(e3));
+    }
+    graph_key_ = key;
+    ++captures_;
   }
 
   void thermo(rbmdx_thermo_t* out) override {
@@ -616,7 +676,6 @@ class Engine : public EngineBase {
   }
 
  private:
-  static constexpr int kSlots = 16;
   Config cfg_;
   int device_;
   bool ready_ = false;
@@ -650,7 +709,13 @@ class Engine : public EngineBase {
   Ctrl* ctrl_host_ = nullptr;      // host view of the control mirror
   hipEvent_t slot_events_[kSlots] = {};
   bool slot_used_[kSlots] = {};
-  int stage_slot_ = 0;
+  StepHdr* hdr_ = nullptr;          // host view of the step-parameter ring
+  bool graph_on_ = true;            // replay evaluations through a hipGraph
+  hipGraphExec_t graph_exec_ = nullptr;
+  GraphKey graph_key_{};
+  hipStream_t cap_stream_ = nullptr;
+  uint64_t gen_ = 0;                // bumped whenever device buffers change
+  int64_t captures_ = 0;
   hipStream_t last_stream_ = nullptr;
   bool last_stream_set_ = false;
   int64_t calls_ = 0, forced_ = 0;
@@ -1067,6 +1132,21 @@ class Engine : public EngineBase {
     params_.num_k = kspace_ == 1 ? c.coulomb_sample_num
                     : kspace_ == 3 ? d_.kcap
                                    : params_.num_k;
+    // step-parameter ring and the per-slot reuse events
+    if (!hdr_) {
+      void* hp = nullptr;
+      HIPCHK(hipHostMalloc(&hp, sizeof(StepHdr) * kSlots, hipHostMallocMapped));
+      hdr_ = (StepHdr*)hp;
+      std::memset(hdr_, 0, sizeof(StepHdr) * kSlots);
+      void* dp = nullptr;
+      HIPCHK(hipHostGetDevicePointer(&dp, hp, 0));
+      d_.hdr_ring = (const StepHdr*)dp;
+      for (int q = 0; q < kSlots; ++q)
+        HIPCHK(hipEventCreateWithFlags(&slot_events_[q], hipEventDisableTiming));
+    }
+    const char* gr = std::getenv("RBMDX_GRAPH");
+    graph_on_ = !(gr && std::atoi(gr) == 0);
+    ++gen_;
     timing_ = std::getenv("RBMDX_TIMING") && std::atoi(std::getenv("RBMDX_TIMING"));
     // k-space beside the pair kernel on a second stream (RBMDX_KSTREAM=0:
     // same stream, in order)
@@ -1091,8 +1171,8 @@ class Engine : public EngineBase {
     void* dp = nullptr;
     HIPCHK(hipHostGetDevicePointer(&dp, hp, 0));
     stage_dev_ = (int32_t*)dp;
-    for (int k = 0; k < kSlots; ++k)
-      HIPCHK(hipEventCreateWithFlags(&slot_events_[k], hipEventDisableTiming));
+    d_.stage_ring = stage_dev_;
+    d_.stage_stride = (int)stage_stride_;
   }
 
   void alloc_kspace(const std::vector<double>& w, const std::vector<int>* mv) {
@@ -1145,6 +1225,7 @@ class Engine : public EngineBase {
 
   void alloc_lists(int max_a, int max_b) {
     lists_stale_ = true;
+    ++gen_;
     // list A is read 8 entries at a time: capacity is a multiple of 8
     const int cap_a = ((int)std::ceil(max_a * 1.2) + 16 + 7) / 8 * 8;
     const int cap_b = (int)std::ceil(max_b * 1.2) + 16;

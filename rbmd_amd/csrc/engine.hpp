@@ -32,7 +32,18 @@ struct Ctrl {
   unsigned long long sum_count_b;
   int max_stage;              // largest brick neighbourhood seen
   int unstaged_bricks;        // bricks whose neighbourhood exceeded LDS
+  long long epoch;            // RBL sampling epoch of the current evaluation
 };
+
+// Per-evaluation parameters, written by the host into a host-mapped ring of
+// kSlots entries; evaluation c reads entry c mod kSlots (c = step_count), so
+// the same launch sequence (or captured graph) serves every evaluation.
+struct StepHdr {
+  long long epoch;  // RBL sampling epoch (the caller's step)
+  int force;        // force a neighbour rebuild
+  int pad;
+};
+constexpr int kSlots = 16;
 
 // Blocks of the charged-pair list build (kChargedLanes lanes per atom).
 inline int list_b_blocks(int nc) {
@@ -160,6 +171,9 @@ struct Dev {
   double* mean = nullptr;        // [3]
   Ctrl* ctrl = nullptr;
   Ctrl* ctrl_host = nullptr;     // host-mapped mirror written each call
+  const StepHdr* hdr_ring = nullptr;  // [kSlots] host-mapped step parameters
+  const int* stage_ring = nullptr;    // [kSlots * stage_stride] host-mapped k-samples
+  int stage_stride = 0;
   int* export_count = nullptr;
   int* export_buf = nullptr;     // [3*cap]
 };
@@ -189,9 +203,7 @@ struct StepArgs {
   Params p;
   const R* lj_tab = nullptr;  // [ntypes^2 * 2] (eps_ij, sigma_ij^2), device
   const int* m_use = nullptr; // [num_k * 3] k-vectors for this call, device
-  const int* m_host = nullptr;// host-mapped staging copied into m_dev, or null
-  const int* src_host = nullptr;  // RBSOG: host-mapped virial-chain sources
-  const int* src_use = nullptr;   // RBSOG: device virial-chain sources
+  int m_from_ring = 0;        // k-sample from the step's ring slot (else m_use)
   unsigned int rebuild_thr;   // float bits of (skin/2)^2
   int force_rebuild = 0;
   int count_only = 0;         // list builders only count (capacity planning)

@@ -80,12 +80,19 @@ __device__ __forceinline__ R min_image(R d, R L, R half) {
 
 // The squared pair distance every classification uses:
 // d2 = fma(dz, dz, fma(dy, dy, dx*dx)) with d = min_image(r_j - r_i).
-template <typename R>
+// MI = false when |r_j - r_i| < L/2 is known (the minimum image is then the
+// identity and the result is the same bit for bit).
+template <typename R, bool MI = true>
 __device__ __forceinline__ R pair_d2(const Box<R>& b, R xi, R yi, R zi, R xj,
                                      R yj, R zj, R& dx, R& dy, R& dz) {
-  dx = min_image<R>(xj - xi, b.L[0], b.half[0]);
-  dy = min_image<R>(yj - yi, b.L[1], b.half[1]);
-  dz = min_image<R>(zj - zi, b.L[2], b.half[2]);
+  dx = xj - xi;
+  dy = yj - yi;
+  dz = zj - zi;
+  if (MI) {
+    dx = min_image<R>(dx, b.L[0], b.half[0]);
+    dy = min_image<R>(dy, b.L[1], b.half[1]);
+    dz = min_image<R>(dz, b.L[2], b.half[2]);
+  }
   return fma_r(dz, dz, fma_r(dy, dy, mul_rn(dx, dx)));
 }
 
@@ -256,6 +263,10 @@ struct Neigh {
   int start[3], cnt[3], full[3], e[3];
   int nslots;
   int cell0, cell1;  // cell id range of the brick
+  // 1 when a pair inside the neighbourhood may need the minimum image: it
+  // wraps around the box or spans half of it in some dimension.  Otherwise
+  // every |r_j - r_i| < L/2 and the minimum image is the identity.
+  int wrap;
 };
 
 __device__ __forceinline__ Neigh brick_neigh(const Grid& g, int b,
@@ -280,6 +291,12 @@ __device__ __forceinline__ Neigh brick_neigh(const Grid& g, int b,
     }
   }
   h.nslots = h.cnt[0] * h.cnt[1] * h.cnt[2];
+  h.wrap = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (h.full[d] || h.start[d] < 0 || h.start[d] + h.cnt[d] > g.nc[d] ||
+        2 * h.cnt[d] + 1 > g.nc[d])
+      h.wrap = 1;
   h.cell0 = g.brick_cell0[b];
   h.cell1 = g.brick_cell0[b + 1];
   return h;
@@ -423,6 +440,17 @@ __device__ __forceinline__ void unpack_coord(int packed, int c[3]) {
 template <typename R>
 __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force) {
   __shared__ float s_max[4];
+  __shared__ int s_force;
+  // block 0 takes this evaluation's parameters from the step ring
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    int f = force;
+    if (!force && d.hdr_ring) {
+      const StepHdr h = d.hdr_ring[d.ctrl->step_count % kSlots];
+      d.ctrl->epoch = h.epoch;
+      f = h.force;
+    }
+    s_force = f;
+  }
   float mx = 0.f;
   if (!force) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
@@ -446,6 +474,7 @@ __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force) {
   if (threadIdx.x == 0) {
     float m = s_max[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, s_max[w]);
+    if (blockIdx.x == 0 && s_force) m = __int_as_float(0x7f800000);
     d.skin_part[blockIdx.x] = m;
   }
 }
@@ -589,7 +618,6 @@ __global__ __launch_bounds__(256) void k_gather(Dev<R> d, Box<R> b,
     const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
     const R z = wrap<R>(d.pz[i], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
     const int t = d.type[i];
-    d.xs[s] = make4<R>(x, y, z, (R)t);
     int c;
     if (rb) {
       d.ref[i] = make4<R>(x, y, z, R(0));
@@ -612,8 +640,17 @@ __global__ __launch_bounds__(256) void k_gather(Dev<R> d, Box<R> b,
     } else {
       c = d.c_of[i];
     }
+    // .w: the type, plus one half for a charged atom (truncation gives the
+    // type back; the half marks the atom for the list builds)
+    d.xs[s] = make4<R>(x, y, z, (R)t + (c >= 0 ? R(0.5) : R(0)));
     if (c >= 0) d.xqc[c] = make4<R>(x, y, z, d.charge[i]);
   }
+}
+
+// Charged flag of a sorted position tile (see k_gather).
+template <typename R>
+__device__ __forceinline__ bool charged_w(R w) {
+  return w != (R)(int)w;
 }
 
 // All-atom candidate list of one brick: every j != i within r2 among the 27
@@ -623,7 +660,7 @@ __global__ __launch_bounds__(256) void k_gather(Dev<R> d, Box<R> b,
 // entries in registers and writes them with one 16-byte store.  count_only
 // (cap 0): no writes.  Neighbourhood positions come from the LDS stage when
 // STAGED, else from the sorted arrays.
-template <typename R, bool STAGED>
+template <typename R, bool STAGED, bool MI>
 __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
                                              const Grid& g, const Neigh& h,
                                              const SlotTables<kMaxSlots>& t,
@@ -669,32 +706,33 @@ __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
         for (int kx = 0; kx < st.n[0]; kx += runlen) {
           const int s0 = srow + st.at(0, kx);
           const int l0 = t.off[s0], l1 = t.off[s0 + runlen];
-          int qa[3], qb[3];  // charged windows of the run's cells
-#pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            qa[q] = q < runlen ? t.off[s0 + q] : 0;
-            qb[q] = (q < runlen && qi) ? qa[q] + t.nq[s0 + q] : qa[q];
-          }
           if (STAGED) {
             for (int l = l0; l < l1; l += 4) {
               R4T<R> xj[4];
 #pragma unroll
               for (int u = 0; u < 4; ++u) xj[u] = stage[min(l + u, l1 - 1)];
+              unsigned ok = 0;
 #pragma unroll
               for (int u = 0; u < 4; ++u) {
                 const int lu = l + u;
                 R dx, dy, dz;
-                const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj[u].x, xj[u].y,
-                                        xj[u].z, dx, dy, dz);
-                const bool excl = (lu >= qa[0] && lu < qb[0]) ||
-                                  (lu >= qa[1] && lu < qb[1]) ||
-                                  (lu >= qa[2] && lu < qb[2]);
-                if (lu < l1 && d2 < r2 && lu != self && !excl) accept(lu);
+                const R d2 = pair_d2<R, MI>(b, xi.x, xi.y, xi.z, xj[u].x, xj[u].y,
+                                            xj[u].z, dx, dy, dz);
+                // charged-charged pairs belong to list B
+                const bool keep = lu < l1 && d2 < r2 && lu != self &&
+                                  !(qi && charged_w<R>(xj[u].w));
+                ok |= keep ? (1u << u) : 0u;
+              }
+              while (ok) {
+                const int u = __builtin_ctz(ok);
+                ok &= ok - 1;
+                accept(l + u);
               }
             }
           } else {
             const int jf = t.first[s0];
-            for (int l = qb[0]; l < l1; ++l) {  // charged i skips the cell's charged
+            const int ls = qi ? l0 + t.nq[s0] : l0;  // charged i skips charged j
+            for (int l = ls; l < l1; ++l) {
               const R4T<R> xj = d.xs[jf + (l - l0)];
               R dx, dy, dz;
               const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z,
@@ -730,10 +768,12 @@ __global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
   // neighbourhoods beyond 16-bit positions are left to the direct scan
   const int lcap = total > 65535 ? 0 : cap;
   int mx = 0, sum = 0;
-  if (staged)
-    list_a_atoms<R, true>(d, b, g, h, t, stage, r2, split, lcap, cap, mx, sum);
+  if (staged && !h.wrap)
+    list_a_atoms<R, true, false>(d, b, g, h, t, stage, r2, split, lcap, cap, mx, sum);
+  else if (staged)
+    list_a_atoms<R, true, true>(d, b, g, h, t, stage, r2, split, lcap, cap, mx, sum);
   else
-    list_a_atoms<R, false>(d, b, g, h, t, stage, r2, split, lcap, cap, mx, sum);
+    list_a_atoms<R, false, true>(d, b, g, h, t, stage, r2, split, lcap, cap, mx, sum);
   // per-block statistics (folded by k_reduce)
   __shared__ int s_red[2 * 8];
   mx = wave_max_i(mx);
@@ -987,7 +1027,7 @@ struct R2Of<double> {
 // branches: entries past the count and pairs outside the cutoffs are masked
 // by selects.  Only RBL shell pairs branch to the hash; SHELL is false when
 // the LJ cutoff does not exceed r_core, so that list A holds no shell pairs.
-template <typename R, bool RBL, bool EFLAG, bool SHELL, bool STAGED>
+template <typename R, bool RBL, bool EFLAG, bool SHELL, bool STAGED, bool MI>
 __device__ __forceinline__ void list_a_atom(
     const Dev<R>& d, const Box<R>& b, const PairConst<R>& pc, const R* s_lj,
     const SlotTables<kMaxSlots>& t, int nslots, const R4T<R>* stage, int i,
@@ -1023,10 +1063,9 @@ __device__ __forceinline__ void list_a_atom(
     }
 #pragma unroll
     for (int kk = half; kk < half + 4; ++kk) {
-      const R dx = min_image<R>(xj[kk].x - xi.x, b.L[0], b.half[0]);
-      const R dy = min_image<R>(xj[kk].y - xi.y, b.L[1], b.half[1]);
-      const R dz = min_image<R>(xj[kk].z - xi.z, b.L[2], b.half[2]);
-      const R d2 = fma_r(dz, dz, fma_r(dy, dy, mul_rn(dx, dx)));
+      R dx, dy, dz;
+      const R d2 = pair_d2<R, MI>(b, xi.x, xi.y, xi.z, xj[kk].x, xj[kk].y,
+                                  xj[kk].z, dx, dy, dz);
       const R2 lj = *reinterpret_cast<const R2*>(s_lj + tbase + 2 * (int)xj[kk].w);
       const R inv = pair_rcp(d2);
       const R s2 = lj.y * inv;
@@ -1097,8 +1136,11 @@ struct kPairWaves {
 template <typename R, bool RBL, bool EFLAG, bool SHELL>
 __global__ __launch_bounds__(512)
 __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_pair(Dev<R> d, Box<R> b, Grid g,
-                                              PairConst<R> pc, const R* lj_tab,
-                                              int split) {
+                                              PairConst<R> pc_arg,
+                                              const R* lj_tab, int split) {
+  // the sampling epoch of this evaluation comes from the step ring (k_skin)
+  PairConst<R> pc = pc_arg;
+  if (RBL) pc.epoch = d.ctrl->epoch;
   __shared__ R4T<R> stage[StageCap<R>::value];
   __shared__ SlotTables<kMaxSlots> t;
   __shared__ double smem[8 * 11];
@@ -1121,12 +1163,15 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_
     const unsigned long long hi = RBL ? rbl_hi(d.gid_s[i]) : 0ull;
     const int cnt = d.cnt_a[i];
     if (cnt <= d.cap_a) {
-      if (staged)
-        list_a_atom<R, RBL, EFLAG, SHELL, true>(d, b, pc, s_lj, t, h.nslots, stage, i,
-                                         cnt, xi, ti, hi, a);
+      if (staged && !h.wrap)
+        list_a_atom<R, RBL, EFLAG, SHELL, true, false>(d, b, pc, s_lj, t, h.nslots,
+                                                       stage, i, cnt, xi, ti, hi, a);
+      else if (staged)
+        list_a_atom<R, RBL, EFLAG, SHELL, true, true>(d, b, pc, s_lj, t, h.nslots,
+                                                      stage, i, cnt, xi, ti, hi, a);
       else
-        list_a_atom<R, RBL, EFLAG, SHELL, false>(d, b, pc, s_lj, t, h.nslots, stage,
-                                          i, cnt, xi, ti, hi, a);
+        list_a_atom<R, RBL, EFLAG, SHELL, false, true>(d, b, pc, s_lj, t, h.nslots,
+                                                       stage, i, cnt, xi, ti, hi, a);
     } else {
       // list overflow since the last rebuild: scan the 27 cells directly
       int c[3];
@@ -1326,10 +1371,22 @@ __device__ __forceinline__ void frac_coords(const Box<R>& b, const R4T<R>& p,
 // virial-chain sources, when given).
 template <typename R>
 __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
-                                            const int* __restrict__ m_src,
-                                            int* __restrict__ m_copy,
-                                            const int* __restrict__ src_src,
-                                            int* __restrict__ src_copy, int K) {
+                                            const int* __restrict__ m_arg,
+                                            int from_ring, int with_src, int K) {
+  // the evaluation's k-sample: either m_arg (device) or its slot of the
+  // host-mapped ring (m, then the RBSOG virial-chain sources), copied to HBM
+  const int* m_src = m_arg;
+  int* m_copy = nullptr;
+  const int* src_src = nullptr;
+  int* src_copy = nullptr;
+  if (from_ring) {
+    m_src = d.stage_ring + (size_t)(d.ctrl->step_count % kSlots) * d.stage_stride;
+    m_copy = d.m_dev;
+    if (with_src) {
+      src_src = m_src + 3 * (size_t)K;
+      src_copy = d.src_dev;
+    }
+  }
   __shared__ double smem[4 * 2 * kSkTile];
   const int chunk = blockIdx.x, k0 = blockIdx.y * kSkTile;
   const int kn = min(kSkTile, K - k0);
@@ -1695,11 +1752,10 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   // by every block of k_bin)
   const int sb = blocks_for(d.n, kBlock, kSkinBlocksMax);
   if (a.ev[0]) hipEventRecord(a.ev[0], s);
-  hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(kBlock), 0, s, d, a.box,
-                     a.force_rebuild);
+  // (a forced rebuild comes through the step ring, like the epoch)
+  hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(kBlock), 0, s, d, a.box, 0);
   hipError_t e = build_part<R>(d, a, sb, s);
   if (e != hipSuccess) return e;
-  if (a.ev[1]) hipEventRecord(a.ev[1], s);
   const Grid& g = a.grid;
   // k-space on the second stream, beside the pair kernel
   const int K = p.kspace ? p.num_k : 0;
@@ -1713,14 +1769,12 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
     if (a.ev[3]) hipEventRecord(a.ev[3], ks);
     const int nchunks = (d.nc + kSkChunk - 1) / kSkChunk;
     const bool sog = p.kspace == 3;
+    const bool ring = a.m_from_ring;
     hipLaunchKernelGGL((k_sk<R>), dim3(nchunks, (K + kSkTile - 1) / kSkTile),
-                       dim3(kBlock), 0, ks, d, a.box,
-                       a.m_host ? a.m_host : a.m_use,
-                       a.m_host ? d.m_dev : nullptr,
-                       (sog && a.src_host) ? a.src_host : nullptr,
-                       (sog && a.src_host) ? d.src_dev : nullptr, K);
+                       dim3(kBlock), 0, ks, d, a.box, a.m_use, ring ? 1 : 0,
+                       sog ? 1 : 0, K);
     hipLaunchKernelGGL((k_sk_fin<R>), dim3(1), dim3(1024), 0, ks, d, a.box,
-                       a.m_host ? (const int*)d.m_dev : a.m_use, K, nchunks, p,
+                       ring ? (const int*)d.m_dev : a.m_use, K, nchunks, p,
                        sog ? 1 : 0, sog ? (const int*)d.src_dev : nullptr);
     hipLaunchKernelGGL((k_rbe_force<R>), dim3(blocks_for(d.nc, 64)),
                        dim3(64 * kRbeWaves), 0, ks, d, a.box, K);
@@ -1731,6 +1785,8 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
     hipEventRecord(a.ev[4], s);
   }
   const size_t lj_bytes = 2 * sizeof(R) * (size_t)a.pc.ntypes * a.pc.ntypes;
+  // ev[1] .. ev[2] bracket the pair kernel alone
+  if (a.ev[1]) hipEventRecord(a.ev[1], s);
 #define RBMDX_PAIR(RBLv, EFv, SHv)                                          \
   hipLaunchKernelGGL((k_pair<R, RBLv, EFv, SHv>), dim3(g.nbrick),            \
                      dim3(g.block), lj_bytes, s, d, a.box, g, a.pc, a.lj_tab, \
@@ -1756,11 +1812,11 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
       RBMDX_PAIR(false, false, false);
   }
 #undef RBMDX_PAIR
+  if (a.ev[2]) hipEventRecord(a.ev[2], s);
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part, g.nbrick,
                      d.thermo, d.mean, p.mean_fix, p.n_global, d.lista_part,
                      g.nbrick, d.listb_part, list_b_grid(p, d.nc), d.ctrl,
                      a.rebuild_thr);
-  if (a.ev[2]) hipEventRecord(a.ev[2], s);
   if (ks_on && ks != s) hipStreamWaitEvent(s, a.ev_join, 0);
   hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,
                      s, d, p.mean_fix, ks_on ? 1 : 0, (R)a.dt, (R)a.dt_ftm);

@@ -371,3 +371,71 @@ def test_kspace_stream_schedules_bitwise_equal(device, small_electrolyte, monkey
         assert out[0][1][key] == out[1][1][key], key
     ref = reference_force(s, s.pos.astype(np.float32), cfg, 2, m)[0]
     assert rel_rms(out[0][0], ref) < 2e-5
+
+
+def test_graph_replays_match_eager_launches(device, monkeypatch):
+    """Evaluations replayed from the captured graph (the default) agree bit for
+    bit with eager launches along a trajectory with rebuilds, a forced
+    rebuild, an energy evaluation (a second graph) and torch work plus
+    synchronisation between the replays."""
+    import torch
+
+    s = systems.electrolyte(4000, seed=31)
+    cfg = systems.ljsalt_config(P=32, seed=777)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("RBMDX_GRAPH", mode)
+        ctx = _ctx(s, cfg)
+        p = torch.tensor(s.pos.T.copy(), dtype=torch.float32, device=device)
+        v = torch.tensor(s.vel.T.copy(), dtype=torch.float32, device=device)
+        f = torch.zeros_like(p)
+        traj = []
+        for k in range(14):
+            if k == 6:
+                ctx.force(k, p[0], p[1], p[2], f[0], f[1], f[2], rebuild=True)
+            else:
+                ctx.force_leapfrog(k, p[0], p[1], p[2], v[0], v[1], v[2], f[0], f[1], f[2],
+                                   0.004, energy=(k % 5 == 4))
+            torch.cuda.synchronize()
+            scratch = (p * 2.0).sum() + v.abs().max()  # eager work between replays
+            assert torch.isfinite(scratch)
+            traj.append((p.cpu().numpy().copy(), f.cpu().numpy().copy()))
+            if k % 5 == 4:
+                th = ctx.thermo()
+                traj.append((np.array([th["e_vdwl"], th["e_coul"], th["e_kspace"]]),
+                             np.array(th["virial"])))
+        th = ctx.thermo()
+        out[mode] = (traj, th["rebuilds"])
+    assert out["1"][1] == out["0"][1] and out["1"][1] >= 3
+    for (a1, b1), (a0, b0) in zip(out["1"][0], out["0"][0]):
+        np.testing.assert_array_equal(a1, a0)
+        np.testing.assert_array_equal(b1, b0)
+
+
+def test_interior_bricks_skip_min_image_bit_exact(device):
+    """A box large enough for bricks whose neighbourhood neither wraps nor spans
+    half the box (those skip the minimum image): pair sets stay bit-exact and
+    forces match the FP64 oracle."""
+    import torch
+
+    s = systems.electrolyte(36000, seed=17)  # 9 cells per edge: interior bricks exist
+    cfg = systems.ljsalt_config(lj_cut_off=3.5, sample_num=30, P=64)
+    ctx = _ctx(s, cfg)
+    (px, py, pz), f = _tensors(s, device, torch.float32)
+    step = 5
+    gi, gj, cls = ctx.export_pairs(step, px, py, pz, cap=1 << 24)
+    pos32 = s.pos.astype(np.float32)
+    nb = cfg["hyper_parameters"]["neighbor"]
+    freq = oracle.rbl_frequency(s.n, s.box_len, nb["r_core"], nb["cut_off"],
+                                nb["neighbor_sample_num"])
+    params = oracle.pair_params(nb["r_core"], nb["cut_off"], 3.5, 0.09, 1.0, freq, step,
+                                1, 1, 1)
+    ref = oracle.rbl_pairs_f32(pos32, s.ids, s.box_len, params)
+    ref_set = set(zip(s.ids[ref[:, 0]].tolist(), s.ids[ref[:, 1]].tolist(),
+                      ref[:, 2].tolist()))
+    assert _sets(gi, gj, cls) == ref_set
+    m = rbe_sample_host(0.09, s.box_len, 64, 3, 0)
+    mt = torch.tensor(m.reshape(-1), dtype=torch.int32, device=device)
+    ctx.force(step, px, py, pz, *f, m_samples=mt, sync=True)
+    refF = reference_force(s, pos32, cfg, step, m)[0]
+    assert rel_rms(_stack(f), refF) < 2e-5

@@ -50,7 +50,8 @@ def _run(ctx, s, device, step, m=None, idx=None):
     if m is not None:
         mt = torch.tensor(np.concatenate([np.asarray(m).reshape(-1), idx]).astype(np.int32),
                           device=device)
-    ctx.force(step, px, py, pz, *f, m_samples=mt, energy=True, sync=True)
+    ctx.force(step, px, py, pz, *f, m_samples=mt, energy=True, sync=True,
+              num_k=None if m is None else len(idx))
     return np.stack([t.double().cpu().numpy() for t in f], axis=1)
 
 
