@@ -891,6 +891,16 @@ class Engine : public EngineBase {
       pc_.t_core = pc_.rs2 - eps;
       pc_.t_shell = pc_.rc2 - eps;
       pc_.eps_min = eps;
+      {
+        // one interval for list-A pairs without shell pairs: the LJ and
+        // Coulomb cutoffs and, with RBL, core (d^2 < r_core^2 - EPSILON,
+        // d^2 < r_core^2) and the guard d^2 > EPSILON
+        const bool rbl = c.neighbor_type == "RBL";
+        R hi = std::min(pc_.rclj2, pc_.rc2);
+        if (rbl) hi = std::min(hi, std::min(pc_.t_core, pc_.rs2));
+        pc_.a_hi = hi;
+        pc_.a_lo = rbl ? eps : (R)-1;
+      }
       pc_.r2_list_a = (R)(ra * ra);
       pc_.r2_list_b = (R)(rb * rb);
       pc_.alpha = (R)alpha_;
@@ -978,7 +988,7 @@ class Engine : public EngineBase {
     d_.pair_part = alloc<double>((size_t)11 * grid_.nbrick);
     d_.skin_part = alloc<float>(kSkinBlocksMax);
     d_.lista_part = alloc<int4>(grid_.nbrick);
-    d_.listb_part = alloc<int2>(std::max(1, list_b_blocks(nc)));
+    d_.listb_part = alloc<int2>(std::max(1, list_b_blocks(nc, grid_.block)));
     d_.thermo = alloc<double>(32);
     d_.mean = alloc<double>(3);
     d_.ctrl = alloc<Ctrl>(1);
@@ -1001,15 +1011,20 @@ class Engine : public EngineBase {
     HIPCHK(hipMemcpy(d_.type, type_.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_.gid, id_.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
     upload_charges();
-    // LJ table: (eps_ij, sigma_ij^2), Lorentz-Berthelot (lj_op.hip.cu:130-131)
+    // LJ table: (-24 eps_ij, sigma_ij^2, eps_ij, 0), Lorentz-Berthelot
+    // (lj_op.hip.cu:130-131); -24 eps is formed in R, as the force term
+    // forms it
     {
-      std::vector<R> t(2 * (size_t)ntypes_ * ntypes_);
+      std::vector<R> t(4 * (size_t)ntypes_ * ntypes_);
       for (int a = 0; a < ntypes_; ++a)
         for (int b = 0; b < ntypes_; ++b) {
           const double e = std::sqrt(eps_[a] * eps_[b]);
           const double s = (sig_[a] + sig_[b]) / 2.0;
-          t[2 * (a * ntypes_ + b)] = (R)e;
-          t[2 * (a * ntypes_ + b) + 1] = (R)(s * s);
+          R* row = &t[4 * (a * ntypes_ + b)];
+          row[0] = (R)-24 * (R)e;
+          row[1] = (R)(s * s);
+          row[2] = (R)e;
+          row[3] = (R)0;
         }
       lj_tab_ = alloc<R>(t.size());
       HIPCHK(hipMemcpy(lj_tab_, t.data(), sizeof(R) * t.size(), hipMemcpyHostToDevice));

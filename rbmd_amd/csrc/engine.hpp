@@ -10,7 +10,7 @@
 namespace rbmdx {
 
 constexpr int kBlock = 256;      // threads per block for per-atom kernels
-constexpr int kSkinBlocksMax = 256;  // skin-check grid (grid-stride beyond)
+constexpr int kSkinBlocksMax = 1024;  // skin-check grid (grid-stride beyond)
 constexpr int kMaxTypes = 32;    // LJ type table held in LDS
 constexpr int kSkTile = 8;       // k-vectors per S(k) block
 constexpr int kSkChunk = 1024;   // charged atoms per S(k) block
@@ -18,7 +18,6 @@ constexpr int kRbeWaves = 8;     // waves per RBE-force block (k split)
 constexpr int kMaxK = 1 << 20;   // k-vector capacity bound
 constexpr int kMaxSlots = 64;    // neighbourhood cells staged per brick
 constexpr int kChargedLanes = 16;  // lanes per charged atom (Coulomb pairs)
-constexpr int kScanPer = 32;     // keys per thread in the one-block scan
 
 // Device control block: rebuild decision, overflow bookkeeping.
 struct Ctrl {
@@ -45,9 +44,10 @@ struct StepHdr {
 };
 constexpr int kSlots = 16;
 
-// Blocks of the charged-pair list build (kChargedLanes lanes per atom).
-inline int list_b_blocks(int nc) {
-  return (nc + kBlock / kChargedLanes - 1) / (kBlock / kChargedLanes);
+// Blocks of the charged-pair list build (kChargedLanes lanes per atom) with
+// `block` threads per block.
+inline int list_b_blocks(int nc, int block) {
+  return (nc + block / kChargedLanes - 1) / (block / kChargedLanes);
 }
 
 
@@ -82,6 +82,10 @@ struct PairConst {
   R t_core, t_shell;  // RBL classification thresholds r_core^2-eps, rc^2-eps
   R rs2, rc2, rclj2;  // force windows
   R eps_min;          // coincident-pair guard (EPSILON)
+  // force window of list-A pairs when the list holds no RBL shell pairs:
+  // a_lo < d^2 < a_hi (min of the LJ and Coulomb cutoffs and, with RBL, of
+  // the core thresholds; a_lo the guard with RBL, else -1)
+  R a_lo, a_hi;
   R r2_list_a, r2_list_b;  // (cut + skin)^2 of the two lists
   R alpha, g, qqr2e, two_g_over_sqrtpi;
   R freq;             // RBL selection frequency as a multiplier
@@ -146,7 +150,7 @@ struct Dev {
   R4* fk = nullptr;        // [nc] k-space force
   // lists
   int cap_a = 0, cap_b = 0;
-  uint16_t* list_a = nullptr;  // [cap_a * n] column-major, brick-local index
+  uint16_t* list_a = nullptr;  // [cap_a * n] groups of 8 (list_a_index), stage byte offsets
   int* cnt_a = nullptr;
   int* list_b = nullptr;       // [nc * cap_b] row-major, charged compact index
   int* cnt_b = nullptr;
@@ -201,7 +205,7 @@ struct StepArgs {
   Grid grid;
   PairConst<R> pc;
   Params p;
-  const R* lj_tab = nullptr;  // [ntypes^2 * 2] (eps_ij, sigma_ij^2), device
+  const R* lj_tab = nullptr;  // [ntypes^2 * 4] (-24 eps_ij, sigma_ij^2, eps_ij, 0), device
   const int* m_use = nullptr; // [num_k * 3] k-vectors for this call, device
   int m_from_ring = 0;        // k-sample from the step's ring slot (else m_use)
   unsigned int rebuild_thr;   // float bits of (skin/2)^2

@@ -1,27 +1,34 @@
 // HIP kernels of the RBMD force path for gfx950 (CDNA4, wave64).
 //
-// Per force evaluation (one stream, no host synchronisation):
-//   k_skin        max displacement since the last rebuild -> rebuild decision
-//                 (also stages the host k-sample into device memory)
-//   k_bin .. k_list_b   (early exit unless the decision says rebuild)
-//                 cell keys, counting sort with a deterministic in-cell order
-//                 (wave bitonic), brick-local all-atom candidate lists built
-//                 from LDS-staged neighbourhoods, charged-pair lists
-//   k_gather      caller-order positions -> cell-sorted tiles
+// Per force evaluation (no host synchronisation; k-space on a second stream
+// beside the pair kernel):
+//   k_skin        rebuild decision, part 1 (max displacement since the last
+//                 rebuild, per block) and the gather of caller-order
+//                 positions into the cell-sorted tiles
+//   k_bin         folds the block maxima into the decision; then, and in
+//                 k_scan, k_scatter, k_cellsort, k_lists only on a rebuild
+//                 (early exit otherwise): cell keys, counting sort with a
+//                 deterministic in-cell order (wave bitonic) and the gather
+//                 in the new order, brick-local all-atom candidate lists
+//                 built from LDS-staged neighbourhoods beside the
+//                 charged-pair lists
 //   k_pair        one workgroup per brick: the brick's 4x4x4-cell
 //                 neighbourhood is staged in LDS; LJ pairs gather from LDS
 //                 through 16-bit brick-local lists with the RBL
 //                 classification applied on the fly; charged atoms then add
-//                 LJ + erfc Coulomb over their charged-pair lists, 16 lanes
-//                 per atom
-//                 (its last block reduces the per-brick FP64 partial sums:
-//                 mean pair force, energies, virial)
-//   k_sk          S(k) partial sums over the charged atoms (k tiles x atoms);
-//                 its last block reduces them in FP64 to rho_k, the k-space
-//                 energy and virial, and the per-k force coefficients
+//                 LJ + Coulomb over their charged-pair lists, 16 lanes per
+//                 atom; FP64 per-brick partial sums
+//   k_reduce      one block: the per-brick partials in a fixed order (mean
+//                 pair force, energies, virial; list statistics)
+//   k_sk          S(k) partial sums over the charged atoms (atom chunks x k
+//                 tiles); k_sk_fin reduces them in FP64 to rho_k, the
+//                 k-space energy and virial and the per-k force coefficients
 //   k_rbe_force   per charged atom, k split over the waves of a block
-//   k_assemble    F = F_pair - <F_pair> + F_kspace in caller order
-// k_sk and k_rbe_force run on a second stream beside k_pair.
+//   k_assemble    F = F_pair - <F_pair> + F_kspace in caller order (and the
+//                 leapfrog update when asked)
+// Cross-block reductions happen at kernel boundaries only: no same-address
+// atomics in the hot kernels and no last-block election (its device-scope
+// fences write back the L2 of every XCD).
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -208,8 +215,10 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
   return v;
 }
 
-// Block sum of NV doubles (blockDim.x multiple of 64, <= 1024).  Result valid
-// in thread 0.  Fixed reduction tree: deterministic.
+// Block sum of NV doubles (blockDim.x multiple of 64, <= 1024; smem holds
+// 16 * NV).  Result valid in thread 0.  Fixed reduction tree: deterministic.
+// The cross-wave sums of the NV values run in NV threads side by side (each
+// in wave order), not as one thread's chain of NV * waves LDS reads.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* smem) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -220,13 +229,17 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* smem) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) smem[wid * NV + k] = v[k];
   __syncthreads();
+  if (threadIdx.x < NV) {
+    const int k = threadIdx.x;
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += smem[w * NV + k];
+    // column k of wave 0 is read by this thread alone
+    smem[k] = s;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < nw; ++w) s += smem[w * NV + k];
-      v[k] = s;
-    }
+    for (int k = 0; k < NV; ++k) v[k] = smem[k];
   }
   __syncthreads();
 }
@@ -263,10 +276,13 @@ struct Neigh {
   int start[3], cnt[3], full[3], e[3];
   int nslots;
   int cell0, cell1;  // cell id range of the brick
-  // 1 when a pair inside the neighbourhood may need the minimum image: it
-  // wraps around the box or spans half of it in some dimension.  Otherwise
-  // every |r_j - r_i| < L/2 and the minimum image is the identity.
-  int wrap;
+  // 1 when the neighbourhood covers a whole dimension: its stage then holds
+  // that dimension once, in box coordinates, and pairs need the minimum
+  // image.  Otherwise stage_atoms shifts the cells that lie across the box
+  // edge by +-L, and every stencil pair (cells at most 2 apart, while a
+  // dimension that is not whole has at least 5 cells) has |r_j - r_i| < L/2
+  // in stage coordinates: the minimum image is the identity.
+  int mi;
 };
 
 __device__ __forceinline__ Neigh brick_neigh(const Grid& g, int b,
@@ -291,12 +307,7 @@ __device__ __forceinline__ Neigh brick_neigh(const Grid& g, int b,
     }
   }
   h.nslots = h.cnt[0] * h.cnt[1] * h.cnt[2];
-  h.wrap = 0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-    if (h.full[d] || h.start[d] < 0 || h.start[d] + h.cnt[d] > g.nc[d] ||
-        2 * h.cnt[d] + 1 > g.nc[d])
-      h.wrap = 1;
+  h.mi = (h.full[0] | h.full[1] | h.full[2]) ? 1 : 0;
   h.cell0 = g.brick_cell0[b];
   h.cell1 = g.brick_cell0[b + 1];
   return h;
@@ -313,13 +324,16 @@ __device__ __forceinline__ int slot_cell(const Grid& g, const Neigh& h, int s) {
 
 // Slot tables in LDS: off (prefix of staged atom counts), first (sorted index
 // of the slot cell's first atom), nq (charged atoms of the slot cell), cfirst
-// (charged compact index of the slot cell's first charged atom).
+// (charged compact index of the slot cell's first charged atom), edge (the
+// slot's box-edge shift: 2 bits per dimension holding 0, 1 or 2 for -L, 0,
+// +L; see stage_atoms).
 template <int NS>
 struct SlotTables {
   int off[NS + 1];
   int first[NS];
   int nq[NS];
   int cfirst[NS];
+  int edge[NS];
   int scan[33];
 };
 
@@ -334,6 +348,17 @@ __device__ __forceinline__ void build_slots(const Grid& g, const Neigh& h,
   int sum = 0;
   for (int s = s0; s < min(h.nslots, s0 + per); ++s) {
     const int cell = slot_cell(g, h, s);
+    {
+      const int kk[3] = {s % h.cnt[0], (s / h.cnt[0]) % h.cnt[1],
+                         s / (h.cnt[0] * h.cnt[1])};
+      int e = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int u = h.start[d] + kk[d];
+        e |= (u < 0 ? 0 : (u >= g.nc[d] ? 2 : 1)) << (2 * d);
+      }
+      t.edge[s] = e;
+    }
     const int a = key_start[2 * cell];
     const int nq = key_start[2 * cell + 1] - a;
     t.first[s] = a;
@@ -370,15 +395,61 @@ __device__ __forceinline__ int slot_of_pos(const SlotTables<NS>& t, int nslots,
   return lo;
 }
 
-template <typename R, int NS>
-__device__ __forceinline__ void stage_atoms(const Neigh& h,
+// Byte offsets of stage entries fit the 16-bit list entries: entry l of the
+// stage is stored as l << kStageShift<R> (sizeof of one entry).
+template <typename R>
+struct StageShift {
+  static constexpr int value = sizeof(R) == 4 ? 4 : 5;
+};
+
+// .w of a stage entry.  Pair kernel: the byte offset of the atom type's
+// entry in one row of the LDS LJ table (4 values per type pair).  List
+// build: 0, or 1e30 for a charged atom (added to d^2 for a charged i, so
+// that charged-charged pairs fail the cutoff test: they belong to list B).
+__device__ __forceinline__ float pair_w(float w) {
+  return __int_as_float(((int)w) << 4);
+}
+__device__ __forceinline__ double pair_w(double w) {
+  return (double)(((int)w) << 5);
+}
+__device__ __forceinline__ int pair_woff(float w) { return __float_as_int(w); }
+__device__ __forceinline__ int pair_woff(double w) { return (int)w; }
+
+// Stage the neighbourhood's atoms in slot order.  Cells that lie across a
+// box edge (neighbourhood coordinate outside [0, nc) in a dimension that is
+// not whole) are shifted by +-L, so that stage coordinates are continuous.
+// The pair kernel also gets a sentinel entry at StageCap - 1, far from every
+// atom: list A pads its last group of 8 with it.
+template <typename R, bool PAIR, int NS>
+__device__ __forceinline__ void stage_atoms(const Grid& g, const Box<R>& b,
+                                            const Neigh& h,
                                             const SlotTables<NS>& t,
                                             const R4T<R>* xs, R4T<R>* stage) {
-  const int total = t.off[h.nslots];
-  for (int l = threadIdx.x; l < total; l += blockDim.x) {
-    const int s = slot_of_pos(t, h.nslots, l);
-    stage[l] = xs[t.first[s] + (l - t.off[s])];
+  // half a wave per slot: the slot's atoms are consecutive in sorted order
+  const int lane = threadIdx.x & 31;
+  for (int s = threadIdx.x >> 5; s < h.nslots; s += blockDim.x >> 5) {
+    const int o = t.off[s], len = t.off[s + 1] - o, f = t.first[s];
+    const int e = t.edge[s];
+    R sh[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int q = (e >> (2 * d)) & 3;
+      sh[d] = q == 0 ? -b.L[d] : (q == 2 ? b.L[d] : R(0));
+    }
+    for (int j = lane; j < len; j += 32) {
+      R4T<R> p = xs[f + j];
+      p.x += sh[0];
+      p.y += sh[1];
+      p.z += sh[2];
+      if (PAIR)
+        p.w = pair_w(p.w);
+      else
+        p.w = p.w != (R)(int)p.w ? R(1e30) : R(0);
+      stage[o + j] = p;
+    }
   }
+  if (PAIR && threadIdx.x == 0)
+    stage[StageCap<R>::value - 1] = make4<R>(R(1e18), R(1e18), R(1e18), pair_w(R(0)));
   __syncthreads();
 }
 
@@ -433,37 +504,58 @@ __device__ __forceinline__ void unpack_coord(int packed, int c[3]) {
 }
 
 // ---------------------------------------------------------------------------
-// Rebuild decision, part 1: per-block max |r - r_ref|^2 (minimum image);
-// +inf when a rebuild is forced.  k_bin folds the block maxima (the kernel
-// boundary makes them visible to every block) into ctrl->maxdisp_bits.
+// Every evaluation, one thread per caller atom (coalesced reads, scattered
+// writes): the rebuild decision, part 1, and the gather of positions into
+// the cell-sorted tiles in the order of the last rebuild.
+//   * per-block max |r - r_ref|^2 (minimum image), +inf when a rebuild is
+//     forced; k_bin folds the block maxima (the kernel boundary makes them
+//     visible to every block) into ctrl->maxdisp_bits;
+//   * xs[inv_perm[i]] = (r, type + 1/2 if charged), and the charged copy.
+// When k_bin then decides to rebuild, k_cellsort gathers again in the new
+// order.  A forced rebuild skips the gather here: the order of the last
+// rebuild may not exist (first evaluation, new system, new lists).
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force) {
   __shared__ float s_max[4];
   __shared__ int s_force;
-  // block 0 takes this evaluation's parameters from the step ring
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  // this evaluation's parameters come from the step ring
+  if (threadIdx.x == 0) {
     int f = force;
     if (!force && d.hdr_ring) {
       const StepHdr h = d.hdr_ring[d.ctrl->step_count % kSlots];
-      d.ctrl->epoch = h.epoch;
+      if (blockIdx.x == 0) d.ctrl->epoch = h.epoch;
       f = h.force;
     }
     s_force = f;
   }
+  __syncthreads();
   float mx = 0.f;
-  if (!force) {
+  if (!s_force) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
          i += gridDim.x * blockDim.x) {
       const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
       const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
       const R z = wrap<R>(d.pz[i], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
       const R4T<R> r = d.ref[i];
-      const R dx = min_image<R>(x - r.x, b.L[0], b.half[0]);
-      const R dy = min_image<R>(y - r.y, b.L[1], b.half[1]);
-      const R dz = min_image<R>(z - r.z, b.L[2], b.half[2]);
+      // minimum image of r - r_ref as r_ref + (r - shift) - r_ref: the
+      // shift also undoes a wrap across the box edge since the rebuild
+      const R ex = x - r.x, ey = y - r.y, ez = z - r.z;
+      const R cx = fabs(ex) > b.half[0] ? copysign(b.L[0], ex) : R(0);
+      const R cy = fabs(ey) > b.half[1] ? copysign(b.L[1], ey) : R(0);
+      const R cz = fabs(ez) > b.half[2] ? copysign(b.L[2], ez) : R(0);
+      const R dx = ex - cx, dy = ey - cy, dz = ez - cz;
       const float d2 = (float)(dx * dx + dy * dy + dz * dz);
       mx = fmaxf(mx, d2 != d2 ? __int_as_float(0x7f800000) : d2);
+      const int s = d.inv_perm[i], c = d.c_of[i];
+      // Sorted tiles keep the frame of the last rebuild: an atom that has
+      // wrapped across the box edge since stays beside its cell (x - shift),
+      // so that stage coordinates remain continuous while lists are reused.
+      // .w: the type, plus one half for a charged atom (truncation gives the
+      // type back; the half marks the atom for the list builds)
+      const R xu = x - cx, yu = y - cy, zu = z - cz;
+      d.xs[s] = make4<R>(xu, yu, zu, (R)d.type[i] + (c >= 0 ? R(0.5) : R(0)));
+      if (c >= 0) d.xqc[c] = make4<R>(xu, yu, zu, d.charge[i]);
     }
   } else {
     mx = __int_as_float(0x7f800000);
@@ -474,7 +566,6 @@ __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force) {
   if (threadIdx.x == 0) {
     float m = s_max[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, s_max[w]);
-    if (blockIdx.x == 0 && s_force) m = __int_as_float(0x7f800000);
     d.skin_part[blockIdx.x] = m;
   }
 }
@@ -514,9 +605,13 @@ __global__ __launch_bounds__(256) void k_bin(Dev<R> d, Box<R> b, Grid g,
   }
 }
 
-// Exclusive scans in one block of 1024: key_start over all keys and cstart
-// over the charged count of every cell, kScanPer consecutive values per
-// thread loaded together; clears key_count for the next rebuild.
+// Exclusive scans in one block of 1024 threads: key_start over all 2 ncell
+// keys and cstart over the charged key (2 cell) of every cell.  Thread t owns
+// the consecutive cells [t per, (t + 1) per): it reads their two counts as
+// one int2, 8 cells per batch of loads (clamped addresses, so every load of
+// a batch issues at once), and one block scan of the thread totals of both
+// scans follows; the second read writes the starts and clears key_count for
+// the next rebuild.
 __global__ __launch_bounds__(1024) void k_scan(int* key_count, int* key_start,
                                                int* cstart, int ncell,
                                                Ctrl* ctrl, unsigned thr) {
@@ -524,34 +619,46 @@ __global__ __launch_bounds__(1024) void k_scan(int* key_count, int* key_start,
   __shared__ int smem[33];
   const int t = threadIdx.x;
   if (t == 0) ctrl->rebuild_count += 1;
-  const int chunk = 1024 * kScanPer;
-  for (int pass = 0; pass < 2; ++pass) {
-    const int n = pass == 0 ? 2 * ncell : ncell;
-    const int stride = pass == 0 ? 1 : 2;
-    int* out = pass == 0 ? key_start : cstart;
-    int carry = 0;
-    for (int base = 0; base < n; base += chunk) {
-      int v[kScanPer];
-      const int a = base + t * kScanPer;
+  int2* kc = reinterpret_cast<int2*>(key_count);
+  const int per = (ncell + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int c0 = min(ncell, t * per), c1 = min(ncell, c0 + per);
+  int sa = 0, sq = 0;
+  for (int c = c0; c < c1; c += 8) {
+    int2 v[8];
 #pragma unroll
-      for (int k = 0; k < kScanPer; ++k)
-        v[k] = (a + k < n) ? key_count[(size_t)(a + k) * stride] : 0;
-      int sum = 0;
+    for (int u = 0; u < 8; ++u) v[u] = kc[min(c + u, c1 - 1)];
 #pragma unroll
-      for (int k = 0; k < kScanPer; ++k) sum += v[k];
-      int total;
-      int run = carry + block_excl_scan(sum, total, smem);
-#pragma unroll
-      for (int k = 0; k < kScanPer; ++k) {
-        if (a + k < n) out[a + k] = run;
-        run += v[k];
+    for (int u = 0; u < 8; ++u)
+      if (c + u < c1) {
+        sa += v[u].x + v[u].y;
+        sq += v[u].x;
       }
-      carry += total;
-    }
-    if (t == 0) out[n] = carry;
   }
-  __syncthreads();
-  for (int k = t; k < 2 * ncell; k += 1024) key_count[k] = 0;
+  int tot_a, tot_q;
+  int ra = block_excl_scan(sa, tot_a, smem);
+  int rq = block_excl_scan(sq, tot_q, smem);
+  // the same batches again, loads first (the stores below may not be
+  // reordered before loads of a later cell)
+  for (int c = c0; c < c1; c += 8) {
+    int2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = kc[min(c + u, c1 - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (c + u < c1) {
+        key_start[2 * (c + u)] = ra;
+        ra += v[u].x;
+        key_start[2 * (c + u) + 1] = ra;
+        ra += v[u].y;
+        cstart[c + u] = rq;
+        rq += v[u].x;
+        kc[c + u] = make_int2(0, 0);
+      }
+  }
+  if (t == 0) {
+    key_start[2 * ncell] = tot_a;
+    cstart[ncell] = tot_q;
+  }
 }
 
 template <typename R>
@@ -563,12 +670,16 @@ __global__ __launch_bounds__(256) void k_scatter(Dev<R> d, unsigned thr) {
 }
 
 // One wave per cell: both key segments are sorted by caller index (bitonic
-// across the wave; serial by lane 0 beyond 64 atoms), then inv_perm is set.
-__global__ __launch_bounds__(256) void k_cellsort(int* perm, int* inv_perm,
-                                                  const int* key_start,
-                                                  int ncell, const Ctrl* ctrl,
-                                                  unsigned thr) {
-  if (!rebuild_now(ctrl, thr)) return;
+// across the wave; serial by lane 0 beyond 64 atoms), then the cell's atoms
+// are gathered into the sorted tiles: positions (and the charged copy),
+// reference positions for the skin check, sorted ids, RBL hashes and cells,
+// the charged map, inv_perm.
+template <typename R>
+__global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
+                                                  int ncell, unsigned thr) {
+  if (!rebuild_now(d.ctrl, thr)) return;
+  int* perm = d.perm;
+  const int* key_start = d.key_start;
   const int lane = threadIdx.x & 63;
   const int cell = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (cell >= ncell) return;
@@ -600,54 +711,40 @@ __global__ __launch_bounds__(256) void k_cellsort(int* perm, int* inv_perm,
   }
   __threadfence_block();
   __builtin_amdgcn_wave_barrier();
-  const int a = key_start[2 * cell], e = key_start[2 * cell + 2];
-  for (int s = a + lane; s < e; s += 64) inv_perm[perm[s]] = s;
-}
-
-// Caller-order positions -> sorted tiles (every evaluation, one thread per
-// caller atom: coalesced reads, scattered writes).  On rebuild also records
-// reference positions, sorted ids/cells and the charged map.
-template <typename R>
-__global__ __launch_bounds__(256) void k_gather(Dev<R> d, Box<R> b,
-                                                unsigned thr) {
-  const bool rb = rebuild_now(d.ctrl, thr);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
-       i += gridDim.x * blockDim.x) {
-    const int s = d.inv_perm[i];
+  // key 2 cell (charged when the work is split) precedes key 2 cell + 1
+  const int a = key_start[2 * cell], aq = key_start[2 * cell + 1],
+            e = key_start[2 * cell + 2];
+  const int cq = d.cstart[cell];
+  for (int s = a + lane; s < e; s += 64) {
+    const int i = perm[s];
+    d.inv_perm[i] = s;
     const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
     const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
     const R z = wrap<R>(d.pz[i], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
     const int t = d.type[i];
-    int c;
-    if (rb) {
-      d.ref[i] = make4<R>(x, y, z, R(0));
-      d.gid_s[s] = d.gid[i];
-      const unsigned long long hj = rbl_hj(d.gid[i]);
-      d.hj_s[s] = hj;
-      const int key = d.key[i];
-      d.cell_s[s] = key >> 1;
-      if (!(key & 1)) {
-        c = d.cstart[key >> 1] + (s - d.key_start[key]);
-        d.c2s[c] = s;
-        d.gidc[c] = d.gid[i];
-        d.hjc[c] = hj;
-        d.typec[c] = t;
-      } else {
-        c = -1;
-      }
-      d.s2c[s] = c;
-      d.c_of[i] = c;
-    } else {
-      c = d.c_of[i];
+    const int gid = d.gid[i];
+    const unsigned long long hj = rbl_hj(gid);
+    d.ref[i] = make4<R>(x, y, z, R(0));
+    d.gid_s[s] = gid;
+    d.hj_s[s] = hj;
+    d.cell_s[s] = cell;
+    int c = -1;
+    if (s < aq) {
+      c = cq + (s - a);
+      d.c2s[c] = s;
+      d.gidc[c] = gid;
+      d.hjc[c] = hj;
+      d.typec[c] = t;
+      d.xqc[c] = make4<R>(x, y, z, d.charge[i]);
     }
-    // .w: the type, plus one half for a charged atom (truncation gives the
-    // type back; the half marks the atom for the list builds)
+    d.s2c[s] = c;
+    d.c_of[i] = c;
+    // .w: the type, plus one half for a charged atom (as in k_skin)
     d.xs[s] = make4<R>(x, y, z, (R)t + (c >= 0 ? R(0.5) : R(0)));
-    if (c >= 0) d.xqc[c] = make4<R>(x, y, z, d.charge[i]);
   }
 }
 
-// Charged flag of a sorted position tile (see k_gather).
+// Charged flag of a sorted position tile (see k_skin).
 template <typename R>
 __device__ __forceinline__ bool charged_w(R w) {
   return w != (R)(int)w;
@@ -675,13 +772,15 @@ __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
     const int os = own_slot(h, c);
     const int self = t.off[os] + (i - t.first[os]);
     const bool qi = split && (i - t.first[os]) < t.nq[os];
+    const R qf = qi ? R(1) : R(0);
     const R4T<R> xi = d.xs[i];
     int count = 0;
     unsigned long long lo = 0, hi = 0;  // entries count&~7 .. count&~7 + 7
     uint4* out = reinterpret_cast<uint4*>(d.list_a) + i;
+    constexpr int sh = StageShift<R>::value;
     auto accept = [&](int l) {
       const int s8 = count & 7;
-      const unsigned long long v = (unsigned long long)l;
+      const unsigned long long v = (unsigned long long)(l << sh);
       if (s8 < 4)
         lo |= v << (16 * s8);
       else
@@ -718,9 +817,10 @@ __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
                 R dx, dy, dz;
                 const R d2 = pair_d2<R, MI>(b, xi.x, xi.y, xi.z, xj[u].x, xj[u].y,
                                             xj[u].z, dx, dy, dz);
-                // charged-charged pairs belong to list B
-                const bool keep = lu < l1 && d2 < r2 && lu != self &&
-                                  !(qi && charged_w<R>(xj[u].w));
+                // charged-charged pairs belong to list B: the stage's .w
+                // (1e30 for a charged j) pushes them past the cutoff
+                const bool keep = lu < l1 && fma_r(qf, xj[u].w, d2) < r2 &&
+                                  lu != self;
                 ok |= keep ? (1u << u) : 0u;
               }
               while (ok) {
@@ -742,33 +842,44 @@ __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
           }
         }
       }
-    if ((count & 7) && count <= lcap)
+    if ((count & 7) && count <= lcap) {
+      // the rest of the last group: the pair kernel's sentinel entry
+      const unsigned long long sent =
+          (unsigned long long)((StageCap<R>::value - 1) << sh);
+      for (int s8 = count & 7; s8 < 8; ++s8) {
+        if (s8 < 4)
+          lo |= sent << (16 * s8);
+        else
+          hi |= sent << (16 * (s8 - 4));
+      }
       out[(size_t)(count >> 3) * d.n] =
           make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi,
                      (unsigned)(hi >> 32));
+    }
     d.cnt_a[i] = (lcap == 0 && cap > 0) ? cap + 1 + count : count;
     mx = max(mx, count);
     sum += count;
   }
 }
 
+// List A of brick `brick`, one block.
 template <typename R>
-__global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
-                                                R r2, int split, int cap,
-                                                unsigned thr) {
-  if (!rebuild_now(d.ctrl, thr)) return;
+__device__ __forceinline__ void list_a_brick(const Dev<R>& d, const Box<R>& b,
+                                             const Grid& g, int brick, R r2,
+                                             int split, int cap) {
   __shared__ R4T<R> stage[StageCap<R>::value];
   __shared__ SlotTables<kMaxSlots> t;
   const int e1[3] = {1, 1, 1};
-  const Neigh h = brick_neigh(g, blockIdx.x, e1);
+  const Neigh h = brick_neigh(g, brick, e1);
   build_slots(g, h, d.key_start, nullptr, false, t);
   const int total = t.off[h.nslots];
-  const bool staged = total <= StageCap<R>::value;
-  if (staged) stage_atoms<R>(h, t, d.xs, stage);
-  // neighbourhoods beyond 16-bit positions are left to the direct scan
-  const int lcap = total > 65535 ? 0 : cap;
+  // (the pair kernel keeps the last stage entry for its sentinel)
+  const bool staged = total < StageCap<R>::value;
+  if (staged) stage_atoms<R, false>(g, b, h, t, d.xs, stage);
+  // neighbourhoods beyond 16-bit byte offsets are left to the direct scan
+  const int lcap = total > (65536 >> StageShift<R>::value) ? 0 : cap;
   int mx = 0, sum = 0;
-  if (staged && !h.wrap)
+  if (staged && !h.mi)
     list_a_atoms<R, true, false>(d, b, g, h, t, stage, r2, split, lcap, cap, mx, sum);
   else if (staged)
     list_a_atoms<R, true, true>(d, b, g, h, t, stage, r2, split, lcap, cap, mx, sum);
@@ -790,7 +901,7 @@ __global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
       m = max(m, s_red[2 * w]);
       su += s_red[2 * w + 1];
     }
-    d.lista_part[blockIdx.x] = make_int4(m, su, total, staged ? 0 : 1);
+    d.lista_part[brick] = make_int4(m, su, total, staged ? 0 : 1);
   }
 }
 
@@ -799,12 +910,13 @@ __global__ __launch_bounds__(512) void k_list_a(Dev<R> d, Box<R> b, Grid g,
 // charged array (20 B per charged atom, cache-resident), count, prefix-sum
 // over the group and write in a fixed order (lane-major over the lane's
 // cells).  Row-major list_b[c * cap + k] of charged compact indices.
+// Block `blk` of list B: blockDim / kChargedLanes charged atoms.
 template <typename R>
-__global__ __launch_bounds__(256) void k_list_b(Dev<R> d, Box<R> b, Grid g,
-                                                R r2, int cap, unsigned thr) {
-  if (!rebuild_now(d.ctrl, thr)) return;
+__device__ __forceinline__ void list_b_block(const Dev<R>& d, const Box<R>& b,
+                                             const Grid& g, int blk, R r2,
+                                             int cap) {
   const int grp = threadIdx.x / kChargedLanes, gl = threadIdx.x % kChargedLanes;
-  const int c = blockIdx.x * (blockDim.x / kChargedLanes) + grp;
+  const int c = blk * (blockDim.x / kChargedLanes) + grp;
   int tot = 0;
   if (c < d.nc) {
     const int i = d.c2s[c];
@@ -852,7 +964,7 @@ __global__ __launch_bounds__(256) void k_list_b(Dev<R> d, Box<R> b, Grid g,
     if (gl == 0) d.cnt_b[c] = tot;
   }
   // per-block statistics (folded by k_reduce)
-  __shared__ int s_red[2 * 4];
+  __shared__ int s_red[2 * 8];
   int mx = wave_max_i(tot);
   int sum = wave_sum((gl == 0) ? tot : 0);
   if ((threadIdx.x & 63) == 0) {
@@ -861,12 +973,27 @@ __global__ __launch_bounds__(256) void k_list_b(Dev<R> d, Box<R> b, Grid g,
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < 4; ++w) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
       mx = max(mx, s_red[2 * w]);
       sum += s_red[2 * w + 1];
     }
-    d.listb_part[blockIdx.x] = make_int2(mx, sum);
+    d.listb_part[blk] = make_int2(mx, sum);
   }
+}
+
+// Both candidate lists in one launch (they are independent): blocks
+// [0, nbrick) build list A of one brick each, the blocks after them list B,
+// so the charged-pair build fills the machine beside the brick builds.
+template <typename R>
+__global__ __launch_bounds__(512) void k_lists(Dev<R> d, Box<R> b, Grid g,
+                                               R r2a, R r2b, int split,
+                                               int cap_a, int cap_b,
+                                               unsigned thr) {
+  if (!rebuild_now(d.ctrl, thr)) return;
+  if ((int)blockIdx.x < g.nbrick)
+    list_a_brick<R>(d, b, g, blockIdx.x, r2a, split, cap_a);
+  else
+    list_b_block<R>(d, b, g, blockIdx.x - g.nbrick, r2b, cap_b);
 }
 
 // ---------------------------------------------------------------------------
@@ -916,12 +1043,12 @@ __device__ __forceinline__ void interact_a(const PairConst<R>& pc,
   const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
   if (!(d2 < pc.rclj2) || !(d2 < pc.rc2)) return;
   const int tj = (int)xj.w;
-  const int tt = 2 * (ti * pc.ntypes + tj);
-  const R eps = s_lj[tt], sig2 = s_lj[tt + 1];
+  const int tt = 4 * (ti * pc.ntypes + tj);
+  const R m24 = s_lj[tt], sig2 = s_lj[tt + 1], eps = s_lj[tt + 2];
   const R inv = R(1) / d2;
   const R s2 = sig2 * inv;
   const R s6 = s2 * s2 * s2;
-  const R flj = R(-24) * eps * (R(2) * s6 - R(1)) * s6 * inv;
+  const R flj = m24 * (R(2) * s6 - R(1)) * s6 * inv;
   const R w = pair_weight<R, RBL>(pc, d2, hi, hj);
   const R f = w * flj;
   a.fx += f * dx;
@@ -979,12 +1106,12 @@ __device__ __forceinline__ void interact_b(const PairConst<R>& pc,
   if (!(d2 < pc.rc2)) return;
   R flj = 0, elj = 0;
   if (d2 < pc.rclj2) {
-    const int tt = 2 * (ti * pc.ntypes + tj);
-    const R eps = s_lj[tt], sig2 = s_lj[tt + 1];
+    const int tt = 4 * (ti * pc.ntypes + tj);
+    const R m24 = s_lj[tt], sig2 = s_lj[tt + 1], eps = s_lj[tt + 2];
     const R inv = R(1) / d2;
     const R s2 = sig2 * inv;
     const R s6 = s2 * s2 * s2;
-    flj = R(-24) * eps * (R(2) * s6 - R(1)) * s6 * inv;
+    flj = m24 * (R(2) * s6 - R(1)) * s6 * inv;
     elj = R(2) * eps * (s6 - R(1)) * s6;
   }
   R fc, ec;
@@ -1020,21 +1147,27 @@ struct R2Of<double> {
   using type = double2;
 };
 
-// All list-A entries of sorted atom i (LJ only).  Entries are read 8 at a
-// time (one 16-byte load of the interleaved list); their 8 neighbour
-// positions are fetched together (from the LDS stage when STAGED, else
-// through the slot tables from the sorted arrays), then evaluated without
-// branches: entries past the count and pairs outside the cutoffs are masked
-// by selects.  Only RBL shell pairs branch to the hash; SHELL is false when
-// the LJ cutoff does not exceed r_core, so that list A holds no shell pairs.
+// All list-A entries of sorted atom i (LJ only).  Entries are byte offsets
+// of stage entries, read 8 at a time (one 16-byte load per group, the next
+// group prefetched).  STAGED: neighbours come from the LDS stage, and the
+// last group's unused entries name the sentinel entry, which fails every
+// cutoff test; otherwise from the sorted arrays, entries beyond cnt masked.
+// The LJ table row of type pair (ti, tj) holds (-24 eps, sigma^2, eps, 0).
+// Without list-A shell pairs and energies the force window is one interval
+// (a_lo, a_hi) of d^2 (the cutoffs, r_core and the coincident-pair guard
+// folded on the host).
 template <typename R, bool RBL, bool EFLAG, bool SHELL, bool STAGED, bool MI>
 __device__ __forceinline__ void list_a_atom(
     const Dev<R>& d, const Box<R>& b, const PairConst<R>& pc, const R* s_lj,
     const SlotTables<kMaxSlots>& t, int nslots, const R4T<R>* stage, int i,
     int cnt, const R4T<R>& xi, int ti, unsigned long long hi, Acc<R>& a) {
   using R2 = typename R2Of<R>::type;
+  using R4 = R4T<R>;
+  constexpr int sh = StageShift<R>::value;
   const uint4* lp = reinterpret_cast<const uint4*>(d.list_a) + i;
-  const int tbase = 2 * ti * pc.ntypes;
+  const char* lj_row =
+      reinterpret_cast<const char*>(s_lj) + (size_t)ti * pc.ntypes * 4 * sizeof(R);
+  const char* st = reinterpret_cast<const char*>(stage);
   auto global_of = [&](int l) {
     const int s = slot_of_pos(t, nslots, l);
     return t.first[s] + (l - t.off[s]);
@@ -1049,15 +1182,15 @@ __device__ __forceinline__ void list_a_atom(
 #pragma unroll
     for (int half = 0; half < 8; half += 4) {
     int lj_[8];
-    R4T<R> xj[8];
+    R4 xj[8];
 #pragma unroll
     for (int kk = half; kk < half + 4; ++kk) {
-      const int l = (int)((wv[kk >> 1] >> (16 * (kk & 1))) & 0xffffu);
+      const int off = (int)((wv[kk >> 1] >> (16 * (kk & 1))) & 0xffffu);
       if (STAGED) {
-        lj_[kk] = kk < kn ? l : 0;  // stage[0] exists whenever i does
-        xj[kk] = stage[lj_[kk]];
+        lj_[kk] = off >> sh;
+        xj[kk] = *reinterpret_cast<const R4*>(st + off);
       } else {
-        lj_[kk] = kk < kn ? global_of(l) : i;
+        lj_[kk] = kk < kn ? global_of(off >> sh) : i;
         xj[kk] = d.xs[lj_[kk]];
       }
     }
@@ -1066,12 +1199,22 @@ __device__ __forceinline__ void list_a_atom(
       R dx, dy, dz;
       const R d2 = pair_d2<R, MI>(b, xi.x, xi.y, xi.z, xj[kk].x, xj[kk].y,
                                   xj[kk].z, dx, dy, dz);
-      const R2 lj = *reinterpret_cast<const R2*>(s_lj + tbase + 2 * (int)xj[kk].w);
+      const int woff = STAGED ? pair_woff(xj[kk].w)
+                              : ((int)xj[kk].w) * 4 * (int)sizeof(R);
+      const R2 lj = *reinterpret_cast<const R2*>(lj_row + woff);
       const R inv = pair_rcp(d2);
       const R s2 = lj.y * inv;
       const R s6 = s2 * s2 * s2;
-      const R flj = R(-24) * lj.x * (R(2) * s6 - R(1)) * s6 * inv;
-      const bool in = kk < kn && d2 < pc.rclj2 && d2 < pc.rc2;
+      const R flj = lj.x * (R(2) * s6 - R(1)) * s6 * inv;
+      if (!SHELL && !EFLAG) {
+        const bool ok = (STAGED || kk < kn) && d2 < pc.a_hi && d2 > pc.a_lo;
+        const R f = ok ? flj : R(0);
+        a.fx = fma_r(f, dx, a.fx);
+        a.fy = fma_r(f, dy, a.fy);
+        a.fz = fma_r(f, dz, a.fz);
+        continue;
+      }
+      const bool in = (STAGED || kk < kn) && d2 < pc.rclj2 && d2 < pc.rc2;
       R w = R(1);
       if (RBL) {
         const bool core = d2 < pc.t_core;
@@ -1087,7 +1230,8 @@ __device__ __forceinline__ void list_a_atom(
       a.fy = fma_r(f, dy, a.fy);
       a.fz = fma_r(f, dz, a.fz);
       if (EFLAG) {
-        a.e_lj += in ? (double)(R(2) * lj.x * (s6 - R(1)) * s6) : 0.0;
+        const R eps = *reinterpret_cast<const R*>(lj_row + woff + 2 * sizeof(R));
+        a.e_lj += in ? (double)(R(2) * eps * (s6 - R(1)) * s6) : 0.0;
         add_virial<R, EFLAG>(a, dx, dy, dz, in ? flj : R(0));
       }
     }
@@ -1097,27 +1241,29 @@ __device__ __forceinline__ void list_a_atom(
 
 template <typename R>
 __device__ __forceinline__ void load_lj(const R* tab, int ntypes, R* s) {
-  for (int t = threadIdx.x; t < 2 * ntypes * ntypes; t += blockDim.x)
+  for (int t = threadIdx.x; t < 4 * ntypes * ntypes; t += blockDim.x)
     s[t] = tab[t];
 }
 
+// Per-brick FP64 partials, component-major (part[k * nbrick + brick]) so
+// that k_reduce reads each component with coalesced loads: force sum, then
+// with energies E_lj, E_coul and the six virial components.
 template <typename R, bool EFLAG>
 __device__ __forceinline__ void write_partials(const Acc<R>& a, double* part,
                                                double* smem) {
   double v[11] = {(double)a.fx, (double)a.fy, (double)a.fz, a.e_lj, a.e_c,
                   a.v[0],       a.v[1],       a.v[2],       a.v[3], a.v[4],
                   a.v[5]};
+  const size_t nb = gridDim.x;
   if (EFLAG) {
     block_sum<11>(v, smem);
     if (threadIdx.x == 0)
-      for (int k = 0; k < 11; ++k) part[11 * blockIdx.x + k] = v[k];
+      for (int k = 0; k < 11; ++k) part[k * nb + blockIdx.x] = v[k];
   } else {
     double u[3] = {v[0], v[1], v[2]};
     block_sum<3>(u, smem);
-    if (threadIdx.x == 0) {
-      for (int k = 0; k < 3; ++k) part[11 * blockIdx.x + k] = u[k];
-      for (int k = 3; k < 11; ++k) part[11 * blockIdx.x + k] = 0.0;
-    }
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 3; ++k) part[k * nb + blockIdx.x] = u[k];
   }
 }
 
@@ -1151,8 +1297,9 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_
   const Neigh h = brick_neigh(g, blockIdx.x, e1);
   build_slots(g, h, d.key_start, nullptr, false, t);
   const int total = t.off[h.nslots];
-  const bool staged = total <= StageCap<R>::value;
-  if (staged) stage_atoms<R>(h, t, d.xs, stage);
+  // the last stage entry is the sentinel
+  const bool staged = total < StageCap<R>::value;
+  if (staged) stage_atoms<R, true>(g, b, h, t, d.xs, stage);
   const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
   Acc<R> acc;
   // phase 1: LJ over list A
@@ -1163,7 +1310,7 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_
     const unsigned long long hi = RBL ? rbl_hi(d.gid_s[i]) : 0ull;
     const int cnt = d.cnt_a[i];
     if (cnt <= d.cap_a) {
-      if (staged && !h.wrap)
+      if (staged && !h.mi)
         list_a_atom<R, RBL, EFLAG, SHELL, true, false>(d, b, pc, s_lj, t, h.nslots,
                                                        stage, i, cnt, xi, ti, hi, a);
       else if (staged)
@@ -1273,20 +1420,30 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_
 
 // One block after the pair kernel: FP64 sums of the per-brick partials in a
 // fixed order (thermo[0..10]: force sum, E_lj, E_coul, virial; the mean pair
-// force when mean_fix) and, after a rebuild, the list statistics of the
-// per-block partials of k_list_a / k_list_b.
+// force when mean_fix; eflag 0 sums the force only and zeroes the rest) and,
+// after a rebuild, the list statistics of the per-block partials of k_lists.
 __global__ __launch_bounds__(1024) void k_reduce(
-    const double* __restrict__ pair_part, int npair, double* thermo,
-    double* mean, int mean_fix, double n_global, const int4* lista_part,
-    int nlista, const int2* listb_part, int nlistb, Ctrl* ctrl,
-    unsigned thr) {
+    const double* __restrict__ pair_part, int npair, int eflag,
+    double* thermo, double* mean, int mean_fix, double n_global,
+    const int4* lista_part, int nlista, const int2* listb_part, int nlistb,
+    Ctrl* ctrl, unsigned thr) {
   __shared__ double smem[16 * 11];
   if (npair > 0) {
     double v[11];
     for (int k = 0; k < 11; ++k) v[k] = 0.0;
-    for (int blk = threadIdx.x; blk < npair; blk += blockDim.x)
-      for (int k = 0; k < 11; ++k) v[k] += pair_part[11 * blk + k];
-    block_sum<11>(v, smem);
+    if (eflag) {
+      for (int blk = threadIdx.x; blk < npair; blk += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 11; ++k) v[k] += pair_part[(size_t)k * npair + blk];
+      block_sum<11>(v, smem);
+    } else {
+      double u[3] = {0.0, 0.0, 0.0};
+      for (int blk = threadIdx.x; blk < npair; blk += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) u[k] += pair_part[(size_t)k * npair + blk];
+      block_sum<3>(u, smem);
+      for (int k = 0; k < 3; ++k) v[k] = u[k];
+    }
     if (threadIdx.x == 0) {
       for (int k = 0; k < 11; ++k) thermo[k] = v[k];
       for (int k = 0; k < 3; ++k) mean[k] = mean_fix ? v[k] / n_global : 0.0;
@@ -1464,6 +1621,7 @@ __global__ __launch_bounds__(1024) void k_sk_fin(Dev<R> d, Box<R> b,
   double kv[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int k = threadIdx.x; k < K; k += blockDim.x) {
     double sre = 0.0, sim = 0.0;
+#pragma unroll 8
     for (int c = 0; c < nchunks; ++c) {
       const double2 s = d.sk_part[(size_t)c * d.kcap + k];
       sre += s.x;
@@ -1655,7 +1813,7 @@ __global__ __launch_bounds__(512) void k_export(Dev<R> d, Box<R> b, Grid g,
     const int gi = d.gid_s[i];
     const int cnt = min(d.cnt_a[i], d.cap_a);
     for (int k = 0; k < cnt; ++k) {
-      const int l = d.list_a[list_a_index(k, i, d.n)];
+      const int l = d.list_a[list_a_index(k, i, d.n)] >> StageShift<R>::value;
       const int s = slot_of_pos(t, h.nslots, l);
       const int j = t.first[s] + (l - t.off[s]);
       const int gj = d.gid_s[j];
@@ -1697,6 +1855,10 @@ inline int blocks_for(int n, int per = kBlock, int cap = 1 << 20) {
   return b > cap ? cap : b;
 }
 
+int list_b_grid(const Params& p, int nc, const Grid& g) {
+  return (p.split && nc > 0) ? list_b_blocks(nc, g.block) : 0;
+}
+
 template <typename R>
 hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, int nskin,
                       hipStream_t s) {
@@ -1710,23 +1872,12 @@ hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, int nskin,
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, d.key_count,
                      d.key_start, d.cstart, g.ncell, d.ctrl, thr);
   hipLaunchKernelGGL((k_scatter<R>), dim3(gb), dim3(kBlock), 0, s, d, thr);
-  hipLaunchKernelGGL(k_cellsort, dim3(blocks_for(g.ncell * 64, kBlock)),
-                     dim3(kBlock), 0, s, d.perm, d.inv_perm, d.key_start,
-                     g.ncell, d.ctrl, thr);
-  hipLaunchKernelGGL((k_gather<R>), dim3(gb), dim3(kBlock), 0, s, d, a.box,
-                     thr);
-  hipLaunchKernelGGL((k_list_a<R>), dim3(g.nbrick), dim3(g.block), 0, s, d,
-                     a.box, g, a.pc.r2_list_a, a.p.split, cap_a, thr);
-  if (a.p.split && d.nc > 0)
-    hipLaunchKernelGGL((k_list_b<R>),
-                       dim3(blocks_for(d.nc, kBlock / kChargedLanes)),
-                       dim3(kBlock), 0, s, d, a.box, g, a.pc.r2_list_b, cap_b,
-                       thr);
+  hipLaunchKernelGGL((k_cellsort<R>), dim3(blocks_for(g.ncell * 64, kBlock)),
+                     dim3(kBlock), 0, s, d, a.box, g.ncell, thr);
+  hipLaunchKernelGGL((k_lists<R>), dim3(g.nbrick + list_b_grid(a.p, d.nc, g)),
+                     dim3(g.block), 0, s, d, a.box, g, a.pc.r2_list_a,
+                     a.pc.r2_list_b, a.p.split, cap_a, cap_b, thr);
   return hipGetLastError();
-}
-
-int list_b_grid(const Params& p, int nc) {
-  return (p.split && nc > 0) ? list_b_blocks(nc) : 0;
 }
 
 }  // namespace
@@ -1737,9 +1888,9 @@ hipError_t launch_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   hipError_t e = build_part<R>(d, a, 1, s);
   if (e != hipSuccess) return e;
   // list statistics only (no pair partials)
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part, 0,
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part, 0, 0,
                      d.thermo, d.mean, 0, 1.0, d.lista_part, a.grid.nbrick,
-                     d.listb_part, list_b_grid(a.p, d.nc), d.ctrl,
+                     d.listb_part, list_b_grid(a.p, d.nc, a.grid), d.ctrl,
                      a.rebuild_thr);
   return hipGetLastError();
 }
@@ -1747,9 +1898,8 @@ hipError_t launch_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
 template <typename R>
 hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   const Params& p = a.p;
-  // skin check: HBM-bound read of positions and reference positions,
-  // grid-stride over at most kSkinBlocksMax blocks (their maxima are folded
-  // by every block of k_bin)
+  // skin check and gather: one thread per atom up to kSkinBlocksMax blocks
+  // (grid-stride beyond; their maxima are folded by every block of k_bin)
   const int sb = blocks_for(d.n, kBlock, kSkinBlocksMax);
   if (a.ev[0]) hipEventRecord(a.ev[0], s);
   // (a forced rebuild comes through the step ring, like the epoch)
@@ -1784,7 +1934,7 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
     hipEventRecord(a.ev[3], s);
     hipEventRecord(a.ev[4], s);
   }
-  const size_t lj_bytes = 2 * sizeof(R) * (size_t)a.pc.ntypes * a.pc.ntypes;
+  const size_t lj_bytes = 4 * sizeof(R) * (size_t)a.pc.ntypes * a.pc.ntypes;
   // ev[1] .. ev[2] bracket the pair kernel alone
   if (a.ev[1]) hipEventRecord(a.ev[1], s);
 #define RBMDX_PAIR(RBLv, EFv, SHv)                                          \
@@ -1814,9 +1964,9 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
 #undef RBMDX_PAIR
   if (a.ev[2]) hipEventRecord(a.ev[2], s);
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part, g.nbrick,
-                     d.thermo, d.mean, p.mean_fix, p.n_global, d.lista_part,
-                     g.nbrick, d.listb_part, list_b_grid(p, d.nc), d.ctrl,
-                     a.rebuild_thr);
+                     p.eflag ? 1 : 0, d.thermo, d.mean, p.mean_fix, p.n_global,
+                     d.lista_part, g.nbrick, d.listb_part,
+                     list_b_grid(p, d.nc, g), d.ctrl, a.rebuild_thr);
   if (ks_on && ks != s) hipStreamWaitEvent(s, a.ev_join, 0);
   hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,
                      s, d, p.mean_fix, ks_on ? 1 : 0, (R)a.dt, (R)a.dt_ftm);

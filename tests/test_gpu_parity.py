@@ -439,3 +439,18 @@ def test_interior_bricks_skip_min_image_bit_exact(device):
     ctx.force(step, px, py, pz, *f, m_samples=mt, sync=True)
     refF = reference_force(s, pos32, cfg, step, m)[0]
     assert rel_rms(_stack(f), refF) < 2e-5
+    # lists reused while atoms near the box edges wrap across it: with 9 cells
+    # per edge the last brick is one cell wide, so a neighbourhood that does
+    # not wrap still holds an edge cell
+    rebuilds = ctx.thermo()["rebuilds"]
+    rng = np.random.default_rng(23)
+    pos = np.mod(s.pos + rng.uniform(-0.05, 0.05, size=s.pos.shape), s.box_len)
+    wrapped = (np.abs(pos - s.pos) > 0.5 * s.box_len).any(1).sum()
+    assert wrapped > 0
+    pos32 = pos.astype(np.float32)
+    for q, t in zip((px, py, pz), range(3)):
+        q.copy_(torch.tensor(pos32[:, t], device=device))
+    ctx.force(step + 1, px, py, pz, *f, m_samples=mt, sync=True)
+    assert ctx.thermo()["rebuilds"] == rebuilds  # reuse
+    refF = reference_force(s, pos32, cfg, step + 1, m)[0]
+    assert rel_rms(_stack(f), refF) < 2e-5
