@@ -412,10 +412,11 @@ def test_graph_replays_match_eager_launches(device, monkeypatch):
         np.testing.assert_array_equal(b1, b0)
 
 
-def test_interior_bricks_skip_min_image_bit_exact(device):
-    """A box large enough for bricks whose neighbourhood neither wraps nor spans
-    half the box (those skip the minimum image): pair sets stay bit-exact and
-    forces match the FP64 oracle."""
+def test_edge_shifted_stage_sets_and_reuse_across_wraps(device):
+    """Stages shift the cells across the box edge by +-L instead of applying the
+    minimum image per pair: pair sets stay bit-exact, forces match the FP64
+    oracle, also on a list-reuse step where atoms have wrapped across the box
+    edge since the rebuild (9 cells per edge: one-cell bricks at the edge)."""
     import torch
 
     s = systems.electrolyte(36000, seed=17)  # 9 cells per edge: interior bricks exist
