@@ -29,6 +29,29 @@ sys.path.insert(0, ROOT)
 
 METRIC = "atom-steps/s, RBE+LJ force, 200k LJ-salt; %HBM peak; force err vs FP64"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pair_traffic_electrolyte200k.md")
+
+
+def measured_traffic(atoms):
+    """HBM bytes per k_pair launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same counter's
+    reading for a copy kernel of known size), when it was taken on this
+    workload; None otherwise."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    vals = {}
+    for line in text.splitlines():
+        if ":" in line and line.strip().startswith("- "):
+            key, _, v = line.strip()[2:].partition(":")
+            vals[key.strip()] = v.strip()
+    try:
+        if int(vals["atoms"]) != atoms:
+            return None
+        return int(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def parse():
@@ -131,7 +154,8 @@ def main():
     t_pair = kt.get("pair", 0.0) * 1e-6
     achieved = bytes_pair / t_pair / 1e9 if t_pair > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(n),
                 "kernel": "k_pair", "kernel_us": round(kt.get("pair", 0.0), 2),
                 "algorithmic_bytes": int(bytes_pair), "pairs_a": int(pairs_a),
                 "pairs_b": int(pairs_b)}
