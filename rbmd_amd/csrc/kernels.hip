@@ -285,6 +285,16 @@ struct Neigh {
   int mi;
 };
 
+// Brick of workgroup b of a one-brick-per-workgroup launch.  Workgroups are
+// dispatched round-robin over the 8 XCDs (workgroup b on XCD b % 8); each XCD
+// gets a contiguous range of the brick-major numbering, a compact slab of the
+// box, so the neighbourhoods it stages overlap in its L2.
+__device__ __forceinline__ int xcd_brick(int b, int nb) {
+  const int x = b & 7, k = b >> 3;
+  const int q = nb >> 3, r = nb & 7;
+  return x * q + min(x, r) + k;
+}
+
 __device__ __forceinline__ Neigh brick_neigh(const Grid& g, int b,
                                              const int e[3]) {
   Neigh h;
@@ -991,7 +1001,7 @@ __global__ __launch_bounds__(512) void k_lists(Dev<R> d, Box<R> b, Grid g,
                                                unsigned thr) {
   if (!rebuild_now(d.ctrl, thr)) return;
   if ((int)blockIdx.x < g.nbrick)
-    list_a_brick<R>(d, b, g, blockIdx.x, r2a, split, cap_a);
+    list_a_brick<R>(d, b, g, xcd_brick(blockIdx.x, g.nbrick), r2a, split, cap_a);
   else
     list_b_block<R>(d, b, g, blockIdx.x - g.nbrick, r2b, cap_b);
 }
@@ -1250,7 +1260,7 @@ __device__ __forceinline__ void load_lj(const R* tab, int ntypes, R* s) {
 // with energies E_lj, E_coul and the six virial components.
 template <typename R, bool EFLAG>
 __device__ __forceinline__ void write_partials(const Acc<R>& a, double* part,
-                                               double* smem) {
+                                               int brick, double* smem) {
   double v[11] = {(double)a.fx, (double)a.fy, (double)a.fz, a.e_lj, a.e_c,
                   a.v[0],       a.v[1],       a.v[2],       a.v[3], a.v[4],
                   a.v[5]};
@@ -1258,12 +1268,12 @@ __device__ __forceinline__ void write_partials(const Acc<R>& a, double* part,
   if (EFLAG) {
     block_sum<11>(v, smem);
     if (threadIdx.x == 0)
-      for (int k = 0; k < 11; ++k) part[k * nb + blockIdx.x] = v[k];
+      for (int k = 0; k < 11; ++k) part[k * nb + brick] = v[k];
   } else {
     double u[3] = {v[0], v[1], v[2]};
     block_sum<3>(u, smem);
     if (threadIdx.x == 0)
-      for (int k = 0; k < 3; ++k) part[k * nb + blockIdx.x] = u[k];
+      for (int k = 0; k < 3; ++k) part[k * nb + brick] = u[k];
   }
 }
 
@@ -1294,7 +1304,8 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_
   R* s_lj = reinterpret_cast<R*>(dyn_lds);
   load_lj<R>(lj_tab, pc.ntypes, s_lj);
   const int e1[3] = {1, 1, 1};
-  const Neigh h = brick_neigh(g, blockIdx.x, e1);
+  const int brick = xcd_brick(blockIdx.x, g.nbrick);
+  const Neigh h = brick_neigh(g, brick, e1);
   build_slots(g, h, d.key_start, nullptr, false, t);
   const int total = t.off[h.nslots];
   // the last stage entry is the sentinel
@@ -1415,7 +1426,7 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_
       }
     }
   }
-  write_partials<R, EFLAG>(acc, d.pair_part, smem);
+  write_partials<R, EFLAG>(acc, d.pair_part, brick, smem);
 }
 
 // One block after the pair kernel: FP64 sums of the per-brick partials in a
