@@ -121,6 +121,24 @@ int rbmdx_set_system(rbmdx_ctx* ctx, int64_t n, const double* box_lo,
 int rbmdx_set_charges(rbmdx_ctx* ctx, const double* charge);
 
 /*
+ * Special-bond topology for force_field.type "CVFF" (host arrays, caller
+ * order, copied): nspecial[3 i + 0..2] are the cumulative counts
+ * n12 <= n13 <= n14 of atom i's 1-2, 1-3 and 1-4 partners, and
+ * special[i * maxspecial + k] their global ids (the same ids as
+ * rbmdx_set_system's id).  A pair (i, j) with j among i's first n12 / n13 /
+ * n14 partners takes the weight w12 / w13 / w14 of
+ * hyper_parameters.extend.special_bonds: its LJ force is scaled by it, and
+ * (1 - w) of the full Coulomb force -qqr2e q_i q_j / r^3 is removed from every
+ * pair the lists keep.  maxspecial 0 (or all counts 0) clears the topology.
+ * Replaces the d_nspecial / d_special / maxspecial fields of DeviceData
+ * (src/data_manager/include/model/device_data.h:270-277) read by
+ * LookupSpecialWeight (src/force/src/cvff_op/rocm/cvff_op.hip.cu:51-110).
+ * maxspecial <= 64; call after rbmdx_set_system; synchronises the device.
+ */
+int rbmdx_set_special(rbmdx_ctx* ctx, int32_t maxspecial,
+                      const int32_t* nspecial, const int32_t* special);
+
+/*
  * One force evaluation.  px/py/pz are device arrays of float (precision
  * fp32) or double (precision fp64); fx/fy/fz are device arrays of the same
  * type and receive the total force (pair + k-space).  `step` is the RBL

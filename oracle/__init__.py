@@ -47,6 +47,10 @@ def lib():
         L.orc_pair_force.restype = None
         L.orc_pair_force.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F, _F, _F, _I, _I,
                                      ctypes.c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
+        L.orc_pair_force_special.restype = None
+        L.orc_pair_force_special.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F, _F, _F, _I,
+                                             _I, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int,
+                                             _I, _I, _D, _D, _D, _D, _D, _D, _D]
         L.orc_verlet_force_f64.restype = None
         L.orc_verlet_force_f64.argtypes = [ctypes.c_int, _D, _D, _D, _D, _I, ctypes.c_int,
                                            _D, _D, _D, ctypes.c_double, ctypes.c_double,
@@ -166,8 +170,11 @@ def rbl_frequency(n, L, r_core, rc, p):
     return lib().orc_rbl_frequency(n, Lp, r_core, rc, p)
 
 
-def pair_force(pos, q, types, gid, eps, sig, L, params, prec=0):
-    """Pair forces (N,3), energies (vdwl, coul), virial[6], mean[3] (FP64 out)."""
+def pair_force(pos, q, types, gid, eps, sig, L, params, prec=0, special=None):
+    """Pair forces (N,3), energies (vdwl, coul), virial[6], mean[3] (FP64 out).
+
+    special: optional CVFF topology (nspecial (N,3), special (N,maxspecial)
+    global ids, (w12, w13, w14))."""
     n = len(pos)
     x, xp = _f(pos[:, 0])
     y, yp = _f(pos[:, 1])
@@ -185,10 +192,19 @@ def pair_force(pos, q, types, gid, eps, sig, L, params, prec=0):
     en = np.zeros(2)
     vir = np.zeros(6)
     mean = np.zeros(3)
-    lib().orc_pair_force(prec, n, xp, yp, zp, qp, tp, gp, len(eps), ep, sp, Lp, ppp,
-                         fx.ctypes.data_as(_D), fy.ctypes.data_as(_D),
-                         fz.ctypes.data_as(_D), en.ctypes.data_as(_D),
-                         vir.ctypes.data_as(_D), mean.ctypes.data_as(_D))
+    outs = (fx.ctypes.data_as(_D), fy.ctypes.data_as(_D), fz.ctypes.data_as(_D),
+            en.ctypes.data_as(_D), vir.ctypes.data_as(_D), mean.ctypes.data_as(_D))
+    if special is None:
+        lib().orc_pair_force(prec, n, xp, yp, zp, qp, tp, gp, len(eps), ep, sp, Lp, ppp,
+                             *outs)
+    else:
+        ns, ids, w = special
+        nsa, nsp = _i(np.asarray(ns).reshape(-1))
+        ida, idp = _i(np.asarray(ids).reshape(-1))
+        wa, wp = _d(w)
+        ms = np.asarray(ids).shape[1]
+        lib().orc_pair_force_special(prec, n, xp, yp, zp, qp, tp, gp, len(eps), ep, sp, Lp,
+                                     ppp, ms, nsp, idp, wp, *outs)
     return np.stack([fx, fy, fz], axis=1), en, vir, mean
 
 

@@ -218,15 +218,45 @@ inline void coul_pair(T d2, T qiqj, T alpha, T qqr2e, T& f, T& e) {
   e = T(0.5) * qqr2e * qiqj * ec / r;
 }
 
+// CVFF special-bond topology of the caller-order atoms: cumulative counts
+// (n12, n13, n14) and partner global ids, weights w12, w13, w14.
+struct Special {
+  int ms = 0;
+  const int* ns = nullptr;
+  const int* ids = nullptr;
+  double w[3] = {1.0, 1.0, 1.0};
+};
+
+// Weight of the pair (i, gid_j): LoadSpecialCounts' clamping, then the first
+// matching partner decides (cvff_op.hip.cu:51-110); 1 when not special.
+inline double special_weight(const Special& sp, int i, int gid_j) {
+  if (sp.ms <= 0 || !sp.ns || !sp.ids) return 1.0;
+  int n12 = std::max(0, sp.ns[3 * i]), n13 = std::max(0, sp.ns[3 * i + 1]),
+      n14 = std::max(0, sp.ns[3 * i + 2]);
+  if (n12 > n13) n13 = n12;
+  if (n13 > n14) n14 = n13;
+  if (n14 > sp.ms) n14 = sp.ms;
+  if (n13 > n14) n13 = n14;
+  if (n12 > n13) n12 = n13;
+  const int* row = sp.ids + (size_t)i * sp.ms;
+  for (int k = 0; k < n14; ++k)
+    if (row[k] == gid_j) return k < n12 ? sp.w[0] : (k < n13 ? sp.w[1] : sp.w[2]);
+  return 1.0;
+}
+
 // Pair forces over float positions: classification in the FP32 mirror,
 // arithmetic in T (double for the error reference, float for the mirror).
+// With special bonds (CVFF, cvff_op.hip.cu:572-599 and :401-520) a pair of
+// weight ws contributes ws f_LJ + f_coul - (1 - ws) f_full, f_full =
+// -qqr2e q_i q_j / r^3 the full Coulomb force, where the RBL multiplier
+// scales ws f_LJ + f_coul only; energies ws E_LJ + E_coul - (1 - ws) E_full.
 template <typename T>
 void pair_force_f32pos(int n, const float* x, const float* y, const float* z,
                        const float* q, const int* type, const int* gid,
                        int ntypes, const double* eps, const double* sig,
                        const double* L64, const double* pp, double* fx,
                        double* fy, double* fz, double* energy, double* virial,
-                       double* mean) {
+                       double* mean, const Special& sp = Special()) {
   const PairParams p = unpack(pp);
   const Thresholds t = thresholds(p);
   const float L[3] = {(float)L64[0], (float)L64[1], (float)L64[2]};
@@ -259,8 +289,15 @@ void pair_force_f32pos(int n, const float* x, const float* y, const float* z,
     if (d2f < t.rclj2) lj_pair<T>(d2, eij, sij, flj, elj);
     const T qq = (T)q[i] * (T)q[j];
     if (p.coulomb && qq != T(0)) coul_pair<T>(d2, qq, (T)p.alpha, (T)p.qqr2e, fc, ec);
+    const T ws = (T)special_weight(sp, i, gid[j]);
+    T ffull = 0, efull = 0;
+    if (p.coulomb && qq != T(0) && ws != T(1)) {
+      const T r = std::sqrt(d2);
+      ffull = -(T)p.qqr2e * qq / (d2 * r);
+      efull = T(0.5) * (T)p.qqr2e * qq / r;
+    }
     if (w != 0.0) {
-      const T f = (T)w * (flj + fc);
+      const T f = (T)w * (ws * flj + fc) - (T(1) - ws) * ffull;
       ax[i] += f * dx;
       ay[i] += f * dy;
       az[i] += f * dz;
@@ -268,9 +305,9 @@ void pair_force_f32pos(int n, const float* x, const float* y, const float* z,
     // energy / virial: the full (non-sampled) list within cut_off, as the
     // reference's energy path (lj_cut_coul_kspace_op.hip.cu:475-565).
     if (d2f < t.rc2) {
-      evdwl += (double)elj;
-      ecoul += (double)ec;
-      const double ff = (double)(flj + fc);
+      evdwl += (double)(ws * elj);
+      ecoul += (double)(ec - (T(1) - ws) * efull);
+      const double ff = (double)(ws * flj + fc - (T(1) - ws) * ffull);
       const double d[3] = {(double)dx, (double)dy, (double)dz};
       vir[0] += -0.5 * d[0] * d[0] * ff;
       vir[1] += -0.5 * d[1] * d[1] * ff;
@@ -399,6 +436,30 @@ void orc_pair_force(int prec, int n, const float* x, const float* y,
   else
     pair_force_f32pos<double>(n, x, y, z, q, type, gid, ntypes, eps, sig, L,
                               pp, fx, fy, fz, energy, virial, mean);
+}
+
+// The same with CVFF special bonds: nspecial[3n], special[n * maxspecial]
+// (global ids), w[3] = w12, w13, w14.
+void orc_pair_force_special(int prec, int n, const float* x, const float* y,
+                            const float* z, const float* q, const int* type,
+                            const int* gid, int ntypes, const double* eps,
+                            const double* sig, const double* L,
+                            const double* pp, int maxspecial,
+                            const int* nspecial, const int* special,
+                            const double* w, double* fx, double* fy,
+                            double* fz, double* energy, double* virial,
+                            double* mean) {
+  Special sp;
+  sp.ms = maxspecial;
+  sp.ns = nspecial;
+  sp.ids = special;
+  for (int k = 0; k < 3; ++k) sp.w[k] = w[k];
+  if (prec == 1)
+    pair_force_f32pos<float>(n, x, y, z, q, type, gid, ntypes, eps, sig, L, pp,
+                             fx, fy, fz, energy, virial, mean, sp);
+  else
+    pair_force_f32pos<double>(n, x, y, z, q, type, gid, ntypes, eps, sig, L,
+                              pp, fx, fy, fz, energy, virial, mean, sp);
 }
 
 // All-double full-list LJ (+ optional erfc Coulomb): the LJ/CUT Verlet path

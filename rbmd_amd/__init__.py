@@ -95,6 +95,7 @@ SIGNATURES = {
     "rbmdx_last_error": (ctypes.c_char_p, [_P]),
     "rbmdx_set_system": (ctypes.c_int, [_P, _I64, _DP, _DP, _IP, _IP, _DP, _I32, _DP, _DP]),
     "rbmdx_set_charges": (ctypes.c_int, [_P, _DP]),
+    "rbmdx_set_special": (ctypes.c_int, [_P, _I32, _IP, _IP]),
     "rbmdx_force": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _I32,
                                    ctypes.c_uint32, _P]),
     "rbmdx_thermo": (ctypes.c_int, [_P, ctypes.POINTER(Thermo)]),
@@ -261,6 +262,15 @@ class ForceContext:
     def set_charges(self, charges):
         q = np.ascontiguousarray(charges, dtype=np.float64)
         self._check(self.lib.rbmdx_set_charges(self._h, _dptr(q)))
+
+    def set_special(self, nspecial, special):
+        """CVFF special-bond topology: nspecial (N, 3) cumulative 1-2/1-3/1-4
+        counts, special (N, maxspecial) partner global ids (caller order)."""
+        ns = np.ascontiguousarray(nspecial, dtype=np.int32).reshape(-1, 3)
+        sp = np.ascontiguousarray(special, dtype=np.int32)
+        ms = sp.shape[1] if sp.ndim == 2 else 0
+        sp = np.ascontiguousarray(sp.reshape(-1))
+        self._check(self.lib.rbmdx_set_special(self._h, ms, _iptr(ns), _iptr(sp)))
 
     def force(self, step, px, py, pz, fx, fy, fz, m_samples=None, energy=False,
               sync=False, rebuild=False, timing=False, stream=None, num_k=None):

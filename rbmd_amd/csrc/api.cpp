@@ -49,7 +49,7 @@ struct Error : std::runtime_error {
 // Configuration (RBMD JSON surface; docs/json-configuration.en.md:75-167)
 // ---------------------------------------------------------------------------
 struct Config {
-  std::string force_type;     // "LJ/CUT" | "LJ/CUT/COUL/LONG"
+  std::string force_type;     // "LJ/CUT" | "LJ/CUT/COUL/LONG" | "CVFF"
   std::string neighbor_type;  // "RBL" | "Verlet-List" | "VERLET-SOG"
   double cut_off = 0, skin = 0, r_core = 0, lj_cut_off = 0;
   int sample_num = 0;         // neighbor_sample_num
@@ -68,6 +68,10 @@ struct Config {
   double qqr2e = 1.0;
   double ftm2v = 1.0;  // force/mass -> velocity/time (unit_factor.cpp:30-50)
   bool fp64 = false;
+  // CVFF: 1-2 / 1-3 / 1-4 special-bond weights, hyper_parameters.extend
+  // .special_bonds; 1 when absent (read_special_bond_weights, cvff.cpp:303-330)
+  double w_special[3] = {1.0, 1.0, 1.0};
+  bool coulomb() const { return force_type != "LJ/CUT"; }
 };
 
 double num_of(const Json* j, const char* what) {
@@ -107,9 +111,19 @@ Config parse_config(const std::string& text) {
       throw Error(RBMDX_E_CONFIG, "unsupported unit style: " + unit);
   }
   c.force_type = str_of(hp->at({"force_field", "type"}), "force_field.type");
-  if (c.force_type != "LJ/CUT" && c.force_type != "LJ/CUT/COUL/LONG")
+  if (c.force_type != "LJ/CUT" && c.force_type != "LJ/CUT/COUL/LONG" &&
+      c.force_type != "CVFF")
     throw Error(RBMDX_E_CONFIG,
                 "force_field.type not supported by this build: " + c.force_type);
+  if (c.force_type == "CVFF") {
+    if (const Json* sb = hp->at({"extend", "special_bonds"})) {
+      if (sb->kind != Json::Array || sb->arr.size() > 3)
+        throw Error(RBMDX_E_CONFIG,
+                    "extend.special_bonds must be an array of at most 3 numbers");
+      for (size_t q = 0; q < sb->arr.size(); ++q)
+        c.w_special[q] = num_of(&sb->arr[q], "extend.special_bonds");
+    }
+  }
   const Json* nb = hp->get("neighbor");
   if (!nb) throw Error(RBMDX_E_CONFIG, "hyper_parameters.neighbor missing");
   c.neighbor_type = str_of(nb->get("type"), "neighbor.type");
@@ -151,8 +165,9 @@ Config parse_config(const std::string& text) {
       throw Error(RBMDX_E_CONFIG, "precision must be fp32 or fp64");
   }
   const Json* cl = hp->get("coulomb");
-  if (c.force_type == "LJ/CUT/COUL/LONG") {
-    if (!cl) throw Error(RBMDX_E_CONFIG, "LJ/CUT/COUL/LONG needs hyper_parameters.coulomb");
+  if (c.coulomb()) {
+    if (!cl)
+      throw Error(RBMDX_E_CONFIG, c.force_type + " needs hyper_parameters.coulomb");
     c.coulomb_type = str_of(cl->get("type"), "coulomb.type");
     if (c.coulomb_type != "RBE" && c.coulomb_type != "EWALD" &&
         c.coulomb_type != "RBSOG")
@@ -272,6 +287,8 @@ struct EngineBase {
                                const void* pz, int candidates, int32_t* gi,
                                int32_t* gj, int32_t* cls, int64_t cap) = 0;
   virtual void set_charges(const double* q) = 0;
+  virtual void set_special(int32_t ms, const int32_t* nspecial,
+                           const int32_t* special) = 0;
   virtual int kernel_times(const char** names, double* usec, int n) = 0;
   virtual void rbe_sample(int64_t call, int32_t* m) = 0;
   virtual int num_samples() const = 0;
@@ -358,6 +375,58 @@ class Engine : public EngineBase {
     upload_charges();
     charge_sums();
     hipDeviceSynchronize();
+  }
+
+  // CVFF special-bond topology (DeviceData d_nspecial / d_special /
+  // maxspecial, device_data.h:270-277), caller order.  Counts are clamped as
+  // LoadSpecialCounts clamps them (cvff_op.hip.cu:51-77); the rows are
+  // copied to sorted order at the next rebuild, which this call forces.
+  void set_special(int32_t ms, const int32_t* nspecial,
+                   const int32_t* special) override {
+    if (!ready_) throw Error(RBMDX_E_STATE, "set_system first");
+    if (cfg_.force_type != "CVFF")
+      throw Error(RBMDX_E_STATE, "special bonds apply to force_field.type CVFF only");
+    if (ms < 0 || ms > kMaxSpecial)
+      throw Error(RBMDX_E_ARG, "maxspecial must be in [0, 64]");
+    if (ms > 0 && (!nspecial || !special))
+      throw Error(RBMDX_E_ARG, "null special-bond arrays");
+    HIPCHK(hipSetDevice(device_));
+    // evaluations in flight may still read the current rows
+    HIPCHK(hipDeviceSynchronize());
+    std::vector<int4> cnt(n_);
+    bool any = false;
+    for (int i = 0; i < n_ && ms > 0; ++i) {
+      int n12 = std::max(0, nspecial[3 * i]), n13 = std::max(0, nspecial[3 * i + 1]),
+          n14 = std::max(0, nspecial[3 * i + 2]);
+      if (n12 > n13) n13 = n12;
+      if (n13 > n14) n14 = n13;
+      if (n14 > ms) n14 = ms;
+      if (n13 > n14) n13 = n14;
+      if (n12 > n13) n12 = n13;
+      cnt[i] = make_int4(n12, n13, n14, 0);
+      any |= n14 > 0;
+    }
+    for (int* p : {d_.spid, d_.spid_s})
+      if (p) release(p);
+    for (int4* p : {d_.spn, d_.spn_s})
+      if (p) release(p);
+    d_.spn = d_.spn_s = nullptr;
+    d_.spid = d_.spid_s = nullptr;
+    params_.special = 0;
+    pc_.ms = 0;
+    if (any) {
+      d_.spn = alloc<int4>(n_);
+      d_.spn_s = alloc<int4>(n_);
+      d_.spid = alloc<int>((size_t)n_ * ms);
+      d_.spid_s = alloc<int>((size_t)n_ * ms);
+      HIPCHK(hipMemcpy(d_.spn, cnt.data(), sizeof(int4) * n_, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_.spid, special, sizeof(int) * (size_t)n_ * ms,
+                       hipMemcpyHostToDevice));
+      params_.special = 1;
+      pc_.ms = ms;
+    }
+    lists_stale_ = true;
+    ++gen_;
   }
 
   int num_samples() const override {
@@ -769,7 +838,7 @@ class Engine : public EngineBase {
   void setup() {
     HIPCHK(hipSetDevice(device_));
     const Config& c = cfg_;
-    const bool coulomb = c.force_type == "LJ/CUT/COUL/LONG";
+    const bool coulomb = c.coulomb();
     kspace_ = !coulomb ? 0
               : c.coulomb_type == "RBE" ? 1
               : c.coulomb_type == "EWALD" ? 2
@@ -940,6 +1009,13 @@ class Engine : public EngineBase {
     params_.kspace = kspace_;
     params_.mean_fix = params_.rbl;
     params_.split = split_ ? 1 : 0;
+    params_.special = 0;
+    d_.spn = d_.spn_s = nullptr;
+    d_.spid = d_.spid_s = nullptr;
+    pc_.ms = 0;
+    pc_.w12 = (R)c.w_special[0];
+    pc_.w13 = (R)c.w_special[1];
+    pc_.w14 = (R)c.w_special[2];
     params_.qqr2e = c.qqr2e;
     params_.alpha = alpha_;
     params_.volume = V;
@@ -1356,6 +1432,12 @@ int rbmdx_set_system(rbmdx_ctx* ctx, int64_t n, const double* box_lo,
       ctx->ef->set_system(n, box_lo, box_len, type, id, charge, ntypes, eps, sigma);
     }
   });
+}
+
+int rbmdx_set_special(rbmdx_ctx* ctx, int32_t maxspecial, const int32_t* nspecial,
+                      const int32_t* special) {
+  if (!ctx) return RBMDX_E_ARG;
+  return guarded(ctx, [&] { ctx->eng->set_special(maxspecial, nspecial, special); });
 }
 
 int rbmdx_set_charges(rbmdx_ctx* ctx, const double* charge) {

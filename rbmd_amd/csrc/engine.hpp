@@ -12,6 +12,7 @@ namespace rbmdx {
 constexpr int kBlock = 256;      // threads per block for per-atom kernels
 constexpr int kSkinBlocksMax = 1024;  // skin-check grid (grid-stride beyond)
 constexpr int kMaxTypes = 32;    // LJ type table held in LDS
+constexpr int kMaxSpecial = 64;  // CVFF special partners per atom
 constexpr int kSkTile = 8;       // k-vectors per S(k) block
 constexpr int kSkChunk = 1024;   // charged atoms per S(k) block
 constexpr int kRbeWaves = 8;     // waves per RBE-force block (k split)
@@ -100,6 +101,9 @@ struct PairConst {
   unsigned c32;       // 2^32 mod ifreq
   long long epoch;
   int ntypes;
+  // CVFF special bonds: weights of 1-2, 1-3, 1-4 partners and the row width
+  R w12 = 1, w13 = 1, w14 = 1;
+  int ms = 0;
 };
 
 // All device buffers of one context (sorted order unless noted).
@@ -125,6 +129,8 @@ struct Dev {
   int* gid = nullptr;      // [n]
   R* charge = nullptr;     // [n]
   R4* ref = nullptr;       // [n] positions at the last rebuild
+  int4* spn = nullptr;     // [n] CVFF special counts (n12, n13, n14, 0)
+  int* spid = nullptr;     // [n * ms] CVFF special partners (global ids)
   int* key = nullptr;      // [n] cell key of the last rebuild
   int* slot = nullptr;     // [n] arrival slot inside the key
   int* inv_perm = nullptr; // [n] caller -> sorted
@@ -141,6 +147,8 @@ struct Dev {
   unsigned long long* hjc = nullptr;   // [nc] the same, charged order
   int* s2c = nullptr;      // [n] sorted -> charged compact, -1 if neutral
   R4* xs = nullptr;        // [n] (x, y, z, type)
+  int4* spn_s = nullptr;   // [n] spn in sorted order
+  int* spid_s = nullptr;   // [n * ms] spid in sorted order
   R4* fa = nullptr;        // [n] total pair force
   // charged compact
   int* c2s = nullptr;      // [nc]
@@ -189,6 +197,7 @@ struct Params {
   int eflag = 0;
   int mean_fix = 0;
   int split = 0;        // charged/neutral partition of the pair work
+  int special = 0;      // CVFF special-bond weights apply
   int num_k = 0;
   double qqr2e = 1.0;
   double alpha = 0.0;

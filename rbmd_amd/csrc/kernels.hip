@@ -430,11 +430,15 @@ __device__ __forceinline__ int pair_woff(double w) { return (int)w; }
 // not whole) are shifted by +-L, so that stage coordinates are continuous.
 // The pair kernel also gets a sentinel entry at StageCap - 1, far from every
 // atom: list A pads its last group of 8 with it.
+// With sgid, the atoms' global ids are staged beside them (gid_s in sorted
+// order; the sentinel's id is -1).
 template <typename R, bool PAIR, int NS>
 __device__ __forceinline__ void stage_atoms(const Grid& g, const Box<R>& b,
                                             const Neigh& h,
                                             const SlotTables<NS>& t,
-                                            const R4T<R>* xs, R4T<R>* stage) {
+                                            const R4T<R>* xs, R4T<R>* stage,
+                                            const int* gid_s = nullptr,
+                                            int* sgid = nullptr) {
   // half a wave per slot: the slot's atoms are consecutive in sorted order
   const int lane = threadIdx.x & 31;
   for (int s = threadIdx.x >> 5; s < h.nslots; s += blockDim.x >> 5) {
@@ -456,10 +460,13 @@ __device__ __forceinline__ void stage_atoms(const Grid& g, const Box<R>& b,
       else
         p.w = p.w != (R)(int)p.w ? R(1e30) : R(0);
       stage[o + j] = p;
+      if (sgid) sgid[o + j] = gid_s[f + j];
     }
   }
-  if (PAIR && threadIdx.x == 0)
+  if (PAIR && threadIdx.x == 0) {
     stage[StageCap<R>::value - 1] = make4<R>(R(1e18), R(1e18), R(1e18), pair_w(R(0)));
+    if (sgid) sgid[StageCap<R>::value - 1] = -1;
+  }
   __syncthreads();
 }
 
@@ -683,10 +690,11 @@ __global__ __launch_bounds__(256) void k_scatter(Dev<R> d, unsigned thr) {
 // across the wave; serial by lane 0 beyond 64 atoms), then the cell's atoms
 // are gathered into the sorted tiles: positions (and the charged copy),
 // reference positions for the skin check, sorted ids, RBL hashes and cells,
-// the charged map, inv_perm.
+// the charged map, inv_perm, and the CVFF special-bond rows (ms per atom).
 template <typename R>
 __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
-                                                  int ncell, unsigned thr) {
+                                                  int ncell, unsigned thr,
+                                                  int ms) {
   if (!rebuild_now(d.ctrl, thr)) return;
   int* perm = d.perm;
   const int* key_start = d.key_start;
@@ -749,6 +757,11 @@ __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
     }
     d.s2c[s] = c;
     d.c_of[i] = c;
+    if (d.spn) {
+      d.spn_s[s] = d.spn[i];
+      for (int q = 0; q < ms; ++q)
+        d.spid_s[(size_t)s * ms + q] = d.spid[(size_t)i * ms + q];
+    }
     // .w: the type, plus one half for a charged atom (as in k_skin)
     d.xs[s] = make4<R>(x, y, z, (R)t + (c >= 0 ? R(0.5) : R(0)));
   }
@@ -1040,14 +1053,45 @@ __device__ __forceinline__ void add_virial(Acc<R>& a, R dx, R dy, R dz, R f) {
   }
 }
 
+// CVFF special-bond row of one atom: cumulative counts n12 <= n13 <= n14 of
+// its 1-2, 1-3 and 1-4 partners (clamped on the host as LoadSpecialCounts
+// clamps them, cvff_op.hip.cu:51-77) and their global ids.
+struct SpecRow {
+  int n12 = 0, n13 = 0, n14 = 0;
+  const int* ids = nullptr;
+};
+
+template <typename R>
+__device__ __forceinline__ SpecRow spec_row(const Dev<R>& d, int ms, int s) {
+  SpecRow r;
+  const int4 c = d.spn_s[s];
+  r.n12 = c.x;
+  r.n13 = c.y;
+  r.n14 = c.z;
+  r.ids = d.spid_s + (size_t)s * ms;
+  return r;
+}
+
+// Weight of the pair (i, gid) for i's row: the first match decides, 1 when
+// gid is no special partner (LookupSpecialWeight, cvff_op.hip.cu:79-110).
+template <typename R>
+__device__ __forceinline__ R special_w(const SpecRow& r, int gid,
+                                       const PairConst<R>& pc) {
+  for (int k = 0; k < r.n14; ++k)
+    if (r.ids[k] == gid) return k < r.n12 ? pc.w12 : (k < r.n13 ? pc.w13 : pc.w14);
+  return R(1);
+}
+
 // LJ-only pair of list A.  hj() yields j's selection hash (evaluated only
-// for RBL shell pairs).
-template <typename R, bool RBL, bool EFLAG, typename GidJ>
+// for RBL shell pairs); with SPECIAL, gj is j's global id and the LJ force
+// carries i's special-bond weight for it.
+template <typename R, bool RBL, bool EFLAG, bool SPECIAL, typename GidJ>
 __device__ __forceinline__ void interact_a(const PairConst<R>& pc,
                                            const Box<R>& b, const R* s_lj,
                                            int ti, const R4T<R>& xi,
                                            unsigned long long hi,
                                            const R4T<R>& xj, GidJ hj,
+                                           const SpecRow& sr, int gj,
                                            Acc<R>& a) {
   R dx, dy, dz;
   const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
@@ -1058,14 +1102,15 @@ __device__ __forceinline__ void interact_a(const PairConst<R>& pc,
   const R inv = R(1) / d2;
   const R s2 = sig2 * inv;
   const R s6 = s2 * s2 * s2;
-  const R flj = m24 * (R(2) * s6 - R(1)) * s6 * inv;
+  const R ws = SPECIAL ? special_w<R>(sr, gj, pc) : R(1);
+  const R flj = ws * (m24 * (R(2) * s6 - R(1)) * s6 * inv);
   const R w = pair_weight<R, RBL>(pc, d2, hi, hj);
   const R f = w * flj;
   a.fx += f * dx;
   a.fy += f * dy;
   a.fz += f * dz;
   if (EFLAG) {
-    a.e_lj += (double)(R(2) * eps * (s6 - R(1)) * s6);
+    a.e_lj += (double)(ws * (R(2) * eps * (s6 - R(1)) * s6));
     add_virial<R, EFLAG>(a, dx, dy, dz, flj);
   }
 }
@@ -1102,15 +1147,20 @@ __device__ __forceinline__ void coul_sog(const PairConst<R>& pc, R d2, R qq,
   }
 }
 
-// LJ + Coulomb pair of list B (both atoms charged).
-template <typename R, bool RBL, bool EFLAG>
+// LJ + Coulomb pair of list B (both atoms charged).  With SPECIAL (CVFF),
+// gj is j's global id; a special-bond weight ws < 1 scales the LJ term and
+// removes (1 - ws) of the full Coulomb force -qqr2e q_i q_j / r^3 from every
+// pair the lists keep, without the RBL multiplier
+// (cvff_op.hip.cu:572-599; energies :401-520).
+template <typename R, bool RBL, bool EFLAG, bool SPECIAL = false>
 __device__ __forceinline__ void interact_b(const PairConst<R>& pc,
                                            const Box<R>& b, const R* s_lj,
                                            int ti, const R4T<R>& xi,
                                            unsigned long long hi,
                                            const R4T<R>& xj, int tj,
                                            const unsigned long long* hjp,
-                                           Acc<R>& a) {
+                                           Acc<R>& a, const SpecRow& sr = SpecRow(),
+                                           int gj = 0) {
   R dx, dy, dz;
   const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
   if (!(d2 < pc.rc2)) return;
@@ -1130,14 +1180,35 @@ __device__ __forceinline__ void interact_b(const PairConst<R>& pc,
   else
     coul<R>(pc, d2, xi.w * xj.w, fc, ec);
   const R w = pair_weight<R, RBL>(pc, d2, hi, [&] { return *hjp; });
-  const R f = w * (flj + fc);
+  if (!SPECIAL) {
+    const R f = w * (flj + fc);
+    a.fx += f * dx;
+    a.fy += f * dy;
+    a.fz += f * dz;
+    if (EFLAG) {
+      a.e_lj += (double)elj;
+      a.e_c += (double)ec;
+      add_virial<R, EFLAG>(a, dx, dy, dz, flj + fc);
+    }
+    return;
+  }
+  const R ws = special_w<R>(sr, gj, pc);
+  R ffull = R(0), efull = R(0);
+  if (ws != R(1)) {
+    const R qq = xi.w * xj.w;
+    const R rinv = R(1) / sqrt(d2);
+    ffull = -pc.qqr2e * qq * rinv * rinv * rinv;
+    efull = R(0.5) * pc.qqr2e * qq * rinv;
+  }
+  const R corr = (R(1) - ws) * ffull;
+  const R f = w * (ws * flj + fc) - (w != R(0) ? corr : R(0));
   a.fx += f * dx;
   a.fy += f * dy;
   a.fz += f * dz;
   if (EFLAG) {
-    a.e_lj += (double)elj;
-    a.e_c += (double)ec;
-    add_virial<R, EFLAG>(a, dx, dy, dz, flj + fc);
+    a.e_lj += (double)(ws * elj);
+    a.e_c += (double)(ec - (R(1) - ws) * efull);
+    add_virial<R, EFLAG>(a, dx, dy, dz, ws * flj + fc - corr);
   }
 }
 
@@ -1165,12 +1236,15 @@ struct R2Of<double> {
 // The LJ table row of type pair (ti, tj) holds (-24 eps, sigma^2, eps, 0).
 // Without list-A shell pairs and energies the force window is one interval
 // (a_lo, a_hi) of d^2 (the cutoffs, r_core and the coincident-pair guard
-// folded on the host).
-template <typename R, bool RBL, bool EFLAG, bool SHELL, bool STAGED, bool MI>
+// folded on the host).  With SPECIAL (CVFF), the LJ force carries i's
+// special-bond weight for j (global ids staged beside the positions).
+template <typename R, bool RBL, bool EFLAG, bool SHELL, bool STAGED, bool MI,
+          bool SPECIAL>
 __device__ __forceinline__ void list_a_atom(
     const Dev<R>& d, const Box<R>& b, const PairConst<R>& pc, const R* s_lj,
     const SlotTables<kMaxSlots>& t, int nslots, const R4T<R>* stage, int i,
-    int cnt, const R4T<R>& xi, int ti, unsigned long long hi, Acc<R>& a) {
+    int cnt, const R4T<R>& xi, int ti, unsigned long long hi, Acc<R>& a,
+    const SpecRow& sr, const int* sgid) {
   using R2 = typename R2Of<R>::type;
   using R4 = R4T<R>;
   constexpr int sh = StageShift<R>::value;
@@ -1215,7 +1289,12 @@ __device__ __forceinline__ void list_a_atom(
       const R inv = pair_rcp(d2);
       const R s2 = lj.y * inv;
       const R s6 = s2 * s2 * s2;
-      const R flj = lj.x * (R(2) * s6 - R(1)) * s6 * inv;
+      R ws = R(1);
+      if (SPECIAL) {
+        const int gj = STAGED ? sgid[lj_[kk]] : d.gid_s[lj_[kk]];
+        ws = special_w<R>(sr, gj, pc);
+      }
+      const R flj = ws * (lj.x * (R(2) * s6 - R(1)) * s6 * inv);
       if (!SHELL && !EFLAG) {
         const bool ok = (STAGED || kk < kn) && d2 < pc.a_hi && d2 > pc.a_lo;
         const R f = ok ? flj : R(0);
@@ -1241,7 +1320,7 @@ __device__ __forceinline__ void list_a_atom(
       a.fz = fma_r(f, dz, a.fz);
       if (EFLAG) {
         const R eps = *reinterpret_cast<const R*>(lj_row + woff + 2 * sizeof(R));
-        a.e_lj += in ? (double)(R(2) * eps * (s6 - R(1)) * s6) : 0.0;
+        a.e_lj += in ? (double)(ws * (R(2) * eps * (s6 - R(1)) * s6)) : 0.0;
         add_virial<R, EFLAG>(a, dx, dy, dz, in ? flj : R(0));
       }
     }
@@ -1289,18 +1368,20 @@ struct kPairWaves {
 // brick (one lane per atom, neighbours from the LDS stage).  Phase 2: the
 // brick's charged atoms add LJ + Coulomb over list B, kChargedLanes lanes per
 // atom.  fa[i] receives the total pair force of atom i.
-template <typename R, bool RBL, bool EFLAG, bool SHELL>
+// SPECIAL: CVFF special-bond weights (partners looked up by global id).
+template <typename R, bool RBL, bool EFLAG, bool SHELL, bool SPECIAL>
 __global__ __launch_bounds__(512)
-__attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_pair(Dev<R> d, Box<R> b, Grid g,
+__attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value))) void k_pair(Dev<R> d, Box<R> b, Grid g,
                                               PairConst<R> pc_arg,
                                               const R* lj_tab, int split) {
   // the sampling epoch of this evaluation comes from the step ring (k_skin)
   PairConst<R> pc = pc_arg;
   if (RBL) pc.epoch = d.ctrl->epoch;
   __shared__ R4T<R> stage[StageCap<R>::value];
+  __shared__ int sgid[SPECIAL ? StageCap<R>::value : 1];
   __shared__ SlotTables<kMaxSlots> t;
   __shared__ double smem[8 * 11];
-  extern __shared__ char dyn_lds[];  // LJ table: 2 * ntypes^2 values of R
+  extern __shared__ char dyn_lds[];  // LJ table: 4 * ntypes^2 values of R
   R* s_lj = reinterpret_cast<R*>(dyn_lds);
   load_lj<R>(lj_tab, pc.ntypes, s_lj);
   const int e1[3] = {1, 1, 1};
@@ -1310,7 +1391,9 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_
   const int total = t.off[h.nslots];
   // the last stage entry is the sentinel
   const bool staged = total < StageCap<R>::value;
-  if (staged) stage_atoms<R, true>(g, b, h, t, d.xs, stage);
+  if (staged)
+    stage_atoms<R, true>(g, b, h, t, d.xs, stage, SPECIAL ? d.gid_s : nullptr,
+                         SPECIAL ? sgid : nullptr);
   const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
   Acc<R> acc;
   // phase 1: LJ over list A
@@ -1320,16 +1403,17 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_
     const int ti = (int)xi.w;
     const unsigned long long hi = RBL ? rbl_hi(d.gid_s[i]) : 0ull;
     const int cnt = d.cnt_a[i];
+    const SpecRow sr = SPECIAL ? spec_row<R>(d, pc.ms, i) : SpecRow();
     if (cnt <= d.cap_a) {
       if (staged && !h.mi)
-        list_a_atom<R, RBL, EFLAG, SHELL, true, false>(d, b, pc, s_lj, t, h.nslots,
-                                                       stage, i, cnt, xi, ti, hi, a);
+        list_a_atom<R, RBL, EFLAG, SHELL, true, false, SPECIAL>(
+            d, b, pc, s_lj, t, h.nslots, stage, i, cnt, xi, ti, hi, a, sr, sgid);
       else if (staged)
-        list_a_atom<R, RBL, EFLAG, SHELL, true, true>(d, b, pc, s_lj, t, h.nslots,
-                                                      stage, i, cnt, xi, ti, hi, a);
+        list_a_atom<R, RBL, EFLAG, SHELL, true, true, SPECIAL>(
+            d, b, pc, s_lj, t, h.nslots, stage, i, cnt, xi, ti, hi, a, sr, sgid);
       else
-        list_a_atom<R, RBL, EFLAG, SHELL, false, true>(d, b, pc, s_lj, t, h.nslots,
-                                                       stage, i, cnt, xi, ti, hi, a);
+        list_a_atom<R, RBL, EFLAG, SHELL, false, true, SPECIAL>(
+            d, b, pc, s_lj, t, h.nslots, stage, i, cnt, xi, ti, hi, a, sr, sgid);
     } else {
       // list overflow since the last rebuild: scan the 27 cells directly
       int c[3];
@@ -1346,8 +1430,9 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_
             for (int k = qi ? t.nq[sl] : 0; k < len; ++k) {
               const int jg = t.first[sl] + k;
               if (jg == i) continue;
-              interact_a<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, hi, d.xs[jg],
-                                        [&] { return d.hj_s[jg]; }, a);
+              interact_a<R, RBL, EFLAG, SPECIAL>(pc, b, s_lj, ti, xi, hi, d.xs[jg],
+                                                 [&] { return d.hj_s[jg]; }, sr,
+                                                 SPECIAL ? d.gid_s[jg] : 0, a);
             }
           }
     }
@@ -1372,12 +1457,14 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_
       const int ti = d.typec[c];
       const unsigned long long hi = RBL ? rbl_hi(d.gidc[c]) : 0ull;
       const int cnt = d.cnt_b[c];
+      const SpecRow sr = SPECIAL ? spec_row<R>(d, pc.ms, d.c2s[c]) : SpecRow();
       if (cnt <= d.cap_b) {
         const int* lp = d.list_b + (size_t)c * d.cap_b;
         for (int k = gl; k < cnt; k += kChargedLanes) {
           const int e = lp[k];
-          interact_b<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, hi, d.xqc[e], d.typec[e],
-                                    d.hjc + e, a);
+          interact_b<R, RBL, EFLAG, SPECIAL>(pc, b, s_lj, ti, xi, hi, d.xqc[e],
+                                             d.typec[e], d.hjc + e, a, sr,
+                                             SPECIAL ? d.gidc[e] : 0);
         }
       } else {
         const int i = d.c2s[c];
@@ -1396,8 +1483,9 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL>::value))) void k_
           for (int j = js; j < je; ++j) {
             if (j == i) continue;
             const int e = d.s2c[j];
-            interact_b<R, RBL, EFLAG>(pc, b, s_lj, ti, xi, hi, d.xqc[e],
-                                      d.typec[e], d.hjc + e, a);
+            interact_b<R, RBL, EFLAG, SPECIAL>(pc, b, s_lj, ti, xi, hi, d.xqc[e],
+                                               d.typec[e], d.hjc + e, a, sr,
+                                               SPECIAL ? d.gidc[e] : 0);
           }
         }
       }
@@ -1884,7 +1972,7 @@ hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, int nskin,
                      d.key_start, d.cstart, g.ncell, d.ctrl, thr);
   hipLaunchKernelGGL((k_scatter<R>), dim3(gb), dim3(kBlock), 0, s, d, thr);
   hipLaunchKernelGGL((k_cellsort<R>), dim3(blocks_for(g.ncell * 64, kBlock)),
-                     dim3(kBlock), 0, s, d, a.box, g.ncell, thr);
+                     dim3(kBlock), 0, s, d, a.box, g.ncell, thr, a.pc.ms);
   hipLaunchKernelGGL((k_lists<R>), dim3(g.nbrick + list_b_grid(a.p, d.nc, g)),
                      dim3(g.block), 0, s, d, a.box, g, a.pc.r2_list_a,
                      a.pc.r2_list_b, a.p.split, cap_a, cap_b, thr);
@@ -1948,29 +2036,42 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   const size_t lj_bytes = 4 * sizeof(R) * (size_t)a.pc.ntypes * a.pc.ntypes;
   // ev[1] .. ev[2] bracket the pair kernel alone
   if (a.ev[1]) hipEventRecord(a.ev[1], s);
-#define RBMDX_PAIR(RBLv, EFv, SHv)                                          \
-  hipLaunchKernelGGL((k_pair<R, RBLv, EFv, SHv>), dim3(g.nbrick),            \
+#define RBMDX_PAIR(RBLv, EFv, SHv, SPv)                                     \
+  hipLaunchKernelGGL((k_pair<R, RBLv, EFv, SHv, SPv>), dim3(g.nbrick),       \
                      dim3(g.block), lj_bytes, s, d, a.box, g, a.pc, a.lj_tab, \
                      p.split)
   // list A holds RBL shell pairs only when the LJ cutoff exceeds r_core
   const bool shell_a = a.pc.rclj2 > a.pc.rs2;
-  if (p.rbl) {
+  if (p.special) {
+    // CVFF special bonds: the general (shell-capable) list-A loop
+    if (p.rbl) {
+      if (p.eflag)
+        RBMDX_PAIR(true, true, true, true);
+      else
+        RBMDX_PAIR(true, false, true, true);
+    } else {
+      if (p.eflag)
+        RBMDX_PAIR(false, true, true, true);
+      else
+        RBMDX_PAIR(false, false, true, true);
+    }
+  } else if (p.rbl) {
     if (p.eflag) {
       if (shell_a)
-        RBMDX_PAIR(true, true, true);
+        RBMDX_PAIR(true, true, true, false);
       else
-        RBMDX_PAIR(true, true, false);
+        RBMDX_PAIR(true, true, false, false);
     } else {
       if (shell_a)
-        RBMDX_PAIR(true, false, true);
+        RBMDX_PAIR(true, false, true, false);
       else
-        RBMDX_PAIR(true, false, false);
+        RBMDX_PAIR(true, false, false, false);
     }
   } else {
     if (p.eflag)
-      RBMDX_PAIR(false, true, false);
+      RBMDX_PAIR(false, true, false, false);
     else
-      RBMDX_PAIR(false, false, false);
+      RBMDX_PAIR(false, false, false, false);
   }
 #undef RBMDX_PAIR
   if (a.ev[2]) hipEventRecord(a.ev[2], s);

@@ -6,6 +6,10 @@
                     uniform positions with overlap rejection (random sequential
                     addition with a minimum distance d_min) at rho* = 0.8.
 * ``ljsalt_config`` the RBMD JSON configuration used with ``electrolyte``.
+* ``spce_water``    rigid SPC/E water (REAL units: Angstrom, kcal/mol) on a cubic
+                    lattice with random orientations, and its CVFF special-bond
+                    topology (1-2: O-H, 1-3: H-H).
+* ``cvff_water_config`` the RBMD JSON for ``spce_water`` (CVFF, RBL, RBE/RBSOG).
 
 Atom ids are 0-based, as RBMD makes them when it reads a LAMMPS data file
 (src/reader/src/format/lammps/lammps_full_reader.cpp:1543).
@@ -141,6 +145,97 @@ def lj_config(neighbor="Verlet-List", cut_off=2.5, skin=0.3, precision="fp64"):
         "hyper_parameters": {
             "force_field": {"type": "LJ/CUT"},
             "neighbor": {"type": neighbor, "cut_off": cut_off, "skin": skin},
+            "precision": precision,
+        },
+    }
+
+
+def _random_rotations(n, rng):
+    """n uniformly distributed rotation matrices (unit quaternions)."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1),
+    ], axis=1)
+
+
+class Water(System):
+    """A System with CVFF special-bond topology: nspecial (N, 3) cumulative
+    1-2/1-3/1-4 counts and special (N, maxspecial) partner ids."""
+
+    def __init__(self, *a, nspecial=None, special=None, **kw):
+        super().__init__(*a, **kw)
+        self.nspecial = np.asarray(nspecial, dtype=np.int32)
+        self.special = np.asarray(special, dtype=np.int32)
+
+
+def spce_water(n_mol=24327, density=33.4, seed=3):
+    """Rigid SPC/E water: O-H 1.0 A, H-O-H 109.47 deg, q_O = -0.8476,
+    q_H = +0.4238; O-O eps 0.15535 kcal/mol, sigma 3.166 A; H carries no LJ.
+    n_mol molecules on the first sites of a cubic m^3 lattice (m the smallest
+    integer with m^3 >= n_mol) in a box of density molecules per nm^3, each
+    with a random orientation.  Atoms O, H1, H2 per molecule, ids 0-based.
+    Special bonds: O -> (H1, H2) as 1-2; H1 -> O as 1-2 and H2 as 1-3."""
+    rng = np.random.default_rng(seed)
+    L = (n_mol / (density * 1e-3)) ** (1.0 / 3.0)
+    m = int(math.ceil(n_mol ** (1.0 / 3.0) - 1e-9))
+    a = L / m
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"),
+                 axis=-1).reshape(-1, 3)[:n_mol]
+    o = (g + 0.5) * a
+    half = math.radians(109.47) / 2.0
+    ref = np.array([[0.0, 0.0, 0.0],
+                    [math.sin(half), 0.0, math.cos(half)],
+                    [-math.sin(half), 0.0, math.cos(half)]])
+    rot = _random_rotations(n_mol, rng)
+    pos = o[:, None, :] + np.einsum("nij,kj->nki", rot, ref)
+    pos = np.mod(pos.reshape(-1, 3), L)
+    n = 3 * n_mol
+    types = np.tile(np.array([0, 1, 1], dtype=np.int32), n_mol)
+    q = np.tile(np.array([-0.8476, 0.4238, 0.4238]), n_mol)
+    ids = np.arange(n, dtype=np.int32)
+    base = 3 * np.arange(n_mol, dtype=np.int32)
+    nspecial = np.tile(np.array([[2, 2, 2], [1, 2, 2], [1, 2, 2]], dtype=np.int32),
+                       (n_mol, 1))
+    special = np.empty((n, 2), dtype=np.int32)
+    special[0::3] = np.stack([base + 1, base + 2], axis=1)
+    special[1::3] = np.stack([base, base + 2], axis=1)
+    special[2::3] = np.stack([base, base + 1], axis=1)
+    return Water(pos, [0, 0, 0], [L, L, L], types, ids, q, [0.15535, 0.0], [3.166, 0.0],
+                 mass=[15.9994, 1.008], name=f"spce_water_{n_mol}", nspecial=nspecial,
+                 special=special)
+
+
+def cvff_water_config(neighbor="RBL", coulomb="RBE", cut_off=10.0, r_core=6.5,
+                      sample_num=100, skin=2.0, alpha=None, accuracy=1e-4, P=300,
+                      rbsog_level=1, special_bonds=(0.0, 0.0, 0.0), seed=2026102301,
+                      precision="fp32"):
+    """RBMD JSON for SPC/E water: CVFF in REAL units, rc 10 A, RBL with
+    r_core 6.5 A and 100 samples, skin 2 A, special_bonds [0, 0, 0] (the
+    reference's benchmark/systems/water.json keys)."""
+    nb = {"type": neighbor, "cut_off": cut_off, "skin": skin}
+    if neighbor == "RBL":
+        nb.update({"r_core": r_core, "neighbor_sample_num": sample_num,
+                   "energy_rbl_flag": "no"})
+    cl = {"type": coulomb, "random_seed": seed}
+    if alpha is not None:
+        cl["alpha"] = alpha
+    elif coulomb != "RBSOG" and neighbor != "VERLET-SOG":
+        cl["accuracy"] = accuracy
+    if coulomb == "RBE":
+        cl.update({"coulomb_sample_num": P, "energy_rbe_flag": "no"})
+    if coulomb == "RBSOG" or neighbor == "VERLET-SOG":
+        cl.update({"coulomb_sample_num": P, "rbsog_level": rbsog_level})
+    return {
+        "init_configuration": {"read_data": {"unit": "REAL", "atom_style": "full"}},
+        "hyper_parameters": {
+            "force_field": {"type": "CVFF", "mixing_rule": "arithmetic"},
+            "neighbor": nb,
+            "coulomb": cl,
+            "extend": {"special_bonds": list(special_bonds)},
             "precision": precision,
         },
     }

@@ -11,6 +11,26 @@ def rel_rms(f, ref):
     return float(np.sqrt(((f - ref) ** 2).sum() / (ref ** 2).sum()))
 
 
+QQR2E = {"LJ": 1.0, "REAL": 332.06371, "METAL": 14.399645}
+
+
+def cfg_qqr2e(cfg):
+    unit = cfg.get("init_configuration", {}).get("read_data", {}).get("unit", "LJ")
+    return QQR2E[unit]
+
+
+def cfg_special(cfg, sys_):
+    """(nspecial, special, (w12, w13, w14)) for a CVFF configuration of a
+    system with topology, else None."""
+    hp = cfg["hyper_parameters"]
+    if hp["force_field"]["type"] != "CVFF" or getattr(sys_, "nspecial", None) is None:
+        return None
+    w = [1.0, 1.0, 1.0]
+    for k, v in enumerate(hp.get("extend", {}).get("special_bonds", [])[:3]):
+        w[k] = float(v)
+    return sys_.nspecial, sys_.special, tuple(w)
+
+
 def cfg_values(cfg):
     hp = cfg["hyper_parameters"]
     nb = hp["neighbor"]
@@ -32,14 +52,16 @@ def reference_force(sys_, pos32, cfg, step, m, prec=0, with_kspace=True):
     hp, nb, rc, rclj, rbl, cl = cfg_values(cfg)
     coulomb = cl is not None
     alpha = cl["alpha"] if coulomb else 0.0
+    qqr2e = cfg_qqr2e(cfg)
     n = len(pos32)
     freq = oracle.rbl_frequency(n, sys_.box_len, nb.get("r_core", 0.0), rc,
                                 nb.get("neighbor_sample_num", 1)) if rbl else 1
-    params = oracle.pair_params(nb.get("r_core", 0.0), rc, rclj, alpha, 1.0, freq, step,
+    params = oracle.pair_params(nb.get("r_core", 0.0), rc, rclj, alpha, qqr2e, freq, step,
                                 1 if rbl else 0, 1 if coulomb else 0, 1 if rbl else 0)
     q32 = sys_.charges.astype(np.float32)
     f, en, vir, mean = oracle.pair_force(pos32, q32, sys_.types, sys_.ids, sys_.eps,
-                                         sys_.sigma, sys_.box_len, params, prec=prec)
+                                         sys_.sigma, sys_.box_len, params, prec=prec,
+                                         special=cfg_special(cfg, sys_))
     ek, vk = 0.0, np.zeros(6)
     if coulomb and with_kspace:
         ci = np.nonzero(sys_.charges)[0]
@@ -49,9 +71,10 @@ def reference_force(sys_, pos32, cfg, step, m, prec=0, with_kspace=True):
             w = np.full(len(m), S / len(m))
         else:
             raise ValueError("reference_force: EWALD weights are built by the caller")
-        fk, rho, ek, vk = oracle.kspace_f64(p64, sys_.charges[ci], sys_.box_len, m, w, 1.0,
+        fk, rho, ek, vk = oracle.kspace_f64(p64, sys_.charges[ci], sys_.box_len, m, w, qqr2e,
                                             alpha)
         f[ci] += fk
         V = float(np.prod(sys_.box_len))
-        ek += oracle.kspace_self((sys_.charges ** 2).sum(), sys_.charges.sum(), alpha, V, 1.0)
+        ek += oracle.kspace_self((sys_.charges ** 2).sum(), sys_.charges.sum(), alpha, V,
+                                 qqr2e)
     return f, en, vir, ek, vk
