@@ -84,6 +84,12 @@ typedef struct rbmdx_thermo_t {
   int64_t pairs_b;    /* stored charged-pair candidate entries              */
   int64_t atoms;      /* atoms in the system                                */
   int64_t charged;    /* charged atoms handled by the k-space kernels       */
+  double e_kspace_exact; /* with coulomb.energy_rbe_flag "yes" (RBE): the
+                            exact Ewald k-space energy over the kmax grid,
+                            incl. the self term, of the last evaluation made
+                            with RBMDX_FLAG_ENERGY; else 0.  RBMD computes it
+                            and then overwrites it with the RBE estimate
+                            (kspace_calculator.cpp:628-669)               */
 } rbmdx_thermo_t;
 
 /*

@@ -60,6 +60,7 @@ class Thermo(ctypes.Structure):
         ("pairs_b", ctypes.c_int64),
         ("atoms", ctypes.c_int64),
         ("charged", ctypes.c_int64),
+        ("e_kspace_exact", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -80,6 +81,7 @@ class Thermo(ctypes.Structure):
             "pairs_b": self.pairs_b,
             "atoms": self.atoms,
             "charged": self.charged,
+            "e_kspace_exact": self.e_kspace_exact,
         }
 
 

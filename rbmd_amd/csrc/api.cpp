@@ -579,6 +579,7 @@ class Engine : public EngineBase {
     ++calls_;
     if (force_rebuild) ++forced_;
     last_num_k_ = (kspace_ ? a.p.num_k : 0);
+    last_eflag_ = a.p.eflag != 0;
     if (flags & RBMDX_FLAG_SYNC) {
       HIPCHK(hipStreamSynchronize(s));
       if (grow_if_needed(s)) {
@@ -670,6 +671,11 @@ class Engine : public EngineBase {
     out->pairs_b = (int64_t)c.sum_count_b;
     out->atoms = n_;
     out->charged = d_.nc;
+    // exact Ewald energy of the last energy evaluation, with the self term
+    // of ComputeSelfEnergy (kspace_calculator.cpp:897-907)
+    if (d_.kex > 0 && last_eflag_)
+      out->e_kspace_exact =
+          t[24] - cfg_.qqr2e * std::sqrt(alpha_ / M_PI) * sum_q2_;
   }
 
   int64_t export_pairs(int64_t step, const void* px, const void* py,
@@ -755,6 +761,7 @@ class Engine : public EngineBase {
   bool split_ = false;
   int kspace_ = 0;  // 0 none, 1 RBE, 2 EWALD
   double alpha_ = 0, rbe_S_ = 0, sum_q_ = 0, sum_q2_ = 0, e_self_ = 0;
+  bool last_eflag_ = false;
   int freq_ = 1;
   Grid grid_{};
   Box<R> box_{};
@@ -1183,42 +1190,33 @@ class Engine : public EngineBase {
       alloc_kspace(w, nullptr);
       alloc_stage(3);
     } else if (kspace_ == 2) {
-      int km[3];
-      if (c.has_kmax) {
-        for (int d = 0; d < 3; ++d) km[d] = c.kmax[d];
-      } else {
-        // SetKspacePara (kspace_calculator.cpp:920-986)
-        const double g = std::sqrt(alpha_);
-        const double acc = c.has_alpha ? 1e-5 * c.qqr2e : c.accuracy * c.qqr2e;
-        const double q2 = c.qqr2e * sum_q2_;
-        for (int d = 0; d < 3; ++d) {
-          int k = 1;
-          auto rms = [&](int kk) {
-            return 2.0 * q2 * g / L_[d] * std::sqrt(1.0 / (M_PI * kk * n_)) *
-                   std::exp(-M_PI * M_PI * kk * kk / (g * g * L_[d] * L_[d]));
-          };
-          while (rms(k) > acc && k < 512) ++k;
-          km[d] = k;
-        }
-      }
       std::vector<int> mv;
       std::vector<double> w;
-      for (int a = -km[0]; a <= km[0]; ++a)
-        for (int b = -km[1]; b <= km[1]; ++b)
-          for (int e = -km[2]; e <= km[2]; ++e) {
-            if (!a && !b && !e) continue;
-            mv.push_back(a);
-            mv.push_back(b);
-            mv.push_back(e);
-            const double kx = 2 * M_PI * a / L_[0], ky = 2 * M_PI * b / L_[1],
-                         kz = 2 * M_PI * e / L_[2];
-            w.push_back(std::exp(-(kx * kx + ky * ky + kz * kz) / (4 * alpha_)));
-          }
+      ewald_grid(mv, w);
       if ((int)w.size() > kMaxK)
         throw Error(RBMDX_E_CONFIG, "EWALD kmax grid too large");
       d_.kcap = (int)w.size();
       alloc_kspace(w, &mv);
       params_.num_k = d_.kcap;
+    }
+    // energy_rbe_flag = "yes" (RBE): the exact Ewald energy on energy
+    // evaluations, reported beside the RBE estimate (kspace_calculator.cpp:
+    // 628-645, which then overwrites it)
+    d_.kex = 0;
+    if (kspace_ == 1 && c.energy_rbe && d_.nc > 0) {
+      std::vector<int> mv;
+      std::vector<double> w;
+      ewald_grid(mv, w);
+      if ((int)w.size() > kMaxK)
+        throw Error(RBMDX_E_CONFIG, "energy_rbe_flag kmax grid too large");
+      const int nchunks = std::max(1, (d_.nc + kSkChunk - 1) / kSkChunk);
+      d_.kex = (int)w.size();
+      d_.m_ex = alloc<int>(mv.size());
+      d_.kw_ex = alloc<double>(w.size());
+      d_.sk_ex = alloc<double2>((size_t)nchunks * d_.kex);
+      HIPCHK(hipMemcpy(d_.m_ex, mv.data(), sizeof(int) * mv.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_.kw_ex, w.data(), sizeof(double) * w.size(),
+                       hipMemcpyHostToDevice));
     }
     params_.num_k = kspace_ == 1 ? c.coulomb_sample_num
                     : kspace_ == 3 ? d_.kcap
@@ -1264,6 +1262,42 @@ class Engine : public EngineBase {
     stage_dev_ = (int32_t*)dp;
     d_.stage_ring = stage_dev_;
     d_.stage_stride = (int)stage_stride_;
+  }
+
+  // The kmax grid of exact Ewald (manual coulomb.kmax, else SetKspacePara's
+  // rms rule, kspace_calculator.cpp:909-986) with weights exp(-k^2 / 4 alpha).
+  void ewald_grid(std::vector<int>& mv, std::vector<double>& w) const {
+    const Config& c = cfg_;
+    int km[3];
+    if (c.has_kmax) {
+      for (int d = 0; d < 3; ++d) km[d] = c.kmax[d];
+    } else {
+      const double g = std::sqrt(alpha_);
+      const double acc = c.has_alpha ? 1e-5 * c.qqr2e : c.accuracy * c.qqr2e;
+      const double q2 = c.qqr2e * sum_q2_;
+      for (int d = 0; d < 3; ++d) {
+        int k = 1;
+        auto rms = [&](int kk) {
+          return 2.0 * q2 * g / L_[d] * std::sqrt(1.0 / (M_PI * kk * n_)) *
+                 std::exp(-M_PI * M_PI * kk * kk / (g * g * L_[d] * L_[d]));
+        };
+        while (rms(k) > acc && k < 512) ++k;
+        km[d] = k;
+      }
+    }
+    mv.clear();
+    w.clear();
+    for (int a = -km[0]; a <= km[0]; ++a)
+      for (int b = -km[1]; b <= km[1]; ++b)
+        for (int e = -km[2]; e <= km[2]; ++e) {
+          if (!a && !b && !e) continue;
+          mv.push_back(a);
+          mv.push_back(b);
+          mv.push_back(e);
+          const double kx = 2 * M_PI * a / L_[0], ky = 2 * M_PI * b / L_[1],
+                       kz = 2 * M_PI * e / L_[2];
+          w.push_back(std::exp(-(kx * kx + ky * ky + kz * kz) / (4 * alpha_)));
+        }
   }
 
   void alloc_kspace(const std::vector<double>& w, const std::vector<int>* mv) {

@@ -178,6 +178,12 @@ struct Dev {
   float4* kf0 = nullptr;         // [kcap] (mx, my, mz, A Re rho)
   float4* kf1 = nullptr;         // [kcap] (kx, ky, kz, A Im rho)
   double2* rho = nullptr;        // [kcap]
+  // energy_rbe_flag diagnostic: the exact Ewald k-space energy over a kmax
+  // grid on energy evaluations (weights exp(-k^2 / 4 alpha))
+  int kex = 0;
+  int* m_ex = nullptr;           // [kex*3]
+  double* kw_ex = nullptr;       // [kex]
+  double2* sk_ex = nullptr;      // [chunks * kex]
   double* thermo = nullptr;      // [32]: [0..10] pair sums (F, E_lj, E_coul,
                                  // virial), [16..22] k-space (E, virial)
   double* mean = nullptr;        // [3]

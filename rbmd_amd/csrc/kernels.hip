@@ -1783,6 +1783,34 @@ __global__ __launch_bounds__(1024) void k_sk_fin(Dev<R> d, Box<R> b,
   }
 }
 
+// The energy_rbe_flag diagnostic (ComputeKspaceEnergy,
+// kspace_calculator.cpp:816-895): FP64 sums of the S(k) chunk partials of the
+// kmax grid, then E = qqr2e (2 pi / V) sum_k exp(-k^2 / 4 alpha) |rho_k|^2 / k^2
+// into thermo[24] (the self term is added on the host).  One block.
+template <typename R>
+__global__ __launch_bounds__(1024) void k_sk_exact(Dev<R> d, Box<R> b, int nchunks,
+                                                   Params p) {
+  const double two_pi = 6.283185307179586476925;
+  double e[1] = {0.0};
+  for (int k = threadIdx.x; k < d.kex; k += blockDim.x) {
+    double sre = 0.0, sim = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < nchunks; ++c) {
+      const double2 s = d.sk_ex[(size_t)c * d.kex + k];
+      sre += s.x;
+      sim += s.y;
+    }
+    const double kx = two_pi * d.m_ex[3 * k] / (double)b.L[0];
+    const double ky = two_pi * d.m_ex[3 * k + 1] / (double)b.L[1];
+    const double kz = two_pi * d.m_ex[3 * k + 2] / (double)b.L[2];
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    e[0] += d.kw_ex[k] * (sre * sre + sim * sim) / k2;
+  }
+  __shared__ double smem[16];
+  block_sum<1>(e, smem);
+  if (threadIdx.x == 0) d.thermo[24] = p.qqr2e * two_pi / p.volume * e[0];
+}
+
 // RBE / Ewald force on the charged atoms: 64 atoms per block, the k range
 // split over kRbeWaves waves, reduced through LDS.
 template <typename R>
@@ -2027,6 +2055,17 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
                        sog ? 1 : 0, sog ? (const int*)d.src_dev : nullptr);
     hipLaunchKernelGGL((k_rbe_force<R>), dim3(blocks_for(d.nc, 64)),
                        dim3(64 * kRbeWaves), 0, ks, d, a.box, K);
+    if (p.eflag && d.kex > 0) {
+      // energy_rbe_flag: the exact Ewald energy over the kmax grid
+      Dev<R> dx = d;
+      dx.sk_part = d.sk_ex;
+      dx.kcap = d.kex;
+      hipLaunchKernelGGL((k_sk<R>), dim3(nchunks, (d.kex + kSkTile - 1) / kSkTile),
+                         dim3(kBlock), 0, ks, dx, a.box, (const int*)d.m_ex, 0, 0,
+                         d.kex);
+      hipLaunchKernelGGL((k_sk_exact<R>), dim3(1), dim3(1024), 0, ks, d, a.box,
+                         nchunks, p);
+    }
     if (a.ev[4]) hipEventRecord(a.ev[4], ks);
     if (ks != s) hipEventRecord(a.ev_join, ks);
   } else if (a.ev[3]) {

@@ -97,6 +97,15 @@ def _cfg(**edits):
      None),
     ({"hyper_parameters.neighbor.skin": -0.1}, "skin"),
     ({"init_configuration.read_data.unit": "SI"}, "unit"),
+    ({"hyper_parameters.force_field.type": "CVFF"}, None),
+    ({"hyper_parameters.force_field.type": "CVFF",
+      "hyper_parameters.extend": {"special_bonds": [0, 0.5, 1]}}, None),
+    ({"hyper_parameters.force_field.type": "CVFF",
+      "hyper_parameters.extend": {"special_bonds": [0, 0, 0, 0]}}, "special_bonds"),
+    ({"hyper_parameters.force_field.type": "CVFF",
+      "hyper_parameters.extend": {"special_bonds": ["x"]}}, "special_bonds"),
+    ({"hyper_parameters.force_field.type": "CVFF", "hyper_parameters.coulomb": None},
+     "CVFF needs hyper_parameters.coulomb"),
 ])
 def test_config_validation(edit, msg):
     rc, err = rbmd_amd.check_config(_cfg(**edit))

@@ -455,3 +455,58 @@ def test_edge_shifted_stage_sets_and_reuse_across_wraps(device):
     assert ctx.thermo()["rebuilds"] == rebuilds  # reuse
     refF = reference_force(s, pos32, cfg, step + 1, m)[0]
     assert rel_rms(_stack(f), refF) < 2e-5
+
+
+def _ewald_grid_energy(s, alpha, kmax, qqr2e=1.0):
+    r = [np.arange(-k, k + 1) for k in kmax]
+    mg = np.stack(np.meshgrid(*r, indexing="ij"), -1).reshape(-1, 3)
+    mg = mg[(mg != 0).any(1)].astype(np.int32)
+    k = 2 * np.pi * mg / s.box_len
+    w = np.exp(-(k ** 2).sum(1) / (4 * alpha))
+    ci = np.nonzero(s.charges)[0]
+    _, _, ek, _ = oracle.kspace_f64(s.pos.astype(np.float32).astype(np.float64)[ci],
+                                    s.charges[ci], s.box_len, mg, w, qqr2e, alpha)
+    return ek - qqr2e * np.sqrt(alpha / np.pi) * (s.charges ** 2).sum()
+
+
+def test_energy_rbe_flag_reports_exact_ewald_beside_rbe(device, small_electrolyte):
+    """energy_rbe_flag "yes": on energy evaluations the exact Ewald k-space
+    energy over the kmax grid (with the self term) is reported as
+    e_kspace_exact; the RBE estimate and the forces are unchanged."""
+    import torch
+
+    s = small_electrolyte
+    cfg = systems.ljsalt_config(P=100, kmax=(5, 5, 5))
+    cfg["hyper_parameters"]["coulomb"]["energy_rbe_flag"] = "yes"
+    plain = systems.ljsalt_config(P=100)
+    m = rbe_sample_host(0.09, s.box_len, 100, 9, 0)
+    mt = torch.tensor(m.reshape(-1), dtype=torch.int32, device=device)
+    out = {}
+    for name, c in (("diag", cfg), ("plain", plain)):
+        ctx = _ctx(s, c)
+        (px, py, pz), f = _tensors(s, device, torch.float32)
+        ctx.force(4, px, py, pz, *f, m_samples=mt, energy=True, sync=True)
+        out[name] = (_stack(f), ctx.thermo())
+    np.testing.assert_array_equal(out["diag"][0], out["plain"][0])
+    assert out["diag"][1]["e_kspace"] == out["plain"][1]["e_kspace"]
+    assert out["plain"][1]["e_kspace_exact"] == 0.0
+    want = _ewald_grid_energy(s, 0.09, (5, 5, 5))
+    assert out["diag"][1]["e_kspace_exact"] == pytest.approx(want, rel=1e-5)
+    # the graph replay (no explicit sample) reports it too
+    ctx = _ctx(s, cfg)
+    (px, py, pz), f = _tensors(s, device, torch.float32)
+    for step in range(3):
+        ctx.force(step, px, py, pz, *f, energy=True)
+    assert ctx.thermo()["e_kspace_exact"] == pytest.approx(want, rel=1e-5)
+    # automatic kmax from the accuracy rule: close to a converged grid
+    cfg2 = systems.ljsalt_config(P=100)
+    cl = cfg2["hyper_parameters"]["coulomb"]
+    del cl["alpha"]
+    cl["accuracy"] = 1e-4
+    cl["energy_rbe_flag"] = "yes"
+    ctx = _ctx(s, cfg2)
+    (px, py, pz), f = _tensors(s, device, torch.float32)
+    ctx.force(0, px, py, pz, *f, energy=True, sync=True)
+    th = ctx.thermo()
+    conv = _ewald_grid_energy(s, th["alpha"], (16, 16, 16))
+    assert th["e_kspace_exact"] == pytest.approx(conv, rel=1e-3)
