@@ -47,6 +47,13 @@ def lib():
         L.orc_pair_force.restype = None
         L.orc_pair_force.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F, _F, _F, _I, _I,
                                      ctypes.c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
+        L.orc_force_f64.restype = None
+        L.orc_force_f64.argtypes = [ctypes.c_int, _D, _D, _D, _I, _I, ctypes.c_int, _D, _D,
+                                    _D, _D, _D, ctypes.c_int, _I, _D, _D, _D, _D]
+        L.orc_force_vjp_f64.restype = None
+        L.orc_force_vjp_f64.argtypes = [ctypes.c_int, _D, _D, _D, _I, _I, ctypes.c_int, _D,
+                                        _D, _D, _D, _D, ctypes.c_int, _I, _D, _D, _D, _D,
+                                        _D, _D]
         L.orc_pair_force_special.restype = None
         L.orc_pair_force_special.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F, _F, _F, _I,
                                              _I, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int,
@@ -414,3 +421,46 @@ def sog_verlet_f64(pos, q, types, eps, sig, L, rc, rc_lj, s, qqr2e=1.0):
                              *[a.ctypes.data_as(_D) for a in f], en.ctypes.data_as(_D),
                              v.ctypes.data_as(_D))
     return np.stack(f, axis=1), en, v
+
+
+def _adj_inputs(pos, types, gid, Q, eps, sig, L, params, m, wk):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    x, xp = _d(pos[:, 0])
+    y, yp = _d(pos[:, 1])
+    z, zp = _d(pos[:, 2])
+    t, tp = _i(types)
+    g, gp = _i(gid)
+    qa, qp = _d(Q)
+    e, ep = _d(eps)
+    s, sp = _d(sig)
+    Ld, Lp = _d(L)
+    pp, ppp = _d(params)
+    K = 0 if m is None else len(m)
+    ma, mp = _i(np.zeros(3, np.int32) if m is None else np.asarray(m).reshape(-1))
+    wa, wp = _d(np.zeros(1) if wk is None else wk)
+    keep = (x, y, z, t, g, qa, e, s, Ld, pp, ma, wa)
+    return keep, (len(pos), xp, yp, zp, tp, gp, len(Q), qp, ep, sp, Lp, ppp, K, mp, wp)
+
+
+def force_f64(pos, types, gid, Q, eps, sig, L, params, m=None, wk=None):
+    """FP64 force (N,3) of one evaluation (RBL/Verlet pairs, mean-force fix,
+    RBE k-space over the atoms of nonzero charge) for per-type Q, eps, sigma."""
+    keep, args = _adj_inputs(pos, types, gid, Q, eps, sig, L, params, m, wk)
+    n = args[0]
+    f = [np.zeros(n) for _ in range(3)]
+    lib().orc_force_f64(*args, *(a.ctypes.data_as(_D) for a in f))
+    del keep
+    return np.stack(f, axis=1)
+
+
+def force_vjp_f64(pos, types, gid, Q, eps, sig, L, params, G, m=None, wk=None):
+    """(dL/dr (N,3), dL/dQ, dL/deps, dL/dsigma) for L = sum G . F."""
+    keep, args = _adj_inputs(pos, types, gid, Q, eps, sig, L, params, m, wk)
+    n, nt = args[0], args[6]
+    Ga, Gp = _d(np.asarray(G, dtype=np.float64).reshape(-1))
+    dr = np.zeros(3 * n)
+    dq, de, ds = np.zeros(nt), np.zeros(nt), np.zeros(nt)
+    lib().orc_force_vjp_f64(*args, Gp, dr.ctypes.data_as(_D), dq.ctypes.data_as(_D),
+                            de.ctypes.data_as(_D), ds.ctypes.data_as(_D))
+    del keep, Ga
+    return dr.reshape(n, 3), dq, de, ds

@@ -817,3 +817,242 @@ void orc_force_step(int n, const float* x, const float* y, const float* z,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Force-matching adjoint (FP64).  The force of one evaluation (RBL pair forces
+// with the mean-force fix, lj_cut_coul_kspace.cpp:494-573; RBE k-space over
+// the atoms of nonzero charge, kspace_calculator_op.hip.cu:98-125,678-738)
+// as a function of positions and the per-type parameters Q (charge), eps and
+// sigma, with q_i = Q[type_i], and the vector-Jacobian product of a cotangent
+// G (N x 3): dL/dtheta for L = sum_i G_i . F_i.  Pair classification (which
+// pairs exist, RBL classes and selections) uses the FP32 mirror distance of
+// the float-rounded positions, as the product does; arithmetic is FP64.
+//
+// Per directed pair (i, j), d = r_j - r_i, u = |d|^2, weight w (RBL class):
+//   f(u)  = w [f_LJ(u) + f_C(u)],   F_i += f d
+//   f_LJ  = -24 eps_ij (2 s6 - 1) s6 / u,   s6 = (sigma_ij^2 / u)^3
+//   f_C   = -qqr2e q_i q_j h(u),  h(u) = erfc(g r) / (u r)
+//           + (2 g / sqrt(pi)) e^{-alpha u} / u,  g = sqrt(alpha), r = sqrt(u).  The pair
+// adds c = f (G'_i . d) to L (G' = G - mean(G) under the mean-force fix), so
+//   dc/dr_j = f G'_i + 2 f'(u) (G'_i . d) d = -dc/dr_i,
+//   dc/d eps_ij = w (-24 (2 s6 - 1) s6 / u) (G'_i . d),
+//   dc/d sigma_ij = w (-24 eps_ij (4 s6 - 1) 6 s6 / (sigma_ij u)) (G'_i . d),
+//   dc/d(q_i q_j) = -w qqr2e h(u) (G'_i . d),
+// and the mixing rules eps_ij = sqrt(eps_a eps_b), sigma_ij = (sigma_a +
+// sigma_b) / 2, q_i q_j = Q_a Q_b carry them to the types a = t_i, b = t_j.
+// k-space: A_k = w_k (-4 pi qqr2e) / (V k^2), theta_i = k . r_i,
+// T_a(k) = sum_{t_i = a} e^{i theta_i}, U_a(k) = sum_{t_i = a} (G_i . k)
+// e^{i theta_i}, rho = sum_a Q_a T_a, beta = sum_a Q_a U_a; then
+// L_k = sum_k A_k Im(conj(beta) rho), so
+//   dL_k/dQ_a = sum_k A_k [Im(conj(U_a) rho) + Im(conj(beta) T_a)],
+//   dL_k/dr_m = q_m sum_k A_k [Re(conj(beta) e^{i theta_m})
+//                              - (G_m . k) Re(e^{-i theta_m} rho)] k.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Adj {
+  std::vector<double> fx, fy, fz;        // forward force
+  std::vector<double> dx, dy, dz;        // dL/dr
+  std::vector<double> dq, de, ds;        // dL/dQ, dL/deps, dL/dsigma per type
+};
+
+void force_adjoint(int n, const double* x, const double* y, const double* z,
+                   const int* type, const int* gid, int ntypes,
+                   const double* Q, const double* eps, const double* sig,
+                   const double* L, const double* pp, int K, const int* m,
+                   const double* wk, const double* G, bool want_adj, Adj& out) {
+  const PairParams p = unpack(pp);
+  const Thresholds t = thresholds(p);
+  const float Lf[3] = {(float)L[0], (float)L[1], (float)L[2]};
+  std::vector<float> xf(n), yf(n), zf(n);
+  for (int i = 0; i < n; ++i) {
+    xf[i] = (float)x[i];
+    yf[i] = (float)y[i];
+    zf[i] = (float)z[i];
+  }
+  out.fx.assign(n, 0.0);
+  out.fy.assign(n, 0.0);
+  out.fz.assign(n, 0.0);
+  out.dx.assign(n, 0.0);
+  out.dy.assign(n, 0.0);
+  out.dz.assign(n, 0.0);
+  out.dq.assign(ntypes, 0.0);
+  out.de.assign(ntypes, 0.0);
+  out.ds.assign(ntypes, 0.0);
+  // mean-force fix: L = sum (G_i - mean G) . F_pair,i
+  double gm[3] = {0, 0, 0};
+  if (want_adj && p.mean_fix) {
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) gm[d] += G[3 * i + d];
+    for (int d = 0; d < 3; ++d) gm[d] /= (double)n;
+  }
+  const double g = std::sqrt(p.alpha), tsp = 2.0 / std::sqrt(M_PI);
+  const double radius = std::sqrt((double)t.rc2) * 1.0001 + 1e-6;
+  std::vector<double> px(n, 0.0), py(n, 0.0), pz(n, 0.0);
+  for_each_candidate<float>(n, xf.data(), yf.data(), zf.data(), L, radius,
+                            [&](int i, int j) {
+    const float d2f = dist2_f32(xf[i], yf[i], zf[i], xf[j], yf[j], zf[j], Lf);
+    if (!(d2f < t.rc2)) return;
+    const int cls = p.rbl ? rbl_class(d2f, t, p, gid[i], gid[j]) : 1;
+    const double w = pair_weight(d2f, cls, t, p);
+    if (w == 0.0) return;
+    const double ddx = min_image_1<double>(x[j] - x[i], L[0]);
+    const double ddy = min_image_1<double>(y[j] - y[i], L[1]);
+    const double ddz = min_image_1<double>(z[j] - z[i], L[2]);
+    const double u = ddx * ddx + ddy * ddy + ddz * ddz;
+    const int a = type[i], b = type[j];
+    double flj = 0, dflj = 0, dle = 0, dls = 0;
+    const double eij = std::sqrt(eps[a] * eps[b]), sij = 0.5 * (sig[a] + sig[b]);
+    if (d2f < t.rclj2) {
+      const double s2 = sij * sij / u, s6 = s2 * s2 * s2;
+      flj = -24.0 * eij * (2.0 * s6 - 1.0) * s6 / u;
+      dflj = 24.0 * eij * (14.0 * s6 * s6 - 4.0 * s6) / (u * u);
+      dle = -24.0 * (2.0 * s6 - 1.0) * s6 / u;
+      dls = sij > 0 ? -24.0 * eij * (4.0 * s6 - 1.0) * 6.0 * s6 / (sij * u) : 0.0;
+    }
+    double fc = 0, dfc = 0, hC = 0;
+    const double qq = Q[a] * Q[b];
+    if (p.coulomb) {
+      const double r = std::sqrt(u), er = std::erfc(g * r), ex = std::exp(-p.alpha * u);
+      hC = er / (u * r) + g * tsp * ex / u;
+      const double dh = -1.5 * er / (u * u * r) - 1.5 * g * tsp * ex / (u * u) -
+                        g * g * g * tsp * ex / u;
+      fc = -p.qqr2e * qq * hC;
+      dfc = -p.qqr2e * qq * dh;
+    }
+    const double f = w * (flj + fc), fp = w * (dflj + dfc);
+    px[i] += f * ddx;
+    py[i] += f * ddy;
+    pz[i] += f * ddz;
+    if (!want_adj) return;
+    const double Gx = G[3 * i] - gm[0], Gy = G[3 * i + 1] - gm[1], Gz = G[3 * i + 2] - gm[2];
+    const double gd = Gx * ddx + Gy * ddy + Gz * ddz;
+    const double vx = f * Gx + 2.0 * fp * gd * ddx, vy = f * Gy + 2.0 * fp * gd * ddy,
+                 vz = f * Gz + 2.0 * fp * gd * ddz;
+    out.dx[j] += vx;
+    out.dy[j] += vy;
+    out.dz[j] += vz;
+    out.dx[i] -= vx;
+    out.dy[i] -= vy;
+    out.dz[i] -= vz;
+    const double ce = w * dle * gd, cs = w * dls * gd;
+    if (eps[a] > 0) out.de[a] += ce * eij / (2.0 * eps[a]);
+    if (eps[b] > 0) out.de[b] += ce * eij / (2.0 * eps[b]);
+    out.ds[a] += 0.5 * cs;
+    out.ds[b] += 0.5 * cs;
+    if (p.coulomb) {
+      const double cq = -w * p.qqr2e * hC * gd;
+      out.dq[a] += cq * Q[b];
+      out.dq[b] += cq * Q[a];
+    }
+  });
+  double mean[3] = {0, 0, 0};
+  if (p.mean_fix) {
+    for (int i = 0; i < n; ++i) {
+      mean[0] += px[i];
+      mean[1] += py[i];
+      mean[2] += pz[i];
+    }
+    for (int d = 0; d < 3; ++d) mean[d] /= (double)n;
+  }
+  for (int i = 0; i < n; ++i) {
+    out.fx[i] = px[i] - mean[0];
+    out.fy[i] = py[i] - mean[1];
+    out.fz[i] = pz[i] - mean[2];
+  }
+  if (!p.coulomb || K <= 0) return;
+  // k-space
+  const double V = L[0] * L[1] * L[2], tp = 2.0 * M_PI;
+  std::vector<double> Tr(ntypes), Ti(ntypes), Ur(ntypes), Ui(ntypes);
+  for (int k = 0; k < K; ++k) {
+    const double kx = tp * m[3 * k] / L[0], ky = tp * m[3 * k + 1] / L[1],
+                 kz = tp * m[3 * k + 2] / L[2];
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double A = wk[k] * (-4.0 * M_PI * p.qqr2e) / (V * k2);
+    std::fill(Tr.begin(), Tr.end(), 0.0);
+    std::fill(Ti.begin(), Ti.end(), 0.0);
+    std::fill(Ur.begin(), Ur.end(), 0.0);
+    std::fill(Ui.begin(), Ui.end(), 0.0);
+    for (int i = 0; i < n; ++i) {
+      const double th = kx * x[i] + ky * y[i] + kz * z[i];
+      const double c = std::cos(th), s = std::sin(th);
+      Tr[type[i]] += c;
+      Ti[type[i]] += s;
+      if (want_adj) {
+        const double gk = G[3 * i] * kx + G[3 * i + 1] * ky + G[3 * i + 2] * kz;
+        Ur[type[i]] += gk * c;
+        Ui[type[i]] += gk * s;
+      }
+    }
+    double rr = 0, ri = 0, br = 0, bi = 0;
+    for (int a = 0; a < ntypes; ++a) {
+      rr += Q[a] * Tr[a];
+      ri += Q[a] * Ti[a];
+      br += Q[a] * Ur[a];
+      bi += Q[a] * Ui[a];
+    }
+    for (int i = 0; i < n; ++i) {
+      const double qi = Q[type[i]];
+      if (qi == 0.0) continue;
+      const double th = kx * x[i] + ky * y[i] + kz * z[i];
+      const double c = std::cos(th), s = std::sin(th);
+      const double fk = A * qi * (c * ri - s * rr);
+      out.fx[i] += fk * kx;
+      out.fy[i] += fk * ky;
+      out.fz[i] += fk * kz;
+      if (want_adj) {
+        const double gk = G[3 * i] * kx + G[3 * i + 1] * ky + G[3 * i + 2] * kz;
+        const double v = A * qi * ((br * c + bi * s) - gk * (c * rr + s * ri));
+        out.dx[i] += v * kx;
+        out.dy[i] += v * ky;
+        out.dz[i] += v * kz;
+      }
+    }
+    if (want_adj)
+      for (int a = 0; a < ntypes; ++a)
+        // Im(conj(U) rho) + Im(conj(beta) T)
+        out.dq[a] += A * ((Ur[a] * ri - Ui[a] * rr) + (br * Ti[a] - bi * Tr[a]));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// FP64 force of one evaluation for double positions and per-type parameters.
+void orc_force_f64(int n, const double* x, const double* y, const double* z,
+                   const int* type, const int* gid, int ntypes, const double* Q,
+                   const double* eps, const double* sig, const double* L,
+                   const double* pp, int K, const int* m, const double* wk,
+                   double* fx, double* fy, double* fz) {
+  Adj a;
+  force_adjoint(n, x, y, z, type, gid, ntypes, Q, eps, sig, L, pp, K, m, wk,
+                nullptr, false, a);
+  std::copy(a.fx.begin(), a.fx.end(), fx);
+  std::copy(a.fy.begin(), a.fy.end(), fy);
+  std::copy(a.fz.begin(), a.fz.end(), fz);
+}
+
+// Its vector-Jacobian product for the cotangent G (N x 3, row-major).
+void orc_force_vjp_f64(int n, const double* x, const double* y, const double* z,
+                       const int* type, const int* gid, int ntypes,
+                       const double* Q, const double* eps, const double* sig,
+                       const double* L, const double* pp, int K, const int* m,
+                       const double* wk, const double* G, double* dr,
+                       double* dq, double* de, double* ds) {
+  Adj a;
+  force_adjoint(n, x, y, z, type, gid, ntypes, Q, eps, sig, L, pp, K, m, wk, G,
+                true, a);
+  for (int i = 0; i < n; ++i) {
+    dr[3 * i] = a.dx[i];
+    dr[3 * i + 1] = a.dy[i];
+    dr[3 * i + 2] = a.dz[i];
+  }
+  for (int t = 0; t < ntypes; ++t) {
+    dq[t] = a.dq[t];
+    de[t] = a.de[t];
+    ds[t] = a.ds[t];
+  }
+}
+
+}  // extern "C"
