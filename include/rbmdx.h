@@ -66,6 +66,27 @@ enum {
                               (also RBMDX_GRAPH=0 in the environment) */
 };
 
+/*
+ * Force-matching adjoint (no counterpart in the reference, which has no
+ * differentiation): the vector-Jacobian product of one evaluation's force F
+ * with a cotangent G = dL/dF (device arrays gx/gy/gz, caller order, the
+ * force's precision):
+ *   dx/dy/dz  dL/dr, device arrays in caller order;
+ *   dparam    device array of 3 ntypes doubles: dL/dQ_t (a common shift of
+ *             the charges of every atom of type t), dL/deps_t, dL/dsigma_t
+ *             through the mixing rules.
+ * The pair set (step is the RBL sampling epoch, as for rbmdx_force) and the
+ * k-sample are those of the last rbmdx_force call; the RBL mean-force fix is
+ * differentiated too.  LJ + erfc Coulomb pairs with RBE, RBSOG or EWALD
+ * k-space; not CVFF special bonds or the VERLET-SOG near field; at most 8
+ * types.  Eager launches on stream; parameter sums use LDS float atomics
+ * inside a block, so their last bits may vary between runs.
+ */
+int rbmdx_force_vjp(rbmdx_ctx* ctx, int64_t step, const void* px,
+                    const void* py, const void* pz, const void* gx,
+                    const void* gy, const void* gz, void* dx, void* dy,
+                    void* dz, double* dparam, uint32_t flags, void* stream);
+
 /* Thermo record filled by rbmdx_thermo (all FP64, extensive totals). */
 typedef struct rbmdx_thermo_t {
   double e_vdwl;      /* LJ pair energy (full list within cut_off)          */
@@ -125,6 +146,10 @@ int rbmdx_set_system(rbmdx_ctx* ctx, int64_t n, const double* box_lo,
 
 /* Replace the charges (host array, n entries); keeps lists. */
 int rbmdx_set_charges(rbmdx_ctx* ctx, const double* charge);
+
+/* Replace the per-type LJ parameters (host arrays, ntypes entries); keeps
+ * lists.  Synchronises the device. */
+int rbmdx_set_lj(rbmdx_ctx* ctx, const double* eps, const double* sigma);
 
 /*
  * Special-bond topology for force_field.type "CVFF" (host arrays, caller

@@ -98,6 +98,9 @@ SIGNATURES = {
     "rbmdx_set_system": (ctypes.c_int, [_P, _I64, _DP, _DP, _IP, _IP, _DP, _I32, _DP, _DP]),
     "rbmdx_set_charges": (ctypes.c_int, [_P, _DP]),
     "rbmdx_set_special": (ctypes.c_int, [_P, _I32, _IP, _IP]),
+    "rbmdx_set_lj": (ctypes.c_int, [_P, _DP, _DP]),
+    "rbmdx_force_vjp": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                       ctypes.c_uint32, _P]),
     "rbmdx_force": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _I32,
                                    ctypes.c_uint32, _P]),
     "rbmdx_thermo": (ctypes.c_int, [_P, ctypes.POINTER(Thermo)]),
@@ -295,6 +298,27 @@ class ForceContext:
     def set_masses(self, masses):
         m = np.ascontiguousarray(masses, dtype=np.float64)
         self._check(self.lib.rbmdx_set_masses(self._h, _dptr(m)))
+
+    def set_lj(self, eps, sigma):
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        s = np.ascontiguousarray(sigma, dtype=np.float64)
+        self._check(self.lib.rbmdx_set_lj(self._h, _dptr(e), _dptr(s)))
+
+    def force_vjp(self, step, px, py, pz, gx, gy, gz, dx, dy, dz, dparam, sync=False,
+                  stream=None):
+        """Vector-Jacobian product of the last evaluation's force with the
+        cotangent (gx, gy, gz): dL/dr into (dx, dy, dz) and [dL/dQ | dL/deps |
+        dL/dsigma] per type into dparam (float64 device tensor of 3 ntypes)."""
+        flags = RBMDX_FLAG_SYNC if sync else 0
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(px.device).cuda_stream
+        args = [_tensor_ptr(t) for t in (px, py, pz, gx, gy, gz, dx, dy, dz)]
+        self._check(self.lib.rbmdx_force_vjp(
+            self._h, int(step), *args, ctypes.cast(_tensor_ptr(dparam),
+                                                   ctypes.POINTER(ctypes.c_double)),
+            flags, ctypes.c_void_p(stream)))
+        self.calls += 1
 
     def force_leapfrog(self, step, px, py, pz, vx, vy, vz, fx, fy, fz, dt, energy=False,
                        sync=False, timing=False, eager=False, stream=None):

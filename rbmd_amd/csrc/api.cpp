@@ -12,6 +12,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <numeric>
 #include <array>
 #include <cmath>
 #include <cstdlib>
@@ -277,6 +278,11 @@ struct EngineBase {
                      const void* pz, void* fx, void* fy, void* fz,
                      const int32_t* m, int32_t nk, uint32_t flags,
                      hipStream_t s) = 0;
+  virtual void force_vjp(int64_t step, const void* px, const void* py,
+                         const void* pz, const void* gx, const void* gy,
+                         const void* gz, void* dx, void* dy, void* dz,
+                         double* dparam, uint32_t flags, hipStream_t s) = 0;
+  virtual void set_lj(const double* eps, const double* sig) = 0;
   virtual void force_leapfrog(int64_t step, void* px, void* py, void* pz,
                               void* vx, void* vy, void* vz, void* fx, void* fy,
                               void* fz, double dt, uint32_t flags,
@@ -473,6 +479,126 @@ class Engine : public EngineBase {
     leap_dt_ = 0;
   }
 
+  // Replace the per-type LJ parameters (keeps lists and the charges).
+  void set_lj(const double* eps, const double* sig) override {
+    if (!ready_) throw Error(RBMDX_E_STATE, "set_system first");
+    HIPCHK(hipSetDevice(device_));
+    HIPCHK(hipDeviceSynchronize());
+    eps_.assign(eps, eps + ntypes_);
+    sig_.assign(sig, sig + ntypes_);
+    upload_lj();
+  }
+
+  // Vector-Jacobian product of one evaluation's force with the cotangent
+  // (gx, gy, gz) = dL/dF: dL/dr into (dx, dy, dz) (caller order) and
+  // [dL/dQ | dL/deps | dL/dsigma] per type into dparam (device, 3 ntypes
+  // doubles).  Same pair set (step = RBL epoch) and the k-sample of the last
+  // forward evaluation.  Eager launches on stream s.
+  void force_vjp(int64_t step, const void* px, const void* py, const void* pz,
+                 const void* gx, const void* gy, const void* gz, void* dx,
+                 void* dy, void* dz, double* dparam, uint32_t flags,
+                 hipStream_t s) override {
+    if (!ready_) throw Error(RBMDX_E_STATE, "rbmdx_set_system first");
+    if (!px || !py || !pz || !gx || !gy || !gz || !dx || !dy || !dz || !dparam)
+      throw Error(RBMDX_E_ARG, "null pointer");
+    if (d_.cap_a == 0 || calls_ == 0)
+      throw Error(RBMDX_E_STATE, "rbmdx_force_vjp needs a forward evaluation first");
+    if (params_.special || pc_.sog)
+      throw Error(RBMDX_E_CONFIG,
+                  "the adjoint covers LJ + erfc Coulomb pair forces (no CVFF special "
+                  "bonds, no VERLET-SOG near field)");
+    if (ntypes_ > kVjpMaxTypes)
+      throw Error(RBMDX_E_CONFIG, "the adjoint supports at most 8 atom types");
+    HIPCHK(hipSetDevice(device_));
+    {
+      // LDS of the adjoint pair pass: two 24 KB stages, slot tables, the LJ
+      // table and per-lane class slots (4 ntypes floats per thread)
+      int max_lds = 0;
+      HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                   device_));
+      const size_t need = 2 * 24576 + 4096 + 4 * sizeof(R) * (size_t)ntypes_ * ntypes_ +
+                          16 * (size_t)ntypes_ * grid_.block;
+      if (need > (size_t)max_lds)
+        throw Error(RBMDX_E_CONFIG, "the adjoint's LDS budget is exceeded for this "
+                                    "number of types and brick size");
+    }
+    bind_positions(px, py, pz);
+    grow_if_needed(s);
+    int force_rebuild = (flags & RBMDX_FLAG_REBUILD) ? 1 : 0;
+    if (lists_stale_) force_rebuild = 1;
+    lists_stale_ = false;
+    alloc_vjp();
+    StepArgs<R> a = args(step, 0);
+    a.force_rebuild = force_rebuild;
+    const int slot = (int)(calls_ % kSlots);
+    if (slot_used_[slot]) HIPCHK(hipEventSynchronize(slot_events_[slot]));
+    hdr_[slot].epoch = (long long)step;
+    hdr_[slot].force = force_rebuild;
+    vjp_.gx = (const R*)gx;
+    vjp_.gy = (const R*)gy;
+    vjp_.gz = (const R*)gz;
+    vjp_.dx = (R*)dx;
+    vjp_.dy = (R*)dy;
+    vjp_.dz = (R*)dz;
+    vjp_.dparam = dparam;
+    a.p.num_k = last_num_k_;
+    // the evaluation's lists and sorted tiles (as the forward path builds them)
+    HIPCHK(launch_skin_build<R>(d_, a, s));
+    HIPCHK(launch_vjp<R>(d_, vjp_, a, s));
+    HIPCHK(hipEventRecord(slot_events_[slot], s));
+    slot_used_[slot] = true;
+    last_stream_ = s;
+    last_stream_set_ = true;
+    ++calls_;
+    if (flags & RBMDX_FLAG_SYNC) HIPCHK(hipStreamSynchronize(s));
+  }
+
+  // Adjoint buffers, sized for the system on first use.
+  void alloc_vjp() {
+    if (vjp_.gs) return;
+    VjpDev<R>& v = vjp_;
+    v.gs = alloc<R4>(n_);
+    v.gc = alloc<R4>(std::max(1, d_.nc));
+    v.drs = alloc<R4>(n_);
+    v.drk = alloc<R4>(std::max(1, d_.nc));
+    v.gather_blocks = std::min(1024, std::max(1, (n_ + kBlock - 1) / kBlock));
+    v.gpart = alloc<double>(3 * (size_t)v.gather_blocks);
+    v.gmean = alloc<double>(3);
+    v.nq_blocks = (params_.split && d_.nc > 0)
+                      ? (d_.nc + kBlock / kChargedLanes - 1) / (kBlock / kChargedLanes)
+                      : 0;
+    v.ncls = grid_.nbrick + v.nq_blocks;
+    v.cls = alloc<float>((size_t)v.ncls * 3 * ntypes_);
+    // type-major order of the caller's atoms, in chunks of at most kSkChunk
+    // atoms of one type
+    std::vector<int> ord(n_), beg, typ;
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int x, int y) { return type_[x] < type_[y]; });
+    for (int s = 0; s < n_;) {
+      int e = s;
+      while (e < n_ && type_[ord[e]] == type_[ord[s]] && e - s < kSkChunk) ++e;
+      beg.push_back(s);
+      typ.push_back(type_[ord[s]]);
+      s = e;
+    }
+    beg.push_back(n_);
+    v.nch = (int)typ.size();
+    v.tord = alloc<int>(n_);
+    v.ch_beg = alloc<int>(beg.size());
+    v.ch_type = alloc<int>(typ.size());
+    HIPCHK(hipMemcpy(v.tord, ord.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(v.ch_beg, beg.data(), sizeof(int) * beg.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(v.ch_type, typ.data(), sizeof(int) * typ.size(),
+                     hipMemcpyHostToDevice));
+    const int K = std::max(1, d_.kcap);
+    v.vsk = alloc<double>((size_t)v.nch * K * 8);
+    v.kv0 = alloc<float4>(K);
+    v.kv1 = alloc<float4>(K);
+    v.kv2 = alloc<float2>(K);
+    v.dqk = alloc<double>(kVjpMaxTypes);
+  }
+
   void force(int64_t step, const void* px, const void* py, const void* pz,
              void* fx, void* fy, void* fz, const int32_t* m, int32_t nk,
              uint32_t flags, hipStream_t s) override {
@@ -511,7 +637,10 @@ class Engine : public EngineBase {
         if (!sog) {
           if (nk <= 0 || nk > d_.kcap)
             throw Error(RBMDX_E_ARG, "num_k out of range for explicit samples");
-          a.m_use = m;
+          // kept in device memory: the adjoint reuses the evaluation's sample
+          HIPCHK(hipMemcpyAsync(d_.m_dev, m, sizeof(int32_t) * 3 * (size_t)nk,
+                                hipMemcpyDeviceToDevice, s));
+          a.m_use = d_.m_dev;
           a.p.num_k = nk;
         } else {
           // RBSOG: P m-vectors followed by P virial-chain indices; the direct
@@ -769,6 +898,7 @@ class Engine : public EngineBase {
   Params params_{};
   Dev<R> d_{};
   R* lj_tab_ = nullptr;
+  VjpDev<R> vjp_;
   std::vector<void*> allocs_;
   std::unique_ptr<KSampler> sampler_;
   std::unique_ptr<SamplePipeline> pipeline_;
@@ -812,6 +942,24 @@ class Engine : public EngineBase {
     return p;
   }
 
+
+  // LJ table: (-24 eps_ij, sigma_ij^2, eps_ij, sigma_ij), Lorentz-Berthelot
+  // (lj_op.hip.cu:130-131); -24 eps is formed in R, as the force term forms it.
+  void upload_lj() {
+    std::vector<R> t(4 * (size_t)ntypes_ * ntypes_);
+    for (int a = 0; a < ntypes_; ++a)
+      for (int b = 0; b < ntypes_; ++b) {
+        const double e = std::sqrt(eps_[a] * eps_[b]);
+        const double s = (sig_[a] + sig_[b]) / 2.0;
+        R* row = &t[4 * (a * ntypes_ + b)];
+        row[0] = (R)-24 * (R)e;
+        row[1] = (R)(s * s);
+        row[2] = (R)e;
+        row[3] = (R)s;
+      }
+    if (!lj_tab_) lj_tab_ = alloc<R>(t.size());
+    HIPCHK(hipMemcpy(lj_tab_, t.data(), sizeof(R) * t.size(), hipMemcpyHostToDevice));
+  }
 
   void bind_positions(const void* px, const void* py, const void* pz) {
     d_.px = (const R*)px;
@@ -1094,24 +1242,8 @@ class Engine : public EngineBase {
     HIPCHK(hipMemcpy(d_.type, type_.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_.gid, id_.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
     upload_charges();
-    // LJ table: (-24 eps_ij, sigma_ij^2, eps_ij, 0), Lorentz-Berthelot
-    // (lj_op.hip.cu:130-131); -24 eps is formed in R, as the force term
-    // forms it
-    {
-      std::vector<R> t(4 * (size_t)ntypes_ * ntypes_);
-      for (int a = 0; a < ntypes_; ++a)
-        for (int b = 0; b < ntypes_; ++b) {
-          const double e = std::sqrt(eps_[a] * eps_[b]);
-          const double s = (sig_[a] + sig_[b]) / 2.0;
-          R* row = &t[4 * (a * ntypes_ + b)];
-          row[0] = (R)-24 * (R)e;
-          row[1] = (R)(s * s);
-          row[2] = (R)e;
-          row[3] = (R)0;
-        }
-      lj_tab_ = alloc<R>(t.size());
-      HIPCHK(hipMemcpy(lj_tab_, t.data(), sizeof(R) * t.size(), hipMemcpyHostToDevice));
-    }
+    lj_tab_ = nullptr;  // sized for this system's ntypes
+    upload_lj();
     // ---- k-space
     if (kspace_ == 3) {
       // RBSOG: P sampled vectors (weights S/(4 pi P)) followed by the direct
@@ -1488,6 +1620,22 @@ int rbmdx_force(rbmdx_ctx* ctx, int64_t step, const void* px, const void* py,
     ctx->eng->force(step, px, py, pz, fx, fy, fz, m_samples, num_k, flags,
                     (hipStream_t)stream);
   });
+}
+
+int rbmdx_force_vjp(rbmdx_ctx* ctx, int64_t step, const void* px, const void* py,
+                    const void* pz, const void* gx, const void* gy, const void* gz,
+                    void* dx, void* dy, void* dz, double* dparam, uint32_t flags,
+                    void* stream) {
+  if (!ctx) return RBMDX_E_ARG;
+  return guarded(ctx, [&] {
+    ctx->eng->force_vjp(step, px, py, pz, gx, gy, gz, dx, dy, dz, dparam, flags,
+                        (hipStream_t)stream);
+  });
+}
+
+int rbmdx_set_lj(rbmdx_ctx* ctx, const double* eps, const double* sigma) {
+  if (!ctx || !eps || !sigma) return RBMDX_E_ARG;
+  return guarded(ctx, [&] { ctx->eng->set_lj(eps, sigma); });
 }
 
 int rbmdx_set_masses(rbmdx_ctx* ctx, const double* mass) {

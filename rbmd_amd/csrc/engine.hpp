@@ -245,4 +245,43 @@ template <typename R>
 hipError_t launch_export(const Dev<R>& d, const StepArgs<R>& a, int candidates,
                          int cap, hipStream_t s);
 
+// Force-matching adjoint: buffers of one context (allocated on first use).
+constexpr int kVjpMaxTypes = 8;  // per-lane class slices in LDS
+template <typename R>
+struct VjpDev {
+  using R4 = typename Dev<R>::R4;
+  const R* gx = nullptr;   // [n] cotangent dL/dF, caller order
+  const R* gy = nullptr;
+  const R* gz = nullptr;
+  R* dx = nullptr;         // [n] dL/dr out, caller order
+  R* dy = nullptr;
+  R* dz = nullptr;
+  double* dparam = nullptr;  // [3 * ntypes] out: dL/dQ, dL/deps, dL/dsigma
+  R4* gs = nullptr;        // [n] cotangent, sorted order
+  R4* gc = nullptr;        // [nc] cotangent, charged order
+  R4* drs = nullptr;       // [n] pair part of dL/dr, sorted order
+  R4* drk = nullptr;       // [nc] k-space part of dL/dr, charged order
+  double* gpart = nullptr; // [gather blocks * 3] partial sums of G
+  double* gmean = nullptr; // [3] mean cotangent (mean-force fix)
+  float* cls = nullptr;    // [(nbrick + nq blocks) * 4 * nt * nt] class sums
+  int ncls = 0;            // rows of cls
+  int nq_blocks = 0;       // blocks of the neutral-charged charge pass
+  int* tord = nullptr;     // [n] caller indices, type-major
+  int* ch_beg = nullptr;   // [nch + 1] type-major chunk starts
+  int* ch_type = nullptr;  // [nch]
+  int nch = 0;
+  double* vsk = nullptr;   // [nch * kcap * 8] per chunk and k: T, U, rho, beta
+  float4* kv0 = nullptr;   // [kcap] (m, A Re beta)
+  float4* kv1 = nullptr;   // [kcap] (k, A Im beta)
+  float2* kv2 = nullptr;   // [kcap] (A Re rho, A Im rho)
+  double* dqk = nullptr;   // [ntypes] k-space part of dL/dQ
+  int gather_blocks = 0;
+};
+template <typename R>
+hipError_t launch_vjp(const Dev<R>& d, const VjpDev<R>& v, const StepArgs<R>& a,
+                      hipStream_t s);
+// Skin check, gather and (when decided) rebuild of one evaluation.
+template <typename R>
+hipError_t launch_skin_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s);
+
 }  // namespace rbmdx

@@ -2138,6 +2138,652 @@ hipError_t launch_export(const Dev<R>& d, const StepArgs<R>& a, int candidates,
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Force-matching adjoint: the vector-Jacobian product of one evaluation's
+// force F(r, Q, eps, sigma) with a cotangent G = dL/dF, for positions and the
+// per-type charge (a common shift of the charges of every atom of the type),
+// LJ eps and sigma.  The pair set, RBL selections and k-sample are those of
+// the evaluation.  Per directed pair (i, j), d = r_j - r_i, u = |d|^2,
+// weights w_ij (i's set) and w_ji (j's set), unweighted pair scalar f0(u)
+// (LJ below rc_LJ, erfc Coulomb) with u-derivative f0':
+//   dL/dr_i += f0 H + 2 f0' (H . d) d,  H = w_ji G'_j - w_ij G'_i,
+// G' = G - mean(G) under the mean-force fix; the owner i of (i, j) adds
+// w_ij (G'_i . d) times df0/d eps_ij, df0/d sigma_ij, df0/d(q_i q_j) to its
+// class sums, which the mixing rules carry to the types.  k-space (weights
+// w_k fixed): A_k = w_k (-4 pi qqr2e) / (V k^2), rho = sum q e^{i theta},
+// beta = sum q (G . k) e^{i theta}, per type T_a = sum e^{i theta},
+// U_a = sum (G . k) e^{i theta}:
+//   dL/dQ_a += sum_k A_k [Im(conj(U_a) rho) + Im(conj(beta) T_a)],
+//   dL/dr_m += q_m sum_k A_k [Re(conj(beta) e^{i theta_m})
+//                            - (G_m . k) Re(e^{-i theta_m} rho)] k.
+// S(k) is recomputed here (no per atom-k storage).  Parameter sums use LDS
+// float atomics inside a block (their last bits may vary between runs);
+// everything else is deterministic.
+// ---------------------------------------------------------------------------
+
+// LJ scalars of one pair from its table row (-24 eps, sigma^2, eps, sigma):
+// f = -24 eps (2 s6 - 1) s6 / u, fp = df/du, fe = df/d eps_ij, fs = df/d sigma_ij.
+template <typename R>
+__device__ __forceinline__ void lj_vjp(const R* row, R u, R& f, R& fp, R& fe,
+                                       R& fs) {
+  const R inv = R(1) / u;
+  const R s2 = row[1] * inv;
+  const R s6 = s2 * s2 * s2;
+  f = row[0] * (R(2) * s6 - R(1)) * s6 * inv;
+  fp = R(24) * row[2] * (R(14) * s6 * s6 - R(4) * s6) * inv * inv;
+  fe = R(-24) * (R(2) * s6 - R(1)) * s6 * inv;
+  fs = row[3] > R(0)
+           ? R(-24) * row[2] * (R(4) * s6 - R(1)) * R(6) * s6 * inv / row[3]
+           : R(0);
+}
+
+// erfc Coulomb without the charges: f_C = -qqr2e q_i q_j h(u),
+// h = erfc(g r) / (u r) + (2 g / sqrt(pi)) e^{-alpha u} / u, hp = dh/du.
+template <typename R>
+__device__ __forceinline__ void coul_vjp(const PairConst<R>& pc, R u, R& h,
+                                         R& hp) {
+  const R r = sqrt(u);
+  const R er = erfc(pc.g * r), ex = exp(-pc.alpha * u);
+  const R inv = R(1) / u;
+  h = er * inv / r + pc.two_g_over_sqrtpi * ex * inv;
+  hp = R(-1.5) * er * inv * inv / r -
+       R(1.5) * pc.two_g_over_sqrtpi * ex * inv * inv -
+       pc.alpha * pc.two_g_over_sqrtpi * ex * inv;
+}
+
+// Directed pair weight (the forward's pair_weight with an explicit hash pair).
+template <typename R, bool RBL>
+__device__ __forceinline__ R vjp_w(const PairConst<R>& pc, R u,
+                                   unsigned long long h_own,
+                                   unsigned long long hj_other) {
+  if (!RBL) return u < pc.rc2 ? R(1) : R(0);
+  if (u < pc.t_core) return (u < pc.rs2 && u > pc.eps_min) ? R(1) : R(0);
+  if (u < pc.t_shell && u > pc.rs2 &&
+      rbl_keep(pc.ifreq, pc.fmask, pc.c32, pc.epoch, h_own, hj_other))
+    return pc.freq;
+  return R(0);
+}
+
+// Per-lane class slots in LDS ([4 * nt][blockDim]: eps, sigma, Q, Q q_j by
+// neighbour type) and the block's per-type table ([3][nt]: Q, eps, sigma).
+struct VjpSlots {
+  float* s;
+  float* tab;
+  int nt;
+  __device__ __forceinline__ float& at(int q, int b) const {
+    return s[(q * nt + b) * (int)blockDim.x + threadIdx.x];
+  }
+  __device__ __forceinline__ void zero() const {
+    for (int q = 0; q < 4 * nt; ++q) s[q * (int)blockDim.x + threadIdx.x] = 0.f;
+  }
+  // Carry the owner's class sums to the types: owner type a, charge qi;
+  // eps_ab from the LJ table row, eps_a from the diagonal.
+  template <typename R>
+  __device__ __forceinline__ void flush(int a, R qi, const R* s_lj) const {
+    const R ea = s_lj[4 * (a * nt + a) + 2];
+    float dq_a = 0.f, de_a = 0.f, ds_a = 0.f;
+    for (int b = 0; b < nt; ++b) {
+      const float ge = at(0, b), gs = at(1, b), gq = at(2, b), gqq = at(3, b);
+      if (ge == 0.f && gs == 0.f && gq == 0.f && gqq == 0.f) continue;
+      const R eab = s_lj[4 * (a * nt + b) + 2];
+      const R eb = s_lj[4 * (b * nt + b) + 2];
+      dq_a += gqq;
+      ds_a += 0.5f * gs;
+      if (ea > R(0)) de_a += (float)(ge * eab / (R(2) * ea));
+      atomicAdd(&tab[0 * nt + b], (float)(qi * gq));
+      if (eb > R(0)) atomicAdd(&tab[1 * nt + b], (float)(ge * eab / (R(2) * eb)));
+      atomicAdd(&tab[2 * nt + b], 0.5f * gs);
+    }
+    atomicAdd(&tab[0 * nt + a], dq_a);
+    atomicAdd(&tab[1 * nt + a], de_a);
+    atomicAdd(&tab[2 * nt + a], ds_a);
+    zero();
+  }
+};
+
+// Cotangent to sorted and charged order, with block sums for its mean.
+template <typename R>
+__global__ __launch_bounds__(256) void k_vjp_gather(Dev<R> d, VjpDev<R> v) {
+  double s[3] = {0, 0, 0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
+       i += gridDim.x * blockDim.x) {
+    const R gx = v.gx[i], gy = v.gy[i], gz = v.gz[i];
+    s[0] += (double)gx;
+    s[1] += (double)gy;
+    s[2] += (double)gz;
+    v.gs[d.inv_perm[i]] = make4<R>(gx, gy, gz, R(0));
+    const int c = d.c_of[i];
+    if (c >= 0) v.gc[c] = make4<R>(gx, gy, gz, R(0));
+  }
+  __shared__ double smem[16 * 3];
+  block_sum<3>(s, smem);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; ++q) v.gpart[3 * blockIdx.x + q] = s[q];
+}
+
+__global__ __launch_bounds__(256) void k_vjp_mean(const double* gpart, int nb,
+                                                  double n, int mean_fix,
+                                                  double* gmean) {
+  double s[3] = {0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int q = 0; q < 3; ++q) s[q] += gpart[3 * b + q];
+  __shared__ double smem[16 * 3];
+  block_sum<3>(s, smem);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; ++q) gmean[q] = mean_fix ? s[q] / n : 0.0;
+}
+
+// One owner pair of the adjoint pair pass.  xi/xj positions (the caller
+// provides d and u), gi/gj the mean-shifted cotangents, row the LJ row (null
+// beyond rc_LJ), qq the charge product, coul whether Coulomb applies.
+template <typename R>
+__device__ __forceinline__ void vjp_pair(R dx, R dy, R dz, R u, const R* row,
+                                         bool coul, const PairConst<R>& pc,
+                                         R qi, R qj, R wij, R wji,
+                                         const R gi[3], const R gj[3], int tj,
+                                         R r[3], const VjpSlots& sl) {
+  R f = 0, fp = 0, fe = 0, fs = 0;
+  if (row) lj_vjp<R>(row, u, f, fp, fe, fs);
+  R gqb = 0;
+  if (coul) {
+    R h, hp;
+    coul_vjp<R>(pc, u, h, hp);
+    f += -pc.qqr2e * qi * qj * h;
+    fp += -pc.qqr2e * qi * qj * hp;
+    gqb = -pc.qqr2e * h;
+  }
+  const R hx = wji * gj[0] - wij * gi[0], hy = wji * gj[1] - wij * gi[1],
+          hz = wji * gj[2] - wij * gi[2];
+  const R hd = hx * dx + hy * dy + hz * dz;
+  r[0] += f * hx + R(2) * fp * hd * dx;
+  r[1] += f * hy + R(2) * fp * hd * dy;
+  r[2] += f * hz + R(2) * fp * hd * dz;
+  if (wij != R(0)) {
+    const R gd = wij * (gi[0] * dx + gi[1] * dy + gi[2] * dz);
+    if (row) {
+      sl.at(0, tj) += (float)(fe * gd);
+      sl.at(1, tj) += (float)(fs * gd);
+    }
+    if (coul) {
+      sl.at(2, tj) += (float)(gqb * gd);
+      sl.at(3, tj) += (float)(gqb * gd * qj);
+    }
+  }
+}
+
+// Adjoint pair pass, one workgroup per brick (the forward's lists and stage;
+// the cotangent and global ids staged beside the positions).
+template <typename R, bool RBL>
+__global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> b,
+                                                  Grid g, PairConst<R> pc_arg,
+                                                  const R* lj_tab, int split) {
+  PairConst<R> pc = pc_arg;
+  if (RBL) pc.epoch = d.ctrl->epoch;
+  __shared__ R4T<R> stage[StageCap<R>::value];
+  __shared__ R4T<R> gstage[StageCap<R>::value];
+  __shared__ SlotTables<kMaxSlots> t;
+  __shared__ float tab[3 * kVjpMaxTypes];
+  extern __shared__ char dyn_lds[];
+  const int nt = pc.ntypes;
+  R* s_lj = reinterpret_cast<R*>(dyn_lds);
+  VjpSlots sl;
+  sl.s = reinterpret_cast<float*>(dyn_lds + 4 * sizeof(R) * nt * nt);
+  sl.tab = tab;
+  sl.nt = nt;
+  load_lj<R>(lj_tab, nt, s_lj);
+  for (int q = threadIdx.x; q < 3 * nt; q += blockDim.x) tab[q] = 0.f;
+  sl.zero();
+  const R gm[3] = {(R)v.gmean[0], (R)v.gmean[1], (R)v.gmean[2]};
+  const int e1[3] = {1, 1, 1};
+  const int brick = xcd_brick(blockIdx.x, g.nbrick);
+  const Neigh h = brick_neigh(g, brick, e1);
+  build_slots(g, h, d.key_start, nullptr, false, t);
+  const int total = t.off[h.nslots];
+  const bool staged = total < StageCap<R>::value;
+  if (staged) {
+    stage_atoms<R, true>(g, b, h, t, d.xs, stage);
+    const int lane = threadIdx.x & 31;
+    for (int s = threadIdx.x >> 5; s < h.nslots; s += blockDim.x >> 5) {
+      const int o = t.off[s], len = t.off[s + 1] - o, f = t.first[s];
+      for (int j = lane; j < len; j += 32) {
+        const R4T<R> q = v.gs[f + j];
+        R4T<R> w;
+        w.x = q.x - gm[0];
+        w.y = q.y - gm[1];
+        w.z = q.z - gm[2];
+        w.w = (R)0;
+        reinterpret_cast<int*>(&w.w)[0] = d.gid_s[f + j];
+        gstage[o + j] = w;
+      }
+    }
+    __syncthreads();
+  }
+  constexpr int sh = StageShift<R>::value;
+  const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const R4T<R> xi = d.xs[i];
+    const int ti = (int)xi.w;
+    const R4T<R> g4 = v.gs[i];
+    const R gi[3] = {g4.x - gm[0], g4.y - gm[1], g4.z - gm[2]};
+    const unsigned long long hi = RBL ? rbl_hi(d.gid_s[i]) : 0ull;
+    const unsigned long long hji = RBL ? d.hj_s[i] : 0ull;
+    const R* lrow = s_lj + 4 * ti * nt;
+    R r[3] = {0, 0, 0};
+    const int cnt = d.cnt_a[i];
+    auto one = [&](R dx, R dy, R dz, R u, int tj, const R gj[3], int gidj,
+                   unsigned long long hjj) {
+      if (!(u < pc.rclj2) || !(u < pc.rc2)) return;
+      const R wij = vjp_w<R, RBL>(pc, u, hi, hjj);
+      const R wji = RBL ? vjp_w<R, RBL>(pc, u, rbl_hi(gidj), hji) : wij;
+      if (wij == R(0) && wji == R(0)) return;
+      vjp_pair<R>(dx, dy, dz, u, lrow + 4 * tj, false, pc, R(0), R(0), wij, wji, gi,
+                  gj, tj, r, sl);
+    };
+    if (staged && cnt <= d.cap_a) {
+      for (int k = 0; k < cnt; ++k) {
+        const int l = (int)d.list_a[list_a_index(k, i, d.n)] >> sh;
+        const R4T<R> xj = stage[l];
+        const R4T<R> gq = gstage[l];
+        const int gidj = reinterpret_cast<const int*>(&gq.w)[0];
+        const R gj[3] = {gq.x, gq.y, gq.z};
+        R dx, dy, dz;
+        const R u = h.mi ? pair_d2<R, true>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz)
+                         : pair_d2<R, false>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+        const int tj = pair_woff(xj.w) / (4 * (int)sizeof(R));
+        one(dx, dy, dz, u, tj, gj, gidj, RBL ? rbl_hj(gidj) : 0ull);
+      }
+    } else {
+      // unstaged brick or list overflow: scan the 27 cells directly
+      int c[3];
+      unpack_coord(g.coord_of_cell[d.cell_s[i]], c);
+      const Stencil st = atom_stencil(g, h, c);
+      const int os = own_slot(h, c);
+      const bool qi = split && (i - t.first[os]) < t.nq[os];
+      for (int kz = 0; kz < st.n[2]; ++kz)
+        for (int ky = 0; ky < st.n[1]; ++ky)
+          for (int kx = 0; kx < st.n[0]; ++kx) {
+            const int sl_ = (st.at(2, kz) * h.cnt[1] + st.at(1, ky)) * h.cnt[0] +
+                            st.at(0, kx);
+            const int len = t.off[sl_ + 1] - t.off[sl_];
+            for (int kk = qi ? t.nq[sl_] : 0; kk < len; ++kk) {
+              const int jg = t.first[sl_] + kk;
+              if (jg == i) continue;
+              const R4T<R> xj = d.xs[jg];
+              const R4T<R> gq = v.gs[jg];
+              const R gj[3] = {gq.x - gm[0], gq.y - gm[1], gq.z - gm[2]};
+              R dx, dy, dz;
+              const R u = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+              one(dx, dy, dz, u, (int)xj.w, gj, d.gid_s[jg], RBL ? d.hj_s[jg] : 0ull);
+            }
+          }
+    }
+    v.drs[i] = make4<R>(r[0], r[1], r[2], R(0));
+    sl.flush<R>(ti, R(0), s_lj);
+  }
+  // charged atoms over list B (LJ + Coulomb), kChargedLanes lanes per atom
+  if (split) {
+    __syncthreads();
+    const int c0 = d.cstart[h.cell0], c1 = d.cstart[h.cell1];
+    const int grp = threadIdx.x / kChargedLanes, gl = threadIdx.x % kChargedLanes;
+    const int ngrp = blockDim.x / kChargedLanes;
+    for (int c = c0 + grp; c < c1; c += ngrp) {
+      const R4T<R> xi = d.xqc[c];
+      const int ti = d.typec[c];
+      const R4T<R> g4 = v.gc[c];
+      const R gi[3] = {g4.x - gm[0], g4.y - gm[1], g4.z - gm[2]};
+      const unsigned long long hi = RBL ? rbl_hi(d.gidc[c]) : 0ull;
+      const unsigned long long hji = RBL ? d.hjc[c] : 0ull;
+      const R* lrow = s_lj + 4 * ti * nt;
+      R r[3] = {0, 0, 0};
+      auto one = [&](int e) {
+        const R4T<R> xj = d.xqc[e];
+        R dx, dy, dz;
+        const R u = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+        if (!(u < pc.rc2)) return;
+        const R wij = vjp_w<R, RBL>(pc, u, hi, RBL ? d.hjc[e] : 0ull);
+        const R wji = RBL ? vjp_w<R, RBL>(pc, u, rbl_hi(d.gidc[e]), hji) : wij;
+        if (wij == R(0) && wji == R(0)) return;
+        const R4T<R> gq = v.gc[e];
+        const R gj[3] = {gq.x - gm[0], gq.y - gm[1], gq.z - gm[2]};
+        const int tj = d.typec[e];
+        vjp_pair<R>(dx, dy, dz, u, u < pc.rclj2 ? lrow + 4 * tj : nullptr, true, pc,
+                    xi.w, xj.w, wij, wji, gi, gj, tj, r, sl);
+      };
+      const int cnt = d.cnt_b[c];
+      if (cnt <= d.cap_b) {
+        const int* lp = d.list_b + (size_t)c * d.cap_b;
+        for (int k = gl; k < cnt; k += kChargedLanes) one(lp[k]);
+      } else {
+        const int i = d.c2s[c];
+        int cc[3];
+        unpack_coord(g.coord_of_cell[d.cell_s[i]], cc);
+        const Span sx = make_span(cc[0], g.ext_b[0], g.nc[0]);
+        const Span sy = make_span(cc[1], g.ext_b[1], g.nc[1]);
+        const Span sz = make_span(cc[2], g.ext_b[2], g.nc[2]);
+        const int nst = sx.count * sy.count * sz.count;
+        for (int s = gl; s < nst; s += kChargedLanes) {
+          const int kx = s % sx.count, ky = (s / sx.count) % sy.count,
+                    kz = s / (sx.count * sy.count);
+          const int cell =
+              g.cell_of_coord[(sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0] + sx.at(kx)];
+          for (int e = d.cstart[cell]; e < d.cstart[cell + 1]; ++e)
+            if (e != c) one(e);
+        }
+      }
+      R fx = r[0], fy = r[1], fz = r[2];
+      for (int o = kChargedLanes / 2; o > 0; o >>= 1) {
+        fx += __shfl_xor(fx, o, kChargedLanes);
+        fy += __shfl_xor(fy, o, kChargedLanes);
+        fz += __shfl_xor(fz, o, kChargedLanes);
+      }
+      if (gl == 0) {
+        const int i = d.c2s[c];
+        R4T<R> q = v.drs[i];
+        q.x += fx;
+        q.y += fy;
+        q.z += fz;
+        v.drs[i] = q;
+      }
+      sl.flush<R>(ti, xi.w, s_lj);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 3 * nt; q += blockDim.x)
+    v.cls[(size_t)brick * 3 * nt + q] = tab[q];
+}
+
+// Charge gradient through Coulomb pairs of a charged and a neutral atom (the
+// forward evaluates them without Coulomb, their charge product being 0):
+// each charged atom scans the neutral atoms of the cells within ext_b,
+// kChargedLanes lanes per atom; only Q of the neutral atom's type moves.
+template <typename R, bool RBL>
+__global__ __launch_bounds__(256) void k_vjp_qnc(Dev<R> d, VjpDev<R> v, Box<R> b,
+                                                 Grid g, PairConst<R> pc_arg,
+                                                 int row0) {
+  PairConst<R> pc = pc_arg;
+  if (RBL) pc.epoch = d.ctrl->epoch;
+  __shared__ float tab[kVjpMaxTypes];
+  const int nt = pc.ntypes;
+  for (int q = threadIdx.x; q < nt; q += blockDim.x) tab[q] = 0.f;
+  __syncthreads();
+  const R gm[3] = {(R)v.gmean[0], (R)v.gmean[1], (R)v.gmean[2]};
+  const int grp = threadIdx.x / kChargedLanes, gl = threadIdx.x % kChargedLanes;
+  const int c = blockIdx.x * (blockDim.x / kChargedLanes) + grp;
+  float acc[kVjpMaxTypes];
+  for (int q = 0; q < kVjpMaxTypes; ++q) acc[q] = 0.f;
+  if (c < d.nc) {
+    const int i = d.c2s[c];
+    const R4T<R> xi = d.xqc[c];
+    const R4T<R> g4 = v.gc[c];
+    const R gi[3] = {g4.x - gm[0], g4.y - gm[1], g4.z - gm[2]};
+    const unsigned long long hi = RBL ? rbl_hi(d.gidc[c]) : 0ull;
+    const unsigned long long hji = RBL ? d.hjc[c] : 0ull;
+    int cc[3];
+    unpack_coord(g.coord_of_cell[d.cell_s[i]], cc);
+    const Span sx = make_span(cc[0], g.ext_b[0], g.nc[0]);
+    const Span sy = make_span(cc[1], g.ext_b[1], g.nc[1]);
+    const Span sz = make_span(cc[2], g.ext_b[2], g.nc[2]);
+    const int nst = sx.count * sy.count * sz.count;
+    for (int s = gl; s < nst; s += kChargedLanes) {
+      const int kx = s % sx.count, ky = (s / sx.count) % sy.count,
+                kz = s / (sx.count * sy.count);
+      const int cell =
+          g.cell_of_coord[(sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0] + sx.at(kx)];
+      const int js = d.key_start[2 * cell + 1], je = d.key_start[2 * cell + 2];
+      for (int j = js; j < je; ++j) {
+        const R4T<R> xj = d.xs[j];
+        R dx, dy, dz;
+        const R u = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+        if (!(u < pc.rc2)) continue;
+        const R wcn = vjp_w<R, RBL>(pc, u, hi, RBL ? d.hj_s[j] : 0ull);
+        const R wnc = RBL ? vjp_w<R, RBL>(pc, u, rbl_hi(d.gid_s[j]), hji) : wcn;
+        if (wcn == R(0) && wnc == R(0)) continue;
+        const R4T<R> gq = v.gs[j];
+        const R gj[3] = {gq.x - gm[0], gq.y - gm[1], gq.z - gm[2]};
+        R hh, hp;
+        coul_vjp<R>(pc, u, hh, hp);
+        const R gqb = -pc.qqr2e * hh;
+        const R gid_ = gi[0] * dx + gi[1] * dy + gi[2] * dz;
+        const R gjd = gj[0] * dx + gj[1] * dy + gj[2] * dz;
+        const int tj = (int)xj.w;
+        const float val = (float)(xi.w * gqb * (wcn * gid_ - wnc * gjd));
+        for (int q = 0; q < kVjpMaxTypes; ++q)
+          if (q == tj) acc[q] += val;
+      }
+    }
+  }
+  for (int q = 0; q < nt; ++q)
+    if (acc[q] != 0.f) atomicAdd(&tab[q], acc[q]);
+  __syncthreads();
+  for (int q = threadIdx.x; q < 3 * nt; q += blockDim.x)
+    v.cls[(size_t)(row0 + blockIdx.x) * 3 * nt + q] = q < nt ? tab[q] : 0.f;
+}
+
+// k-vectors per block of the adjoint structure sums (8 sums per k).
+constexpr int kVjpTile = 4;
+
+// Per-type structure sums over all atoms (type-major chunks, caller order):
+// per chunk and k, (T, U, rho, beta) with T = sum e^{i theta},
+// U = sum (G . k) e^{i theta}, rho = sum q e^{i theta}, beta = sum q (G . k)
+// e^{i theta}.
+template <typename R>
+__global__ __launch_bounds__(256) void k_vjp_sk(Dev<R> d, VjpDev<R> v, Box<R> b,
+                                                const int* __restrict__ m, int K) {
+  __shared__ double smem[16 * 8 * kVjpTile];
+  const int ch = blockIdx.x, k0 = blockIdx.y * kVjpTile;
+  const int kn = min(kVjpTile, K - k0);
+  const double two_pi = 6.283185307179586476925;
+  float mx[kVjpTile], my[kVjpTile], mz[kVjpTile], kxv[kVjpTile], kyv[kVjpTile], kzv[kVjpTile];
+#pragma unroll
+  for (int kk = 0; kk < kVjpTile; ++kk) {
+    const int k = k0 + (kk < kn ? kk : 0);
+    mx[kk] = (float)m[3 * k];
+    my[kk] = (float)m[3 * k + 1];
+    mz[kk] = (float)m[3 * k + 2];
+    kxv[kk] = (float)(two_pi * m[3 * k] / (double)b.L[0]);
+    kyv[kk] = (float)(two_pi * m[3 * k + 1] / (double)b.L[1]);
+    kzv[kk] = (float)(two_pi * m[3 * k + 2] / (double)b.L[2]);
+  }
+  float acc[8 * kVjpTile];
+#pragma unroll
+  for (int q = 0; q < 8 * kVjpTile; ++q) acc[q] = 0.f;
+  const int a0 = v.ch_beg[ch], a1 = v.ch_beg[ch + 1];
+  for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
+    const int i = v.tord[a];
+    const float sx = (float)((d.px[i] - b.lo[0]) * b.invL[0]);
+    const float sy = (float)((d.py[i] - b.lo[1]) * b.invL[1]);
+    const float sz = (float)((d.pz[i] - b.lo[2]) * b.invL[2]);
+    const float gx = (float)v.gx[i], gy = (float)v.gy[i], gz = (float)v.gz[i];
+    const float q = (float)d.charge[i];
+#pragma unroll
+    for (int kk = 0; kk < kVjpTile; ++kk) {
+      const float t = fma_r(mx[kk], sx, fma_r(my[kk], sy, mz[kk] * sz));
+      float sn, cs;
+      sincos_rev(t, sn, cs);
+      const float gk = fma_r(gx, kxv[kk], fma_r(gy, kyv[kk], gz * kzv[kk]));
+      float* A = acc + 8 * kk;
+      A[0] += cs;
+      A[1] += sn;
+      A[2] = fma_r(gk, cs, A[2]);
+      A[3] = fma_r(gk, sn, A[3]);
+      A[4] = fma_r(q, cs, A[4]);
+      A[5] = fma_r(q, sn, A[5]);
+      A[6] = fma_r(q * gk, cs, A[6]);
+      A[7] = fma_r(q * gk, sn, A[7]);
+    }
+  }
+  double dv[8 * kVjpTile];
+#pragma unroll
+  for (int q = 0; q < 8 * kVjpTile; ++q) dv[q] = (double)acc[q];
+  block_sum<8 * kVjpTile>(dv, smem);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int kk = 0; kk < kVjpTile; ++kk)
+      if (kk < kn)
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          v.vsk[((size_t)ch * K + k0 + kk) * 8 + q] = dv[8 * kk + q];
+  }
+}
+
+// FP64 sums over chunks per k: the k-space charge gradient per type and the
+// per-k coefficients of the position pass.  One block.
+template <typename R>
+__global__ __launch_bounds__(1024) void k_vjp_sk_fin(Dev<R> d, VjpDev<R> v,
+                                                     Box<R> b, const int* m, int K,
+                                                     Params p, int nt) {
+  const double two_pi = 6.283185307179586476925;
+  double dq[kVjpMaxTypes];
+  for (int a = 0; a < kVjpMaxTypes; ++a) dq[a] = 0.0;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    double rr = 0, ri = 0, br = 0, bi = 0;
+    for (int ch = 0; ch < v.nch; ++ch) {
+      const double* s = v.vsk + ((size_t)ch * K + k) * 8;
+      rr += s[4];
+      ri += s[5];
+      br += s[6];
+      bi += s[7];
+    }
+    const double kx = two_pi * m[3 * k] / (double)b.L[0];
+    const double ky = two_pi * m[3 * k + 1] / (double)b.L[1];
+    const double kz = two_pi * m[3 * k + 2] / (double)b.L[2];
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double A = d.kw[k] * (-4.0 * 3.141592653589793238463 * p.qqr2e) / (p.volume * k2);
+    for (int ch = 0; ch < v.nch; ++ch) {
+      const double* s = v.vsk + ((size_t)ch * K + k) * 8;
+      const int a = v.ch_type[ch];
+      const double c = A * ((s[2] * ri - s[3] * rr) + (br * s[1] - bi * s[0]));
+      for (int q = 0; q < kVjpMaxTypes; ++q)
+        if (q == a) dq[q] += c;
+    }
+    v.kv0[k] = make_float4((float)m[3 * k], (float)m[3 * k + 1], (float)m[3 * k + 2],
+                           (float)(A * br));
+    v.kv1[k] = make_float4((float)kx, (float)ky, (float)kz, (float)(A * bi));
+    v.kv2[k] = make_float2((float)(A * rr), (float)(A * ri));
+  }
+  __shared__ double smem[16 * kVjpMaxTypes];
+  block_sum<kVjpMaxTypes>(dq, smem);
+  if (threadIdx.x == 0)
+    for (int a = 0; a < nt; ++a) v.dqk[a] = dq[a];
+}
+
+// k-space part of dL/dr for the charged atoms (charged order).
+template <typename R>
+__global__ __launch_bounds__(256) void k_vjp_kforce(Dev<R> d, VjpDev<R> v,
+                                                    Box<R> b, int K) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.nc) return;
+  const R4T<R> p = d.xqc[c];
+  float sx, sy, sz;
+  frac_coords<R>(b, p, sx, sy, sz);
+  const R4T<R> g4 = v.gc[c];
+  const float gx = (float)g4.x, gy = (float)g4.y, gz = (float)g4.z;
+  float rx = 0.f, ry = 0.f, rz = 0.f;
+  for (int k = 0; k < K; ++k) {
+    const float4 a = v.kv0[k];
+    const float4 e = v.kv1[k];
+    const float2 rh = v.kv2[k];
+    const float t = fma_r(a.x, sx, fma_r(a.y, sy, a.z * sz));
+    float sn, cs;
+    sincos_rev(t, sn, cs);
+    const float gk = fma_r(gx, e.x, fma_r(gy, e.y, gz * e.z));
+    const float s = fma_r(a.w, cs, e.w * sn) - gk * fma_r(rh.x, cs, rh.y * sn);
+    rx = fma_r(s, e.x, rx);
+    ry = fma_r(s, e.y, ry);
+    rz = fma_r(s, e.z, rz);
+  }
+  const float q = (float)p.w;
+  v.drk[c] = make4<R>((R)(q * rx), (R)(q * ry), (R)(q * rz), R(0));
+}
+
+// dL/dr in caller order; keeps the evaluation counter in step with the
+// forward path (the step ring is indexed by it).
+template <typename R>
+__global__ __launch_bounds__(256) void k_vjp_assemble(Dev<R> d, VjpDev<R> v, int has_k) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    d.ctrl->maxdisp_bits = 0u;
+    d.ctrl->step_count += 1;
+    if (d.ctrl_host) d.ctrl_host->step_count = d.ctrl->step_count;
+  }
+  if (i >= d.n) return;
+  R4T<R> r = v.drs[d.inv_perm[i]];
+  const int c = d.c_of[i];
+  if (has_k && c >= 0) {
+    const R4T<R> k = v.drk[c];
+    r.x += k.x;
+    r.y += k.y;
+    r.z += k.z;
+  }
+  v.dx[i] = r.x;
+  v.dy[i] = r.y;
+  v.dz[i] = r.z;
+}
+
+// Parameter gradients: FP64 sums of the per-block tables in a fixed order,
+// plus the k-space charge gradient.  dparam = [dL/dQ | dL/deps | dL/dsigma].
+template <typename R>
+__global__ __launch_bounds__(256) void k_vjp_params(VjpDev<R> v, int nt, int has_k) {
+  __shared__ double smem[16 * 3 * kVjpMaxTypes];
+  double s[3 * kVjpMaxTypes];
+  for (int q = 0; q < 3 * kVjpMaxTypes; ++q) s[q] = 0.0;
+  for (int row = threadIdx.x; row < v.ncls; row += blockDim.x)
+    for (int q = 0; q < 3 * nt; ++q) s[q] += (double)v.cls[(size_t)row * 3 * nt + q];
+  block_sum<3 * kVjpMaxTypes>(s, smem);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3 * nt; ++q)
+      v.dparam[q] = s[q] + ((has_k && q < nt) ? v.dqk[q] : 0.0);
+}
+
+// The rebuild part of an evaluation alone: skin check and gather, and the
+// rebuild when the decision says so (lists and sorted tiles current).
+template <typename R>
+hipError_t launch_skin_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
+  const int sb = blocks_for(d.n, kBlock, kSkinBlocksMax);
+  hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(kBlock), 0, s, d, a.box, 0);
+  return build_part<R>(d, a, sb, s);
+}
+
+template <typename R>
+hipError_t launch_vjp(const Dev<R>& d, const VjpDev<R>& v, const StepArgs<R>& a,
+                      hipStream_t s) {
+  const Params& p = a.p;
+  const Grid& g = a.grid;
+  const int nt = a.pc.ntypes;
+  hipLaunchKernelGGL((k_vjp_gather<R>), dim3(v.gather_blocks), dim3(kBlock), 0, s, d, v);
+  hipLaunchKernelGGL(k_vjp_mean, dim3(1), dim3(256), 0, s, v.gpart, v.gather_blocks,
+                     (double)d.n, p.mean_fix, v.gmean);
+  const size_t lds = 4 * sizeof(R) * (size_t)nt * nt + 4 * sizeof(float) * (size_t)nt * g.block;
+  if (p.rbl)
+    hipLaunchKernelGGL((k_pair_vjp<R, true>), dim3(g.nbrick), dim3(g.block), lds, s, d, v,
+                       a.box, g, a.pc, a.lj_tab, p.split);
+  else
+    hipLaunchKernelGGL((k_pair_vjp<R, false>), dim3(g.nbrick), dim3(g.block), lds, s, d, v,
+                       a.box, g, a.pc, a.lj_tab, p.split);
+  if (v.nq_blocks > 0) {
+    if (p.rbl)
+      hipLaunchKernelGGL((k_vjp_qnc<R, true>), dim3(v.nq_blocks), dim3(kBlock), 0, s, d, v,
+                         a.box, g, a.pc, g.nbrick);
+    else
+      hipLaunchKernelGGL((k_vjp_qnc<R, false>), dim3(v.nq_blocks), dim3(kBlock), 0, s, d, v,
+                         a.box, g, a.pc, g.nbrick);
+  }
+  const int K = p.kspace ? p.num_k : 0;
+  const bool has_k = K > 0 && d.nc > 0;
+  if (has_k) {
+    hipLaunchKernelGGL((k_vjp_sk<R>), dim3(v.nch, (K + kVjpTile - 1) / kVjpTile),
+                       dim3(kBlock), 0, s, d, v, a.box, (const int*)d.m_dev, K);
+    hipLaunchKernelGGL((k_vjp_sk_fin<R>), dim3(1), dim3(1024), 0, s, d, v, a.box,
+                       (const int*)d.m_dev, K, p, nt);
+    hipLaunchKernelGGL((k_vjp_kforce<R>), dim3(blocks_for(d.nc)), dim3(kBlock), 0, s, d,
+                       v, a.box, K);
+  }
+  hipLaunchKernelGGL((k_vjp_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0, s, d,
+                     v, has_k ? 1 : 0);
+  hipLaunchKernelGGL((k_vjp_params<R>), dim3(1), dim3(256), 0, s, v, nt, has_k ? 1 : 0);
+  return hipGetLastError();
+}
+
 template hipError_t launch_force<float>(const Dev<float>&,
                                         const StepArgs<float>&, hipStream_t);
 template hipError_t launch_force<double>(const Dev<double>&,
@@ -2153,4 +2799,15 @@ template hipError_t launch_export<double>(const Dev<double>&,
                                           const StepArgs<double>&, int, int,
                                           hipStream_t);
 
+}  // namespace rbmdx
+
+namespace rbmdx {
+template hipError_t launch_skin_build<float>(const Dev<float>&, const StepArgs<float>&,
+                                             hipStream_t);
+template hipError_t launch_skin_build<double>(const Dev<double>&, const StepArgs<double>&,
+                                              hipStream_t);
+template hipError_t launch_vjp<float>(const Dev<float>&, const VjpDev<float>&,
+                                      const StepArgs<float>&, hipStream_t);
+template hipError_t launch_vjp<double>(const Dev<double>&, const VjpDev<double>&,
+                                       const StepArgs<double>&, hipStream_t);
 }  // namespace rbmdx
