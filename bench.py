@@ -63,17 +63,99 @@ def parse():
     ap.add_argument("--dt", type=float, default=0.002)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--workload", choices=["electrolyte", "adjoint"], default="electrolyte",
+                    help="electrolyte: the headline force step (default); adjoint: "
+                         "forward and force-matching adjoint on the 2,000,000-particle "
+                         "electrolyte with P = 400 (prints its own JSON line)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB three times (a "
                          "plain copy kernel of known size, to scale HBM counters)")
     return ap.parse_args()
 
 
+def bench_adjoint(args):
+    """Forward evaluation and its force-matching adjoint (dL/dr and per-type
+    dL/dQ, dL/deps, dL/dsigma) on the 2,000,000-particle LJ electrolyte, RBL +
+    RBE with P = 400: GPU time of each, and their ratio.  dL/dQ shifts the
+    charged atoms of a type (the neutral solvent's charge is fixed at 0); the
+    line also carries the time with the neutral type's dL/dQ at Q = 0
+    (neutral_q: Coulomb pairs of charged and neutral atoms and the neutral
+    atoms' structure sums added)."""
+    import torch
+
+    from rbmd_amd import ForceContext, rbe_sample_host, systems
+
+    n = args.atoms if args.atoms != 200000 else 2000000
+    dev = torch.device("cuda:0")
+    # a seeded box of n / 8 particles replicated 2 x 2 x 2 (RBMD's replicate)
+    s = systems.replicate(systems.electrolyte(n // 8, seed=5), (2, 2, 2))
+    n = s.n
+    cfg = systems.ljsalt_config(P=400)
+    ctx = ForceContext(cfg, device=0)
+    ctx.set_system(s.box_lo, s.box_len, s.types, s.ids, s.charges, s.eps, s.sigma)
+    p = torch.tensor(s.pos.astype(np.float32), device=dev)
+    px, py, pz = (p[:, d].contiguous() for d in range(3))
+    f = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    G = torch.randn((3, n), dtype=torch.float32, device=dev, generator=torch.Generator(
+        device=dev).manual_seed(3))
+    dr = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    dparam = torch.zeros(9, dtype=torch.float64, device=dev)
+    m = torch.tensor(rbe_sample_host(0.09, s.box_len, 400, 7, 0).reshape(-1),
+                     dtype=torch.int32, device=dev)
+    steps, warm = max(1, min(args.steps, 20)), max(1, min(args.warmup, 5))
+
+    def fwd(k):
+        ctx.force(k, px, py, pz, *f, m_samples=m)
+
+    def vjp(k, neutral_q=False):
+        ctx.force_vjp(k, px, py, pz, G[0], G[1], G[2], *dr, dparam, neutral_q=neutral_q)
+
+    for k in range(warm):
+        fwd(k)
+        vjp(k)
+        vjp(k, True)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record()
+    for k in range(steps):
+        fwd(k)
+    ev[1].record()
+    for k in range(steps):
+        fwd(k)
+        vjp(k)
+    ev[2].record()
+    for k in range(steps):
+        fwd(k)
+        vjp(k, True)
+    ev[3].record()
+    torch.cuda.synchronize()
+    t_f = ev[0].elapsed_time(ev[1]) / steps
+    t_b = ev[1].elapsed_time(ev[2]) / steps - t_f
+    t_bn = ev[2].elapsed_time(ev[3]) / steps - t_f
+    line = {
+        "metric": "force-matching adjoint / forward GPU time, 2M LJ-salt RBE P=400",
+        "value": round(t_b / t_f, 3), "unit": "ratio", "n_gpus": 1, "steps": steps,
+        "warmup": warm, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+        "forward_ms": round(t_f, 4), "adjoint_ms": round(t_b, 4),
+        "adjoint_neutral_q_ms": round(t_bn, 4), "ratio_neutral_q": round(t_bn / t_f, 3),
+        "config": {"workload": f"LJ electrolyte RBL+RBE, {n} atoms, P=400, adjoint of "
+                               "L = sum |F - F_t|^2 w.r.t. positions and per-type Q, "
+                               "eps, sigma", "atoms": n, "P": 400},
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
+    if args.workload == "adjoint":
+        if args.gpus != 1:
+            sys.exit(2)
+        bench_adjoint(args)
+        return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the "
-              "multi-GPU decomposition is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); how the "
+              "path would shard over several GPUs is a design note, not a measurement",
+              file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -61,9 +61,11 @@ enum {
                               (read with rbmdx_kernel_times; the events
                               themselves cost a few microseconds per call;
                               implies RBMDX_FLAG_EAGER) */
-  RBMDX_FLAG_EAGER = 16    /* launch the kernels one by one instead of
+  RBMDX_FLAG_EAGER = 16,   /* launch the kernels one by one instead of
                               replaying the context's captured hipGraph
                               (also RBMDX_GRAPH=0 in the environment) */
+  RBMDX_FLAG_VJP_NEUTRAL_Q = 32 /* rbmdx_force_vjp: let dL/dQ_t include the
+                              atoms of zero charge (see there) */
 };
 
 /*
@@ -73,14 +75,22 @@ enum {
  * force's precision):
  *   dx/dy/dz  dL/dr, device arrays in caller order;
  *   dparam    device array of 3 ntypes doubles: dL/dQ_t (a common shift of
- *             the charges of every atom of type t), dL/deps_t, dL/dsigma_t
- *             through the mixing rules.
+ *             the charges of the charged atoms of type t), dL/deps_t,
+ *             dL/dsigma_t through the mixing rules.
+ * The charged/neutral partition is fixed for a system (rbmdx_set_charges
+ * keeps zeros zero), so by default the atoms of zero charge stay out of
+ * dL/dQ_t.  With RBMDX_FLAG_VJP_NEUTRAL_Q the shift covers every atom of
+ * type t: the Coulomb pairs of a charged and a neutral atom and the neutral
+ * atoms' structure sums enter as well (several times the cost when most atoms
+ * are neutral).
  * The pair set (step is the RBL sampling epoch, as for rbmdx_force) and the
  * k-sample are those of the last rbmdx_force call; the RBL mean-force fix is
  * differentiated too.  LJ + erfc Coulomb pairs with RBE, RBSOG or EWALD
  * k-space; not CVFF special bonds or the VERLET-SOG near field; at most 8
  * types.  Eager launches on stream; parameter sums use LDS float atomics
- * inside a block, so their last bits may vary between runs.
+ * inside a workgroup and FP64 atomics across workgroups, so their last bits
+ * may vary between runs.  Flags:
+ * RBMDX_FLAG_SYNC, RBMDX_FLAG_REBUILD, RBMDX_FLAG_VJP_NEUTRAL_Q.
  */
 int rbmdx_force_vjp(rbmdx_ctx* ctx, int64_t step, const void* px,
                     const void* py, const void* pz, const void* gx,

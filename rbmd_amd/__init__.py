@@ -24,6 +24,7 @@ RBMDX_FLAG_SYNC = 2
 RBMDX_FLAG_REBUILD = 4
 RBMDX_FLAG_TIMING = 8
 RBMDX_FLAG_EAGER = 16
+RBMDX_FLAG_VJP_NEUTRAL_Q = 32
 
 _STATUS = {
     0: "RBMDX_OK",
@@ -305,11 +306,14 @@ class ForceContext:
         self._check(self.lib.rbmdx_set_lj(self._h, _dptr(e), _dptr(s)))
 
     def force_vjp(self, step, px, py, pz, gx, gy, gz, dx, dy, dz, dparam, sync=False,
-                  stream=None):
+                  neutral_q=False, stream=None):
         """Vector-Jacobian product of the last evaluation's force with the
         cotangent (gx, gy, gz): dL/dr into (dx, dy, dz) and [dL/dQ | dL/deps |
-        dL/dsigma] per type into dparam (float64 device tensor of 3 ntypes)."""
-        flags = RBMDX_FLAG_SYNC if sync else 0
+        dL/dsigma] per type into dparam (float64 device tensor of 3 ntypes).
+        dL/dQ shifts the charged atoms of a type; neutral_q=True shifts every
+        atom of the type (neutral ones included)."""
+        flags = ((RBMDX_FLAG_SYNC if sync else 0)
+                 | (RBMDX_FLAG_VJP_NEUTRAL_Q if neutral_q else 0))
         if stream is None:
             import torch
             stream = torch.cuda.current_stream(px.device).cuda_stream

@@ -8,7 +8,10 @@ recomputed in the backward pass, not stored per atom and k.
 
 Per-type charges are applied as per-atom charges Q[type]; an update may not
 turn a neutral type charged or back (the charged/neutral partition of the pair
-work is fixed when the system is set).  The pair set (RBL sampling epoch
+work is fixed when the system is set).  dL/dQ of a neutral type is therefore 0
+unless ``neutral_charge_grad=True``, which computes it as the derivative at
+Q = 0 (the Coulomb pairs of charged and neutral atoms and the neutral atoms'
+structure sums enter the backward pass).  The pair set (RBL sampling epoch
 ``step``) and the k-sample are those of the forward evaluation; the backward
 pass must run before the next forward evaluation of the same context.
 """
@@ -42,7 +45,8 @@ class _ForceFn(torch.autograd.Function):
         d = [torch.empty_like(px) for _ in range(3)]
         nt = model.ntypes
         dparam = torch.empty(3 * nt, dtype=torch.float64, device=px.device)
-        model.ctx.force_vjp(fctx.step, px, py, pz, gx, gy, gz, *d, dparam)
+        model.ctx.force_vjp(fctx.step, px, py, pz, gx, gy, gz, *d, dparam,
+                            neutral_q=model.neutral_charge_grad)
         dpos = torch.stack(d, dim=1).to(gF.dtype)
         grads = [dparam[k * nt:(k + 1) * nt].to(device=dev, dtype=dt)
                  for k, (dev, dt) in enumerate(zip(fctx.pdev, fctx.pdtype))]
@@ -58,7 +62,8 @@ class DifferentiableForce:
     >>> ((F - F_target) ** 2).sum().backward()
     """
 
-    def __init__(self, cfg, system, device=0):
+    def __init__(self, cfg, system, device=0, neutral_charge_grad=False):
+        self.neutral_charge_grad = bool(neutral_charge_grad)
         self.ctx = ForceContext(cfg, device=device)
         self.types = np.asarray(system.types, dtype=np.int32)
         self.ntypes = len(system.eps)

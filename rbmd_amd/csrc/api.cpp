@@ -510,18 +510,6 @@ class Engine : public EngineBase {
     if (ntypes_ > kVjpMaxTypes)
       throw Error(RBMDX_E_CONFIG, "the adjoint supports at most 8 atom types");
     HIPCHK(hipSetDevice(device_));
-    {
-      // LDS of the adjoint pair pass: two 24 KB stages, slot tables, the LJ
-      // table and per-lane class slots (4 ntypes floats per thread)
-      int max_lds = 0;
-      HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
-                                   device_));
-      const size_t need = 2 * 24576 + 4096 + 4 * sizeof(R) * (size_t)ntypes_ * ntypes_ +
-                          16 * (size_t)ntypes_ * grid_.block;
-      if (need > (size_t)max_lds)
-        throw Error(RBMDX_E_CONFIG, "the adjoint's LDS budget is exceeded for this "
-                                    "number of types and brick size");
-    }
     bind_positions(px, py, pz);
     grow_if_needed(s);
     int force_rebuild = (flags & RBMDX_FLAG_REBUILD) ? 1 : 0;
@@ -541,6 +529,7 @@ class Engine : public EngineBase {
     vjp_.dy = (R*)dy;
     vjp_.dz = (R*)dz;
     vjp_.dparam = dparam;
+    vjp_.neutral_q = (flags & RBMDX_FLAG_VJP_NEUTRAL_Q) ? 1 : 0;
     a.p.num_k = last_num_k_;
     // the evaluation's lists and sorted tiles (as the forward path builds them)
     HIPCHK(launch_skin_build<R>(d_, a, s));
@@ -564,39 +553,59 @@ class Engine : public EngineBase {
     v.gather_blocks = std::min(1024, std::max(1, (n_ + kBlock - 1) / kBlock));
     v.gpart = alloc<double>(3 * (size_t)v.gather_blocks);
     v.gmean = alloc<double>(3);
-    v.nq_blocks = (params_.split && d_.nc > 0)
-                      ? (d_.nc + kBlock / kChargedLanes - 1) / (kBlock / kChargedLanes)
-                      : 0;
-    v.ncls = grid_.nbrick + v.nq_blocks;
-    v.cls = alloc<float>((size_t)v.ncls * 3 * ntypes_);
-    // type-major order of the caller's atoms, in chunks of at most kSkChunk
-    // atoms of one type
-    std::vector<int> ord(n_), beg, typ;
+    // the neutral-charged charge pass: one row per brick
+    v.nq_blocks = (params_.split && d_.nc > 0 && d_.nc < n_) ? grid_.nbrick : 0;
+    v.pacc = alloc<double>(3 * kVjpMaxTypes);
+    // caller indices, charged atoms type-major then neutral atoms type-major,
+    // cut into chunks of at most kSkChunk atoms of one type and part
+    auto neutral = [&](int i) { return (R)q_[i] == R(0) ? 1 : 0; };
+    std::vector<int> ord(n_), beg, gbeg, gtyp;
     std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(),
-                     [&](int x, int y) { return type_[x] < type_[y]; });
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+      const int nx = neutral(x), ny = neutral(y);
+      return nx != ny ? nx < ny : type_[x] < type_[y];
+    });
+    int nch_c = -1, ngrp_c = -1;
     for (int s = 0; s < n_;) {
+      const int i0 = ord[s];
+      if (neutral(i0) && nch_c < 0) {
+        nch_c = (int)beg.size();
+        ngrp_c = (int)gtyp.size();
+      }
+      if (s == 0 || neutral(i0) != neutral(ord[s - 1]) || type_[i0] != type_[ord[s - 1]]) {
+        gbeg.push_back((int)beg.size());
+        gtyp.push_back(type_[i0]);
+      }
       int e = s;
-      while (e < n_ && type_[ord[e]] == type_[ord[s]] && e - s < kSkChunk) ++e;
+      while (e < n_ && neutral(ord[e]) == neutral(i0) && type_[ord[e]] == type_[i0] &&
+             e - s < kSkChunk)
+        ++e;
       beg.push_back(s);
-      typ.push_back(type_[ord[s]]);
       s = e;
     }
+    v.nch = (int)beg.size();
+    v.ngrp = (int)gtyp.size();
+    v.nch_c = nch_c < 0 ? v.nch : nch_c;
+    v.ngrp_c = ngrp_c < 0 ? v.ngrp : ngrp_c;
     beg.push_back(n_);
-    v.nch = (int)typ.size();
+    gbeg.push_back(v.nch);
     v.tord = alloc<int>(n_);
     v.ch_beg = alloc<int>(beg.size());
-    v.ch_type = alloc<int>(typ.size());
+    v.grp_beg = alloc<int>(gbeg.size());
+    v.grp_type = alloc<int>(std::max<size_t>(1, gtyp.size()));
     HIPCHK(hipMemcpy(v.tord, ord.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(v.ch_beg, beg.data(), sizeof(int) * beg.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(v.ch_type, typ.data(), sizeof(int) * typ.size(),
+    HIPCHK(hipMemcpy(v.grp_beg, gbeg.data(), sizeof(int) * gbeg.size(),
                      hipMemcpyHostToDevice));
+    if (!gtyp.empty())
+      HIPCHK(hipMemcpy(v.grp_type, gtyp.data(), sizeof(int) * gtyp.size(),
+                       hipMemcpyHostToDevice));
     const int K = std::max(1, d_.kcap);
     v.vsk = alloc<double>((size_t)v.nch * K * 8);
     v.kv0 = alloc<float4>(K);
     v.kv1 = alloc<float4>(K);
     v.kv2 = alloc<float2>(K);
-    v.dqk = alloc<double>(kVjpMaxTypes);
+    v.dqk = alloc<double>((size_t)K * kVjpMaxTypes);
   }
 
   void force(int64_t step, const void* px, const void* py, const void* pz,

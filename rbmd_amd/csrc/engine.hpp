@@ -246,7 +246,7 @@ hipError_t launch_export(const Dev<R>& d, const StepArgs<R>& a, int candidates,
                          int cap, hipStream_t s);
 
 // Force-matching adjoint: buffers of one context (allocated on first use).
-constexpr int kVjpMaxTypes = 8;  // per-lane class slices in LDS
+constexpr int kVjpMaxTypes = 8;  // types of the adjoint's fixed-size per-type tables
 template <typename R>
 struct VjpDev {
   using R4 = typename Dev<R>::R4;
@@ -263,18 +263,21 @@ struct VjpDev {
   R4* drk = nullptr;       // [nc] k-space part of dL/dr, charged order
   double* gpart = nullptr; // [gather blocks * 3] partial sums of G
   double* gmean = nullptr; // [3] mean cotangent (mean-force fix)
-  float* cls = nullptr;    // [(nbrick + nq blocks) * 4 * nt * nt] class sums
-  int ncls = 0;            // rows of cls
-  int nq_blocks = 0;       // blocks of the neutral-charged charge pass
-  int* tord = nullptr;     // [n] caller indices, type-major
-  int* ch_beg = nullptr;   // [nch + 1] type-major chunk starts
-  int* ch_type = nullptr;  // [nch]
-  int nch = 0;
+  double* pacc = nullptr;  // [3 * kVjpMaxTypes] pair passes' per-type sums
+  int nq_blocks = 0;       // workgroups of the neutral-charged charge pass
+  int neutral_q = 0;       // this call: also dL/dQ through the neutral atoms
+  int* tord = nullptr;     // [n] caller indices: charged atoms type-major, then
+                           // neutral atoms type-major
+  int* ch_beg = nullptr;   // [nch + 1] chunk starts (one type, one of the two parts)
+  int nch = 0, nch_c = 0;  // chunks, of them charged (first)
+  int* grp_beg = nullptr;  // [ngrp + 1] chunk ranges of one (part, type)
+  int* grp_type = nullptr; // [ngrp]
+  int ngrp = 0, ngrp_c = 0;
   double* vsk = nullptr;   // [nch * kcap * 8] per chunk and k: T, U, rho, beta
   float4* kv0 = nullptr;   // [kcap] (m, A Re beta)
   float4* kv1 = nullptr;   // [kcap] (k, A Im beta)
   float2* kv2 = nullptr;   // [kcap] (A Re rho, A Im rho)
-  double* dqk = nullptr;   // [ntypes] k-space part of dL/dQ
+  double* dqk = nullptr;   // [kcap * kVjpMaxTypes] k-space dL/dQ terms per k
   int gather_blocks = 0;
 };
 template <typename R>

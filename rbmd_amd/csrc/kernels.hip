@@ -2157,24 +2157,21 @@ hipError_t launch_export(const Dev<R>& d, const StepArgs<R>& a, int candidates,
 //   dL/dr_m += q_m sum_k A_k [Re(conj(beta) e^{i theta_m})
 //                            - (G_m . k) Re(e^{-i theta_m} rho)] k.
 // S(k) is recomputed here (no per atom-k storage).  Parameter sums use LDS
-// float atomics inside a block (their last bits may vary between runs);
-// everything else is deterministic.
+// float atomics inside a workgroup and FP64 atomics across workgroups (their
+// last bits may vary between runs); everything else is deterministic.
 // ---------------------------------------------------------------------------
 
-// LJ scalars of one pair from its table row (-24 eps, sigma^2, eps, sigma):
-// f = -24 eps (2 s6 - 1) s6 / u, fp = df/du, fe = df/d eps_ij, fs = df/d sigma_ij.
+// LJ scalars of one pair from its adjoint table row (-24 eps, sigma^2, eps,
+// 1/sigma or 0): f = -24 eps (2 s6 - 1) s6 / u (= eps df/deps), fp = df/du,
+// fs = df/d sigma_ij.
 template <typename R>
-__device__ __forceinline__ void lj_vjp(const R* row, R u, R& f, R& fp, R& fe,
-                                       R& fs) {
+__device__ __forceinline__ void lj_vjp(const R* row, R u, R& f, R& fp, R& fs) {
   const R inv = R(1) / u;
   const R s2 = row[1] * inv;
   const R s6 = s2 * s2 * s2;
   f = row[0] * (R(2) * s6 - R(1)) * s6 * inv;
   fp = R(24) * row[2] * (R(14) * s6 * s6 - R(4) * s6) * inv * inv;
-  fe = R(-24) * (R(2) * s6 - R(1)) * s6 * inv;
-  fs = row[3] > R(0)
-           ? R(-24) * row[2] * (R(4) * s6 - R(1)) * R(6) * s6 * inv / row[3]
-           : R(0);
+  fs = R(-144) * row[2] * (R(4) * s6 - R(1)) * s6 * inv * row[3];
 }
 
 // erfc Coulomb without the charges: f_C = -qqr2e q_i q_j h(u),
@@ -2204,46 +2201,31 @@ __device__ __forceinline__ R vjp_w(const PairConst<R>& pc, R u,
   return R(0);
 }
 
-// Per-lane class slots in LDS ([4 * nt][blockDim]: eps, sigma, Q, Q q_j by
-// neighbour type) and the block's per-type table ([3][nt]: Q, eps, sigma).
-struct VjpSlots {
-  float* s;
-  float* tab;
-  int nt;
-  __device__ __forceinline__ float& at(int q, int b) const {
-    return s[(q * nt + b) * (int)blockDim.x + threadIdx.x];
-  }
-  __device__ __forceinline__ void zero() const {
-    for (int q = 0; q < 4 * nt; ++q) s[q * (int)blockDim.x + threadIdx.x] = 0.f;
-  }
-  // Carry the owner's class sums to the types: owner type a, charge qi;
-  // eps_ab from the LJ table row, eps_a from the diagonal.
-  template <typename R>
-  __device__ __forceinline__ void flush(int a, R qi, const R* s_lj) const {
-    const R ea = s_lj[4 * (a * nt + a) + 2];
-    float dq_a = 0.f, de_a = 0.f, ds_a = 0.f;
-    for (int b = 0; b < nt; ++b) {
-      const float ge = at(0, b), gs = at(1, b), gq = at(2, b), gqq = at(3, b);
-      if (ge == 0.f && gs == 0.f && gq == 0.f && gqq == 0.f) continue;
-      const R eab = s_lj[4 * (a * nt + b) + 2];
-      const R eb = s_lj[4 * (b * nt + b) + 2];
-      dq_a += gqq;
-      ds_a += 0.5f * gs;
-      if (ea > R(0)) de_a += (float)(ge * eab / (R(2) * ea));
-      atomicAdd(&tab[0 * nt + b], (float)(qi * gq));
-      if (eb > R(0)) atomicAdd(&tab[1 * nt + b], (float)(ge * eab / (R(2) * eb)));
-      atomicAdd(&tab[2 * nt + b], 0.5f * gs);
-    }
-    atomicAdd(&tab[0 * nt + a], dq_a);
-    atomicAdd(&tab[1 * nt + a], de_a);
-    atomicAdd(&tab[2 * nt + a], ds_a);
-    zero();
-  }
+// Parameter cotangents of one owner atom.  Every pair is listed once from
+// each side, so the owner i adds what both directed pairs (i, j) and (j, i)
+// contribute to its own type a: with m = w_ij G'_i . d - w_ji G'_j . d
+// (= -H . d), dL/dQ_a += q_j (-qqr2e h) m, dL/deps_a += f_LJ m / (2 eps_a)
+// (f_LJ = eps_ab df/deps_ab and deps_ab/deps_a = eps_ab / (2 eps_a)),
+// dL/dsigma_a += 0.5 (df/dsigma_ab) m.
+struct VjpAcc {
+  float dq = 0.f, de = 0.f, ds = 0.f;
 };
+
+// Adds an owner's sums to the block's per-type table ([3][nt]: Q, eps,
+// sigma); eps_a from the LJ table diagonal.
+template <typename R>
+__device__ __forceinline__ void vjp_flush(float* tab, int nt, int a, const R* s_lj,
+                                          const VjpAcc& c) {
+  const R ea = s_lj[4 * (a * nt + a) + 2];
+  if (c.dq != 0.f) atomicAdd(&tab[a], c.dq);
+  if (c.de != 0.f && ea > R(0)) atomicAdd(&tab[nt + a], (float)(c.de / (R(2) * ea)));
+  if (c.ds != 0.f) atomicAdd(&tab[2 * nt + a], c.ds);
+}
 
 // Cotangent to sorted and charged order, with block sums for its mean.
 template <typename R>
 __global__ __launch_bounds__(256) void k_vjp_gather(Dev<R> d, VjpDev<R> v) {
+  if (blockIdx.x == 0 && threadIdx.x < 3 * kVjpMaxTypes) v.pacc[threadIdx.x] = 0.0;
   double s[3] = {0, 0, 0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
        i += gridDim.x * blockDim.x) {
@@ -2273,18 +2255,18 @@ __global__ __launch_bounds__(256) void k_vjp_mean(const double* gpart, int nb,
     for (int q = 0; q < 3; ++q) gmean[q] = mean_fix ? s[q] / n : 0.0;
 }
 
-// One owner pair of the adjoint pair pass.  xi/xj positions (the caller
-// provides d and u), gi/gj the mean-shifted cotangents, row the LJ row (null
-// beyond rc_LJ), qq the charge product, coul whether Coulomb applies.
+// One owner pair of the adjoint pair pass.  The caller provides d and u;
+// gi/gj are the mean-shifted cotangents, row the LJ row (null beyond rc_LJ),
+// coul whether Coulomb applies, qj the neighbour's charge.
 template <typename R>
 __device__ __forceinline__ void vjp_pair(R dx, R dy, R dz, R u, const R* row,
                                          bool coul, const PairConst<R>& pc,
                                          R qi, R qj, R wij, R wji,
-                                         const R gi[3], const R gj[3], int tj,
-                                         R r[3], const VjpSlots& sl) {
-  R f = 0, fp = 0, fe = 0, fs = 0;
-  if (row) lj_vjp<R>(row, u, f, fp, fe, fs);
-  R gqb = 0;
+                                         const R gi[3], const R gj[3], R r[3],
+                                         VjpAcc& acc) {
+  R flj = 0, fp = 0, fs = 0;
+  if (row) lj_vjp<R>(row, u, flj, fp, fs);
+  R f = flj, gqb = 0;
   if (coul) {
     R h, hp;
     coul_vjp<R>(pc, u, h, hp);
@@ -2298,17 +2280,11 @@ __device__ __forceinline__ void vjp_pair(R dx, R dy, R dz, R u, const R* row,
   r[0] += f * hx + R(2) * fp * hd * dx;
   r[1] += f * hy + R(2) * fp * hd * dy;
   r[2] += f * hz + R(2) * fp * hd * dz;
-  if (wij != R(0)) {
-    const R gd = wij * (gi[0] * dx + gi[1] * dy + gi[2] * dz);
-    if (row) {
-      sl.at(0, tj) += (float)(fe * gd);
-      sl.at(1, tj) += (float)(fs * gd);
-    }
-    if (coul) {
-      sl.at(2, tj) += (float)(gqb * gd);
-      sl.at(3, tj) += (float)(gqb * gd * qj);
-    }
+  if (row) {
+    acc.de -= (float)(flj * hd);
+    acc.ds -= (float)(R(0.5) * fs * hd);
   }
+  if (coul) acc.dq -= (float)(qj * gqb * hd);
 }
 
 // Adjoint pair pass, one workgroup per brick (the forward's lists and stage;
@@ -2326,13 +2302,12 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
   extern __shared__ char dyn_lds[];
   const int nt = pc.ntypes;
   R* s_lj = reinterpret_cast<R*>(dyn_lds);
-  VjpSlots sl;
-  sl.s = reinterpret_cast<float*>(dyn_lds + 4 * sizeof(R) * nt * nt);
-  sl.tab = tab;
-  sl.nt = nt;
-  load_lj<R>(lj_tab, nt, s_lj);
+  // the LJ table with 1/sigma_ab in place of sigma_ab
+  for (int q = threadIdx.x; q < 4 * nt * nt; q += blockDim.x) {
+    const R x = lj_tab[q];
+    s_lj[q] = (q & 3) == 3 ? (x > R(0) ? R(1) / x : R(0)) : x;
+  }
   for (int q = threadIdx.x; q < 3 * nt; q += blockDim.x) tab[q] = 0.f;
-  sl.zero();
   const R gm[3] = {(R)v.gmean[0], (R)v.gmean[1], (R)v.gmean[2]};
   const int e1[3] = {1, 1, 1};
   const int brick = xcd_brick(blockIdx.x, g.nbrick);
@@ -2369,28 +2344,41 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
     const unsigned long long hji = RBL ? d.hj_s[i] : 0ull;
     const R* lrow = s_lj + 4 * ti * nt;
     R r[3] = {0, 0, 0};
+    VjpAcc acc;
     const int cnt = d.cnt_a[i];
-    auto one = [&](R dx, R dy, R dz, R u, int tj, const R gj[3], int gidj,
-                   unsigned long long hjj) {
+    // the hashes enter only for shell pairs (core pairs have weight 1 both
+    // ways), so they are formed there alone
+    auto one = [&](R dx, R dy, R dz, R u, int tj, const R gj[3], int gidj) {
       if (!(u < pc.rclj2) || !(u < pc.rc2)) return;
-      const R wij = vjp_w<R, RBL>(pc, u, hi, hjj);
-      const R wji = RBL ? vjp_w<R, RBL>(pc, u, rbl_hi(gidj), hji) : wij;
+      R wij, wji;
+      if (!RBL || u < pc.t_core) {
+        wij = wji = vjp_w<R, RBL>(pc, u, 0ull, 0ull);
+      } else {
+        wij = vjp_w<R, RBL>(pc, u, hi, rbl_hj(gidj));
+        wji = vjp_w<R, RBL>(pc, u, rbl_hi(gidj), hji);
+      }
       if (wij == R(0) && wji == R(0)) return;
       vjp_pair<R>(dx, dy, dz, u, lrow + 4 * tj, false, pc, R(0), R(0), wij, wji, gi,
-                  gj, tj, r, sl);
+                  gj, r, acc);
     };
     if (staged && cnt <= d.cap_a) {
-      for (int k = 0; k < cnt; ++k) {
-        const int l = (int)d.list_a[list_a_index(k, i, d.n)] >> sh;
-        const R4T<R> xj = stage[l];
-        const R4T<R> gq = gstage[l];
-        const int gidj = reinterpret_cast<const int*>(&gq.w)[0];
-        const R gj[3] = {gq.x, gq.y, gq.z};
-        R dx, dy, dz;
-        const R u = h.mi ? pair_d2<R, true>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz)
-                         : pair_d2<R, false>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
-        const int tj = pair_woff(xj.w) / (4 * (int)sizeof(R));
-        one(dx, dy, dz, u, tj, gj, gidj, RBL ? rbl_hj(gidj) : 0ull);
+      const uint4* lp = reinterpret_cast<const uint4*>(d.list_a) + i;
+      for (int k0 = 0; k0 < cnt; k0 += 8) {
+        const uint4 q8 = lp[(size_t)(k0 >> 3) * d.n];
+        const unsigned wv[4] = {q8.x, q8.y, q8.z, q8.w};
+        const int kn = min(8, cnt - k0);
+        for (int kk = 0; kk < kn; ++kk) {
+          const int l = (int)((wv[kk >> 1] >> (16 * (kk & 1))) & 0xffffu) >> sh;
+          const R4T<R> xj = stage[l];
+          const R4T<R> gq = gstage[l];
+          const int gidj = reinterpret_cast<const int*>(&gq.w)[0];
+          const R gj[3] = {gq.x, gq.y, gq.z};
+          R dx, dy, dz;
+          const R u = h.mi ? pair_d2<R, true>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz)
+                           : pair_d2<R, false>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+          const int tj = pair_woff(xj.w) / (4 * (int)sizeof(R));
+          one(dx, dy, dz, u, tj, gj, gidj);
+        }
       }
     } else {
       // unstaged brick or list overflow: scan the 27 cells directly
@@ -2413,12 +2401,12 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
               const R gj[3] = {gq.x - gm[0], gq.y - gm[1], gq.z - gm[2]};
               R dx, dy, dz;
               const R u = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
-              one(dx, dy, dz, u, (int)xj.w, gj, d.gid_s[jg], RBL ? d.hj_s[jg] : 0ull);
+              one(dx, dy, dz, u, (int)xj.w, gj, d.gid_s[jg]);
             }
           }
     }
     v.drs[i] = make4<R>(r[0], r[1], r[2], R(0));
-    sl.flush<R>(ti, R(0), s_lj);
+    vjp_flush<R>(tab, nt, ti, s_lj, acc);
   }
   // charged atoms over list B (LJ + Coulomb), kChargedLanes lanes per atom
   if (split) {
@@ -2435,19 +2423,25 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
       const unsigned long long hji = RBL ? d.hjc[c] : 0ull;
       const R* lrow = s_lj + 4 * ti * nt;
       R r[3] = {0, 0, 0};
+      VjpAcc acc;
       auto one = [&](int e) {
         const R4T<R> xj = d.xqc[e];
         R dx, dy, dz;
         const R u = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
         if (!(u < pc.rc2)) return;
-        const R wij = vjp_w<R, RBL>(pc, u, hi, RBL ? d.hjc[e] : 0ull);
-        const R wji = RBL ? vjp_w<R, RBL>(pc, u, rbl_hi(d.gidc[e]), hji) : wij;
+        R wij, wji;
+        if (!RBL || u < pc.t_core) {
+          wij = wji = vjp_w<R, RBL>(pc, u, 0ull, 0ull);
+        } else {
+          wij = vjp_w<R, RBL>(pc, u, hi, d.hjc[e]);
+          wji = vjp_w<R, RBL>(pc, u, rbl_hi(d.gidc[e]), hji);
+        }
         if (wij == R(0) && wji == R(0)) return;
         const R4T<R> gq = v.gc[e];
         const R gj[3] = {gq.x - gm[0], gq.y - gm[1], gq.z - gm[2]};
         const int tj = d.typec[e];
         vjp_pair<R>(dx, dy, dz, u, u < pc.rclj2 ? lrow + 4 * tj : nullptr, true, pc,
-                    xi.w, xj.w, wij, wji, gi, gj, tj, r, sl);
+                    xi.w, xj.w, wij, wji, gi, gj, r, acc);
       };
       const int cnt = d.cnt_b[c];
       if (cnt <= d.cap_b) {
@@ -2475,6 +2469,9 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
         fx += __shfl_xor(fx, o, kChargedLanes);
         fy += __shfl_xor(fy, o, kChargedLanes);
         fz += __shfl_xor(fz, o, kChargedLanes);
+        acc.dq += __shfl_xor(acc.dq, o, kChargedLanes);
+        acc.de += __shfl_xor(acc.de, o, kChargedLanes);
+        acc.ds += __shfl_xor(acc.ds, o, kChargedLanes);
       }
       if (gl == 0) {
         const int i = d.c2s[c];
@@ -2483,217 +2480,335 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
         q.y += fy;
         q.z += fz;
         v.drs[i] = q;
+        vjp_flush<R>(tab, nt, ti, s_lj, acc);
       }
-      sl.flush<R>(ti, xi.w, s_lj);
     }
   }
   __syncthreads();
   for (int q = threadIdx.x; q < 3 * nt; q += blockDim.x)
-    v.cls[(size_t)brick * 3 * nt + q] = tab[q];
+    if (tab[q] != 0.f) atomicAdd(&v.pacc[q], (double)tab[q]);
 }
 
 // Charge gradient through Coulomb pairs of a charged and a neutral atom (the
-// forward evaluates them without Coulomb, their charge product being 0):
-// each charged atom scans the neutral atoms of the cells within ext_b,
-// kChargedLanes lanes per atom; only Q of the neutral atom's type moves.
+// forward evaluates them without Coulomb, their charge product being 0): only
+// Q of the neutral atom's type moves.  One workgroup per brick: the charged
+// atoms of the brick's cells widened by the charged-pair stencil (ext_b) are
+// staged in LDS (shifted across box edges as stage_atoms does), and each
+// neutral atom of the brick, kQncLanes lanes per atom, scans the staged rows
+// of cells within ext_b of its own cell.  A neighbourhood beyond the stage
+// reads the same atoms from global memory.
+constexpr int kQncSlots = 512;   // neighbourhood cells per brick
+constexpr int kQncCap = 1024;    // staged charged atoms
+constexpr int kQncLanes = 2;     // lanes per neutral atom
+
 template <typename R, bool RBL>
 __global__ __launch_bounds__(256) void k_vjp_qnc(Dev<R> d, VjpDev<R> v, Box<R> b,
-                                                 Grid g, PairConst<R> pc_arg,
-                                                 int row0) {
+                                                 Grid g, PairConst<R> pc_arg) {
   PairConst<R> pc = pc_arg;
   if (RBL) pc.epoch = d.ctrl->epoch;
+  __shared__ SlotTables<kQncSlots> t;
+  __shared__ R4T<R> sx[kQncCap];  // (x, y, z, q), edge-shifted
+  __shared__ R4T<R> sg[kQncCap];  // (G' x, y, z, global id bits)
   __shared__ float tab[kVjpMaxTypes];
   const int nt = pc.ntypes;
   for (int q = threadIdx.x; q < nt; q += blockDim.x) tab[q] = 0.f;
-  __syncthreads();
   const R gm[3] = {(R)v.gmean[0], (R)v.gmean[1], (R)v.gmean[2]};
-  const int grp = threadIdx.x / kChargedLanes, gl = threadIdx.x % kChargedLanes;
-  const int c = blockIdx.x * (blockDim.x / kChargedLanes) + grp;
-  float acc[kVjpMaxTypes];
-  for (int q = 0; q < kVjpMaxTypes; ++q) acc[q] = 0.f;
-  if (c < d.nc) {
-    const int i = d.c2s[c];
-    const R4T<R> xi = d.xqc[c];
-    const R4T<R> g4 = v.gc[c];
-    const R gi[3] = {g4.x - gm[0], g4.y - gm[1], g4.z - gm[2]};
-    const unsigned long long hi = RBL ? rbl_hi(d.gidc[c]) : 0ull;
-    const unsigned long long hji = RBL ? d.hjc[c] : 0ull;
-    int cc[3];
-    unpack_coord(g.coord_of_cell[d.cell_s[i]], cc);
-    const Span sx = make_span(cc[0], g.ext_b[0], g.nc[0]);
-    const Span sy = make_span(cc[1], g.ext_b[1], g.nc[1]);
-    const Span sz = make_span(cc[2], g.ext_b[2], g.nc[2]);
-    const int nst = sx.count * sy.count * sz.count;
-    for (int s = gl; s < nst; s += kChargedLanes) {
-      const int kx = s % sx.count, ky = (s / sx.count) % sy.count,
-                kz = s / (sx.count * sy.count);
-      const int cell =
-          g.cell_of_coord[(sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0] + sx.at(kx)];
-      const int js = d.key_start[2 * cell + 1], je = d.key_start[2 * cell + 2];
-      for (int j = js; j < je; ++j) {
-        const R4T<R> xj = d.xs[j];
-        R dx, dy, dz;
-        const R u = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
-        if (!(u < pc.rc2)) continue;
-        const R wcn = vjp_w<R, RBL>(pc, u, hi, RBL ? d.hj_s[j] : 0ull);
-        const R wnc = RBL ? vjp_w<R, RBL>(pc, u, rbl_hi(d.gid_s[j]), hji) : wcn;
-        if (wcn == R(0) && wnc == R(0)) continue;
-        const R4T<R> gq = v.gs[j];
-        const R gj[3] = {gq.x - gm[0], gq.y - gm[1], gq.z - gm[2]};
-        R hh, hp;
-        coul_vjp<R>(pc, u, hh, hp);
-        const R gqb = -pc.qqr2e * hh;
-        const R gid_ = gi[0] * dx + gi[1] * dy + gi[2] * dz;
-        const R gjd = gj[0] * dx + gj[1] * dy + gj[2] * dz;
-        const int tj = (int)xj.w;
-        const float val = (float)(xi.w * gqb * (wcn * gid_ - wnc * gjd));
-        for (int q = 0; q < kVjpMaxTypes; ++q)
-          if (q == tj) acc[q] += val;
+  const int brick = xcd_brick(blockIdx.x, g.nbrick);
+  const Neigh h = brick_neigh(g, brick, g.ext_b);
+  const bool fits = h.nslots <= kQncSlots;
+  int total = 0;
+  if (fits) {
+    build_slots(g, h, d.key_start, d.cstart, true, t);
+    total = t.off[h.nslots];
+  }
+  const bool staged = fits && total <= kQncCap;
+  if (staged) {
+    const int lane = threadIdx.x & 31;
+    for (int s = threadIdx.x >> 5; s < h.nslots; s += blockDim.x >> 5) {
+      const int o = t.off[s], len = t.off[s + 1] - o, cf = t.cfirst[s];
+      const int e = t.edge[s];
+      R sh[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int w = (e >> (2 * q)) & 3;
+        sh[q] = w == 0 ? -b.L[q] : (w == 2 ? b.L[q] : R(0));
+      }
+      for (int j = lane; j < len; j += 32) {
+        R4T<R> p = d.xqc[cf + j];
+        p.x += sh[0];
+        p.y += sh[1];
+        p.z += sh[2];
+        sx[o + j] = p;
+        const R4T<R> gq = v.gc[cf + j];
+        R4T<R> w;
+        w.x = gq.x - gm[0];
+        w.y = gq.y - gm[1];
+        w.z = gq.z - gm[2];
+        w.w = (R)0;
+        reinterpret_cast<int*>(&w.w)[0] = d.gidc[cf + j];
+        sg[o + j] = w;
       }
     }
   }
-  for (int q = 0; q < nt; ++q)
-    if (acc[q] != 0.f) atomicAdd(&tab[q], acc[q]);
   __syncthreads();
-  for (int q = threadIdx.x; q < 3 * nt; q += blockDim.x)
-    v.cls[(size_t)(row0 + blockIdx.x) * 3 * nt + q] = q < nt ? tab[q] : 0.f;
-}
-
-// k-vectors per block of the adjoint structure sums (8 sums per k).
-constexpr int kVjpTile = 4;
-
-// Per-type structure sums over all atoms (type-major chunks, caller order):
-// per chunk and k, (T, U, rho, beta) with T = sum e^{i theta},
-// U = sum (G . k) e^{i theta}, rho = sum q e^{i theta}, beta = sum q (G . k)
-// e^{i theta}.
-template <typename R>
-__global__ __launch_bounds__(256) void k_vjp_sk(Dev<R> d, VjpDev<R> v, Box<R> b,
-                                                const int* __restrict__ m, int K) {
-  __shared__ double smem[16 * 8 * kVjpTile];
-  const int ch = blockIdx.x, k0 = blockIdx.y * kVjpTile;
-  const int kn = min(kVjpTile, K - k0);
-  const double two_pi = 6.283185307179586476925;
-  float mx[kVjpTile], my[kVjpTile], mz[kVjpTile], kxv[kVjpTile], kyv[kVjpTile], kzv[kVjpTile];
+  const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
+  const int sub = threadIdx.x % kQncLanes;
+  for (int ib = i0; ib < i1; ib += blockDim.x / kQncLanes) {
+    const int i = ib + threadIdx.x / kQncLanes;
+    float acc = 0.f;
+    int ti = 0;
+    if (i < i1 && d.s2c[i] < 0) {
+      const R4T<R> xi = d.xs[i];
+      ti = (int)xi.w;
+      const R4T<R> g4 = v.gs[i];
+      const R gi[3] = {g4.x - gm[0], g4.y - gm[1], g4.z - gm[2]};
+      const unsigned long long hi = RBL ? rbl_hi(d.gid_s[i]) : 0ull;
+      const unsigned long long hji = RBL ? d.hj_s[i] : 0ull;
+      // (owner i neutral, j charged): m = w_ij G'_i . d - w_ji G'_j . d
+      auto one = [&](const R4T<R>& xj, const R gj[3], int gidj, bool mi) {
+        R dx, dy, dz;
+        const R u = mi ? pair_d2<R, true>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz)
+                       : pair_d2<R, false>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+        if (!(u < pc.rc2)) return;
+        R wij, wji;
+        if (!RBL || u < pc.t_core) {
+          wij = wji = vjp_w<R, RBL>(pc, u, 0ull, 0ull);
+        } else {
+          wij = vjp_w<R, RBL>(pc, u, hi, rbl_hj(gidj));
+          wji = vjp_w<R, RBL>(pc, u, rbl_hi(gidj), hji);
+        }
+        if (wij == R(0) && wji == R(0)) return;
+        R hh, hp;
+        coul_vjp<R>(pc, u, hh, hp);
+        const R m = wij * (gi[0] * dx + gi[1] * dy + gi[2] * dz) -
+                    wji * (gj[0] * dx + gj[1] * dy + gj[2] * dz);
+        acc += (float)(xj.w * (-pc.qqr2e * hh) * m);
+      };
+      if (staged && !h.mi) {
+        // rows of cells within ext_b of the owner's cell: contiguous along x
+        int c[3];
+        unpack_coord(g.coord_of_cell[d.cell_s[i]], c);
+        int k0[3];
 #pragma unroll
-  for (int kk = 0; kk < kVjpTile; ++kk) {
-    const int k = k0 + (kk < kn ? kk : 0);
-    mx[kk] = (float)m[3 * k];
-    my[kk] = (float)m[3 * k + 1];
-    mz[kk] = (float)m[3 * k + 2];
-    kxv[kk] = (float)(two_pi * m[3 * k] / (double)b.L[0]);
-    kyv[kk] = (float)(two_pi * m[3 * k + 1] / (double)b.L[1]);
-    kzv[kk] = (float)(two_pi * m[3 * k + 2] / (double)b.L[2]);
-  }
-  float acc[8 * kVjpTile];
-#pragma unroll
-  for (int q = 0; q < 8 * kVjpTile; ++q) acc[q] = 0.f;
-  const int a0 = v.ch_beg[ch], a1 = v.ch_beg[ch + 1];
-  for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
-    const int i = v.tord[a];
-    const float sx = (float)((d.px[i] - b.lo[0]) * b.invL[0]);
-    const float sy = (float)((d.py[i] - b.lo[1]) * b.invL[1]);
-    const float sz = (float)((d.pz[i] - b.lo[2]) * b.invL[2]);
-    const float gx = (float)v.gx[i], gy = (float)v.gy[i], gz = (float)v.gz[i];
-    const float q = (float)d.charge[i];
-#pragma unroll
-    for (int kk = 0; kk < kVjpTile; ++kk) {
-      const float t = fma_r(mx[kk], sx, fma_r(my[kk], sy, mz[kk] * sz));
-      float sn, cs;
-      sincos_rev(t, sn, cs);
-      const float gk = fma_r(gx, kxv[kk], fma_r(gy, kyv[kk], gz * kzv[kk]));
-      float* A = acc + 8 * kk;
-      A[0] += cs;
-      A[1] += sn;
-      A[2] = fma_r(gk, cs, A[2]);
-      A[3] = fma_r(gk, sn, A[3]);
-      A[4] = fma_r(q, cs, A[4]);
-      A[5] = fma_r(q, sn, A[5]);
-      A[6] = fma_r(q * gk, cs, A[6]);
-      A[7] = fma_r(q * gk, sn, A[7]);
+        for (int q = 0; q < 3; ++q) k0[q] = wrapi(c[q] - h.start[q], g.nc[q]) - g.ext_b[q];
+        const int nx = 2 * g.ext_b[0] + 1;
+        for (int kz = 0; kz <= 2 * g.ext_b[2]; ++kz)
+          for (int ky = 0; ky <= 2 * g.ext_b[1]; ++ky) {
+            const int s0 = ((k0[2] + kz) * h.cnt[1] + k0[1] + ky) * h.cnt[0] + k0[0];
+            const int l1 = t.off[s0 + nx];
+            for (int l = t.off[s0] + sub; l < l1; l += kQncLanes) {
+              const R4T<R> gq = sg[l];
+              const R gj[3] = {gq.x, gq.y, gq.z};
+              one(sx[l], gj, reinterpret_cast<const int*>(&gq.w)[0], false);
+            }
+          }
+      } else if (staged) {
+        for (int l = sub; l < total; l += kQncLanes) {
+          const R4T<R> gq = sg[l];
+          const R gj[3] = {gq.x, gq.y, gq.z};
+          one(sx[l], gj, reinterpret_cast<const int*>(&gq.w)[0], true);
+        }
+      } else {
+        // the stencil of the owner's cell from global memory, minimum image
+        int cc[3];
+        unpack_coord(g.coord_of_cell[d.cell_s[i]], cc);
+        const Span spx = make_span(cc[0], g.ext_b[0], g.nc[0]);
+        const Span spy = make_span(cc[1], g.ext_b[1], g.nc[1]);
+        const Span spz = make_span(cc[2], g.ext_b[2], g.nc[2]);
+        const int nst = spx.count * spy.count * spz.count;
+        for (int s = sub; s < nst; s += kQncLanes) {
+          const int kx = s % spx.count, ky = (s / spx.count) % spy.count,
+                    kz = s / (spx.count * spy.count);
+          const int cell = g.cell_of_coord[(spz.at(kz) * g.nc[1] + spy.at(ky)) * g.nc[0] +
+                                           spx.at(kx)];
+          for (int e = d.cstart[cell]; e < d.cstart[cell + 1]; ++e) {
+            const R4T<R> gq = v.gc[e];
+            const R gj[3] = {gq.x - gm[0], gq.y - gm[1], gq.z - gm[2]};
+            one(d.xqc[e], gj, d.gidc[e], true);
+          }
+        }
+      }
     }
+    for (int o = kQncLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kQncLanes);
+    if (sub == 0 && acc != 0.f) atomicAdd(&tab[ti], acc);
   }
-  double dv[8 * kVjpTile];
+  __syncthreads();
+  for (int q = threadIdx.x; q < nt; q += blockDim.x)
+    if (tab[q] != 0.f) atomicAdd(&v.pacc[q], (double)tab[q]);
+}
+
+// Adjoint structure sums: one k-vector per thread, atoms staged through LDS.
+constexpr int kVjpSkBlock = 128;  // k-vectors per block
+constexpr int kVjpSkTile = 256;   // atoms per LDS tile
+
+// Per chunk (type-major runs of at most kSkChunk atoms of one type, the
+// charged atoms' chunks first) and k: (T, U, rho, beta) with
+// T = sum e^{i theta}, U = sum (G . k) e^{i theta}, rho = sum q e^{i theta},
+// beta = sum q (G . k) e^{i theta}; a neutral chunk has rho = beta = 0.
+// Float sums over a tile, FP64 across tiles.
+template <typename R>
+__global__ __launch_bounds__(kVjpSkBlock) void k_vjp_sk(Dev<R> d, VjpDev<R> v, Box<R> b,
+                                                        const int* __restrict__ m, int K) {
+  __shared__ float4 sa[kVjpSkTile];  // (s_x, s_y, s_z, q)
+  __shared__ float4 sg[kVjpSkTile];  // (G_x, G_y, G_z, 0)
+  const int ch = blockIdx.x;
+  const int k = blockIdx.y * kVjpSkBlock + threadIdx.x;
+  const int kk = k < K ? k : K - 1;
+  const double two_pi = 6.283185307179586476925;
+  const float mx = (float)m[3 * kk], my = (float)m[3 * kk + 1], mz = (float)m[3 * kk + 2];
+  const float kx = (float)(two_pi * m[3 * kk] / (double)b.L[0]);
+  const float ky = (float)(two_pi * m[3 * kk + 1] / (double)b.L[1]);
+  const float kz = (float)(two_pi * m[3 * kk + 2] / (double)b.L[2]);
+  const bool charged = ch < v.nch_c;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int a0 = v.ch_beg[ch], a1 = v.ch_beg[ch + 1];
+  for (int t0 = a0; t0 < a1; t0 += kVjpSkTile) {
+    const int nt = min(kVjpSkTile, a1 - t0);
+    __syncthreads();
+    for (int j = threadIdx.x; j < nt; j += blockDim.x) {
+      const int i = v.tord[t0 + j];
+      sa[j] = make_float4((float)((d.px[i] - b.lo[0]) * b.invL[0]),
+                          (float)((d.py[i] - b.lo[1]) * b.invL[1]),
+                          (float)((d.pz[i] - b.lo[2]) * b.invL[2]), (float)d.charge[i]);
+      sg[j] = make_float4((float)v.gx[i], (float)v.gy[i], (float)v.gz[i], 0.f);
+    }
+    __syncthreads();
+    float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (charged) {
+      for (int j = 0; j < nt; ++j) {
+        const float4 a = sa[j], g = sg[j];
+        float sn, cs;
+        sincos_rev(fma_r(mx, a.x, fma_r(my, a.y, mz * a.z)), sn, cs);
+        const float gk = fma_r(g.x, kx, fma_r(g.y, ky, g.z * kz));
+        const float qg = a.w * gk;
+        f[0] += cs;
+        f[1] += sn;
+        f[2] = fma_r(gk, cs, f[2]);
+        f[3] = fma_r(gk, sn, f[3]);
+        f[4] = fma_r(a.w, cs, f[4]);
+        f[5] = fma_r(a.w, sn, f[5]);
+        f[6] = fma_r(qg, cs, f[6]);
+        f[7] = fma_r(qg, sn, f[7]);
+      }
+    } else {
+      for (int j = 0; j < nt; ++j) {
+        const float4 a = sa[j], g = sg[j];
+        float sn, cs;
+        sincos_rev(fma_r(mx, a.x, fma_r(my, a.y, mz * a.z)), sn, cs);
+        const float gk = fma_r(g.x, kx, fma_r(g.y, ky, g.z * kz));
+        f[0] += cs;
+        f[1] += sn;
+        f[2] = fma_r(gk, cs, f[2]);
+        f[3] = fma_r(gk, sn, f[3]);
+      }
+    }
 #pragma unroll
-  for (int q = 0; q < 8 * kVjpTile; ++q) dv[q] = (double)acc[q];
-  block_sum<8 * kVjpTile>(dv, smem);
-  if (threadIdx.x == 0) {
+    for (int q = 0; q < 8; ++q) acc[q] += (double)f[q];
+  }
+  if (k < K) {
+    double* o = v.vsk + ((size_t)ch * K + k) * 8;
 #pragma unroll
-    for (int kk = 0; kk < kVjpTile; ++kk)
-      if (kk < kn)
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-          v.vsk[((size_t)ch * K + k0 + kk) * 8 + q] = dv[8 * kk + q];
+    for (int q = 0; q < 8; ++q) o[q] = acc[q];
   }
 }
 
-// FP64 sums over chunks per k: the k-space charge gradient per type and the
-// per-k coefficients of the position pass.  One block.
+// One block per k: FP64 sums of the chunk partials per group (the chunks of
+// one type, charged or neutral), then the k-space charge gradient's terms of
+// this k per type and the k's coefficients of the position pass.
 template <typename R>
-__global__ __launch_bounds__(1024) void k_vjp_sk_fin(Dev<R> d, VjpDev<R> v,
-                                                     Box<R> b, const int* m, int K,
-                                                     Params p, int nt) {
+__global__ __launch_bounds__(256) void k_vjp_sk_red(Dev<R> d, VjpDev<R> v, Box<R> b,
+                                                    const int* m, int K, Params p,
+                                                    int ngrp) {
+  __shared__ double gsum[2 * kVjpMaxTypes][8];
+  __shared__ double smem[16 * 8];
+  const int k = blockIdx.x;
+  for (int g = 0; g < ngrp; ++g) {
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int ch = v.grp_beg[g] + threadIdx.x; ch < v.grp_beg[g + 1]; ch += blockDim.x) {
+      const double* x = v.vsk + ((size_t)ch * K + k) * 8;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s[q] += x[q];
+    }
+    block_sum<8>(s, smem);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) gsum[g][q] = s[q];
+  }
+  if (threadIdx.x != 0) return;
+  double rr = 0, ri = 0, br = 0, bi = 0;
+  for (int g = 0; g < ngrp; ++g) {
+    rr += gsum[g][4];
+    ri += gsum[g][5];
+    br += gsum[g][6];
+    bi += gsum[g][7];
+  }
   const double two_pi = 6.283185307179586476925;
+  const double kx = two_pi * m[3 * k] / (double)b.L[0];
+  const double ky = two_pi * m[3 * k + 1] / (double)b.L[1];
+  const double kz = two_pi * m[3 * k + 2] / (double)b.L[2];
+  const double k2 = kx * kx + ky * ky + kz * kz;
+  const double A = d.kw[k] * (-4.0 * 3.141592653589793238463 * p.qqr2e) / (p.volume * k2);
   double dq[kVjpMaxTypes];
   for (int a = 0; a < kVjpMaxTypes; ++a) dq[a] = 0.0;
-  for (int k = threadIdx.x; k < K; k += blockDim.x) {
-    double rr = 0, ri = 0, br = 0, bi = 0;
-    for (int ch = 0; ch < v.nch; ++ch) {
-      const double* s = v.vsk + ((size_t)ch * K + k) * 8;
-      rr += s[4];
-      ri += s[5];
-      br += s[6];
-      bi += s[7];
-    }
-    const double kx = two_pi * m[3 * k] / (double)b.L[0];
-    const double ky = two_pi * m[3 * k + 1] / (double)b.L[1];
-    const double kz = two_pi * m[3 * k + 2] / (double)b.L[2];
-    const double k2 = kx * kx + ky * ky + kz * kz;
-    const double A = d.kw[k] * (-4.0 * 3.141592653589793238463 * p.qqr2e) / (p.volume * k2);
-    for (int ch = 0; ch < v.nch; ++ch) {
-      const double* s = v.vsk + ((size_t)ch * K + k) * 8;
-      const int a = v.ch_type[ch];
-      const double c = A * ((s[2] * ri - s[3] * rr) + (br * s[1] - bi * s[0]));
-      for (int q = 0; q < kVjpMaxTypes; ++q)
-        if (q == a) dq[q] += c;
-    }
-    v.kv0[k] = make_float4((float)m[3 * k], (float)m[3 * k + 1], (float)m[3 * k + 2],
-                           (float)(A * br));
-    v.kv1[k] = make_float4((float)kx, (float)ky, (float)kz, (float)(A * bi));
-    v.kv2[k] = make_float2((float)(A * rr), (float)(A * ri));
+  for (int g = 0; g < ngrp; ++g) {
+    const double* s = gsum[g];
+    const double c = A * ((s[2] * ri - s[3] * rr) + (br * s[1] - bi * s[0]));
+    const int a = v.grp_type[g];
+    for (int q = 0; q < kVjpMaxTypes; ++q)
+      if (q == a) dq[q] += c;
   }
-  __shared__ double smem[16 * kVjpMaxTypes];
-  block_sum<kVjpMaxTypes>(dq, smem);
-  if (threadIdx.x == 0)
-    for (int a = 0; a < nt; ++a) v.dqk[a] = dq[a];
+  for (int q = 0; q < kVjpMaxTypes; ++q) v.dqk[(size_t)k * kVjpMaxTypes + q] = dq[q];
+  v.kv0[k] = make_float4((float)m[3 * k], (float)m[3 * k + 1], (float)m[3 * k + 2],
+                         (float)(A * br));
+  v.kv1[k] = make_float4((float)kx, (float)ky, (float)kz, (float)(A * bi));
+  v.kv2[k] = make_float2((float)(A * rr), (float)(A * ri));
 }
 
-// k-space part of dL/dr for the charged atoms (charged order).
+// k-space part of dL/dr for the charged atoms (charged order); the per-k
+// coefficients pass through LDS in tiles of kVjpKTile.
+constexpr int kVjpKTile = 256;
 template <typename R>
 __global__ __launch_bounds__(256) void k_vjp_kforce(Dev<R> d, VjpDev<R> v,
                                                     Box<R> b, int K) {
+  __shared__ float4 s0[kVjpKTile], s1[kVjpKTile];
+  __shared__ float2 s2[kVjpKTile];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= d.nc) return;
-  const R4T<R> p = d.xqc[c];
-  float sx, sy, sz;
-  frac_coords<R>(b, p, sx, sy, sz);
-  const R4T<R> g4 = v.gc[c];
-  const float gx = (float)g4.x, gy = (float)g4.y, gz = (float)g4.z;
-  float rx = 0.f, ry = 0.f, rz = 0.f;
-  for (int k = 0; k < K; ++k) {
-    const float4 a = v.kv0[k];
-    const float4 e = v.kv1[k];
-    const float2 rh = v.kv2[k];
-    const float t = fma_r(a.x, sx, fma_r(a.y, sy, a.z * sz));
-    float sn, cs;
-    sincos_rev(t, sn, cs);
-    const float gk = fma_r(gx, e.x, fma_r(gy, e.y, gz * e.z));
-    const float s = fma_r(a.w, cs, e.w * sn) - gk * fma_r(rh.x, cs, rh.y * sn);
-    rx = fma_r(s, e.x, rx);
-    ry = fma_r(s, e.y, ry);
-    rz = fma_r(s, e.z, rz);
+  const bool live = c < d.nc;
+  float sx = 0.f, sy = 0.f, sz = 0.f, gx = 0.f, gy = 0.f, gz = 0.f, q = 0.f;
+  if (live) {
+    const R4T<R> p = d.xqc[c];
+    frac_coords<R>(b, p, sx, sy, sz);
+    const R4T<R> g4 = v.gc[c];
+    gx = (float)g4.x;
+    gy = (float)g4.y;
+    gz = (float)g4.z;
+    q = (float)p.w;
   }
-  const float q = (float)p.w;
-  v.drk[c] = make4<R>((R)(q * rx), (R)(q * ry), (R)(q * rz), R(0));
+  float rx = 0.f, ry = 0.f, rz = 0.f;
+  for (int k0 = 0; k0 < K; k0 += kVjpKTile) {
+    const int kn = min(kVjpKTile, K - k0);
+    __syncthreads();
+    for (int j = threadIdx.x; j < kn; j += blockDim.x) {
+      s0[j] = v.kv0[k0 + j];
+      s1[j] = v.kv1[k0 + j];
+      s2[j] = v.kv2[k0 + j];
+    }
+    __syncthreads();
+    for (int k = 0; k < kn; ++k) {
+      const float4 a = s0[k];
+      const float4 e = s1[k];
+      const float2 rh = s2[k];
+      const float t = fma_r(a.x, sx, fma_r(a.y, sy, a.z * sz));
+      float sn, cs;
+      sincos_rev(t, sn, cs);
+      const float gk = fma_r(gx, e.x, fma_r(gy, e.y, gz * e.z));
+      const float s = fma_r(a.w, cs, e.w * sn) - gk * fma_r(rh.x, cs, rh.y * sn);
+      rx = fma_r(s, e.x, rx);
+      ry = fma_r(s, e.y, ry);
+      rz = fma_r(s, e.z, rz);
+    }
+  }
+  if (live) v.drk[c] = make4<R>((R)(q * rx), (R)(q * ry), (R)(q * rz), R(0));
 }
 
 // dL/dr in caller order; keeps the evaluation counter in step with the
@@ -2720,19 +2835,23 @@ __global__ __launch_bounds__(256) void k_vjp_assemble(Dev<R> d, VjpDev<R> v, int
   v.dz[i] = r.z;
 }
 
-// Parameter gradients: FP64 sums of the per-block tables in a fixed order,
-// plus the k-space charge gradient.  dparam = [dL/dQ | dL/deps | dL/dsigma].
+// Parameter gradients: the pair passes' FP64 per-type sums (each workgroup
+// adds its LDS table with FP64 atomics) plus the k-space charge terms per k,
+// summed in a fixed order.  dparam = [dL/dQ | dL/deps | dL/dsigma].
 template <typename R>
-__global__ __launch_bounds__(256) void k_vjp_params(VjpDev<R> v, int nt, int has_k) {
-  __shared__ double smem[16 * 3 * kVjpMaxTypes];
-  double s[3 * kVjpMaxTypes];
-  for (int q = 0; q < 3 * kVjpMaxTypes; ++q) s[q] = 0.0;
-  for (int row = threadIdx.x; row < v.ncls; row += blockDim.x)
-    for (int q = 0; q < 3 * nt; ++q) s[q] += (double)v.cls[(size_t)row * 3 * nt + q];
-  block_sum<3 * kVjpMaxTypes>(s, smem);
+__global__ __launch_bounds__(256) void k_vjp_params(VjpDev<R> v, int nt, int K) {
+  __shared__ double smem[16 * kVjpMaxTypes];
+  double s[kVjpMaxTypes];
+#pragma unroll
+  for (int q = 0; q < kVjpMaxTypes; ++q) s[q] = 0.0;
+  for (int k = threadIdx.x; k < K; k += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < kVjpMaxTypes; ++q) s[q] += v.dqk[(size_t)k * kVjpMaxTypes + q];
+  block_sum<kVjpMaxTypes>(s, smem);
   if (threadIdx.x == 0)
-    for (int q = 0; q < 3 * nt; ++q)
-      v.dparam[q] = s[q] + ((has_k && q < nt) ? v.dqk[q] : 0.0);
+#pragma unroll
+    for (int q = 0; q < 3 * kVjpMaxTypes; ++q)
+      if (q < 3 * nt) v.dparam[q] = v.pacc[q] + (q < nt ? s[q % kVjpMaxTypes] : 0.0);
 }
 
 // The rebuild part of an evaluation alone: skin check and gather, and the
@@ -2753,34 +2872,37 @@ hipError_t launch_vjp(const Dev<R>& d, const VjpDev<R>& v, const StepArgs<R>& a,
   hipLaunchKernelGGL((k_vjp_gather<R>), dim3(v.gather_blocks), dim3(kBlock), 0, s, d, v);
   hipLaunchKernelGGL(k_vjp_mean, dim3(1), dim3(256), 0, s, v.gpart, v.gather_blocks,
                      (double)d.n, p.mean_fix, v.gmean);
-  const size_t lds = 4 * sizeof(R) * (size_t)nt * nt + 4 * sizeof(float) * (size_t)nt * g.block;
+  const size_t lds = 4 * sizeof(R) * (size_t)nt * nt;
   if (p.rbl)
     hipLaunchKernelGGL((k_pair_vjp<R, true>), dim3(g.nbrick), dim3(g.block), lds, s, d, v,
                        a.box, g, a.pc, a.lj_tab, p.split);
   else
     hipLaunchKernelGGL((k_pair_vjp<R, false>), dim3(g.nbrick), dim3(g.block), lds, s, d, v,
                        a.box, g, a.pc, a.lj_tab, p.split);
-  if (v.nq_blocks > 0) {
+  if (v.neutral_q && v.nq_blocks > 0) {
     if (p.rbl)
-      hipLaunchKernelGGL((k_vjp_qnc<R, true>), dim3(v.nq_blocks), dim3(kBlock), 0, s, d, v,
-                         a.box, g, a.pc, g.nbrick);
+      hipLaunchKernelGGL((k_vjp_qnc<R, true>), dim3(g.nbrick), dim3(256), 0, s, d, v,
+                         a.box, g, a.pc);
     else
-      hipLaunchKernelGGL((k_vjp_qnc<R, false>), dim3(v.nq_blocks), dim3(kBlock), 0, s, d, v,
-                         a.box, g, a.pc, g.nbrick);
+      hipLaunchKernelGGL((k_vjp_qnc<R, false>), dim3(g.nbrick), dim3(256), 0, s, d, v,
+                         a.box, g, a.pc);
   }
   const int K = p.kspace ? p.num_k : 0;
   const bool has_k = K > 0 && d.nc > 0;
+  // with the neutral atoms' charges held at zero, only the charged chunks
+  const int nch = v.neutral_q ? v.nch : v.nch_c;
+  const int ngrp = v.neutral_q ? v.ngrp : v.ngrp_c;
   if (has_k) {
-    hipLaunchKernelGGL((k_vjp_sk<R>), dim3(v.nch, (K + kVjpTile - 1) / kVjpTile),
-                       dim3(kBlock), 0, s, d, v, a.box, (const int*)d.m_dev, K);
-    hipLaunchKernelGGL((k_vjp_sk_fin<R>), dim3(1), dim3(1024), 0, s, d, v, a.box,
-                       (const int*)d.m_dev, K, p, nt);
+    hipLaunchKernelGGL((k_vjp_sk<R>), dim3(nch, (K + kVjpSkBlock - 1) / kVjpSkBlock),
+                       dim3(kVjpSkBlock), 0, s, d, v, a.box, (const int*)d.m_dev, K);
+    hipLaunchKernelGGL((k_vjp_sk_red<R>), dim3(K), dim3(256), 0, s, d, v, a.box,
+                       (const int*)d.m_dev, K, p, ngrp);
     hipLaunchKernelGGL((k_vjp_kforce<R>), dim3(blocks_for(d.nc)), dim3(kBlock), 0, s, d,
                        v, a.box, K);
   }
   hipLaunchKernelGGL((k_vjp_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0, s, d,
                      v, has_k ? 1 : 0);
-  hipLaunchKernelGGL((k_vjp_params<R>), dim3(1), dim3(256), 0, s, v, nt, has_k ? 1 : 0);
+  hipLaunchKernelGGL((k_vjp_params<R>), dim3(1), dim3(256), 0, s, v, nt, has_k ? K : 0);
   return hipGetLastError();
 }
 

@@ -111,6 +111,22 @@ def electrolyte(n=200000, rho=0.8, frac_ion=0.05, d_min=0.85, seed=2, temperatur
                   [1.0, 1.0, 1.0], vel=vel, name=f"electrolyte_{n}")
 
 
+def replicate(s, reps):
+    """The system tiled reps = (nx, ny, nz) times along the box edges, ids
+    offset per copy (the read_data "replicate" option of RBMD's input,
+    docs/json-configuration.en.md): periodic images of a valid box stay valid."""
+    reps = np.asarray(reps, dtype=int)
+    shifts = np.stack(np.meshgrid(*[np.arange(r) for r in reps], indexing="ij"),
+                      -1).reshape(-1, 3)
+    pos = np.concatenate([s.pos + sh * s.box_len for sh in shifts])
+    n = s.n
+    k = len(shifts)
+    vel = None if s.vel is None else np.tile(s.vel, (k, 1))
+    return System(pos, s.box_lo, s.box_len * reps, np.tile(s.types, k),
+                  np.concatenate([s.ids + c * n for c in range(k)]), np.tile(s.charges, k),
+                  s.eps, s.sigma, vel=vel, mass=s.mass, name=f"{s.name}_x{k}")
+
+
 def ljsalt_config(neighbor="RBL", coulomb="RBE", cut_off=3.5, lj_cut_off=2.5, r_core=2.5,
                   sample_num=30, skin=0.3, alpha=0.09, P=200, seed=20261202,
                   precision="fp32", kmax=None):

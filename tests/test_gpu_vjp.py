@@ -3,6 +3,10 @@ torch.autograd.Function over it) against the FP64 oracle adjoint
 (oracle/oracle.cpp force_adjoint, pinned by central finite differences in
 tests/test_oracle_vjp.py), and a finite difference of the GPU forward itself.
 
+dL/dQ of the neutral solvent type (charge 0) is the oracle's with
+neutral_q / neutral_charge_grad, and exactly 0 without (the charged types'
+values agree either way).
+
 Tolerances: dL/dr relative vector RMS against FP64 <= 1e-4; per-type
 gradients within 1e-4 of the largest of their kind (FP32 pair and phase terms,
 FP32 per-lane sums, FP64 block sums); the GPU finite difference of eps within
@@ -34,7 +38,7 @@ def _ctx(s, cfg):
     return ctx
 
 
-def _run(ctx, s, device, step, m, G):
+def _run(ctx, s, device, step, m, G, neutral_q=True):
     import torch
 
     p = torch.tensor(s.pos.astype(np.float32), device=device)
@@ -46,7 +50,7 @@ def _run(ctx, s, device, step, m, G):
     gx, gy, gz = (g[:, d].contiguous() for d in range(3))
     dr = [torch.zeros_like(px) for _ in range(3)]
     dp = torch.zeros(9, dtype=torch.float64, device=device)
-    ctx.force_vjp(step, px, py, pz, gx, gy, gz, *dr, dp, sync=True)
+    ctx.force_vjp(step, px, py, pz, gx, gy, gz, *dr, dp, sync=True, neutral_q=neutral_q)
     F = np.stack([t.double().cpu().numpy() for t in f], 1)
     return F, np.stack([t.double().cpu().numpy() for t in dr], 1), dp.cpu().numpy()
 
@@ -65,22 +69,28 @@ def _oracle(s, cfg, step, m, G):
                                    params, G, m, wk)
 
 
-@pytest.mark.parametrize("lj_cut", [2.5, 3.5])
-def test_vjp_matches_fp64_oracle(device, lj_cut):
+@pytest.mark.parametrize("lj_cut,neutral_q", [(2.5, True), (3.5, True), (2.5, False)])
+def test_vjp_matches_fp64_oracle(device, lj_cut, neutral_q):
     s = _system()
     cfg = systems.ljsalt_config(lj_cut_off=lj_cut, P=100)
     ctx = _ctx(s, cfg)
     m = rbe_sample_host(0.09, s.box_len, 100, 13, 0)
     G = np.random.default_rng(2).normal(size=(s.n, 3))
     step = 6
-    F, dr, dp = _run(ctx, s, device, step, m, G)
+    F, dr, dp = _run(ctx, s, device, step, m, G, neutral_q=neutral_q)
     Fo, (dro, dqo, deo, dso) = _oracle(s, cfg, step, m, G)
     assert rel_rms(F, Fo) < 2e-5
     err = rel_rms(dr, dro)
-    print(f"adjoint lj_cut {lj_cut}: dL/dr rel RMS vs FP64 = {err:.3e}")
+    print(f"adjoint lj_cut {lj_cut} neutral_q {neutral_q}: dL/dr rel RMS vs FP64 = {err:.3e}")
     print("dQ", dp[:3], dqo, "deps", dp[3:6], deo, "dsig", dp[6:], dso)
     assert err < 1e-4
-    for got, want in ((dp[:3], dqo), (dp[3:6], deo), (dp[6:], dso)):
+    tol_q = 1e-4 * np.abs(dqo).max()
+    if neutral_q:
+        np.testing.assert_allclose(dp[:3], dqo, rtol=0, atol=tol_q)
+    else:
+        np.testing.assert_allclose(dp[:2], dqo[:2], rtol=0, atol=tol_q)
+        assert dp[2] == 0.0
+    for got, want in ((dp[3:6], deo), (dp[6:], dso)):
         np.testing.assert_allclose(got, want, rtol=0, atol=1e-4 * np.abs(want).max())
 
 
@@ -120,7 +130,7 @@ def test_autograd_force_matching(device):
 
     s = _system(seed=14)
     cfg = systems.ljsalt_config(P=100)
-    model = DifferentiableForce(cfg, s)
+    model = DifferentiableForce(cfg, s, neutral_charge_grad=True)
     target = DifferentiableForce(cfg, s)
     m = rbe_sample_host(0.09, s.box_len, 100, 21, 0)
     mt = torch.tensor(m.reshape(-1).astype(np.int32), device=device)
@@ -141,3 +151,25 @@ def test_autograd_force_matching(device):
     assert rel_rms(pos.grad.double().cpu().numpy(), dro) < 1e-4
     for got, want in ((Q.grad, dqo), (eps.grad, deo), (sig.grad, dso)):
         np.testing.assert_allclose(got.numpy(), want, rtol=0, atol=1e-4 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("frac_ion", [0.05, 0.3])
+def test_vjp_neutral_charge_grad_large_box(device, frac_ion):
+    """40,000 atoms: the neutral-charged Coulomb pass stages each brick's
+    charged neighbourhood and scans rows of cells (5 % + 5 % ions), or reads
+    it from global memory when it exceeds the stage (30 % + 30 % ions)."""
+    s = systems.electrolyte(40000, frac_ion=frac_ion, seed=17)
+    s.eps = EPS.copy()
+    s.sigma = SIG.copy()
+    cfg = systems.ljsalt_config(P=50)
+    ctx = _ctx(s, cfg)
+    m = rbe_sample_host(0.09, s.box_len, 50, 29, 0)
+    G = np.random.default_rng(9).normal(size=(s.n, 3))
+    step = 3
+    F, dr, dp = _run(ctx, s, device, step, m, G, neutral_q=True)
+    Fo, (dro, dqo, deo, dso) = _oracle(s, cfg, step, m, G)
+    assert rel_rms(F, Fo) < 2e-5
+    assert rel_rms(dr, dro) < 1e-4
+    print("dQ", dp[:3], dqo)
+    for got, want in ((dp[:3], dqo), (dp[3:6], deo), (dp[6:], dso)):
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-4 * np.abs(want).max())
