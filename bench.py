@@ -63,10 +63,15 @@ def parse():
     ap.add_argument("--dt", type=float, default=0.002)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--workload", choices=["electrolyte", "adjoint"], default="electrolyte",
+    ap.add_argument("--workload", choices=["electrolyte", "adjoint", "replicas"],
+                    default="electrolyte",
                     help="electrolyte: the headline force step (default); adjoint: "
                          "forward and force-matching adjoint on the 2,000,000-particle "
-                         "electrolyte with P = 400 (prints its own JSON line)")
+                         "electrolyte with P = 400; replicas: 256 independent "
+                         "electrolytes of 500-8000 atoms, batched vs per replica (each "
+                         "prints its own JSON line)")
+    ap.add_argument("--tune-cache", default=None,
+                    help="replicas: JSON cache of the batch autotuner (default: measure)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB three times (a "
                          "plain copy kernel of known size, to scale HBM counters)")
@@ -145,12 +150,101 @@ def bench_adjoint(args):
     print(json.dumps(line))
 
 
+def bench_replicas(args):
+    """256 independent LJ electrolytes, N_r = round(exp(U[ln 500, ln 8000]))
+    (seed 4; composition, density and cutoffs of the 200k electrolyte), RBL +
+    RBE with P = 100 k-samples per replica and a fresh k-sample every step.
+    Times one force evaluation of every replica three ways: the replica batch
+    as one graph replay (launch shape from its autotuner), the same batch
+    kernels launched one replica at a time, and the single-system engine with
+    one context per replica (its own graph per replica)."""
+    import torch
+
+    from rbmd_amd import BatchContext, ForceContext, rbe_sample_host, systems
+
+    dev = torch.device("cuda:0")
+    reps = systems.replica_set(256, seed=4)
+    P = 100
+    cfg = systems.ljsalt_config(P=P)
+    n = sum(s.n for s in reps)
+    b = BatchContext(cfg, device=0)
+    b.set_systems(reps)
+    pos = torch.tensor(np.concatenate([s.pos for s in reps]).astype(np.float32), device=dev)
+    px, py, pz = (pos[:, d].contiguous() for d in range(3))
+    f = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    steps, warm = max(1, min(args.steps, 20)), max(1, min(args.warmup, 5))
+    # k-samples of every replica for each evaluation, drawn on the host beforehand
+    pool = []
+    for k in range(4):
+        ms = [rbe_sample_host(0.09, s.box_len, P, 100 + r, k) for r, s in enumerate(reps)]
+        pool.append(torch.tensor(np.concatenate(ms).reshape(-1).astype(np.int32), device=dev))
+    m = torch.empty_like(pool[0])
+    m.copy_(pool[0])
+    shape, tune_us = b.autotune(0, px, py, pz, *f, m_samples=m, cache_path=args.tune_cache)
+
+    def timed(fn):
+        for k in range(warm):
+            fn(k)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in range(steps):
+            fn(warm + k)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    def batched(k):
+        m.copy_(pool[k % 4])
+        b.force(k, px, py, pz, *f, m_samples=m)
+
+    def per_replica(k):
+        m.copy_(pool[k % 4])
+        b.force(k, px, py, pz, *f, m_samples=m, per_replica=True)
+
+    t_graph = timed(batched)
+    t_plain = timed(per_replica)
+    # the single-system engine, one context per replica (graph replay each)
+    ctxs, views = [], []
+    off = b.offsets
+    for r, s in enumerate(reps):
+        c = ForceContext(cfg, device=0)
+        c.set_system(s.box_lo, s.box_len, s.types, s.ids, s.charges, s.eps, s.sigma)
+        a, e = int(off[r]), int(off[r + 1])
+        ctxs.append(c)
+        views.append((px[a:e], py[a:e], pz[a:e], f[0][a:e], f[1][a:e], f[2][a:e],
+                      m[3 * P * r:3 * P * (r + 1)]))
+
+    def contexts(k):
+        m.copy_(pool[k % 4])
+        for c, v in zip(ctxs, views):
+            c.force(k, *v[:6], m_samples=v[6])
+
+    t_ctx = timed(contexts)
+    line = {
+        "metric": "batched replicas: speedup of one graph-replayed batch evaluation over "
+                  "per-replica launches (256 LJ-salt boxes, 500-8000 atoms, RBE P=100)",
+        "value": round(t_plain / t_graph, 2), "unit": "x", "n_gpus": 1, "steps": steps,
+        "warmup": warm, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "ms_batched_graph": round(t_graph, 4), "ms_per_replica_launches": round(t_plain, 4),
+        "ms_per_replica_contexts": round(t_ctx, 4),
+        "speedup_vs_contexts": round(t_ctx / t_graph, 2),
+        "atom_steps_per_s_batched": round(n / (t_graph * 1e-3), 1),
+        "tuned_shape": shape, "tune_us": round(tune_us, 2),
+        "config": {"workload": f"256 LJ electrolytes, {n} atoms (500-8000 each, seed 4), "
+                               "RBL + RBE, P=100 per replica", "atoms": n, "replicas": 256,
+                   "P": P},
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
-    if args.workload == "adjoint":
+    if args.workload in ("adjoint", "replicas"):
         if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
             sys.exit(2)
-        bench_adjoint(args)
+        (bench_adjoint if args.workload == "adjoint" else bench_replicas)(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); how the "

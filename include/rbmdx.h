@@ -64,8 +64,11 @@ enum {
   RBMDX_FLAG_EAGER = 16,   /* launch the kernels one by one instead of
                               replaying the context's captured hipGraph
                               (also RBMDX_GRAPH=0 in the environment) */
-  RBMDX_FLAG_VJP_NEUTRAL_Q = 32 /* rbmdx_force_vjp: let dL/dQ_t include the
+  RBMDX_FLAG_VJP_NEUTRAL_Q = 32, /* rbmdx_force_vjp: let dL/dQ_t include the
                               atoms of zero charge (see there) */
+  RBMDX_FLAG_PER_REPLICA = 64 /* rbmdx_batch_force: launch the stage kernels
+                              one replica at a time (eager) instead of the
+                              batch's captured graph */
 };
 
 /*
@@ -305,6 +308,68 @@ int rbmdx_kernel_times(rbmdx_ctx* ctx, const char** names, double* usec,
  * the k-space force, on the second stream when that runs beside the pair
  * kernel), "assemble" (the pair-sum reduction, the wait for k-space and the
  * assembly). */
+
+/*
+ * Replica batch (no counterpart in the reference, which runs one system per
+ * process): many independent periodic boxes of one force field evaluated
+ * together, FP32 positions and forces, FP64 energies and virials per
+ * replica.  The JSON surface is rbmdx_create's; the batch takes LJ/CUT or
+ * LJ/CUT/COUL/LONG, RBL or Verlet-List, RBE with a given coulomb.alpha,
+ * precision fp32, at most 8 types and 512 k-samples.  Pairs are found from
+ * cells rebuilt at every evaluation (neighbor.skin is not used); boxes with
+ * fewer than three cells per dimension are handled exactly.  RBL selection
+ * and its mean-force removal are per replica (frequency from the replica's
+ * size), keyed by the replica's own atom ids and the step.
+ */
+typedef struct rbmdx_batch rbmdx_batch;
+
+int rbmdx_batch_create(const char* json_text, int device, rbmdx_batch** out);
+void rbmdx_batch_destroy(rbmdx_batch* ctx);
+const char* rbmdx_batch_last_error(const rbmdx_batch* ctx);
+
+/* nrep replicas of natoms[r] atoms each; per-atom arrays are the replicas'
+ * atoms concatenated in replica order (that order is also the order of
+ * positions and forces); box_lo/box_len hold 3 doubles per replica; eps and
+ * sigma are per type and shared.  Once per context. */
+int rbmdx_batch_set_systems(rbmdx_batch* ctx, int32_t nrep, const int32_t* natoms,
+                            const double* box_lo, const double* box_len,
+                            const int32_t* type, const int32_t* id,
+                            const double* charge, int32_t ntypes,
+                            const double* eps, const double* sigma);
+
+/* One evaluation of every replica.  Device arrays; m_samples holds
+ * rbmdx_batch_num_samples() integer k-vectors (3 ints) per replica, replica
+ * after replica (RBE; may be null without Coulomb).  step is the RBL
+ * sampling epoch.  Flags: RBMDX_FLAG_ENERGY, RBMDX_FLAG_SYNC,
+ * RBMDX_FLAG_EAGER (the batch's kernels launched directly),
+ * RBMDX_FLAG_PER_REPLICA (the same kernels, one replica per launch).  By
+ * default the evaluation is one replay of a graph captured on first use for
+ * these pointers. */
+int rbmdx_batch_force(rbmdx_batch* ctx, int64_t step, const float* px,
+                      const float* py, const float* pz, float* fx, float* fy,
+                      float* fz, const int32_t* m_samples, uint32_t flags,
+                      void* stream);
+
+/* 9 doubles per replica: E_vdwl, E_coul, E_kspace (with the self term),
+ * virial xx yy zz xy xz yz (pair + k-space), of the last evaluation, which
+ * must have been made with RBMDX_FLAG_ENERGY.  Synchronises. */
+int rbmdx_batch_energies(rbmdx_batch* ctx, double* out);
+
+/* Launch shape (brick width in cells, threads per pair workgroup, persistent
+ * pair workgroups per CU, lanes per charged atom): measured on the caller's
+ * inputs by coordinate descent over {1, 2} x {64, 128, 256}, {1, 2, 4, 8},
+ * {4, 8, 16} and kept; with cache_path, a choice stored for this device and
+ * batch shape is reused without measuring and a new one is written there
+ * (JSON).  us_out (may be null) receives the chosen evaluation time in
+ * microseconds, 0 when it came from the cache.  The forces written are those
+ * of step. */
+int rbmdx_batch_autotune(rbmdx_batch* ctx, int64_t step, const float* px,
+                         const float* py, const float* pz, float* fx, float* fy,
+                         float* fz, const int32_t* m_samples,
+                         const char* cache_path, void* stream, double* us_out);
+int rbmdx_batch_set_shape(rbmdx_batch* ctx, const int32_t* shape /* [4] */);
+int rbmdx_batch_get_shape(rbmdx_batch* ctx, int32_t* shape /* [4] */);
+int32_t rbmdx_batch_num_samples(const rbmdx_batch* ctx);
 
 #ifdef __cplusplus
 }

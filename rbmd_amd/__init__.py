@@ -25,6 +25,7 @@ RBMDX_FLAG_REBUILD = 4
 RBMDX_FLAG_TIMING = 8
 RBMDX_FLAG_EAGER = 16
 RBMDX_FLAG_VJP_NEUTRAL_Q = 32
+RBMDX_FLAG_PER_REPLICA = 64
 
 _STATUS = {
     0: "RBMDX_OK",
@@ -121,6 +122,19 @@ SIGNATURES = {
                                             _DP]),
     "rbmdx_sog_sample_host": (ctypes.c_int, [_I32, ctypes.c_double, _DP, _I32, _I32,
                                              ctypes.c_uint32, _I64, _IP, _IP]),
+    "rbmdx_batch_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_P)]),
+    "rbmdx_batch_destroy": (None, [_P]),
+    "rbmdx_batch_last_error": (ctypes.c_char_p, [_P]),
+    "rbmdx_batch_set_systems": (ctypes.c_int, [_P, _I32, _IP, _DP, _DP, _IP, _IP, _DP, _I32,
+                                               _DP, _DP]),
+    "rbmdx_batch_force": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P,
+                                         ctypes.c_uint32, _P]),
+    "rbmdx_batch_energies": (ctypes.c_int, [_P, _DP]),
+    "rbmdx_batch_autotune": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P,
+                                            ctypes.c_char_p, _P, _DP]),
+    "rbmdx_batch_set_shape": (ctypes.c_int, [_P, _IP]),
+    "rbmdx_batch_get_shape": (ctypes.c_int, [_P, _IP]),
+    "rbmdx_batch_num_samples": (_I32, [_P]),
 }
 
 _lib = None
@@ -381,3 +395,106 @@ class ForceContext:
         if n < 0:
             self._check(-n)
         return {names[k].decode(): float(us[k]) for k in range(n)}
+
+
+class BatchContext:
+    """Many independent periodic boxes of one force field on one GPU (wraps an
+    ``rbmdx_batch``).  Positions and forces are the replicas' atoms
+    concatenated in replica order."""
+
+    def __init__(self, config, device=0):
+        self.lib = load_library()
+        text = config if isinstance(config, str) else json.dumps(config)
+        h = _P()
+        rc = self.lib.rbmdx_batch_create(text.encode(), int(device), ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self.lib.rbmdx_batch_last_error(h).decode() if h else ""
+            self.lib.rbmdx_batch_destroy(h)
+            self._h = None
+            raise RbmdxError(rc, msg)
+        self.device = device
+        self.nrep = 0
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.rbmdx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RbmdxError(rc, self.lib.rbmdx_batch_last_error(self._h).decode())
+
+    def set_systems(self, systems, eps=None, sigma=None):
+        """systems: objects with box_lo, box_len, types, ids, charges (and eps,
+        sigma per type, taken from the first unless given)."""
+        nat = np.array([len(s.types) for s in systems], dtype=np.int32)
+        lo = np.ascontiguousarray(np.concatenate([np.asarray(s.box_lo, dtype=np.float64)
+                                                  for s in systems]))
+        L = np.ascontiguousarray(np.concatenate([np.asarray(s.box_len, dtype=np.float64)
+                                                 for s in systems]))
+        types = np.ascontiguousarray(np.concatenate([s.types for s in systems]), dtype=np.int32)
+        ids = np.ascontiguousarray(np.concatenate([s.ids for s in systems]), dtype=np.int32)
+        q = np.ascontiguousarray(np.concatenate([s.charges for s in systems]), dtype=np.float64)
+        e = np.ascontiguousarray(systems[0].eps if eps is None else eps, dtype=np.float64)
+        sg = np.ascontiguousarray(systems[0].sigma if sigma is None else sigma,
+                                  dtype=np.float64)
+        self._check(self.lib.rbmdx_batch_set_systems(
+            self._h, len(systems), _iptr(nat), _dptr(lo), _dptr(L), _iptr(types), _iptr(ids),
+            _dptr(q), len(e), _dptr(e), _dptr(sg)))
+        self.nrep = len(systems)
+        self.n = int(nat.sum())
+        self.offsets = np.concatenate([[0], np.cumsum(nat)])
+
+    def num_samples(self):
+        return int(self.lib.rbmdx_batch_num_samples(self._h))
+
+    def force(self, step, px, py, pz, fx, fy, fz, m_samples=None, energy=False, sync=False,
+              eager=False, per_replica=False, stream=None):
+        flags = ((RBMDX_FLAG_ENERGY if energy else 0) | (RBMDX_FLAG_SYNC if sync else 0)
+                 | (RBMDX_FLAG_EAGER if eager else 0)
+                 | (RBMDX_FLAG_PER_REPLICA if per_replica else 0))
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(px.device).cuda_stream
+        self._check(self.lib.rbmdx_batch_force(
+            self._h, int(step), _tensor_ptr(px), _tensor_ptr(py), _tensor_ptr(pz),
+            _tensor_ptr(fx), _tensor_ptr(fy), _tensor_ptr(fz), _tensor_ptr(m_samples), flags,
+            ctypes.c_void_p(stream)))
+
+    def energies(self):
+        """(nrep, 9): E_vdwl, E_coul, E_kspace, virial xx yy zz xy xz yz."""
+        out = np.zeros(9 * self.nrep, dtype=np.float64)
+        self._check(self.lib.rbmdx_batch_energies(self._h, _dptr(out)))
+        return out.reshape(self.nrep, 9)
+
+    def autotune(self, step, px, py, pz, fx, fy, fz, m_samples=None, cache_path=None,
+                 stream=None):
+        """Measure and keep the fastest launch shape; returns (shape, us)."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(px.device).cuda_stream
+        us = ctypes.c_double(0.0)
+        self._check(self.lib.rbmdx_batch_autotune(
+            self._h, int(step), _tensor_ptr(px), _tensor_ptr(py), _tensor_ptr(pz),
+            _tensor_ptr(fx), _tensor_ptr(fy), _tensor_ptr(fz), _tensor_ptr(m_samples),
+            None if cache_path is None else str(cache_path).encode(), ctypes.c_void_p(stream),
+            ctypes.byref(us)))
+        return self.shape(), us.value
+
+    def shape(self):
+        s = np.zeros(4, dtype=np.int32)
+        self._check(self.lib.rbmdx_batch_get_shape(self._h, _iptr(s)))
+        return {"bw": int(s[0]), "pair_block": int(s[1]), "grid_per_cu": int(s[2]),
+                "qlanes": int(s[3])}
+
+    def set_shape(self, bw, pair_block, grid_per_cu, qlanes):
+        s = np.array([bw, pair_block, grid_per_cu, qlanes], dtype=np.int32)
+        self._check(self.lib.rbmdx_batch_set_shape(self._h, _iptr(s)))

@@ -44,6 +44,10 @@ def build(verbose=False):
     jobs = [
         [hipcc] + HIPCC_FLAGS + ["-c", "-o", os.path.join(BUILD, "kernels.o"),
                                  os.path.join(CSRC, "kernels.hip")],
+        [hipcc] + HIPCC_FLAGS + ["-c", "-o", os.path.join(BUILD, "batch_kernels.o"),
+                                 os.path.join(CSRC, "batch.hip")],
+        ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "batch.o"),
+                               os.path.join(CSRC, "batch.cpp")],
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "api.o"),
                                os.path.join(CSRC, "api.cpp")],
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "sampler.o"),
@@ -51,9 +55,10 @@ def build(verbose=False):
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "sog.o"),
                                os.path.join(CSRC, "sog.cpp")],
     ]
-    with ThreadPoolExecutor(max_workers=4) as ex:
+    with ThreadPoolExecutor(max_workers=6) as ex:
         list(ex.map(_run, jobs))
-    objs = [os.path.join(BUILD, f) for f in ("kernels.o", "api.o", "sampler.o", "sog.o")]
+    objs = [os.path.join(BUILD, f) for f in ("kernels.o", "batch_kernels.o", "api.o",
+                                             "batch.o", "sampler.o", "sog.o")]
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
          + ["-lpthread"])

@@ -6,6 +6,8 @@
                     uniform positions with overlap rejection (random sequential
                     addition with a minimum distance d_min) at rho* = 0.8.
 * ``ljsalt_config`` the RBMD JSON configuration used with ``electrolyte``.
+* ``replica_set``  independent electrolytes of log-uniformly drawn sizes (the
+                    replica batch's workload).
 * ``spce_water``    rigid SPC/E water (REAL units: Angstrom, kcal/mol) on a cubic
                     lattice with random orientations, and its CVFF special-bond
                     topology (1-2: O-H, 1-3: H-H).
@@ -109,6 +111,16 @@ def electrolyte(n=200000, rho=0.8, frac_ion=0.05, d_min=0.85, seed=2, temperatur
     vel -= vel.mean(axis=0)
     return System(pos, [0, 0, 0], [L, L, L], types, np.arange(n), q, [1.0, 1.0, 1.0],
                   [1.0, 1.0, 1.0], vel=vel, name=f"electrolyte_{n}")
+
+
+def replica_set(nrep=256, n_min=500, n_max=8000, seed=4, frac_ion=0.05):
+    """Independent LJ electrolytes (composition and density of ``electrolyte``)
+    with atom counts N_r = round(exp(U[ln n_min, ln n_max])) drawn from the
+    seed, each box placed with its own seed."""
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(math.log(n_min), math.log(n_max), nrep))).astype(int)
+    return [electrolyte(int(n), frac_ion=frac_ion, seed=int(1000 * seed + r))
+            for r, n in enumerate(sizes)]
 
 
 def replicate(s, reps):

@@ -169,3 +169,36 @@ def test_sog_sampler_stream_equals_oracle(level, P, seed):
             mo, io = oracle.sog_sample(seed, call, P, L, s)
             np.testing.assert_array_equal(m, mo)
             np.testing.assert_array_equal(idx, io)
+
+
+@pytest.mark.parametrize("edit,msg", [
+    (lambda hp: hp.update(precision="fp64"), "FP32"),
+    (lambda hp: hp["coulomb"].update(type="EWALD"), "RBE k-space only"),
+    (lambda hp: (hp["coulomb"].pop("alpha"), hp["coulomb"].update(accuracy=1e-4)),
+     "coulomb.alpha"),
+    (lambda hp: hp["coulomb"].update(coulomb_sample_num=600), "sample_num"),
+    (lambda hp: hp["force_field"].update(type="CVFF"), "not CVFF"),
+])
+def test_batch_config_validation(edit, msg):
+    """The replica batch rejects what it does not implement, before touching
+    a device."""
+    cfg = systems.ljsalt_config(P=100)
+    edit(cfg["hyper_parameters"])
+    lib = rbmd_amd.load_library()
+    h = ctypes.c_void_p()
+    import json
+
+    rc = lib.rbmdx_batch_create(json.dumps(cfg).encode(), 0, ctypes.byref(h))
+    try:
+        assert rc == 1, rc  # RBMDX_E_CONFIG
+        assert msg in lib.rbmdx_batch_last_error(h).decode()
+    finally:
+        lib.rbmdx_batch_destroy(h)
+
+
+def test_replica_set_sizes_are_log_uniform_and_seeded():
+    a = systems.replica_set(6, seed=3)
+    b = systems.replica_set(6, seed=3)
+    assert [s.n for s in a] == [s.n for s in b]
+    assert all(500 <= s.n <= 8000 for s in a)
+    assert np.array_equal(a[2].pos, b[2].pos)

@@ -1,0 +1,134 @@
+"""GPU parity of the replica batch (rbmdx_batch_*): every replica's forces
+against the FP64 oracle of that box alone (its own RBL frequency, k-sample and
+self term), energies and virials per replica, the batched graph against the
+eager and per-replica launches, positions outside the box, boxes with fewer
+than three cells per dimension, and the autotuner's JSON cache.
+
+Tolerances: forces relative vector RMS <= 2e-5 per replica (FP32 pair and
+phase terms against FP64; the single-system engine's bar); pair energies and
+virials relative 1e-5, k-space energy relative 1e-5 (FP64 S(k) sums of FP32
+phases).
+"""
+import json
+
+import numpy as np
+import pytest
+
+from rbmd_amd import BatchContext, rbe_sample_host, systems
+from tests.refs import reference_force, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [500, 650, 1200, 2600, 4100, 800]
+
+
+def _batch(cfg, sizes, seed=7):
+    reps = [systems.electrolyte(n, seed=seed + r) for r, n in enumerate(sizes)]
+    b = BatchContext(cfg, device=0)
+    b.set_systems(reps)
+    return b, reps
+
+
+def _tensors(device, reps, P, step, seed=11):
+    import torch
+
+    pos = np.concatenate([s.pos for s in reps]).astype(np.float32)
+    p = torch.tensor(pos, device=device)
+    px, py, pz = (p[:, d].contiguous() for d in range(3))
+    f = [torch.zeros(len(pos), dtype=torch.float32, device=device) for _ in range(3)]
+    ms = [rbe_sample_host(0.09, s.box_len, P, seed + r, step) for r, s in enumerate(reps)]
+    m = torch.tensor(np.concatenate(ms).reshape(-1).astype(np.int32), device=device)
+    return px, py, pz, f, ms, m
+
+
+def _forces(f):
+    return np.stack([t.double().cpu().numpy() for t in f], 1)
+
+
+@pytest.mark.parametrize("neighbor", ["RBL", "Verlet-List"])
+def test_batch_matches_fp64_oracle_per_replica(device, neighbor):
+    P = 100
+    cfg = systems.ljsalt_config(neighbor=neighbor, P=P)
+    b, reps = _batch(cfg, SIZES)
+    step = 5
+    px, py, pz, f, ms, m = _tensors(device, reps, P, step)
+    b.force(step, px, py, pz, *f, m_samples=m, energy=True, sync=True)
+    F = _forces(f)
+    E = b.energies()
+    off = b.offsets
+    for r, s in enumerate(reps):
+        Fo, en, vir, ek, vk = reference_force(s, s.pos.astype(np.float32), cfg, step, ms[r])
+        err = rel_rms(F[off[r]:off[r + 1]], Fo)
+        print(f"{neighbor} replica {r} N={s.n} L={s.box_len[0]:.2f}: "
+              f"rel RMS {err:.2e}")
+        assert err < 2e-5
+        np.testing.assert_allclose(E[r, 0], en[0], rtol=1e-5)
+        np.testing.assert_allclose(E[r, 1], en[1], rtol=1e-5)
+        np.testing.assert_allclose(E[r, 2], ek, rtol=1e-5)
+        w = np.asarray(vir) + np.asarray(vk)
+        np.testing.assert_allclose(E[r, 3:], w, rtol=1e-5, atol=1e-5 * np.abs(w).max())
+
+
+def test_batch_graph_eager_and_per_replica_agree(device):
+    P = 100
+    cfg = systems.ljsalt_config(P=P)
+    b, reps = _batch(cfg, SIZES, seed=21)
+    px, py, pz, f, ms, m = _tensors(device, reps, P, 3)
+    out = []
+    for kw in ({}, {"eager": True}, {"per_replica": True}, {}):
+        for t in f:
+            t.zero_()
+        b.force(3, px, py, pz, *f, m_samples=m, sync=True, **kw)
+        out.append(_forces(f))
+    for o in out[1:]:
+        np.testing.assert_allclose(o, out[0], rtol=0, atol=1e-6 * np.abs(out[0]).max())
+
+
+def test_batch_wraps_positions_outside_the_box(device):
+    """Atoms shifted by whole box lengths: the batch wraps them and matches the
+    oracle at the wrapped positions."""
+    import torch
+
+    P = 50
+    step = 2
+    cfg = systems.ljsalt_config(P=P)
+    b, reps = _batch(cfg, [900, 1500], seed=31)
+    _, _, _, f, ms, m = _tensors(device, reps, P, step)
+    rng = np.random.default_rng(1)
+    shifted = [(s.pos + rng.integers(-2, 3, size=(s.n, 3)) * s.box_len).astype(np.float32)
+               for s in reps]
+    p = torch.tensor(np.concatenate(shifted), device=device)
+    qx, qy, qz = (p[:, d].contiguous() for d in range(3))
+    b.force(step, qx, qy, qz, *f, m_samples=m, sync=True)
+    F = _forces(f)
+    off = b.offsets
+    for r, s in enumerate(reps):
+        L = s.box_len
+        wrapped = np.mod(shifted[r].astype(np.float64) - s.box_lo, L) + s.box_lo
+        Fo = reference_force(s, wrapped.astype(np.float32), cfg, step, ms[r])[0]
+        assert rel_rms(F[off[r]:off[r + 1]], Fo) < 2e-5
+
+
+def test_batch_autotune_cache_roundtrip(device, tmp_path):
+    P = 50
+    cfg = systems.ljsalt_config(P=P)
+    b, reps = _batch(cfg, [700, 1800, 3000], seed=41)
+    px, py, pz, f, ms, m = _tensors(device, reps, P, 1)
+    cache = tmp_path / "tune.json"
+    shape, us = b.autotune(1, px, py, pz, *f, m_samples=m, cache_path=cache)
+    assert us > 0
+    stored = json.loads(cache.read_text())
+    (key, val), = stored.items()
+    assert {k: val[k] for k in shape} == shape
+    F = _forces(f)
+    b2, _ = _batch(cfg, [700, 1800, 3000], seed=41)
+    shape2, us2 = b2.autotune(1, px, py, pz, *f, m_samples=m, cache_path=cache)
+    assert us2 == 0.0 and shape2 == shape
+    # the tuned shape changes launch geometry only
+    b2.force(1, px, py, pz, *f, m_samples=m, sync=True)
+    np.testing.assert_allclose(_forces(f), F, rtol=0, atol=1e-6 * np.abs(F).max())
+    for s in ({"bw": 1, "pair_block": 64, "grid_per_cu": 1, "qlanes": 4},
+              {"bw": 2, "pair_block": 256, "grid_per_cu": 8, "qlanes": 16}):
+        b2.set_shape(**s)
+        b2.force(1, px, py, pz, *f, m_samples=m, sync=True)
+        np.testing.assert_allclose(_forces(f), F, rtol=0, atol=1e-6 * np.abs(F).max())
