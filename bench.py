@@ -63,13 +63,14 @@ def parse():
     ap.add_argument("--dt", type=float, default=0.002)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--workload", choices=["electrolyte", "adjoint", "replicas"],
+    ap.add_argument("--workload", choices=["electrolyte", "adjoint", "replicas", "water"],
                     default="electrolyte",
                     help="electrolyte: the headline force step (default); adjoint: "
                          "forward and force-matching adjoint on the 2,000,000-particle "
                          "electrolyte with P = 400; replicas: 256 independent "
-                         "electrolytes of 500-8000 atoms, batched vs per replica (each "
-                         "prints its own JSON line)")
+                         "electrolytes of 500-8000 atoms, batched vs per replica; water: "
+                         "72,981-atom SPC/E, CVFF + RBL + RBSOG (each prints its own JSON "
+                         "line)")
     ap.add_argument("--tune-cache", default=None,
                     help="replicas: JSON cache of the batch autotuner (default: measure)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -238,13 +239,82 @@ def bench_replicas(args):
     print(json.dumps(line))
 
 
+def bench_water(args):
+    """72,981-atom rigid SPC/E water (24,327 molecules on a seeded lattice, random
+    orientations, 33.4 / nm^3), CVFF with special_bonds [0, 0, 0], RBL (r_core
+    6.5 A, 100 samples, rc 10 A, skin 2 A) and RBSOG level 1 with P = 500 (512):
+    force evaluations at fixed positions with the seeded k-sample stream and a
+    neighbour rebuild forced every 10th evaluation; the force error against
+    the FP64 oracle (pair + SOG k-space) and the oracle's CPU time for one
+    evaluation."""
+    import torch
+
+    import oracle
+    from rbmd_amd import ForceContext, systems
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refs import reference_force, rel_rms, cfg_qqr2e
+
+    dev = torch.device("cuda:0")
+    s = systems.spce_water(24327, seed=3)
+    cfg = systems.cvff_water_config(coulomb="RBSOG", alpha=0.0784, P=500, rbsog_level=1)
+    ctx = ForceContext(cfg, device=0)
+    ctx.set_system(s.box_lo, s.box_len, s.types, s.ids, s.charges, s.eps, s.sigma)
+    ctx.set_special(s.nspecial, s.special)
+    n = s.n
+    P = ctx.num_samples()
+    p = torch.tensor(s.pos.astype(np.float32), device=dev)
+    px, py, pz = (p[:, d].contiguous() for d in range(3))
+    f = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    steps, warm = max(1, min(args.steps, 200)), max(1, min(args.warmup, 20))
+    for k in range(warm):
+        ctx.force(k, px, py, pz, *f, rebuild=(k % 10 == 0))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(warm, warm + steps):
+        ctx.force(k, px, py, pz, *f, rebuild=(k % 10 == 0))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # force error at a fixed explicit sample
+    sset = oracle.sog_setup(1, 10.0, s.box_len)
+    m, idx = oracle.sog_sample(31, 0, P, s.box_len, sset)
+    mt = torch.tensor(np.concatenate([m.reshape(-1), idx]).astype(np.int32), device=dev)
+    ctx.force(1, px, py, pz, *f, m_samples=mt, num_k=P, sync=True)
+    got = np.stack([t.double().cpu().numpy() for t in f], 1)
+    pos32 = s.pos.astype(np.float32)
+    tc = time.perf_counter()
+    ref = reference_force(s, pos32, cfg, 1, None, with_kspace=False)[0]
+    fk, _, _ = oracle.sog_kspace_f64(pos32.astype(np.float64), s.charges, s.box_len, m, idx,
+                                     sset, qqr2e=cfg_qqr2e(cfg))
+    t_cpu = time.perf_counter() - tc
+    ferr = rel_rms(got, ref + fk)
+    th = ctx.thermo()
+    line = {
+        "metric": "atom-steps/s, CVFF SPC/E water, RBL + RBSOG, 72,981 atoms",
+        "value": round(n * steps / elapsed, 1), "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": warm, "ms_per_step": round(1e3 * elapsed / steps, 4),
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "force_rel_rms_vs_fp64": float(ferr),
+        "config": {"workload": "SPC/E water 24,327 molecules, CVFF special_bonds [0,0,0], "
+                               "RBL r_core 6.5 A / 100 samples, rc 10 A, RBSOG level 1 "
+                               "P=512, positions fixed, rebuild every 10th evaluation",
+                   "atoms": n, "P": P, "rbl_frequency": th["rbl_frequency"],
+                   "pairs_a": th["pairs_a"], "pairs_b": th["pairs_b"]},
+        "cpu_baseline": {"value": round(n / t_cpu, 1), "unit": "atom-steps/s", "cores": 1,
+                         "kind": "port",
+                         "sample": "one evaluation (pair forces with the cell search, SOG "
+                                   "k-space in FP64) of the same configuration"},
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
-    if args.workload in ("adjoint", "replicas"):
+    if args.workload in ("adjoint", "replicas", "water"):
         if args.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)
             sys.exit(2)
-        (bench_adjoint if args.workload == "adjoint" else bench_replicas)(args)
+        {"adjoint": bench_adjoint, "replicas": bench_replicas,
+         "water": bench_water}[args.workload](args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); how the "
