@@ -130,15 +130,22 @@ typedef struct rbmdx_thermo_t {
  * Create a context from an RBMD JSON configuration (the whole document, or
  * just its "hyper_parameters" object).  Recognised keys:
  *   init_configuration.read_data.unit   "LJ" | "REAL" | "METAL"
- *   hyper_parameters.force_field.type   "LJ/CUT" | "LJ/CUT/COUL/LONG"
+ *   hyper_parameters.force_field.type   "LJ/CUT" | "LJ/CUT/COUL/LONG" | "CVFF"
+ *       (CVFF: the non-bonded part with special-bond weights; topology from
+ *        rbmdx_set_special)
+ *   hyper_parameters.extend.special_bonds  [w12, w13, w14] (CVFF; default 1)
  *   hyper_parameters.neighbor.{type,cut_off,skin,r_core,neighbor_sample_num,
  *                              energy_rbl_flag}
- *       type: "RBL" | "Verlet-List"
+ *       type: "RBL" | "Verlet-List" | "VERLET-SOG"
  *   hyper_parameters.neighbor.lj_cut_off   (extension: separate LJ cutoff,
  *                                           default cut_off)
  *   hyper_parameters.coulomb.{type,alpha,accuracy,coulomb_sample_num,kmax,
- *                             energy_rbe_flag,random_seed}
- *       type: "RBE" | "EWALD"
+ *                             energy_rbe_flag,random_seed,rbsog_level}
+ *       type: "RBE" | "EWALD" | "RBSOG"; rbsog_level 0-4 selects the
+ *       reference's SOG presets (RBSOG, VERLET-SOG)
+ *   hyper_parameters.coulomb.rbsog_exact   (extension) SOG coefficients and
+ *                                           normalisation without the
+ *                                           reference's float defects
  *   hyper_parameters.precision             (extension) "fp32" | "fp64"
  * `device` is the HIP device ordinal.
  */
