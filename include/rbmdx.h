@@ -378,6 +378,37 @@ int rbmdx_batch_set_shape(rbmdx_batch* ctx, const int32_t* shape /* [4] */);
 int rbmdx_batch_get_shape(rbmdx_batch* ctx, int32_t* shape /* [4] */);
 int32_t rbmdx_batch_num_samples(const rbmdx_batch* ctx);
 
+/*
+ * LAMMPS data files (replaces the reader path of RBMD's input,
+ * src/reader/src/format/lammps/lammps_full_reader.cpp: ReadData :454-633,
+ * ParseAtom :1496-1563, ApplyReplicate :899-1000, and the special-bond
+ * lists of src/reader/src/mpi/special_bond_builder.cpp).  atom_style is
+ * "atomic", "charge" or "full" (init_configuration.read_data.atom_style);
+ * replicate (may be null) holds read_data.replicate [nx, ny, nz] (systems
+ * without bonds).  Ids, types and molecules become 0-based; atoms keep the
+ * file's order.  The special-bond topology follows LAMMPS' rules as RBMD
+ * builds it: 1-2 partners from the bonds, 1-3 through a common partner and
+ * held by an angle or dihedral, 1-4 held by a dihedral, each partner once.
+ * Errors (RBMDX_E_CONFIG) name the file and line.  Host-only; no device.
+ */
+typedef struct rbmdx_data rbmdx_data;
+int rbmdx_data_read(const char* path, const char* atom_style,
+                    const int32_t* replicate, rbmdx_data** out);
+void rbmdx_data_destroy(rbmdx_data* data);
+const char* rbmdx_data_last_error(const rbmdx_data* data);
+/* flags: 1 Pair Coeffs present, 2 Velocities present, 4 bonds present */
+int rbmdx_data_info(const rbmdx_data* data, int64_t* natoms, int32_t* ntypes,
+                    int32_t* maxspecial, int32_t* flags);
+/* Any output may be null.  pos, vel, image: 3 per atom (x y z); mass, eps,
+ * sigma: per type (0 where the file has none). */
+int rbmdx_data_get(const rbmdx_data* data, double* box_lo, double* box_len,
+                   int32_t* type, int32_t* id, double* charge, double* pos,
+                   double* vel, int32_t* image, double* mass, double* eps,
+                   double* sigma);
+/* nspecial: 3 per atom; special: maxspecial per atom (rbmdx_set_special's
+ * layout, unused entries -1). */
+int rbmdx_data_special(const rbmdx_data* data, int32_t* nspecial, int32_t* special);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,6 +135,14 @@ SIGNATURES = {
     "rbmdx_batch_set_shape": (ctypes.c_int, [_P, _IP]),
     "rbmdx_batch_get_shape": (ctypes.c_int, [_P, _IP]),
     "rbmdx_batch_num_samples": (_I32, [_P]),
+    "rbmdx_data_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _IP,
+                                       ctypes.POINTER(_P)]),
+    "rbmdx_data_destroy": (None, [_P]),
+    "rbmdx_data_last_error": (ctypes.c_char_p, [_P]),
+    "rbmdx_data_info": (ctypes.c_int, [_P, ctypes.POINTER(_I64), _IP, _IP, _IP]),
+    "rbmdx_data_get": (ctypes.c_int, [_P, _DP, _DP, _IP, _IP, _DP, _DP, _DP, _IP, _DP, _DP,
+                                      _DP]),
+    "rbmdx_data_special": (ctypes.c_int, [_P, _IP, _IP]),
 }
 
 _lib = None
@@ -498,3 +506,47 @@ class BatchContext:
     def set_shape(self, bw, pair_block, grid_per_cu, qlanes):
         s = np.array([bw, pair_block, grid_per_cu, qlanes], dtype=np.int32)
         self._check(self.lib.rbmdx_batch_set_shape(self._h, _iptr(s)))
+
+
+def read_data(path, atom_style="full", replicate=None):
+    """A LAMMPS data file as RBMD reads it (rbmdx_data_read): a
+    ``systems.System`` with 0-based ids and types, per-type eps / sigma from
+    Pair Coeffs and masses, velocities and image flags, and the special-bond
+    topology (``nspecial`` (N, 3), ``special`` (N, maxspecial)) built from the
+    bonds, angles and dihedrals."""
+    from . import systems
+
+    lib = load_library()
+    h = _P()
+    rep = None if replicate is None else np.ascontiguousarray(replicate, dtype=np.int32)
+    rc = lib.rbmdx_data_read(str(path).encode(), atom_style.encode(),
+                             None if rep is None else _iptr(rep), ctypes.byref(h))
+    try:
+        if rc != 0:
+            raise RbmdxError(rc, lib.rbmdx_data_last_error(h).decode() if h else "")
+        n, nt, ms, fl = ctypes.c_int64(0), ctypes.c_int32(0), ctypes.c_int32(0), \
+            ctypes.c_int32(0)
+        lib.rbmdx_data_info(h, ctypes.byref(n), ctypes.byref(nt), ctypes.byref(ms),
+                            ctypes.byref(fl))
+        n, nt, ms = n.value, nt.value, ms.value
+        lo, L = np.zeros(3), np.zeros(3)
+        types, ids = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+        q, pos, vel = np.zeros(n), np.zeros(3 * n), np.zeros(3 * n)
+        img = np.zeros(3 * n, dtype=np.int32)
+        mass, eps, sig = np.zeros(nt), np.zeros(nt), np.zeros(nt)
+        lib.rbmdx_data_get(h, _dptr(lo), _dptr(L), _iptr(types), _iptr(ids), _dptr(q),
+                           _dptr(pos), _dptr(vel), _iptr(img), _dptr(mass), _dptr(eps),
+                           _dptr(sig))
+        nsp = np.zeros(3 * n, dtype=np.int32)
+        sp = np.zeros(n * ms, dtype=np.int32)
+        lib.rbmdx_data_special(h, _iptr(nsp), _iptr(sp))
+    finally:
+        lib.rbmdx_data_destroy(h)
+    s = systems.Water(pos.reshape(n, 3), lo, L, types, ids, q, eps, sig,
+                      vel=vel.reshape(n, 3), mass=mass, name=os.path.basename(str(path)),
+                      nspecial=nsp.reshape(n, 3), special=sp.reshape(n, ms))
+    s.image = img.reshape(n, 3)
+    s.has_pair_coeffs = bool(fl.value & 1)
+    s.has_velocities = bool(fl.value & 2)
+    s.has_bonds = bool(fl.value & 4)
+    return s

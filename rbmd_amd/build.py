@@ -48,6 +48,10 @@ def build(verbose=False):
                                  os.path.join(CSRC, "batch.hip")],
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "batch.o"),
                                os.path.join(CSRC, "batch.cpp")],
+        ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "lammps_data.o"),
+                               os.path.join(CSRC, "lammps_data.cpp")],
+        ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "data_api.o"),
+                               os.path.join(CSRC, "data_api.cpp")],
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "api.o"),
                                os.path.join(CSRC, "api.cpp")],
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "sampler.o"),
@@ -55,10 +59,11 @@ def build(verbose=False):
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "sog.o"),
                                os.path.join(CSRC, "sog.cpp")],
     ]
-    with ThreadPoolExecutor(max_workers=6) as ex:
+    with ThreadPoolExecutor(max_workers=8) as ex:
         list(ex.map(_run, jobs))
     objs = [os.path.join(BUILD, f) for f in ("kernels.o", "batch_kernels.o", "api.o",
-                                             "batch.o", "sampler.o", "sog.o")]
+                                             "batch.o", "lammps_data.o", "data_api.o",
+                                             "sampler.o", "sog.o")]
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
          + ["-lpthread"])

@@ -46,6 +46,11 @@ struct DeviceData {
   double box_len[3] = {0, 0, 0};
   std::vector<int32_t> type, id;
   std::vector<double> charge, eps, sigma;
+  // CVFF special-bond topology (RBMD's d_nspecial / d_special / maxspecial):
+  // cumulative 1-2/1-3/1-4 counts, 3 per atom, and maxspecial partner ids per
+  // atom; maxspecial 0 when there is none
+  int32_t maxspecial = 0;
+  std::vector<int32_t> nspecial, special;
   void* stream = nullptr;  // hipStream_t
 };
 
@@ -73,6 +78,8 @@ class Force {
     check(rbmdx_set_system(ctx_, d.num_atoms, d.box_lo, d.box_len, d.type.data(),
                            d.id.data(), d.charge.data(),
                            (int32_t)d.eps.size(), d.eps.data(), d.sigma.data()));
+    if (d.maxspecial > 0)
+      check(rbmdx_set_special(ctx_, d.maxspecial, d.nspecial.data(), d.special.data()));
   }
 
   // One force evaluation into d_fx/d_fy/d_fz (Force::Execute).
@@ -127,6 +134,13 @@ class LJ : public Force {
   using Force::Force;
 };
 
+// "CVFF": the non-bonded part (LJ + Coulomb with special-bond weights from
+// extend.special_bonds and the topology in DeviceData).
+class CVFF : public Force {
+ public:
+  using Force::Force;
+};
+
 using ForceFactory =
     std::function<std::unique_ptr<Force>(std::string, DeviceData*, int)>;
 
@@ -139,6 +153,10 @@ inline const std::map<std::string, ForceFactory>& ForceRegistry() {
       {"LJ/CUT",
        [](std::string j, DeviceData* d, int dev) -> std::unique_ptr<Force> {
          return std::unique_ptr<Force>(new LJ(std::move(j), d, dev));
+       }},
+      {"CVFF",
+       [](std::string j, DeviceData* d, int dev) -> std::unique_ptr<Force> {
+         return std::unique_ptr<Force>(new CVFF(std::move(j), d, dev));
        }},
   };
   return reg;

@@ -269,5 +269,53 @@ def cvff_water_config(neighbor="RBL", coulomb="RBE", cut_off=10.0, r_core=6.5,
     }
 
 
+def write_lammps_data(s, path, atom_style="full", bonds=(), angles=(), dihedrals=(),
+                      image=None, title="written by rbmd_amd.systems"):
+    """Write a System as a LAMMPS data file (ids and types 1-based in the
+    file; bonds / angles / dihedrals as tuples of 0-based atom ids, one type
+    each).  Positions with %.17g, so a reader recovers them exactly."""
+    n, nt = len(s.types), len(s.eps)
+    out = [title, "", f"{n} atoms", f"{nt} atom types"]
+    for name, lst in (("bonds", bonds), ("angles", angles), ("dihedrals", dihedrals)):
+        if len(lst):
+            out += [f"{len(lst)} {name}", f"1 {name[:-1]} types"]
+    for d, ax in enumerate("xyz"):
+        out.append(f"{float(s.box_lo[d])!r} {float(s.box_lo[d] + s.box_len[d])!r} {ax}lo {ax}hi")
+    out += ["", "Masses", ""] + [f"{t + 1} {float(s.mass[t])!r}" for t in range(nt)]
+    out += ["", "Pair Coeffs # lj/cut/coul/long", ""]
+    out += [f"{t + 1} {float(s.eps[t])!r} {float(s.sigma[t])!r}" for t in range(nt)]
+    out += ["", f"Atoms # {atom_style}", ""]
+    for i in range(n):
+        x, y, z = ("%.17g" % v for v in s.pos[i])
+        im = "" if image is None else " %d %d %d" % tuple(image[i])
+        idx, t = int(s.ids[i]) + 1, int(s.types[i]) + 1
+        if atom_style == "atomic":
+            out.append(f"{idx} {t} {x} {y} {z}{im}")
+        elif atom_style == "charge":
+            out.append(f"{idx} {t} {float(s.charges[i])!r} {x} {y} {z}{im}")
+        else:
+            out.append(f"{idx} {i // 3 + 1} {t} {float(s.charges[i])!r} {x} {y} {z}{im}")
+    if s.vel is not None:
+        out += ["", "Velocities", ""]
+        out += [f"{int(s.ids[i]) + 1} " + " ".join(repr(float(v)) for v in s.vel[i])
+                for i in range(n)]
+    for name, lst in (("Bonds", bonds), ("Angles", angles), ("Dihedrals", dihedrals)):
+        if len(lst):
+            out += ["", name, ""]
+            out += [f"{k + 1} 1 " + " ".join(str(int(a) + 1) for a in tup)
+                    for k, tup in enumerate(lst)]
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def water_topology(n_mol):
+    """Bonds (O-H1, O-H2) and angles (H1-O-H2) of n_mol molecules laid out
+    O, H1, H2 (0-based atom ids)."""
+    base = 3 * np.arange(n_mol)
+    bonds = [(b, b + 1) for b in base] + [(b, b + 2) for b in base]
+    angles = [(b + 1, b, b + 2) for b in base]
+    return bonds, angles
+
+
 def dumps(cfg):
     return json.dumps(cfg)

@@ -1,0 +1,176 @@
+"""The LAMMPS data reader and special-bond builder (rbmdx_data_*), on the CPU.
+
+Pins: files written by systems.write_lammps_data read back exactly; the
+special-bond lists follow LAMMPS' rules as RBMD's builder applies them (1-2 from
+bonds; 1-3 through a common partner, kept only when an angle or dihedral holds
+the pair; 1-4 kept only when a dihedral holds it; each partner once, in 1-2,
+1-3, 1-4 order), checked on molecules whose lists follow from their graph
+(water, a chain, 3- and 5-rings, a branched centre); replicate of a charge-style
+box; and, where the reference tree is present, its own example data files
+(counts from their headers, the molecules' topology census).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import rbmd_amd
+from rbmd_amd import RbmdxError, systems
+
+REF = "/root/reference/examples"
+
+
+def _graph_system(n, pos=None):
+    pos = np.arange(3 * n, dtype=float).reshape(n, 3) * 0.7 + 1.0 if pos is None else pos
+    return systems.System(pos, [0, 0, 0], [30.0, 30.0, 30.0], np.zeros(n, dtype=np.int32),
+                          np.arange(n), np.zeros(n), [0.2], [1.5], mass=[12.0])
+
+
+def _lists(s):
+    out = []
+    for i in range(s.n):
+        n12, n13, n14 = s.nspecial[i]
+        row = list(s.special[i])
+        out.append((set(row[:n12]), set(row[n12:n13]), set(row[n13:n14])))
+    return out
+
+
+def test_water_roundtrip_is_exact(tmp_path):
+    s = systems.spce_water(60, seed=4)
+    bonds, angles = systems.water_topology(60)
+    p = tmp_path / "w.data"
+    systems.write_lammps_data(s, p, "full", bonds=bonds, angles=angles)
+    r = rbmd_amd.read_data(p, "full")
+    assert r.n == s.n and r.has_bonds and r.has_pair_coeffs
+    np.testing.assert_array_equal(r.pos, s.pos)
+    np.testing.assert_array_equal(r.types, s.types)
+    np.testing.assert_array_equal(r.ids, s.ids)
+    np.testing.assert_array_equal(r.charges, s.charges)
+    np.testing.assert_array_equal(r.box_len, s.box_len)
+    np.testing.assert_array_equal(r.eps, s.eps)
+    # the generator's analytic topology: O -> H1, H2 (1-2); H -> O (1-2), other H (1-3)
+    np.testing.assert_array_equal(r.nspecial, s.nspecial)
+    np.testing.assert_array_equal(r.special, s.special)
+
+
+def test_chain_with_dihedral_and_trims(tmp_path):
+    # 0-1-2-3-4 chain
+    s = _graph_system(5)
+    bonds = [(0, 1), (1, 2), (2, 3), (3, 4)]
+    angles = [(0, 1, 2), (1, 2, 3), (2, 3, 4)]
+    dihedrals = [(0, 1, 2, 3), (1, 2, 3, 4)]
+    p = tmp_path / "c.data"
+    systems.write_lammps_data(s, p, "full", bonds, angles, dihedrals)
+    L = _lists(rbmd_amd.read_data(p, "full"))
+    assert L[0] == ({1}, {2}, {3})
+    assert L[2] == ({1, 3}, {0, 4}, set())
+    assert L[4] == ({3}, {2}, {1})
+    # without dihedrals there are no 1-4 partners; without angles no 1-3
+    systems.write_lammps_data(s, p, "full", bonds, angles)
+    L = _lists(rbmd_amd.read_data(p, "full"))
+    assert L[0] == ({1}, {2}, set())
+    systems.write_lammps_data(s, p, "full", bonds)
+    L = _lists(rbmd_amd.read_data(p, "full"))
+    assert L[0] == ({1}, set(), set()) and L[2] == ({1, 3}, set(), set())
+    # an angle alone keeps only the 1-3 pair it holds
+    systems.write_lammps_data(s, p, "full", bonds, [(0, 1, 2)])
+    L = _lists(rbmd_amd.read_data(p, "full"))
+    assert L[0] == ({1}, {2}, set()) and L[1] == ({0, 2}, set(), set())
+    assert L[3] == ({2, 4}, set(), set())
+
+
+def test_rings_count_each_partner_once(tmp_path):
+    p = tmp_path / "r.data"
+    # triangle: every 1-3 candidate is already a 1-2 partner
+    s = _graph_system(3)
+    systems.write_lammps_data(s, p, "full", [(0, 1), (1, 2), (2, 0)],
+                              [(0, 1, 2), (1, 2, 0), (2, 0, 1)])
+    r = rbmd_amd.read_data(p, "full")
+    np.testing.assert_array_equal(r.nspecial, [[2, 2, 2]] * 3)
+    # 5-ring: two 1-2, two 1-3; the 1-4 candidates are all earlier partners
+    s = _graph_system(5)
+    b = [(k, (k + 1) % 5) for k in range(5)]
+    a = [((k - 1) % 5, k, (k + 1) % 5) for k in range(5)]
+    d = [(k, (k + 1) % 5, (k + 2) % 5, (k + 3) % 5) for k in range(5)]
+    systems.write_lammps_data(s, p, "full", b, a, d)
+    r = rbmd_amd.read_data(p, "full")
+    np.testing.assert_array_equal(r.nspecial, [[2, 4, 4]] * 5)
+    assert _lists(r)[0] == ({1, 4}, {2, 3}, set())
+
+
+def test_branched_centre(tmp_path):
+    # methane-like: C(0) bonded to H1..H4, all H-C-H angles
+    s = _graph_system(5)
+    b = [(0, k) for k in range(1, 5)]
+    a = [(i, 0, j) for i in range(1, 5) for j in range(i + 1, 5)]
+    p = tmp_path / "m.data"
+    systems.write_lammps_data(s, p, "full", b, a)
+    r = rbmd_amd.read_data(p, "full")
+    np.testing.assert_array_equal(r.nspecial, [[4, 4, 4]] + [[1, 4, 4]] * 4)
+    assert _lists(r)[1] == ({0}, {2, 3, 4}, set())
+
+
+def test_replicate_charge_style(tmp_path):
+    s = systems.electrolyte(300, seed=3)
+    img = np.random.default_rng(2).integers(-1, 2, size=(s.n, 3))
+    p = tmp_path / "e.data"
+    systems.write_lammps_data(s, p, "charge", image=img)
+    r1 = rbmd_amd.read_data(p, "charge")
+    np.testing.assert_array_equal(r1.image, img)
+    r = rbmd_amd.read_data(p, "charge", replicate=[2, 1, 3])
+    assert r.n == 6 * s.n
+    np.testing.assert_allclose(r.box_len, s.box_len * [2, 1, 3])
+    span = s.ids.max() + 1
+    for rep, (kx, ky, kz) in enumerate([(x, y, z) for z in range(3) for y in range(1)
+                                        for x in range(2)]):
+        blk = slice(rep * s.n, (rep + 1) * s.n)
+        np.testing.assert_array_equal(r.ids[blk], s.ids + rep * span)
+        np.testing.assert_array_equal(r.charges[blk], s.charges)
+        shift = np.array([kx, ky, kz]) * s.box_len
+        d = r.pos[blk] - (s.pos + shift)
+        np.testing.assert_allclose(d - np.round(d / r.box_len) * r.box_len, 0, atol=1e-9)
+        assert (r.pos[blk] >= 0).all() and (r.pos[blk] < r.box_len).all()
+    # a box with bonds is not replicated by this reader
+    w = systems.spce_water(8, seed=1)
+    bonds, angles = systems.water_topology(8)
+    systems.write_lammps_data(w, p, "full", bonds=bonds, angles=angles)
+    with pytest.raises(RbmdxError, match="replicate"):
+        rbmd_amd.read_data(p, "full", replicate=[2, 1, 1])
+
+
+def test_errors_name_file_and_line(tmp_path):
+    s = systems.spce_water(4, seed=1)
+    bonds, angles = systems.water_topology(4)
+    p = tmp_path / "bad.data"
+    systems.write_lammps_data(s, p, "full", bonds=bonds, angles=angles)
+    with pytest.raises(RbmdxError, match="atom_style"):
+        rbmd_amd.read_data(p, "charge")
+    lines = p.read_text().splitlines()
+    b0 = lines.index("Bonds") + 2
+    assert lines[b0] == "1 1 1 2"
+    lines[b0] = "1 1 1 99"  # a bond to an unknown atom
+    p.write_text("\n".join(lines) + "\n")
+    with pytest.raises(RbmdxError, match="unknown atom"):
+        rbmd_amd.read_data(p, "full")
+    lines[b0] = "1 1 1 2"
+    a0 = lines.index("Atoms # full") + 2
+    del lines[a0 + 5]  # one atom short
+    p.write_text("\n".join(lines) + "\n")
+    with pytest.raises(RbmdxError, match="bad.data"):
+        rbmd_amd.read_data(p, "full")
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference examples absent")
+def test_reference_example_data_files():
+    """The reference's own example inputs: counts from their headers and the
+    molecules' topology census (water O (2,2,2), H (1,2,2); methane C
+    (4,4,4), H (1,4,4))."""
+    r = rbmd_amd.read_data(os.path.join(REF, "02-water-methane", "H2O-CH4.data"), "full")
+    assert r.n == 16050 and len(r.eps) == 4
+    assert abs(r.charges.sum()) < 1e-9
+    census = {}
+    for t, ns in zip(r.types, map(tuple, r.nspecial)):
+        census.setdefault(int(t), set()).add(ns)
+    assert census == {0: {(2, 2, 2)}, 1: {(1, 2, 2)}, 2: {(4, 4, 4)}, 3: {(1, 4, 4)}}
+    p = rbmd_amd.read_data(os.path.join(REF, "01-peo", "peo+Li.data"), "full")
+    assert p.n == 26802 and len(p.eps) == 12 and p.special.shape[1] <= 64
