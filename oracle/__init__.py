@@ -58,6 +58,12 @@ def lib():
         L.orc_pair_force_special.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F, _F, _F, _I,
                                              _I, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int,
                                              _I, _I, _D, _D, _D, _D, _D, _D, _D]
+        L.orc_pair_force_sog.restype = None
+        L.orc_pair_force_sog.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F, _F, _F, _I, _I,
+                                         ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, _I, _I,
+                                         _D, _D, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_double, _D, _D, _D, _D, _D,
+                                         _D]
         L.orc_verlet_force_f64.restype = None
         L.orc_verlet_force_f64.argtypes = [ctypes.c_int, _D, _D, _D, _D, _I, ctypes.c_int,
                                            _D, _D, _D, ctypes.c_double, ctypes.c_double,
@@ -177,11 +183,13 @@ def rbl_frequency(n, L, r_core, rc, p):
     return lib().orc_rbl_frequency(n, Lp, r_core, rc, p)
 
 
-def pair_force(pos, q, types, gid, eps, sig, L, params, prec=0, special=None):
+def pair_force(pos, q, types, gid, eps, sig, L, params, prec=0, special=None,
+               sog_near=None):
     """Pair forces (N,3), energies (vdwl, coul), virial[6], mean[3] (FP64 out).
 
     special: optional CVFF topology (nspecial (N,3), special (N,maxspecial)
-    global ids, (w12, w13, w14))."""
+    global ids, (w12, w13, w14)).  sog_near: optional SOG set (sog_setup) whose
+    near field replaces erfc for Coulomb pairs."""
     n = len(pos)
     x, xp = _f(pos[:, 0])
     y, yp = _f(pos[:, 1])
@@ -201,7 +209,23 @@ def pair_force(pos, q, types, gid, eps, sig, L, params, prec=0, special=None):
     mean = np.zeros(3)
     outs = (fx.ctypes.data_as(_D), fy.ctypes.data_as(_D), fz.ctypes.data_as(_D),
             en.ctypes.data_as(_D), vir.ctypes.data_as(_D), mean.ctypes.data_as(_D))
-    if special is None:
+    if sog_near is not None:
+        s_ = sog_near
+        T, Tp = _d(sog_taylor(s_))
+        if special is None:
+            ns = np.zeros((n, 3), dtype=np.int32)
+            ids = np.zeros((n, 1), dtype=np.int32)
+            w = (1.0, 1.0, 1.0)
+        else:
+            ns, ids, w = special
+        nsa, nsp = _i(np.asarray(ns).reshape(-1))
+        ida, idp = _i(np.asarray(ids).reshape(-1))
+        wa, wp = _d(w)
+        ms = np.asarray(ids).shape[1] if special is not None else 0
+        lib().orc_pair_force_sog(prec, n, xp, yp, zp, qp, tp, gp, len(eps), ep, sp, Lp, ppp,
+                                 ms, nsp, idp, wp, Tp, s_["sigma"], s_["b"], s_["mmax"],
+                                 s_["w0"], *outs)
+    elif special is None:
         lib().orc_pair_force(prec, n, xp, yp, zp, qp, tp, gp, len(eps), ep, sp, Lp, ppp,
                              *outs)
     else:

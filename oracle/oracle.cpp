@@ -218,6 +218,21 @@ inline void coul_pair(T d2, T qiqj, T alpha, T qqr2e, T& f, T& e) {
   e = T(0.5) * qqr2e * qiqj * ec / r;
 }
 
+// SOG near field (LJCoulSOGInit / CoulCutForceUser, lj_cut_coul_kspace.cpp:
+// 651-711, lj_cut_coul_kspace_op.hip.cu:196-252) in place of erfc:
+// f = -qqr2e qi qj (1/r^3 + sum_n T_n r^{2n}), e = 0.5 qqr2e qi qj (1/r -
+// far(r)), far from the SOG set (sigma, b, mmax, w0).
+struct SogNearO {
+  bool on = false;
+  double T[6] = {0, 0, 0, 0, 0, 0};
+  double sigma = 0, b = 0, w0 = 0;
+  int mmax = 0;
+};
+
+}  // namespace
+extern "C" double orc_sog_far_r(double r, double sigma, double b, int mmax, double w0);
+namespace {
+
 // CVFF special-bond topology of the caller-order atoms: cumulative counts
 // (n12, n13, n14) and partner global ids, weights w12, w13, w14.
 struct Special {
@@ -256,7 +271,8 @@ void pair_force_f32pos(int n, const float* x, const float* y, const float* z,
                        int ntypes, const double* eps, const double* sig,
                        const double* L64, const double* pp, double* fx,
                        double* fy, double* fz, double* energy, double* virial,
-                       double* mean, const Special& sp = Special()) {
+                       double* mean, const Special& sp = Special(),
+                       const SogNearO& sn = SogNearO()) {
   const PairParams p = unpack(pp);
   const Thresholds t = thresholds(p);
   const float L[3] = {(float)L64[0], (float)L64[1], (float)L64[2]};
@@ -288,7 +304,21 @@ void pair_force_f32pos(int n, const float* x, const float* y, const float* z,
     T flj = 0, elj = 0, fc = 0, ec = 0;
     if (d2f < t.rclj2) lj_pair<T>(d2, eij, sij, flj, elj);
     const T qq = (T)q[i] * (T)q[j];
-    if (p.coulomb && qq != T(0)) coul_pair<T>(d2, qq, (T)p.alpha, (T)p.qqr2e, fc, ec);
+    if (p.coulomb && qq != T(0)) {
+      if (sn.on) {
+        const T r = std::sqrt(d2);
+        T poly = 0, pw = 1;
+        for (int k = 0; k < 6; ++k) {
+          poly += (T)sn.T[k] * pw;
+          pw *= d2;
+        }
+        fc = -(T)p.qqr2e * qq * (T(1) / (r * d2) + poly);
+        ec = T(0.5) * (T)p.qqr2e * qq *
+             (T(1) / r - (T)orc_sog_far_r((double)r, sn.sigma, sn.b, sn.mmax, sn.w0));
+      } else {
+        coul_pair<T>(d2, qq, (T)p.alpha, (T)p.qqr2e, fc, ec);
+      }
+    }
     const T ws = (T)special_weight(sp, i, gid[j]);
     T ffull = 0, efull = 0;
     if (p.coulomb && qq != T(0) && ws != T(1)) {
@@ -460,6 +490,36 @@ void orc_pair_force_special(int prec, int n, const float* x, const float* y,
   else
     pair_force_f32pos<double>(n, x, y, z, q, type, gid, ntypes, eps, sig, L,
                               pp, fx, fy, fz, energy, virial, mean, sp);
+}
+
+// The pair force with the SOG near field in place of erfc (RBL or full list,
+// optional special bonds: maxspecial 0 for none).
+void orc_pair_force_sog(int prec, int n, const float* x, const float* y, const float* z,
+                        const float* q, const int* type, const int* gid, int ntypes,
+                        const double* eps, const double* sig, const double* L,
+                        const double* pp, int maxspecial, const int* nspecial,
+                        const int* special, const double* w, const double* T,
+                        double sigma, double b, int mmax, double w0, double* fx,
+                        double* fy, double* fz, double* energy, double* virial,
+                        double* mean) {
+  Special sp;
+  sp.ms = maxspecial;
+  sp.ns = nspecial;
+  sp.ids = special;
+  for (int k = 0; k < 3; ++k) sp.w[k] = w ? w[k] : 1.0;
+  SogNearO sn;
+  sn.on = true;
+  for (int k = 0; k < 6; ++k) sn.T[k] = T[k];
+  sn.sigma = sigma;
+  sn.b = b;
+  sn.mmax = mmax;
+  sn.w0 = w0;
+  if (prec == 1)
+    pair_force_f32pos<float>(n, x, y, z, q, type, gid, ntypes, eps, sig, L, pp, fx, fy,
+                             fz, energy, virial, mean, sp, sn);
+  else
+    pair_force_f32pos<double>(n, x, y, z, q, type, gid, ntypes, eps, sig, L, pp, fx, fy,
+                              fz, energy, virial, mean, sp, sn);
 }
 
 // All-double full-list LJ (+ optional erfc Coulomb): the LJ/CUT Verlet path

@@ -917,7 +917,7 @@ class Engine : public EngineBase {
     if (c.rbsog_level >= 0) {
       sog_ = sog_set(c.rbsog_level, rc, L_, 1, c.rbsog_exact);
       sog_ready_ = true;
-      if (c.neighbor_type == "VERLET-SOG") {
+      if (c.neighbor_type == "VERLET-SOG" || c.sog_near) {
         const SogNear nf = sog_near(sog_);
         pc_.sog = 1;
         for (int q = 0; q < 6; ++q) pc_.T[q] = (R)nf.T[q];

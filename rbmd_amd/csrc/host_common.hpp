@@ -43,6 +43,8 @@ struct Config {
   std::string coulomb_type;   // "" | "RBE" | "EWALD" | "RBSOG"
   int rbsog_level = -1;       // coulomb.rbsog_level (RBSOG, VERLET-SOG)
   bool rbsog_exact = false;   // extension: corrected SOG set-up
+  bool sog_near = false;      // extension: SOG-consistent near field with RBSOG
+                              // (coulomb.rbsog_near_field "sog"; RBMD: erfc)
   bool has_alpha = false;
   double alpha = 0, accuracy = 0;
   int coulomb_sample_num = 0;
@@ -181,6 +183,19 @@ Config parse_config(const std::string& text) {
           throw Error(RBMDX_E_CONFIG, "coulomb.rbsog_exact must be true or false");
         c.rbsog_exact = ex->b;
       }
+    }
+    // RBMD screens the RBL / Verlet near field with erfc even under RBSOG
+    // (lj_cut_coul_kspace.cpp:473-492); "sog" pairs the SOG far field with its
+    // own near field, as VERLET-SOG does
+    if (const Json* nf = cl->get("rbsog_near_field")) {
+      const std::string v = str_of(nf, "coulomb.rbsog_near_field");
+      if (v != "erfc" && v != "sog")
+        throw Error(RBMDX_E_CONFIG, "coulomb.rbsog_near_field must be \"erfc\" or \"sog\"");
+      if (c.coulomb_type != "RBSOG")
+        throw Error(RBMDX_E_CONFIG, "coulomb.rbsog_near_field applies to RBSOG only");
+      if (c.neighbor_type == "VERLET-SOG" && v == "erfc")
+        throw Error(RBMDX_E_CONFIG, "VERLET-SOG always uses the SOG near field");
+      c.sog_near = v == "sog";
     }
     if (const Json* a = cl->get("alpha")) {
       c.has_alpha = true;

@@ -95,6 +95,11 @@ def _cfg(**edits):
     ({"hyper_parameters.neighbor.type": "VERLET-SOG",
       "hyper_parameters.coulomb.type": "RBSOG", "hyper_parameters.coulomb.rbsog_level": 2},
      None),
+    ({"hyper_parameters.coulomb.type": "RBSOG", "hyper_parameters.coulomb.rbsog_level": 1,
+      "hyper_parameters.coulomb.rbsog_near_field": "sog"}, None),
+    ({"hyper_parameters.coulomb.type": "RBSOG", "hyper_parameters.coulomb.rbsog_level": 1,
+      "hyper_parameters.coulomb.rbsog_near_field": "gauss"}, "rbsog_near_field"),
+    ({"hyper_parameters.coulomb.rbsog_near_field": "sog"}, "RBSOG only"),
     ({"hyper_parameters.neighbor.skin": -0.1}, "skin"),
     ({"init_configuration.read_data.unit": "SI"}, "unit"),
     ({"hyper_parameters.force_field.type": "CVFF"}, None),

@@ -18,7 +18,8 @@ from tests.refs import rel_rms
 pytestmark = pytest.mark.gpu
 
 
-def _sog_config(level=1, P=64, neighbor="RBL", exact=False, lj_cut=3.5, seed=20261202):
+def _sog_config(level=1, P=64, neighbor="RBL", exact=False, lj_cut=3.5, seed=20261202,
+                near=None):
     cfg = systems.ljsalt_config(lj_cut_off=lj_cut, P=P, seed=seed)
     cl = cfg["hyper_parameters"]["coulomb"]
     cl["type"] = "RBSOG"
@@ -26,6 +27,8 @@ def _sog_config(level=1, P=64, neighbor="RBL", exact=False, lj_cut=3.5, seed=202
     cl.pop("energy_rbe_flag", None)
     if exact:
         cl["rbsog_exact"] = True
+    if near is not None:
+        cl["rbsog_near_field"] = near
     if neighbor != "RBL":
         nb = cfg["hyper_parameters"]["neighbor"]
         nb["type"] = neighbor
@@ -139,3 +142,28 @@ def test_verlet_sog_force_parity(device, electrolyte, level):
     assert th["e_kspace"] == pytest.approx(ek, rel=1e-5)
     np.testing.assert_allclose(np.array(th["virial"]) - np.array(th["virial_kspace"]), vir,
                                rtol=1e-4, atol=1e-5 * abs(vir).max())
+
+
+@pytest.mark.parametrize("neighbor", ["RBL", "Verlet-List"])
+def test_rbsog_with_sog_near_field(device, electrolyte, neighbor):
+    """RBL / Verlet + RBSOG with coulomb.rbsog_near_field "sog": the SOG near
+    field in place of RBMD's erfc screening (the far field's own partner)."""
+    s = electrolyte
+    cfg = _sog_config(level=1, P=64, neighbor=neighbor, lj_cut=2.5, near="sog")
+    ctx = _ctx(s, cfg)
+    sset = oracle.sog_setup(1, 3.5, s.box_len)
+    m, idx = oracle.sog_sample(9, 0, ctx.num_samples(), s.box_len, sset)
+    step = 2
+    got = _run(ctx, s, device, step, m, idx)
+    pos32 = s.pos.astype(np.float32)
+    rbl = neighbor == "RBL"
+    freq = oracle.rbl_frequency(s.n, s.box_len, 2.5, 3.5, 30) if rbl else 1
+    params = oracle.pair_params(2.5, 3.5, 2.5, 0.09, 1.0, freq, step, int(rbl), 1, int(rbl))
+    fp, en, vir, _ = oracle.pair_force(pos32, s.charges.astype(np.float32), s.types, s.ids,
+                                       s.eps, s.sigma, s.box_len, params, sog_near=sset)
+    fk, ek, vk = _kspace_ref(s, pos32, m, idx, sset)
+    err = rel_rms(got, fp + fk)
+    print(f"{neighbor} + RBSOG, SOG near field: force rel RMS vs FP64 = {err:.3e}")
+    assert err < 1e-4
+    th = ctx.thermo()
+    assert th["e_coul"] == pytest.approx(en[1], rel=1e-4)

@@ -237,3 +237,22 @@ def test_sampler_marginal_chi_square():
     dof = sel.sum() - 1
     # consecutive kept samples are correlated; allow a generous margin
     assert chi2 < 4 * dof + 40, (chi2, dof)
+
+
+def test_pair_force_sog_near_field_equals_verlet_sog():
+    """The pair oracle with the SOG near field in place of erfc (RBL / Verlet +
+    RBSOG with coulomb.rbsog_near_field "sog") equals the VERLET-SOG oracle
+    on the full pair set (an independent O(N^2) implementation)."""
+    from rbmd_amd import systems
+
+    s = systems.electrolyte(1500, seed=3)
+    sset = oracle.sog_setup(1, 3.5, s.box_len)
+    pos32 = s.pos.astype(np.float32)
+    params = oracle.pair_params(2.5, 3.5, 2.5, 0.09, 1.0, 1, 0, 0, 1, 0)
+    F, en, vir, _ = oracle.pair_force(pos32, s.charges.astype(np.float32), s.types, s.ids,
+                                      s.eps, s.sigma, s.box_len, params, sog_near=sset)
+    Fv, env, virv = oracle.sog_verlet_f64(pos32.astype(np.float64), s.charges, s.types,
+                                          s.eps, s.sigma, s.box_len, 3.5, 2.5, sset)
+    np.testing.assert_allclose(F, Fv, rtol=0, atol=1e-12 * np.abs(Fv).max())
+    np.testing.assert_allclose(en, env, rtol=1e-12)
+    np.testing.assert_allclose(vir, virv, rtol=0, atol=1e-12 * np.abs(virv).max())
