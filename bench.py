@@ -63,14 +63,15 @@ def parse():
     ap.add_argument("--dt", type=float, default=0.002)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--workload", choices=["electrolyte", "adjoint", "replicas", "water"],
+    ap.add_argument("--workload",
+                    choices=["electrolyte", "adjoint", "replicas", "water", "lj"],
                     default="electrolyte",
                     help="electrolyte: the headline force step (default); adjoint: "
                          "forward and force-matching adjoint on the 2,000,000-particle "
                          "electrolyte with P = 400; replicas: 256 independent "
                          "electrolytes of 500-8000 atoms, batched vs per replica; water: "
-                         "72,981-atom SPC/E, CVFF + RBL + RBSOG (each prints its own JSON "
-                         "line)")
+                         "72,981-atom SPC/E, CVFF + RBL + RBSOG; lj: 4,000-atom FP64 LJ fcc, "
+                         "100 NVE steps (each prints its own JSON line)")
     ap.add_argument("--tune-cache", default=None,
                     help="replicas: JSON cache of the batch autotuner (default: measure)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -307,14 +308,76 @@ def bench_water(args):
     print(json.dumps(line))
 
 
+def bench_lj(args):
+    """Plain LJ fluid: 4,000 atoms (10 x 10 x 10 fcc, rho* 0.8442, +-0.05 sigma
+    jitter, Maxwell velocities at T* = 1), rc 2.5, skin 0.3, full Verlet list,
+    FP64, energy and virial every step; 100 NVE steps at dt = 0.005 (the force
+    fused with the leapfrog update, unit masses).  Reports the GPU rate, the
+    total-energy drift over the run, the force error against the FP64 oracle
+    and the oracle's CPU time (the reference has no CPU path)."""
+    import torch
+
+    import oracle
+    from rbmd_amd import ForceContext, systems
+
+    dev = torch.device("cuda:0")
+    s = systems.lj_fcc(cells=10, seed=1)
+    cfg = systems.lj_config(precision="fp64")
+    ctx = ForceContext(cfg, device=0)
+    ctx.set_system(s.box_lo, s.box_len, s.types, s.ids, s.charges, s.eps, s.sigma)
+    ctx.set_masses(np.ones(1))
+    n, dt, steps = s.n, 0.005, 100
+    pos = torch.tensor(s.pos.T.copy(), dtype=torch.float64, device=dev)
+    vel = torch.tensor(s.vel.T.copy(), dtype=torch.float64, device=dev)
+    frc = torch.zeros((3, n), dtype=torch.float64, device=dev)
+    # initial force, then half a kick back so the leapfrog velocities are v(-dt/2)
+    ctx.force(0, pos[0], pos[1], pos[2], frc[0], frc[1], frc[2], energy=True, sync=True)
+    e0 = ctx.thermo()["e_vdwl"] + 0.5 * float((vel ** 2).sum())
+    vel.sub_(frc, alpha=0.5 * dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(1, steps + 1):
+        ctx.force_leapfrog(k, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], frc[0],
+                           frc[1], frc[2], dt, energy=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # total energy at the end: the force at the final positions and v(t) from
+    # the half-step velocities
+    ctx.force(steps + 1, pos[0], pos[1], pos[2], frc[0], frc[1], frc[2], energy=True,
+              sync=True)
+    v_t = vel + 0.5 * dt * frc
+    e1 = ctx.thermo()["e_vdwl"] + 0.5 * float((v_t ** 2).sum())
+    P = pos.T.double().cpu().numpy()
+    F = frc.T.double().cpu().numpy()
+    tc = time.perf_counter()
+    ref, _, _ = oracle.verlet_force_f64(P, s.charges, s.types, s.eps, s.sigma, s.box_len, 2.5)
+    t_cpu = time.perf_counter() - tc
+    ferr = float(np.sqrt(((F - ref) ** 2).sum() / (ref ** 2).sum()))
+    line = {
+        "metric": "atom-steps/s, LJ fcc 4,000 atoms, FP64 full list with energy and virial",
+        "value": round(n * steps / elapsed, 1), "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": 1, "ms_per_step": round(1e3 * elapsed / steps, 4),
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "energy_drift_rel": abs(e1 - e0) / abs(e0), "force_rel_rms_vs_fp64": ferr,
+        "config": {"workload": "LJ fcc 10x10x10 (4,000 atoms), rho* 0.8442, rc 2.5, skin "
+                               "0.3, Verlet-List, FP64, NVE dt 0.005, 100 steps",
+                   "atoms": n, "rebuilds": ctx.thermo()["rebuilds"]},
+        "cpu_baseline": {"value": round(n / t_cpu, 1), "unit": "atom-steps/s", "cores": 1,
+                         "kind": "port",
+                         "sample": "one FP64 force evaluation of the final configuration "
+                                   "(cell search + full pair list)"},
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
-    if args.workload in ("adjoint", "replicas", "water"):
+    if args.workload in ("adjoint", "replicas", "water", "lj"):
         if args.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)
             sys.exit(2)
-        {"adjoint": bench_adjoint, "replicas": bench_replicas,
-         "water": bench_water}[args.workload](args)
+        {"adjoint": bench_adjoint, "replicas": bench_replicas, "water": bench_water,
+         "lj": bench_lj}[args.workload](args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); how the "
