@@ -833,6 +833,15 @@ class Engine : public EngineBase {
       grid_.nbrick = nb;
       const int per = (int)std::ceil(1.25 * apc * grid_.bw * grid_.bw * grid_.bw);
       grid_.block = std::min(512, std::max(64, (per + 63) / 64 * 64));
+      // mostly charged systems put nearly all pairs in list B, whose gathers
+      // want more waves in flight than one workgroup per brick gives: share
+      // each brick's charged atoms among up to 16 workgroups (about 16
+      // waves per SIMD over the chip's 1024 SIMDs)
+      grid_.psub = 1;
+      if (split_ && 2 * (long)nc > (long)n_) {
+        const double waves = (double)nb * (grid_.block / 64);
+        grid_.psub = std::max(1, std::min(16, (int)std::ceil(16384.0 / waves)));
+      }
       // brick-major cell numbering
       std::vector<int> c_of_xyz(grid_.ncell), xyz_of_c(grid_.ncell),
           b0(nb + 1);
@@ -981,6 +990,7 @@ class Engine : public EngineBase {
     d_.typec = alloc<int>(nc);
     d_.xqc = alloc<R4>(nc);
     d_.fk = alloc<R4>(nc);
+    d_.fb = alloc<R4>(nc);
     d_.cnt_a = alloc<int>(n_);
     d_.cnt_b = alloc<int>(nc);
     d_.inv_mass = alloc<R>(ntypes_);
@@ -989,8 +999,8 @@ class Engine : public EngineBase {
       HIPCHK(hipMemcpy(d_.inv_mass, one.data(), sizeof(R) * ntypes_,
                        hipMemcpyHostToDevice));
     }
-    d_.pair_part_blocks = grid_.nbrick;
-    d_.pair_part = alloc<double>((size_t)11 * grid_.nbrick);
+    d_.pair_part_blocks = grid_.nbrick * grid_.psub;
+    d_.pair_part = alloc<double>((size_t)11 * grid_.nbrick * grid_.psub);
     d_.skin_part = alloc<float>(kSkinBlocksMax);
     d_.lista_part = alloc<int4>(grid_.nbrick);
     d_.listb_part = alloc<int2>(std::max(1, list_b_blocks(nc, grid_.block)));

@@ -73,6 +73,7 @@ struct Grid {
   int nb[3];       // bricks per dimension
   int nbrick;
   int block;       // threads per brick block
+  int psub = 1;    // pair-kernel workgroups per brick (they share list B)
   const int* cell_of_coord = nullptr;  // [ncell] (z*ny+y)*nx+x -> cell id
   const int* coord_of_cell = nullptr;  // [ncell] cell id -> packed x|y<<10|z<<20
   const int* brick_cell0 = nullptr;    // [nbrick+1] first cell id of a brick
@@ -156,6 +157,7 @@ struct Dev {
   int* typec = nullptr;    // [nc]
   R4* xqc = nullptr;       // [nc] (x, y, z, q)
   R4* fk = nullptr;        // [nc] k-space force
+  R4* fb = nullptr;        // [nc] charged-pair (list B) force
   // lists
   int cap_a = 0, cap_b = 0;
   uint16_t* list_a = nullptr;  // [cap_a * n] groups of 8 (list_a_index), stage byte offsets

@@ -1113,64 +1113,70 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
   R* s_lj = reinterpret_cast<R*>(dyn_lds);
   load_lj<R>(lj_tab, pc.ntypes, s_lj);
   const int e1[3] = {1, 1, 1};
-  const int brick = xcd_brick(blockIdx.x, g.nbrick);
+  // workgroup b serves brick b % nbrick: the first of the brick's g.psub
+  // workgroups stages it and runs list A; all of them share the brick's
+  // charged atoms over list B
+  const int sub = blockIdx.x / g.nbrick;
+  const int brick = xcd_brick(blockIdx.x - sub * g.nbrick, g.nbrick);
   const Neigh h = brick_neigh(g, brick, e1);
-  build_slots(g, h, d.key_start, nullptr, false, t);
-  const int total = t.off[h.nslots];
-  // the last stage entry is the sentinel
-  const bool staged = total < StageCap<R>::value;
-  if (staged)
-    stage_atoms<R, true>(g, b, h, t, d.xs, stage, SPECIAL ? d.gid_s : nullptr,
-                         SPECIAL ? sgid : nullptr);
-  const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
   Acc<R> acc;
-  // phase 1: LJ over list A
-  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    Acc<R> a;
-    const R4T<R> xi = d.xs[i];
-    const int ti = (int)xi.w;
-    const unsigned long long hi = RBL ? rbl_hi(d.gid_s[i]) : 0ull;
-    const int cnt = d.cnt_a[i];
-    const SpecRow sr = SPECIAL ? spec_row<R>(d, pc.ms, i) : SpecRow();
-    if (cnt <= d.cap_a) {
-      if (staged && !h.mi)
-        list_a_atom<R, RBL, EFLAG, SHELL, true, false, SPECIAL>(
-            d, b, pc, s_lj, t, h.nslots, stage, i, cnt, xi, ti, hi, a, sr, sgid);
-      else if (staged)
-        list_a_atom<R, RBL, EFLAG, SHELL, true, true, SPECIAL>(
-            d, b, pc, s_lj, t, h.nslots, stage, i, cnt, xi, ti, hi, a, sr, sgid);
-      else
-        list_a_atom<R, RBL, EFLAG, SHELL, false, true, SPECIAL>(
-            d, b, pc, s_lj, t, h.nslots, stage, i, cnt, xi, ti, hi, a, sr, sgid);
-    } else {
-      // list overflow since the last rebuild: scan the 27 cells directly
-      int c[3];
-      unpack_coord(g.coord_of_cell[d.cell_s[i]], c);
-      const Stencil st = atom_stencil(g, h, c);
-      const int os = own_slot(h, c);
-      const bool qi = split && (i - t.first[os]) < t.nq[os];
-      for (int kz = 0; kz < st.n[2]; ++kz)
-        for (int ky = 0; ky < st.n[1]; ++ky)
-          for (int kx = 0; kx < st.n[0]; ++kx) {
-            const int sl = (st.at(2, kz) * h.cnt[1] + st.at(1, ky)) * h.cnt[0] +
-                           st.at(0, kx);
-            const int len = t.off[sl + 1] - t.off[sl];
-            for (int k = qi ? t.nq[sl] : 0; k < len; ++k) {
-              const int jg = t.first[sl] + k;
-              if (jg == i) continue;
-              interact_a<R, RBL, EFLAG, SPECIAL>(pc, b, s_lj, ti, xi, hi, d.xs[jg],
-                                                 [&] { return d.hj_s[jg]; }, sr,
-                                                 SPECIAL ? d.gid_s[jg] : 0, a);
+  if (sub == 0) {
+    build_slots(g, h, d.key_start, nullptr, false, t);
+    const int total = t.off[h.nslots];
+    // the last stage entry is the sentinel
+    const bool staged = total < StageCap<R>::value;
+    if (staged)
+      stage_atoms<R, true>(g, b, h, t, d.xs, stage, SPECIAL ? d.gid_s : nullptr,
+                           SPECIAL ? sgid : nullptr);
+    const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
+    // phase 1: LJ over list A
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+      Acc<R> a;
+      const R4T<R> xi = d.xs[i];
+      const int ti = (int)xi.w;
+      const unsigned long long hi = RBL ? rbl_hi(d.gid_s[i]) : 0ull;
+      const int cnt = d.cnt_a[i];
+      const SpecRow sr = SPECIAL ? spec_row<R>(d, pc.ms, i) : SpecRow();
+      if (cnt <= d.cap_a) {
+        if (staged && !h.mi)
+          list_a_atom<R, RBL, EFLAG, SHELL, true, false, SPECIAL>(
+              d, b, pc, s_lj, t, h.nslots, stage, i, cnt, xi, ti, hi, a, sr, sgid);
+        else if (staged)
+          list_a_atom<R, RBL, EFLAG, SHELL, true, true, SPECIAL>(
+              d, b, pc, s_lj, t, h.nslots, stage, i, cnt, xi, ti, hi, a, sr, sgid);
+        else
+          list_a_atom<R, RBL, EFLAG, SHELL, false, true, SPECIAL>(
+              d, b, pc, s_lj, t, h.nslots, stage, i, cnt, xi, ti, hi, a, sr, sgid);
+      } else {
+        // list overflow since the last rebuild: scan the 27 cells directly
+        int c[3];
+        unpack_coord(g.coord_of_cell[d.cell_s[i]], c);
+        const Stencil st = atom_stencil(g, h, c);
+        const int os = own_slot(h, c);
+        const bool qi = split && (i - t.first[os]) < t.nq[os];
+        for (int kz = 0; kz < st.n[2]; ++kz)
+          for (int ky = 0; ky < st.n[1]; ++ky)
+            for (int kx = 0; kx < st.n[0]; ++kx) {
+              const int sl = (st.at(2, kz) * h.cnt[1] + st.at(1, ky)) * h.cnt[0] +
+                             st.at(0, kx);
+              const int len = t.off[sl + 1] - t.off[sl];
+              for (int k = qi ? t.nq[sl] : 0; k < len; ++k) {
+                const int jg = t.first[sl] + k;
+                if (jg == i) continue;
+                interact_a<R, RBL, EFLAG, SPECIAL>(pc, b, s_lj, ti, xi, hi, d.xs[jg],
+                                                   [&] { return d.hj_s[jg]; }, sr,
+                                                   SPECIAL ? d.gid_s[jg] : 0, a);
+              }
             }
-          }
-    }
-    d.fa[i] = make4<R>(a.fx, a.fy, a.fz, R(0));
-    acc.fx += a.fx;
-    acc.fy += a.fy;
-    acc.fz += a.fz;
-    if (EFLAG) {
-      acc.e_lj += a.e_lj;
-      for (int k = 0; k < 6; ++k) acc.v[k] += a.v[k];
+      }
+      d.fa[i] = make4<R>(a.fx, a.fy, a.fz, R(0));
+      acc.fx += a.fx;
+      acc.fy += a.fy;
+      acc.fz += a.fz;
+      if (EFLAG) {
+        acc.e_lj += a.e_lj;
+        for (int k = 0; k < 6; ++k) acc.v[k] += a.v[k];
+      }
     }
   }
   // phase 2: charged atoms of the brick over list B
@@ -1179,7 +1185,7 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
     const int c0 = d.cstart[h.cell0], c1 = d.cstart[h.cell1];
     const int grp = threadIdx.x / kChargedLanes, gl = threadIdx.x % kChargedLanes;
     const int ngrp = blockDim.x / kChargedLanes;
-    for (int c = c0 + grp; c < c1; c += ngrp) {
+    for (int c = c0 + sub * ngrp + grp; c < c1; c += ngrp * g.psub) {
       Acc<R> a;
       const R4T<R> xi = d.xqc[c];
       const int ti = d.typec[c];
@@ -1188,7 +1194,28 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
       const SpecRow sr = SPECIAL ? spec_row<R>(d, pc.ms, d.c2s[c]) : SpecRow();
       if (cnt <= d.cap_b) {
         const int* lp = d.list_b + (size_t)c * d.cap_b;
-        for (int k = gl; k < cnt; k += kChargedLanes) {
+        int k = gl;
+        // variants without the occupancy hint (shell pairs in list A,
+        // special bonds: the list-B-heavy systems) keep four entries'
+        // gathers in flight before any of them is evaluated
+        if (SHELL || SPECIAL)
+        for (; k + 3 * kChargedLanes < cnt; k += 4 * kChargedLanes) {
+          int e[4], tj[4], gj[4];
+          R4T<R> xj[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) e[u] = lp[k + u * kChargedLanes];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            xj[u] = d.xqc[e[u]];
+            tj[u] = d.typec[e[u]];
+            gj[u] = SPECIAL ? d.gidc[e[u]] : 0;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            interact_b<R, RBL, EFLAG, SPECIAL>(pc, b, s_lj, ti, xi, hi, xj[u], tj[u],
+                                               d.hjc + e[u], a, sr, gj[u]);
+        }
+        for (; k < cnt; k += kChargedLanes) {
           const int e = lp[k];
           interact_b<R, RBL, EFLAG, SPECIAL>(pc, b, s_lj, ti, xi, hi, d.xqc[e],
                                              d.typec[e], d.hjc + e, a, sr,
@@ -1224,14 +1251,7 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
         fy += __shfl_xor(fy, o, kChargedLanes);
         fz += __shfl_xor(fz, o, kChargedLanes);
       }
-      if (gl == 0) {
-        const int i = d.c2s[c];
-        R4T<R> f = d.fa[i];
-        f.x += fx;
-        f.y += fy;
-        f.z += fz;
-        d.fa[i] = f;
-      }
+      if (gl == 0) d.fb[c] = make4<R>(fx, fy, fz, R(0));
       acc.fx += a.fx;
       acc.fy += a.fy;
       acc.fz += a.fz;
@@ -1242,7 +1262,7 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
       }
     }
   }
-  write_partials<R, EFLAG>(acc, d.pair_part, brick, smem);
+  write_partials<R, EFLAG>(acc, d.pair_part, blockIdx.x, smem);
 }
 
 // One block after the pair kernel: FP64 sums of the per-brick partials in a
@@ -1571,7 +1591,8 @@ __global__ __launch_bounds__(64 * kRbeWaves) void k_rbe_force(Dev<R> d,
 // velocity_controller/velocity_controller_op/rocm/update_velocity_op.hip.cu:5-27).
 template <typename R>
 __global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix,
-                                                  int has_k, R dt, R dt_ftm) {
+                                                  int has_k, int has_b, R dt,
+                                                  R dt_ftm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     d.ctrl->maxdisp_bits = 0u;
@@ -1592,13 +1613,21 @@ __global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix,
     f.y -= (R)d.mean[1];
     f.z -= (R)d.mean[2];
   }
-  if (has_k) {
+  if (has_k || has_b) {
     const int c = d.c_of[i];
     if (c >= 0) {
-      const R4T<R> g = d.fk[c];
-      f.x += g.x;
-      f.y += g.y;
-      f.z += g.z;
+      if (has_b) {
+        const R4T<R> g = d.fb[c];
+        f.x += g.x;
+        f.y += g.y;
+        f.z += g.z;
+      }
+      if (has_k) {
+        const R4T<R> g = d.fk[c];
+        f.x += g.x;
+        f.y += g.y;
+        f.z += g.z;
+      }
     }
   }
   d.fx[i] = f.x;
@@ -1782,7 +1811,7 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   // ev[1] .. ev[2] bracket the pair kernel alone
   if (a.ev[1]) hipEventRecord(a.ev[1], s);
 #define RBMDX_PAIR(RBLv, EFv, SHv, SPv)                                     \
-  hipLaunchKernelGGL((k_pair<R, RBLv, EFv, SHv, SPv>), dim3(g.nbrick),       \
+  hipLaunchKernelGGL((k_pair<R, RBLv, EFv, SHv, SPv>), dim3(g.nbrick * g.psub), \
                      dim3(g.block), lj_bytes, s, d, a.box, g, a.pc, a.lj_tab, \
                      p.split)
   // list A holds RBL shell pairs only when the LJ cutoff exceeds r_core
@@ -1820,13 +1849,14 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   }
 #undef RBMDX_PAIR
   if (a.ev[2]) hipEventRecord(a.ev[2], s);
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part, g.nbrick,
-                     p.eflag ? 1 : 0, d.thermo, d.mean, p.mean_fix, p.n_global,
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part,
+                     g.nbrick * g.psub, p.eflag ? 1 : 0, d.thermo, d.mean, p.mean_fix, p.n_global,
                      d.lista_part, g.nbrick, d.listb_part,
                      list_b_grid(p, d.nc, g), d.ctrl, a.rebuild_thr);
   if (ks_on && ks != s) hipStreamWaitEvent(s, a.ev_join, 0);
   hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,
-                     s, d, p.mean_fix, ks_on ? 1 : 0, (R)a.dt, (R)a.dt_ftm);
+                     s, d, p.mean_fix, ks_on ? 1 : 0, p.split ? 1 : 0, (R)a.dt,
+                     (R)a.dt_ftm);
   if (a.ev[5]) hipEventRecord(a.ev[5], s);
   return hipGetLastError();
 }
