@@ -488,6 +488,26 @@ __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
           v = (lower == up) ? min(v, p) : max(v, p);
         }
       if (lane < len) perm[a + lane] = v;
+    } else if (len <= 512) {
+      // rank sort by the whole wave (the values are distinct caller
+      // indices): every lane ranks its up to 8 values against the segment,
+      // and all ranks are known before the first store
+      int v[8], r[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int p = a + lane + 64 * u;
+        v[u] = p < e ? perm[p] : 0x7fffffff;
+        r[u] = 0;
+      }
+      for (int q = a; q < e; ++q) {
+        const int w = perm[q];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) r[u] += w < v[u] ? 1 : 0;
+      }
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (a + lane + 64 * u < e) perm[a + r[u]] = v[u];
     } else if (lane == 0) {
       for (int p = a + 1; p < e; ++p) {
         const int v = perm[p];
