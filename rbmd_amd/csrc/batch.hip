@@ -104,13 +104,41 @@ __global__ __launch_bounds__(256) void kb_scatter(BatchDev d, BatchRange rg) {
   d.tmp[d.start[d.cell_of[i]] + d.slot[i]] = i;
 }
 
-// One thread per cell: order the bucket (charged first, then caller index)
-// and write the sorted arrays.
+// 32 lanes per cell: order the bucket (charged first, then caller index) by
+// ranking every atom's key within the cell, and write the sorted arrays in
+// parallel; a cell of more than 32 atoms is ordered by its first lane.
+__device__ __forceinline__ void batch_sorted_write(const BatchDev& d, int i, int k,
+                                                   int qslot) {
+  const RepInfo& p = d.rep[d.rep_of[i]];
+  float w[3];
+  batch_cell(p, d.px[i], d.py[i], d.pz[i], w);
+  const float qi = d.charge[i];
+  d.xs[k] = make_float4(w[0], w[1], w[2], qi);
+  const int g = d.gid[i];
+  d.tg[k] = make_int2(d.type[i], g);
+  d.hj[k] = rbl_hj(g);
+  d.perm[k] = i;
+  if (qslot >= 0) d.qidx[qslot] = k;
+}
+
 __global__ __launch_bounds__(256) void kb_cellsort(BatchDev d, BatchRange rg) {
-  const int c = rg.c0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= rg.c1) return;
-  const int s0 = d.start[c], s1 = d.start[c + 1];
+  const int lane = threadIdx.x & 31;
+  const int c = rg.c0 + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 5);
+  if (c >= rg.c1) return;  // whole 32-lane groups leave together
+  const int s0 = d.start[c], s1 = d.start[c + 1], len = s1 - s0;
   auto key = [&](int i) { return (d.charge[i] != 0.f ? 0 : (1 << 30)) + i; };
+  if (len <= 32) {
+    const int i = lane < len ? d.tmp[s0 + lane] : 0;
+    const int kv = lane < len ? key(i) : 0x7fffffff;
+    int r = 0;
+    for (int q = 0; q < len; ++q) r += __shfl(kv, q, 32) < kv ? 1 : 0;
+    if (lane < len) {
+      const int q0 = d.qstart[c];
+      batch_sorted_write(d, i, s0 + r, kv < (1 << 30) ? q0 + r : -1);
+    }
+    return;
+  }
+  if (lane != 0) return;
   for (int k = s0 + 1; k < s1; ++k) {
     const int v = d.tmp[k], kv = key(v);
     int j = k - 1;
@@ -123,16 +151,7 @@ __global__ __launch_bounds__(256) void kb_cellsort(BatchDev d, BatchRange rg) {
   int q = d.qstart[c];
   for (int k = s0; k < s1; ++k) {
     const int i = d.tmp[k];
-    const RepInfo& p = d.rep[d.rep_of[i]];
-    float w[3];
-    batch_cell(p, d.px[i], d.py[i], d.pz[i], w);
-    const float qi = d.charge[i];
-    d.xs[k] = make_float4(w[0], w[1], w[2], qi);
-    const int g = d.gid[i];
-    d.tg[k] = make_int2(d.type[i], g);
-    d.hj[k] = rbl_hj(g);
-    d.perm[k] = i;
-    if (qi != 0.f) d.qidx[q++] = k;
+    batch_sorted_write(d, i, k, d.charge[i] != 0.f ? q++ : -1);
   }
 }
 
@@ -319,7 +338,12 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
       own_end[u] = (u > 0 ? own_end[u - 1] : 0) + len;
     }
     const int nown = own_end[7];
-    for (int t = threadIdx.x; t < nown; t += blockDim.x) {
+    // lpo adjacent lanes per owner (up to 4, as many as the block leaves
+    // idle); lane `sub` takes every lpo-th row of the owner's stencil
+    int lpo = 1;
+    while (lpo < 4 && nown * lpo * 2 <= (int)blockDim.x) lpo *= 2;
+    for (int tt = threadIdx.x; tt < nown * lpo; tt += blockDim.x) {
+      const int t = tt / lpo, sub = tt % lpo;
       int u = 0, ub = 0;
 #pragma unroll
       for (int q = 0; q < 7; ++q)
@@ -346,41 +370,48 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
         lo3[q] = h.full[q] ? 0 : ko[q] - 1;
         hi3[q] = h.full[q] ? h.rn[q] - 1 : ko[q] + 1;
       }
-      for (int kz = lo3[2]; kz <= hi3[2]; ++kz)
-        for (int ky = lo3[1]; ky <= hi3[1]; ++ky) {
-          const int srow = (kz * h.rn[1] + ky) * h.rn[0];
-          if (staged) {
-            const int l0 = soff[srow + lo3[0]], l1 = soff[srow + hi3[0] + 1];
-            for (int l = l0; l < l1; ++l) {
-              const float4 xj = sx[l];
-              if (l == li || (qi && xj.w != 0.f)) continue;
-              const int2 tgj = stg[l];
-              if (h.mi)
-                batch_lj<RBL, EFLAG, true>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x, tgj.y, fx,
-                                           fy, fz, a);
-              else
-                batch_lj<RBL, EFLAG, false>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x, tgj.y,
-                                            fx, fy, fz, a);
-            }
-          } else {
-            // beyond the stage: the same cells from global memory, minimum image
-            for (int kx = lo3[0]; kx <= hi3[0]; ++kx) {
-              const int s = srow + kx;
-              const int f = sfirst[s], len = soff[s + 1] - soff[s];
-              for (int j = 0; j < len; ++j) {
-                const float4 xj = d.xs[f + j];
-                if (f + j == i || (qi && xj.w != 0.f)) continue;
-                const int2 tgj = d.tg[f + j];
-                batch_lj<RBL, EFLAG, true>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x, tgj.y, fx,
-                                           fy, fz, a);
-              }
+      const int nyr = hi3[1] - lo3[1] + 1;
+      const int nrow = (hi3[2] - lo3[2] + 1) * nyr;
+      for (int row = sub; row < nrow; row += lpo) {
+        const int kz = lo3[2] + row / nyr, ky = lo3[1] + row % nyr;
+        const int srow = (kz * h.rn[1] + ky) * h.rn[0];
+        if (staged) {
+          const int l0 = soff[srow + lo3[0]], l1 = soff[srow + hi3[0] + 1];
+          for (int l = l0; l < l1; ++l) {
+            const float4 xj = sx[l];
+            if (l == li || (qi && xj.w != 0.f)) continue;
+            const int2 tgj = stg[l];
+            if (h.mi)
+              batch_lj<RBL, EFLAG, true>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x, tgj.y, fx,
+                                         fy, fz, a);
+            else
+              batch_lj<RBL, EFLAG, false>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x, tgj.y,
+                                          fx, fy, fz, a);
+          }
+        } else {
+          // beyond the stage: the same cells from global memory, minimum image
+          for (int kx = lo3[0]; kx <= hi3[0]; ++kx) {
+            const int s = srow + kx;
+            const int f = sfirst[s], len = soff[s + 1] - soff[s];
+            for (int j = 0; j < len; ++j) {
+              const float4 xj = d.xs[f + j];
+              if (f + j == i || (qi && xj.w != 0.f)) continue;
+              const int2 tgj = d.tg[f + j];
+              batch_lj<RBL, EFLAG, true>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x, tgj.y, fx,
+                                         fy, fz, a);
             }
           }
         }
-      d.fs[i] = make_float4(fx, fy, fz, 0.f);
+      }
       fsx += fx;
       fsy += fy;
       fsz += fz;
+      for (int o = lpo / 2; o > 0; o >>= 1) {
+        fx += __shfl_xor(fx, o, lpo);
+        fy += __shfl_xor(fy, o, lpo);
+        fz += __shfl_xor(fz, o, lpo);
+      }
+      if (sub == 0) d.fs[i] = make_float4(fx, fy, fz, 0.f);
     }
     // charged owners: LJ + Coulomb with the charged atoms of the ext_c
     // stencil (global memory, minimum image), qlanes lanes per owner
@@ -610,7 +641,7 @@ hipError_t batch_launch(const BatchDev& d, const PairConst<float>& pc, int rbl,
   hipLaunchKernelGGL(kb_bin, dim3(cdiv(na, 256)), dim3(256), 0, s, d, rg);
   hipLaunchKernelGGL(kb_scan, dim3(1), dim3(1024), 0, s, d, rg);
   hipLaunchKernelGGL(kb_scatter, dim3(cdiv(na, 256)), dim3(256), 0, s, d, rg);
-  hipLaunchKernelGGL(kb_cellsort, dim3(cdiv(ncl, 256)), dim3(256), 0, s, d, rg);
+  hipLaunchKernelGGL(kb_cellsort, dim3(cdiv(ncl, 8)), dim3(256), 0, s, d, rg);
   const int pg = std::max(1, std::min(nw, t.grid_per_cu * num_cu));
 #define RBMDX_KB_PAIR(RB, EF)                                                      \
   hipLaunchKernelGGL((kb_pair<RB, EF>), dim3(pg), dim3(t.pair_block), 0, s, d, pc, rg, \
