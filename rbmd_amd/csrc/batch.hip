@@ -212,15 +212,14 @@ __device__ __forceinline__ void batch_lj(const PairConst<float>& pc, const Box<f
 
 // LJ + erfc Coulomb of a charged pair (interact_b's scalars, weight and
 // energy convention; minimum image).
-template <bool RBL, bool EFLAG>
+template <bool RBL, bool EFLAG, bool MI = true>
 __device__ __forceinline__ void batch_charged(const PairConst<float>& pc,
                                               const Box<float>& b, const float* s_lj,
                                               int ti, const float4& xi,
                                               unsigned long long hi, const float4& xj,
-                                              int tj, unsigned long long hjj,
-                                              Acc<float>& a) {
+                                              int tj, int gj, Acc<float>& a) {
   float dx, dy, dz;
-  const float d2 = pair_d2<float>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+  const float d2 = pair_d2<float, MI>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
   if (!(d2 < pc.rc2)) return;
   float flj = 0.f, elj = 0.f;
   if (d2 < pc.rclj2) {
@@ -234,7 +233,7 @@ __device__ __forceinline__ void batch_charged(const PairConst<float>& pc,
   }
   float fc, ec;
   coul<float>(pc, d2, xi.w * xj.w, fc, ec);
-  const float w = pair_weight<float, RBL>(pc, d2, hi, [&] { return hjj; });
+  const float w = pair_weight<float, RBL>(pc, d2, hi, [&] { return rbl_hj(gj); });
   const float f = w * (flj + fc);
   a.fx += f * dx;
   a.fy += f * dy;
@@ -251,9 +250,12 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
                                                BatchRange rg, int bw, int qlanes) {
   __shared__ float4 sx[kBatchStageCap];
   __shared__ int2 stg[kBatchStageCap];
-  __shared__ int soff[kBatchMaxSlots + 1];
-  __shared__ int sfirst[kBatchMaxSlots];
-  __shared__ int sedge[kBatchMaxSlots];
+  // slot tables of the staged neighbourhood: up to kBatchMaxSlots cells for
+  // the LJ pass, up to kBatchQSlots for the charged pass
+  __shared__ int soff[kBatchQSlots + 1];
+  __shared__ int sfirst[kBatchQSlots];
+  __shared__ int sedge[kBatchQSlots];
+  __shared__ int sscan[33];
   __shared__ float s_lj[4 * kBatchMaxTypes * kBatchMaxTypes];
   __shared__ double smem[16 * kBatchAcc];
   const int nt = pc_arg.ntypes;
@@ -414,9 +416,75 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
       if (sub == 0) d.fs[i] = make_float4(fx, fy, fz, 0.f);
     }
     // charged owners: LJ + Coulomb with the charged atoms of the ext_c
-    // stencil (global memory, minimum image), qlanes lanes per owner
+    // stencil, qlanes lanes per owner.  The charged atoms of the brick
+    // widened by ext_c (a dimension that would wrap onto itself taken whole)
+    // are restaged into the LJ stage buffers, edge-shifted; a neighbourhood
+    // beyond the tables or the stage is read from global memory.
     __syncthreads();
     {
+      int qs[3], qn[3], qfull[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        qfull[q] = h.w[q] + 2 * p.ext_c[q] >= p.nc[q];
+        qs[q] = qfull[q] ? 0 : h.o[q] - p.ext_c[q];
+        qn[q] = qfull[q] ? p.nc[q] : h.w[q] + 2 * p.ext_c[q];
+      }
+      const int qslots = qn[0] * qn[1] * qn[2];
+      const bool qmi = qfull[0] | qfull[1] | qfull[2];
+      bool qstaged = false;
+      if (qslots <= kBatchQSlots) {
+        const int per = (qslots + blockDim.x - 1) / blockDim.x;
+        const int s0 = threadIdx.x * per, s1 = min(qslots, s0 + per);
+        int sum = 0;
+        for (int s = s0; s < s1; ++s) {
+          const int kk[3] = {s % qn[0], (s / qn[0]) % qn[1], s / (qn[0] * qn[1])};
+          int cw[3], e = 0;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const int u = qs[q] + kk[q];
+            cw[q] = wrapi(u, p.nc[q]);
+            e |= (u < 0 ? 0 : (u >= p.nc[q] ? 2 : 1)) << (2 * q);
+          }
+          const int cell = p.c0 + (cw[2] * p.nc[1] + cw[1]) * p.nc[0] + cw[0];
+          sfirst[s] = d.start[cell];
+          sedge[s] = e;
+          const int cnt = d.qstart[cell + 1] - d.qstart[cell];
+          soff[s] = cnt;
+          sum += cnt;
+        }
+        __syncthreads();
+        int tot;
+        int run = block_excl_scan(sum, tot, sscan);
+        for (int s = s0; s < s1; ++s) {
+          const int c = soff[s];
+          soff[s] = run;
+          run += c;
+        }
+        if (threadIdx.x == 0) soff[qslots] = tot;
+        __syncthreads();
+        qstaged = tot <= kBatchStageCap;
+        if (qstaged) {
+          const int lane = threadIdx.x & 15;
+          for (int s = threadIdx.x >> 4; s < qslots; s += blockDim.x >> 4) {
+            const int o = soff[s], len = soff[s + 1] - o, f = sfirst[s], ec = sedge[s];
+            float sh[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              const int cc = (ec >> (2 * q)) & 3;
+              sh[q] = cc == 0 ? -b.L[q] : (cc == 2 ? b.L[q] : 0.f);
+            }
+            for (int j = lane; j < len; j += 16) {
+              float4 v = d.xs[f + j];
+              v.x += sh[0];
+              v.y += sh[1];
+              v.z += sh[2];
+              sx[o + j] = v;
+              stg[o + j] = d.tg[f + j];
+            }
+          }
+        }
+        __syncthreads();
+      }
       const int grp = threadIdx.x / qlanes, gl = threadIdx.x % qlanes;
       const int ngrp = blockDim.x / qlanes;
       int qend[8];
@@ -451,21 +519,48 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
           const float4 xi = d.xs[i];
           const int ti = d.tg[i].x;
           const unsigned long long hi = RBL ? rbl_hi(d.tg[i].y) : 0ull;
-          const Span spx = make_span(c[0], p.ext_c[0], p.nc[0]);
-          const Span spy = make_span(c[1], p.ext_c[1], p.nc[1]);
-          const Span spz = make_span(c[2], p.ext_c[2], p.nc[2]);
-          const int nst = spx.count * spy.count * spz.count;
           Acc<float> ac;
-          for (int s = gl; s < nst; s += qlanes) {
-            const int kx = s % spx.count, ky = (s / spx.count) % spy.count,
-                      kz = s / (spx.count * spy.count);
-            const int cj =
-                p.c0 + (spz.at(kz) * p.nc[1] + spy.at(ky)) * p.nc[0] + spx.at(kx);
-            const int j0 = d.start[cj], j1 = j0 + (d.qstart[cj + 1] - d.qstart[cj]);
-            for (int j = j0; j < j1; ++j) {
-              if (j == i) continue;
-              batch_charged<RBL, EFLAG>(pc, b, s_lj, ti, xi, hi, d.xs[j], d.tg[j].x,
-                                        RBL ? d.hj[j] : 0ull, ac);
+          if (qstaged) {
+            // rows of the stencil in the restaged neighbourhood
+            int lo3[3], nn[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              lo3[q] = qfull[q] ? 0 : c[q] - h.o[q];
+              nn[q] = qfull[q] ? qn[q] : 2 * p.ext_c[q] + 1;
+            }
+            const int li = soff[((c[2] - qs[2]) * qn[1] + (c[1] - qs[1])) * qn[0] +
+                                (c[0] - qs[0])] + (t - ub);
+            for (int row = gl; row < nn[1] * nn[2]; row += qlanes) {
+              const int ky = lo3[1] + row % nn[1], kz = lo3[2] + row / nn[1];
+              const int srow = (kz * qn[1] + ky) * qn[0];
+              const int l1 = soff[srow + lo3[0] + nn[0]];
+              for (int l = soff[srow + lo3[0]]; l < l1; ++l) {
+                if (l == li) continue;
+                const int2 tgj = stg[l];
+                if (qmi)
+                  batch_charged<RBL, EFLAG, true>(pc, b, s_lj, ti, xi, hi, sx[l], tgj.x,
+                                                  tgj.y, ac);
+                else
+                  batch_charged<RBL, EFLAG, false>(pc, b, s_lj, ti, xi, hi, sx[l], tgj.x,
+                                                   tgj.y, ac);
+              }
+            }
+          } else {
+            const Span spx = make_span(c[0], p.ext_c[0], p.nc[0]);
+            const Span spy = make_span(c[1], p.ext_c[1], p.nc[1]);
+            const Span spz = make_span(c[2], p.ext_c[2], p.nc[2]);
+            const int nst = spx.count * spy.count * spz.count;
+            for (int s = gl; s < nst; s += qlanes) {
+              const int kx = s % spx.count, ky = (s / spx.count) % spy.count,
+                        kz = s / (spx.count * spy.count);
+              const int cj =
+                  p.c0 + (spz.at(kz) * p.nc[1] + spy.at(ky)) * p.nc[0] + spx.at(kx);
+              const int j0 = d.start[cj], j1 = j0 + (d.qstart[cj + 1] - d.qstart[cj]);
+              for (int j = j0; j < j1; ++j) {
+                if (j == i) continue;
+                batch_charged<RBL, EFLAG>(pc, b, s_lj, ti, xi, hi, d.xs[j], d.tg[j].x,
+                                          d.tg[j].y, ac);
+              }
             }
           }
           fx = ac.fx;

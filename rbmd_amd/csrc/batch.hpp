@@ -12,8 +12,9 @@
 
 namespace rbmdx {
 
-constexpr int kBatchStageCap = 2048;  // atoms staged per brick (LDS)
+constexpr int kBatchStageCap = 1536;  // atoms staged per brick (LDS)
 constexpr int kBatchMaxSlots = 64;    // neighbourhood cells per brick (bw <= 2)
+constexpr int kBatchQSlots = 512;     // cells of the charged pass's neighbourhood
 constexpr int kBatchMaxK = 512;       // k-vectors per replica
 constexpr int kBatchMaxTypes = 8;     // atom types (LDS LJ table)
 constexpr int kBatchAcc = 11;         // per-replica pair sums: F (3), E_lj, E_c, W (6)

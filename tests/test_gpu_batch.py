@@ -81,7 +81,8 @@ def test_batch_graph_eager_and_per_replica_agree(device):
         b.force(3, px, py, pz, *f, m_samples=m, sync=True, **kw)
         out.append(_forces(f))
     for o in out[1:]:
-        np.testing.assert_allclose(o, out[0], rtol=0, atol=1e-6 * np.abs(out[0]).max())
+        assert rel_rms(o, out[0]) < 1e-6
+        np.testing.assert_allclose(o, out[0], rtol=0, atol=1e-5 * np.abs(out[0]).max())
 
 
 def test_batch_wraps_positions_outside_the_box(device):
@@ -124,11 +125,16 @@ def test_batch_autotune_cache_roundtrip(device, tmp_path):
     b2, _ = _batch(cfg, [700, 1800, 3000], seed=41)
     shape2, us2 = b2.autotune(1, px, py, pz, *f, m_samples=m, cache_path=cache)
     assert us2 == 0.0 and shape2 == shape
-    # the tuned shape changes launch geometry only
+    # a launch shape changes the geometry and with it the FP32 summation order
+    # of a few terms per atom, nothing else
+    def same(G):
+        assert rel_rms(G, F) < 1e-6
+        np.testing.assert_allclose(G, F, rtol=0, atol=1e-5 * np.abs(F).max())
+
     b2.force(1, px, py, pz, *f, m_samples=m, sync=True)
-    np.testing.assert_allclose(_forces(f), F, rtol=0, atol=1e-6 * np.abs(F).max())
+    same(_forces(f))
     for s in ({"bw": 1, "pair_block": 64, "grid_per_cu": 1, "qlanes": 4},
               {"bw": 2, "pair_block": 256, "grid_per_cu": 8, "qlanes": 16}):
         b2.set_shape(**s)
         b2.force(1, px, py, pz, *f, m_samples=m, sync=True)
-        np.testing.assert_allclose(_forces(f), F, rtol=0, atol=1e-6 * np.abs(F).max())
+        same(_forces(f))
