@@ -126,6 +126,25 @@ typedef struct rbmdx_thermo_t {
                             (kspace_calculator.cpp:628-669)               */
 } rbmdx_thermo_t;
 
+/* State of the molecular-dynamics step (rbmdx_md_*). */
+typedef struct rbmdx_md_state_t {
+  int64_t step;          /* steps completed since rbmdx_md_init              */
+  int32_t mode;          /* 0 NVE, 1 NVE + energy-stable rescale, 2 NVT
+                            Nose-Hoover chain                                */
+  int32_t pad;
+  double temperature;    /* as the thermostat last sampled it: NVT after the
+                            chain's scaling (the current velocities), NVE
+                            energy-stable before the rescale by xi          */
+  double ke;             /* kinetic energy per atom, sampled with temperature */
+  double pe;             /* potential energy per atom of the last evaluation
+                            (0 unless energies were computed)                */
+  double t_target;       /* thermostat target of the last step               */
+  double xi;             /* last energy-stable factor (1 when not used)      */
+  double eta[3], eta_dot[3], eta_mass[3];  /* thermostat chain              */
+  double thermostat_energy;  /* chain energy (total): sum Q eta_dot^2 / 2 +
+                                tdof kT eta_0 + kT sum_{i>0} eta_i           */
+} rbmdx_md_state_t;
+
 /*
  * Create a context from an RBMD JSON configuration (the whole document, or
  * just its "hyper_parameters" object).  Recognised keys:
@@ -228,6 +247,46 @@ int rbmdx_force_leapfrog(rbmdx_ctx* ctx, int64_t step, void* px, void* py,
 
 /* Synchronise the context's last stream and read energies/virial. */
 int rbmdx_thermo(rbmdx_ctx* ctx, rbmdx_thermo_t* out);
+
+/*
+ * Molecular dynamics on the context's system: RBMD's step either side of the
+ * force, velocity Verlet with periodic wrapping and image flags
+ * (src/simulate_pipeline/src/controller/velocity_controller/
+ * velocity_controller_op/rocm/update_velocity_op.hip.cu:5-27,
+ * src/simulate_pipeline/src/controller/position_controller/
+ * position_controller_op/rocm/update_position_op.hip.cu:17-37,
+ * src/data_manager/include/model/box.h:133-173), with
+ *   - execution.ensemble "NVT", temp_ctrl_type "NOSE_HOOVER": the Nose-Hoover
+ *     chain thermostat (3 links) with temperature [start, stop, damp]
+ *     (src/simulate_pipeline/src/controller/pressure_controller/
+ *     nose_hoover_controller.cpp:89-250,415-470,633-840);
+ *   - execution.ensemble "NVE" with RBL lists: the energy-stable velocity
+ *     rescale (src/simulate_pipeline/src/controller/energy_stable_controller/
+ *     energy_stable_scheme_controller.cpp:56-144), against the Hamiltonian of
+ *     step 1; execution.energy_stable "no" switches it off (extension).
+ * Settings come from the context's JSON (execution.timestep, ensemble,
+ * temp_ctrl_type, temperature, num_steps; init_configuration.read_data.unit)
+ * and the masses from rbmdx_set_masses.  Arrays are the caller's, in caller
+ * order and the context's precision; image flags (int32, may be null) are
+ * updated with every wrap.  The thermostat state stays on the device: a run
+ * of many steps does not synchronise.
+ *
+ * rbmdx_md_init: wrap the positions into the box, evaluate the force of
+ * step 0 into fx..fz and set the thermostat up (NoseHooverController::SetUp).
+ * rbmdx_md_run: nsteps steps; positions, velocities, image flags and forces
+ * are updated in place, the force of the last step is left in fx..fz.  flags:
+ * RBMDX_FLAG_ENERGY computes energies every step (always on with the
+ * energy-stable rescale); other force flags pass through.
+ * rbmdx_md_state: synchronise and read the thermostat state.
+ */
+int rbmdx_md_init(rbmdx_ctx* ctx, void* px, void* py, void* pz, void* vx,
+                  void* vy, void* vz, int32_t* ix, int32_t* iy, int32_t* iz,
+                  void* fx, void* fy, void* fz, uint32_t flags, void* stream);
+int rbmdx_md_run(rbmdx_ctx* ctx, int32_t nsteps, void* px, void* py, void* pz,
+                 void* vx, void* vy, void* vz, int32_t* ix, int32_t* iy,
+                 int32_t* iz, void* fx, void* fy, void* fz, uint32_t flags,
+                 void* stream);
+int rbmdx_md_state(rbmdx_ctx* ctx, rbmdx_md_state_t* out);
 
 /*
  * Host copy of the k-sample the seeded stream yields for force evaluation
@@ -395,6 +454,19 @@ typedef struct rbmdx_data rbmdx_data;
 int rbmdx_data_read(const char* path, const char* atom_style,
                     const int32_t* replicate, rbmdx_data** out);
 void rbmdx_data_destroy(rbmdx_data* data);
+/*
+ * GAUSS velocities (init_configuration.read_data.velocity_type "GAUSS",
+ * RBMD's LammpsFullReader::ApplyGaussianVelocitiesIfRequested,
+ * src/reader/src/format/lammps/lammps_full_reader.cpp:710-866): per atom
+ * std::mt19937_64 seeded by MixVelocitySeed(seed, id + 1), three
+ * std::normal_distribution draws scaled by 1/sqrt(m), centre-of-mass velocity
+ * removed, rescaled to `temperature` with 3N - 3 degrees of freedom in the
+ * unit style `unit` ("LJ", "REAL", "METAL").  Masses from the Masses section;
+ * nothing changes when the file had a Velocities section.  Read the result
+ * with rbmdx_data_get.
+ */
+int rbmdx_data_gauss_velocities(rbmdx_data* data, double temperature, int32_t seed,
+                                 const char* unit);
 const char* rbmdx_data_last_error(const rbmdx_data* data);
 /* flags: 1 Pair Coeffs present, 2 Velocities present, 4 bonds present */
 int rbmdx_data_info(const rbmdx_data* data, int64_t* natoms, int32_t* ntypes,

@@ -87,6 +87,30 @@ class Thermo(ctypes.Structure):
         }
 
 
+class MdState(ctypes.Structure):
+    _fields_ = [
+        ("step", ctypes.c_int64),
+        ("mode", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("temperature", ctypes.c_double),
+        ("ke", ctypes.c_double),
+        ("pe", ctypes.c_double),
+        ("t_target", ctypes.c_double),
+        ("xi", ctypes.c_double),
+        ("eta", ctypes.c_double * 3),
+        ("eta_dot", ctypes.c_double * 3),
+        ("eta_mass", ctypes.c_double * 3),
+        ("thermostat_energy", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        out = {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+        for f in ("eta", "eta_dot", "eta_mass"):
+            out[f] = list(out[f])
+        out["mode"] = ("NVE", "NVE-energy-stable", "NVT-Nose-Hoover")[self.mode]
+        return out
+
+
 # Every symbol include/rbmdx.h declares, with its ctypes signature.
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -115,6 +139,11 @@ SIGNATURES = {
     "rbmdx_set_masses": (ctypes.c_int, [_P, _DP]),
     "rbmdx_force_leapfrog": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                             ctypes.c_double, ctypes.c_uint32, _P]),
+    "rbmdx_md_init": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                     ctypes.c_uint32, _P]),
+    "rbmdx_md_run": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                    _P, ctypes.c_uint32, _P]),
+    "rbmdx_md_state": (ctypes.c_int, [_P, ctypes.POINTER(MdState)]),
     "rbmdx_rbe_sample_host": (ctypes.c_int, [ctypes.c_double, _DP, _I32, ctypes.c_uint32,
                                              _I64, _IP]),
     "rbmdx_check_config": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _I32]),
@@ -138,6 +167,8 @@ SIGNATURES = {
     "rbmdx_data_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _IP,
                                        ctypes.POINTER(_P)]),
     "rbmdx_data_destroy": (None, [_P]),
+    "rbmdx_data_gauss_velocities": (ctypes.c_int, [_P, ctypes.c_double, _I32,
+                                                   ctypes.c_char_p]),
     "rbmdx_data_last_error": (ctypes.c_char_p, [_P]),
     "rbmdx_data_info": (ctypes.c_int, [_P, ctypes.POINTER(_I64), _IP, _IP, _IP]),
     "rbmdx_data_get": (ctypes.c_int, [_P, _DP, _DP, _IP, _IP, _DP, _DP, _DP, _IP, _DP, _DP,
@@ -359,6 +390,40 @@ class ForceContext:
                                                   ctypes.c_void_p(stream)))
         self.calls += 1
 
+    def _md_args(self, x, v, img, f):
+        ptrs = [_tensor_ptr(t) for t in (*x, *v)]
+        ptrs += [None if img is None else _tensor_ptr(t) for t in (img if img else (None,) * 3)]
+        ptrs += [_tensor_ptr(t) for t in f]
+        return ptrs
+
+    def md_init(self, x, v, f, image=None, energy=False, stream=None):
+        """Molecular-dynamics start: wrap x = (px, py, pz) into the box (image
+        = (ix, iy, iz) int32 tensors updated, or None), evaluate the force of
+        step 0 into f and set the thermostat up.  Settings come from the
+        configuration's execution section; masses from set_masses."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(x[0].device).cuda_stream
+        flags = RBMDX_FLAG_ENERGY if energy else 0
+        self._check(self.lib.rbmdx_md_init(self._h, *self._md_args(x, v, image, f), flags,
+                                           ctypes.c_void_p(stream)))
+
+    def md_run(self, nsteps, x, v, f, image=None, energy=False, stream=None):
+        """nsteps velocity-Verlet steps with the configured thermostat (in place)."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(x[0].device).cuda_stream
+        flags = RBMDX_FLAG_ENERGY if energy else 0
+        self._check(self.lib.rbmdx_md_run(self._h, int(nsteps),
+                                          *self._md_args(x, v, image, f), flags,
+                                          ctypes.c_void_p(stream)))
+        self.calls += int(nsteps)
+
+    def md_state(self):
+        s = MdState()
+        self._check(self.lib.rbmdx_md_state(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
     def thermo(self):
         t = Thermo()
         self._check(self.lib.rbmdx_thermo(self._h, ctypes.byref(t)))
@@ -508,12 +573,15 @@ class BatchContext:
         self._check(self.lib.rbmdx_batch_set_shape(self._h, _iptr(s)))
 
 
-def read_data(path, atom_style="full", replicate=None):
+def read_data(path, atom_style="full", replicate=None, velocity_type=None,
+              temperature=None, velocity_seed=12345, unit="LJ"):
     """A LAMMPS data file as RBMD reads it (rbmdx_data_read): a
     ``systems.System`` with 0-based ids and types, per-type eps / sigma from
     Pair Coeffs and masses, velocities and image flags, and the special-bond
     topology (``nspecial`` (N, 3), ``special`` (N, maxspecial)) built from the
-    bonds, angles and dihedrals."""
+    bonds, angles and dihedrals.  ``velocity_type="GAUSS"`` creates velocities
+    at ``temperature`` from ``velocity_seed`` when the file has none
+    (rbmdx_data_gauss_velocities)."""
     from . import systems
 
     lib = load_library()
@@ -524,6 +592,14 @@ def read_data(path, atom_style="full", replicate=None):
     try:
         if rc != 0:
             raise RbmdxError(rc, lib.rbmdx_data_last_error(h).decode() if h else "")
+        if velocity_type is not None and velocity_type.upper() not in ("NONE", ""):
+            if velocity_type.upper() not in ("GAUSS", "GAUSSIAN"):
+                raise RbmdxError(1, "Unsupported velocity_type: " + velocity_type)
+            rc = lib.rbmdx_data_gauss_velocities(
+                h, float(temperature if temperature is not None else float("nan")),
+                int(velocity_seed), unit.encode())
+            if rc != 0:
+                raise RbmdxError(rc, lib.rbmdx_data_last_error(h).decode())
         n, nt, ms, fl = ctypes.c_int64(0), ctypes.c_int32(0), ctypes.c_int32(0), \
             ctypes.c_int32(0)
         lib.rbmdx_data_info(h, ctypes.byref(n), ctypes.byref(nt), ctypes.byref(ms),

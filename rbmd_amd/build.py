@@ -46,6 +46,8 @@ def build(verbose=False):
                                  os.path.join(CSRC, "kernels.hip")],
         [hipcc] + HIPCC_FLAGS + ["-c", "-o", os.path.join(BUILD, "batch_kernels.o"),
                                  os.path.join(CSRC, "batch.hip")],
+        [hipcc] + HIPCC_FLAGS + ["-c", "-o", os.path.join(BUILD, "md_kernels.o"),
+                                 os.path.join(CSRC, "md.hip")],
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "batch.o"),
                                os.path.join(CSRC, "batch.cpp")],
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "lammps_data.o"),
@@ -61,7 +63,8 @@ def build(verbose=False):
     ]
     with ThreadPoolExecutor(max_workers=8) as ex:
         list(ex.map(_run, jobs))
-    objs = [os.path.join(BUILD, f) for f in ("kernels.o", "batch_kernels.o", "api.o",
+    objs = [os.path.join(BUILD, f) for f in ("kernels.o", "batch_kernels.o",
+                                             "md_kernels.o", "api.o",
                                              "batch.o", "lammps_data.o", "data_api.o",
                                              "sampler.o", "sog.o")]
     tmp = LIB + ".tmp"

@@ -28,6 +28,7 @@
 #include "json.hpp"
 #include "rbmdx.h"
 #include "sampler.hpp"
+#include "md.hpp"
 #include "sog.hpp"
 
 using namespace rbmdx;
@@ -37,6 +38,12 @@ namespace {
 // ---------------------------------------------------------------------------
 // Engine
 // ---------------------------------------------------------------------------
+// Caller arrays of a molecular-dynamics call.
+struct MdCall {
+  void *px, *py, *pz, *vx, *vy, *vz, *fx, *fy, *fz;
+  int32_t *ix, *iy, *iz;
+};
+
 struct EngineBase {
   virtual ~EngineBase() = default;
   virtual void force(int64_t step, const void* px, const void* py,
@@ -53,6 +60,10 @@ struct EngineBase {
                               void* fz, double dt, uint32_t flags,
                               hipStream_t s) = 0;
   virtual void set_masses(const double* m) = 0;
+  virtual void md_init(const MdCall& c, uint32_t flags, hipStream_t s) = 0;
+  virtual void md_run(int32_t nsteps, const MdCall& c, uint32_t flags,
+                      hipStream_t s) = 0;
+  virtual void md_state(rbmdx_md_state_t* out) = 0;
   virtual void thermo(rbmdx_thermo_t* out) = 0;
   virtual int64_t export_pairs(int64_t step, const void* px, const void* py,
                                const void* pz, int candidates, int32_t* gi,
@@ -220,6 +231,135 @@ class Engine : public EngineBase {
     }
     HIPCHK(hipMemcpy(d_.inv_mass, inv.data(), sizeof(R) * ntypes_,
                      hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(md_mass_, m, sizeof(double) * ntypes_, hipMemcpyHostToDevice));
+  }
+
+  // --- molecular-dynamics step (md.hip) ------------------------------------
+  MdArrays<R> md_arrays(const MdCall& c) const {
+    if (!c.px || !c.py || !c.pz || !c.vx || !c.vy || !c.vz || !c.fx || !c.fy ||
+        !c.fz)
+      throw Error(RBMDX_E_ARG, "null position, velocity or force pointer");
+    MdArrays<R> a;
+    a.px = (R*)c.px;
+    a.py = (R*)c.py;
+    a.pz = (R*)c.pz;
+    a.vx = (R*)c.vx;
+    a.vy = (R*)c.vy;
+    a.vz = (R*)c.vz;
+    a.fx = (const R*)c.fx;
+    a.fy = (const R*)c.fy;
+    a.fz = (const R*)c.fz;
+    a.ix = c.ix;
+    a.iy = c.iy;
+    a.iz = c.iz;
+    a.type = d_.type;
+    a.mass = md_mass_;
+    for (int d = 0; d < 3; ++d) {
+      a.lo[d] = lo_[d];
+      a.L[d] = L_[d];
+      a.hi[d] = lo_[d] + L_[d];
+    }
+    a.n = n_;
+    a.part = md_part_;
+    a.thermo = d_.thermo;
+    return a;
+  }
+
+  uint32_t md_force_flags(uint32_t flags) const {
+    return flags | (md_.mode == kMdNveStable ? (uint32_t)RBMDX_FLAG_ENERGY : 0u);
+  }
+
+  void md_init(const MdCall& c, uint32_t flags, hipStream_t s) override {
+    if (!ready_) throw Error(RBMDX_E_STATE, "rbmdx_set_system first");
+    if (n_ < 2) throw Error(RBMDX_E_ARG, "molecular dynamics needs at least 2 atoms");
+    HIPCHK(hipSetDevice(device_));
+    const MdConfig m = parse_md(cfg_);
+    MdParams p;
+    p.mode = m.nose_hoover ? kMdNvtNh : (m.energy_stable ? kMdNveStable : kMdNve);
+    p.has_k = (kspace_ != 0 && d_.nc > 0) ? 1 : 0;
+    p.natoms = n_;
+    p.dt = m.dt;
+    p.ftm2v = cfg_.ftm2v;
+    p.mvv2e = m.mvv2e;
+    p.kb = m.kb;
+    p.tdof = 3.0 * (double)n_ - 3.0;
+    p.t_start = m.t_start;
+    p.t_stop = m.t_stop;
+    p.t_freq = m.nose_hoover ? 1.0 / m.t_damp : 0.0;
+    p.num_steps = m.num_steps;
+    p.e_self = e_self_;
+    if (!md_state_) {
+      md_state_ = alloc<MdState>(2);
+      md_part_ = alloc<double>(kMdParts);
+    }
+    const MdArrays<R> a = md_arrays(c);
+    md_ = p;
+    md_ready_ = false;
+    HIPCHK(hipMemsetAsync(md_state_, 0, 2 * sizeof(MdState), s));
+    md_cur_ = 0;
+    HIPCHK(md_launch_wrap(a, s));
+    const uint32_t ff = md_force_flags(flags);
+    force(0, c.px, c.py, c.pz, c.fx, c.fy, c.fz, nullptr, 0, ff, s);
+    HIPCHK(md_launch_second(a, md_, 0, s));
+    HIPCHK(md_launch_thermostat(md_, md_part_, d_.thermo, md_state_ + md_cur_,
+                                md_state_ + (md_cur_ ^ 1), 0, 1,
+                                (ff & RBMDX_FLAG_ENERGY) ? 1 : 0, s));
+    md_cur_ ^= 1;
+    md_step_ = 0;
+    md_ready_ = true;
+  }
+
+  void md_run(int32_t nsteps, const MdCall& c, uint32_t flags,
+              hipStream_t s) override {
+    if (!md_ready_) throw Error(RBMDX_E_STATE, "rbmdx_md_init first");
+    if (nsteps < 0) throw Error(RBMDX_E_ARG, "nsteps must be >= 0");
+    HIPCHK(hipSetDevice(device_));
+    const MdArrays<R> a = md_arrays(c);
+    const uint32_t ff = md_force_flags(flags);
+    const int with_pe = (ff & RBMDX_FLAG_ENERGY) ? 1 : 0;
+    for (int32_t k = 0; k < nsteps; ++k) {
+      const int64_t step = md_step_ + 1;
+      HIPCHK(md_launch_first(a, md_, md_state_ + md_cur_, md_state_ + (md_cur_ ^ 1),
+                             step, s));
+      md_cur_ ^= 1;
+      force(step, c.px, c.py, c.pz, c.fx, c.fy, c.fz, nullptr, 0, ff, s);
+      HIPCHK(md_launch_second(a, md_, 1, s));
+      HIPCHK(md_launch_thermostat(md_, md_part_, d_.thermo, md_state_ + md_cur_,
+                                  md_state_ + (md_cur_ ^ 1), step, 0, with_pe, s));
+      md_cur_ ^= 1;
+      md_step_ = step;
+    }
+    // leave the caller's velocities with the last thermostat factor applied
+    HIPCHK(md_launch_apply(a, md_state_ + md_cur_, md_state_ + (md_cur_ ^ 1), s));
+    md_cur_ ^= 1;
+  }
+
+  void md_state(rbmdx_md_state_t* out) override {
+    if (!md_ready_) throw Error(RBMDX_E_STATE, "rbmdx_md_init first");
+    HIPCHK(hipSetDevice(device_));
+    if (last_stream_set_) HIPCHK(hipStreamSynchronize(last_stream_));
+    MdState st;
+    HIPCHK(hipMemcpy(&st, md_state_ + md_cur_, sizeof(MdState), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->step = md_step_;
+    out->mode = md_.mode;
+    out->temperature = st.T;
+    out->ke = st.ke;
+    out->pe = st.pe;
+    out->t_target = st.t_target;
+    out->xi = md_.mode == kMdNveStable ? st.xi : 1.0;
+    // FixNH-style chain energy: ke_target eta_0 + kT sum_{i>0} eta_i +
+    // sum Q_i eta_dot_i^2 / 2
+    const double kt = md_.kb * st.t_target;
+    double e = 0.0;
+    for (int q = 0; q < 3; ++q) {
+      out->eta[q] = st.eta[q];
+      out->eta_dot[q] = st.eta_dot[q];
+      out->eta_mass[q] = st.eta_mass[q];
+      e += 0.5 * st.eta_mass[q] * st.eta_dot[q] * st.eta_dot[q];
+      e += (q == 0 ? md_.tdof * kt : kt) * st.eta[q];
+    }
+    out->thermostat_energy = md_.mode == kMdNvtNh ? e : 0.0;
   }
 
   void force_leapfrog(int64_t step, void* px, void* py, void* pz, void* vx,
@@ -708,6 +848,15 @@ class Engine : public EngineBase {
   unsigned rebuild_thr_ = 0;
   bool lists_stale_ = true;  // lists (re)allocated since the last rebuild
   double leap_dt_ = 0;        // fused leapfrog time step of the current call
+  // molecular-dynamics step: per-type masses (FP64), the two alternating
+  // thermostat states, kinetic-energy partials, settings and progress
+  double* md_mass_ = nullptr;
+  MdState* md_state_ = nullptr;
+  double* md_part_ = nullptr;
+  MdParams md_;
+  int md_cur_ = 0;
+  int64_t md_step_ = 0;
+  bool md_ready_ = false;
 
   template <typename T>
   T* alloc(size_t n) {
@@ -994,11 +1143,17 @@ class Engine : public EngineBase {
     d_.cnt_a = alloc<int>(n_);
     d_.cnt_b = alloc<int>(nc);
     d_.inv_mass = alloc<R>(ntypes_);
+    md_mass_ = alloc<double>(ntypes_);
     {
       std::vector<R> one(ntypes_, (R)1);
       HIPCHK(hipMemcpy(d_.inv_mass, one.data(), sizeof(R) * ntypes_,
                        hipMemcpyHostToDevice));
+      std::vector<double> oned(ntypes_, 1.0);
+      HIPCHK(hipMemcpy(md_mass_, oned.data(), sizeof(double) * ntypes_,
+                       hipMemcpyHostToDevice));
     }
+    md_ready_ = false;
+    md_state_ = nullptr;
     d_.pair_part_blocks = grid_.nbrick * grid_.psub;
     d_.pair_part = alloc<double>((size_t)11 * grid_.nbrick * grid_.psub);
     d_.skin_part = alloc<float>(kSkinBlocksMax);
@@ -1436,6 +1591,29 @@ int rbmdx_force_leapfrog(rbmdx_ctx* ctx, int64_t step, void* px, void* py,
     ctx->eng->force_leapfrog(step, px, py, pz, vx, vy, vz, fx, fy, fz, dt, flags,
                              (hipStream_t)stream);
   });
+}
+
+int rbmdx_md_init(rbmdx_ctx* ctx, void* px, void* py, void* pz, void* vx,
+                  void* vy, void* vz, int32_t* ix, int32_t* iy, int32_t* iz,
+                  void* fx, void* fy, void* fz, uint32_t flags, void* stream) {
+  if (!ctx) return RBMDX_E_ARG;
+  const MdCall c{px, py, pz, vx, vy, vz, fx, fy, fz, ix, iy, iz};
+  return guarded(ctx, [&] { ctx->eng->md_init(c, flags, (hipStream_t)stream); });
+}
+
+int rbmdx_md_run(rbmdx_ctx* ctx, int32_t nsteps, void* px, void* py, void* pz,
+                 void* vx, void* vy, void* vz, int32_t* ix, int32_t* iy,
+                 int32_t* iz, void* fx, void* fy, void* fz, uint32_t flags,
+                 void* stream) {
+  if (!ctx) return RBMDX_E_ARG;
+  const MdCall c{px, py, pz, vx, vy, vz, fx, fy, fz, ix, iy, iz};
+  return guarded(ctx,
+                 [&] { ctx->eng->md_run(nsteps, c, flags, (hipStream_t)stream); });
+}
+
+int rbmdx_md_state(rbmdx_ctx* ctx, rbmdx_md_state_t* out) {
+  if (!ctx || !out) return RBMDX_E_ARG;
+  return guarded(ctx, [&] { ctx->eng->md_state(out); });
 }
 
 int rbmdx_thermo(rbmdx_ctx* ctx, rbmdx_thermo_t* out) {

@@ -1,4 +1,5 @@
 // C ABI of the LAMMPS data reader (rbmdx_data_*, include/rbmdx.h).
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -47,6 +48,26 @@ int rbmdx_data_read(const char* path, const char* atom_style, const int32_t* rep
   });
   *out = c;  // returned even on error so rbmdx_data_last_error can explain
   return rc;
+}
+
+int rbmdx_data_gauss_velocities(rbmdx_data* data, double temperature, int32_t seed,
+                                 const char* unit) {
+  if (!data || !data->d) return RBMDX_E_STATE;
+  const std::string u = unit ? unit : "LJ";
+  double kb = 1.0, mvv2e = 1.0;  // src/common/unit_factor.cpp:23-37
+  if (u == "REAL") {
+    kb = 1.9872067 * std::pow(10.0, -3.0);
+    mvv2e = 1.0 / (4.186 * std::pow(10.0, -4.0));
+  } else if (u == "METAL") {
+    kb = 8.617343e-5;
+    mvv2e = 1.0364269e-4;
+  } else if (u != "LJ") {
+    data->err = "unsupported unit style: " + u;
+    return RBMDX_E_CONFIG;
+  }
+  return guarded_d(data, [&] {
+    rbmdx::gauss_velocities(*data->d, temperature, seed, kb, mvv2e);
+  });
 }
 
 void rbmdx_data_destroy(rbmdx_data* data) { delete data; }

@@ -59,6 +59,8 @@ struct Config {
   // CVFF: 1-2 / 1-3 / 1-4 special-bond weights, hyper_parameters.extend
   // .special_bonds; 1 when absent (read_special_bond_weights, cvff.cpp:303-330)
   double w_special[3] = {1.0, 1.0, 1.0};
+  std::string text;  // the JSON text (its execution section is read by the
+                     // molecular-dynamics step, parse_md)
   bool coulomb() const { return force_type != "LJ/CUT"; }
 };
 
@@ -248,7 +250,91 @@ Config parse_config(const std::string& text) {
       c.seed = (uint32_t)v;
     }
   }
+  c.text = text;
   return c;
+}
+
+// ---------------------------------------------------------------------------
+// Molecular-dynamics step settings: the execution section RBMD's ensembles
+// read (src/simulate_pipeline/src/nvt_ensemble.cpp:66-97,
+// src/simulate_pipeline/src/controller/pressure_controller/
+// nose_hoover_controller.cpp:89-120) and the unit constants
+// (src/common/unit_factor.cpp:23-37).
+// ---------------------------------------------------------------------------
+struct MdConfig {
+  std::string ensemble;        // "NVE" | "NVT"
+  bool nose_hoover = false;    // NVT with temp_ctrl_type NOSE_HOOVER
+  double dt = 0;               // execution.timestep
+  double t_start = 0, t_stop = 0, t_damp = 0;  // execution.temperature
+  int64_t num_steps = 0;       // execution.num_steps (temperature ramp)
+  bool energy_stable = false;  // NVE with RBL lists (nve_ensemble.cpp:81-84)
+  double mvv2e = 1.0, kb = 1.0;
+};
+
+MdConfig parse_md(const Config& c) {
+  Json root;
+  try {
+    root = parse_json(c.text);
+  } catch (const std::exception& e) {
+    throw Error(RBMDX_E_CONFIG, e.what());
+  }
+  MdConfig m;
+  const Json* ex = root.get("execution");
+  if (!ex) throw Error(RBMDX_E_CONFIG, "execution section missing");
+  if (const Json* u = root.at({"init_configuration", "read_data", "unit"})) {
+    const std::string unit = str_of(u, "unit");
+    if (unit == "REAL") {
+      m.kb = 1.9872067 * std::pow(10.0, -3.0);
+      m.mvv2e = 1.0 / (4.186 * std::pow(10.0, -4.0));
+    } else if (unit == "METAL") {
+      m.kb = 8.617343e-5;
+      m.mvv2e = 1.0364269e-4;
+    }
+  }
+  m.ensemble = str_of(ex->get("ensemble"), "execution.ensemble");
+  m.dt = num_of(ex->get("timestep"), "execution.timestep");
+  if (!(m.dt > 0) || !std::isfinite(m.dt))
+    throw Error(RBMDX_E_CONFIG, "execution.timestep must be > 0");
+  if (const Json* it = ex->get("integration_type"))
+    if (str_of(it, "execution.integration_type") != "vv")
+      throw Error(RBMDX_E_CONFIG,
+                  "execution.integration_type not supported by this build "
+                  "(only \"vv\")");
+  if (const Json* cb = ex->get("com_bias"))
+    if (str_of(cb, "execution.com_bias") == "yes")
+      throw Error(RBMDX_E_CONFIG, "execution.com_bias \"yes\" not supported by this build");
+  if (const Json* sh = root.at({"hyper_parameters", "extend", "fix_shake"}))
+    if (sh->kind == Json::Bool && sh->b)
+      throw Error(RBMDX_E_CONFIG, "extend.fix_shake not supported by this build");
+  if (const Json* ns = ex->get("num_steps"))
+    m.num_steps = (int64_t)num_of(ns, "execution.num_steps");
+  if (m.ensemble == "NVE") {
+    m.energy_stable = c.neighbor_type == "RBL";
+    // extension: the energy-stable rescale can be switched off
+    if (const Json* es = ex->get("energy_stable"))
+      m.energy_stable = m.energy_stable && str_of(es, "execution.energy_stable") != "no";
+  } else if (m.ensemble == "NVT") {
+    const std::string tc = str_of(ex->get("temp_ctrl_type"), "execution.temp_ctrl_type");
+    if (tc != "NOSE_HOOVER")
+      throw Error(RBMDX_E_CONFIG,
+                  "execution.temp_ctrl_type not supported by this build: " + tc);
+    m.nose_hoover = true;
+    const Json* t = ex->get("temperature");
+    if (!t || t->kind != Json::Array || t->arr.size() < 3)
+      throw Error(RBMDX_E_CONFIG,
+                  "execution.temperature must be [start, stop, damp]");
+    m.t_start = num_of(&t->arr[0], "execution.temperature");
+    m.t_stop = num_of(&t->arr[1], "execution.temperature");
+    m.t_damp = num_of(&t->arr[2], "execution.temperature");
+    if (!(m.t_start > 0) || !(m.t_stop > 0) || !(m.t_damp > 0))
+      throw Error(RBMDX_E_CONFIG, "execution.temperature entries must be > 0");
+    if (m.t_start != m.t_stop && m.num_steps <= 0)
+      throw Error(RBMDX_E_CONFIG, "a temperature ramp needs execution.num_steps > 0");
+  } else {
+    throw Error(RBMDX_E_CONFIG,
+                "execution.ensemble not supported by this build: " + m.ensemble);
+  }
+  return m;
 }
 
 // RblFullNeighborListBuilder::GetRblParams

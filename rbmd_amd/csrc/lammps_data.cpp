@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <fstream>
+#include <random>
 #include <sstream>
 #include <stdexcept>
 #include <unordered_map>
@@ -306,6 +307,73 @@ void replicate_lammps_data(LammpsData& d, int nx, int ny, int nz) {
           d.vz.push_back(s.vz[i]);
         }
       }
+}
+
+// ApplyGaussianVelocitiesIfRequested and MixVelocitySeed
+// (src/reader/src/format/lammps/lammps_full_reader.cpp:134-143,710-866): the
+// seed mix is a SplitMix64 finaliser over seed + golden-ratio increment + a
+// shifted id; the generator and the normal draws are the C++ standard
+// library's (std::mt19937_64, std::normal_distribution<double>), as there.
+namespace {
+uint64_t velocity_seed(uint64_t seed, uint64_t id1) {
+  uint64_t z = seed + 0x9e3779b97f4a7c15ULL + (id1 << 6U) + (id1 >> 2U);
+  z = (z ^ (z >> 30U)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27U)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31U);
+}
+}  // namespace
+
+void gauss_velocities(LammpsData& d, double temperature, int seed, double kb,
+                      double mvv2e) {
+  if (d.has_velocities) return;
+  if (!(temperature > 0) || !std::isfinite(temperature))
+    throw std::runtime_error("velocity_type GAUSS requires a temperature > 0");
+  if (seed <= 0) throw std::runtime_error("velocity_type GAUSS requires a positive seed");
+  const int64_t n = d.size();
+  if (n < 2) throw std::runtime_error("velocity_type GAUSS needs at least 2 atoms");
+  auto mass_of = [&](int32_t t) {
+    const double m = (t >= 0 && t < (int32_t)d.mass.size()) ? d.mass[t] : 0.0;
+    if (!(m > 0) || !std::isfinite(m))
+      throw std::runtime_error("velocity_type GAUSS requires a positive mass for every atom type");
+    return m;
+  };
+  d.vx.assign(n, 0.0);
+  d.vy.assign(n, 0.0);
+  d.vz.assign(n, 0.0);
+  double mt = 0, p[3] = {0, 0, 0};
+  for (int64_t i = 0; i < n; ++i) {
+    const double m = mass_of(d.type[i]);
+    std::mt19937_64 gen(velocity_seed((uint64_t)seed, (uint64_t)(d.id[i] + 1)));
+    std::normal_distribution<double> normal(0.0, 1.0);
+    const double s = 1.0 / std::sqrt(m);
+    d.vx[i] = normal(gen) * s;
+    d.vy[i] = normal(gen) * s;
+    d.vz[i] = normal(gen) * s;
+    mt += m;
+    p[0] += m * d.vx[i];
+    p[1] += m * d.vy[i];
+    p[2] += m * d.vz[i];
+  }
+  const double b[3] = {p[0] / mt, p[1] / mt, p[2] / mt};
+  double ts = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    d.vx[i] -= b[0];
+    d.vy[i] -= b[1];
+    d.vz[i] -= b[2];
+    ts += mass_of(d.type[i]) *
+          (d.vx[i] * d.vx[i] + d.vy[i] * d.vy[i] + d.vz[i] * d.vz[i]);
+  }
+  ts *= mvv2e;
+  if (!(ts > 0) || !std::isfinite(ts))
+    throw std::runtime_error("velocity_type GAUSS failed to create finite kinetic energy");
+  const double dof = (double)(3 * n - 3);
+  const double scale = std::sqrt(temperature * dof * kb / ts);
+  for (int64_t i = 0; i < n; ++i) {
+    d.vx[i] *= scale;
+    d.vy[i] *= scale;
+    d.vz[i] *= scale;
+  }
+  d.has_velocities = true;
 }
 
 void build_special(const LammpsData& d, std::vector<int32_t>& nspecial,

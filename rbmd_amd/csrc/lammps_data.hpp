@@ -39,6 +39,16 @@ LammpsData read_lammps_data(const std::string& path, const std::string& atom_sty
 // Systems with bonds are rejected.
 void replicate_lammps_data(LammpsData& d, int nx, int ny, int nz);
 
+// GAUSS velocities as RBMD's reader creates them when the configuration asks
+// for velocity_type GAUSS and the file has no Velocities section: per atom a
+// 64-bit Mersenne Twister seeded from (seed, id + 1), three standard normal
+// draws scaled by 1 / sqrt(m), the centre-of-mass velocity removed, then a
+// rescale to `temperature` with 3 N - 3 degrees of freedom.  kb and mvv2e are
+// the unit style's constants.  Throws on a non-positive temperature or seed,
+// or a type without a positive mass.  No-op when the file had velocities.
+void gauss_velocities(LammpsData& d, double temperature, int seed, double kb,
+                      double mvv2e);
+
 // Special-bond topology (1-2, 1-3, 1-4 partners) as RBMD's reader builds
 // it: cumulative counts nspecial[3 i + k] and partner ids special[i * maxs
 // + k] (global ids as in `id`), in caller (file) order; maxs >= 1.
