@@ -481,6 +481,27 @@ int rbmdx_data_get(const rbmdx_data* data, double* box_lo, double* box_len,
  * layout, unused entries -1). */
 int rbmdx_data_special(const rbmdx_data* data, int32_t* nspecial, int32_t* special);
 
+/*
+ * Output formats (host arrays, caller order; ids and types 0-based, written
+ * 1-based; pos / vel / force are N x 3 row-major; image (N x 3) may be null).
+ * rbmdx_write_dump: one LAMMPS-style trajectory frame as RBMD's trajectory
+ * output writes it (src/output/src/TrajectoryOutput.cpp:433-500): header
+ * with TIMESTEP, NUMBER OF ATOMS, BOX BOUNDS pp pp pp, then
+ * "id type q x y z xu yu zu vx vy vz" in id order (xu = x + image L).
+ * rbmdx_write_force_samples: rows "sample,id,x,y,z,fx,fy,fz" in id order, the
+ * header written when append is 0 -- the CSV RBMD's force-accuracy analysis
+ * reads (benchmark/common/analyze_force_samples.py:30-45).
+ * Numbers are the shortest decimals that read back exactly.  append = 0
+ * truncates the file.  Returns RBMDX_E_ARG on bad arguments or I/O failure.
+ */
+int rbmdx_write_dump(const char* path, int32_t append, int64_t timestep, int64_t n,
+                     const double* box_lo, const double* box_len, const int32_t* id,
+                     const int32_t* type, const double* charge, const double* pos,
+                     const int32_t* image, const double* vel);
+int rbmdx_write_force_samples(const char* path, int32_t append, int64_t sample,
+                              int64_t n, const int32_t* id, const double* pos,
+                              const double* force);
+
 #ifdef __cplusplus
 }
 #endif

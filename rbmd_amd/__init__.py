@@ -166,6 +166,10 @@ SIGNATURES = {
     "rbmdx_batch_num_samples": (_I32, [_P]),
     "rbmdx_data_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _IP,
                                        ctypes.POINTER(_P)]),
+    "rbmdx_write_dump": (ctypes.c_int, [ctypes.c_char_p, _I32, _I64, _I64, _DP, _DP, _IP,
+                                        _IP, _DP, _DP, _IP, _DP]),
+    "rbmdx_write_force_samples": (ctypes.c_int, [ctypes.c_char_p, _I32, _I64, _I64, _IP,
+                                                 _DP, _DP]),
     "rbmdx_data_destroy": (None, [_P]),
     "rbmdx_data_gauss_velocities": (ctypes.c_int, [_P, ctypes.c_double, _I32,
                                                    ctypes.c_char_p]),
@@ -626,3 +630,32 @@ def read_data(path, atom_style="full", replicate=None, velocity_type=None,
     s.has_velocities = bool(fl.value & 2)
     s.has_bonds = bool(fl.value & 4)
     return s
+
+
+def write_dump(path, timestep, box_lo, box_len, ids, types, pos, charges=None, image=None,
+               vel=None, append=False):
+    """Append (or write) one LAMMPS-style trajectory frame (rbmdx_write_dump)."""
+    lib = load_library()
+    n = len(ids)
+    c = lambda a, t: None if a is None else np.ascontiguousarray(a, dtype=t)
+    ids_, types_ = c(ids, np.int32), c(types, np.int32)
+    pos_, q_, img_, vel_ = c(pos, np.float64), c(charges, np.float64), c(image, np.int32), \
+        c(vel, np.float64)
+    lo, L = c(box_lo, np.float64), c(box_len, np.float64)
+    p = lambda a: None if a is None else (_iptr(a) if a.dtype == np.int32 else _dptr(a))
+    rc = lib.rbmdx_write_dump(str(path).encode(), int(bool(append)), int(timestep), n, p(lo),
+                              p(L), p(ids_), p(types_), p(q_), p(pos_), p(img_), p(vel_))
+    if rc != 0:
+        raise RbmdxError(rc, f"cannot write {path}")
+
+
+def write_force_samples(path, sample, ids, pos, force, append=False):
+    """Rows sample,id,x,y,z,fx,fy,fz of one sample (rbmdx_write_force_samples)."""
+    lib = load_library()
+    ids_ = np.ascontiguousarray(ids, dtype=np.int32)
+    pos_ = np.ascontiguousarray(pos, dtype=np.float64)
+    f_ = np.ascontiguousarray(force, dtype=np.float64)
+    rc = lib.rbmdx_write_force_samples(str(path).encode(), int(bool(append)), int(sample),
+                                       len(ids_), _iptr(ids_), _dptr(pos_), _dptr(f_))
+    if rc != 0:
+        raise RbmdxError(rc, f"cannot write {path}")

@@ -54,6 +54,8 @@ def build(verbose=False):
                                os.path.join(CSRC, "lammps_data.cpp")],
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "data_api.o"),
                                os.path.join(CSRC, "data_api.cpp")],
+        ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "output.o"),
+                               os.path.join(CSRC, "output.cpp")],
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "api.o"),
                                os.path.join(CSRC, "api.cpp")],
         ["g++"] + CXX_FLAGS + ["-c", "-o", os.path.join(BUILD, "sampler.o"),
@@ -65,7 +67,7 @@ def build(verbose=False):
         list(ex.map(_run, jobs))
     objs = [os.path.join(BUILD, f) for f in ("kernels.o", "batch_kernels.o",
                                              "md_kernels.o", "api.o",
-                                             "batch.o", "lammps_data.o", "data_api.o",
+                                             "batch.o", "lammps_data.o", "data_api.o", "output.o",
                                              "sampler.o", "sog.o")]
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs

@@ -141,7 +141,7 @@ def test_energy_stable_rescale(device):
      "temperature"),
 ])
 def test_unsupported_settings_rejected(device, extra, match):
-    s = systems.lj_fcc(cells=3, seed=1)
+    s = systems.lj_fcc(cells=4, seed=1)
     cfg = _config("NVE")
     cfg["execution"].update(extra)
     ctx, x, v, f, img = _setup(device, cfg, s)
@@ -149,3 +149,56 @@ def test_unsupported_settings_rejected(device, extra, match):
         ctx.md_init(x, v, f, image=img)
     with pytest.raises(RbmdxError, match="md_init"):
         ctx.md_run(1, x, v, f, image=img)
+
+
+def test_driver_md_loop_matches_python(device, tmp_path):
+    """The C++ driver's simulate loop (--md: data file, GAUSS velocities, NVT,
+    thermo lines and rbmd.trj frames) against the same run through Python."""
+    import json
+    import os
+    import subprocess
+
+    import torch
+
+    s = systems.lj_fcc(cells=5, seed=21)
+    s.vel = None
+    systems.write_lammps_data(s, tmp_path / "lj.data", "atomic")
+    cfg = _config("NVT", temp_ctrl_type="NOSE_HOOVER", temperature=[1.2, 1.2, 0.1],
+                  num_steps=20)
+    cfg["init_configuration"]["read_data"].update(
+        {"file": "lj.data", "velocity_type": "GAUSS", "velocity_seed": 99})
+    cfg["outputs"] = {"thermo_out": {"interval": 10}, "trajectory_out": {"interval": 10}}
+    (tmp_path / "rbmd.json").write_text(json.dumps(cfg))
+    drv = os.path.join(os.path.dirname(rbmd_amd.LIB_PATH), "rbmdx_plugin_driver")
+    out = subprocess.run([drv, "--md", str(tmp_path / "rbmd.json")], capture_output=True,
+                         text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    thermo = [ln.split() for ln in out.stdout.splitlines() if ln.startswith("thermo ")]
+    assert [int(t[1]) for t in thermo] == [0, 10, 20]
+    text = (tmp_path / "rbmd.trj").read_text().splitlines()
+    starts = [k for k, ln in enumerate(text) if ln == "ITEM: TIMESTEP"]
+    assert [int(text[k + 1]) for k in starts] == [0, 10, 20]
+    last = np.array([[float(v) for v in ln.split()] for ln in text[starts[-1] + 9:]])
+
+    r = rbmd_amd.read_data(tmp_path / "lj.data", "atomic", velocity_type="GAUSS",
+                           temperature=1.2, velocity_seed=99)
+    ctx = ForceContext(cfg, device=0)
+    ctx.set_system(r.box_lo, r.box_len, r.types, r.ids, r.charges, r.eps, r.sigma)
+    ctx.set_masses(r.mass)
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)
+    x = [t(r.pos[:, d]) for d in range(3)]
+    v = [t(r.vel[:, d]) for d in range(3)]
+    f = [torch.zeros(r.n, dtype=torch.float64, device=device) for _ in range(3)]
+    img = [torch.tensor(r.image[:, d].copy(), device=device) for d in range(3)]
+    ctx.md_init(x, v, f, image=img, energy=True)
+    temps = [ctx.md_state()["temperature"]]
+    for _ in range(2):
+        ctx.md_run(10, x, v, f, image=img, energy=True)
+        temps.append(ctx.md_state()["temperature"])
+    np.testing.assert_allclose([float(t[2]) for t in thermo], temps, rtol=1e-9)
+    o = np.argsort(r.ids)
+    X, V, I = _host(x)[o], _host(v)[o], _host(img)[o]
+    np.testing.assert_array_equal(last[:, 0], r.ids[o] + 1)
+    assert _rel(last[:, 3:6], X) < 1e-12
+    assert _rel(last[:, 6:9], X + I * r.box_len) < 1e-12
+    assert _rel(last[:, 9:12], V) < 1e-12
