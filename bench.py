@@ -64,7 +64,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--workload",
-                    choices=["electrolyte", "adjoint", "replicas", "water", "lj"],
+                    choices=["electrolyte", "adjoint", "replicas", "water", "lj", "md"],
                     default="electrolyte",
                     help="electrolyte: the headline force step (default); adjoint: "
                          "forward and force-matching adjoint on the 2,000,000-particle "
@@ -370,8 +370,72 @@ def bench_lj(args):
     print(json.dumps(line))
 
 
+def bench_md(args):
+    """The whole NVT step of the 200k electrolyte (RBL + RBE, P = 200): the
+    Nose-Hoover chain (temperature [1, 1, 1]), velocity Verlet with image
+    flags and the force, through rbmdx_md_run, against the same number of
+    force-only evaluations.  Positions start from the warmed-up electrolyte
+    (capped-velocity relaxation of the random start, not timed)."""
+    import torch
+
+    from rbmd_amd import ForceContext, systems
+
+    dev = torch.device("cuda:0")
+    s = systems.electrolyte(args.atoms, seed=2)
+    cfg = systems.ljsalt_config()
+    cfg["execution"] = {"ensemble": "NVT", "integration_type": "vv",
+                        "temp_ctrl_type": "NOSE_HOOVER", "temperature": [1.0, 1.0, 1.0],
+                        "timestep": args.dt}
+    ctx = ForceContext(cfg, device=0)
+    ctx.set_system(s.box_lo, s.box_len, s.types, s.ids, s.charges, s.eps, s.sigma)
+    ctx.set_masses(np.ones(len(s.eps)))
+    n, dt = s.n, args.dt
+    x = [torch.tensor(s.pos[:, d].copy(), dtype=torch.float32, device=dev) for d in range(3)]
+    v = [torch.tensor(s.vel[:, d].copy(), dtype=torch.float32, device=dev) * 0.5
+         for d in range(3)]
+    f = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    img = [torch.zeros(n, dtype=torch.int32, device=dev) for _ in range(3)]
+    vmax = 0.05 / dt
+    for k in range(args.warmup):
+        ctx.force(k, *x, *f)
+        for d in range(3):
+            v[d].add_(f[d], alpha=dt).clamp_(-vmax, vmax)
+            x[d].add_(v[d], alpha=dt)
+    ctx.md_init(x, v, f, image=img)
+    ctx.md_run(20, x, v, f, image=img)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.md_run(args.steps, x, v, f, image=img)
+    torch.cuda.synchronize()
+    t_md = time.perf_counter() - t0
+    st = ctx.md_state()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.force(10 ** 6 + k, *x, *f)
+    torch.cuda.synchronize()
+    t_force = time.perf_counter() - t0
+    line = {
+        "metric": "atom-steps/s, NVT Nose-Hoover MD step (integrator + RBL/RBE force), 200k",
+        "value": round(n * args.steps / t_md, 1), "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_md / args.steps, 4),
+        "force_only_ms_per_step": round(1e3 * t_force / args.steps, 4),
+        "higher_is_better": True, "dtype": "f32 (thermostat f64)", "data": "synthetic",
+        "temperature_end": st["temperature"], "thermostat_energy": st["thermostat_energy"],
+        "config": {"workload": f"LJ electrolyte RBL+RBE, {n} atoms, P=200, NVT "
+                               f"Nose-Hoover [1, 1, 1], dt {dt}", "atoms": n},
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
+    if args.workload == "md":
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            sys.exit(2)
+        bench_md(args)
+        return
     if args.workload in ("adjoint", "replicas", "water", "lj"):
         if args.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)
