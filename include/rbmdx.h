@@ -455,6 +455,17 @@ int rbmdx_data_read(const char* path, const char* atom_style,
                     const int32_t* replicate, rbmdx_data** out);
 void rbmdx_data_destroy(rbmdx_data* data);
 /*
+ * Replicate the data read so far nx x ny x nz times (RBMD's ApplyReplicate,
+ * src/reader/src/format/lammps/lammps_full_reader.cpp:899-1050) and rebuild
+ * the special-bond lists.  bond_periodic = 1 (RBMD's default, read_data
+ * "bond_periodic"): topology partners are taken from the tile of their nearest
+ * seed image; 0: positions are unwrapped by their image flags (required on
+ * every atom of molecular data) and partners stay in their tile.
+ * rbmdx_data_read's replicate argument is this call with bond_periodic = 1.
+ */
+int rbmdx_data_replicate(rbmdx_data* data, const int32_t* factors /* [3] */,
+                         int32_t bond_periodic);
+/*
  * GAUSS velocities (init_configuration.read_data.velocity_type "GAUSS",
  * RBMD's LammpsFullReader::ApplyGaussianVelocitiesIfRequested,
  * src/reader/src/format/lammps/lammps_full_reader.cpp:710-866): per atom

@@ -171,6 +171,7 @@ SIGNATURES = {
     "rbmdx_write_force_samples": (ctypes.c_int, [ctypes.c_char_p, _I32, _I64, _I64, _IP,
                                                  _DP, _DP]),
     "rbmdx_data_destroy": (None, [_P]),
+    "rbmdx_data_replicate": (ctypes.c_int, [_P, _IP, _I32]),
     "rbmdx_data_gauss_velocities": (ctypes.c_int, [_P, ctypes.c_double, _I32,
                                                    ctypes.c_char_p]),
     "rbmdx_data_last_error": (ctypes.c_char_p, [_P]),
@@ -578,24 +579,30 @@ class BatchContext:
 
 
 def read_data(path, atom_style="full", replicate=None, velocity_type=None,
-              temperature=None, velocity_seed=12345, unit="LJ"):
+              temperature=None, velocity_seed=12345, unit="LJ", bond_periodic=True):
     """A LAMMPS data file as RBMD reads it (rbmdx_data_read): a
     ``systems.System`` with 0-based ids and types, per-type eps / sigma from
     Pair Coeffs and masses, velocities and image flags, and the special-bond
     topology (``nspecial`` (N, 3), ``special`` (N, maxspecial)) built from the
     bonds, angles and dihedrals.  ``velocity_type="GAUSS"`` creates velocities
     at ``temperature`` from ``velocity_seed`` when the file has none
-    (rbmdx_data_gauss_velocities)."""
+    (rbmdx_data_gauss_velocities).  ``replicate`` tiles the box; with
+    ``bond_periodic`` (RBMD's default) bonded partners come from the tile of
+    their nearest image, otherwise from their own tile after unwrapping by
+    image flags (rbmdx_data_replicate)."""
     from . import systems
 
     lib = load_library()
     h = _P()
     rep = None if replicate is None else np.ascontiguousarray(replicate, dtype=np.int32)
-    rc = lib.rbmdx_data_read(str(path).encode(), atom_style.encode(),
-                             None if rep is None else _iptr(rep), ctypes.byref(h))
+    rc = lib.rbmdx_data_read(str(path).encode(), atom_style.encode(), None, ctypes.byref(h))
     try:
         if rc != 0:
             raise RbmdxError(rc, lib.rbmdx_data_last_error(h).decode() if h else "")
+        if rep is not None:
+            rc = lib.rbmdx_data_replicate(h, _iptr(rep), int(bool(bond_periodic)))
+            if rc != 0:
+                raise RbmdxError(rc, lib.rbmdx_data_last_error(h).decode())
         if velocity_type is not None and velocity_type.upper() not in ("NONE", ""):
             if velocity_type.upper() not in ("GAUSS", "GAUSSIAN"):
                 raise RbmdxError(1, "Unsupported velocity_type: " + velocity_type)

@@ -50,6 +50,19 @@ int rbmdx_data_read(const char* path, const char* atom_style, const int32_t* rep
   return rc;
 }
 
+int rbmdx_data_replicate(rbmdx_data* data, const int32_t* factors, int32_t bond_periodic) {
+  if (!data || !data->d) return RBMDX_E_STATE;
+  if (!factors) return RBMDX_E_ARG;
+  return guarded_d(data, [&] {
+    rbmdx::LammpsData d = *data->d;
+    rbmdx::replicate_lammps_data(d, factors[0], factors[1], factors[2], bond_periodic != 0);
+    for (int64_t v : d.id)
+      if (v > 0x7fffffffLL) throw std::runtime_error("atom ids beyond 2^31 - 1");
+    rbmdx::build_special(d, data->nspecial, data->special, data->maxs);
+    *data->d = std::move(d);
+  });
+}
+
 int rbmdx_data_gauss_velocities(rbmdx_data* data, double temperature, int32_t seed,
                                  const char* unit) {
   if (!data || !data->d) return RBMDX_E_STATE;

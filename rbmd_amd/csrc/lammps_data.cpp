@@ -183,6 +183,8 @@ LammpsData read_lammps_data(const std::string& path, const std::string& atom_sty
         int im[3] = {0, 0, 0};
         if (is >> im[0]) {
           if (!(is >> im[1] >> im[2])) fail(path, lno, "incomplete image flags");
+        } else {
+          d.image_flags_all = false;
         }
         if (id <= 0) fail(path, lno, "atom id must be positive");
         if (type <= 0 || type > d.ntypes) fail(path, lno, "atom type out of range");
@@ -247,17 +249,48 @@ LammpsData read_lammps_data(const std::string& path, const std::string& atom_sty
   return d;
 }
 
-void replicate_lammps_data(LammpsData& d, int nx, int ny, int nz) {
+namespace {
+// Replica tile of a topology partner (BondTargetReplica,
+// src/reader/include/core/periodic_replica.h:10-38): the partner's tile
+// is the anchor's tile shifted by the partner's nearest image in the seed
+// box (no shift at exactly half a box).
+int64_t partner_tile(const LammpsData& s, int64_t ia, int64_t it, int64_t anchor_tile,
+                     const int n[3]) {
+  const double from[3] = {s.x[ia], s.y[ia], s.z[ia]};
+  const double to[3] = {s.x[it], s.y[it], s.z[it]};
+  int64_t t[3];
+  for (int d = 0; d < 3; ++d) {
+    const int64_t cur = anchor_tile % n[d];
+    anchor_tile /= n[d];
+    const double ratio = (to[d] - from[d]) / s.L[d];
+    if (!std::isfinite(ratio) || std::abs(ratio) > 1e15)
+      throw std::runtime_error("invalid coordinate in periodic replication");
+    const long long shift = ratio > 0.5    ? -(long long)std::ceil(ratio - 0.5)
+                            : ratio < -0.5 ? -(long long)std::floor(ratio + 0.5)
+                                           : 0;
+    int64_t w = (cur + shift) % n[d];
+    if (w < 0) w += n[d];
+    t[d] = w;
+  }
+  return (t[2] * n[1] + t[1]) * n[0] + t[0];
+}
+}  // namespace
+
+void replicate_lammps_data(LammpsData& d, int nx, int ny, int nz, bool bond_periodic) {
   if (nx < 1 || ny < 1 || nz < 1) throw std::runtime_error("replicate factors must be >= 1");
   if (nx * ny * nz == 1) return;
-  if (!d.bonds.empty() || !d.angles.empty() || !d.dihedrals.empty())
-    throw std::runtime_error("replicate of a system with bonds, angles or dihedrals is "
-                             "not supported by this reader");
+  const bool topo = !d.bonds.empty() || !d.angles.empty() || !d.dihedrals.empty();
+  if (!bond_periodic && (topo || d.atom_style == "full") && !d.image_flags_all)
+    throw std::runtime_error("replicate of molecular data without bond_periodic needs "
+                             "image flags (ix iy iz) on every Atoms line");
   const LammpsData s = d;
   int64_t span = 0;
   int32_t mspan = 0;
   for (int64_t v : s.id) span = std::max(span, v + 1);
   for (int32_t m : s.mol) mspan = std::max(mspan, m + 1);
+  std::unordered_map<int64_t, int64_t> index_of;
+  if (topo && bond_periodic)
+    for (int64_t i = 0; i < s.size(); ++i) index_of.emplace(s.id[i], i);
   const int n[3] = {nx, ny, nz};
   for (int k = 0; k < 3; ++k) d.L[k] = s.L[k] * n[k];
   const int64_t ns = s.size();
@@ -272,6 +305,9 @@ void replicate_lammps_data(LammpsData& d, int nx, int ny, int nz) {
   d.ix.clear();
   d.iy.clear();
   d.iz.clear();
+  d.bonds.clear();
+  d.angles.clear();
+  d.dihedrals.clear();
   auto remap = [](double v, double lo, double L, int32_t& img) {
     long long i = (long long)std::floor((v - lo) / L);
     double w = v - (double)i * L;
@@ -286,6 +322,18 @@ void replicate_lammps_data(LammpsData& d, int nx, int ny, int nz) {
     img = (int32_t)i;
     return w;
   };
+  // partner ids of one replica (SameReplicaAtomId / BondPeriodicAtomId /
+  // BondPeriodicChainEnd, lammps_full_reader.cpp:352-383,988-1040)
+  auto same = [&](int64_t a, int64_t r) { return a + r * span; };
+  auto idx = [&](int64_t a) {
+    auto it = index_of.find(a);
+    if (it == index_of.end()) throw std::runtime_error("topology names an unknown atom");
+    return it->second;
+  };
+  auto partner = [&](int64_t anchor, int64_t target, int64_t r) {
+    if (!bond_periodic) return same(target, r);
+    return same(target, partner_tile(s, idx(anchor), idx(target), r, n));
+  };
   for (int kz = 0; kz < nz; ++kz)
     for (int ky = 0; ky < ny; ++ky)
       for (int kx = 0; kx < nx; ++kx) {
@@ -295,16 +343,31 @@ void replicate_lammps_data(LammpsData& d, int nx, int ny, int nz) {
           d.type.push_back(s.type[i]);
           d.mol.push_back(s.mol[i] >= 0 ? (int32_t)(s.mol[i] + r * mspan) : -1);
           d.q.push_back(s.q[i]);
+          // BuildReplicatedAtom (lammps_full_reader.cpp:386-421): tile shift,
+          // plus the image-flag unwrap when bonds do not pick partner tiles
+          const double u[3] = {bond_periodic ? 0.0 : s.ix[i] * s.L[0],
+                               bond_periodic ? 0.0 : s.iy[i] * s.L[1],
+                               bond_periodic ? 0.0 : s.iz[i] * s.L[2]};
           int32_t im[3];
-          d.x.push_back(remap(s.x[i] + kx * s.L[0], d.lo[0], d.L[0], im[0]));
-          d.y.push_back(remap(s.y[i] + ky * s.L[1], d.lo[1], d.L[1], im[1]));
-          d.z.push_back(remap(s.z[i] + kz * s.L[2], d.lo[2], d.L[2], im[2]));
+          d.x.push_back(remap(s.x[i] + u[0] + kx * s.L[0], d.lo[0], d.L[0], im[0]));
+          d.y.push_back(remap(s.y[i] + u[1] + ky * s.L[1], d.lo[1], d.L[1], im[1]));
+          d.z.push_back(remap(s.z[i] + u[2] + kz * s.L[2], d.lo[2], d.L[2], im[2]));
           d.ix.push_back(im[0]);
           d.iy.push_back(im[1]);
           d.iz.push_back(im[2]);
           d.vx.push_back(s.vx[i]);
           d.vy.push_back(s.vy[i]);
           d.vz.push_back(s.vz[i]);
+        }
+        for (const auto& b : s.bonds)
+          d.bonds.push_back({same(b[0], r), partner(b[0], b[1], r)});
+        for (const auto& a : s.angles)
+          d.angles.push_back({partner(a[1], a[0], r), same(a[1], r), partner(a[1], a[2], r)});
+        for (const auto& h : s.dihedrals) {
+          const int64_t a3 = partner(h[1], h[2], r);
+          const int64_t a4 = bond_periodic ? partner(h[2], h[3], (a3 - h[2]) / span)
+                                           : same(h[3], r);
+          d.dihedrals.push_back({partner(h[1], h[0], r), same(h[1], r), a3, a4});
         }
       }
 }

@@ -23,6 +23,7 @@ struct LammpsData {
   std::vector<int32_t> ix, iy, iz;  // image flags (0 when absent)
   std::vector<double> vx, vy, vz;   // Velocities section (0 when absent)
   bool has_velocities = false;
+  bool image_flags_all = true;      // every Atoms line carried ix iy iz
   // topology, atom ids 0-based
   std::vector<std::array<int64_t, 2>> bonds;
   std::vector<std::array<int64_t, 3>> angles;
@@ -33,11 +34,17 @@ struct LammpsData {
 // Parse a data file (throws std::runtime_error naming the file and line).
 LammpsData read_lammps_data(const std::string& path, const std::string& atom_style);
 
-// Tile an orthogonal box nx x ny x nz times: copy r = (iz ny + iy) nx + ix
-// of atom id a gets id a + r * span (span = largest id + 1), positions
-// shifted by the tile and folded into the new box (image flags updated).
-// Systems with bonds are rejected.
-void replicate_lammps_data(LammpsData& d, int nx, int ny, int nz);
+// Tile an orthogonal box nx x ny x nz times (RBMD's ApplyReplicate,
+// src/reader/src/format/lammps/lammps_full_reader.cpp:899-1050): copy
+// r = (iz ny + iy) nx + ix of atom id a gets id a + r * span (span = largest
+// id + 1), molecules likewise; positions shifted by the tile and folded into
+// the new box (image flags updated).  Bonds, angles and dihedrals are copied
+// per tile; with bond_periodic (RBMD's default) each partner is taken from
+// the tile its nearest seed image lies in, so bonds that crossed the seed
+// box's edge stay short; without it positions are first unwrapped by their
+// image flags (required on every atom) and partners stay in their tile.
+void replicate_lammps_data(LammpsData& d, int nx, int ny, int nz,
+                           bool bond_periodic = true);
 
 // GAUSS velocities as RBMD's reader creates them when the configuration asks
 // for velocity_type GAUSS and the file has no Velocities section: per atom a

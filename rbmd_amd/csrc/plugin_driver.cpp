@@ -86,6 +86,16 @@ static int run(const std::string& json, const std::string& fftype, DeviceData& d
   return 0;
 }
 
+// read_data "replicate_bond_periodic", "bond_periodic" or "bond/periodic",
+// the first present (default on, as RBMD's
+// LammpsFullReader, lammps_full_reader.cpp:680-707).
+static int32_t bond_periodic(const rbmdx::Json* rd) {
+  for (const char* key : {"replicate_bond_periodic", "bond_periodic", "bond/periodic"})
+    if (const rbmdx::Json* b = rd->get(key))
+      if (b->kind == rbmdx::Json::Bool) return b->b ? 1 : 0;
+  return 1;
+}
+
 // RBMD's input: the JSON and the data file it names.
 static int run_rbmd(const char* json_path, int steps) {
   const std::string json = slurp(json_path);
@@ -120,8 +130,8 @@ static int run_rbmd(const char* json_path, int steps) {
     if (const rbmdx::Json* ff = hp->get("force_field"))
       if (const rbmdx::Json* t = ff->get("type")) fftype = t->str;
   rbmdx_data* data = nullptr;
-  if (rbmdx_data_read(path.c_str(), style.c_str(), has_rep ? rep : nullptr, &data) !=
-      RBMDX_OK) {
+  if (rbmdx_data_read(path.c_str(), style.c_str(), nullptr, &data) != RBMDX_OK ||
+      (has_rep && rbmdx_data_replicate(data, rep, bond_periodic(rd)) != RBMDX_OK)) {
     std::fprintf(stderr, "error: %s\n", rbmdx_data_last_error(data));
     rbmdx_data_destroy(data);
     return 1;
@@ -269,7 +279,8 @@ static int run_md(const char* json_path, int64_t steps_override) {
   if (r && r->kind == rbmdx::Json::Array && r->arr.size() == 3)
     for (int k = 0; k < 3; ++k) rep[k] = (int32_t)r->arr[k].num;
   rbmdx_data* data = nullptr;
-  if (rbmdx_data_read(path.c_str(), style.c_str(), r ? rep : nullptr, &data) != RBMDX_OK) {
+  if (rbmdx_data_read(path.c_str(), style.c_str(), nullptr, &data) != RBMDX_OK ||
+      (r && rbmdx_data_replicate(data, rep, bond_periodic(rd)) != RBMDX_OK)) {
     std::fprintf(stderr, "error: %s\n", rbmdx_data_last_error(data));
     rbmdx_data_destroy(data);
     return 1;

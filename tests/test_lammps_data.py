@@ -130,12 +130,72 @@ def test_replicate_charge_style(tmp_path):
         d = r.pos[blk] - (s.pos + shift)
         np.testing.assert_allclose(d - np.round(d / r.box_len) * r.box_len, 0, atol=1e-9)
         assert (r.pos[blk] >= 0).all() and (r.pos[blk] < r.box_len).all()
-    # a box with bonds is not replicated by this reader
-    w = systems.spce_water(8, seed=1)
-    bonds, angles = systems.water_topology(8)
+
+
+def _bond_lengths(r):
+    """Minimum-image lengths of every 1-2 pair of the special lists."""
+    pos = {int(i): r.pos[k] for k, i in enumerate(r.ids)}
+    out = []
+    for k in range(r.n):
+        for j in r.special[k][:r.nspecial[k][0]]:
+            d = pos[int(j)] - r.pos[k]
+            d -= np.round(d / r.box_len) * r.box_len
+            out.append(np.linalg.norm(d))
+    return np.array(out)
+
+
+def test_replicate_bonded_water_bond_periodic(tmp_path):
+    """Partners come from the tile of their nearest seed image: every bond of
+    the tiled box stays a 1 A O-H bond, also those that crossed the seed edge."""
+    w = systems.spce_water(60, seed=4)
+    # shift so the first oxygen sits just below the upper edge, then wrap:
+    # molecules are cut by the box edge
+    w.pos = np.mod(w.pos - w.pos[0] + w.box_len - 0.05, w.box_len) + w.box_lo
+    bonds, angles = systems.water_topology(60)
+    raw = np.array([np.abs(w.pos[a] - w.pos[b]).max() for a, b in bonds])
+    assert (raw > w.box_len[0] / 2).any()  # some bonds cross the seed box edge
+    p = tmp_path / "w.data"
     systems.write_lammps_data(w, p, "full", bonds=bonds, angles=angles)
-    with pytest.raises(RbmdxError, match="replicate"):
-        rbmd_amd.read_data(p, "full", replicate=[2, 1, 1])
+    r = rbmd_amd.read_data(p, "full", replicate=[2, 1, 3])
+    assert r.n == 6 * w.n
+    np.testing.assert_allclose(r.box_len, w.box_len * [2, 1, 3])
+    census = {}
+    for t, ns in zip(r.types, map(tuple, r.nspecial)):
+        census.setdefault(int(t), set()).add(ns)
+    assert census == {0: {(2, 2, 2)}, 1: {(1, 2, 2)}}
+    lengths = _bond_lengths(r)
+    assert len(lengths) == 4 * 60 * 6 and np.abs(lengths - 1.0).max() < 1e-6
+    # the same tiling with bond_periodic off needs image flags on every atom
+    with pytest.raises(RbmdxError, match="image flags"):
+        rbmd_amd.read_data(p, "full", replicate=[2, 1, 1], bond_periodic=False)
+
+
+def test_replicate_bonded_water_image_flags(tmp_path):
+    """Without bond_periodic, positions are unwrapped by image flags and
+    partners stay in their own tile: molecules written whole (image flags
+    that undo the wrap) tile into whole molecules."""
+    w = systems.spce_water(60, seed=4)
+    w.pos = np.mod(w.pos - w.pos[0] + w.box_len - 0.05, w.box_len) + w.box_lo
+    bonds, angles = systems.water_topology(60)
+    unwrapped = w.pos.copy()
+    for a, b in bonds:  # hydrogens next to their oxygen
+        d = w.pos[b] - w.pos[a]
+        unwrapped[b] = w.pos[a] + d - np.round(d / w.box_len) * w.box_len
+    image = np.floor((unwrapped - w.box_lo) / w.box_len).astype(np.int32)
+    w.pos = unwrapped - image * w.box_len
+    p = tmp_path / "w.data"
+    systems.write_lammps_data(w, p, "full", bonds=bonds, angles=angles, image=image)
+    r = rbmd_amd.read_data(p, "full", replicate=[1, 2, 2], bond_periodic=False)
+    lengths = _bond_lengths(r)
+    assert np.abs(lengths - 1.0).max() < 1e-6
+    # each copy's partners are the same-tile copies of the seed's partners
+    span = int(w.ids.max()) + 1
+    seed = rbmd_amd.read_data(p, "full")
+    for k in range(0, r.n, 37):
+        tile = int(r.ids[k]) // span
+        base = int(np.nonzero(seed.ids == r.ids[k] % span)[0][0])
+        np.testing.assert_array_equal(r.special[k][:r.nspecial[k][2]],
+                                      seed.special[base][:seed.nspecial[base][2]] + tile * span)
 
 
 def test_errors_name_file_and_line(tmp_path):
