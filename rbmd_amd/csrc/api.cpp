@@ -289,25 +289,29 @@ class Engine : public EngineBase {
     p.num_steps = m.num_steps;
     p.e_self = e_self_;
     if (!md_state_) {
-      md_state_ = alloc<MdState>(2);
-      md_part_ = alloc<double>(kMdParts);
+      // [0], [1]: the alternating running states; [2]: the next step's opened state
+      md_state_ = alloc<MdState>(3);
+      md_part_ = alloc<double>(std::max<int64_t>(kMdParts, md_assemble_parts()));
     }
     const MdArrays<R> a = md_arrays(c);
     md_ = p;
     md_ready_ = false;
-    HIPCHK(hipMemsetAsync(md_state_, 0, 2 * sizeof(MdState), s));
+    HIPCHK(hipMemsetAsync(md_state_, 0, 3 * sizeof(MdState), s));
     md_cur_ = 0;
     HIPCHK(md_launch_wrap(a, s));
     const uint32_t ff = md_force_flags(flags);
     force(0, c.px, c.py, c.pz, c.fx, c.fy, c.fz, nullptr, 0, ff, s);
     HIPCHK(md_launch_second(a, md_, 0, s));
-    HIPCHK(md_launch_thermostat(md_, md_part_, d_.thermo, md_state_ + md_cur_,
-                                md_state_ + (md_cur_ ^ 1), 0, 1,
-                                (ff & RBMDX_FLAG_ENERGY) ? 1 : 0, s));
+    HIPCHK(md_launch_thermostat(md_, md_part_, md_init_parts(n_), d_.thermo,
+                                md_state_ + md_cur_, md_state_ + (md_cur_ ^ 1),
+                                md_state_ + 2, 0, 1, (ff & RBMDX_FLAG_ENERGY) ? 1 : 0, s));
     md_cur_ ^= 1;
     md_step_ = 0;
     md_ready_ = true;
   }
+
+  // k_assemble's workgroups: one partial of the kinetic energy each
+  int64_t md_assemble_parts() const { return ((int64_t)n_ + kBlock - 1) / kBlock; }
 
   void md_run(int32_t nsteps, const MdCall& c, uint32_t flags,
               hipStream_t s) override {
@@ -317,15 +321,32 @@ class Engine : public EngineBase {
     const MdArrays<R> a = md_arrays(c);
     const uint32_t ff = md_force_flags(flags);
     const int with_pe = (ff & RBMDX_FLAG_ENERGY) ? 1 : 0;
+    MdState* open = md_state_ + 2;
+    if (nsteps > 0) HIPCHK(md_launch_open(md_, md_state_ + md_cur_, open, md_step_ + 1, s));
+    const int parts = (int)md_assemble_parts();
     for (int32_t k = 0; k < nsteps; ++k) {
       const int64_t step = md_step_ + 1;
-      HIPCHK(md_launch_first(a, md_, md_state_ + md_cur_, md_state_ + (md_cur_ ^ 1),
-                             step, s));
+      HIPCHK(md_launch_first(a, md_, open, md_state_ + (md_cur_ ^ 1), s));
       md_cur_ ^= 1;
-      force(step, c.px, c.py, c.pz, c.fx, c.fy, c.fz, nullptr, 0, ff, s);
-      HIPCHK(md_launch_second(a, md_, 1, s));
-      HIPCHK(md_launch_thermostat(md_, md_part_, d_.thermo, md_state_ + md_cur_,
-                                  md_state_ + (md_cur_ ^ 1), step, 0, with_pe, s));
+      // the force with the closing half-kick and kinetic-energy partials
+      // folded into its assembly kernel
+      d_.vx = (R*)c.vx;
+      d_.vy = (R*)c.vy;
+      d_.vz = (R*)c.vz;
+      d_.md_mass = md_mass_;
+      d_.md_part = md_part_;
+      md_kick_ = true;
+      try {
+        force(step, c.px, c.py, c.pz, c.fx, c.fy, c.fz, nullptr, 0, ff, s);
+      } catch (...) {
+        d_.vx = d_.vy = d_.vz = nullptr;
+        md_kick_ = false;
+        throw;
+      }
+      d_.vx = d_.vy = d_.vz = nullptr;
+      md_kick_ = false;
+      HIPCHK(md_launch_thermostat(md_, md_part_, parts, d_.thermo, md_state_ + md_cur_,
+                                  md_state_ + (md_cur_ ^ 1), open, step, 0, with_pe, s));
       md_cur_ ^= 1;
       md_step_ = step;
     }
@@ -636,7 +657,7 @@ class Engine : public EngineBase {
   // the device buffers (whose reallocation bumps gen_).
   struct GraphKey {
     const void* ptr[12];
-    int eflag, num_k, m_from_ring, pad;
+    int eflag, num_k, m_from_ring, md_kick;
     double dt;
     uint64_t gen;
   };
@@ -649,6 +670,7 @@ class Engine : public EngineBase {
     g.eflag = a.p.eflag;
     g.num_k = a.p.num_k;
     g.m_from_ring = a.m_from_ring;
+    g.md_kick = a.md_kick;
     g.dt = a.dt;
     g.gen = gen_;
     return g;
@@ -857,6 +879,7 @@ class Engine : public EngineBase {
   int md_cur_ = 0;
   int64_t md_step_ = 0;
   bool md_ready_ = false;
+  bool md_kick_ = false;  // the current evaluation is an MD step's force
 
   template <typename T>
   T* alloc(size_t n) {
@@ -1399,6 +1422,11 @@ class Engine : public EngineBase {
     a.rebuild_thr = rebuild_thr_;
     a.dt = leap_dt_;
     a.dt_ftm = leap_dt_ * cfg_.ftm2v;
+    if (md_kick_) {  // the MD step's closing half-kick in the assembly kernel
+      a.md_kick = 1;
+      a.dt = md_.dt;
+      a.ftm2v = cfg_.ftm2v;
+    }
     a.e_self = e_self_;
     a.m_use = d_.m_dev;
     a.ks = ks_;

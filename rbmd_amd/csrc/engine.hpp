@@ -126,6 +126,8 @@ struct Dev {
   R* yw = nullptr;
   R* zw = nullptr;
   R* inv_mass = nullptr;   // [ntypes]
+  const double* md_mass = nullptr;  // [ntypes] FP64 masses (MD closing half-kick)
+  double* md_part = nullptr;        // [assemble workgroups] sum m v^2 (MD step)
   int* type = nullptr;     // [n]
   int* gid = nullptr;      // [n]
   R* charge = nullptr;     // [n]
@@ -233,6 +235,8 @@ struct StepArgs {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double dt = 0.0;            // fused leapfrog: x += dt v
   double dt_ftm = 0.0;        //                 v += dt ftm2v F / m
+  int md_kick = 0;            // MD step: closing half-kick + kinetic energy
+  double ftm2v = 1.0;         //          (dt above is the time step)
   hipEvent_t ev[6] = {};      // optional section timing events
 };
 

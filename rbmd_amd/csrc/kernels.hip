@@ -1603,16 +1603,27 @@ __global__ __launch_bounds__(64 * kRbeWaves) void k_rbe_force(Dev<R> d,
   }
 }
 
+// v + 0.5 F / m dt ftm2v evaluated left to right, as RBMD's UpdateVelocity
+// (update_velocity_op.hip.cu:19-21), without contraction into FMA.
+template <typename R>
+__device__ __forceinline__ R md_half_kick(R v, R f, R m, R dt, R ftm2v) {
+#pragma clang fp contract(off)
+  return v + (R)0.5 * f / m * dt * ftm2v;
+}
+
 // Total force in caller order (one thread per caller atom); resets the skin
 // accumulator for the next call and mirrors the counters to the host.  With
 // velocities bound it also applies the leapfrog update
 // v += dt ftm2v F / m, x += dt v (RBMD's UpdateVelocity half-kicks of two
 // consecutive steps merged, src/simulate_pipeline/src/controller/
 // velocity_controller/velocity_controller_op/rocm/update_velocity_op.hip.cu:5-27).
+// With md_kick (the molecular-dynamics step) it applies the closing half-kick
+// v += 0.5 F / m dt ftm2v instead and writes the workgroup's FP64 sum of
+// m v^2 to d.md_part[blockIdx.x].
 template <typename R>
 __global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix,
                                                   int has_k, int has_b, R dt,
-                                                  R dt_ftm) {
+                                                  R dt_ftm, int md_kick, R ftm2v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     d.ctrl->maxdisp_bits = 0u;
@@ -1625,43 +1636,62 @@ __global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix,
       d.ctrl_host->unstaged_bricks = d.ctrl->unstaged_bricks;
     }
   }
-  if (i >= d.n) return;
-  const int s = d.inv_perm[i];
-  R4T<R> f = d.fa[s];
-  if (mean_fix) {
-    f.x -= (R)d.mean[0];
-    f.y -= (R)d.mean[1];
-    f.z -= (R)d.mean[2];
-  }
-  if (has_k || has_b) {
-    const int c = d.c_of[i];
-    if (c >= 0) {
-      if (has_b) {
-        const R4T<R> g = d.fb[c];
-        f.x += g.x;
-        f.y += g.y;
-        f.z += g.z;
-      }
-      if (has_k) {
-        const R4T<R> g = d.fk[c];
-        f.x += g.x;
-        f.y += g.y;
-        f.z += g.z;
+  double ke = 0.0;
+  if (i < d.n) {
+    const int s = d.inv_perm[i];
+    R4T<R> f = d.fa[s];
+    if (mean_fix) {
+      f.x -= (R)d.mean[0];
+      f.y -= (R)d.mean[1];
+      f.z -= (R)d.mean[2];
+    }
+    if (has_k || has_b) {
+      const int c = d.c_of[i];
+      if (c >= 0) {
+        if (has_b) {
+          const R4T<R> g = d.fb[c];
+          f.x += g.x;
+          f.y += g.y;
+          f.z += g.z;
+        }
+        if (has_k) {
+          const R4T<R> g = d.fk[c];
+          f.x += g.x;
+          f.y += g.y;
+          f.z += g.z;
+        }
       }
     }
+    d.fx[i] = f.x;
+    d.fy[i] = f.y;
+    d.fz[i] = f.z;
+    if (md_kick) {
+      const double md = d.md_mass[d.type[i]];
+      const R m = (R)md;
+      const R vx = md_half_kick(d.vx[i], f.x, m, dt, ftm2v);
+      const R vy = md_half_kick(d.vy[i], f.y, m, dt, ftm2v);
+      const R vz = md_half_kick(d.vz[i], f.z, m, dt, ftm2v);
+      d.vx[i] = vx;
+      d.vy[i] = vy;
+      d.vz[i] = vz;
+      const double x = vx, y = vy, z = vz;
+      ke = md * (x * x + y * y + z * z);
+    } else if (d.vx) {
+      const R s = dt_ftm * d.inv_mass[d.type[i]];
+      const R vx = d.vx[i] + s * f.x, vy = d.vy[i] + s * f.y, vz = d.vz[i] + s * f.z;
+      d.vx[i] = vx;
+      d.vy[i] = vy;
+      d.vz[i] = vz;
+      d.xw[i] = d.px[i] + dt * vx;
+      d.yw[i] = d.py[i] + dt * vy;
+      d.zw[i] = d.pz[i] + dt * vz;
+    }
   }
-  d.fx[i] = f.x;
-  d.fy[i] = f.y;
-  d.fz[i] = f.z;
-  if (d.vx) {
-    const R s = dt_ftm * d.inv_mass[d.type[i]];
-    const R vx = d.vx[i] + s * f.x, vy = d.vy[i] + s * f.y, vz = d.vz[i] + s * f.z;
-    d.vx[i] = vx;
-    d.vy[i] = vy;
-    d.vz[i] = vz;
-    d.xw[i] = d.px[i] + dt * vx;
-    d.yw[i] = d.py[i] + dt * vy;
-    d.zw[i] = d.pz[i] + dt * vz;
+  if (md_kick) {  // uniform: every thread of the workgroup reaches the sum
+    __shared__ double smem[16];
+    double v[1] = {ke};
+    block_sum<1>(v, smem);
+    if (threadIdx.x == 0) d.md_part[blockIdx.x] = v[0];
   }
 }
 
@@ -1876,7 +1906,7 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   if (ks_on && ks != s) hipStreamWaitEvent(s, a.ev_join, 0);
   hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,
                      s, d, p.mean_fix, ks_on ? 1 : 0, p.split ? 1 : 0, (R)a.dt,
-                     (R)a.dt_ftm);
+                     (R)a.dt_ftm, a.md_kick, (R)a.ftm2v);
   if (a.ev[5]) hipEventRecord(a.ev[5], s);
   return hipGetLastError();
 }

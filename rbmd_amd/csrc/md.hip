@@ -9,13 +9,17 @@
 //   F = force(x),  v += dt/2 F/m,
 //   [NVT] temperature, chain half-step (scales v)
 //   [NVE + RBL] temperature, energy-stable rescale of v.
-// Here that is three launches around the force: k_md_first (the pending
-// end-of-step factor and the chain half-step of step n folded into one scale,
-// then kick, drift and wrap), k_md_second (kick and per-workgroup FP64
-// kinetic-energy partials) and k_md_thermostat (one workgroup: temperature,
-// then the end-of-step chain half or the energy-stable factor, left pending
-// for the next k_md_first or k_md_apply).  All thermostat scalars stay in
-// device memory, so a run of many steps needs no host round trip.
+// Here that is two launches around the force: k_md_first (one scale -- the
+// pending end-of-step factor times the chain's opening factor of step n --
+// then kick, drift and wrap) and k_md_thermostat (one workgroup: temperature
+// from the kinetic-energy partials that the force's assembly kernel wrote
+// with the closing half-kick, the end-of-step chain half or the
+// energy-stable factor, left pending, and the opening half of step n + 1
+// computed ahead for the next k_md_first).  k_md_open opens the first step
+// of a run; k_md_apply applies the pending factor at its end; k_md_second
+// (kick = 0) gives the partials of the starting velocities.  All thermostat
+// scalars stay in device memory, so a run of many steps needs no host round
+// trip.
 //
 // Arithmetic follows the reference's expression order (no contraction into
 // FMA), so the FP64 path agrees with a plain restatement to rounding.
@@ -116,24 +120,36 @@ __device__ __forceinline__ R half_kick(R v, R f, R m, R dt, R ftm2v) {
   return v + (R)0.5 * f / m * dt * ftm2v;
 }
 
+// The opening half of step `next` on a closed state whose scale holds the
+// pending end-of-step factor: NVT folds the chain's opening factor into it.
+__device__ void open_step(MdState& st, const MdParams& p, int64_t next) {
+  if (p.mode == kMdNvtNh) {
+    st.t_target = target_temperature(p, next);
+    st.scale *= nhc_half(st, p);
+  }
+}
+
+// One thread: the opened state of the first step of a run.
+__global__ void k_md_open(MdParams p, const MdState* __restrict__ in,
+                          MdState* __restrict__ out, int64_t next) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  MdState st = *in;
+  open_step(st, p, next);
+  *out = st;
+}
+
+// Scale by the opened state's factor, half-kick, drift, wrap.  Workgroup 0
+// stores the opened state (scale applied) as the step's running state.
 template <typename R>
 __global__ __launch_bounds__(kMdBlock) void k_md_first(
-    MdArrays<R> a, MdParams p, const MdState* __restrict__ in,
-    MdState* __restrict__ out, int64_t step) {
-  __shared__ double sfac;
-  if (threadIdx.x == 0) {
-    MdState st = *in;
-    double f = st.scale;
+    MdArrays<R> a, MdParams p, const MdState* __restrict__ open,
+    MdState* __restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    MdState st = *open;
     st.scale = 1.0;
-    if (p.mode == kMdNvtNh) {
-      st.t_target = target_temperature(p, step);
-      f *= nhc_half(st, p);
-    }
-    sfac = f;
-    if (blockIdx.x == 0) *out = st;
+    *out = st;
   }
-  __syncthreads();
-  const R f = (R)sfac, dt = (R)p.dt, c = (R)p.ftm2v;
+  const R f = (R)open->scale, dt = (R)p.dt, c = (R)p.ftm2v;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
        i += gridDim.x * blockDim.x) {
     const R m = (R)a.mass[a.type[i]];
@@ -189,9 +205,12 @@ __global__ __launch_bounds__(kMdBlock) void k_md_second(MdArrays<R> a, MdParams 
 __global__ __launch_bounds__(1024) void k_md_thermostat(
     MdParams p, const double* __restrict__ part, int nparts,
     const double* __restrict__ thermo, const MdState* __restrict__ in,
-    MdState* __restrict__ out, int64_t step, int setup, int with_pe) {
+    MdState* __restrict__ out, MdState* __restrict__ open, int64_t step, int setup,
+    int with_pe) {
   __shared__ double smem[16];
-  double v[1] = {(int)threadIdx.x < nparts ? part[threadIdx.x] : 0.0};
+  double acc = 0.0;
+  for (int q = threadIdx.x; q < nparts; q += blockDim.x) acc += part[q];
+  double v[1] = {acc};
   block_sum<1>(v, smem);
   if (threadIdx.x != 0) return;
   MdState st = *in;
@@ -242,6 +261,9 @@ __global__ __launch_bounds__(1024) void k_md_thermostat(
     st.ke = st.ke * st.scale * st.scale;
   }
   *out = st;
+  // the next step's opening half, ready for its k_md_first
+  open_step(st, p, step + 1);
+  *open = st;
 }
 
 template <typename R>
@@ -278,12 +300,17 @@ hipError_t md_launch_wrap(const MdArrays<R>& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t md_launch_open(const MdParams& p, const MdState* in, MdState* out,
+                          int64_t next, hipStream_t s) {
+  hipLaunchKernelGGL(k_md_open, dim3(1), dim3(64), 0, s, p, in, out, next);
+  return hipGetLastError();
+}
+
 template <typename R>
 hipError_t md_launch_first(const MdArrays<R>& a, const MdParams& p,
-                           const MdState* in, MdState* out, int64_t step,
-                           hipStream_t s) {
+                           const MdState* open, MdState* out, hipStream_t s) {
   hipLaunchKernelGGL((k_md_first<R>), dim3(md_blocks(a.n)), dim3(kMdBlock), 0, s,
-                     a, p, in, out, step);
+                     a, p, open, out);
   return hipGetLastError();
 }
 
@@ -295,12 +322,14 @@ hipError_t md_launch_second(const MdArrays<R>& a, const MdParams& p, int kick,
   return hipGetLastError();
 }
 
-hipError_t md_launch_thermostat(const MdParams& p, const double* part,
+int md_init_parts(int64_t n) { return md_blocks(n); }
+
+hipError_t md_launch_thermostat(const MdParams& p, const double* part, int nparts,
                                 const double* thermo, const MdState* in,
-                                MdState* out, int64_t step, int setup,
+                                MdState* out, MdState* open, int64_t step, int setup,
                                 int with_pe, hipStream_t s) {
-  hipLaunchKernelGGL(k_md_thermostat, dim3(1), dim3(1024), 0, s, p, part,
-                     md_blocks(p.natoms), thermo, in, out, step, setup, with_pe);
+  hipLaunchKernelGGL(k_md_thermostat, dim3(1), dim3(1024), 0, s, p, part, nparts,
+                     thermo, in, out, open, step, setup, with_pe);
   return hipGetLastError();
 }
 
@@ -315,8 +344,7 @@ hipError_t md_launch_apply(const MdArrays<R>& a, const MdState* in, MdState* out
 #define RBMDX_MD_INST(R)                                                        \
   template hipError_t md_launch_wrap<R>(const MdArrays<R>&, hipStream_t);      \
   template hipError_t md_launch_first<R>(const MdArrays<R>&, const MdParams&,  \
-                                         const MdState*, MdState*, int64_t,    \
-                                         hipStream_t);                         \
+                                         const MdState*, MdState*, hipStream_t); \
   template hipError_t md_launch_second<R>(const MdArrays<R>&, const MdParams&, \
                                           int, hipStream_t);                   \
   template hipError_t md_launch_apply<R>(const MdArrays<R>&, const MdState*,   \

@@ -5,9 +5,10 @@
 //
 // Every thermostat scalar lives in device memory (MdState, FP64), so a run of
 // many steps needs no host round trip: the chain is advanced by the kernels
-// themselves.  Two copies of the state alternate: a kernel that advances it
-// reads copy `cur` and writes copy `cur ^ 1` (all workgroups read the same
-// input, workgroup 0 writes the output), and the host flips `cur`.
+// themselves.  Two copies of the running state alternate: a kernel that
+// advances it reads copy `cur` and writes copy `cur ^ 1` (all workgroups read
+// the same input, workgroup 0 writes the output), and the host flips `cur`;
+// a third copy holds the next step's opened state.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -16,7 +17,7 @@
 namespace rbmdx {
 
 constexpr int kMdChain = 3;     // thermostat chain length (RBMD's default)
-constexpr int kMdParts = 1024;  // per-workgroup kinetic-energy partial sums
+constexpr int kMdParts = 1024;  // k_md_second's kinetic-energy partials (at most)
 constexpr int kMdBlock = 256;
 
 enum MdMode { kMdNve = 0, kMdNveStable = 1, kMdNvtNh = 2 };
@@ -62,12 +63,15 @@ struct MdArrays {
 // Wrap positions into the box once, updating image flags (start of a run).
 template <typename R>
 hipError_t md_launch_wrap(const MdArrays<R>& a, hipStream_t s);
-// Thermostat half-step of a new step (Nose-Hoover), pending scale, half-kick,
-// drift and wrap: state in -> out.
+// The opening half of step `next` (Nose-Hoover) on a closed state: in -> out
+// (the opened state, its scale the product of the pending and opening factors).
+hipError_t md_launch_open(const MdParams& p, const MdState* in, MdState* out,
+                          int64_t next, hipStream_t s);
+// Scale by the opened state's factor, half-kick, drift and wrap; the opened
+// state (scale applied) is written to out.
 template <typename R>
 hipError_t md_launch_first(const MdArrays<R>& a, const MdParams& p,
-                           const MdState* in, MdState* out, int64_t step,
-                           hipStream_t s);
+                           const MdState* open, MdState* out, hipStream_t s);
 // Second half-kick (kick = 0: none) and the kinetic-energy partial sums.
 template <typename R>
 hipError_t md_launch_second(const MdArrays<R>& a, const MdParams& p, int kick,
@@ -75,10 +79,13 @@ hipError_t md_launch_second(const MdArrays<R>& a, const MdParams& p, int kick,
 // Temperature and the end-of-step thermostat (setup = 1: the chain's set-up
 // at step 0 instead; with_pe = 1: the force engine's energy sums are valid):
 // state in -> out.
-hipError_t md_launch_thermostat(const MdParams& p, const double* part,
+// nparts partials are summed; the opening half of step + 1 goes to `open`.
+hipError_t md_launch_thermostat(const MdParams& p, const double* part, int nparts,
                                 const double* thermo, const MdState* in,
-                                MdState* out, int64_t step, int setup,
+                                MdState* out, MdState* open, int64_t step, int setup,
                                 int with_pe, hipStream_t s);
+// Number of partials md_launch_second writes for n atoms.
+int md_init_parts(int64_t n);
 // Apply the pending velocity factor: state in -> out with scale 1.
 template <typename R>
 hipError_t md_launch_apply(const MdArrays<R>& a, const MdState* in, MdState* out,
