@@ -202,3 +202,45 @@ def test_driver_md_loop_matches_python(device, tmp_path):
     assert _rel(last[:, 3:6], X) < 1e-12
     assert _rel(last[:, 6:9], X + I * r.box_len) < 1e-12
     assert _rel(last[:, 9:12], V) < 1e-12
+
+
+def test_real_units_argon_nvt(device, tmp_path):
+    """REAL units end to end: GAUSS velocities at 120 K from the data reader,
+    then 200 Nose-Hoover steps of FP64 LJ argon (eps 0.238 kcal/mol, sigma
+    3.405 A, m 39.948 g/mol, dt 2 fs).  The sampled temperature at step 0 is
+    the GAUSS target exactly (same kB, mvv2e and 3N - 3); the extended energy
+    N (ke + pe) + chain energy stays within 2 % of the kinetic energy (a
+    wrong ftm2v, mvv2e or kB breaks it by orders of magnitude)."""
+    import torch
+
+    s = systems.lj_fcc(cells=6, jitter=0.05, seed=31)
+    sigma, eps, mass = 3.405, 0.238, 39.948
+    s.pos *= sigma
+    s.box_len = s.box_len * sigma
+    s.eps, s.sigma, s.mass, s.vel = np.array([eps]), np.array([sigma]), np.array([mass]), None
+    systems.write_lammps_data(s, tmp_path / "ar.data", "atomic")
+    r = rbmd_amd.read_data(tmp_path / "ar.data", "atomic", velocity_type="GAUSS",
+                           temperature=120.0, velocity_seed=7, unit="REAL")
+    cfg = _config("NVT", temp_ctrl_type="NOSE_HOOVER", temperature=[120.0, 120.0, 100.0])
+    cfg["execution"]["timestep"] = 2.0
+    cfg["init_configuration"]["read_data"]["unit"] = "REAL"
+    cfg["hyper_parameters"]["neighbor"].update({"cut_off": 2.5 * sigma, "skin": 1.0})
+    ctx = ForceContext(cfg, device=0)
+    ctx.set_system(r.box_lo, r.box_len, r.types, r.ids, r.charges, r.eps, r.sigma)
+    ctx.set_masses(r.mass)
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)
+    x = [t(r.pos[:, d]) for d in range(3)]
+    v = [t(r.vel[:, d]) for d in range(3)]
+    f = [torch.zeros(r.n, dtype=torch.float64, device=device) for _ in range(3)]
+    ctx.md_init(x, v, f, energy=True)
+    st0 = ctx.md_state()
+    assert st0["temperature"] == pytest.approx(120.0, rel=1e-9)
+    h0 = r.n * (st0["ke"] + st0["pe"]) + st0["thermostat_energy"]
+    for _ in range(4):
+        ctx.md_run(50, x, v, f, energy=True)
+        st = ctx.md_state()
+        h = r.n * (st["ke"] + st["pe"]) + st["thermostat_energy"]
+        print(f"step {st['step']}: T {st['temperature']:.2f} K, H_ext drift "
+              f"{(h - h0) / (r.n * st0['ke']):.2e}")
+        assert np.isfinite(h) and 60.0 < st["temperature"] < 240.0
+        assert abs(h - h0) < 0.02 * r.n * st0["ke"]
