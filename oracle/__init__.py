@@ -4,6 +4,9 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
 this package.  It wraps liboracle.so (oracle/oracle.cpp, built by the Makefile
 in this directory), a plain CPU restatement of the reference algorithm; see the
 header of oracle.cpp for what each function restates and how it is pinned.
+Two pure-Python modules sit beside it: oracle.md (the molecular-dynamics step
+around the force) and oracle.gauss (GAUSS velocity creation), each with its
+own header of what it restates and how it is pinned.
 """
 import ctypes
 import os
