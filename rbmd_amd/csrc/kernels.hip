@@ -2044,14 +2044,25 @@ __global__ __launch_bounds__(256) void k_vjp_mean(const double* gpart, int nb,
 // One owner pair of the adjoint pair pass.  The caller provides d and u;
 // gi/gj are the mean-shifted cotangents, row the LJ row (null beyond rc_LJ),
 // coul whether Coulomb applies, qj the neighbour's charge.
+//
+// CVFF special bonds (ws < 1, the same weight both ways): the LJ terms carry
+// ws, and a kept charged pair also removes (1 - ws) of the full Coulomb force
+// without the RBL multiplier (interact_b): f_c = (1 - ws) qqr2e q_i q_j u^-3/2
+// along d, weighted by cij / cji = 1 where the directed pair is kept.
 template <typename R>
 __device__ __forceinline__ void vjp_pair(R dx, R dy, R dz, R u, const R* row,
                                          bool coul, const PairConst<R>& pc,
                                          R qi, R qj, R wij, R wji,
                                          const R gi[3], const R gj[3], R r[3],
-                                         VjpAcc& acc) {
+                                         VjpAcc& acc, R ws = R(1), R cij = R(0),
+                                         R cji = R(0)) {
   R flj = 0, fp = 0, fs = 0;
   if (row) lj_vjp<R>(row, u, flj, fp, fs);
+  if (ws != R(1)) {
+    flj *= ws;
+    fp *= ws;
+    fs *= ws;
+  }
   R f = flj, gqb = 0;
   if (coul) {
     R h, hp;
@@ -2071,11 +2082,23 @@ __device__ __forceinline__ void vjp_pair(R dx, R dy, R dz, R u, const R* row,
     acc.ds -= (float)(R(0.5) * fs * hd);
   }
   if (coul) acc.dq -= (float)(qj * gqb * hd);
+  if (coul && ws != R(1)) {
+    const R rinv = R(1) / sqrt(u);
+    const R gc = (R(1) - ws) * pc.qqr2e * rinv * rinv * rinv;  // f_c / (q_i q_j)
+    const R fc = qi * qj * gc, fcp = R(-1.5) * fc / u;
+    const R kx = cji * gj[0] - cij * gi[0], ky = cji * gj[1] - cij * gi[1],
+            kz = cji * gj[2] - cij * gi[2];
+    const R kd = kx * dx + ky * dy + kz * dz;
+    r[0] += fc * kx + R(2) * fcp * kd * dx;
+    r[1] += fc * ky + R(2) * fcp * kd * dy;
+    r[2] += fc * kz + R(2) * fcp * kd * dz;
+    acc.dq -= (float)(qj * gc * kd);
+  }
 }
 
 // Adjoint pair pass, one workgroup per brick (the forward's lists and stage;
 // the cotangent and global ids staged beside the positions).
-template <typename R, bool RBL>
+template <typename R, bool RBL, bool SPECIAL>
 __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> b,
                                                   Grid g, PairConst<R> pc_arg,
                                                   const R* lj_tab, int split) {
@@ -2129,6 +2152,7 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
     const unsigned long long hi = RBL ? rbl_hi(d.gid_s[i]) : 0ull;
     const unsigned long long hji = RBL ? d.hj_s[i] : 0ull;
     const R* lrow = s_lj + 4 * ti * nt;
+    const SpecRow sr = SPECIAL ? spec_row<R>(d, pc.ms, i) : SpecRow();
     R r[3] = {0, 0, 0};
     VjpAcc acc;
     const int cnt = d.cnt_a[i];
@@ -2144,8 +2168,9 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
         wji = vjp_w<R, RBL>(pc, u, rbl_hi(gidj), hji);
       }
       if (wij == R(0) && wji == R(0)) return;
+      const R ws = SPECIAL ? special_w<R>(sr, gidj, pc) : R(1);
       vjp_pair<R>(dx, dy, dz, u, lrow + 4 * tj, false, pc, R(0), R(0), wij, wji, gi,
-                  gj, r, acc);
+                  gj, r, acc, ws);
     };
     if (staged && cnt <= d.cap_a) {
       const uint4* lp = reinterpret_cast<const uint4*>(d.list_a) + i;
@@ -2208,6 +2233,7 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
       const unsigned long long hi = RBL ? rbl_hi(d.gidc[c]) : 0ull;
       const unsigned long long hji = RBL ? d.hjc[c] : 0ull;
       const R* lrow = s_lj + 4 * ti * nt;
+      const SpecRow sr = SPECIAL ? spec_row<R>(d, pc.ms, d.c2s[c]) : SpecRow();
       R r[3] = {0, 0, 0};
       VjpAcc acc;
       auto one = [&](int e) {
@@ -2226,8 +2252,10 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
         const R4T<R> gq = v.gc[e];
         const R gj[3] = {gq.x - gm[0], gq.y - gm[1], gq.z - gm[2]};
         const int tj = d.typec[e];
+        const R ws = SPECIAL ? special_w<R>(sr, d.gidc[e], pc) : R(1);
         vjp_pair<R>(dx, dy, dz, u, u < pc.rclj2 ? lrow + 4 * tj : nullptr, true, pc,
-                    xi.w, xj.w, wij, wji, gi, gj, r, acc);
+                    xi.w, xj.w, wij, wji, gi, gj, r, acc, ws,
+                    wij != R(0) ? R(1) : R(0), wji != R(0) ? R(1) : R(0));
       };
       const int cnt = d.cnt_b[c];
       if (cnt <= d.cap_b) {
@@ -2659,12 +2687,20 @@ hipError_t launch_vjp(const Dev<R>& d, const VjpDev<R>& v, const StepArgs<R>& a,
   hipLaunchKernelGGL(k_vjp_mean, dim3(1), dim3(256), 0, s, v.gpart, v.gather_blocks,
                      (double)d.n, p.mean_fix, v.gmean);
   const size_t lds = 4 * sizeof(R) * (size_t)nt * nt;
-  if (p.rbl)
-    hipLaunchKernelGGL((k_pair_vjp<R, true>), dim3(g.nbrick), dim3(g.block), lds, s, d, v,
-                       a.box, g, a.pc, a.lj_tab, p.split);
-  else
-    hipLaunchKernelGGL((k_pair_vjp<R, false>), dim3(g.nbrick), dim3(g.block), lds, s, d, v,
-                       a.box, g, a.pc, a.lj_tab, p.split);
+#define RBMDX_PAIR_VJP(RBL, SP)                                                 \
+  hipLaunchKernelGGL((k_pair_vjp<R, RBL, SP>), dim3(g.nbrick), dim3(g.block), lds, s, \
+                     d, v, a.box, g, a.pc, a.lj_tab, p.split)
+  if (p.special) {
+    if (p.rbl)
+      RBMDX_PAIR_VJP(true, true);
+    else
+      RBMDX_PAIR_VJP(false, true);
+  } else if (p.rbl) {
+    RBMDX_PAIR_VJP(true, false);
+  } else {
+    RBMDX_PAIR_VJP(false, false);
+  }
+#undef RBMDX_PAIR_VJP
   if (v.neutral_q && v.nq_blocks > 0) {
     if (p.rbl)
       hipLaunchKernelGGL((k_vjp_qnc<R, true>), dim3(g.nbrick), dim3(256), 0, s, d, v,

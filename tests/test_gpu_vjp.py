@@ -173,3 +173,81 @@ def test_vjp_neutral_charge_grad_large_box(device, frac_ion):
     print("dQ", dp[:3], dqo)
     for got, want in ((dp[:3], dqo), (dp[3:6], deo), (dp[6:], dso)):
         np.testing.assert_allclose(got, want, rtol=0, atol=1e-4 * np.abs(want).max())
+
+
+def _cvff_water_adjoint_and_fd(device, weights, directions):
+    """The adjoint's dL/dr . delta and dL/dparam, and central differences of
+    L = sum G . F, for FP64 SPC/E water with CVFF special_bonds `weights`."""
+    import torch
+
+    s = systems.spce_water(200, seed=3)
+    s.eps = np.array([0.1553, 0.05])
+    s.sigma = np.array([3.166, 1.0])
+    cfg = systems.cvff_water_config(neighbor="Verlet-List", coulomb="RBE", cut_off=7.0,
+                                    skin=1.0, P=100, special_bonds=weights,
+                                    precision="fp64")
+    ctx = _ctx(s, cfg)
+    ctx.set_special(s.nspecial, s.special)
+    m = ctx.rbe_sample(0)
+    mt = torch.tensor(m.reshape(-1).astype(np.int32), device=device)
+    G = np.random.default_rng(5).normal(size=(s.n, 3))
+    Q = np.array([-0.8476, 0.4238])
+
+    def loss(pos, eps=s.eps, sig=s.sigma, q=Q):
+        ctx.set_lj(eps, sig)
+        ctx.set_charges(q[s.types])
+        p = torch.tensor(pos, dtype=torch.float64, device=device)
+        f = [torch.zeros(s.n, dtype=torch.float64, device=device) for _ in range(3)]
+        ctx.force(0, *(p[:, d].contiguous() for d in range(3)), *f, m_samples=mt, sync=True)
+        return float(sum((f[d].cpu().numpy() * G[:, d]).sum() for d in range(3)))
+
+    loss(s.pos)
+    p = torch.tensor(s.pos, dtype=torch.float64, device=device)
+    px, py, pz = (p[:, d].contiguous() for d in range(3))
+    g = torch.tensor(G, dtype=torch.float64, device=device)
+    dr = [torch.zeros_like(px) for _ in range(3)]
+    dp = torch.zeros(6, dtype=torch.float64, device=device)
+    ctx.force_vjp(0, px, py, pz, *(g[:, d].contiguous() for d in range(3)), *dr, dp,
+                  sync=True)
+    DR = np.stack([t.cpu().numpy() for t in dr], 1)
+    h = 1e-6
+    ad = np.array([(DR * dlt).sum() for dlt in directions])
+    fd = np.array([(loss(s.pos + h * dlt) - loss(s.pos - h * dlt)) / (2 * h)
+                   for dlt in directions])
+    fdp = np.zeros(6)
+    for t in range(2):
+        e = np.zeros(2)
+        e[t] = 1.0
+        hq, he, hs = 1e-6, 1e-6 * s.eps[t], 1e-6 * s.sigma[t]
+        fdp[t] = (loss(s.pos, q=Q + hq * e) - loss(s.pos, q=Q - hq * e)) / (2 * hq)
+        fdp[2 + t] = (loss(s.pos, eps=s.eps + he * e)
+                      - loss(s.pos, eps=s.eps - he * e)) / (2 * he)
+        fdp[4 + t] = (loss(s.pos, sig=s.sigma + hs * e)
+                      - loss(s.pos, sig=s.sigma - hs * e)) / (2 * hs)
+    return ad, fd, dp.cpu().numpy(), fdp
+
+
+def test_vjp_cvff_special_bonds_matches_finite_differences(device):
+    """CVFF special bonds in the adjoint: SPC/E water with a small H LJ well
+    (so the 1-3 H-H pairs carry LJ), special_bonds [0, 0.6, 0.5] against the
+    same system with unit weights, Verlet list, RBE with a fixed k-sample,
+    FP64.  The forward's k-space force coefficients are FP32, so a central
+    difference of L = sum G . F carries their rounding; it is the same in both
+    systems (same positions, charges and k-vectors) and cancels in the
+    difference, which isolates the special-bond terms: the adjoint's
+    special-minus-unit dL/dr along random directions and per-type dL/dQ,
+    dL/deps, dL/dsigma equal the finite differences' within 1e-6 relative
+    (1e-4 of the largest per-type sum: FP32 per-lane parameter sums).  The
+    terms without special bonds are pinned to the FP64 oracle adjoint by the
+    tests above; the forward is pinned to the CVFF oracle in
+    tests/test_gpu_cvff.py."""
+    rng = np.random.default_rng(6)
+    dirs = [rng.normal(size=(600, 3)) for _ in range(3)]
+    ad0, fd0, dp0, fdp0 = _cvff_water_adjoint_and_fd(device, (0.0, 0.6, 0.5), dirs)
+    ad1, fd1, dp1, fdp1 = _cvff_water_adjoint_and_fd(device, (1.0, 1.0, 1.0), dirs)
+    print("special - unit, adjoint", ad0 - ad1, "finite differences", fd0 - fd1)
+    print("dparam special - unit, adjoint", dp0 - dp1, "finite differences", fdp0 - fdp1)
+    np.testing.assert_allclose(ad0 - ad1, fd0 - fd1, rtol=1e-6)
+    for k0 in (0, 2, 4):
+        dd = (fdp0 - fdp1)[k0:k0 + 2]
+        np.testing.assert_allclose((dp0 - dp1)[k0:k0 + 2], dd, atol=1e-4 * np.abs(dd).max())
