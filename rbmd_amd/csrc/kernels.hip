@@ -1962,9 +1962,22 @@ __device__ __forceinline__ void lj_vjp(const R* row, R u, R& f, R& fp, R& fs) {
 
 // erfc Coulomb without the charges: f_C = -qqr2e q_i q_j h(u),
 // h = erfc(g r) / (u r) + (2 g / sqrt(pi)) e^{-alpha u} / u, hp = dh/du.
+// With the SOG near field (VERLET-SOG, rbsog_near_field "sog"; coul_sog):
+// h = u^{-3/2} + sum_n T_n u^n, hp = -1.5 u^{-5/2} + sum_n n T_n u^{n-1}.
 template <typename R>
 __device__ __forceinline__ void coul_vjp(const PairConst<R>& pc, R u, R& h,
                                          R& hp) {
+  if (pc.sog) {
+    const R r3inv = R(1) / (sqrt(u) * u);
+    R poly = pc.T[5], dpoly = R(5) * pc.T[5];
+#pragma unroll
+    for (int n = 4; n >= 0; --n) poly = poly * u + pc.T[n];
+#pragma unroll
+    for (int n = 4; n >= 1; --n) dpoly = dpoly * u + R(n) * pc.T[n];
+    h = r3inv + poly;
+    hp = R(-1.5) * r3inv / u + dpoly;
+    return;
+  }
   const R r = sqrt(u);
   const R er = erfc(pc.g * r), ex = exp(-pc.alpha * u);
   const R inv = R(1) / u;

@@ -251,3 +251,75 @@ def test_vjp_cvff_special_bonds_matches_finite_differences(device):
     for k0 in (0, 2, 4):
         dd = (fdp0 - fdp1)[k0:k0 + 2]
         np.testing.assert_allclose((dp0 - dp1)[k0:k0 + 2], dd, atol=1e-4 * np.abs(dd).max())
+
+
+def _sog_adjoint_and_fd(device, near, directions):
+    """Adjoint dL/dr . delta and dL/dQ (charged types) with central
+    differences, FP64 Verlet + RBSOG on a 2,000-atom electrolyte with
+    coulomb.rbsog_near_field `near`, k-sample fixed."""
+    import torch
+
+    s = _system(2000, seed=12)
+    cfg = systems.ljsalt_config(P=64)
+    cl = cfg["hyper_parameters"]["coulomb"]
+    cl.update({"type": "RBSOG", "rbsog_level": 1, "rbsog_near_field": near})
+    cl.pop("energy_rbe_flag", None)
+    nb = cfg["hyper_parameters"]["neighbor"]
+    nb["type"] = "Verlet-List"
+    for key in ("r_core", "neighbor_sample_num", "energy_rbl_flag"):
+        nb.pop(key, None)
+    cfg["hyper_parameters"]["precision"] = "fp64"
+    ctx = _ctx(s, cfg)
+    m, idx = ctx.rbe_sample(0, sog=True)
+    mt = torch.tensor(np.concatenate([m.reshape(-1), idx]).astype(np.int32), device=device)
+    G = np.random.default_rng(8).normal(size=(s.n, 3))
+    Q = np.array([1.0, -1.0, 0.0])
+
+    def loss(pos, q=Q):
+        ctx.set_charges(q[s.types])
+        p = torch.tensor(pos, dtype=torch.float64, device=device)
+        f = [torch.zeros(s.n, dtype=torch.float64, device=device) for _ in range(3)]
+        ctx.force(0, *(p[:, d].contiguous() for d in range(3)), *f, m_samples=mt,
+                  num_k=len(idx), sync=True)
+        return float(sum((f[d].cpu().numpy() * G[:, d]).sum() for d in range(3)))
+
+    loss(s.pos)
+    p = torch.tensor(s.pos, dtype=torch.float64, device=device)
+    px, py, pz = (p[:, d].contiguous() for d in range(3))
+    g = torch.tensor(G, dtype=torch.float64, device=device)
+    dr = [torch.zeros_like(px) for _ in range(3)]
+    dp = torch.zeros(9, dtype=torch.float64, device=device)
+    ctx.force_vjp(0, px, py, pz, *(g[:, d].contiguous() for d in range(3)), *dr, dp,
+                  sync=True)
+    DR = np.stack([t.cpu().numpy() for t in dr], 1)
+    h = 1e-6
+    ad = np.array([(DR * dlt).sum() for dlt in directions])
+    fd = np.array([(loss(s.pos + h * dlt) - loss(s.pos - h * dlt)) / (2 * h)
+                   for dlt in directions])
+    fdq = np.zeros(2)
+    for t in range(2):
+        e = np.zeros(3)
+        e[t] = 1e-6
+        fdq[t] = (loss(s.pos, q=Q + e) - loss(s.pos, q=Q - e)) / 2e-6
+    return ad, fd, dp.cpu().numpy()[:2], fdq
+
+
+def test_vjp_sog_near_field_matches_finite_differences(device):
+    """The SOG near field in the adjoint (VERLET-SOG's and rbsog_near_field
+    "sog"): the sog-minus-erfc difference of dL/dr along random directions and
+    of dL/dQ per charged type equals that of central differences of the
+    forward (pinned to the oracle in tests/test_gpu_sog.py) within 1e-6
+    relative (1e-4 of the largest for dL/dQ).  The RBSOG k-space part, whose
+    FP32 coefficients make a single central difference noisy, is the same in
+    both systems and cancels; the erfc adjoint itself is pinned to the FP64
+    oracle adjoint above."""
+    rng = np.random.default_rng(9)
+    dirs = [rng.normal(size=(2000, 3)) for _ in range(3)]
+    ad0, fd0, dq0, fdq0 = _sog_adjoint_and_fd(device, "sog", dirs)
+    ad1, fd1, dq1, fdq1 = _sog_adjoint_and_fd(device, "erfc", dirs)
+    print("sog - erfc, adjoint", ad0 - ad1, "finite differences", fd0 - fd1)
+    print("dQ sog - erfc, adjoint", dq0 - dq1, "finite differences", fdq0 - fdq1)
+    assert np.abs(fd0 - fd1).min() > 1e-5 * np.abs(fd0).max()  # the near fields differ
+    np.testing.assert_allclose(ad0 - ad1, fd0 - fd1, rtol=1e-6)
+    dd = fdq0 - fdq1
+    np.testing.assert_allclose(dq0 - dq1, dd, atol=1e-4 * np.abs(dd).max())
