@@ -405,7 +405,8 @@ int rbmdx_batch_set_systems(rbmdx_batch* ctx, int32_t nrep, const int32_t* natom
 
 /* One evaluation of every replica.  Device arrays; m_samples holds
  * rbmdx_batch_num_samples() integer k-vectors (3 ints) per replica, replica
- * after replica (RBE; may be null without Coulomb).  step is the RBL
+ * after replica (RBE; null without Coulomb and with EWALD, whose kmax grid
+ * weighted exp(-k^2 / 4 alpha) the batch holds itself).  step is the RBL
  * sampling epoch.  Flags: RBMDX_FLAG_ENERGY, RBMDX_FLAG_SYNC,
  * RBMDX_FLAG_EAGER (the batch's kernels launched directly),
  * RBMDX_FLAG_PER_REPLICA (the same kernels, one replica per launch).  By

@@ -37,12 +37,20 @@ class Batch {
                   "the replica batch supports LJ/CUT and LJ/CUT/COUL/LONG (not CVFF)");
     if (c.neighbor_type != "RBL" && c.neighbor_type != "Verlet-List")
       throw Error(RBMDX_E_CONFIG, "the replica batch supports RBL and Verlet-List");
-    if (c.coulomb() && c.coulomb_type != "RBE")
-      throw Error(RBMDX_E_CONFIG, "the replica batch supports RBE k-space only");
+    if (c.coulomb() && c.coulomb_type != "RBE" && c.coulomb_type != "EWALD")
+      throw Error(RBMDX_E_CONFIG, "the replica batch supports RBE and EWALD k-space");
+    if (c.coulomb() && c.coulomb_type == "EWALD") {
+      if (!c.has_kmax) throw Error(RBMDX_E_CONFIG, "EWALD needs coulomb.kmax");
+      const long K = (2L * c.kmax[0] + 1) * (2L * c.kmax[1] + 1) * (2L * c.kmax[2] + 1) - 1;
+      if (K < 1 || K > kBatchMaxK)
+        throw Error(RBMDX_E_CONFIG,
+                    "the replica batch's EWALD grid must hold 1 to 512 k-vectors");
+    }
     if (c.coulomb() && !c.has_alpha)
       throw Error(RBMDX_E_CONFIG,
                   "the replica batch needs coulomb.alpha (one alpha for every replica)");
-    if (c.coulomb() && (c.coulomb_sample_num < 1 || c.coulomb_sample_num > kBatchMaxK))
+    if (c.coulomb() && c.coulomb_type == "RBE" &&
+        (c.coulomb_sample_num < 1 || c.coulomb_sample_num > kBatchMaxK))
       throw Error(RBMDX_E_CONFIG, "coulomb.sample_num must be in [1, 512] for the batch");
     HIPCHK(hipSetDevice(device_));
     hipDeviceProp_t prop;
@@ -155,6 +163,26 @@ class Batch {
     NQ_ = q0_[R];
     ncell_ = c0_[R];
     P_ = coul ? cfg_.coulomb_sample_num : 0;
+    d_.ewald = 0;
+    if (coul && cfg_.coulomb_type == "EWALD") {
+      // every m != 0 with |m_d| <= kmax_d, the same integer grid in every
+      // replica (KSpaceCalculator's EWALD k-set, weights on the device)
+      std::vector<int> grid;
+      for (int z = -cfg_.kmax[2]; z <= cfg_.kmax[2]; ++z)
+        for (int y = -cfg_.kmax[1]; y <= cfg_.kmax[1]; ++y)
+          for (int x = -cfg_.kmax[0]; x <= cfg_.kmax[0]; ++x)
+            if (x || y || z) {
+              grid.push_back(x);
+              grid.push_back(y);
+              grid.push_back(z);
+            }
+      P_ = (int)(grid.size() / 3);
+      std::vector<int> all;
+      all.reserve(grid.size() * R);
+      for (int r = 0; r < R; ++r) all.insert(all.end(), grid.begin(), grid.end());
+      ewald_m_ = upload(all);
+      d_.ewald = 1;
+    }
     // pair constants shared by every replica (thresholds formed in FP32 as
     // the single-system engine forms them)
     {
@@ -269,6 +297,10 @@ class Batch {
             float* fz, const int32_t* m) {
     if (!px || !py || !pz || !fx || !fy || !fz)
       throw Error(RBMDX_E_ARG, "null position or force pointer");
+    if (d_.ewald) {
+      if (m) throw Error(RBMDX_E_ARG, "EWALD uses the kmax grid: pass no k-samples");
+      m = ewald_m_;
+    }
     if (P_ > 0 && !m) throw Error(RBMDX_E_ARG, "RBE needs the k-samples of every replica");
     d_.px = px;
     d_.py = py;
@@ -445,6 +477,8 @@ class Batch {
     allocs_.push_back(p);
     return (T*)p;
   }
+  const int* ewald_m_ = nullptr;  // EWALD: the kmax grid for every replica
+
   template <typename T>
   const T* upload(const std::vector<T>& v) {
     T* p = alloc<T>(v.size());

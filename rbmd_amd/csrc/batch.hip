@@ -646,14 +646,15 @@ __global__ __launch_bounds__(256) void kb_kspace(BatchDev d, PairConst<float> pc
         const double ky = two_pi * m[3 * k + 1] / (double)p.box.L[1];
         const double kz = two_pi * m[3 * k + 2] / (double)p.box.L[2];
         const double k2 = kx * kx + ky * ky + kz * kz;
-        const double A = p.kw * (-4.0 * 3.141592653589793238463 * (double)pc.qqr2e) /
+        const double wk = d.ewald ? exp(-k2 / (4.0 * (double)pc.alpha)) : p.kw;
+        const double A = wk * (-4.0 * 3.141592653589793238463 * (double)pc.qqr2e) /
                          (p.volume * k2);
         kf0[k] = make_float4((float)m[3 * k], (float)m[3 * k + 1], (float)m[3 * k + 2],
                              (float)(A * re));
         kf1[k] = make_float4((float)kx, (float)ky, (float)kz, (float)(A * im));
         if (EFLAG) {
           const double base =
-              (double)pc.qqr2e * two_pi / p.volume * p.kw * (re * re + im * im) / k2;
+              (double)pc.qqr2e * two_pi / p.volume * wk * (re * re + im * im) / k2;
           const double cc = 2.0 * (1.0 / (4.0 * (double)pc.alpha) + 1.0 / k2);
           kv[0] += base;
           kv[1] += base * (1.0 - cc * kx * kx);

@@ -58,6 +58,8 @@ struct BatchDev {
   float* fy = nullptr;
   float* fz = nullptr;
   const int* m = nullptr;         // [R * P * 3] k-vectors, replica-major
+  int ewald = 0;                  // 1: the kmax grid weighted exp(-k^2 / 4 alpha)
+                                  // (EWALD) instead of the RBE weight kw
   // scratch
   int* cnt = nullptr;             // [ncell] atoms per cell (zero between calls)
   int* cntq = nullptr;            // [ncell] charged atoms per cell (likewise)

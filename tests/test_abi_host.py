@@ -178,7 +178,9 @@ def test_sog_sampler_stream_equals_oracle(level, P, seed):
 
 @pytest.mark.parametrize("edit,msg", [
     (lambda hp: hp.update(precision="fp64"), "FP32"),
-    (lambda hp: hp["coulomb"].update(type="EWALD"), "RBE k-space only"),
+    (lambda hp: hp["coulomb"].update(type="RBSOG", rbsog_level=1), "RBE and EWALD"),
+    (lambda hp: hp["coulomb"].update(type="EWALD"), "kmax"),
+    (lambda hp: hp["coulomb"].update(type="EWALD", kmax=[5, 5, 5]), "512 k-vectors"),
     (lambda hp: (hp["coulomb"].pop("alpha"), hp["coulomb"].update(accuracy=1e-4)),
      "coulomb.alpha"),
     (lambda hp: hp["coulomb"].update(coulomb_sample_num=600), "sample_num"),

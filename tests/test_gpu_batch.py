@@ -138,3 +138,42 @@ def test_batch_autotune_cache_roundtrip(device, tmp_path):
         b2.set_shape(**s)
         b2.force(1, px, py, pz, *f, m_samples=m, sync=True)
         same(_forces(f))
+
+
+def test_batch_ewald_matches_fp64_oracle_per_replica(device):
+    """EWALD k-space in the batch: the kmax grid (|m_d| <= 3, 342 k-vectors)
+    weighted exp(-k^2 / 4 alpha) in every replica, against the FP64 oracle of
+    each box alone (pair forces plus the weighted k-sum); forces 2e-5, k-space
+    energy and virial 1e-5."""
+    import oracle
+
+    alpha, kmax = 0.25, (3, 3, 3)
+    cfg = systems.ljsalt_config(neighbor="Verlet-List", coulomb="EWALD", kmax=kmax,
+                                alpha=alpha)
+    b, reps = _batch(cfg, SIZES[:4])
+    assert b.num_samples() == 7 ** 3 - 1
+    px, py, pz, f, _, _ = _tensors(device, reps, 1, 0)
+    b.force(0, px, py, pz, *f, energy=True, sync=True)
+    F = _forces(f)
+    E = b.energies()
+    off = b.offsets
+    g = np.arange(-3, 4)
+    m = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    m = m[np.any(m != 0, axis=1)]
+    for r, s in enumerate(reps):
+        pos32 = s.pos.astype(np.float32)
+        Fo, en, vir, _, _ = reference_force(s, pos32, cfg, 0, None, with_kspace=False)
+        ci = np.nonzero(s.charges)[0]
+        k2 = ((2 * np.pi * m / s.box_len) ** 2).sum(1)
+        fk, _, ek, vk = oracle.kspace_f64(pos32.astype(np.float64)[ci], s.charges[ci],
+                                          s.box_len, m, np.exp(-k2 / (4 * alpha)), 1.0,
+                                          alpha)
+        Fo[ci] += fk
+        V = float(np.prod(s.box_len))
+        ek += oracle.kspace_self((s.charges ** 2).sum(), s.charges.sum(), alpha, V, 1.0)
+        err = rel_rms(F[off[r]:off[r + 1]], Fo)
+        print(f"EWALD replica {r} N={s.n}: rel RMS {err:.2e}")
+        assert err < 2e-5
+        np.testing.assert_allclose(E[r, 2], ek, rtol=1e-5)
+        w = np.asarray(vir) + np.asarray(vk)
+        np.testing.assert_allclose(E[r, 3:], w, rtol=1e-5, atol=1e-5 * np.abs(w).max())
