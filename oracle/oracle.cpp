@@ -724,15 +724,24 @@ void orc_ewald_real_f64(int n, const double* x, const double* y,
 
 // KSpaceCalculator::DeriveAlphaFromAccuracy, kspace_calculator.cpp:120-157.
 // Returns alpha = g_ewald^2; q2 = qqr2e * sum q^2.
+// DeriveAlphaFromAccuracy (kspace_calculator.cpp:120-157) with the
+// reference's types: accuracy, g and q2 are double, the cutoff is float
+// (rbmd::Real), SQRT / LOG are sqrtf / logf (rbmd_define.h:199-209) so their
+// double arguments are rounded to float, and alpha = g * g is stored as float.
+// qqr2e and q2 = qqr2e * sum q^2 are float products (kspace_calculator.h:103,123;
+// kspace_calculator.cpp:740); q2 is taken as given and rounded to float.
+// Returns alpha as that float value.
 double orc_alpha_from_accuracy(double acc_rel, double qqr2e, double n_global,
-                               double rc, double V, double q2) {
-  const double acc = acc_rel * qqr2e;
-  double g = acc * std::sqrt(n_global * rc * V) / (2.0 * q2);
+                               double rc_in, double V, double q2_in) {
+  const float rc = (float)rc_in;
+  const double q2 = (double)(float)q2_in;
+  const double acc = (double)(float)acc_rel * (double)(float)qqr2e;
+  double g = acc * (double)sqrtf((float)(n_global * (double)rc * V)) / (2.0 * q2);
   if (g >= 1.0)
-    g = (1.35 - 0.15 * std::log(acc)) / rc;
+    g = (1.35 - 0.15 * (double)logf((float)acc)) / (double)rc;
   else
-    g = std::sqrt(-std::log(g)) / rc;
-  return g * g;
+    g = (double)(sqrtf(-logf((float)g)) / rc);
+  return (double)(float)(g * g);
 }
 
 // ---- RBE k-sampler ----------------------------------------------------------

@@ -957,15 +957,19 @@ class Engine : public EngineBase {
       if (c.has_alpha) {
         alpha_ = c.alpha;
       } else {
-        const double q2 = c.qqr2e * sum_q2_;
+        // q2 = qqr2e * sum q^2 as the float product of float factors
+        const double q2 = (double)((float)c.qqr2e * (float)sum_q2_);
         if (!(q2 > 0))
           throw Error(RBMDX_E_CONFIG,
                       "automatic alpha derivation requires a charged system");
-        const double acc = c.accuracy * c.qqr2e;
-        double g = acc * std::sqrt((double)n_ * c.cut_off * V) / (2.0 * q2);
-        g = g >= 1.0 ? (1.35 - 0.15 * std::log(acc)) / c.cut_off
-                     : std::sqrt(-std::log(g)) / c.cut_off;
-        alpha_ = g * g;
+        // the reference's types: double accuracy, g and q2, float cutoff,
+        // sqrtf / logf of double arguments rounded to float, float alpha
+        const float rcf = (float)c.cut_off;
+        const double acc = (double)(float)c.accuracy * (double)(float)c.qqr2e;
+        double g = acc * (double)sqrtf((float)((double)n_ * (double)rcf * V)) / (2.0 * q2);
+        g = g >= 1.0 ? (1.35 - 0.15 * (double)logf((float)acc)) / (double)rcf
+                     : (double)(sqrtf(-logf((float)g)) / rcf);
+        alpha_ = (double)(float)(g * g);
       }
     }
     charge_sums();  // self energy needs alpha

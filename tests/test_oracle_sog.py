@@ -1,9 +1,15 @@
 """Pins of the RBSOG / VERLET-SOG oracle (oracle/sog.cpp), checked against what
 the mathematics fixes rather than against a retyped formula:
 
+* the exact set's k-space coefficients against a numerical 3-D Fourier
+  transform of the real-space far field (itself pinned against 1/r), and its
+  virial coefficients against -4 df/dk^2;
+* the reference set's coefficients (its defect Q5: every level l >= 1 carries
+  an extra w0) against the exact set, level by level;
 * the normalisation S (and S_npt) of the exact set against a brute-force 3-D
-  lattice sum, and the reference's float S against an independent double
-  evaluation of its level-0-only analytic series;
+  lattice sum, and the reference's float S (its defect Q4: the analytic
+  high-k series runs for level 0 only) against an independent double
+  evaluation built from the pinned coefficients above;
 * the far-field Gaussian sum against 1/r beyond the cutoff, to the accuracy the
   level preset states;
 * the Taylor near-field force against the derivative of the Gaussian sum;
@@ -49,25 +55,99 @@ def test_exact_S_equals_brute_force_lattice_sum(level):
     assert s["S_npt"] == pytest.approx(Sn, rel=1e-10)
 
 
+@pytest.mark.parametrize("level,rc", [(0, 3.0), (1, 3.0), (1, 10.0)])
+def test_exact_coefficients_are_fourier_transform_of_far_field(level, rc):
+    """f(k) = sum_l coef_l exp(-b^2l sigma^2 k^2 / 2) is the 3-D Fourier
+    transform of the real-space far field F(r) (sog_far_r, pinned against 1/r
+    beyond the cutoff): 4 pi / k int_0^inf r F(r) sin(k r) dr, integrated
+    numerically.  A wrong level weight, width or normalisation of any level
+    fails at some k."""
+    from scipy.integrate import quad
+
+    s = oracle.sog_setup(level, rc, [40.0] * 3, exact=True)
+    f_small = oracle.sog_f(0.04, s)
+    for k in (0.2, 0.45, 0.6, 1.0, 1.5, 3.0):
+        fk = oracle.sog_f(k * k, s)
+        if fk < 1e-6 * f_small:
+            continue
+        v, _ = quad(lambda x: x * oracle.sog_far_r(x, s), 0, np.inf, weight="sin", wvar=k,
+                    limlst=200)
+        assert 4 * np.pi * v / k == pytest.approx(fk, rel=1e-6), k
+
+
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_exact_virial_coefficients_are_k2_derivative(level):
+    """The virial chain's f_npt(k^2) = -4 df/dk^2 (the strain derivative of
+    exp(-b^2l sigma^2 k^2 / 2) per level)."""
+    s = oracle.sog_setup(level, 3.5, [30.0] * 3, exact=True)
+    for k2 in (0.05, 0.4, 1.3):
+        h = 1e-5 * k2
+        d = (oracle.sog_f(k2 + h, s) - oracle.sog_f(k2 - h, s)) / (2 * h)
+        assert oracle.sog_f(k2, s, npt=True) == pytest.approx(-4 * d, rel=1e-7)
+
+
+@pytest.mark.parametrize("level,rc", [(0, 3.0), (1, 3.5), (1, 10.0), (2, 3.5)])
+def test_reference_coefficients_carry_w0_on_every_level(level, rc):
+    """The reference's coefficients (RBSOGInit, kspace_calculator.cpp:1006-1019):
+    its i == 1 special case is overwritten by coef[i] = coef[i-1] b^2, so every
+    level l >= 1 keeps the factor w0 of level 0.  Checked against the exact set
+    (pinned above): ref coef_0 = coef_0, ref coef_l = w0 coef_l for l >= 1, to
+    float rounding; the same for the virial coefficients."""
+    L = [30.0, 31.0, 33.0]
+    ex = oracle.sog_setup(level, rc, L, exact=True)
+    ref = oracle.sog_setup(level, rc, L)
+    assert ref["mmax"] == ex["mmax"]
+    # the reference's w0 is a float sum of 200 terms (Compute_W0): the exact
+    # w0 (pinned by the far field) to float accumulation error
+    w0 = ref["w0"]
+    assert w0 == pytest.approx(ex["w0"], rel=2e-5)
+    wl = np.r_[w0 / ex["w0"], np.full(ex["mmax"] - 1, w0)]
+    want = ex["coef"] * wl
+    want_n = ex["coef_npt"] * wl
+    np.testing.assert_allclose(ref["coef"], want, rtol=3e-6 * ex["mmax"])
+    np.testing.assert_allclose(ref["coef_npt"], want_n, rtol=3e-6 * ex["mmax"])
+    # w0 differs from 1 by far more than the tolerance: the defect is visible
+    assert abs(ex["w0"] - 1) > 1e-3
+
+
 @pytest.mark.parametrize("level", [0, 1])
 def test_reference_S_is_level0_series_minus_all_level_low_k(level):
-    """The reference's S: its analytic high-k series runs for level 0 only, the
-    low-k subtraction covers every level.  Recomputed here in double from the
-    reference's own (float) coefficients."""
+    """The reference's S (RBSOGSetup, kspace_calculator.cpp:1061-1126): its
+    analytic high-k series runs for level 0 only (the loop flag bool_bound is
+    never reset), the low-k subtraction (|m|^2 <= Kcut) covers every level.
+    Recomputed here in double from coefficients built from the exact set with
+    the reference's w0 factor (pinned above), not from the oracle's own
+    reference coefficients."""
     L = [20.0, 20.0, 20.0]
     s = oracle.sog_setup(level, 3.5, L)
-    sl2 = 2 * s["sigma"] ** 2  # sl[0]^2 = 2 sigma^2
+    ex = oracle.sog_setup(level, 3.5, L, exact=True)
+    w0 = s["w0"]  # the float w0, pinned to the exact one above
+    wl = np.r_[w0 / ex["w0"], np.full(ex["mmax"] - 1, w0)]
+    coef = ex["coef"] * wl
+    coef_n = ex["coef_npt"] * wl
+    sig2 = ex["sigma"] ** 2
     M = 60
     r = np.arange(-M, M + 1)
     mx, my, mz = np.meshgrid(r, r, r, indexing="ij")
     k2 = sum((2 * np.pi * m / Ld) ** 2 for m, Ld in zip((mx, my, mz), L))
-    # the series' exponent is (pi j / L)^2 sl^2 = k^2 sigma^2 / 2
-    high = s["coef"][0] * (k2 * np.exp(-k2 * sl2 / 4)).sum()
-    low = 0.0
+    # level 0 only: the series' exponent (pi j / L)^2 sl_0^2 = k^2 sigma^2 / 2
+    g0 = np.exp(-k2 * sig2 / 2)
+    high = coef[0] * (k2 * g0).sum()
+    high_n = coef_n[0] * (k2 * k2 * g0).sum()
+
+    def f_all(kk, c):
+        return sum(c[l] * np.exp(-ex["b"] ** (2 * l) * sig2 * kk / 2)
+                   for l in range(ex["mmax"]))
+
+    low = low_n = 0.0
     for m in ([1, 0, 0], [0, 1, 0], [0, 0, 1]):
         kk = _k2(m, L)
-        low += 2 * oracle.sog_f(kk, s) * kk
+        low += 2 * f_all(kk, coef) * kk
+        low_n += 2 * f_all(kk, coef_n) * kk * kk
     assert s["S"] == pytest.approx(high - low, rel=2e-5)
+    assert s["S_npt"] == pytest.approx(high_n - low_n, rel=2e-5)
+    # the exact S (every level's series) differs: the defect is visible
+    assert abs(ex["S"] - s["S"]) > 1e-3 * abs(ex["S"])
 
 
 @pytest.mark.parametrize("level", [0, 1, 2, 3])
