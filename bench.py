@@ -32,11 +32,27 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pair_traffic_electrolyte200k.md")
 
 
+# the sources that define the pair kernel; the traffic profile records their
+# hash, and a profile taken on other sources is not used
+PAIR_SOURCES = ("rbmd_amd/csrc/kernels.hip", "rbmd_amd/csrc/device.hpp",
+                "rbmd_amd/csrc/engine.hpp")
+
+
+def pair_source_hash():
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in PAIR_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(atoms):
     """HBM bytes per k_pair launch from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same counter's
-    reading for a copy kernel of known size), when it was taken on this
-    workload; None otherwise."""
+    (read and write bytes per dispatch from the memory-request counters), when
+    it was taken on this workload with the current pair-kernel sources; None
+    otherwise."""
     try:
         text = open(TRAFFIC_FILE).read()
     except OSError:
@@ -47,7 +63,7 @@ def measured_traffic(atoms):
             key, _, v = line.strip()[2:].partition(":")
             vals[key.strip()] = v.strip()
     try:
-        if int(vals["atoms"]) != atoms:
+        if int(vals["atoms"]) != atoms or vals["pair_source_sha256"] != pair_source_hash():
             return None
         return int(vals["traffic_bytes_per_launch"])
     except (KeyError, ValueError):
@@ -62,7 +78,7 @@ def parse():
     ap.add_argument("--atoms", type=int, default=200000)
     ap.add_argument("--dt", type=float, default=0.002)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--workload",
                     choices=["electrolyte", "adjoint", "replicas", "water", "lj", "md"],
                     default="electrolyte",
@@ -76,7 +92,11 @@ def parse():
                     help="replicas: JSON cache of the batch autotuner (default: measure)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB three times (a "
-                         "plain copy kernel of known size, to scale HBM counters)")
+                         "plain copy kernel of known size, to scale HBM counters), and "
+                         "gather it three times in a random order of 16-byte rows (a "
+                         "gather kernel of known size)")
+    ap.add_argument("--print-source-hash", action="store_true",
+                    help="print the hash of the pair-kernel sources and exit")
     return ap.parse_args()
 
 
@@ -430,6 +450,9 @@ def bench_md(args):
 
 def main():
     args = parse()
+    if args.print_source_hash:
+        print(pair_source_hash())
+        return
     if args.workload == "md":
         if args.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)
@@ -471,7 +494,10 @@ def main():
         vel.clamp_(-vmax, vmax)
         pos.add_(vel, alpha=dt)
 
-    for k in range(args.warmup):
+    # warm-up: W untimed steps, the first W - 20 with the velocity cap (relaxes
+    # the random start), the last 20 (or all W when fewer) with the timed call
+    n_leap = min(args.warmup, 20)
+    for k in range(args.warmup - n_leap):
         warm_step(k)
     torch.cuda.synchronize()
 
@@ -481,10 +507,16 @@ def main():
             ctx.force_leapfrog(k, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], frc[0],
                                frc[1], frc[2], dt, timing=timing)
 
+    # the timed call itself, warmed up: its graph (keyed on the velocity
+    # buffers and dt) is captured here, not inside the timed region
+    md_steps(args.warmup - n_leap, n_leap, False)
+    torch.cuda.synchronize()
+    first = args.warmup
+
     # timed region: no timing events (each event adds a gap on the stream)
     th0 = ctx.thermo()
     t0 = time.perf_counter()
-    md_steps(args.warmup, args.steps, False)
+    md_steps(first, args.steps, False)
     host_elapsed = time.perf_counter() - t0  # enqueue time on the host
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -495,7 +527,7 @@ def main():
     # events on the launch streams: average kernel durations
     ctx.kernel_times()
     t1 = time.perf_counter()
-    md_steps(args.warmup + args.steps, args.steps, True)
+    md_steps(first + args.steps, args.steps, True)
     torch.cuda.synchronize()
     elapsed_ev = time.perf_counter() - t1
     kt = ctx.kernel_times()
@@ -504,7 +536,7 @@ def main():
     # host cost of one call, measured over 10 calls from an idle device (fewer
     # than the 16 in-flight evaluations the context allows, so no call waits)
     def host_cost(eager):
-        base = args.warmup + 2 * args.steps + (100 if eager else 0)
+        base = first + 2 * args.steps + (100 if eager else 0)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for k in range(base, base + 10):
@@ -541,7 +573,16 @@ def main():
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        del src, dst
+        # gather: every 16-byte row read once, in a random order (index_select
+        # reads the int64 index, 8 B per row, and writes the rows in order)
+        rows = src.view(-1, 4)
+        perm = torch.randperm(rows.shape[0], device=dev)
+        out = torch.empty_like(rows)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.index_select(rows, 0, perm, out=out)
+        torch.cuda.synchronize()
+        del src, dst, rows, perm, out
 
     # force error vs FP64 and the CPU baseline (oracle restatement, one core)
     cpu = None
@@ -579,6 +620,17 @@ def main():
     print(json.dumps(line))
 
 
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo, as lscpu reports it)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_leg(ctx, sys_, cfg, pos, cpu_steps):
     """Oracle CPU restatement timed on a bounded sample, and the GPU force error
     against the oracle's FP64 evaluation at the same positions and k-sample."""
@@ -610,17 +662,27 @@ def cpu_leg(ctx, sys_, cfg, pos, cpu_steps):
     params = oracle.pair_params(nb["r_core"], nb["cut_off"], nb["lj_cut_off"], 0.09, 1.0, freq,
                                 step, 1, 1, 1)
     w = oracle.rbe_S(0.09, L) / P
-    t0 = time.perf_counter()
-    for k in range(cpu_steps):
-        oracle.force_step(pos32, sys_.charges.astype(np.float32), sys_.types, sys_.ids,
-                          sys_.eps, sys_.sigma, L, params, m, w)
-    el = time.perf_counter() - t0
-    cpu = {"value": round(n * cpu_steps / el, 1), "unit": "atom-steps/s", "cores": 1,
+    q32 = sys_.charges.astype(np.float32)
+    # the GPU run rebuilds its lists about once in 8 steps: the CPU sample is
+    # 8 steps with one rebuild, on one thread and on the host's cores (at
+    # most 16, the CPU share of one GPU)
+    cores = min(16, len(os.sched_getaffinity(0)))
+    runs = {}
+    for threads in (1, cores):
+        t0 = time.perf_counter()
+        oracle.cpu_force_steps(pos32, q32, sys_.types, sys_.ids, sys_.eps, sys_.sigma, L,
+                               params, nb["skin"], m, w, cpu_steps, 8, threads)
+        runs[threads] = time.perf_counter() - t0
+    el = runs[cores]
+    cpu = {"value": round(n * cpu_steps / el, 1), "unit": "atom-steps/s", "cores": cores,
            "kind": "port",
-           "sample": f"{cpu_steps} force evaluations of the same 200k-atom configuration "
-                     "(FP32 pair + FP64 k-space oracle, one thread, neighbour search each "
-                     "evaluation)",
-           "seconds": round(el, 2)}
+           "sample": f"{cpu_steps} force evaluations of the same 200k-atom configuration, "
+                     "a list rebuild every 8 (RBL candidate list rc + skin, FP32 pair, RBE "
+                     "k-space with FP32 phases and FP64 sums), OpenMP over atoms",
+           "seconds": round(el, 3),
+           "value_1_thread": round(n * cpu_steps / runs[1], 1),
+           "seconds_1_thread": round(runs[1], 3),
+           "cpu_model": cpu_model()}
     return cpu, ferr
 
 

@@ -57,6 +57,11 @@ def lib():
         L.orc_force_vjp_f64.argtypes = [ctypes.c_int, _D, _D, _D, _I, _I, ctypes.c_int, _D,
                                         _D, _D, _D, _D, ctypes.c_int, _I, _D, _D, _D, _D,
                                         _D, _D]
+        L.orc_cpu_force_steps.restype = None
+        L.orc_cpu_force_steps.argtypes = [ctypes.c_int, _F, _F, _F, _F, _I, _I, ctypes.c_int,
+                                          _D, _D, _D, _D, ctypes.c_double, ctypes.c_int, _I,
+                                          ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, _D, _D, _D]
         L.orc_pair_force_special.restype = None
         L.orc_pair_force_special.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F, _F, _F, _I,
                                              _I, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int,
@@ -348,6 +353,31 @@ def force_step(pos, q, types, gid, eps, sig, L, params, m, rbe_weight):
     f = [np.zeros(n) for _ in range(3)]
     lib().orc_force_step(n, xp, yp, zp, qp, tp, gp, len(eps), ep, sp, Lp, ppp,
                          len(mm) // 3, mp, rbe_weight, *[a.ctypes.data_as(_D) for a in f])
+    return np.stack(f, axis=1)
+
+
+def cpu_force_steps(pos, q, types, gid, eps, sig, L, params, skin, m, rbe_weight, nsteps,
+                    rebuild_every, threads):
+    """The CPU baseline program: nsteps force evaluations at fixed positions
+    over a candidate list (rc + skin) rebuilt every rebuild_every steps, RBL
+    epoch advancing per step, OpenMP over atoms with `threads` threads.
+    Returns the forces (N,3) of the last step."""
+    n = len(pos)
+    x, xp = _f(pos[:, 0])
+    y, yp = _f(pos[:, 1])
+    z, zp = _f(pos[:, 2])
+    qq, qp = _f(q)
+    t, tp = _i(types)
+    g, gp = _i(gid)
+    e, ep = _d(eps)
+    s, sp = _d(sig)
+    Ld, Lp = _d(L)
+    pp, ppp = _d(params)
+    mm, mp = _i(np.asarray(m).reshape(-1))
+    f = [np.zeros(n) for _ in range(3)]
+    lib().orc_cpu_force_steps(n, xp, yp, zp, qp, tp, gp, len(eps), ep, sp, Lp, ppp, skin,
+                              len(mm) // 3, mp, rbe_weight, nsteps, rebuild_every, threads,
+                              *[a.ctypes.data_as(_D) for a in f])
     return np.stack(f, axis=1)
 
 

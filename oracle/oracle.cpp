@@ -885,6 +885,213 @@ void orc_force_step(int n, const float* x, const float* y, const float* z,
   }
 }
 
+// ---- the CPU baseline: rebuild-amortised force steps over stored lists ------
+// The per-step force path as a CPU program, OpenMP over atoms, for timing on
+// the host cores: every `rebuild_every` steps a full candidate list within
+// rc + skin (the reference's single RBL candidate radius,
+// rbl_full_neighbor_list_builder.cpp:90-185), CSR per atom, from a cell grid;
+// every step the RBL core / selected-shell classification of the step's epoch
+// (rbl_full_neighbor_list_op.hip.cu:237-304), the FP32 pair force of
+// lj_cut_coul_kspace_op.hip.cu:569-682 with the mean-force fix
+// (lj_cut_coul_kspace.cpp:537-563), and the RBE k-space of the charged atoms
+// with FP32 phases and FP64 sums (kspace_calculator_op.hip.cu:238-289,
+// 678-738).  Each atom's force is summed by one thread in list order and each
+// k's structure factor in atom order, so the result does not depend on the
+// thread count.  Positions stay fixed (a timing sample); forces of the last
+// step are returned.
+void orc_cpu_force_steps(int n, const float* x, const float* y, const float* z,
+                         const float* q, const int* type, const int* gid,
+                         int ntypes, const double* eps, const double* sig,
+                         const double* L64, const double* pp, double skin, int K,
+                         const int* m, double rbe_weight, int nsteps,
+                         int rebuild_every, int nthreads, double* fx, double* fy,
+                         double* fz) {
+  PairParams p = unpack(pp);
+  const Thresholds t = thresholds(p);
+  const float L[3] = {(float)L64[0], (float)L64[1], (float)L64[2]};
+  const long long epoch0 = p.epoch;
+  const double r_list = p.rc + skin;
+  const float rl2 = (float)(r_list * r_list);
+  std::vector<float> e_ij((size_t)ntypes * ntypes), s_ij((size_t)ntypes * ntypes);
+  for (int a = 0; a < ntypes; ++a)
+    for (int b = 0; b < ntypes; ++b) {
+      e_ij[a * ntypes + b] = (float)std::sqrt(eps[a] * eps[b]);
+      s_ij[a * ntypes + b] = (float)((sig[a] + sig[b]) / 2.0);
+    }
+  // atoms are processed in cell order: order[a] = caller index of sorted atom
+  // a; lists hold sorted indices
+  std::vector<int> order(n), cell(n), tys(n), gids(n), cstart;
+  std::vector<float> xs(n), ys(n), zs(n), qs(n), ax(n), ay(n), az(n);
+  std::vector<long> off(n + 1, 0);
+  std::vector<int> nbr, cnt(n), ci;
+  std::vector<double> re(K), im(K), kxv(K), kyv(K), kzv(K);
+  const double V = L64[0] * L64[1] * L64[2];
+  int ncell[3] = {1, 1, 1};
+  for (int s = 0; s < nsteps; ++s) {
+    const bool rebuild = s % std::max(1, rebuild_every) == 0;
+    if (rebuild) {
+      for (int d = 0; d < 3; ++d) {
+        ncell[d] = (int)std::floor(L64[d] / r_list);
+        if (ncell[d] < 3) ncell[d] = 1;
+      }
+      const int ncells = ncell[0] * ncell[1] * ncell[2];
+      cstart.assign(ncells + 1, 0);
+      for (int i = 0; i < n; ++i) {
+        const float pos[3] = {x[i], y[i], z[i]};
+        int c[3];
+        for (int d = 0; d < 3; ++d) {
+          double f = (double)pos[d] / L64[d];
+          f -= std::floor(f);
+          c[d] = std::min(ncell[d] - 1, std::max(0, (int)(f * ncell[d])));
+        }
+        cell[i] = (c[2] * ncell[1] + c[1]) * ncell[0] + c[0];
+        ++cstart[cell[i] + 1];
+      }
+      for (int c = 0; c < ncells; ++c) cstart[c + 1] += cstart[c];
+      std::vector<int> fill(cstart.begin(), cstart.end() - 1);
+      for (int i = 0; i < n; ++i) order[fill[cell[i]]++] = i;
+      ci.clear();
+      for (int a = 0; a < n; ++a) {
+        const int i = order[a];
+        tys[a] = type[i];
+        gids[a] = gid[i];
+        qs[a] = q[i];
+        if (q[i] != 0.0f) ci.push_back(a);
+      }
+    }
+    // this step's positions in cell order
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int a = 0; a < n; ++a) {
+      xs[a] = x[order[a]];
+      ys[a] = y[order[a]];
+      zs[a] = z[order[a]];
+    }
+    if (rebuild) {
+      auto scan = [&](int a, auto&& emit) {
+        const int id = cell[order[a]];
+        const int cx = id % ncell[0], cy = (id / ncell[0]) % ncell[1],
+                  cz = id / (ncell[0] * ncell[1]);
+        const int ex = ncell[0] > 1, ey = ncell[1] > 1, ez = ncell[2] > 1;
+        for (int oz = -ez; oz <= ez; ++oz)
+          for (int oy = -ey; oy <= ey; ++oy)
+            for (int ox = -ex; ox <= ex; ++ox) {
+              const int jc = (((cz + oz + ncell[2]) % ncell[2]) * ncell[1] +
+                              (cy + oy + ncell[1]) % ncell[1]) * ncell[0] +
+                             (cx + ox + ncell[0]) % ncell[0];
+              for (int b = cstart[jc]; b < cstart[jc + 1]; ++b)
+                if (b != a && dist2_f32(xs[a], ys[a], zs[a], xs[b], ys[b], zs[b], L) < rl2)
+                  emit(b);
+            }
+      };
+      // one scan: each thread appends its atoms' rows to its own buffer,
+      // then the rows are copied into CSR order
+      const int nchunk = (n + 255) / 256;
+      std::vector<std::vector<int>> rows(nchunk);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+      for (int ch = 0; ch < nchunk; ++ch) {
+        std::vector<int>& r = rows[ch];
+        for (int a = ch * 256; a < std::min(n, ch * 256 + 256); ++a) {
+          const size_t before = r.size();
+          scan(a, [&](int b) { r.push_back(b); });
+          cnt[a] = (int)(r.size() - before);
+        }
+      }
+      for (int a = 0; a < n; ++a) off[a + 1] = off[a] + cnt[a];
+      nbr.resize(off[n]);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+      for (int ch = 0; ch < nchunk; ++ch)
+        std::copy(rows[ch].begin(), rows[ch].end(), nbr.begin() + off[ch * 256]);
+    }
+    p.epoch = epoch0 + s;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+    for (int a = 0; a < n; ++a) {
+      float fxi = 0, fyi = 0, fzi = 0;
+      const int ti = tys[a];
+      for (long k = off[a]; k < off[a + 1]; ++k) {
+        const int b = nbr[k];
+        const float dx = min_image_1<float>(xs[b] - xs[a], L[0]);
+        const float dy = min_image_1<float>(ys[b] - ys[a], L[1]);
+        const float dz = min_image_1<float>(zs[b] - zs[a], L[2]);
+        const float d2 = std::fmaf(dz, dz, std::fmaf(dy, dy, dx * dx));
+        if (!(d2 < t.rc2)) continue;
+        const int cls = p.rbl ? rbl_class(d2, t, p, gids[a], gids[b]) : 1;
+        const double w = pair_weight(d2, cls, t, p);
+        if (w == 0.0) continue;
+        float flj = 0, elj = 0, fcl = 0, ecl = 0;
+        if (d2 < t.rclj2)
+          lj_pair<float>(d2, e_ij[ti * ntypes + tys[b]], s_ij[ti * ntypes + tys[b]], flj,
+                         elj);
+        const float qq = qs[a] * qs[b];
+        if (p.coulomb && qq != 0.0f)
+          coul_pair<float>(d2, qq, (float)p.alpha, (float)p.qqr2e, fcl, ecl);
+        const float f = (float)w * (flj + fcl);
+        fxi += f * dx;
+        fyi += f * dy;
+        fzi += f * dz;
+      }
+      ax[a] = fxi;
+      ay[a] = fyi;
+      az[a] = fzi;
+    }
+    double mean[3] = {0, 0, 0};
+    if (p.mean_fix) {
+      for (int a = 0; a < n; ++a) {
+        mean[0] += ax[a];
+        mean[1] += ay[a];
+        mean[2] += az[a];
+      }
+      for (int d = 0; d < 3; ++d) mean[d] /= (double)n;
+    }
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int a = 0; a < n; ++a) {
+      const int i = order[a];
+      fx[i] = (double)ax[a] - mean[0];
+      fy[i] = (double)ay[a] - mean[1];
+      fz[i] = (double)az[a] - mean[2];
+    }
+    const int nc = (int)ci.size();
+    if (K > 0 && p.coulomb && nc > 0) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+      for (int k = 0; k < K; ++k) {
+        const float kx = (float)(2.0 * M_PI * m[3 * k] / L64[0]),
+                    ky = (float)(2.0 * M_PI * m[3 * k + 1] / L64[1]),
+                    kz = (float)(2.0 * M_PI * m[3 * k + 2] / L64[2]);
+        double sr = 0, si = 0;
+        for (int c = 0; c < nc; ++c) {
+          const int a = ci[c];
+          const float th = kx * xs[a] + ky * ys[a] + kz * zs[a];
+          sr += (double)(qs[a] * std::cos(th));
+          si += (double)(qs[a] * std::sin(th));
+        }
+        re[k] = sr;
+        im[k] = si;
+        kxv[k] = kx;
+        kyv[k] = ky;
+        kzv[k] = kz;
+      }
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+      for (int c = 0; c < nc; ++c) {
+        const int a = ci[c];
+        double gx = 0, gy = 0, gz = 0;
+        for (int k = 0; k < K; ++k) {
+          const float th =
+              (float)kxv[k] * xs[a] + (float)kyv[k] * ys[a] + (float)kzv[k] * zs[a];
+          const double k2 = kxv[k] * kxv[k] + kyv[k] * kyv[k] + kzv[k] * kzv[k];
+          const double g = rbe_weight * (-4.0 * M_PI * p.qqr2e * qs[a] / (V * k2)) *
+                           ((double)std::cos(th) * im[k] - (double)std::sin(th) * re[k]);
+          gx += g * kxv[k];
+          gy += g * kyv[k];
+          gz += g * kzv[k];
+        }
+        const int i = order[a];
+        fx[i] += gx;
+        fy[i] += gy;
+        fz[i] += gz;
+      }
+    }
+  }
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

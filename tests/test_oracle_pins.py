@@ -403,3 +403,35 @@ def test_rbe_force_unbiased_against_ewald():
     err = np.sqrt(((acc / calls - fe) ** 2).sum() / (fe ** 2).sum())
     assert err < 0.08
     assert eacc / calls == pytest.approx(ee, rel=0.05)
+
+
+def test_cpu_force_steps_equals_force_step_and_ignores_thread_count():
+    """The CPU baseline program (stored candidate lists, OpenMP over atoms,
+    FP32 phases in the k-space) computes the force of the oracle's one-shot
+    evaluation (cell search per call, FP64 k-space) at the last step's RBL
+    epoch, and gives the same bits on 1 and 4 threads."""
+    from rbmd_amd import rbe_sample_host, systems
+
+    s = systems.electrolyte(4000, seed=11)
+    L = s.box_len
+    pos32 = s.pos.astype(np.float32)
+    q32 = s.charges.astype(np.float32)
+    freq = oracle.rbl_frequency(s.n, L, 2.5, 3.5, 30)
+    m = rbe_sample_host(0.09, L, 200, 1, 0)
+    w = oracle.rbe_S(0.09, L) / 200
+
+    def pp(epoch):
+        return oracle.pair_params(2.5, 3.5, 2.5, 0.09, 1.0, freq, epoch, 1, 1, 1)
+
+    ref = oracle.force_step(pos32, q32, s.types, s.ids, s.eps, s.sigma, L, pp(7), m, w)
+    a = oracle.cpu_force_steps(pos32, q32, s.types, s.ids, s.eps, s.sigma, L, pp(5), 0.3, m,
+                               w, 3, 2, 1)
+    b = oracle.cpu_force_steps(pos32, q32, s.types, s.ids, s.eps, s.sigma, L, pp(5), 0.3, m,
+                               w, 3, 2, 4)
+    err = np.sqrt(((a - ref) ** 2).sum() / (ref ** 2).sum())
+    assert err < 1e-6
+    np.testing.assert_array_equal(a, b)
+    # a different last epoch selects other shell pairs
+    c = oracle.cpu_force_steps(pos32, q32, s.types, s.ids, s.eps, s.sigma, L, pp(5), 0.3, m,
+                               w, 2, 2, 1)
+    assert np.sqrt(((c - ref) ** 2).sum() / (ref ** 2).sum()) > 1e-4
