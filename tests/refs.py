@@ -78,3 +78,25 @@ def reference_force(sys_, pos32, cfg, step, m, prec=0, with_kspace=True):
         ek += oracle.kspace_self((sys_.charges ** 2).sum(), sys_.charges.sum(), alpha, V,
                                  qqr2e)
     return f, en, vir, ek, vk
+
+
+def coul_abs_scale(sys_, rc, alpha, qqr2e):
+    """Sum over charged pairs within rc of |E_pair| = 1/2 |q_i q_j| erfc(g r) / r
+    (both directions, g = sqrt(alpha)): the size of the terms the real-space
+    Coulomb energy adds up.  Their signs cancel in the total, so an FP32
+    pair evaluation is bounded relative to this sum, not to the total."""
+    from scipy.special import erfc
+
+    ci = np.nonzero(sys_.charges)[0]
+    p = sys_.pos[ci].astype(np.float64)
+    q = sys_.charges[ci].astype(np.float64)
+    L = np.asarray(sys_.box_len, dtype=np.float64)
+    tot = 0.0
+    for a in range(0, len(ci), 512):
+        d = p[None, :, :] - p[a:a + 512, None, :]
+        d -= L * np.round(d / L)
+        r = np.sqrt((d * d).sum(-1))
+        qq = np.abs(q[a:a + 512, None] * q[None, :])
+        m = (r > 0) & (r < rc)
+        tot += 0.5 * (qq[m] * erfc(np.sqrt(alpha) * r[m]) / r[m]).sum()
+    return qqr2e * tot
