@@ -223,6 +223,34 @@ int rbmdx_force(rbmdx_ctx* ctx, int64_t step, const void* px, const void* py,
                 void* stream);
 
 /*
+ * rbmdx_force without any device allocation.  No counterpart in the
+ * reference: this entry point is for callers that cannot allocate device
+ * memory during the call (tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as rbmdx_force; the kernels are
+ * the same and are launched one by one on `stream` (no graph instantiation).
+ * The per-evaluation scratch (per-block reduction partials, S(k) chunk
+ * partials, rho and the per-k coefficients) lives in a static per-device
+ * array of 4 MiB (held by every device the library runs on until the
+ * process ends) instead of the context's buffers; the neighbour lists and
+ * sorted arrays are the context's, allocated by rbmdx_set_system and the
+ * first rbmdx_force call, and are never grown here (an overflowed list is
+ * still evaluated exactly; the next rbmdx_force grows it).
+ * Errors: RBMDX_E_CAPACITY when the system's scratch exceeds 4 MiB (the
+ * 200,000-atom electrolyte at P = 200 needs about 0.23 MiB, 2,000,000 atoms
+ * at P = 400 about 2.9 MiB), RBMDX_E_STATE before the context's first
+ * rbmdx_force call, or with m_samples NULL before the seeded k-sample stream
+ * has started on an rbmdx_force call; RBMDX_FLAG_TIMING is rejected.
+ * Every call on one device shares that device's array and leaves its last
+ * contents there: calls that overlap in time on one device (other streams,
+ * other contexts, other host threads) race and must be serialised by the
+ * caller.
+ */
+int rbmdx_force_noalloc(rbmdx_ctx* ctx, int64_t step, const void* px,
+                        const void* py, const void* pz, void* fx, void* fy,
+                        void* fz, const int32_t* m_samples, int32_t num_k,
+                        uint32_t flags, void* stream);
+
+/*
  * Per-type masses (ntypes entries, default 1) for rbmdx_force_leapfrog.
  * Replaces the mass table RBMD's integrator reads (DeviceData _d_mass).
  */

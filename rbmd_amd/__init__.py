@@ -129,6 +129,8 @@ SIGNATURES = {
                                        ctypes.c_uint32, _P]),
     "rbmdx_force": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _I32,
                                    ctypes.c_uint32, _P]),
+    "rbmdx_force_noalloc": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _I32,
+                                           ctypes.c_uint32, _P]),
     "rbmdx_thermo": (ctypes.c_int, [_P, ctypes.POINTER(Thermo)]),
     "rbmdx_rbe_sample": (ctypes.c_int, [_P, _I64, _IP]),
     "rbmdx_num_samples": (_I32, [_P]),
@@ -349,6 +351,23 @@ class ForceContext:
             import torch
             stream = torch.cuda.current_stream(px.device).cuda_stream
         self._check(self.lib.rbmdx_force(
+            self._h, int(step), _tensor_ptr(px), _tensor_ptr(py), _tensor_ptr(pz),
+            _tensor_ptr(fx), _tensor_ptr(fy), _tensor_ptr(fz),
+            _tensor_ptr(m_samples), nk, flags, ctypes.c_void_p(stream)))
+        self.calls += 1
+
+    def force_noalloc(self, step, px, py, pz, fx, fy, fz, m_samples=None, energy=False,
+                      sync=False, rebuild=False, stream=None, num_k=None):
+        """force() through rbmdx_force_noalloc: no device allocation in the
+        call, the per-evaluation scratch in the library's static per-device
+        array (after one force() call has sized the lists)."""
+        flags = ((RBMDX_FLAG_ENERGY if energy else 0) | (RBMDX_FLAG_SYNC if sync else 0)
+                 | (RBMDX_FLAG_REBUILD if rebuild else 0))
+        nk = 0 if m_samples is None else int(num_k or m_samples.numel() // 3)
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(px.device).cuda_stream
+        self._check(self.lib.rbmdx_force_noalloc(
             self._h, int(step), _tensor_ptr(px), _tensor_ptr(py), _tensor_ptr(pz),
             _tensor_ptr(fx), _tensor_ptr(fy), _tensor_ptr(fz),
             _tensor_ptr(m_samples), nk, flags, ctypes.c_void_p(stream)))

@@ -50,6 +50,10 @@ struct EngineBase {
                      const void* pz, void* fx, void* fy, void* fz,
                      const int32_t* m, int32_t nk, uint32_t flags,
                      hipStream_t s) = 0;
+  virtual void force_noalloc(int64_t step, const void* px, const void* py,
+                             const void* pz, void* fx, void* fy, void* fz,
+                             const int32_t* m, int32_t nk, uint32_t flags,
+                             hipStream_t s) = 0;
   virtual void force_vjp(int64_t step, const void* px, const void* py,
                          const void* pz, const void* gx, const void* gy,
                          const void* gz, void* dx, void* dy, void* dz,
@@ -536,16 +540,92 @@ class Engine : public EngineBase {
   void force(int64_t step, const void* px, const void* py, const void* pz,
              void* fx, void* fy, void* fz, const int32_t* m, int32_t nk,
              uint32_t flags, hipStream_t s) override {
+    evaluate(step, px, py, pz, fx, fy, fz, m, nk, flags, s, false);
+  }
+
+  // The same evaluation without any device allocation: eager launches on
+  // the caller's stream (no graph instantiation), no list growth (an
+  // overflowed list is covered exactly by the pair kernel's stencil
+  // fallback), and the per-evaluation scratch in the static per-device array
+  // instead of the context's buffers.
+  void force_noalloc(int64_t step, const void* px, const void* py, const void* pz,
+                     void* fx, void* fy, void* fz, const int32_t* m, int32_t nk,
+                     uint32_t flags, hipStream_t s) override {
+    evaluate(step, px, py, pz, fx, fy, fz, m, nk, flags, s, true);
+  }
+
+  // Lays out the per-evaluation scratch of one force evaluation from `base`
+  // (nullptr: sizes only), every buffer 256-byte aligned, and points dd's
+  // buffers there: the pair kernel's and k_lists' per-block partials, the
+  // skin maxima, the mean pair force, and with k-space the S(k) chunk
+  // partials, rho and the per-k force coefficients.  Each is written and
+  // read within one evaluation (the list statistics only on a rebuild, which
+  // writes them first), so none needs clearing between calls.  Returns the
+  // bytes used; the sizes are those of the context's own buffers.
+  size_t scratch_layout(Dev<R>& dd, char* base) const {
+    size_t off = 0;
+    auto take = [&](auto*& p, size_t count) {
+      using T = std::remove_reference_t<decltype(*p)>;
+      off = (off + 255) / 256 * 256;
+      p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+      off += sizeof(T) * std::max<size_t>(1, count);
+    };
+    take(dd.pair_part, (size_t)11 * grid_.nbrick * grid_.psub);
+    take(dd.skin_part, kSkinBlocksMax);
+    take(dd.lista_part, grid_.nbrick);
+    take(dd.listb_part, std::max(1, list_b_blocks(dd.nc, grid_.block)));
+    take(dd.mean, 3);
+    const size_t nchunks = std::max(1, (d_.nc + kSkChunk - 1) / kSkChunk);
+    if (d_.sk_part) {
+      take(dd.sk_part, nchunks * d_.kcap);
+      take(dd.kf0, d_.kcap);
+      take(dd.kf1, d_.kcap);
+      take(dd.rho, d_.kcap);
+    }
+    if (d_.sk_ex) take(dd.sk_ex, nchunks * d_.kex);
+    return off;
+  }
+
+  void evaluate(int64_t step, const void* px, const void* py, const void* pz,
+                void* fx, void* fy, void* fz, const int32_t* m, int32_t nk,
+                uint32_t flags, hipStream_t s, bool noalloc) {
     if (!ready_) throw Error(RBMDX_E_STATE, "rbmdx_set_system first");
     if (!px || !py || !pz || !fx || !fy || !fz)
       throw Error(RBMDX_E_ARG, "null position or force pointer");
     HIPCHK(hipSetDevice(device_));
+    Dev<R> dd;
+    if (noalloc) {
+      Dev<R> probe = d_;
+      const size_t need = scratch_layout(probe, nullptr);
+      if (need > kNoAllocScratchBytes)
+        throw Error(RBMDX_E_CAPACITY,
+                    "rbmdx_force_noalloc: this system needs " + std::to_string(need) +
+                        " bytes of scratch; the static scratch holds " +
+                        std::to_string(kNoAllocScratchBytes));
+      if (d_.cap_a == 0)
+        throw Error(RBMDX_E_STATE,
+                    "rbmdx_force_noalloc: the context has no neighbour lists yet "
+                    "(its first rbmdx_force call sizes them)");
+      if (flags & RBMDX_FLAG_TIMING)
+        throw Error(RBMDX_E_ARG, "rbmdx_force_noalloc: RBMDX_FLAG_TIMING is not supported");
+      if (!m && (kspace_ == 1 || kspace_ == 3) && !pipeline_)
+        throw Error(RBMDX_E_STATE,
+                    "rbmdx_force_noalloc: the seeded k-sample stream starts on an "
+                    "rbmdx_force call; pass m_samples");
+      char* base = static_cast<char*>(noalloc_scratch(device_));
+      if (!base)
+        throw Error(RBMDX_E_STATE, "static scratch not resolved on this device");
+      dd = d_;
+      scratch_layout(dd, base);
+    }
     bind_positions(px, py, pz);
     d_.fx = (R*)fx;
     d_.fy = (R*)fy;
     d_.fz = (R*)fz;
     int force_rebuild = (flags & RBMDX_FLAG_REBUILD) ? 1 : 0;
-    if (d_.cap_a == 0) {
+    if (noalloc) {
+      // capacity stays as it is
+    } else if (d_.cap_a == 0) {
       plan_capacity(s);
     } else {
       grow_if_needed(s);
@@ -618,8 +698,23 @@ class Engine : public EngineBase {
     } else if (kspace_ == 2) {
       a.m_use = d_.m_dev;
     }
-    const bool timed = timing_ || (flags & RBMDX_FLAG_TIMING);
-    if (graph_on_ && !timed && !explicit_m && !(flags & RBMDX_FLAG_EAGER)) {
+    const bool timed = !noalloc && (timing_ || (flags & RBMDX_FLAG_TIMING));
+    if (noalloc) {
+      // the context's buffers with the scratch redirected, bound to this
+      // call's caller arrays
+      Dev<R> dn = d_;
+      dn.pair_part = dd.pair_part;
+      dn.skin_part = dd.skin_part;
+      dn.lista_part = dd.lista_part;
+      dn.listb_part = dd.listb_part;
+      dn.mean = dd.mean;
+      dn.sk_part = dd.sk_part;
+      dn.kf0 = dd.kf0;
+      dn.kf1 = dd.kf1;
+      dn.rho = dd.rho;
+      dn.sk_ex = dd.sk_ex;
+      HIPCHK(launch_force<R>(dn, a, s));
+    } else if (graph_on_ && !timed && !explicit_m && !(flags & RBMDX_FLAG_EAGER)) {
       const GraphKey key = graph_key(a);
       if (!graph_exec_ || std::memcmp(&key, &graph_key_, sizeof key) != 0)
         capture(a, key);
@@ -645,7 +740,7 @@ class Engine : public EngineBase {
     last_eflag_ = a.p.eflag != 0;
     if (flags & RBMDX_FLAG_SYNC) {
       HIPCHK(hipStreamSynchronize(s));
-      if (grow_if_needed(s)) {
+      if (!noalloc && grow_if_needed(s)) {
         // lists were outgrown: the result is still exact (stencil fallback);
         // capacity is now larger for the next call.
       }
@@ -1530,6 +1625,8 @@ int rbmdx_create(const char* json_text, int device, rbmdx_ctx** out) {
     if (device < 0 || device >= ndev)
       throw Error(RBMDX_E_ARG, "device ordinal out of range");
     HIPCHK(hipSetDevice(device));
+    // the allocation-free evaluation's static scratch on this device
+    HIPCHK(noalloc_scratch_init(device));
     if (c->cfg.fp64) {
       c->ed = new Engine<double>(c->cfg, device);
       c->eng.reset(c->ed);
@@ -1589,6 +1686,18 @@ int rbmdx_force(rbmdx_ctx* ctx, int64_t step, const void* px, const void* py,
   return guarded(ctx, [&] {
     ctx->eng->force(step, px, py, pz, fx, fy, fz, m_samples, num_k, flags,
                     (hipStream_t)stream);
+  });
+}
+
+int rbmdx_force_noalloc(rbmdx_ctx* ctx, int64_t step, const void* px, const void* py,
+                        const void* pz, void* fx, void* fy, void* fz,
+                        const int32_t* m_samples, int32_t num_k, uint32_t flags,
+                        void* stream) {
+  if (!ctx) return RBMDX_E_ARG;
+  return guarded(ctx, [&] {
+    if (!ctx->eng) throw Error(RBMDX_E_STATE, "context was not created");
+    ctx->eng->force_noalloc(step, px, py, pz, fx, fy, fz, m_samples, num_k, flags,
+                            (hipStream_t)stream);
   });
 }
 

@@ -243,6 +243,18 @@ struct StepArgs {
 // Launch one complete force evaluation (kernels.hip).
 template <typename R>
 hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s);
+
+// Static per-device scratch of the allocation-free force evaluation
+// (kernels.hip): a file-scope __device__ array of this many bytes holds the
+// per-evaluation partial results (reduction partials, S(k) chunk partials,
+// per-k coefficients) that a context otherwise allocates.  Every call on one
+// device shares the device's copy.
+constexpr size_t kNoAllocScratchBytes = (size_t)4 << 20;
+// Resolves the array's address on `device`, which must be the current
+// device; the first call per device does the lookup, later calls keep it.
+hipError_t noalloc_scratch_init(int device);
+// The address resolved for `device` by noalloc_scratch_init, else nullptr.
+void* noalloc_scratch(int device);
 // Launch only the (forced) rebuild: skin, binning, sort, gather, lists.
 template <typename R>
 hipError_t launch_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s);

@@ -31,6 +31,7 @@
 // fences write back the L2 of every XCD).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "device.hpp"
@@ -2767,4 +2768,37 @@ template hipError_t launch_vjp<float>(const Dev<float>&, const VjpDev<float>&,
                                       const StepArgs<float>&, hipStream_t);
 template hipError_t launch_vjp<double>(const Dev<double>&, const VjpDev<double>&,
                                        const StepArgs<double>&, hipStream_t);
+}  // namespace rbmdx
+
+// ---------------------------------------------------------------------------
+// Static scratch of the allocation-free force evaluation.  The array's name
+// is only a host-side stand-in: host code uses the address that
+// hipGetSymbolAddress returns on each device (each device holds its own copy
+// of the array), kept per device below.
+// ---------------------------------------------------------------------------
+namespace rbmdx {
+namespace {
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kScratchDevices = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[kScratchDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_ptr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!ptr) return hipErrorInvalidSymbol;
+  scratch_ptr[device] = ptr;
+  return hipSuccess;
+}
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  return scratch_ptr[device];
+}
 }  // namespace rbmdx
