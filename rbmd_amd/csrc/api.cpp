@@ -106,10 +106,9 @@ class Engine : public EngineBase {
         if (e) hipEventDestroy(e);
     for (auto e : slot_events_)
       if (e) hipEventDestroy(e);
-    if (ev_fork_) hipEventDestroy(ev_fork_);
-    if (ev_join_) hipEventDestroy(ev_join_);
-    if (ks_) hipStreamDestroy(ks_);
-    if (graph_exec_) hipGraphExecDestroy(graph_exec_);
+    for (auto& ge : graphs_)
+      for (auto x : ge.exec)
+        if (x) hipGraphExecDestroy(x);
     if (cap_stream_) hipStreamDestroy(cap_stream_);
     if (hdr_) hipHostFree(hdr_);
     if (stage_) hipHostFree(stage_);
@@ -716,9 +715,16 @@ class Engine : public EngineBase {
       HIPCHK(launch_force<R>(dn, a, s));
     } else if (graph_on_ && !timed && !explicit_m && !(flags & RBMDX_FLAG_EAGER)) {
       const GraphKey key = graph_key(a);
-      if (!graph_exec_ || std::memcmp(&key, &graph_key_, sizeof key) != 0)
-        capture(a, key);
-      HIPCHK(hipGraphLaunch(graph_exec_, s));
+      GraphEntry* ge = nullptr;
+      for (auto& e : graphs_)
+        if (std::memcmp(&key, &e.key, sizeof key) == 0) ge = &e;
+      if (!ge) ge = &capture(a, key);
+      ge->last_use = calls_;
+      // consecutive calls use different instantiated copies: relaunching an
+      // exec whose previous launch is still running costs the host about
+      // twice as much
+      HIPCHK(hipGraphLaunch(ge->exec[ge->next], s));
+      ge->next = (ge->next + 1) % kGraphCopies;
     } else {
       if (timed) {
         if (timing_used_ == timing_events_.size()) {
@@ -755,6 +761,14 @@ class Engine : public EngineBase {
     double dt;
     uint64_t gen;
   };
+  static constexpr int kGraphCopies = 2;  // executable copies launched in turn
+  static constexpr int kGraphCache = 4;   // captured evaluation shapes kept
+  struct GraphEntry {
+    GraphKey key{};
+    hipGraphExec_t exec[kGraphCopies] = {};
+    int next = 0;
+    int64_t last_use = 0;
+  };
   GraphKey graph_key(const StepArgs<R>& a) const {
     GraphKey g;
     std::memset(&g, 0, sizeof g);
@@ -770,11 +784,19 @@ class Engine : public EngineBase {
     return g;
   }
 
-  // Captures one evaluation (both streams) into a graph and instantiates it.
-  void capture(const StepArgs<R>& a, const GraphKey& key) {
-    if (graph_exec_) {
-      HIPCHK(hipGraphExecDestroy(graph_exec_));
-      graph_exec_ = nullptr;
+  // Captures one evaluation (both streams) into a graph and instantiates
+  // kGraphCopies executable copies of it, in a cache of kGraphCache keys (a
+  // periodic energy evaluation, say, keeps its own entry); the least
+  // recently used entry makes room, after the device has finished with it.
+  GraphEntry& capture(const StepArgs<R>& a, const GraphKey& key) {
+    if ((int)graphs_.size() >= kGraphCache) {
+      size_t lru = 0;
+      for (size_t q = 1; q < graphs_.size(); ++q)
+        if (graphs_[q].last_use < graphs_[lru].last_use) lru = q;
+      if (last_stream_set_) HIPCHK(hipStreamSynchronize(last_stream_));
+      for (auto x : graphs_[lru].exec)
+        if (x) HIPCHK(hipGraphExecDestroy(x));
+      graphs_.erase(graphs_.begin() + lru);
     }
     if (!cap_stream_) HIPCHK(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
     HIPCHK(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeRelaxed));
@@ -787,15 +809,21 @@ class Engine : public EngineBase {
       throw Error(RBMDX_E_HIP, std::string("graph capture failed: ") +
                                    hipGetErrorString(e1 != hipSuccess ? e1 : e2));
     }
-    const hipError_t e3 = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+    GraphEntry ge;
+    ge.key = key;
+    hipError_t e3 = hipSuccess;
+    for (int c = 0; c < kGraphCopies && e3 == hipSuccess; ++c)
+      e3 = hipGraphInstantiate(&ge.exec[c], g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (e3 != hipSuccess) {
-      graph_exec_ = nullptr;
+      for (auto x : ge.exec)
+        if (x) hipGraphExecDestroy(x);
       throw Error(RBMDX_E_HIP, std::string("graph instantiation failed: ") +
                                    hipGetErrorString(e3));
     }
-    graph_key_ = key;
     ++captures_;
+    graphs_.push_back(ge);
+    return graphs_.back();
   }
 
   void thermo(rbmdx_thermo_t* out) override {
@@ -947,8 +975,7 @@ class Engine : public EngineBase {
   bool slot_used_[kSlots] = {};
   StepHdr* hdr_ = nullptr;          // host view of the step-parameter ring
   bool graph_on_ = true;            // replay evaluations through a hipGraph
-  hipGraphExec_t graph_exec_ = nullptr;
-  GraphKey graph_key_{};
+  std::vector<GraphEntry> graphs_;
   hipStream_t cap_stream_ = nullptr;
   uint64_t gen_ = 0;                // bumped whenever device buffers change
   int64_t captures_ = 0;
@@ -957,8 +984,6 @@ class Engine : public EngineBase {
   int64_t calls_ = 0, forced_ = 0;
   int last_num_k_ = 0;
   bool timing_ = false;
-  hipStream_t ks_ = nullptr;  // k-space stream (default priority)
-  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
   std::vector<std::array<hipEvent_t, 6>> timing_events_;  // reusable pool
   size_t timing_used_ = 0;
   unsigned rebuild_thr_ = 0;
@@ -1261,6 +1286,19 @@ class Engine : public EngineBase {
     d_.xqc = alloc<R4>(nc);
     d_.fk = alloc<R4>(nc);
     d_.fb = alloc<R4>(nc);
+    // the k-space kernels' fixed order of the charged atoms: ascending caller
+    // index (zero charges stay zero, so the set is fixed for the system)
+    d_.xqs = alloc<R4>(nc);
+    d_.qpos = alloc<int>(n_);
+    {
+      std::vector<int> qp(n_, -1);
+      int k = 0;
+      if (nc > 0)
+        for (int i = 0; i < n_; ++i)
+          if ((R)q_[i] != R(0)) qp[i] = k++;
+      if (k != nc) throw Error(RBMDX_E_STATE, "charged-atom count mismatch");
+      HIPCHK(hipMemcpy(d_.qpos, qp.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
+    }
     d_.cnt_a = alloc<int>(n_);
     d_.cnt_b = alloc<int>(nc);
     d_.inv_mass = alloc<R>(ntypes_);
@@ -1429,14 +1467,6 @@ class Engine : public EngineBase {
     graph_on_ = !(gr && std::atoi(gr) == 0);
     ++gen_;
     timing_ = std::getenv("RBMDX_TIMING") && std::atoi(std::getenv("RBMDX_TIMING"));
-    // k-space beside the pair kernel on a second stream (RBMDX_KSTREAM=0:
-    // same stream, in order)
-    const char* kst = std::getenv("RBMDX_KSTREAM");
-    if (kspace_ && !(kst && std::atoi(kst) == 0) && !ks_) {
-      HIPCHK(hipStreamCreateWithFlags(&ks_, hipStreamNonBlocking));
-      HIPCHK(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
-      HIPCHK(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
-    }
     HIPCHK(hipDeviceSynchronize());
     ready_ = true;
   }
@@ -1527,9 +1557,6 @@ class Engine : public EngineBase {
     }
     a.e_self = e_self_;
     a.m_use = d_.m_dev;
-    a.ks = ks_;
-    a.ev_fork = ev_fork_;
-    a.ev_join = ev_join_;
     return a;
   }
 

@@ -158,8 +158,12 @@ struct Dev {
   int* gidc = nullptr;     // [nc]
   int* typec = nullptr;    // [nc]
   R4* xqc = nullptr;       // [nc] (x, y, z, q)
-  R4* fk = nullptr;        // [nc] k-space force
   R4* fb = nullptr;        // [nc] charged-pair (list B) force
+  // charged atoms in a fixed order (ascending caller index), independent of
+  // the cell sort: the k-space kernels' input and output
+  int* qpos = nullptr;     // [n] caller -> fixed charged index, -1 if neutral
+  R4* xqs = nullptr;       // [nc] (x, y, z, q) in the fixed order
+  R4* fk = nullptr;        // [nc] k-space force, fixed order
   // lists
   int cap_a = 0, cap_b = 0;
   uint16_t* list_a = nullptr;  // [cap_a * n] groups of 8 (list_a_index), stage byte offsets
@@ -231,8 +235,6 @@ struct StepArgs {
   int force_rebuild = 0;
   int count_only = 0;         // list builders only count (capacity planning)
   double e_self = 0.0;        // k-space self + background energy
-  hipStream_t ks = nullptr;   // k-space stream (runs beside the pair kernel)
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double dt = 0.0;            // fused leapfrog: x += dt v
   double dt_ftm = 0.0;        //                 v += dt ftm2v F / m
   int md_kick = 0;            // MD step: closing half-kick + kinetic energy

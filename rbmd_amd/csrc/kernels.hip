@@ -301,7 +301,9 @@ __device__ __forceinline__ void unpack_coord(int packed, int c[3]) {
 //   * per-block max |r - r_ref|^2 (minimum image), +inf when a rebuild is
 //     forced; k_bin folds the block maxima (the kernel boundary makes them
 //     visible to every block) into ctrl->maxdisp_bits;
-//   * xs[inv_perm[i]] = (r, type + 1/2 if charged), and the charged copy.
+//   * xs[inv_perm[i]] = (r, type + 1/2 if charged), and the charged copy;
+//   * xqs[qpos[i]] = (r, q): the charged atoms in a fixed order (ascending
+//     caller index) for the k-space kernels.
 // When k_bin then decides to rebuild, k_cellsort gathers again in the new
 // order.  A forced rebuild skips the gather here: the order of the last
 // rebuild may not exist (first evaluation, new system, new lists).
@@ -322,12 +324,19 @@ __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force) {
   }
   __syncthreads();
   float mx = 0.f;
-  if (!s_force) {
+  {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
          i += gridDim.x * blockDim.x) {
       const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
       const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
       const R z = wrap<R>(d.pz[i], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
+      // the charged atoms in their fixed order for the k-space kernels
+      // (independent of the sorted order, so also on a forced rebuild)
+      if (d.qpos) {
+        const int qc = d.qpos[i];
+        if (qc >= 0) d.xqs[qc] = make4<R>(x, y, z, d.charge[i]);
+      }
+      if (s_force) continue;
       const R4T<R> r = d.ref[i];
       // minimum image of r - r_ref as r_ref + (r - shift) - r_ref: the
       // shift also undoes a wrap across the box edge since the rebuild
@@ -348,9 +357,8 @@ __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force) {
       d.xs[s] = make4<R>(xu, yu, zu, (R)d.type[i] + (c >= 0 ? R(0.5) : R(0)));
       if (c >= 0) d.xqc[c] = make4<R>(xu, yu, zu, d.charge[i]);
     }
-  } else {
-    mx = __int_as_float(0x7f800000);
   }
+  if (s_force) mx = __int_as_float(0x7f800000);
   mx = wave_max(mx);
   if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = mx;
   __syncthreads();
@@ -364,10 +372,11 @@ __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force) {
 // Cell key per atom (caller order): key = 2 * cell + neutral, so that inside
 // every cell the charged atoms come first.  Arrival slots from atomics; the
 // in-cell order is made deterministic by k_cellsort.
+// Block `bid` of `nblk` binning blocks.
 template <typename R>
-__global__ __launch_bounds__(256) void k_bin(Dev<R> d, Box<R> b, Grid g,
-                                             int split, unsigned thr,
-                                             int nskin) {
+__device__ __forceinline__ void bin_block(const Dev<R>& d, const Box<R>& b,
+                                          const Grid& g, int split, unsigned thr,
+                                          int nskin, int bid, int nblk) {
   // rebuild decision, part 2: every block folds the skin-check maxima
   __shared__ float s_max[4];
   float m = 0.f;
@@ -377,11 +386,10 @@ __global__ __launch_bounds__(256) void k_bin(Dev<R> d, Box<R> b, Grid g,
   if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
   __syncthreads();
   m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
-  if (blockIdx.x == 0 && threadIdx.x == 0)
+  if (bid == 0 && threadIdx.x == 0)
     d.ctrl->maxdisp_bits = __float_as_uint(m);
   if (!(__float_as_uint(m) > thr)) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
-       i += gridDim.x * blockDim.x) {
+  for (int i = bid * blockDim.x + threadIdx.x; i < d.n; i += nblk * blockDim.x) {
     const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
     const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
     const R z = wrap<R>(d.pz[i], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
@@ -403,9 +411,9 @@ __global__ __launch_bounds__(256) void k_bin(Dev<R> d, Box<R> b, Grid g,
 // a batch issues at once), and one block scan of the thread totals of both
 // scans follows; the second read writes the starts and clears key_count for
 // the next rebuild.
-__global__ __launch_bounds__(1024) void k_scan(int* key_count, int* key_start,
-                                               int* cstart, int ncell,
-                                               Ctrl* ctrl, unsigned thr) {
+__device__ __forceinline__ void scan_block(int* key_count, int* key_start,
+                                           int* cstart, int ncell, Ctrl* ctrl,
+                                           unsigned thr) {
   if (!rebuild_now(ctrl, thr)) return;
   __shared__ int smem[33];
   const int t = threadIdx.x;
@@ -453,10 +461,10 @@ __global__ __launch_bounds__(1024) void k_scan(int* key_count, int* key_start,
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_scatter(Dev<R> d, unsigned thr) {
+__device__ __forceinline__ void scatter_block(const Dev<R>& d, unsigned thr, int bid,
+                                              int nblk) {
   if (!rebuild_now(d.ctrl, thr)) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
-       i += gridDim.x * blockDim.x)
+  for (int i = bid * blockDim.x + threadIdx.x; i < d.n; i += nblk * blockDim.x)
     d.perm[d.key_start[d.key[i]] + d.slot[i]] = i;
 }
 
@@ -1366,16 +1374,17 @@ __global__ __launch_bounds__(1024) void k_reduce(
   }
 }
 
-// S(k) partial sums: block (chunk of kSkChunk charged atoms) x (kSkTile k's).
-// m_src holds the K integer k-vectors, in device memory or, for the sampled
-// stream, in host-mapped staging memory; in that case m_copy (device) is
-// given and the chunk-0 block of every k tile copies its tile there for
-// k_sk_fin and later calls (likewise src_src -> src_copy, the RBSOG
-// virial-chain sources, when given).
+// S(k) partial sums of one block: charged atoms [chunk kSkChunk, ...) (in
+// the fixed order of xqs) x the kSkTile k's of k tile `tile`.  m_src holds
+// the K integer k-vectors, in device memory or, for the sampled stream, in
+// host-mapped staging memory; in that case m_copy (device) is given and the
+// chunk-0 block of every k tile copies its tile there for the reduction and
+// later calls (likewise src_src -> src_copy, the RBSOG virial-chain sources,
+// when given).
 template <typename R>
-__global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
-                                            const int* __restrict__ m_arg,
-                                            int from_ring, int with_src, int K) {
+__device__ __forceinline__ void sk_block(const Dev<R>& d, const Box<R>& b,
+                                         const int* __restrict__ m_arg, int from_ring,
+                                         int with_src, int K, int chunk, int tile) {
   // the evaluation's k-sample: either m_arg (device) or its slot of the
   // host-mapped ring (m, then the RBSOG virial-chain sources), copied to HBM
   const int* m_src = m_arg;
@@ -1390,8 +1399,8 @@ __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
       src_copy = d.src_dev;
     }
   }
-  __shared__ double smem[4 * 2 * kSkTile];
-  const int chunk = blockIdx.x, k0 = blockIdx.y * kSkTile;
+  __shared__ double smem[16 * 2 * kSkTile];
+  const int k0 = tile * kSkTile;
   const int kn = min(kSkTile, K - k0);
   float mx[kSkTile], my[kSkTile], mz[kSkTile];
   int mi[3 * kSkTile];
@@ -1419,7 +1428,7 @@ __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
   for (int kk = 0; kk < kSkTile; ++kk) re[kk] = im[kk] = 0.f;
   const int a0 = chunk * kSkChunk, a1 = min(d.nc, a0 + kSkChunk);
   for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
-    const R4T<R> pp = d.xqc[a];
+    const R4T<R> pp = d.xqs[a];
     float sx, sy, sz;
     frac_coords<R>(b, pp, sx, sy, sz);
     const float q = (float)pp.w;
@@ -1445,23 +1454,31 @@ __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b,
           make_double2(v[2 * kk], v[2 * kk + 1]);
 }
 
+// S(k) partials as a kernel of its own (grid: chunks x k tiles), for the
+// kmax grid of the energy diagnostic.
+template <typename R>
+__global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b, const int* __restrict__ m_arg,
+                                            int from_ring, int with_src, int K) {
+  sk_block<R>(d, b, m_arg, from_ring, with_src, K, (int)blockIdx.x, (int)blockIdx.y);
+}
+
 // FP64 reduction of the S(k) chunk partials (fixed order, one thread per k)
 // into rho_k and the per-k force coefficients
 //   A_k = w_k (-4 pi qqr2e) / (V k^2),  kf0 = (m, A Re rho), kf1 = (k, A Im rho)
 // and the k-space energy and virial (without the self terms):
 //   E = qqr2e (2 pi / V) sum_k w_k |rho|^2 / k^2,
 //   W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/(4 alpha) + 1/k^2)).
-// One block; the kernel boundary after k_sk makes every partial visible.
+// One block; the kernel boundary after the S(k) blocks makes every partial
+// visible.
 // vmode 0 (RBE, EWALD): W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/(4 alpha)
 // + 1/k^2)).  vmode 1 (RBSOG, kspace_calculator_op.hip.cu:1133-1222 and
 // :1308-1352): W_aa += E_k, and W_ab += tau_k |rho_s|^2 k_s,a k_s,b with
 // s = src[k] and tau_k = kvir[k].x, divided by |k_s|^4 when kvir[k].y != 0.
 template <typename R>
-__global__ __launch_bounds__(1024) void k_sk_fin(Dev<R> d, Box<R> b,
-                                                 const int* __restrict__ m,
-                                                 int K, int nchunks, Params p,
-                                                 int vmode,
-                                                 const int* __restrict__ src) {
+__device__ __forceinline__ void skfin_block(const Dev<R>& d, const Box<R>& b,
+                                            const int* __restrict__ m, int K,
+                                            int nchunks, const Params& p, int vmode,
+                                            const int* __restrict__ src) {
   const double V = p.volume;
   const double two_pi = 6.283185307179586476925;
   double kv[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1558,22 +1575,23 @@ __global__ __launch_bounds__(1024) void k_sk_exact(Dev<R> d, Box<R> b, int nchun
   if (threadIdx.x == 0) d.thermo[24] = p.qqr2e * two_pi / p.volume * e[0];
 }
 
-// RBE / Ewald force on the charged atoms: 64 atoms per block, the k range
-// split over kRbeWaves waves, reduced through LDS.
+// RBE / Ewald force on 64 charged atoms (block `bid`, the fixed order of
+// xqs and fk), the k range split over the block's waves (at most
+// kRbeWaves), reduced through LDS.
 template <typename R>
-__global__ __launch_bounds__(64 * kRbeWaves) void k_rbe_force(Dev<R> d,
-                                                              Box<R> b,
-                                                              int K) {
+__device__ __forceinline__ void rbe_block(const Dev<R>& d, const Box<R>& b, int K,
+                                          int bid) {
   __shared__ float s_f[kRbeWaves][3][64];
+  const int nw = (int)blockDim.x >> 6;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int c = blockIdx.x * 64 + lane;
-  const int per = (K + kRbeWaves - 1) / kRbeWaves;
+  const int c = bid * 64 + lane;
+  const int per = (K + nw - 1) / nw;
   const int k0 = wv * per, k1 = min(K, k0 + per);
   float fx = 0.f, fy = 0.f, fz = 0.f;
   float q = 0.f;
   if (c < d.nc) {
-    const R4T<R> p = d.xqc[c];
+    const R4T<R> p = d.xqs[c];
     float sx, sy, sz;
     frac_coords<R>(b, p, sx, sy, sz);
     q = (float)p.w;
@@ -1595,13 +1613,67 @@ __global__ __launch_bounds__(64 * kRbeWaves) void k_rbe_force(Dev<R> d,
   __syncthreads();
   if (wv == 0 && c < d.nc) {
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    for (int w = 0; w < kRbeWaves; ++w) {
+    for (int w = 0; w < nw; ++w) {
       sx += s_f[w][0][lane];
       sy += s_f[w][1][lane];
       sz += s_f[w][2][lane];
     }
     d.fk[c] = make4<R>((R)(q * sx), (R)(q * sy), (R)(q * sz), R(0));
   }
+}
+
+// k-space work that rides on the rebuild kernels' launches (their blocks
+// exit at once unless the evaluation rebuilds), so that one evaluation is one
+// stream of kernels: the S(k) partials after the skin kernel has written the
+// fixed-order charged copy, their reduction next, the RBE force after that.
+// The k-space kernels index the charged atoms in that fixed order, which a
+// rebuild does not change.
+struct KsArgs {
+  int K = 0;                    // k-vectors of the evaluation (0: none)
+  const int* m_arg = nullptr;   // device k-vectors for the S(k) blocks, or
+  int from_ring = 0;            // the step's ring slot (copied to m_dev)
+  int with_src = 0;             // RBSOG virial-chain sources in the slot
+  const int* m_fin = nullptr;   // k-vectors as the reduction reads them
+  const int* src = nullptr;     // RBSOG virial-chain sources (device)
+  int vmode = 0;                // virial form (0 RBE / EWALD, 1 RBSOG)
+  int nchunks = 0, ntiles = 0;  // S(k) blocks: nchunks x ntiles
+  int nrbe = 0;                 // RBE force blocks (64 charged atoms each)
+  Params p;
+};
+
+// Binning (blocks [0, nbin)) and the S(k) partials (the blocks after).
+template <typename R>
+__global__ __launch_bounds__(256) void k_bin_sk(Dev<R> d, Box<R> b, Grid g, int split,
+                                                unsigned thr, int nskin, int nbin,
+                                                KsArgs ks) {
+  if ((int)blockIdx.x < nbin) {
+    bin_block<R>(d, b, g, split, thr, nskin, blockIdx.x, nbin);
+    return;
+  }
+  const int q = (int)blockIdx.x - nbin;
+  sk_block<R>(d, b, ks.m_arg, ks.from_ring, ks.with_src, ks.K, q % ks.nchunks,
+              q / ks.nchunks);
+}
+
+// The cell scans (block 0) and the FP64 S(k) reduction (block 1).
+template <typename R>
+__global__ __launch_bounds__(1024) void k_scan_fin(Dev<R> d, Box<R> b, int ncell,
+                                                   unsigned thr, KsArgs ks) {
+  if (blockIdx.x == 0)
+    scan_block(d.key_count, d.key_start, d.cstart, ncell, d.ctrl, thr);
+  else
+    skfin_block<R>(d, b, ks.m_fin, ks.K, ks.nchunks, ks.p, ks.vmode, ks.src);
+}
+
+// The sorted-order scatter (blocks [0, nsc)) and the RBE force (the blocks
+// after).
+template <typename R>
+__global__ __launch_bounds__(256) void k_scatter_rbe(Dev<R> d, Box<R> b, unsigned thr,
+                                                     int nsc, KsArgs ks) {
+  if ((int)blockIdx.x < nsc)
+    scatter_block<R>(d, thr, blockIdx.x, nsc);
+  else
+    rbe_block<R>(d, b, ks.K, (int)blockIdx.x - nsc);
 }
 
 // v + 0.5 F / m dt ftm2v evaluated left to right, as RBMD's UpdateVelocity
@@ -1646,21 +1718,22 @@ __global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix,
       f.y -= (R)d.mean[1];
       f.z -= (R)d.mean[2];
     }
-    if (has_k || has_b) {
+    if (has_b) {
       const int c = d.c_of[i];
       if (c >= 0) {
-        if (has_b) {
-          const R4T<R> g = d.fb[c];
-          f.x += g.x;
-          f.y += g.y;
-          f.z += g.z;
-        }
-        if (has_k) {
-          const R4T<R> g = d.fk[c];
-          f.x += g.x;
-          f.y += g.y;
-          f.z += g.z;
-        }
+        const R4T<R> g = d.fb[c];
+        f.x += g.x;
+        f.y += g.y;
+        f.z += g.z;
+      }
+    }
+    if (has_k) {
+      const int qc = d.qpos[i];
+      if (qc >= 0) {
+        const R4T<R> g = d.fk[qc];
+        f.x += g.x;
+        f.y += g.y;
+        f.z += g.z;
       }
     }
     d.fx[i] = f.x;
@@ -1774,17 +1847,19 @@ int list_b_grid(const Params& p, int nc, const Grid& g) {
 
 template <typename R>
 hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, int nskin,
-                      hipStream_t s) {
+                      hipStream_t s, const KsArgs& ks = KsArgs()) {
   const Grid& g = a.grid;
   const unsigned thr = a.rebuild_thr;
   const int gb = blocks_for(d.n, kBlock, 2048);
   const int cap_a = a.count_only ? 0 : d.cap_a;
   const int cap_b = a.count_only ? 0 : d.cap_b;
-  hipLaunchKernelGGL((k_bin<R>), dim3(gb), dim3(kBlock), 0, s, d, a.box, g,
-                     a.p.split, thr, nskin);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, d.key_count,
-                     d.key_start, d.cstart, g.ncell, d.ctrl, thr);
-  hipLaunchKernelGGL((k_scatter<R>), dim3(gb), dim3(kBlock), 0, s, d, thr);
+  const bool kon = ks.K > 0;
+  hipLaunchKernelGGL((k_bin_sk<R>), dim3(gb + (kon ? ks.nchunks * ks.ntiles : 0)),
+                     dim3(kBlock), 0, s, d, a.box, g, a.p.split, thr, nskin, gb, ks);
+  hipLaunchKernelGGL((k_scan_fin<R>), dim3(kon ? 2 : 1), dim3(1024), 0, s, d, a.box,
+                     g.ncell, thr, ks);
+  hipLaunchKernelGGL((k_scatter_rbe<R>), dim3(gb + (kon ? ks.nrbe : 0)), dim3(kBlock), 0,
+                     s, d, a.box, thr, gb, ks);
   hipLaunchKernelGGL((k_cellsort<R>), dim3(blocks_for(g.ncell * 64, kBlock)),
                      dim3(kBlock), 0, s, d, a.box, g.ncell, thr, a.pc.ms);
   hipLaunchKernelGGL((k_lists<R>), dim3(g.nbrick + list_b_grid(a.p, d.nc, g)),
@@ -1817,47 +1892,39 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   if (a.ev[0]) hipEventRecord(a.ev[0], s);
   // (a forced rebuild comes through the step ring, like the epoch)
   hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(kBlock), 0, s, d, a.box, 0);
-  hipError_t e = build_part<R>(d, a, sb, s);
-  if (e != hipSuccess) return e;
-  const Grid& g = a.grid;
-  // k-space on the second stream, beside the pair kernel
+  // k-space blocks beside the rebuild kernels' blocks (one stream, no fork)
   const int K = p.kspace ? p.num_k : 0;
   const bool ks_on = K > 0 && d.nc > 0;
-  hipStream_t ks = a.ks ? a.ks : s;
+  KsArgs ks;
   if (ks_on) {
-    if (ks != s) {
-      hipEventRecord(a.ev_fork, s);
-      hipStreamWaitEvent(ks, a.ev_fork, 0);
-    }
-    if (a.ev[3]) hipEventRecord(a.ev[3], ks);
-    const int nchunks = (d.nc + kSkChunk - 1) / kSkChunk;
     const bool sog = p.kspace == 3;
-    const bool ring = a.m_from_ring;
-    hipLaunchKernelGGL((k_sk<R>), dim3(nchunks, (K + kSkTile - 1) / kSkTile),
-                       dim3(kBlock), 0, ks, d, a.box, a.m_use, ring ? 1 : 0,
-                       sog ? 1 : 0, K);
-    hipLaunchKernelGGL((k_sk_fin<R>), dim3(1), dim3(1024), 0, ks, d, a.box,
-                       ring ? (const int*)d.m_dev : a.m_use, K, nchunks, p,
-                       sog ? 1 : 0, sog ? (const int*)d.src_dev : nullptr);
-    hipLaunchKernelGGL((k_rbe_force<R>), dim3(blocks_for(d.nc, 64)),
-                       dim3(64 * kRbeWaves), 0, ks, d, a.box, K);
-    if (p.eflag && d.kex > 0) {
-      // energy_rbe_flag: the exact Ewald energy over the kmax grid
-      Dev<R> dx = d;
-      dx.sk_part = d.sk_ex;
-      dx.kcap = d.kex;
-      hipLaunchKernelGGL((k_sk<R>), dim3(nchunks, (d.kex + kSkTile - 1) / kSkTile),
-                         dim3(kBlock), 0, ks, dx, a.box, (const int*)d.m_ex, 0, 0,
-                         d.kex);
-      hipLaunchKernelGGL((k_sk_exact<R>), dim3(1), dim3(1024), 0, ks, d, a.box,
-                         nchunks, p);
-    }
-    if (a.ev[4]) hipEventRecord(a.ev[4], ks);
-    if (ks != s) hipEventRecord(a.ev_join, ks);
-  } else if (a.ev[3]) {
-    hipEventRecord(a.ev[3], s);
-    hipEventRecord(a.ev[4], s);
+    ks.K = K;
+    ks.from_ring = a.m_from_ring;
+    ks.m_arg = a.m_use;
+    ks.with_src = sog ? 1 : 0;
+    ks.m_fin = a.m_from_ring ? (const int*)d.m_dev : a.m_use;
+    ks.src = sog ? (const int*)d.src_dev : nullptr;
+    ks.vmode = sog ? 1 : 0;
+    ks.nchunks = (d.nc + kSkChunk - 1) / kSkChunk;
+    ks.ntiles = (K + kSkTile - 1) / kSkTile;
+    ks.nrbe = blocks_for(d.nc, 64);
+    ks.p = p;
   }
+  if (a.ev[3]) hipEventRecord(a.ev[3], s);
+  hipError_t e = build_part<R>(d, a, sb, s, ks);
+  if (e != hipSuccess) return e;
+  if (ks_on && p.eflag && d.kex > 0) {
+    // energy_rbe_flag: the exact Ewald energy over the kmax grid
+    const int nchunks = (d.nc + kSkChunk - 1) / kSkChunk;
+    Dev<R> dx = d;
+    dx.sk_part = d.sk_ex;
+    dx.kcap = d.kex;
+    hipLaunchKernelGGL((k_sk<R>), dim3(nchunks, (d.kex + kSkTile - 1) / kSkTile),
+                       dim3(kBlock), 0, s, dx, a.box, (const int*)d.m_ex, 0, 0, d.kex);
+    hipLaunchKernelGGL((k_sk_exact<R>), dim3(1), dim3(1024), 0, s, d, a.box, nchunks, p);
+  }
+  if (a.ev[4]) hipEventRecord(a.ev[4], s);
+  const Grid& g = a.grid;
   const size_t lj_bytes = 4 * sizeof(R) * (size_t)a.pc.ntypes * a.pc.ntypes;
   // ev[1] .. ev[2] bracket the pair kernel alone
   if (a.ev[1]) hipEventRecord(a.ev[1], s);
@@ -1904,7 +1971,6 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
                      g.nbrick * g.psub, p.eflag ? 1 : 0, d.thermo, d.mean, p.mean_fix, p.n_global,
                      d.lista_part, g.nbrick, d.listb_part,
                      list_b_grid(p, d.nc, g), d.ctrl, a.rebuild_thr);
-  if (ks_on && ks != s) hipStreamWaitEvent(s, a.ev_join, 0);
   hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,
                      s, d, p.mean_fix, ks_on ? 1 : 0, p.split ? 1 : 0, (R)a.dt,
                      (R)a.dt_ftm, a.md_kick, (R)a.ftm2v);
