@@ -1137,6 +1137,9 @@ class Engine : public EngineBase {
         const double waves = (double)nb * (grid_.block / 64);
         grid_.psub = std::max(1, std::min(16, (int)std::ceil(16384.0 / waves)));
       }
+      // mostly neutral: one more wave per brick block takes the charged
+      // pairs beside the list-A waves
+      grid_.qwave = (split_ && grid_.psub == 1 && grid_.block <= 448) ? 64 : 0;
       // brick-major cell numbering
       std::vector<int> c_of_xyz(grid_.ncell), xyz_of_c(grid_.ncell),
           b0(nb + 1);
@@ -1313,6 +1316,7 @@ class Engine : public EngineBase {
     }
     md_ready_ = false;
     md_state_ = nullptr;
+    d_.brick_tab = alloc<int>((size_t)grid_.nbrick * kBrickTabInts);
     d_.pair_part_blocks = grid_.nbrick * grid_.psub;
     d_.pair_part = alloc<double>((size_t)11 * grid_.nbrick * grid_.psub);
     d_.skin_part = alloc<float>(kSkinBlocksMax);

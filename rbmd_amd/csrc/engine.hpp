@@ -19,6 +19,10 @@ constexpr int kRbeWaves = 8;     // waves per RBE-force block (k split)
 constexpr int kMaxK = 1 << 20;   // k-vector capacity bound
 constexpr int kMaxSlots = 64;    // neighbourhood cells staged per brick
 constexpr int kChargedLanes = 16;  // lanes per charged atom (Coulomb pairs)
+// Per-brick slot table of the last rebuild (written by the list build, read
+// by the pair kernel): slot offsets [kMaxSlots + 1], first sorted rows, charged
+// counts and box-edge shifts [kMaxSlots each].
+constexpr int kBrickTabInts = 4 * kMaxSlots + 1;
 
 // Device control block: rebuild decision, overflow bookkeeping.
 struct Ctrl {
@@ -73,6 +77,10 @@ struct Grid {
   int nb[3];       // bricks per dimension
   int nbrick;
   int block;       // threads per brick block
+  // a wave of the pair kernel's brick block that takes the brick's charged
+  // pairs while the others take list A (0 or 64 threads; mostly neutral
+  // systems with one workgroup per brick)
+  int qwave = 0;
   int psub = 1;    // pair-kernel workgroups per brick (they share list B)
   const int* cell_of_coord = nullptr;  // [ncell] (z*ny+y)*nx+x -> cell id
   const int* coord_of_cell = nullptr;  // [ncell] cell id -> packed x|y<<10|z<<20
@@ -166,6 +174,7 @@ struct Dev {
   R4* fk = nullptr;        // [nc] k-space force, fixed order
   // lists
   int cap_a = 0, cap_b = 0;
+  int* brick_tab = nullptr;    // [nbrick * kBrickTabInts] slot tables (list build)
   uint16_t* list_a = nullptr;  // [cap_a * n] groups of 8 (list_a_index), stage byte offsets
   int* cnt_a = nullptr;
   int* list_b = nullptr;       // [nc * cap_b] row-major, charged compact index

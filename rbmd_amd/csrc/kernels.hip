@@ -213,28 +213,49 @@ __device__ __forceinline__ void stage_atoms(const Grid& g, const Box<R>& b,
                                             const R4T<R>* xs, R4T<R>* stage,
                                             const int* gid_s = nullptr,
                                             int* sgid = nullptr) {
-  // half a wave per slot: the slot's atoms are consecutive in sorted order
-  const int lane = threadIdx.x & 31;
-  for (int s = threadIdx.x >> 5; s < h.nslots; s += blockDim.x >> 5) {
-    const int o = t.off[s], len = t.off[s + 1] - o, f = t.first[s];
-    const int e = t.edge[s];
-    R sh[3];
+  // every thread takes stage entries tid, tid + T, ... in batches of
+  // kStageBatch: the entries' slots (binary search of the offsets) and
+  // source rows first, then all their loads, then the shifted stores, so
+  // that a thread waits for memory once per batch
+  constexpr int kStageBatch = 8;
+  const int total = t.off[h.nslots];
+  const int T = (int)blockDim.x;
+  for (int l0 = threadIdx.x; l0 < total; l0 += kStageBatch * T) {
+    int src[kStageBatch], sl[kStageBatch];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const int q = (e >> (2 * d)) & 3;
-      sh[d] = q == 0 ? -b.L[d] : (q == 2 ? b.L[d] : R(0));
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int l = min(l0 + u * T, total - 1);
+      sl[u] = slot_of_pos(t, h.nslots, l);
+      src[u] = t.first[sl[u]] + (l - t.off[sl[u]]);
     }
-    for (int j = lane; j < len; j += 32) {
-      R4T<R> p = xs[f + j];
-      p.x += sh[0];
-      p.y += sh[1];
-      p.z += sh[2];
+    R4T<R> p[kStageBatch];
+    int gs[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      p[u] = xs[src[u]];
+      gs[u] = sgid ? gid_s[src[u]] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int l = l0 + u * T;
+      if (l >= total) break;
+      const int e = t.edge[sl[u]];
+      R sh[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int q = (e >> (2 * d)) & 3;
+        sh[d] = q == 0 ? -b.L[d] : (q == 2 ? b.L[d] : R(0));
+      }
+      R4T<R> v = p[u];
+      v.x += sh[0];
+      v.y += sh[1];
+      v.z += sh[2];
       if (PAIR)
-        p.w = pair_w(p.w);
+        v.w = pair_w(v.w);
       else
-        p.w = p.w != (R)(int)p.w ? R(1e30) : R(0);
-      stage[o + j] = p;
-      if (sgid) sgid[o + j] = gid_s[f + j];
+        v.w = v.w != (R)(int)v.w ? R(1e30) : R(0);
+      stage[l] = v;
+      if (sgid) sgid[l] = gs[u];
     }
   }
   if (PAIR && threadIdx.x == 0) {
@@ -697,6 +718,16 @@ __device__ __forceinline__ void list_a_brick(const Dev<R>& d, const Box<R>& b,
   const int e1[3] = {1, 1, 1};
   const Neigh h = brick_neigh(g, brick, e1);
   build_slots(g, h, d.key_start, nullptr, false, t);
+  // the pair kernel reads this brick's slot table until the next rebuild
+  {
+    int* bt = d.brick_tab + (size_t)brick * kBrickTabInts;
+    for (int k = threadIdx.x; k < kMaxSlots + 1; k += blockDim.x) bt[k] = t.off[k];
+    for (int k = threadIdx.x; k < kMaxSlots; k += blockDim.x) {
+      bt[kMaxSlots + 1 + k] = t.first[k];
+      bt[2 * kMaxSlots + 1 + k] = t.nq[k];
+      bt[3 * kMaxSlots + 1 + k] = t.edge[k];
+    }
+  }
   const int total = t.off[h.nslots];
   // (the pair kernel keeps the last stage entry for its sentinel)
   const bool staged = total < StageCap<R>::value;
@@ -1149,8 +1180,22 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
   const int brick = xcd_brick(blockIdx.x - sub * g.nbrick, g.nbrick);
   const Neigh h = brick_neigh(g, brick, e1);
   Acc<R> acc;
+  // threads [0, TA) take list A; with a list-B wave (g.qwave), the threads
+  // after them take the charged pairs from the start
+  const int TA = (int)blockDim.x - g.qwave;
   if (sub == 0) {
-    build_slots(g, h, d.key_start, nullptr, false, t);
+    // the brick's slot table of the last rebuild (one coalesced read instead
+    // of the cell lookups and the block scan)
+    {
+      const int* bt = d.brick_tab + (size_t)brick * kBrickTabInts;
+      for (int k = threadIdx.x; k < kMaxSlots + 1; k += blockDim.x) t.off[k] = bt[k];
+      for (int k = threadIdx.x; k < kMaxSlots; k += blockDim.x) {
+        t.first[k] = bt[kMaxSlots + 1 + k];
+        t.nq[k] = bt[2 * kMaxSlots + 1 + k];
+        t.edge[k] = bt[3 * kMaxSlots + 1 + k];
+      }
+      __syncthreads();
+    }
     const int total = t.off[h.nslots];
     // the last stage entry is the sentinel
     const bool staged = total < StageCap<R>::value;
@@ -1159,7 +1204,7 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
                            SPECIAL ? sgid : nullptr);
     const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
     // phase 1: LJ over list A
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0 + (int)threadIdx.x; i < i1 && (int)threadIdx.x < TA; i += TA) {
       Acc<R> a;
       const R4T<R> xi = d.xs[i];
       const int ti = (int)xi.w;
@@ -1208,12 +1253,24 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
       }
     }
   }
-  // phase 2: charged atoms of the brick over list B
-  if (split) {
-    __syncthreads();
-    const int c0 = d.cstart[h.cell0], c1 = d.cstart[h.cell1];
-    const int grp = threadIdx.x / kChargedLanes, gl = threadIdx.x % kChargedLanes;
-    const int ngrp = blockDim.x / kChargedLanes;
+  // phase 2: charged atoms of the brick over list B, QL lanes per atom (8 in
+  // the mostly neutral hot variant, so that a brick's charged atoms take one
+  // pass; 16 where list B carries most of the work).  No LDS is read here but
+  // the LJ table, so a wave starts as soon as its list-A atoms are done.
+  const bool in_b = g.qwave ? (int)threadIdx.x >= TA : true;
+  if (split && in_b) {
+    constexpr int QL = (SHELL || SPECIAL) ? kChargedLanes : 8;
+    // the list-B wave takes an equal share of all charged atoms (list B reads
+    // no brick stage, and per-brick charged counts vary several-fold); without
+    // one, the brick's own charged atoms, shared by its g.psub workgroups
+    const int c0 = g.qwave ? (int)((long long)blockIdx.x * d.nc / gridDim.x)
+                           : d.cstart[h.cell0];
+    const int c1 = g.qwave ? (int)((long long)(blockIdx.x + 1) * d.nc / gridDim.x)
+                           : d.cstart[h.cell1];
+    // the list-B wave alone, or (without one) every thread after list A
+    const int tb = g.qwave ? (int)threadIdx.x - TA : (int)threadIdx.x;
+    const int grp = tb / QL, gl = tb % QL;
+    const int ngrp = (g.qwave ? g.qwave : (int)blockDim.x) / QL;
     for (int c = c0 + sub * ngrp + grp; c < c1; c += ngrp * g.psub) {
       Acc<R> a;
       const R4T<R> xi = d.xqc[c];
@@ -1223,16 +1280,14 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
       const SpecRow sr = SPECIAL ? spec_row<R>(d, pc.ms, d.c2s[c]) : SpecRow();
       if (cnt <= d.cap_b) {
         const int* lp = d.list_b + (size_t)c * d.cap_b;
-        int k = gl;
-        // variants without the occupancy hint (shell pairs in list A,
-        // special bonds: the list-B-heavy systems) keep four entries'
-        // gathers in flight before any of them is evaluated
-        if (SHELL || SPECIAL)
-        for (; k + 3 * kChargedLanes < cnt; k += 4 * kChargedLanes) {
+        // four entries per lane at a time: their indices, then their
+        // gathers, are all in flight before any of them is evaluated
+        // (indices past the count are clamped, so every load issues)
+        for (int k0 = gl; k0 < cnt; k0 += 4 * QL) {
           int e[4], tj[4], gj[4];
           R4T<R> xj[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) e[u] = lp[k + u * kChargedLanes];
+          for (int u = 0; u < 4; ++u) e[u] = lp[min(k0 + u * QL, cnt - 1)];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             xj[u] = d.xqc[e[u]];
@@ -1241,14 +1296,9 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
-            interact_b<R, RBL, EFLAG, SPECIAL>(pc, b, s_lj, ti, xi, hi, xj[u], tj[u],
-                                               d.hjc + e[u], a, sr, gj[u]);
-        }
-        for (; k < cnt; k += kChargedLanes) {
-          const int e = lp[k];
-          interact_b<R, RBL, EFLAG, SPECIAL>(pc, b, s_lj, ti, xi, hi, d.xqc[e],
-                                             d.typec[e], d.hjc + e, a, sr,
-                                             SPECIAL ? d.gidc[e] : 0);
+            if (k0 + u * QL < cnt)
+              interact_b<R, RBL, EFLAG, SPECIAL>(pc, b, s_lj, ti, xi, hi, xj[u], tj[u],
+                                                 d.hjc + e[u], a, sr, gj[u]);
         }
       } else {
         const int i = d.c2s[c];
@@ -1258,7 +1308,7 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
         const Span sy = make_span(cc[1], g.ext_b[1], g.nc[1]);
         const Span sz = make_span(cc[2], g.ext_b[2], g.nc[2]);
         const int nst = sx.count * sy.count * sz.count;
-        for (int s = gl; s < nst; s += kChargedLanes) {
+        for (int s = gl; s < nst; s += QL) {
           const int kx = s % sx.count, ky = (s / sx.count) % sy.count,
                     kz = s / (sx.count * sy.count);
           const int cell =
@@ -1275,10 +1325,10 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
       }
       // reduce over the group's lanes (fixed tree)
       R fx = a.fx, fy = a.fy, fz = a.fz;
-      for (int o = kChargedLanes / 2; o > 0; o >>= 1) {
-        fx += __shfl_xor(fx, o, kChargedLanes);
-        fy += __shfl_xor(fy, o, kChargedLanes);
-        fz += __shfl_xor(fz, o, kChargedLanes);
+      for (int o = QL / 2; o > 0; o >>= 1) {
+        fx += __shfl_xor(fx, o, QL);
+        fy += __shfl_xor(fy, o, QL);
+        fz += __shfl_xor(fz, o, QL);
       }
       if (gl == 0) d.fb[c] = make4<R>(fx, fy, fz, R(0));
       acc.fx += a.fx;
@@ -1930,8 +1980,8 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   if (a.ev[1]) hipEventRecord(a.ev[1], s);
 #define RBMDX_PAIR(RBLv, EFv, SHv, SPv)                                     \
   hipLaunchKernelGGL((k_pair<R, RBLv, EFv, SHv, SPv>), dim3(g.nbrick * g.psub), \
-                     dim3(g.block), lj_bytes, s, d, a.box, g, a.pc, a.lj_tab, \
-                     p.split)
+                     dim3(g.block + g.qwave), lj_bytes, s, d, a.box, g, a.pc, \
+                     a.lj_tab, p.split)
   // list A holds RBL shell pairs only when the LJ cutoff exceeds r_core
   const bool shell_a = a.pc.rclj2 > a.pc.rs2;
   if (p.special) {
