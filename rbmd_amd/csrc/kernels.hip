@@ -652,27 +652,32 @@ __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
           const int s0 = srow + st.at(0, kx);
           const int l0 = t.off[s0], l1 = t.off[s0 + runlen];
           if (STAGED) {
-            for (int l = l0; l < l1; l += 4) {
-              R4T<R> xj[4];
+            // chunks of up to 64 candidates: first the accept mask (no
+            // branches), then the accepted entries in candidate order
+            for (int cb = l0; cb < l1; cb += 64) {
+              const int ce = min(l1, cb + 64);
+              unsigned long long ok = 0;
+              for (int l = cb; l < ce; l += 4) {
+                R4T<R> xj[4];
 #pragma unroll
-              for (int u = 0; u < 4; ++u) xj[u] = stage[min(l + u, l1 - 1)];
-              unsigned ok = 0;
+                for (int u = 0; u < 4; ++u) xj[u] = stage[min(l + u, ce - 1)];
 #pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const int lu = l + u;
-                R dx, dy, dz;
-                const R d2 = pair_d2<R, MI>(b, xi.x, xi.y, xi.z, xj[u].x, xj[u].y,
-                                            xj[u].z, dx, dy, dz);
-                // charged-charged pairs belong to list B: the stage's .w
-                // (1e30 for a charged j) pushes them past the cutoff
-                const bool keep = lu < l1 && fma_r(qf, xj[u].w, d2) < r2 &&
-                                  lu != self;
-                ok |= keep ? (1u << u) : 0u;
+                for (int u = 0; u < 4; ++u) {
+                  const int lu = l + u;
+                  R dx, dy, dz;
+                  const R d2 = pair_d2<R, MI>(b, xi.x, xi.y, xi.z, xj[u].x, xj[u].y,
+                                              xj[u].z, dx, dy, dz);
+                  // charged-charged pairs belong to list B: the stage's .w
+                  // (1e30 for a charged j) pushes them past the cutoff
+                  const bool keep = lu < ce && fma_r(qf, xj[u].w, d2) < r2 &&
+                                    lu != self;
+                  ok |= keep ? (1ull << (lu - cb)) : 0ull;
+                }
               }
               while (ok) {
-                const int u = __builtin_ctz(ok);
+                const int u = __builtin_ctzll(ok);
                 ok &= ok - 1;
-                accept(l + u);
+                accept(cb + u);
               }
             }
           } else {
