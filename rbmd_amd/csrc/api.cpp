@@ -570,9 +570,10 @@ class Engine : public EngineBase {
       off += sizeof(T) * std::max<size_t>(1, count);
     };
     take(dd.pair_part, (size_t)11 * grid_.nbrick * grid_.psub);
+    if (grid_.qblocks) take(dd.pairb_part, (size_t)11 * grid_.qblocks);
     take(dd.skin_part, kSkinBlocksMax);
     take(dd.lista_part, grid_.nbrick);
-    take(dd.listb_part, std::max(1, list_b_blocks(dd.nc, grid_.block)));
+    take(dd.listb_part, std::max(1, list_b_count(grid_, dd.nc)));
     take(dd.mean, 3);
     const size_t nchunks = std::max(1, (d_.nc + kSkChunk - 1) / kSkChunk);
     if (d_.sk_part) {
@@ -703,6 +704,7 @@ class Engine : public EngineBase {
       // call's caller arrays
       Dev<R> dn = d_;
       dn.pair_part = dd.pair_part;
+      dn.pairb_part = dd.pairb_part;
       dn.skin_part = dd.skin_part;
       dn.lista_part = dd.lista_part;
       dn.listb_part = dd.listb_part;
@@ -940,6 +942,7 @@ class Engine : public EngineBase {
  private:
   Config cfg_;
   int device_;
+  int num_cu_ = 256;  // compute units of the device (dense charged-pair grid)
   bool ready_ = false;
   int n_ = 0, ntypes_ = 0;
   double lo_[3], L_[3];
@@ -1093,6 +1096,11 @@ class Engine : public EngineBase {
       }
     }
     charge_sums();  // self energy needs alpha
+    {
+      hipDeviceProp_t prop;
+      HIPCHK(hipGetDeviceProperties(&prop, device_));
+      num_cu_ = std::max(1, prop.multiProcessorCount);
+    }
     // interaction radii and grid
     const double rc = c.cut_off;
     const double rclj = c.lj_cut_off;
@@ -1128,18 +1136,17 @@ class Engine : public EngineBase {
       grid_.nbrick = nb;
       const int per = (int)std::ceil(1.25 * apc * grid_.bw * grid_.bw * grid_.bw);
       grid_.block = std::min(512, std::max(64, (per + 63) / 64 * 64));
-      // mostly charged systems put nearly all pairs in list B, whose gathers
-      // want more waves in flight than one workgroup per brick gives: share
-      // each brick's charged atoms among up to 16 workgroups (about 16
-      // waves per SIMD over the chip's 1024 SIMDs)
+      // mostly charged systems put nearly all pairs in list B: kernels of
+      // their own take them, one wave per charged atom, with one resident
+      // round of 5 waves per SIMD (contiguous runs of atoms per wave)
       grid_.psub = 1;
-      if (split_ && 2 * (long)nc > (long)n_) {
-        const double waves = (double)nb * (grid_.block / 64);
-        grid_.psub = std::max(1, std::min(16, (int)std::ceil(16384.0 / waves)));
-      }
+      grid_.qblocks = 0;
+      if (split_ && 2 * (long)nc > (long)n_)
+        grid_.qblocks = std::max(1, std::min((nc + kQWaves - 1) / kQWaves,
+                                             num_cu_ * 4 * 5 / kQWaves));
       // mostly neutral: one more wave per brick block takes the charged
       // pairs beside the list-A waves
-      grid_.qwave = (split_ && grid_.psub == 1 && grid_.block <= 448) ? 64 : 0;
+      grid_.qwave = (split_ && grid_.qblocks == 0 && grid_.block <= 448) ? 64 : 0;
       // brick-major cell numbering
       std::vector<int> c_of_xyz(grid_.ncell), xyz_of_c(grid_.ncell),
           b0(nb + 1);
@@ -1319,9 +1326,10 @@ class Engine : public EngineBase {
     d_.brick_tab = alloc<int>((size_t)grid_.nbrick * kBrickTabInts);
     d_.pair_part_blocks = grid_.nbrick * grid_.psub;
     d_.pair_part = alloc<double>((size_t)11 * grid_.nbrick * grid_.psub);
+    d_.pairb_part = grid_.qblocks ? alloc<double>((size_t)11 * grid_.qblocks) : nullptr;
     d_.skin_part = alloc<float>(kSkinBlocksMax);
     d_.lista_part = alloc<int4>(grid_.nbrick);
-    d_.listb_part = alloc<int2>(std::max(1, list_b_blocks(nc, grid_.block)));
+    d_.listb_part = alloc<int2>(std::max(1, list_b_count(grid_, nc)));
     d_.thermo = alloc<double>(32);
     d_.mean = alloc<double>(3);
     d_.ctrl = alloc<Ctrl>(1);
@@ -1581,14 +1589,19 @@ class Engine : public EngineBase {
     ++gen_;
     // list A is read 8 entries at a time: capacity is a multiple of 8
     const int cap_a = ((int)std::ceil(max_a * 1.2) + 16 + 7) / 8 * 8;
-    const int cap_b = (int)std::ceil(max_b * 1.2) + 16;
+    // list B rows start 16-byte aligned (the dense build stores 4 entries
+    // at a time): capacity is a multiple of 4
+    const int cap_b = ((int)std::ceil(max_b * 1.2) + 16 + 3) / 4 * 4;
     if (d_.list_a) release(d_.list_a);
     if (d_.list_b) release(d_.list_b);
+    if (d_.list_t) release(d_.list_t);
+    d_.list_t = nullptr;
     d_.list_a = alloc<uint16_t>((size_t)cap_a * n_);
     d_.cap_a = cap_a;
     if (d_.nc > 0) {
       d_.list_b = alloc<int>((size_t)cap_b * d_.nc);
       d_.cap_b = cap_b;
+      if (grid_.qblocks) d_.list_t = alloc<unsigned char>((size_t)cap_b * d_.nc);
     } else {
       d_.cap_b = 0;
     }

@@ -19,6 +19,7 @@ constexpr int kRbeWaves = 8;     // waves per RBE-force block (k split)
 constexpr int kMaxK = 1 << 20;   // k-vector capacity bound
 constexpr int kMaxSlots = 64;    // neighbourhood cells staged per brick
 constexpr int kChargedLanes = 16;  // lanes per charged atom (Coulomb pairs)
+constexpr int kQWaves = 4;  // waves per block of the dense charged-pair kernel
 // Per-brick slot table of the last rebuild (written by the list build, read
 // by the pair kernel): slot offsets [kMaxSlots + 1], first sorted rows, charged
 // counts and box-edge shifts [kMaxSlots each].
@@ -82,10 +83,19 @@ struct Grid {
   // systems with one workgroup per brick)
   int qwave = 0;
   int psub = 1;    // pair-kernel workgroups per brick (they share list B)
+  // blocks of the dense charged-pair kernel (one wave per charged atom) that
+  // takes list B in mostly charged systems; 0: list B inside the pair kernel
+  int qblocks = 0;
   const int* cell_of_coord = nullptr;  // [ncell] (z*ny+y)*nx+x -> cell id
   const int* coord_of_cell = nullptr;  // [ncell] cell id -> packed x|y<<10|z<<20
   const int* brick_cell0 = nullptr;    // [nbrick+1] first cell id of a brick
 };
+
+// Blocks of the charged-pair list build: one per cell in mostly charged
+// systems, else kChargedLanes lanes per charged atom.
+inline int list_b_count(const Grid& g, int nc) {
+  return g.qblocks ? g.ncell : list_b_blocks(nc, g.block);
+}
 
 template <typename R>
 struct PairConst {
@@ -179,11 +189,14 @@ struct Dev {
   int* cnt_a = nullptr;
   int* list_b = nullptr;       // [nc * cap_b] row-major, charged compact index
   int* cnt_b = nullptr;
+  // list B of a mostly charged system: the type of each entry's atom
+  unsigned char* list_t = nullptr;  // [nc * cap_b]
   // pair partial sums per block: 3 force + 2 energy + 6 virial
   float* skin_part = nullptr;   // [skin blocks] per-block max |dr|^2
   int4* lista_part = nullptr;   // [nbrick] (max count, sum, stage size, unstaged)
   int2* listb_part = nullptr;   // [list-B blocks] (max count, sum)
   double* pair_part = nullptr;
+  double* pairb_part = nullptr;  // [11 * qblocks] dense charged-pair partials
   int pair_part_blocks = 0;
   // k-space
   int kcap = 0;

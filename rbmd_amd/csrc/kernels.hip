@@ -842,9 +842,237 @@ __device__ __forceinline__ void list_b_block(const Dev<R>& d, const Box<R>& b,
   }
 }
 
+// Charged-pair candidate list of a mostly charged system: one block per
+// cell, one lane per charged atom of the cell (rounds of blockDim atoms).
+// The charged atoms of the stencil cells are staged in LDS, blockDim at a
+// time, double-buffered (the next chunk's loads are in flight while the
+// current one is scanned) and padded to whole groups of 64 with far-away
+// sentinels.  Every lane tests 64 staged atoms at a time with broadcast
+// reads into an acceptance mask; the accepted ones are appended in order
+// (stencil-cell order, ascending within a cell) with their types in list_t,
+// by the wave one atom at a time with coalesced stores when some lane of the
+// wave accepted more than kDenseCoop of the 64 (near cells), else by every
+// lane for itself.  Across a periodic edge the pair vector takes the
+// stencil cell's shift by L, which is exactly the minimum-image shift of
+// each of its pairs when the dimension has at least 4 cells (both are one
+// subtraction of L from r_j - r_i); with fewer the minimum image is applied
+// per pair.
+constexpr int kDenseCoop = 12;
+
+template <typename R, bool MI>
+__device__ __forceinline__ unsigned long long dense_mask(const Box<R>& b,
+                                                         const R4T<R>& xi,
+                                                         const R4T<R>* st, R r2,
+                                                         R sx, R sy, R sz) {
+  unsigned long long m = 0;
+#pragma unroll
+  for (int u = 0; u < 64; ++u) {
+    const R4T<R> xj = st[u];
+    R dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+    if (MI) {
+      dx = min_image<R>(dx, b.L[0], b.half[0]);
+      dy = min_image<R>(dy, b.L[1], b.half[1]);
+      dz = min_image<R>(dz, b.L[2], b.half[2]);
+    } else {
+      dx -= sx;
+      dy -= sy;
+      dz -= sz;
+    }
+    const R d2 = fma_r(dz, dz, fma_r(dy, dy, mul_rn(dx, dx)));
+    m |= (d2 < r2 ? 1ull : 0ull) << u;
+  }
+  return m;
+}
+
+// The stencil cell `st` of a dense list-B block: its charged range, its
+// periodic shift (0 or +-L per dimension) and whether its pairs need the
+// minimum image per pair (a dimension of fewer than 4 cells).
+template <typename R>
+struct DenseCell {
+  int j0, j1;
+  bool mi;
+  R s[3];
+};
+template <typename R>
+__device__ __forceinline__ DenseCell<R> dense_cell(const Grid& g, const Box<R>& b,
+                                                   const int* cstart, const Span* sp,
+                                                   int st) {
+  const int k3[3] = {st % sp[0].count, (st / sp[0].count) % sp[1].count,
+                     st / (sp[0].count * sp[1].count)};
+  DenseCell<R> dc;
+  dc.mi = false;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int u = sp[q].start + k3[q];
+    dc.mi = dc.mi || sp[q].n < 4 || sp[q].count == sp[q].n;
+    // a cell below 0 is the top cell of the box: its atoms sit about L above
+    // the atom's image (r_j - r_i - L), and one above n - 1 about L below
+    dc.s[q] = u < 0 ? b.L[q] : (u >= sp[q].n ? -b.L[q] : R(0));
+  }
+  const int cl = g.cell_of_coord[(sp[2].at(k3[2]) * g.nc[1] + sp[1].at(k3[1])) * g.nc[0] +
+                                 sp[0].at(k3[0])];
+  dc.j0 = cstart[cl];
+  dc.j1 = cstart[cl + 1];
+  return dc;
+}
+
+template <typename R>
+__device__ __forceinline__ void list_b_dense(const Dev<R>& d, const Box<R>& b,
+                                             const Grid& g, int cell, R r2, int cap,
+                                             int ntypes) {
+  __shared__ R4T<R> st_x[2][512];
+  __shared__ unsigned char st_t[2][512];
+  const int cs = d.cstart[cell], ce = d.cstart[cell + 1];
+  int cc[3];
+  unpack_coord(g.coord_of_cell[cell], cc);
+  const Span sp[3] = {make_span(cc[0], g.ext_b[0], g.nc[0]),
+                      make_span(cc[1], g.ext_b[1], g.nc[1]),
+                      make_span(cc[2], g.ext_b[2], g.nc[2])};
+  const int nst = sp[0].count * sp[1].count * sp[2].count;
+  const R far = sizeof(R) == 4 ? R(1e30) : R(1e200);
+  const int bd = (int)blockDim.x, t = (int)threadIdx.x, lane = t & 63;
+  // type bits (types < 32)
+  const int nbt = ntypes > 1 ? 32 - __builtin_clz((unsigned)(ntypes - 1)) : 0;
+  int mx = 0, sum = 0;
+  for (int i0 = cs; i0 < ce; i0 += bd) {
+    const int c = i0 + t;
+    const bool act = c < ce;
+    const R4T<R> xi = d.xqc[act ? c : cs];
+    int* row = d.list_b + (size_t)(act ? c : cs) * cap;
+    unsigned char* trow = d.list_t + (size_t)(act ? c : cs) * cap;
+    int pos = 0;
+    // the staged chunks in order: (stencil cell, first atom); (st, jb) is
+    // the chunk whose loads are in flight
+    int st = 0, jb = 0;
+    DenseCell<R> dc;
+    auto seek = [&]() {
+      // from (st, jb) to the first chunk with atoms, st == nst at the end
+      while (st < nst) {
+        dc = dense_cell<R>(g, b, d.cstart, sp, st);
+        if (jb == 0) jb = dc.j0;
+        if (jb < dc.j1) return;
+        ++st;
+        jb = 0;
+      }
+    };
+    seek();
+    R4T<R> px = make4<R>(far, far, far, R(0));
+    int pt = 0;
+    if (st < nst && t < dc.j1 - jb) {
+      px = d.xqc[jb + t];
+      pt = d.typec[jb + t];
+    }
+    int buf = 0;
+    while (st < nst) {
+      const int nj = min(dc.j1 - jb, bd);
+      const int npad = (nj + 63) & ~63;
+      const int cjb = jb;
+      const DenseCell<R> cdc = dc;
+      // buffer `buf` was last read two chunks ago, before the barrier of the
+      // previous chunk
+      if (t < npad) {
+        st_x[buf][t] = t < nj ? px : make4<R>(far, far, far, R(0));
+        st_t[buf][t] = (unsigned char)(t < nj ? pt : 0);
+      }
+      __syncthreads();
+      // the next chunk's loads, in flight during the scan
+      if (jb + bd < dc.j1) {
+        jb += bd;
+      } else {
+        ++st;
+        jb = 0;
+        seek();
+      }
+      if (st < nst && t < dc.j1 - jb) {
+        px = d.xqc[jb + t];
+        pt = d.typec[jb + t];
+      } else {
+        px = make4<R>(far, far, far, R(0));
+      }
+      for (int k0 = 0; k0 < npad; k0 += 64) {
+        const int tyl = st_t[buf][k0 + lane];  // the type of staged atom k0 + lane
+        unsigned long long m = 0;
+        if (act) {
+          m = cdc.mi ? dense_mask<R, true>(b, xi, st_x[buf] + k0, r2, R(0), R(0), R(0))
+                     : dense_mask<R, false>(b, xi, st_x[buf] + k0, r2, cdc.s[0], cdc.s[1],
+                                            cdc.s[2]);
+          const int self = c - (cjb + k0);
+          if (self >= 0 && self < 64) m &= ~(1ull << self);
+        }
+        const int na = __popcll(m);
+        if (wave_max_i(na) > kDenseCoop) {
+          // many accepted: the wave writes each atom's entries of the chunk
+          // with one coalesced store, lane k taking staged atom k0 + k
+          const int abase = i0 + (t & ~63);
+          for (int l = 0; l < 64; ++l) {
+            const unsigned lo = __builtin_amdgcn_readlane((unsigned)m, l);
+            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(m >> 32), l);
+            if ((lo | hi) == 0u) continue;
+            const unsigned long long ml = ((unsigned long long)hi << 32) | lo;
+            if ((ml >> lane) & 1ull) {
+              const int r = __builtin_amdgcn_readlane(pos, l) +
+                            (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+              if (r < cap) {
+                const size_t o = (size_t)(abase + l) * cap + r;
+                d.list_b[o] = cjb + k0 + lane;
+                d.list_t[o] = (unsigned char)tyl;
+              }
+            }
+          }
+        } else {
+          // few: each lane appends its own, the types from per-chunk bit
+          // planes (one ballot per type bit)
+          unsigned long long pl[5];
+#pragma unroll
+          for (int q = 0; q < 5; ++q) pl[q] = q < nbt ? __ballot((tyl >> q) & 1) : 0ull;
+          int p = pos;
+          while (m) {
+            const int u = __builtin_ctzll(m);
+            m &= m - 1;
+            unsigned ty = 0;
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+              if (q < nbt) ty |= (unsigned)((pl[q] >> u) & 1ull) << q;
+            if (p < cap) {
+              row[p] = cjb + k0 + u;
+              trow[p] = (unsigned char)ty;
+            }
+            ++p;
+          }
+        }
+        pos += na;
+      }
+      buf ^= 1;
+    }
+    if (act) {
+      d.cnt_b[c] = pos;
+      mx = max(mx, pos);
+      sum += pos;
+    }
+  }
+  // per-block statistics (folded by k_reduce)
+  __shared__ int s_red[2 * 8];
+  mx = wave_max_i(mx);
+  sum = wave_sum(sum);
+  if ((threadIdx.x & 63) == 0) {
+    s_red[2 * (threadIdx.x >> 6)] = mx;
+    s_red[2 * (threadIdx.x >> 6) + 1] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      mx = max(mx, s_red[2 * w]);
+      sum += s_red[2 * w + 1];
+    }
+    d.listb_part[cell] = make_int2(mx, sum);
+  }
+}
+
 // Both candidate lists in one launch (they are independent): blocks
 // [0, nbrick) build list A of one brick each, the blocks after them list B,
 // so the charged-pair build fills the machine beside the brick builds.
+// In mostly charged systems list B has a kernel of its own (k_lists_dense,
+// one block per cell) and this launch holds the list-A blocks alone.
 template <typename R>
 __global__ __launch_bounds__(512) void k_lists(Dev<R> d, Box<R> b, Grid g,
                                                R r2a, R r2b, int split,
@@ -855,6 +1083,13 @@ __global__ __launch_bounds__(512) void k_lists(Dev<R> d, Box<R> b, Grid g,
     list_a_brick<R>(d, b, g, xcd_brick(blockIdx.x, g.nbrick), r2a, split, cap_a);
   else
     list_b_block<R>(d, b, g, blockIdx.x - g.nbrick, r2b, cap_b);
+}
+
+template <typename R>
+__global__ __launch_bounds__(512) void k_lists_dense(Dev<R> d, Box<R> b, Grid g, R r2b,
+                                                     int cap_b, unsigned thr, int ntypes) {
+  if (!rebuild_now(d.ctrl, thr)) return;
+  list_b_dense<R>(d, b, g, blockIdx.x, r2b, cap_b, ntypes);
 }
 
 // CVFF special-bond row of one atom: cumulative counts n12 <= n13 <= n14 of
@@ -881,9 +1116,17 @@ __device__ __forceinline__ SpecRow spec_row(const Dev<R>& d, int ms, int s) {
 template <typename R>
 __device__ __forceinline__ R special_w(const SpecRow& r, int gid,
                                        const PairConst<R>& pc) {
+  // the weights are selected as values: a select of their addresses would
+  // keep a kernel's private copy of pc in scratch memory
+  const R w12 = pc.w12, w13 = pc.w13, w14 = pc.w14;
+  int hit = -1;
   for (int k = 0; k < r.n14; ++k)
-    if (r.ids[k] == gid) return k < r.n12 ? pc.w12 : (k < r.n13 ? pc.w13 : pc.w14);
-  return R(1);
+    if (hit < 0 && r.ids[k] == gid) hit = k;
+  R w = R(1);
+  if (hit >= 0) w = w14;
+  if (hit >= 0 && hit < r.n13) w = w13;
+  if (hit >= 0 && hit < r.n12) w = w12;
+  return w;
 }
 
 // LJ-only pair of list A.  hj() yields j's selection hash (evaluated only
@@ -940,6 +1183,59 @@ __device__ __forceinline__ void coul_sog(const PairConst<R>& pc, R d2, R qq,
   }
 }
 
+// The terms of a charged pair inside r_c once its RBL weight w is known:
+// qq = q_i q_j, (dx, dy, dz) = r_j - r_i, d2 its square norm.
+template <typename R, bool EFLAG, bool SPECIAL>
+__device__ __forceinline__ void eval_b(const PairConst<R>& pc, const R* s_lj,
+                                       int ti, int tj, R qq, R dx, R dy, R dz,
+                                       R d2, R w, Acc<R>& a, const SpecRow& sr,
+                                       int gj) {
+  R flj = 0, elj = 0;
+  if (d2 < pc.rclj2) {
+    const int tt = 4 * (ti * pc.ntypes + tj);
+    const R m24 = s_lj[tt], sig2 = s_lj[tt + 1], eps = s_lj[tt + 2];
+    const R inv = R(1) / d2;
+    const R s2 = sig2 * inv;
+    const R s6 = s2 * s2 * s2;
+    flj = m24 * (R(2) * s6 - R(1)) * s6 * inv;
+    elj = R(2) * eps * (s6 - R(1)) * s6;
+  }
+  R fc, ec;
+  if (pc.sog)
+    coul_sog<R, EFLAG>(pc, d2, qq, fc, ec);
+  else
+    coul<R>(pc, d2, qq, fc, ec);
+  if (!SPECIAL) {
+    const R f = w * (flj + fc);
+    a.fx += f * dx;
+    a.fy += f * dy;
+    a.fz += f * dz;
+    if (EFLAG) {
+      a.e_lj += (double)elj;
+      a.e_c += (double)ec;
+      add_virial<R, EFLAG>(a, dx, dy, dz, flj + fc);
+    }
+    return;
+  }
+  const R ws = special_w<R>(sr, gj, pc);
+  R ffull = R(0), efull = R(0);
+  if (ws != R(1)) {
+    const R rinv = R(1) / sqrt(d2);
+    ffull = -pc.qqr2e * qq * rinv * rinv * rinv;
+    efull = R(0.5) * pc.qqr2e * qq * rinv;
+  }
+  const R corr = (R(1) - ws) * ffull;
+  const R f = w * (ws * flj + fc) - (w != R(0) ? corr : R(0));
+  a.fx += f * dx;
+  a.fy += f * dy;
+  a.fz += f * dz;
+  if (EFLAG) {
+    a.e_lj += (double)(ws * elj);
+    a.e_c += (double)(ec - (R(1) - ws) * efull);
+    add_virial<R, EFLAG>(a, dx, dy, dz, ws * flj + fc - corr);
+  }
+}
+
 // LJ + Coulomb pair of list B (both atoms charged).  With SPECIAL (CVFF),
 // gj is j's global id; a special-bond weight ws < 1 scales the LJ term and
 // removes (1 - ws) of the full Coulomb force -qqr2e q_i q_j / r^3 from every
@@ -957,52 +1253,8 @@ __device__ __forceinline__ void interact_b(const PairConst<R>& pc,
   R dx, dy, dz;
   const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
   if (!(d2 < pc.rc2)) return;
-  R flj = 0, elj = 0;
-  if (d2 < pc.rclj2) {
-    const int tt = 4 * (ti * pc.ntypes + tj);
-    const R m24 = s_lj[tt], sig2 = s_lj[tt + 1], eps = s_lj[tt + 2];
-    const R inv = R(1) / d2;
-    const R s2 = sig2 * inv;
-    const R s6 = s2 * s2 * s2;
-    flj = m24 * (R(2) * s6 - R(1)) * s6 * inv;
-    elj = R(2) * eps * (s6 - R(1)) * s6;
-  }
-  R fc, ec;
-  if (pc.sog)
-    coul_sog<R, EFLAG>(pc, d2, xi.w * xj.w, fc, ec);
-  else
-    coul<R>(pc, d2, xi.w * xj.w, fc, ec);
   const R w = pair_weight<R, RBL>(pc, d2, hi, [&] { return *hjp; });
-  if (!SPECIAL) {
-    const R f = w * (flj + fc);
-    a.fx += f * dx;
-    a.fy += f * dy;
-    a.fz += f * dz;
-    if (EFLAG) {
-      a.e_lj += (double)elj;
-      a.e_c += (double)ec;
-      add_virial<R, EFLAG>(a, dx, dy, dz, flj + fc);
-    }
-    return;
-  }
-  const R ws = special_w<R>(sr, gj, pc);
-  R ffull = R(0), efull = R(0);
-  if (ws != R(1)) {
-    const R qq = xi.w * xj.w;
-    const R rinv = R(1) / sqrt(d2);
-    ffull = -pc.qqr2e * qq * rinv * rinv * rinv;
-    efull = R(0.5) * pc.qqr2e * qq * rinv;
-  }
-  const R corr = (R(1) - ws) * ffull;
-  const R f = w * (ws * flj + fc) - (w != R(0) ? corr : R(0));
-  a.fx += f * dx;
-  a.fy += f * dy;
-  a.fz += f * dz;
-  if (EFLAG) {
-    a.e_lj += (double)(ws * elj);
-    a.e_c += (double)(ec - (R(1) - ws) * efull);
-    add_virial<R, EFLAG>(a, dx, dy, dz, ws * flj + fc - corr);
-  }
+  eval_b<R, EFLAG, SPECIAL>(pc, s_lj, ti, tj, xi.w * xj.w, dx, dy, dz, d2, w, a, sr, gj);
 }
 
 // Reciprocal of d^2 in the force terms: the hardware reciprocal (1 ulp) in
@@ -1263,7 +1515,7 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
   // pass; 16 where list B carries most of the work).  No LDS is read here but
   // the LJ table, so a wave starts as soon as its list-A atoms are done.
   const bool in_b = g.qwave ? (int)threadIdx.x >= TA : true;
-  if (split && in_b) {
+  if (split && in_b && g.qblocks == 0) {
     constexpr int QL = (SHELL || SPECIAL) ? kChargedLanes : 8;
     // the list-B wave takes an equal share of all charged atoms (list B reads
     // no brick stage, and per-brick charged counts vary several-fold); without
@@ -1349,29 +1601,209 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
   write_partials<R, EFLAG>(acc, d.pair_part, blockIdx.x, smem);
 }
 
+// ---------------------------------------------------------------------------
+// Charged pairs of mostly charged systems (water, molten salts): one wave per
+// charged atom.  The wave reads the atom's list-B row and its type bytes 64
+// entries per lane step, kQBatch steps' loads in flight, applies the distance
+// test and the RBL class (the selection hash is read for shell pairs only)
+// and packs the pairs that add a force -- with energies, every pair inside
+// r_c -- into a per-wave LDS ring in list order: the pair vector and d^2,
+// q_j, j with a weight code and the type of j.  Every 64 packed pairs then
+// take LJ, the Coulomb term and the special-bond weight on a full wave
+// without further gathers (but j's global id for special bonds), so with RBL
+// at r_c + skin only the core and the selected shell (about a quarter of a
+// row) reach that part.  Per-atom sums: each lane's terms in ring order, then
+// a fixed tree over the wave; block partials of the thermo sums in `part`
+// (k_reduce's second segment).  Each wave takes a contiguous run of atoms,
+// and the runs of one XCD are contiguous, so an XCD's L2 holds one region's
+// neighbours.
+// ---------------------------------------------------------------------------
+constexpr int kQBatch = 3;  // list steps of 64 entries per load batch
+constexpr int kRing = 256;  // ring entries per wave (>= 63 + 64 * kQBatch)
+static_assert(kRing >= 63 + 64 * kQBatch && (kRing & (kRing - 1)) == 0, "ring size");
+
+template <typename R, bool RBL, bool EFLAG, bool SPECIAL>
+__global__ __launch_bounds__(64 * kQWaves)
+__attribute__((amdgpu_waves_per_eu(4))) void k_pair_b(Dev<R> d, Box<R> b, Grid g,
+                                                      PairConst<R> pc_arg,
+                                                      const R* lj_tab, double* part) {
+  PairConst<R> pc = pc_arg;
+  if (RBL) pc.epoch = d.ctrl->epoch;
+  __shared__ R4T<R> r_d[kQWaves][kRing];         // (dx, dy, dz, d^2)
+  __shared__ R r_q[kQWaves][kRing];              // q_j
+  __shared__ int r_e[kQWaves][kRing];            // j | weight code << 30
+  __shared__ unsigned char r_t[kQWaves][kRing];  // type of j
+  __shared__ double smem[kQWaves * 11];
+  extern __shared__ char dyn_lds[];  // LJ table: 4 * ntypes^2 values of R
+  R* s_lj = reinterpret_cast<R*>(dyn_lds);
+  load_lj<R>(lj_tab, pc.ntypes, s_lj);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  R4T<R>* rd = r_d[wv];
+  R* rq = r_q[wv];
+  int* re = r_e[wv];
+  unsigned char* rt = r_t[wv];
+  const int nwav = gridDim.x * kQWaves;
+  const int wid = xcd_brick(blockIdx.x, gridDim.x) * kQWaves + wv;
+  const int c0 = (int)((long long)wid * d.nc / nwav);
+  const int c1 = (int)((long long)(wid + 1) * d.nc / nwav);
+  Acc<R> acc;
+  for (int c = c0; c < c1; ++c) {
+    const R4T<R> xi = d.xqc[c];
+    const int ti = d.typec[c];
+    const unsigned long long hi = RBL ? rbl_hi(d.gidc[c]) : 0ull;
+    const int cnt = d.cnt_b[c];
+    const SpecRow sr = SPECIAL ? spec_row<R>(d, pc.ms, d.c2s[c]) : SpecRow();
+    Acc<R> a;
+    unsigned head = 0, tail = 0;  // ring positions (wave-uniform)
+    // one candidate per lane, appended in lane order when kept; code 1:
+    // weight 1, 2: the RBL shell multiplier, 0: weight 0 (energies only)
+    auto offer = [&](bool keep, int e, int tj, R dx, R dy, R dz, R d2, R w, R qj) {
+      const unsigned long long m = __ballot(keep);
+      if (keep) {
+        const unsigned r = __builtin_amdgcn_mbcnt_hi(
+            (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        const int s = (int)((tail + r) & (kRing - 1));
+        const int code = w == R(0) ? 0 : ((RBL && w != R(1)) ? 2 : 1);
+        rd[s] = make4<R>(dx, dy, dz, d2);
+        rq[s] = qj;
+        re[s] = e | (code << 30);
+        rt[s] = (unsigned char)tj;
+      }
+      tail += (unsigned)__popcll(m);
+    };
+    // the terms of the ring's entries from head, 64 at a time, while at
+    // least `keep` remain (all of them for keep = 1)
+    auto drain = [&](unsigned keep) {
+      while (tail - head >= keep && tail != head) {
+        __builtin_amdgcn_wave_barrier();
+        const unsigned m = min(tail - head, 64u);
+        if ((unsigned)lane < m) {
+          const int s = (int)((head + lane) & (kRing - 1));
+          const R4T<R> v = rd[s];
+          const int ev = re[s];
+          const int code = (ev >> 30) & 3;
+          const int gj = SPECIAL ? d.gidc[ev & 0x3fffffff] : 0;
+          const R w = code == 0 ? R(0) : (code == 1 ? R(1) : pc.freq);
+          eval_b<R, EFLAG, SPECIAL>(pc, s_lj, ti, (int)rt[s], xi.w * rq[s], v.x, v.y, v.z,
+                                    v.w, w, a, sr, gj);
+        }
+        head += m;
+        __builtin_amdgcn_wave_barrier();
+      }
+    };
+    if (cnt <= d.cap_b) {
+      const int* lp = d.list_b + (size_t)c * d.cap_b;
+      const unsigned char* tp = d.list_t + (size_t)c * d.cap_b;
+      for (int k0 = 0; k0 < cnt; k0 += 64 * kQBatch) {
+        int e[kQBatch], tj[kQBatch];
+        R4T<R> xj[kQBatch];
+        R dx[kQBatch], dy[kQBatch], dz[kQBatch], d2[kQBatch];
+        unsigned long long hj[kQBatch];
+        // indices past the count are clamped, so every load issues
+#pragma unroll
+        for (int u = 0; u < kQBatch; ++u) {
+          const int k = min(k0 + 64 * u + lane, cnt - 1);
+          e[u] = lp[k];
+          tj[u] = tp[k];
+        }
+#pragma unroll
+        for (int u = 0; u < kQBatch; ++u) xj[u] = d.xqc[e[u]];
+#pragma unroll
+        for (int u = 0; u < kQBatch; ++u) {
+          d2[u] = pair_d2<R>(b, xi.x, xi.y, xi.z, xj[u].x, xj[u].y, xj[u].z, dx[u], dy[u],
+                             dz[u]);
+          // the selection hash of j is read for RBL shell candidates only
+          hj[u] = (RBL && !(d2[u] < pc.t_core) && d2[u] < pc.t_shell && d2[u] > pc.rs2)
+                      ? d.hjc[e[u]]
+                      : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < kQBatch; ++u) {
+          const bool in = k0 + 64 * u + lane < cnt && d2[u] < pc.rc2;
+          const R w = in ? pair_weight<R, RBL>(pc, d2[u], hi, [&] { return hj[u]; }) : R(0);
+          offer(EFLAG ? in : w != R(0), e[u], tj[u], dx[u], dy[u], dz[u], d2[u], w, xj[u].w);
+        }
+        drain(64);
+      }
+    } else {
+      // list overflow since the last rebuild: the charged atoms of the cells
+      // within ext_b, 64 per lane step
+      int cc[3];
+      unpack_coord(g.coord_of_cell[d.cell_s[d.c2s[c]]], cc);
+      const Span sx = make_span(cc[0], g.ext_b[0], g.nc[0]);
+      const Span sy = make_span(cc[1], g.ext_b[1], g.nc[1]);
+      const Span sz = make_span(cc[2], g.ext_b[2], g.nc[2]);
+      const int nst = sx.count * sy.count * sz.count;
+      for (int st = 0; st < nst; ++st) {
+        const int kx = st % sx.count, ky = (st / sx.count) % sy.count,
+                  kz = st / (sx.count * sy.count);
+        const int cell =
+            g.cell_of_coord[(sz.at(kz) * g.nc[1] + sy.at(ky)) * g.nc[0] + sx.at(kx)];
+        const int j0 = d.cstart[cell], j1 = d.cstart[cell + 1];
+        for (int jb = j0; jb < j1; jb += 64) {
+          const int j = min(jb + lane, j1 - 1);
+          const R4T<R> xj = d.xqc[j];
+          R dx, dy, dz;
+          const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+          const bool in = jb + lane < j1 && j != c && d2 < pc.rc2;
+          const R w = in ? pair_weight<R, RBL>(pc, d2, hi, [&] { return d.hjc[j]; }) : R(0);
+          offer(EFLAG ? in : w != R(0), j, d.typec[j], dx, dy, dz, d2, w, xj.w);
+          drain(64);
+        }
+      }
+    }
+    drain(1);
+    R fx = a.fx, fy = a.fy, fz = a.fz;
+    for (int o = 32; o > 0; o >>= 1) {
+      fx += __shfl_xor(fx, o, 64);
+      fy += __shfl_xor(fy, o, 64);
+      fz += __shfl_xor(fz, o, 64);
+    }
+    if (lane == 0) d.fb[c] = make4<R>(fx, fy, fz, R(0));
+    acc.fx += a.fx;
+    acc.fy += a.fy;
+    acc.fz += a.fz;
+    if (EFLAG) {
+      acc.e_lj += a.e_lj;
+      acc.e_c += a.e_c;
+      for (int k = 0; k < 6; ++k) acc.v[k] += a.v[k];
+    }
+  }
+  write_partials<R, EFLAG>(acc, part, blockIdx.x, smem);
+}
+
 // One block after the pair kernel: FP64 sums of the per-brick partials in a
 // fixed order (thermo[0..10]: force sum, E_lj, E_coul, virial; the mean pair
 // force when mean_fix; eflag 0 sums the force only and zeroes the rest) and,
 // after a rebuild, the list statistics of the per-block partials of k_lists.
 __global__ __launch_bounds__(1024) void k_reduce(
-    const double* __restrict__ pair_part, int npair, int eflag,
+    const double* __restrict__ pair_part, int npair,
+    const double* __restrict__ part2, int npair2, int eflag,
     double* thermo, double* mean, int mean_fix, double n_global,
     const int4* lista_part, int nlista, const int2* listb_part, int nlistb,
     Ctrl* ctrl, unsigned thr) {
   __shared__ double smem[16 * 11];
-  if (npair > 0) {
+  // partials: the pair kernel's blocks, then the dense charged-pair kernel's
+  const int ntot = npair + npair2;
+  auto col = [&](int t, int k) {
+    return t < npair ? pair_part[(size_t)k * npair + t]
+                     : part2[(size_t)k * npair2 + (t - npair)];
+  };
+  if (ntot > 0) {
     double v[11];
     for (int k = 0; k < 11; ++k) v[k] = 0.0;
     if (eflag) {
-      for (int blk = threadIdx.x; blk < npair; blk += blockDim.x)
+      for (int blk = threadIdx.x; blk < ntot; blk += blockDim.x)
 #pragma unroll
-        for (int k = 0; k < 11; ++k) v[k] += pair_part[(size_t)k * npair + blk];
+        for (int k = 0; k < 11; ++k) v[k] += col(blk, k);
       block_sum<11>(v, smem);
     } else {
       double u[3] = {0.0, 0.0, 0.0};
-      for (int blk = threadIdx.x; blk < npair; blk += blockDim.x)
+      for (int blk = threadIdx.x; blk < ntot; blk += blockDim.x)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) u[k] += pair_part[(size_t)k * npair + blk];
+        for (int k = 0; k < 3; ++k) u[k] += col(blk, k);
       block_sum<3>(u, smem);
       for (int k = 0; k < 3; ++k) v[k] = u[k];
     }
@@ -1897,7 +2329,7 @@ inline int blocks_for(int n, int per = kBlock, int cap = 1 << 20) {
 }
 
 int list_b_grid(const Params& p, int nc, const Grid& g) {
-  return (p.split && nc > 0) ? list_b_blocks(nc, g.block) : 0;
+  return (p.split && nc > 0) ? list_b_count(g, nc) : 0;
 }
 
 template <typename R>
@@ -1917,9 +2349,13 @@ hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, int nskin,
                      s, d, a.box, thr, gb, ks);
   hipLaunchKernelGGL((k_cellsort<R>), dim3(blocks_for(g.ncell * 64, kBlock)),
                      dim3(kBlock), 0, s, d, a.box, g.ncell, thr, a.pc.ms);
-  hipLaunchKernelGGL((k_lists<R>), dim3(g.nbrick + list_b_grid(a.p, d.nc, g)),
+  const bool dense = a.p.split && d.nc > 0 && g.qblocks;
+  hipLaunchKernelGGL((k_lists<R>), dim3(g.nbrick + (dense ? 0 : list_b_grid(a.p, d.nc, g))),
                      dim3(g.block), 0, s, d, a.box, g, a.pc.r2_list_a,
                      a.pc.r2_list_b, a.p.split, cap_a, cap_b, thr);
+  if (dense)
+    hipLaunchKernelGGL((k_lists_dense<R>), dim3(g.ncell), dim3(g.block), 0, s, d, a.box, g,
+                       a.pc.r2_list_b, cap_b, thr, a.pc.ntypes);
   return hipGetLastError();
 }
 
@@ -1931,8 +2367,8 @@ hipError_t launch_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   hipError_t e = build_part<R>(d, a, 1, s);
   if (e != hipSuccess) return e;
   // list statistics only (no pair partials)
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part, 0, 0,
-                     d.thermo, d.mean, 0, 1.0, d.lista_part, a.grid.nbrick,
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part, 0,
+                     (const double*)nullptr, 0, 0, d.thermo, d.mean, 0, 1.0, d.lista_part, a.grid.nbrick,
                      d.listb_part, list_b_grid(a.p, d.nc, a.grid), d.ctrl,
                      a.rebuild_thr);
   return hipGetLastError();
@@ -2021,9 +2457,40 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
       RBMDX_PAIR(false, false, false, false);
   }
 #undef RBMDX_PAIR
+  if (g.qblocks > 0) {
+#define RBMDX_PAIRB(RBLv, EFv, SPv)                                             \
+  hipLaunchKernelGGL((k_pair_b<R, RBLv, EFv, SPv>), dim3(g.qblocks),           \
+                     dim3(64 * kQWaves), lj_bytes, s, d, a.box, g, a.pc, a.lj_tab, \
+                     d.pairb_part)
+    if (p.special) {
+      if (p.rbl) {
+        if (p.eflag)
+          RBMDX_PAIRB(true, true, true);
+        else
+          RBMDX_PAIRB(true, false, true);
+      } else {
+        if (p.eflag)
+          RBMDX_PAIRB(false, true, true);
+        else
+          RBMDX_PAIRB(false, false, true);
+      }
+    } else if (p.rbl) {
+      if (p.eflag)
+        RBMDX_PAIRB(true, true, false);
+      else
+        RBMDX_PAIRB(true, false, false);
+    } else {
+      if (p.eflag)
+        RBMDX_PAIRB(false, true, false);
+      else
+        RBMDX_PAIRB(false, false, false);
+    }
+#undef RBMDX_PAIRB
+  }
   if (a.ev[2]) hipEventRecord(a.ev[2], s);
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part,
-                     g.nbrick * g.psub, p.eflag ? 1 : 0, d.thermo, d.mean, p.mean_fix, p.n_global,
+                     g.nbrick * g.psub, (const double*)d.pairb_part, g.qblocks,
+                     p.eflag ? 1 : 0, d.thermo, d.mean, p.mean_fix, p.n_global,
                      d.lista_part, g.nbrick, d.listb_part,
                      list_b_grid(p, d.nc, g), d.ctrl, a.rebuild_thr);
   hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,
