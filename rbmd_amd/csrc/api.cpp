@@ -577,7 +577,8 @@ class Engine : public EngineBase {
     take(dd.mean, 3);
     const size_t nchunks = std::max(1, (d_.nc + kSkChunk - 1) / kSkChunk);
     if (d_.sk_part) {
-      take(dd.sk_part, nchunks * d_.kcap);
+      take(dd.sk_part, (size_t)std::max(1, (d_.nc + kSkAtoms - 1) / kSkAtoms) * d_.kcap);
+      take(dd.kfin_part, (size_t)8 * d_.kcap);
       take(dd.kf0, d_.kcap);
       take(dd.kf1, d_.kcap);
       take(dd.rho, d_.kcap);
@@ -710,6 +711,7 @@ class Engine : public EngineBase {
       dn.listb_part = dd.listb_part;
       dn.mean = dd.mean;
       dn.sk_part = dd.sk_part;
+      dn.kfin_part = dd.kfin_part;
       dn.kf0 = dd.kf0;
       dn.kf1 = dd.kf1;
       dn.rho = dd.rho;
@@ -1535,10 +1537,12 @@ class Engine : public EngineBase {
   }
 
   void alloc_kspace(const std::vector<double>& w, const std::vector<int>* mv) {
-    const int nchunks = std::max(1, (d_.nc + kSkChunk - 1) / kSkChunk);
+    // S(k) partials: one row per kSkAtoms charged atoms
+    const int nchunks = std::max(1, (d_.nc + kSkAtoms - 1) / kSkAtoms);
     d_.m_dev = alloc<int>(3 * (size_t)d_.kcap);
     d_.kw = alloc<double>(d_.kcap);
     d_.sk_part = alloc<double2>((size_t)nchunks * d_.kcap);
+    d_.kfin_part = alloc<double>((size_t)8 * d_.kcap);
     d_.kf0 = alloc<float4>(d_.kcap);
     d_.kf1 = alloc<float4>(d_.kcap);
     d_.rho = alloc<double2>(d_.kcap);

@@ -16,6 +16,9 @@ constexpr int kMaxSpecial = 64;  // CVFF special partners per atom
 constexpr int kSkTile = 8;       // k-vectors per S(k) block
 constexpr int kSkChunk = 1024;   // charged atoms per S(k) block
 constexpr int kRbeWaves = 8;     // waves per RBE-force block (k split)
+constexpr int kRbeStage = 256;   // k coefficients per LDS stage of the RBE force
+constexpr int kSkAtoms = 256;    // charged atoms per S(k) block (4 waves of 64)
+constexpr int kSkK = 64;         // k-vectors per S(k) block and per reduction block
 constexpr int kMaxK = 1 << 20;   // k-vector capacity bound
 constexpr int kMaxSlots = 64;    // neighbourhood cells staged per brick
 constexpr int kChargedLanes = 16;  // lanes per charged atom (Coulomb pairs)
@@ -205,6 +208,7 @@ struct Dev {
   double* kw = nullptr;          // [kcap] per-k weight
   double2* kvir = nullptr;       // [kcap] (tau, divide by |k_src|^4) (RBSOG)
   double2* sk_part = nullptr;    // [chunks * kcap]
+  double* kfin_part = nullptr;   // [kcap * 8] k-space energy/virial terms per k
   float4* kf0 = nullptr;         // [kcap] (mx, my, mz, A Re rho)
   float4* kf1 = nullptr;         // [kcap] (kx, ky, kz, A Im rho)
   double2* rho = nullptr;        // [kcap]
@@ -273,7 +277,7 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s);
 // per-evaluation partial results (reduction partials, S(k) chunk partials,
 // per-k coefficients) that a context otherwise allocates.  Every call on one
 // device shares the device's copy.
-constexpr size_t kNoAllocScratchBytes = (size_t)4 << 20;
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
 // Resolves the array's address on `device`, which must be the current
 // device; the first call per device does the lookup, later calls keep it.
 hipError_t noalloc_scratch_init(int device);

@@ -1774,42 +1774,105 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_pair_b(Dev<R> d, Box<R> b, Grid g
   write_partials<R, EFLAG>(acc, part, blockIdx.x, smem);
 }
 
-// One block after the pair kernel: FP64 sums of the per-brick partials in a
-// fixed order (thermo[0..10]: force sum, E_lj, E_coul, virial; the mean pair
-// force when mean_fix; eflag 0 sums the force only and zeroes the rest) and,
-// after a rebuild, the list statistics of the per-block partials of k_lists.
+// The k-space sums k_reduce adds up after the force kernels: the per-k
+// energy and virial terms and, for RBSOG, the virial tensor part
+// (kspace_calculator_op.hip.cu:1308-1352): W_ab += tau_k |rho_s|^2 k_s,a k_s,b
+// with s = src[k] and tau_k = kvir[k].x, divided by |k_s|^4 when kvir[k].y
+// != 0.
+struct KSum {
+  const double* part = nullptr;  // [K * 8]
+  int K = 0;                     // 0: no k-space this evaluation
+  int tensor = 0;                // RBSOG tensor part
+  const int* m = nullptr;
+  const int* src = nullptr;
+  const double2* kvir = nullptr;
+  const double2* rho = nullptr;
+  double L[3] = {1.0, 1.0, 1.0};
+};
+
+// Per-thread sums in a fixed order, then one block sum of NP pair values
+// (force sum; with energies E_lj, E_coul and the virial) and, with k-space,
+// the seven k-space values.
+template <int NP, bool KS>
+__device__ __forceinline__ void reduce_sums(const double* __restrict__ pair_part,
+                                            int npair, const double* __restrict__ part2,
+                                            int npair2, const KSum& ks, double* thermo,
+                                            double* mean, int mean_fix, double n_global,
+                                            double* smem) {
+  constexpr int NV = NP + (KS ? 7 : 0);
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = 0.0;
+  const int ntot = npair + npair2;
+  for (int blk = threadIdx.x; blk < ntot; blk += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+      v[q] += blk < npair ? pair_part[(size_t)q * npair + blk]
+                          : part2[(size_t)q * npair2 + (blk - npair)];
+  if (KS) {
+    const double two_pi = 6.283185307179586476925;
+    for (int k = threadIdx.x; k < ks.K; k += blockDim.x) {
+#pragma unroll
+      for (int q = 0; q < 7; ++q) v[NP + q] += ks.part[(size_t)k * 8 + q];
+      if (ks.tensor) {
+        const int sk = ks.src[k];
+        const double kx = two_pi * ks.m[3 * sk] / ks.L[0];
+        const double ky = two_pi * ks.m[3 * sk + 1] / ks.L[1];
+        const double kz = two_pi * ks.m[3 * sk + 2] / ks.L[2];
+        const double2 r = ks.rho[sk];
+        const double2 tv = ks.kvir[k];
+        double tt = tv.x * (r.x * r.x + r.y * r.y);
+        if (tv.y != 0.0) {
+          const double k2 = kx * kx + ky * ky + kz * kz;
+          tt /= k2 * k2;
+        }
+        v[NP + 1] += tt * kx * kx;
+        v[NP + 2] += tt * ky * ky;
+        v[NP + 3] += tt * kz * kz;
+        v[NP + 4] += tt * kx * ky;
+        v[NP + 5] += tt * kx * kz;
+        v[NP + 6] += tt * ky * kz;
+      }
+    }
+  }
+  block_sum<NV>(v, smem);
+  if (threadIdx.x == 0) {
+    if (ntot > 0) {
+      for (int q = 0; q < 11; ++q) thermo[q] = q < NP ? v[q] : 0.0;
+      for (int q = 0; q < 3; ++q) mean[q] = mean_fix ? v[q] / n_global : 0.0;
+    }
+    if (KS)
+      for (int q = 0; q < 7; ++q) thermo[16 + q] = v[NP + q];
+  }
+}
+
+// One block after the pair kernels: FP64 sums of the per-block partials in
+// a fixed order (thermo[0..10]: force sum, E_lj, E_coul, virial; the mean
+// pair force when mean_fix; eflag 0 sums the force only and zeroes the rest),
+// the k-space energy and virial (thermo[16..22]) and, after a rebuild, the
+// list statistics of the per-block partials of k_lists.
 __global__ __launch_bounds__(1024) void k_reduce(
     const double* __restrict__ pair_part, int npair,
     const double* __restrict__ part2, int npair2, int eflag,
     double* thermo, double* mean, int mean_fix, double n_global,
     const int4* lista_part, int nlista, const int2* listb_part, int nlistb,
-    Ctrl* ctrl, unsigned thr) {
-  __shared__ double smem[16 * 11];
-  // partials: the pair kernel's blocks, then the dense charged-pair kernel's
-  const int ntot = npair + npair2;
-  auto col = [&](int t, int k) {
-    return t < npair ? pair_part[(size_t)k * npair + t]
-                     : part2[(size_t)k * npair2 + (t - npair)];
-  };
-  if (ntot > 0) {
-    double v[11];
-    for (int k = 0; k < 11; ++k) v[k] = 0.0;
-    if (eflag) {
-      for (int blk = threadIdx.x; blk < ntot; blk += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < 11; ++k) v[k] += col(blk, k);
-      block_sum<11>(v, smem);
+    Ctrl* ctrl, unsigned thr, KSum ksum) {
+  __shared__ double smem[16 * 18];
+  if (npair + npair2 > 0 || ksum.K > 0) {
+    if (ksum.K > 0) {
+      if (eflag)
+        reduce_sums<11, true>(pair_part, npair, part2, npair2, ksum, thermo, mean, mean_fix,
+                              n_global, smem);
+      else
+        reduce_sums<3, true>(pair_part, npair, part2, npair2, ksum, thermo, mean, mean_fix,
+                             n_global, smem);
     } else {
-      double u[3] = {0.0, 0.0, 0.0};
-      for (int blk = threadIdx.x; blk < ntot; blk += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) u[k] += col(blk, k);
-      block_sum<3>(u, smem);
-      for (int k = 0; k < 3; ++k) v[k] = u[k];
-    }
-    if (threadIdx.x == 0) {
-      for (int k = 0; k < 11; ++k) thermo[k] = v[k];
-      for (int k = 0; k < 3; ++k) mean[k] = mean_fix ? v[k] / n_global : 0.0;
+      if (eflag)
+        reduce_sums<11, false>(pair_part, npair, part2, npair2, ksum, thermo, mean,
+                               mean_fix, n_global, smem);
+      else
+        reduce_sums<3, false>(pair_part, npair, part2, npair2, ksum, thermo, mean, mean_fix,
+                              n_global, smem);
     }
   }
   if (!rebuild_now(ctrl, thr)) return;
@@ -1949,91 +2012,6 @@ __global__ __launch_bounds__(256) void k_sk(Dev<R> d, Box<R> b, const int* __res
   sk_block<R>(d, b, m_arg, from_ring, with_src, K, (int)blockIdx.x, (int)blockIdx.y);
 }
 
-// FP64 reduction of the S(k) chunk partials (fixed order, one thread per k)
-// into rho_k and the per-k force coefficients
-//   A_k = w_k (-4 pi qqr2e) / (V k^2),  kf0 = (m, A Re rho), kf1 = (k, A Im rho)
-// and the k-space energy and virial (without the self terms):
-//   E = qqr2e (2 pi / V) sum_k w_k |rho|^2 / k^2,
-//   W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/(4 alpha) + 1/k^2)).
-// One block; the kernel boundary after the S(k) blocks makes every partial
-// visible.
-// vmode 0 (RBE, EWALD): W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/(4 alpha)
-// + 1/k^2)).  vmode 1 (RBSOG, kspace_calculator_op.hip.cu:1133-1222 and
-// :1308-1352): W_aa += E_k, and W_ab += tau_k |rho_s|^2 k_s,a k_s,b with
-// s = src[k] and tau_k = kvir[k].x, divided by |k_s|^4 when kvir[k].y != 0.
-template <typename R>
-__device__ __forceinline__ void skfin_block(const Dev<R>& d, const Box<R>& b,
-                                            const int* __restrict__ m, int K,
-                                            int nchunks, const Params& p, int vmode,
-                                            const int* __restrict__ src) {
-  const double V = p.volume;
-  const double two_pi = 6.283185307179586476925;
-  double kv[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int k = threadIdx.x; k < K; k += blockDim.x) {
-    double sre = 0.0, sim = 0.0;
-#pragma unroll 8
-    for (int c = 0; c < nchunks; ++c) {
-      const double2 s = d.sk_part[(size_t)c * d.kcap + k];
-      sre += s.x;
-      sim += s.y;
-    }
-    d.rho[k] = make_double2(sre, sim);
-    const double kx = two_pi * m[3 * k] / (double)b.L[0];
-    const double ky = two_pi * m[3 * k + 1] / (double)b.L[1];
-    const double kz = two_pi * m[3 * k + 2] / (double)b.L[2];
-    const double k2 = kx * kx + ky * ky + kz * kz;
-    const double w = d.kw[k];
-    const double A = w * (-4.0 * 3.141592653589793238463 * p.qqr2e) / (V * k2);
-    d.kf0[k] = make_float4((float)m[3 * k], (float)m[3 * k + 1],
-                           (float)m[3 * k + 2], (float)(A * sre));
-    d.kf1[k] = make_float4((float)kx, (float)ky, (float)kz, (float)(A * sim));
-    const double base = p.qqr2e * two_pi / V * w * (sre * sre + sim * sim) / k2;
-    kv[0] += base;
-    if (vmode == 0) {
-      const double cc = 2.0 * (1.0 / (4.0 * p.alpha) + 1.0 / k2);
-      kv[1] += base * (1.0 - cc * kx * kx);
-      kv[2] += base * (1.0 - cc * ky * ky);
-      kv[3] += base * (1.0 - cc * kz * kz);
-      kv[4] += base * (-cc * kx * ky);
-      kv[5] += base * (-cc * kx * kz);
-      kv[6] += base * (-cc * ky * kz);
-    } else {
-      kv[1] += base;
-      kv[2] += base;
-      kv[3] += base;
-    }
-  }
-  if (vmode == 1) {
-    // the tensor part reads rho of another k, written above by other threads
-    __syncthreads();
-    __threadfence_block();
-    for (int k = threadIdx.x; k < K; k += blockDim.x) {
-      const int s = src[k];
-      const double kx = two_pi * m[3 * s] / (double)b.L[0];
-      const double ky = two_pi * m[3 * s + 1] / (double)b.L[1];
-      const double kz = two_pi * m[3 * s + 2] / (double)b.L[2];
-      const double2 r = d.rho[s];
-      const double2 tv = d.kvir[k];
-      double t = tv.x * (r.x * r.x + r.y * r.y);
-      if (tv.y != 0.0) {
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        t /= k2 * k2;
-      }
-      kv[1] += t * kx * kx;
-      kv[2] += t * ky * ky;
-      kv[3] += t * kz * kz;
-      kv[4] += t * kx * ky;
-      kv[5] += t * kx * kz;
-      kv[6] += t * ky * kz;
-    }
-  }
-  __shared__ double smem7[16 * 7];
-  block_sum<7>(kv, smem7);
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < 7; ++q) d.thermo[16 + q] = kv[q];
-  }
-}
-
 // The energy_rbe_flag diagnostic (ComputeKspaceEnergy,
 // kspace_calculator.cpp:816-895): FP64 sums of the S(k) chunk partials of the
 // kmax grid, then E = qqr2e (2 pi / V) sum_k exp(-k^2 / 4 alpha) |rho_k|^2 / k^2
@@ -2062,32 +2040,193 @@ __global__ __launch_bounds__(1024) void k_sk_exact(Dev<R> d, Box<R> b, int nchun
   if (threadIdx.x == 0) d.thermo[24] = p.qqr2e * two_pi / p.volume * e[0];
 }
 
-// RBE / Ewald force on 64 charged atoms (block `bid`, the fixed order of
-// xqs and fk), the k range split over the block's waves (at most
-// kRbeWaves), reduced through LDS.
+// S(k) partial sums of one block, one lane per k-vector: the kSkAtoms
+// charged atoms of chunk `chunk` (in the fixed order of xqs) are staged in
+// LDS as fractional coordinates and charge, each wave takes 64 of them with
+// broadcast reads, and lane l of every wave accumulates k = tile * kSkK + l
+// in FP32 over its atoms (two alternating accumulators); the four waves'
+// sums are added in FP64 in wave order into sk_part[chunk][k].  m_src holds
+// the K integer k-vectors, in device memory or, for the sampled stream, in
+// host-mapped staging memory; in that case m_copy (device) is given and the
+// chunk-0 block of every k tile copies its tile there for the reduction and
+// later calls (likewise src_src -> src_copy, the RBSOG virial-chain sources,
+// when given).
 template <typename R>
-__device__ __forceinline__ void rbe_block(const Dev<R>& d, const Box<R>& b, int K,
-                                          int bid) {
+__device__ __forceinline__ void sk2_block(const Dev<R>& d, const Box<R>& b,
+                                          const int* __restrict__ m_arg, int from_ring,
+                                          int with_src, int K, int chunk, int tile) {
+  const int* m_src = m_arg;
+  int* m_copy = nullptr;
+  const int* src_src = nullptr;
+  int* src_copy = nullptr;
+  if (from_ring) {
+    m_src = d.stage_ring + (size_t)(d.ctrl->step_count % kSlots) * d.stage_stride;
+    m_copy = d.m_dev;
+    if (with_src) {
+      src_src = m_src + 3 * (size_t)K;
+      src_copy = d.src_dev;
+    }
+  }
+  __shared__ float4 s_a[kSkAtoms];
+  __shared__ double s_p[kSkAtoms / 64][2][kSkK];
+  const int t = (int)threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int k0 = tile * kSkK;
+  const int kn = min(kSkK, K - k0);
+  const int a0 = chunk * kSkAtoms, na = min(kSkAtoms, d.nc - a0);
+  if (t < na) {
+    const R4T<R> pp = d.xqs[a0 + t];
+    float sx, sy, sz;
+    frac_coords<R>(b, pp, sx, sy, sz);
+    s_a[t] = make_float4(sx, sy, sz, (float)pp.w);
+  } else {
+    s_a[t] = make_float4(0.f, 0.f, 0.f, 0.f);  // charge 0 adds nothing
+  }
+  if (m_copy && chunk == 0) {
+    for (int q = t; q < 3 * kn; q += blockDim.x) m_copy[3 * k0 + q] = m_src[3 * k0 + q];
+    if (src_copy && t < kn) src_copy[k0 + t] = src_src[k0 + t];
+  }
+  const int kl = k0 + min(lane, kn - 1);
+  const float mx = (float)m_src[3 * kl], my = (float)m_src[3 * kl + 1],
+              mz = (float)m_src[3 * kl + 2];
+  __syncthreads();
+  float re0 = 0.f, im0 = 0.f, re1 = 0.f, im1 = 0.f;
+  const float4* sa = s_a + 64 * wv;
+#pragma unroll 4
+  for (int a = 0; a < 64; a += 2) {
+    const float4 u = sa[a], v = sa[a + 1];
+    float sn, cs, sn1, cs1;
+    sincos_rev(fma_r(mx, u.x, fma_r(my, u.y, mz * u.z)), sn, cs);
+    sincos_rev(fma_r(mx, v.x, fma_r(my, v.y, mz * v.z)), sn1, cs1);
+    re0 = fma_r(u.w, cs, re0);
+    im0 = fma_r(u.w, sn, im0);
+    re1 = fma_r(v.w, cs1, re1);
+    im1 = fma_r(v.w, sn1, im1);
+  }
+  s_p[wv][0][lane] = (double)re0 + (double)re1;
+  s_p[wv][1][lane] = (double)im0 + (double)im1;
+  __syncthreads();
+  if (t < kn) {
+    double sre = 0.0, sim = 0.0;
+#pragma unroll
+    for (int w = 0; w < kSkAtoms / 64; ++w) {
+      sre += s_p[w][0][t];
+      sim += s_p[w][1][t];
+    }
+    d.sk_part[(size_t)chunk * d.kcap + k0 + t] = make_double2(sre, sim);
+  }
+}
+
+// FP64 reduction of the S(k) chunk partials of the kSkK k's of k tile
+// `tile` (16 threads per k, each a contiguous run of chunks, then the 16 in
+// order; the kernel boundary after the S(k) blocks makes every partial
+// visible) into rho_k and the per-k force coefficients
+//   A_k = w_k (-4 pi qqr2e) / (V k^2),  kf0 = (m, A Re rho), kf1 = (k, A Im rho),
+// and each k's terms of the k-space energy and virial (without the self
+// terms) into kfin_part[k][0..6], which k_reduce adds up:
+//   E = qqr2e (2 pi / V) sum_k w_k |rho|^2 / k^2,
+// vmode 0 (RBE, EWALD): W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/(4 alpha)
+// + 1/k^2)); vmode 1 (RBSOG, kspace_calculator_op.hip.cu:1133-1222): the
+// isotropic part W_aa += E_k here, the tensor part, which reads rho of other
+// k's, in k_reduce (ksum_tensor).
+template <typename R>
+__device__ __forceinline__ void skfin2_block(const Dev<R>& d, const Box<R>& b,
+                                             const int* __restrict__ m, int K, int nchunks,
+                                             const Params& p, int vmode, int tile) {
+  constexpr int kSub = 16;  // threads per k (blockDim = kSkK * kSub)
+  __shared__ double s_s[kSub][2][kSkK];
+  const int t = (int)threadIdx.x;
+  const int kk = t % kSkK, r = t / kSkK;
+  const int k = tile * kSkK + kk;
+  const int per = (nchunks + kSub - 1) / kSub;
+  const int c0 = r * per, c1 = min(nchunks, c0 + per);
+  double sre = 0.0, sim = 0.0;
+  if (k < K) {
+#pragma unroll 4
+    for (int c = c0; c < c1; ++c) {
+      const double2 v = d.sk_part[(size_t)c * d.kcap + k];
+      sre += v.x;
+      sim += v.y;
+    }
+  }
+  s_s[r][0][kk] = sre;
+  s_s[r][1][kk] = sim;
+  __syncthreads();
+  const double V = p.volume;
+  const double two_pi = 6.283185307179586476925;
+  double kv[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (r == 0 && k < K) {
+    sre = 0.0;
+    sim = 0.0;
+    for (int q = 0; q < kSub; ++q) {
+      sre += s_s[q][0][kk];
+      sim += s_s[q][1][kk];
+    }
+    d.rho[k] = make_double2(sre, sim);
+    const double kx = two_pi * m[3 * k] / (double)b.L[0];
+    const double ky = two_pi * m[3 * k + 1] / (double)b.L[1];
+    const double kz = two_pi * m[3 * k + 2] / (double)b.L[2];
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double w = d.kw[k];
+    const double A = w * (-4.0 * 3.141592653589793238463 * p.qqr2e) / (V * k2);
+    d.kf0[k] = make_float4((float)m[3 * k], (float)m[3 * k + 1], (float)m[3 * k + 2],
+                           (float)(A * sre));
+    d.kf1[k] = make_float4((float)kx, (float)ky, (float)kz, (float)(A * sim));
+    const double base = p.qqr2e * two_pi / V * w * (sre * sre + sim * sim) / k2;
+    kv[0] = base;
+    if (vmode == 0) {
+      const double cc = 2.0 * (1.0 / (4.0 * p.alpha) + 1.0 / k2);
+      kv[1] = base * (1.0 - cc * kx * kx);
+      kv[2] = base * (1.0 - cc * ky * ky);
+      kv[3] = base * (1.0 - cc * kz * kz);
+      kv[4] = base * (-cc * kx * ky);
+      kv[5] = base * (-cc * kx * kz);
+      kv[6] = base * (-cc * ky * kz);
+    } else {
+      kv[1] = base;
+      kv[2] = base;
+      kv[3] = base;
+    }
+  }
+  if (r == 0 && k < K)
+    for (int q = 0; q < 7; ++q) d.kfin_part[(size_t)k * 8 + q] = kv[q];
+}
+
+// RBE / Ewald force on 64 charged atoms (block `bid`, one lane per atom, the
+// fixed order of xqs and fk).  The per-k coefficients are staged in LDS
+// kRbeStage k's at a time; wave w takes k's w, w + nw, ... of each stage
+// with broadcast reads, and the waves' sums are added in wave order.
+template <typename R>
+__device__ __forceinline__ void rbe2_block(const Dev<R>& d, const Box<R>& b, int K,
+                                           int bid) {
+  __shared__ float4 s_k0[kRbeStage], s_k1[kRbeStage];
   __shared__ float s_f[kRbeWaves][3][64];
   const int nw = (int)blockDim.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = (int)threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const int c = bid * 64 + lane;
-  const int per = (K + nw - 1) / nw;
-  const int k0 = wv * per, k1 = min(K, k0 + per);
-  float fx = 0.f, fy = 0.f, fz = 0.f;
-  float q = 0.f;
+  float sx = 0.f, sy = 0.f, sz = 0.f, q = 0.f;
   if (c < d.nc) {
-    const R4T<R> p = d.xqs[c];
-    float sx, sy, sz;
-    frac_coords<R>(b, p, sx, sy, sz);
-    q = (float)p.w;
-    for (int k = k0; k < k1; ++k) {
-      const float4 a = d.kf0[k];
-      const float4 e = d.kf1[k];
-      const float t = fma_r(a.x, sx, fma_r(a.y, sy, a.z * sz));
+    const R4T<R> pp = d.xqs[c];
+    frac_coords<R>(b, pp, sx, sy, sz);
+    q = (float)pp.w;
+  }
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  for (int s0 = 0; s0 < K; s0 += kRbeStage) {
+    const int ns = min(kRbeStage, K - s0);
+    __syncthreads();  // the previous stage is consumed
+    for (int k = t; k < ns; k += blockDim.x) {
+      s_k0[k] = d.kf0[s0 + k];
+      s_k1[k] = d.kf1[s0 + k];
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = wv; k < ns; k += nw) {
+      const float4 a = s_k0[k];
+      const float4 e = s_k1[k];
+      const float ph = fma_r(a.x, sx, fma_r(a.y, sy, a.z * sz));
       float sn, cs;
-      sincos_rev(t, sn, cs);
+      sincos_rev(ph, sn, cs);
       const float gg = fma_r(cs, e.w, -sn * a.w);
       fx = fma_r(gg, e.x, fx);
       fy = fma_r(gg, e.y, fy);
@@ -2099,13 +2238,13 @@ __device__ __forceinline__ void rbe_block(const Dev<R>& d, const Box<R>& b, int 
   s_f[wv][2][lane] = fz;
   __syncthreads();
   if (wv == 0 && c < d.nc) {
-    float sx = 0.f, sy = 0.f, sz = 0.f;
+    float ax = 0.f, ay = 0.f, az = 0.f;
     for (int w = 0; w < nw; ++w) {
-      sx += s_f[w][0][lane];
-      sy += s_f[w][1][lane];
-      sz += s_f[w][2][lane];
+      ax += s_f[w][0][lane];
+      ay += s_f[w][1][lane];
+      az += s_f[w][2][lane];
     }
-    d.fk[c] = make4<R>((R)(q * sx), (R)(q * sy), (R)(q * sz), R(0));
+    d.fk[c] = make4<R>((R)(q * ax), (R)(q * ay), (R)(q * az), R(0));
   }
 }
 
@@ -2123,7 +2262,7 @@ struct KsArgs {
   const int* m_fin = nullptr;   // k-vectors as the reduction reads them
   const int* src = nullptr;     // RBSOG virial-chain sources (device)
   int vmode = 0;                // virial form (0 RBE / EWALD, 1 RBSOG)
-  int nchunks = 0, ntiles = 0;  // S(k) blocks: nchunks x ntiles
+  int nchunks = 0, ntiles = 0;  // S(k) blocks: kSkAtoms-atom chunks x kSkK-k tiles
   int nrbe = 0;                 // RBE force blocks (64 charged atoms each)
   Params p;
 };
@@ -2138,18 +2277,19 @@ __global__ __launch_bounds__(256) void k_bin_sk(Dev<R> d, Box<R> b, Grid g, int 
     return;
   }
   const int q = (int)blockIdx.x - nbin;
-  sk_block<R>(d, b, ks.m_arg, ks.from_ring, ks.with_src, ks.K, q % ks.nchunks,
-              q / ks.nchunks);
+  sk2_block<R>(d, b, ks.m_arg, ks.from_ring, ks.with_src, ks.K, q % ks.nchunks,
+               q / ks.nchunks);
 }
 
-// The cell scans (block 0) and the FP64 S(k) reduction (block 1).
+// The cell scans (block 0) and the FP64 S(k) reduction (blocks 1 .. ntiles,
+// one k tile each).
 template <typename R>
 __global__ __launch_bounds__(1024) void k_scan_fin(Dev<R> d, Box<R> b, int ncell,
                                                    unsigned thr, KsArgs ks) {
   if (blockIdx.x == 0)
     scan_block(d.key_count, d.key_start, d.cstart, ncell, d.ctrl, thr);
   else
-    skfin_block<R>(d, b, ks.m_fin, ks.K, ks.nchunks, ks.p, ks.vmode, ks.src);
+    skfin2_block<R>(d, b, ks.m_fin, ks.K, ks.nchunks, ks.p, ks.vmode, (int)blockIdx.x - 1);
 }
 
 // The sorted-order scatter (blocks [0, nsc)) and the RBE force (the blocks
@@ -2160,7 +2300,7 @@ __global__ __launch_bounds__(256) void k_scatter_rbe(Dev<R> d, Box<R> b, unsigne
   if ((int)blockIdx.x < nsc)
     scatter_block<R>(d, thr, blockIdx.x, nsc);
   else
-    rbe_block<R>(d, b, ks.K, (int)blockIdx.x - nsc);
+    rbe2_block<R>(d, b, ks.K, (int)blockIdx.x - nsc);
 }
 
 // v + 0.5 F / m dt ftm2v evaluated left to right, as RBMD's UpdateVelocity
@@ -2343,7 +2483,8 @@ hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, int nskin,
   const bool kon = ks.K > 0;
   hipLaunchKernelGGL((k_bin_sk<R>), dim3(gb + (kon ? ks.nchunks * ks.ntiles : 0)),
                      dim3(kBlock), 0, s, d, a.box, g, a.p.split, thr, nskin, gb, ks);
-  hipLaunchKernelGGL((k_scan_fin<R>), dim3(kon ? 2 : 1), dim3(1024), 0, s, d, a.box,
+  hipLaunchKernelGGL((k_scan_fin<R>), dim3(kon ? 1 + ks.ntiles : 1), dim3(1024), 0, s,
+                     d, a.box,
                      g.ncell, thr, ks);
   hipLaunchKernelGGL((k_scatter_rbe<R>), dim3(gb + (kon ? ks.nrbe : 0)), dim3(kBlock), 0,
                      s, d, a.box, thr, gb, ks);
@@ -2367,10 +2508,10 @@ hipError_t launch_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   hipError_t e = build_part<R>(d, a, 1, s);
   if (e != hipSuccess) return e;
   // list statistics only (no pair partials)
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part, 0,
-                     (const double*)nullptr, 0, 0, d.thermo, d.mean, 0, 1.0, d.lista_part, a.grid.nbrick,
-                     d.listb_part, list_b_grid(a.p, d.nc, a.grid), d.ctrl,
-                     a.rebuild_thr);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, d.pair_part, 0,
+                     (const double*)nullptr, 0, 0, d.thermo, d.mean, 0, 1.0, d.lista_part,
+                     a.grid.nbrick, d.listb_part, list_b_grid(a.p, d.nc, a.grid), d.ctrl,
+                     a.rebuild_thr, KSum());
   return hipGetLastError();
 }
 
@@ -2396,8 +2537,8 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
     ks.m_fin = a.m_from_ring ? (const int*)d.m_dev : a.m_use;
     ks.src = sog ? (const int*)d.src_dev : nullptr;
     ks.vmode = sog ? 1 : 0;
-    ks.nchunks = (d.nc + kSkChunk - 1) / kSkChunk;
-    ks.ntiles = (K + kSkTile - 1) / kSkTile;
+    ks.nchunks = (d.nc + kSkAtoms - 1) / kSkAtoms;
+    ks.ntiles = (K + kSkK - 1) / kSkK;
     ks.nrbe = blocks_for(d.nc, 64);
     ks.p = p;
   }
@@ -2488,11 +2629,22 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
 #undef RBMDX_PAIRB
   }
   if (a.ev[2]) hipEventRecord(a.ev[2], s);
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, d.pair_part,
+  KSum ksum;
+  if (ks_on) {
+    ksum.part = d.kfin_part;
+    ksum.tensor = ks.vmode == 1;
+    ksum.K = K;
+    ksum.m = ks.m_fin;
+    ksum.src = ks.src;
+    ksum.kvir = d.kvir;
+    ksum.rho = d.rho;
+    for (int q = 0; q < 3; ++q) ksum.L[q] = (double)a.box.L[q];
+  }
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, d.pair_part,
                      g.nbrick * g.psub, (const double*)d.pairb_part, g.qblocks,
                      p.eflag ? 1 : 0, d.thermo, d.mean, p.mean_fix, p.n_global,
                      d.lista_part, g.nbrick, d.listb_part,
-                     list_b_grid(p, d.nc, g), d.ctrl, a.rebuild_thr);
+                     list_b_grid(p, d.nc, g), d.ctrl, a.rebuild_thr, ksum);
   hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,
                      s, d, p.mean_fix, ks_on ? 1 : 0, p.split ? 1 : 0, (R)a.dt,
                      (R)a.dt_ftm, a.md_kick, (R)a.ftm2v);

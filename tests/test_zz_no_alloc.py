@@ -78,8 +78,9 @@ def test_noalloc_errors(device):
     with pytest.raises(RbmdxError) as e:
         ctx.force_noalloc(0, *pos, *f, m_samples=mt, sync=True)
     assert e.value.code == RBMDX_E_STATE
-    # a sample count whose per-k scratch exceeds the 4 MiB array
-    s, ctx, pos, mt = _setup(2000, 34, 80000, device)
+    # a sample count whose per-k scratch (at least 64 bytes per k) exceeds
+    # the 16 MiB array
+    s, ctx, pos, mt = _setup(2000, 34, 300000, device)
     with pytest.raises(RbmdxError) as e:
         ctx.force_noalloc(0, *pos, *_forces(s.n, device), m_samples=mt, sync=True)
     assert e.value.code == RBMDX_E_CAPACITY
