@@ -89,8 +89,9 @@ enum {
  * The pair set (step is the RBL sampling epoch, as for rbmdx_force) and the
  * k-sample are those of the last rbmdx_force call; the RBL mean-force fix is
  * differentiated too.  LJ + erfc Coulomb pairs with RBE, RBSOG or EWALD
- * k-space, CVFF special bonds (not with RBMDX_FLAG_VJP_NEUTRAL_Q) and the
- * SOG near field included; at most 8 types.  Eager launches on stream; parameter sums use LDS float atomics
+ * k-space, CVFF special bonds (with RBMDX_FLAG_VJP_NEUTRAL_Q the special
+ * weight of a charged-neutral pair also scales its full-Coulomb removal) and
+ * the SOG near field included; at most 8 types.  Eager launches on stream; parameter sums use LDS float atomics
  * inside a workgroup and FP64 atomics across workgroups, so their last bits
  * may vary between runs.  Flags:
  * RBMDX_FLAG_SYNC, RBMDX_FLAG_REBUILD, RBMDX_FLAG_VJP_NEUTRAL_Q.

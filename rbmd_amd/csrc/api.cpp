@@ -432,9 +432,6 @@ class Engine : public EngineBase {
       throw Error(RBMDX_E_ARG, "null pointer");
     if (d_.cap_a == 0 || calls_ == 0)
       throw Error(RBMDX_E_STATE, "rbmdx_force_vjp needs a forward evaluation first");
-    if (params_.special && (flags & RBMDX_FLAG_VJP_NEUTRAL_Q))
-      throw Error(RBMDX_E_CONFIG,
-                  "RBMDX_FLAG_VJP_NEUTRAL_Q is not supported with CVFF special bonds");
     if (ntypes_ > kVjpMaxTypes)
       throw Error(RBMDX_E_CONFIG, "the adjoint supports at most 8 atom types");
     HIPCHK(hipSetDevice(device_));

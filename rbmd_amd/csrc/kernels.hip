@@ -3064,12 +3064,16 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
 // staged in LDS (shifted across box edges as stage_atoms does), and each
 // neutral atom of the brick, kQncLanes lanes per atom, scans the staged rows
 // of cells within ext_b of its own cell.  A neighbourhood beyond the stage
-// reads the same atoms from global memory.
+// reads the same atoms from global memory.  With CVFF special bonds
+// (SPECIAL, ws the neutral atom's weight for the pair, the same both ways) a
+// kept pair also carries the charge derivative of the (1 - ws) full-Coulomb
+// removal, (1 - ws) qqr2e q_j r^-3 along d for each directed pair the lists
+// keep, as vjp_pair adds it for charged pairs.
 constexpr int kQncSlots = 512;   // neighbourhood cells per brick
 constexpr int kQncCap = 1024;    // staged charged atoms
 constexpr int kQncLanes = 2;     // lanes per neutral atom
 
-template <typename R, bool RBL>
+template <typename R, bool RBL, bool SPECIAL>
 __global__ __launch_bounds__(256) void k_vjp_qnc(Dev<R> d, VjpDev<R> v, Box<R> b,
                                                  Grid g, PairConst<R> pc_arg) {
   PairConst<R> pc = pc_arg;
@@ -3132,6 +3136,7 @@ __global__ __launch_bounds__(256) void k_vjp_qnc(Dev<R> d, VjpDev<R> v, Box<R> b
       const R gi[3] = {g4.x - gm[0], g4.y - gm[1], g4.z - gm[2]};
       const unsigned long long hi = RBL ? rbl_hi(d.gid_s[i]) : 0ull;
       const unsigned long long hji = RBL ? d.hj_s[i] : 0ull;
+      const SpecRow sr = SPECIAL ? spec_row<R>(d, pc.ms, i) : SpecRow();
       // (owner i neutral, j charged): m = w_ij G'_i . d - w_ji G'_j . d
       auto one = [&](const R4T<R>& xj, const R gj[3], int gidj, bool mi) {
         R dx, dy, dz;
@@ -3151,6 +3156,16 @@ __global__ __launch_bounds__(256) void k_vjp_qnc(Dev<R> d, VjpDev<R> v, Box<R> b
         const R m = wij * (gi[0] * dx + gi[1] * dy + gi[2] * dz) -
                     wji * (gj[0] * dx + gj[1] * dy + gj[2] * dz);
         acc += (float)(xj.w * (-pc.qqr2e * hh) * m);
+        if (SPECIAL) {
+          const R ws = special_w<R>(sr, gidj, pc);
+          if (ws != R(1)) {
+            const R rinv = R(1) / sqrt(u);
+            const R gc = (R(1) - ws) * pc.qqr2e * rinv * rinv * rinv;
+            const R ms = (wij != R(0) ? R(1) : R(0)) * (gi[0] * dx + gi[1] * dy + gi[2] * dz) -
+                         (wji != R(0) ? R(1) : R(0)) * (gj[0] * dx + gj[1] * dy + gj[2] * dz);
+            acc += (float)(xj.w * gc * ms);
+          }
+        }
       };
       if (staged && !h.mi) {
         // rows of cells within ext_b of the owner's cell: contiguous along x
@@ -3456,12 +3471,20 @@ hipError_t launch_vjp(const Dev<R>& d, const VjpDev<R>& v, const StepArgs<R>& a,
   }
 #undef RBMDX_PAIR_VJP
   if (v.neutral_q && v.nq_blocks > 0) {
-    if (p.rbl)
-      hipLaunchKernelGGL((k_vjp_qnc<R, true>), dim3(g.nbrick), dim3(256), 0, s, d, v,
-                         a.box, g, a.pc);
-    else
-      hipLaunchKernelGGL((k_vjp_qnc<R, false>), dim3(g.nbrick), dim3(256), 0, s, d, v,
-                         a.box, g, a.pc);
+#define RBMDX_QNC(RBLv, SPv)                                                            \
+  hipLaunchKernelGGL((k_vjp_qnc<R, RBLv, SPv>), dim3(g.nbrick), dim3(256), 0, s, d, v, \
+                     a.box, g, a.pc)
+    if (p.special) {
+      if (p.rbl)
+        RBMDX_QNC(true, true);
+      else
+        RBMDX_QNC(false, true);
+    } else if (p.rbl) {
+      RBMDX_QNC(true, false);
+    } else {
+      RBMDX_QNC(false, false);
+    }
+#undef RBMDX_QNC
   }
   const int K = p.kspace ? p.num_k : 0;
   const bool has_k = K > 0 && d.nc > 0;

@@ -323,3 +323,64 @@ def test_vjp_sog_near_field_matches_finite_differences(device):
     np.testing.assert_allclose(ad0 - ad1, fd0 - fd1, rtol=1e-6)
     dd = fdq0 - fdq1
     np.testing.assert_allclose(dq0 - dq1, dd, atol=1e-4 * np.abs(dd).max())
+
+
+def _polar_water(n_mol, seed):
+    """SPC/E geometry and topology with a neutral O (type 0) and the two H
+    sites charged +0.4238 (type 1) and -0.4238 (type 2): every molecule holds
+    charged-neutral special pairs (O-H, 1-2) and a charged 1-3 pair (H-H)."""
+    s = systems.spce_water(n_mol, seed=seed)
+    s.types = s.types.copy()
+    s.types[2::3] = 2
+    s.charges = np.tile([0.0, 0.4238, -0.4238], n_mol)
+    s.eps = np.array([0.15535, 0.02, 0.02])
+    s.sigma = np.array([3.166, 1.0, 1.0])
+    return s
+
+
+def test_vjp_cvff_neutral_charge_grad(device):
+    """dL/dQ of a neutral type with CVFF special bonds and
+    RBMDX_FLAG_VJP_NEUTRAL_Q: the neutral O of a polar 3-site water carries
+    1-2 partners that are charged.  special_bonds [0, 0.6, 0.5] against unit
+    weights (RBL, RBE): the adjoint's special-minus-unit dL/dQ_O equals the
+    central difference in Q_O at 0 (h = 2^-10) of the FP64 oracle's CVFF pair
+    force, L = sum G' . F_pair with G' the mean-shifted cotangent, special
+    minus unit.  k-space does not see special bonds and cancels in the
+    difference, as do the terms the two systems share, which the FP64
+    oracle adjoint pins above.  Tolerance 1e-4 of the difference (FP32
+    pair terms and per-lane sums)."""
+    from tests.refs import cfg_qqr2e, cfg_special
+
+    s = _polar_water(1000, seed=9)
+    G = np.random.default_rng(10).normal(size=(s.n, 3))
+    step = 4
+    res = {}
+    for key, w in (("special", (0.0, 0.6, 0.5)), ("unit", (1.0, 1.0, 1.0))):
+        cfg = systems.cvff_water_config(alpha=0.0784, P=100, special_bonds=w)
+        ctx = _ctx(s, cfg)
+        ctx.set_special(s.nspecial, s.special)
+        m = rbe_sample_host(0.0784, s.box_len, 100, 21, 0)
+        _, _, dp = _run(ctx, s, device, step, m, G, neutral_q=True)
+        # the oracle's FP64 pair force with Q_O = +-h (same RBL epoch and set)
+        nb = cfg["hyper_parameters"]["neighbor"]
+        freq = ctx.thermo()["rbl_frequency"]
+        params = oracle.pair_params(nb["r_core"], nb["cut_off"], nb["cut_off"],
+                                    ctx.thermo()["alpha"], cfg_qqr2e(cfg), freq, step, 1, 1, 1)
+        pos = s.pos.astype(np.float32)
+        Gs = G - G.mean(axis=0)
+
+        def loss(qo):
+            q = np.array([qo, 0.4238, -0.4238])[s.types].astype(np.float32)
+            f = oracle.pair_force(pos, q, s.types, s.ids, s.eps, s.sigma, s.box_len, params,
+                                  special=cfg_special(cfg, s))[0]
+            return float((f * Gs).sum())
+
+        # L is quadratic in Q_O, so the central difference is exact up to
+        # rounding; h a power of two is exact in the oracle's FP32 charges
+        h = 2.0 ** -10
+        res[key] = (dp[0], (loss(h) - loss(-h)) / (2 * h))
+    ad = res["special"][0] - res["unit"][0]
+    fd = res["special"][1] - res["unit"][1]
+    print(f"dL/dQ_O special - unit: adjoint {ad:.6e}, oracle central difference {fd:.6e}")
+    assert abs(fd) > 1e-3 * abs(res["unit"][1])  # the special terms matter here
+    assert ad == pytest.approx(fd, rel=1e-4)
