@@ -432,10 +432,22 @@ int rbmdx_batch_set_systems(rbmdx_batch* ctx, int32_t nrep, const int32_t* natom
                             const double* charge, int32_t ntypes,
                             const double* eps, const double* sigma);
 
+/* CVFF special bonds of the batch (the counterpart of rbmdx_set_special,
+ * which replaces SpecialBondsManager's topology for one system,
+ * cvff.cpp:1091-1191): per atom of the concatenated replicas (caller order)
+ * the cumulative counts n12 <= n13 <= n14 (3 ints, clamped to maxspecial)
+ * and maxspecial partner ids, which are ids within the atom's own replica.
+ * force_field.type "CVFF" only; maxspecial in [0, 64]. */
+int rbmdx_batch_set_special(rbmdx_batch* ctx, int32_t maxspecial,
+                            const int32_t* nspecial, const int32_t* special);
+
 /* One evaluation of every replica.  Device arrays; m_samples holds
  * rbmdx_batch_num_samples() integer k-vectors (3 ints) per replica, replica
- * after replica (RBE; null without Coulomb and with EWALD, whose kmax grid
- * weighted exp(-k^2 / 4 alpha) the batch holds itself).  step is the RBL
+ * after replica (RBE), or for RBSOG per replica the P k-vectors followed by
+ * their P virial-chain source indices (as rbmdx_sog_sample_host returns them;
+ * the batch adds the direct vectors |m|^2 <= 1 itself); null without Coulomb
+ * and with EWALD, whose kmax grid weighted exp(-k^2 / 4 alpha) the batch holds
+ * itself.  step is the RBL
  * sampling epoch.  Flags: RBMDX_FLAG_ENERGY, RBMDX_FLAG_SYNC,
  * RBMDX_FLAG_EAGER (the batch's kernels launched directly),
  * RBMDX_FLAG_PER_REPLICA (the same kernels, one replica per launch).  By

@@ -158,6 +158,7 @@ SIGNATURES = {
     "rbmdx_batch_last_error": (ctypes.c_char_p, [_P]),
     "rbmdx_batch_set_systems": (ctypes.c_int, [_P, _I32, _IP, _DP, _DP, _IP, _IP, _DP, _I32,
                                                _DP, _DP]),
+    "rbmdx_batch_set_special": (ctypes.c_int, [_P, _I32, _IP, _IP]),
     "rbmdx_batch_force": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P,
                                          ctypes.c_uint32, _P]),
     "rbmdx_batch_energies": (ctypes.c_int, [_P, _DP]),
@@ -549,6 +550,20 @@ class BatchContext:
         self.nrep = len(systems)
         self.n = int(nat.sum())
         self.offsets = np.concatenate([[0], np.cumsum(nat)])
+
+    def set_special(self, systems):
+        """CVFF special bonds of every replica from the systems' nspecial (N, 3)
+        and special (N, maxspecial) arrays (partner ids within the replica)."""
+        ms = max(np.asarray(s.special).shape[1] for s in systems)
+        ns = np.ascontiguousarray(np.concatenate([s.nspecial for s in systems]), dtype=np.int32)
+        sp = np.zeros((len(ns), ms), dtype=np.int32)
+        o = 0
+        for s in systems:
+            a = np.asarray(s.special, dtype=np.int32)
+            sp[o:o + len(a), :a.shape[1]] = a
+            o += len(a)
+        self._check(self.lib.rbmdx_batch_set_special(self._h, int(ms), _iptr(ns),
+                                                     _iptr(np.ascontiguousarray(sp))))
 
     def num_samples(self):
         return int(self.lib.rbmdx_batch_num_samples(self._h))

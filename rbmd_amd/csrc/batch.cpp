@@ -22,23 +22,35 @@
 #include "batch.hpp"
 #include "host_common.hpp"
 #include "rbmdx.h"
+#include "sog.hpp"
 
 using namespace rbmdx;
 
 namespace {
+
+// RBSOG direct vectors: every m != 0 with |m|^2 <= 1 (the single-system
+// engine's kcut = 1 set, in its order)
+constexpr int kSogDirect = 6;
+constexpr int kSogDirectM[3 * kSogDirect] = {-1, 0, 0, 0, -1, 0, 0, 0, -1,
+                                              0,  0, 1, 0, 1,  0, 1, 0, 0};
 
 class Batch {
  public:
   Batch(const Config& c, int device) : cfg_(c), device_(device) {
     if (c.fp64)
       throw Error(RBMDX_E_CONFIG, "the replica batch computes in FP32 (precision fp32)");
-    if (c.force_type != "LJ/CUT" && c.force_type != "LJ/CUT/COUL/LONG")
+    if (c.force_type != "LJ/CUT" && c.force_type != "LJ/CUT/COUL/LONG" &&
+        c.force_type != "CVFF")
       throw Error(RBMDX_E_CONFIG,
-                  "the replica batch supports LJ/CUT and LJ/CUT/COUL/LONG (not CVFF)");
+                  "the replica batch supports LJ/CUT, LJ/CUT/COUL/LONG and CVFF");
     if (c.neighbor_type != "RBL" && c.neighbor_type != "Verlet-List")
       throw Error(RBMDX_E_CONFIG, "the replica batch supports RBL and Verlet-List");
-    if (c.coulomb() && c.coulomb_type != "RBE" && c.coulomb_type != "EWALD")
-      throw Error(RBMDX_E_CONFIG, "the replica batch supports RBE and EWALD k-space");
+    if (c.coulomb() && c.coulomb_type != "RBE" && c.coulomb_type != "EWALD" &&
+        c.coulomb_type != "RBSOG")
+      throw Error(RBMDX_E_CONFIG, "the replica batch supports RBE, EWALD and RBSOG k-space");
+    if (c.coulomb() && c.coulomb_type == "RBSOG" && c.sog_near)
+      throw Error(RBMDX_E_CONFIG,
+                  "the replica batch's RBSOG uses the erfc near field (no rbsog_near_field)");
     if (c.coulomb() && c.coulomb_type == "EWALD") {
       if (!c.has_kmax) throw Error(RBMDX_E_CONFIG, "EWALD needs coulomb.kmax");
       const long K = (2L * c.kmax[0] + 1) * (2L * c.kmax[1] + 1) * (2L * c.kmax[2] + 1) - 1;
@@ -52,6 +64,10 @@ class Batch {
     if (c.coulomb() && c.coulomb_type == "RBE" &&
         (c.coulomb_sample_num < 1 || c.coulomb_sample_num > kBatchMaxK))
       throw Error(RBMDX_E_CONFIG, "coulomb.sample_num must be in [1, 512] for the batch");
+    // RBSOG: the P sampled k's and the 6 direct vectors |m|^2 <= 1
+    if (c.coulomb() && c.coulomb_type == "RBSOG" &&
+        (c.coulomb_sample_num < 1 || c.coulomb_sample_num + 6 > kBatchMaxK))
+      throw Error(RBMDX_E_CONFIG, "coulomb.sample_num must be in [1, 506] for the batch's RBSOG");
     HIPCHK(hipSetDevice(device_));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device_));
@@ -98,6 +114,7 @@ class Batch {
       if (type[i] < 0 || type[i] >= ntypes)
         throw Error(RBMDX_E_ARG, "atom type out of range");
     std::vector<int> rep_of(N_);
+    std::vector<double> kwd, kvd;  // RBSOG direct vectors, replica-major
     for (int r = 0; r < R; ++r) {
       RepInfo& p = rep_[r];
       const int n = natoms[r];
@@ -144,7 +161,25 @@ class Batch {
       p.ifreq = f;
       p.fmask = (f >= 2 && (f & (f - 1)) == 0) ? (unsigned)(f - 1) : 0u;
       p.c32 = f >= 2 ? (unsigned)((1ULL << 32) % (unsigned long long)f) : 0u;
-      if (coul) {
+      if (coul && cfg_.coulomb_type == "RBSOG") {
+        // this box's SOG set (coefficients and normalisation), as the
+        // single-system engine forms it: sampled k's weigh S / (4 pi P),
+        // the direct vectors f(k) k^2 / (4 pi)
+        const SogSet ss = sog_set(cfg_.rbsog_level, rc, Ld, 1, cfg_.rbsog_exact);
+        const int P = cfg_.coulomb_sample_num;
+        p.kw = ss.S / (4 * M_PI * P);
+        p.ktau = -(ss.S_npt / P) * cfg_.qqr2e / (4.0 * p.volume);
+        for (int e = 0; e < kSogDirect; ++e) {
+          double k2 = 0;
+          for (int d = 0; d < 3; ++d) {
+            const double kd = 2 * M_PI * kSogDirectM[3 * e + d] / Ld[d];
+            k2 += kd * kd;
+          }
+          kwd.push_back(sog_f(ss, ss.coef, k2) * k2 / (4 * M_PI));
+          kvd.push_back(-cfg_.qqr2e / (4.0 * p.volume) * sog_f(ss, ss.coef_npt, k2));
+        }
+        self_[r] = sog_self_energy(ss, sq2, cfg_.qqr2e);
+      } else if (coul) {
         const double a = cfg_.alpha;
         // RBE normalisation S = prod_d H_d - 1 (kspace_calculator.cpp:68-78)
         double h[3];
@@ -163,7 +198,17 @@ class Batch {
     NQ_ = q0_[R];
     ncell_ = c0_[R];
     P_ = coul ? cfg_.coulomb_sample_num : 0;
-    d_.ewald = 0;
+    d_.kmode = 0;
+    d_.mstride = 3 * P_;
+    d_.nd = 0;
+    if (coul && cfg_.coulomb_type == "RBSOG") {
+      d_.kmode = 2;
+      d_.mstride = 4 * P_;
+      d_.nd = kSogDirect;
+      d_.dm = upload(std::vector<int>(kSogDirectM, kSogDirectM + 3 * kSogDirect));
+      d_.kwd = upload(kwd);
+      d_.kvd = upload(kvd);
+    }
     if (coul && cfg_.coulomb_type == "EWALD") {
       // every m != 0 with |m_d| <= kmax_d, the same integer grid in every
       // replica (KSpaceCalculator's EWALD k-set, weights on the device)
@@ -181,7 +226,8 @@ class Batch {
       all.reserve(grid.size() * R);
       for (int r = 0; r < R; ++r) all.insert(all.end(), grid.begin(), grid.end());
       ewald_m_ = upload(all);
-      d_.ewald = 1;
+      d_.kmode = 1;
+      d_.mstride = 3 * P_;
     }
     // pair constants shared by every replica (thresholds formed in FP32 as
     // the single-system engine forms them)
@@ -203,6 +249,10 @@ class Batch {
       pc_.ifreq = 1;
       pc_.ntypes = ntypes;
       pc_.sog = 0;
+      pc_.ms = 0;
+      pc_.w12 = (float)cfg_.w_special[0];
+      pc_.w13 = (float)cfg_.w_special[1];
+      pc_.w14 = (float)cfg_.w_special[2];
     }
     HIPCHK(hipSetDevice(device_));
     // device arrays
@@ -259,6 +309,44 @@ class Batch {
     ready_ = true;
   }
 
+  // CVFF special-bond topology of every replica, caller order (the atoms of
+  // all replicas concatenated): counts (n12, n13, n14) per atom, clamped as
+  // the single-system engine clamps them, and ms partner ids per atom, ids
+  // within the atom's replica.  Caller-order rows need no re-sort.
+  void set_special(int32_t ms, const int32_t* nspecial, const int32_t* special) {
+    if (!ready_) throw Error(RBMDX_E_STATE, "rbmdx_batch_set_systems first");
+    if (cfg_.force_type != "CVFF")
+      throw Error(RBMDX_E_STATE, "special bonds apply to force_field.type CVFF only");
+    if (ms < 0 || ms > kMaxSpecial)
+      throw Error(RBMDX_E_ARG, "maxspecial must be in [0, 64]");
+    if (ms > 0 && (!nspecial || !special))
+      throw Error(RBMDX_E_ARG, "null special-bond arrays");
+    HIPCHK(hipSetDevice(device_));
+    if (last_stream_set_) HIPCHK(hipStreamSynchronize(last_stream_));
+    std::vector<int4> cnt(N_);
+    bool any = false;
+    for (int i = 0; i < N_ && ms > 0; ++i) {
+      int n12 = std::max(0, nspecial[3 * i]), n13 = std::max(0, nspecial[3 * i + 1]),
+          n14 = std::max(0, nspecial[3 * i + 2]);
+      if (n12 > n13) n13 = n12;
+      if (n13 > n14) n14 = n13;
+      if (n14 > ms) n14 = ms;
+      if (n13 > n14) n13 = n14;
+      if (n12 > n13) n12 = n13;
+      cnt[i] = make_int4(n12, n13, n14, 0);
+      any |= n14 > 0;
+    }
+    special_ = 0;
+    pc_.ms = 0;
+    if (any) {
+      d_.spn = upload(cnt);
+      d_.spid = upload(std::vector<int>(special, special + (size_t)N_ * ms));
+      special_ = 1;
+      pc_.ms = ms;
+    }
+    drop_graphs();
+  }
+
   // Bricks of width bw: per-replica brick counts and the replica-major work
   // list (one item per brick).
   void layout(int bw) {
@@ -297,7 +385,7 @@ class Batch {
             float* fz, const int32_t* m) {
     if (!px || !py || !pz || !fx || !fy || !fz)
       throw Error(RBMDX_E_ARG, "null position or force pointer");
-    if (d_.ewald) {
+    if (d_.kmode == 1) {
       if (m) throw Error(RBMDX_E_ARG, "EWALD uses the kmax grid: pass no k-samples");
       m = ewald_m_;
     }
@@ -312,7 +400,7 @@ class Batch {
   }
 
   void launch_all(int eflag, hipStream_t s) {
-    HIPCHK(batch_launch(d_, pc_, rbl(), cfg_.coulomb() ? 1 : 0, eflag, range(0, R_), tune_,
+    HIPCHK(batch_launch(d_, pc_, rbl(), cfg_.coulomb() ? 1 : 0, eflag, special_, range(0, R_), tune_,
                         num_cu_, s));
   }
 
@@ -325,8 +413,8 @@ class Batch {
     HIPCHK(batch_set_epoch(d_.epoch, (long long)step, s));
     if (flags & RBMDX_FLAG_PER_REPLICA) {
       for (int r = 0; r < R_; ++r)
-        HIPCHK(batch_launch(d_, pc_, rbl(), cfg_.coulomb() ? 1 : 0, eflag, range(r, r + 1),
-                            tune_, num_cu_, s));
+        HIPCHK(batch_launch(d_, pc_, rbl(), cfg_.coulomb() ? 1 : 0, eflag, special_,
+                            range(r, r + 1), tune_, num_cu_, s));
     } else if (flags & RBMDX_FLAG_EAGER) {
       launch_all(eflag, s);
     } else {
@@ -469,6 +557,7 @@ class Batch {
 
  private:
   int rbl() const { return cfg_.neighbor_type == "RBL" ? 1 : 0; }
+  int special_ = 0;  // CVFF special-bond rows are set
 
   template <typename T>
   T* alloc(size_t n) {
@@ -504,7 +593,7 @@ class Batch {
     if (!cap_stream_) HIPCHK(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
     HIPCHK(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeRelaxed));
     const hipError_t e1 = batch_launch(d_, pc_, rbl(), cfg_.coulomb() ? 1 : 0, eflag,
-                                       range(0, R_), tune_, num_cu_, cap_stream_);
+                                       special_, range(0, R_), tune_, num_cu_, cap_stream_);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(cap_stream_, &g);
     if (e1 != hipSuccess || e2 != hipSuccess || !g) {
@@ -527,7 +616,7 @@ class Batch {
     std::ostringstream k;
     k << arch_ << "|cu" << num_cu_ << "|R" << R_ << "|N" << N_ << "|NQ" << NQ_ << "|P"
       << P_ << "|rc" << cfg_.cut_off << "|rl" << cfg_.lj_cut_off << "|" << cfg_.neighbor_type
-      << "|t" << ntypes_;
+      << "|t" << ntypes_ << "|k" << d_.kmode << "|s" << special_;
     return k.str();
   }
 
@@ -671,6 +760,15 @@ int rbmdx_batch_set_systems(rbmdx_batch* ctx, int32_t nrep, const int32_t* natom
     if (!natoms || !box_lo || !box_len || !type || !id || !charge || !eps || !sigma)
       throw Error(RBMDX_E_ARG, "null argument");
     ctx->b->set_systems(nrep, natoms, box_lo, box_len, type, id, charge, ntypes, eps, sigma);
+  });
+}
+
+int rbmdx_batch_set_special(rbmdx_batch* ctx, int32_t maxspecial, const int32_t* nspecial,
+                            const int32_t* special) {
+  if (!ctx) return RBMDX_E_ARG;
+  return guarded_b(ctx, [&] {
+    if (!ctx->b) throw Error(RBMDX_E_STATE, "context was not created");
+    ctx->b->set_special(maxspecial, nspecial, special);
   });
 }
 

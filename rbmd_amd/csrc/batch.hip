@@ -183,12 +183,12 @@ __device__ __forceinline__ BrickNb batch_brick(const RepInfo& p, int b, int bw) 
 
 // LJ interaction of a pair that involves a neutral atom (interact_a's
 // scalars and energy convention).
-template <bool RBL, bool EFLAG, bool MI>
+template <bool RBL, bool EFLAG, bool MI, bool SPECIAL>
 __device__ __forceinline__ void batch_lj(const PairConst<float>& pc, const Box<float>& b,
                                          const float* s_lj, int ti, const float4& xi,
                                          unsigned long long hi, const float4& xj, int tj,
                                          int gj, float& fx, float& fy, float& fz,
-                                         Acc<float>& a) {
+                                         Acc<float>& a, const SpecRow& sr) {
   float dx, dy, dz;
   const float d2 = pair_d2<float, MI>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
   if (!(d2 < pc.rclj2) || !(d2 < pc.rc2)) return;
@@ -197,7 +197,9 @@ __device__ __forceinline__ void batch_lj(const PairConst<float>& pc, const Box<f
   const float inv = 1.f / d2;
   const float s2 = sig2 * inv;
   const float s6 = s2 * s2 * s2;
-  const float flj = m24 * (2.f * s6 - 1.f) * s6 * inv;
+  // CVFF: the owner's special-bond weight scales the LJ term
+  const float ws = SPECIAL ? special_w<float>(sr, gj, pc) : 1.f;
+  const float flj = ws * (m24 * (2.f * s6 - 1.f) * s6 * inv);
   // the neighbour's hash enters only for shell pairs
   const float w = pair_weight<float, RBL>(pc, d2, hi, [&] { return rbl_hj(gj); });
   const float f = w * flj;
@@ -205,47 +207,44 @@ __device__ __forceinline__ void batch_lj(const PairConst<float>& pc, const Box<f
   fy += f * dy;
   fz += f * dz;
   if (EFLAG) {
-    a.e_lj += (double)(2.f * eps * (s6 - 1.f) * s6);
+    a.e_lj += (double)(ws * (2.f * eps * (s6 - 1.f) * s6));
     add_virial<float, EFLAG>(a, dx, dy, dz, flj);
   }
 }
 
-// LJ + erfc Coulomb of a charged pair (interact_b's scalars, weight and
-// energy convention; minimum image).
-template <bool RBL, bool EFLAG, bool MI = true>
+// LJ + erfc Coulomb of a charged pair (interact_b's terms: eval_b, with the
+// owner's special-bond weight under CVFF; minimum image).
+template <bool RBL, bool EFLAG, bool MI, bool SPECIAL>
 __device__ __forceinline__ void batch_charged(const PairConst<float>& pc,
                                               const Box<float>& b, const float* s_lj,
                                               int ti, const float4& xi,
                                               unsigned long long hi, const float4& xj,
-                                              int tj, int gj, Acc<float>& a) {
+                                              int tj, int gj, Acc<float>& a,
+                                              const SpecRow& sr) {
   float dx, dy, dz;
   const float d2 = pair_d2<float, MI>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
   if (!(d2 < pc.rc2)) return;
-  float flj = 0.f, elj = 0.f;
-  if (d2 < pc.rclj2) {
-    const int tt = 4 * (ti * pc.ntypes + tj);
-    const float m24 = s_lj[tt], sig2 = s_lj[tt + 1], eps = s_lj[tt + 2];
-    const float inv = 1.f / d2;
-    const float s2 = sig2 * inv;
-    const float s6 = s2 * s2 * s2;
-    flj = m24 * (2.f * s6 - 1.f) * s6 * inv;
-    elj = 2.f * eps * (s6 - 1.f) * s6;
-  }
-  float fc, ec;
-  coul<float>(pc, d2, xi.w * xj.w, fc, ec);
   const float w = pair_weight<float, RBL>(pc, d2, hi, [&] { return rbl_hj(gj); });
-  const float f = w * (flj + fc);
-  a.fx += f * dx;
-  a.fy += f * dy;
-  a.fz += f * dz;
-  if (EFLAG) {
-    a.e_lj += (double)elj;
-    a.e_c += (double)ec;
-    add_virial<float, EFLAG>(a, dx, dy, dz, flj + fc);
-  }
+  eval_b<float, EFLAG, SPECIAL>(pc, s_lj, ti, tj, xi.w * xj.w, dx, dy, dz, d2, w, a, sr, gj);
 }
 
-template <bool RBL, bool EFLAG>
+// The CVFF special-bond row of sorted atom s (counts and partner ids in
+// caller order).
+template <bool SPECIAL>
+__device__ __forceinline__ SpecRow batch_spec_row(const BatchDev& d, int ms, int s) {
+  SpecRow r;
+  if (SPECIAL) {
+    const int i = d.perm[s];
+    const int4 c = d.spn[i];
+    r.n12 = c.x;
+    r.n13 = c.y;
+    r.n14 = c.z;
+    r.ids = d.spid + (size_t)i * ms;
+  }
+  return r;
+}
+
+template <bool RBL, bool EFLAG, bool SPECIAL>
 __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_arg,
                                                BatchRange rg, int bw, int qlanes) {
   __shared__ float4 sx[kBatchStageCap];
@@ -365,6 +364,7 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
       const int2 tgi = d.tg[i];
       const unsigned long long hi = RBL ? rbl_hi(tgi.y) : 0ull;
       const bool qi = xi.w != 0.f;
+      const SpecRow sr = batch_spec_row<SPECIAL>(d, pc.ms, i);
       float fx = 0.f, fy = 0.f, fz = 0.f;
       int lo3[3], hi3[3];
 #pragma unroll
@@ -384,11 +384,11 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
             if (l == li || (qi && xj.w != 0.f)) continue;
             const int2 tgj = stg[l];
             if (h.mi)
-              batch_lj<RBL, EFLAG, true>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x, tgj.y, fx,
-                                         fy, fz, a);
+              batch_lj<RBL, EFLAG, true, SPECIAL>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x,
+                                                  tgj.y, fx, fy, fz, a, sr);
             else
-              batch_lj<RBL, EFLAG, false>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x, tgj.y,
-                                          fx, fy, fz, a);
+              batch_lj<RBL, EFLAG, false, SPECIAL>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x,
+                                                   tgj.y, fx, fy, fz, a, sr);
           }
         } else {
           // beyond the stage: the same cells from global memory, minimum image
@@ -399,8 +399,8 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
               const float4 xj = d.xs[f + j];
               if (f + j == i || (qi && xj.w != 0.f)) continue;
               const int2 tgj = d.tg[f + j];
-              batch_lj<RBL, EFLAG, true>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x, tgj.y, fx,
-                                         fy, fz, a);
+              batch_lj<RBL, EFLAG, true, SPECIAL>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x,
+                                                  tgj.y, fx, fy, fz, a, sr);
             }
           }
         }
@@ -519,6 +519,7 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
           const float4 xi = d.xs[i];
           const int ti = d.tg[i].x;
           const unsigned long long hi = RBL ? rbl_hi(d.tg[i].y) : 0ull;
+          const SpecRow sr = batch_spec_row<SPECIAL>(d, pc.ms, i);
           Acc<float> ac;
           if (qstaged) {
             // rows of the stencil in the restaged neighbourhood
@@ -538,11 +539,11 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
                 if (l == li) continue;
                 const int2 tgj = stg[l];
                 if (qmi)
-                  batch_charged<RBL, EFLAG, true>(pc, b, s_lj, ti, xi, hi, sx[l], tgj.x,
-                                                  tgj.y, ac);
+                  batch_charged<RBL, EFLAG, true, SPECIAL>(pc, b, s_lj, ti, xi, hi, sx[l],
+                                                           tgj.x, tgj.y, ac, sr);
                 else
-                  batch_charged<RBL, EFLAG, false>(pc, b, s_lj, ti, xi, hi, sx[l], tgj.x,
-                                                   tgj.y, ac);
+                  batch_charged<RBL, EFLAG, false, SPECIAL>(pc, b, s_lj, ti, xi, hi, sx[l],
+                                                            tgj.x, tgj.y, ac, sr);
               }
             }
           } else {
@@ -558,8 +559,8 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
               const int j0 = d.start[cj], j1 = j0 + (d.qstart[cj + 1] - d.qstart[cj]);
               for (int j = j0; j < j1; ++j) {
                 if (j == i) continue;
-                batch_charged<RBL, EFLAG>(pc, b, s_lj, ti, xi, hi, d.xs[j], d.tg[j].x,
-                                          d.tg[j].y, ac);
+                batch_charged<RBL, EFLAG, true, SPECIAL>(pc, b, s_lj, ti, xi, hi, d.xs[j],
+                                                         d.tg[j].x, d.tg[j].y, ac, sr);
               }
             }
           }
@@ -598,9 +599,16 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
   }
 }
 
-// RBE k-space of one replica per workgroup pass: S(k) (one k per thread,
+// k-space of one replica per workgroup pass: S(k) (one k per thread,
 // charged atoms staged in tiles, FP32 within a tile and FP64 across), the
-// energy and virial (FP64), then the force on every charged atom.
+// energy and virial (FP64), then the force on every charged atom.  RBE and
+// EWALD: the k's of m, weights kw or exp(-k^2 / 4 alpha), virial
+// E_k (delta_ab - 2 k_a k_b (1/(4 alpha) + 1/k^2)).  RBSOG: the P sampled k's
+// (weight kw) and the nd direct vectors (per-replica weights kwd), every k in
+// the RBE form E_k = qqr2e (2 pi / V) w |rho|^2 / k^2; virial W_aa += E_k and
+// W_ab += tau_k |rho_s|^2 k_s,a k_s,b with s the virial-chain source of a
+// sampled k (tau = ktau, over |k_s|^4) or the direct vector itself (tau = kvd)
+// (kspace_calculator_op.hip.cu:1133-1222, :1308-1352).
 constexpr int kKsTile = 256;
 template <bool EFLAG>
 __global__ __launch_bounds__(256) void kb_kspace(BatchDev d, PairConst<float> pc,
@@ -608,17 +616,22 @@ __global__ __launch_bounds__(256) void kb_kspace(BatchDev d, PairConst<float> pc
   __shared__ float4 sa[kKsTile];
   __shared__ float4 kf0[kBatchMaxK];
   __shared__ float4 kf1[kBatchMaxK];
+  __shared__ double2 srho[kBatchMaxK];
   __shared__ double smem[16 * kBatchKAcc];
   const double two_pi = 6.283185307179586476925;
   const int P = d.P;
+  const bool sog = d.kmode == 2;
+  const int K = P + (sog ? d.nd : 0);
   for (int r = rg.r0 + blockIdx.x; r < rg.r1; r += gridDim.x) {
     const RepInfo& p = d.rep[r];
-    const int* m = d.m + (size_t)r * P * 3;
+    const int* m = d.m + (size_t)r * d.mstride;
+    // integer vector of k: sampled (replica's m) or direct (shared table)
+    auto mvec = [&](int k) { return k < P ? m + 3 * k : d.dm + 3 * (k - P); };
     double kv[kBatchKAcc] = {0, 0, 0, 0, 0, 0, 0};
-    for (int k0 = 0; k0 < P; k0 += blockDim.x) {
+    for (int k0 = 0; k0 < K; k0 += blockDim.x) {
       const int k = k0 + threadIdx.x;
-      const int kk = k < P ? k : 0;
-      const float mx = (float)m[3 * kk], my = (float)m[3 * kk + 1], mz = (float)m[3 * kk + 2];
+      const int* mk = mvec(k < K ? k : 0);
+      const float mx = (float)mk[0], my = (float)mk[1], mz = (float)mk[2];
       double re = 0.0, im = 0.0;
       for (int t0 = 0; t0 < p.nq; t0 += kKsTile) {
         const int tn = min(kKsTile, p.nq - t0);
@@ -641,32 +654,61 @@ __global__ __launch_bounds__(256) void kb_kspace(BatchDev d, PairConst<float> pc
         re += (double)fr;
         im += (double)fi;
       }
-      if (k < P) {
-        const double kx = two_pi * m[3 * k] / (double)p.box.L[0];
-        const double ky = two_pi * m[3 * k + 1] / (double)p.box.L[1];
-        const double kz = two_pi * m[3 * k + 2] / (double)p.box.L[2];
+      if (k < K) {
+        const double kx = two_pi * mk[0] / (double)p.box.L[0];
+        const double ky = two_pi * mk[1] / (double)p.box.L[1];
+        const double kz = two_pi * mk[2] / (double)p.box.L[2];
         const double k2 = kx * kx + ky * ky + kz * kz;
-        const double wk = d.ewald ? exp(-k2 / (4.0 * (double)pc.alpha)) : p.kw;
+        const double wk = d.kmode == 1 ? exp(-k2 / (4.0 * (double)pc.alpha))
+                          : (k < P ? p.kw : d.kwd[(size_t)r * d.nd + (k - P)]);
         const double A = wk * (-4.0 * 3.141592653589793238463 * (double)pc.qqr2e) /
                          (p.volume * k2);
-        kf0[k] = make_float4((float)m[3 * k], (float)m[3 * k + 1], (float)m[3 * k + 2],
-                             (float)(A * re));
+        kf0[k] = make_float4((float)mk[0], (float)mk[1], (float)mk[2], (float)(A * re));
         kf1[k] = make_float4((float)kx, (float)ky, (float)kz, (float)(A * im));
+        srho[k] = make_double2(re, im);
         if (EFLAG) {
           const double base =
               (double)pc.qqr2e * two_pi / p.volume * wk * (re * re + im * im) / k2;
-          const double cc = 2.0 * (1.0 / (4.0 * (double)pc.alpha) + 1.0 / k2);
           kv[0] += base;
-          kv[1] += base * (1.0 - cc * kx * kx);
-          kv[2] += base * (1.0 - cc * ky * ky);
-          kv[3] += base * (1.0 - cc * kz * kz);
-          kv[4] += base * (-cc * kx * ky);
-          kv[5] += base * (-cc * kx * kz);
-          kv[6] += base * (-cc * ky * kz);
+          if (sog) {
+            kv[1] += base;
+            kv[2] += base;
+            kv[3] += base;
+          } else {
+            const double cc = 2.0 * (1.0 / (4.0 * (double)pc.alpha) + 1.0 / k2);
+            kv[1] += base * (1.0 - cc * kx * kx);
+            kv[2] += base * (1.0 - cc * ky * ky);
+            kv[3] += base * (1.0 - cc * kz * kz);
+            kv[4] += base * (-cc * kx * ky);
+            kv[5] += base * (-cc * kx * kz);
+            kv[6] += base * (-cc * ky * kz);
+          }
         }
       }
     }
+    __syncthreads();  // srho of every k is written
     if (EFLAG) {
+      if (sog)
+        for (int k = threadIdx.x; k < K; k += blockDim.x) {
+          const int src = k < P ? m[3 * P + k] : k;
+          const int* ms = mvec(src);
+          const double kx = two_pi * ms[0] / (double)p.box.L[0];
+          const double ky = two_pi * ms[1] / (double)p.box.L[1];
+          const double kz = two_pi * ms[2] / (double)p.box.L[2];
+          const double2 rs = srho[src];
+          double t = (k < P ? p.ktau : d.kvd[(size_t)r * d.nd + (k - P)]) *
+                     (rs.x * rs.x + rs.y * rs.y);
+          if (k < P) {
+            const double k2 = kx * kx + ky * ky + kz * kz;
+            t /= k2 * k2;
+          }
+          kv[1] += t * kx * kx;
+          kv[2] += t * ky * ky;
+          kv[3] += t * kz * kz;
+          kv[4] += t * kx * ky;
+          kv[5] += t * kx * kz;
+          kv[6] += t * ky * kz;
+        }
       block_sum<kBatchKAcc>(kv, smem);
       if (threadIdx.x == 0)
         for (int q = 0; q < kBatchKAcc; ++q) d.kacc[(size_t)r * kBatchKAcc + q] = kv[q];
@@ -679,7 +721,7 @@ __global__ __launch_bounds__(256) void kb_kspace(BatchDev d, PairConst<float> pc
       const float sy = (x.y - p.box.lo[1]) * p.box.invL[1];
       const float sz = (x.z - p.box.lo[2]) * p.box.invL[2];
       float fx = 0.f, fy = 0.f, fz = 0.f;
-      for (int k = 0; k < P; ++k) {
+      for (int k = 0; k < K; ++k) {
         const float4 a = kf0[k];
         const float4 e = kf1[k];
         float sn, cs;
@@ -730,7 +772,7 @@ hipError_t batch_set_epoch(long long* epoch, long long step, hipStream_t s) {
 }
 
 hipError_t batch_launch(const BatchDev& d, const PairConst<float>& pc, int rbl,
-                        int coulomb, int eflag, const BatchRange& rg,
+                        int coulomb, int eflag, int special, const BatchRange& rg,
                         const BatchTune& t, int num_cu, hipStream_t s) {
   const int na = rg.a1 - rg.a0, ncl = rg.c1 - rg.c0, nw = rg.w1 - rg.w0;
   if (na <= 0) return hipSuccess;
@@ -739,17 +781,24 @@ hipError_t batch_launch(const BatchDev& d, const PairConst<float>& pc, int rbl,
   hipLaunchKernelGGL(kb_scatter, dim3(cdiv(na, 256)), dim3(256), 0, s, d, rg);
   hipLaunchKernelGGL(kb_cellsort, dim3(cdiv(ncl, 8)), dim3(256), 0, s, d, rg);
   const int pg = std::max(1, std::min(nw, t.grid_per_cu * num_cu));
-#define RBMDX_KB_PAIR(RB, EF)                                                      \
-  hipLaunchKernelGGL((kb_pair<RB, EF>), dim3(pg), dim3(t.pair_block), 0, s, d, pc, rg, \
+#define RBMDX_KB_PAIR(RB, EF, SP)                                                      \
+  hipLaunchKernelGGL((kb_pair<RB, EF, SP>), dim3(pg), dim3(t.pair_block), 0, s, d, pc, rg, \
                      t.bw, t.qlanes)
-  if (rbl && eflag)
-    RBMDX_KB_PAIR(true, true);
-  else if (rbl)
-    RBMDX_KB_PAIR(true, false);
-  else if (eflag)
-    RBMDX_KB_PAIR(false, true);
-  else
-    RBMDX_KB_PAIR(false, false);
+#define RBMDX_KB_PAIR2(RB, EF)   \
+  if (special)                   \
+    RBMDX_KB_PAIR(RB, EF, true); \
+  else                           \
+    RBMDX_KB_PAIR(RB, EF, false)
+  if (rbl && eflag) {
+    RBMDX_KB_PAIR2(true, true);
+  } else if (rbl) {
+    RBMDX_KB_PAIR2(true, false);
+  } else if (eflag) {
+    RBMDX_KB_PAIR2(false, true);
+  } else {
+    RBMDX_KB_PAIR2(false, false);
+  }
+#undef RBMDX_KB_PAIR2
 #undef RBMDX_KB_PAIR
   if (coulomb && d.P > 0) {
     const int kg = std::min(rg.r1 - rg.r0, 4 * num_cu);

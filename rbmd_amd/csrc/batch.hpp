@@ -33,7 +33,8 @@ struct RepInfo {
   int ifreq;
   unsigned fmask, c32;
   double volume;
-  double kw;        // RBE weight of every sampled k: S / P
+  double kw;        // weight of every sampled k: S / P (RBE), S / (4 pi P) (RBSOG)
+  double ktau;      // RBSOG: virial coefficient of a sampled k, -(S_npt / P) qqr2e / (4 V)
 };
 
 // One work item of the pair stage: a brick of one replica.
@@ -57,9 +58,19 @@ struct BatchDev {
   float* fx = nullptr;
   float* fy = nullptr;
   float* fz = nullptr;
-  const int* m = nullptr;         // [R * P * 3] k-vectors, replica-major
-  int ewald = 0;                  // 1: the kmax grid weighted exp(-k^2 / 4 alpha)
-                                  // (EWALD) instead of the RBE weight kw
+  const int* m = nullptr;         // [R * mstride] k-vectors (and RBSOG sources), replica-major
+  int mstride = 0;                // ints per replica in m: 3 P, or 4 P for RBSOG (the P
+                                  // k-vectors, then their virial-chain sources)
+  int kmode = 0;                  // 0 RBE (weight kw), 1 EWALD (the kmax grid weighted
+                                  // exp(-k^2 / 4 alpha)), 2 RBSOG (P sampled k's with kw
+                                  // and nd direct vectors with per-replica weights)
+  int nd = 0;                     // RBSOG direct vectors (|m|^2 <= 1)
+  const int* dm = nullptr;        // [3 nd]
+  const double* kwd = nullptr;    // [R * nd] direct-vector weights f(k) k^2 / (4 pi)
+  const double* kvd = nullptr;    // [R * nd] their virial coefficients -qqr2e f_npt(k) / (4 V)
+  // CVFF special bonds (caller order; partner ids are ids within the replica)
+  const int4* spn = nullptr;      // [N] (n12, n13, n14, 0)
+  const int* spid = nullptr;      // [N * ms]
   // scratch
   int* cnt = nullptr;             // [ncell] atoms per cell (zero between calls)
   int* cntq = nullptr;            // [ncell] charged atoms per cell (likewise)
@@ -99,7 +110,7 @@ struct BatchRange {
 
 hipError_t batch_set_epoch(long long* epoch, long long step, hipStream_t s);
 hipError_t batch_launch(const BatchDev& d, const PairConst<float>& pc, int rbl,
-                        int coulomb, int eflag, const BatchRange& rg,
+                        int coulomb, int eflag, int special, const BatchRange& rg,
                         const BatchTune& t, int num_cu, hipStream_t s);
 
 }  // namespace rbmdx

@@ -1092,14 +1092,6 @@ __global__ __launch_bounds__(512) void k_lists_dense(Dev<R> d, Box<R> b, Grid g,
   list_b_dense<R>(d, b, g, blockIdx.x, r2b, cap_b, ntypes);
 }
 
-// CVFF special-bond row of one atom: cumulative counts n12 <= n13 <= n14 of
-// its 1-2, 1-3 and 1-4 partners (clamped on the host as LoadSpecialCounts
-// clamps them, cvff_op.hip.cu:51-77) and their global ids.
-struct SpecRow {
-  int n12 = 0, n13 = 0, n14 = 0;
-  const int* ids = nullptr;
-};
-
 template <typename R>
 __device__ __forceinline__ SpecRow spec_row(const Dev<R>& d, int ms, int s) {
   SpecRow r;
@@ -1109,24 +1101,6 @@ __device__ __forceinline__ SpecRow spec_row(const Dev<R>& d, int ms, int s) {
   r.n14 = c.z;
   r.ids = d.spid_s + (size_t)s * ms;
   return r;
-}
-
-// Weight of the pair (i, gid) for i's row: the first match decides, 1 when
-// gid is no special partner (LookupSpecialWeight, cvff_op.hip.cu:79-110).
-template <typename R>
-__device__ __forceinline__ R special_w(const SpecRow& r, int gid,
-                                       const PairConst<R>& pc) {
-  // the weights are selected as values: a select of their addresses would
-  // keep a kernel's private copy of pc in scratch memory
-  const R w12 = pc.w12, w13 = pc.w13, w14 = pc.w14;
-  int hit = -1;
-  for (int k = 0; k < r.n14; ++k)
-    if (hit < 0 && r.ids[k] == gid) hit = k;
-  R w = R(1);
-  if (hit >= 0) w = w14;
-  if (hit >= 0 && hit < r.n13) w = w13;
-  if (hit >= 0 && hit < r.n12) w = w12;
-  return w;
 }
 
 // LJ-only pair of list A.  hj() yields j's selection hash (evaluated only
@@ -1159,80 +1133,6 @@ __device__ __forceinline__ void interact_a(const PairConst<R>& pc,
   if (EFLAG) {
     a.e_lj += (double)(ws * (R(2) * eps * (s6 - R(1)) * s6));
     add_virial<R, EFLAG>(a, dx, dy, dz, flj);
-  }
-}
-
-// SOG near-field Coulomb (VERLET-SOG, lj_cut_coul_kspace_op.hip.cu:196-252):
-// f = -qqr2e qq (1/r^3 + sum_n T_n r^{2n}) and, when energies are wanted,
-// e = 1/2 qqr2e qq (1/r - sum_l omega_l exp(-r^2 / (2 s_l^2))).
-template <typename R, bool EFLAG>
-__device__ __forceinline__ void coul_sog(const PairConst<R>& pc, R d2, R qq,
-                                         R& f, R& e) {
-  const R rinv = R(1) / sqrt(d2);
-  const R r3inv = rinv / d2;
-  R poly = pc.T[5];
-#pragma unroll
-  for (int n = 4; n >= 0; --n) poly = fma_r(poly, d2, pc.T[n]);
-  f = -pc.qqr2e * qq * (r3inv + poly);
-  e = R(0);
-  if (EFLAG) {
-    R sume = R(0);
-    for (int l = 0; l < pc.far_m; ++l)
-      sume += pc.far_tab[2 * l] * exp(-d2 * pc.far_tab[2 * l + 1]);
-    e = R(0.5) * pc.qqr2e * qq * (rinv - sume);
-  }
-}
-
-// The terms of a charged pair inside r_c once its RBL weight w is known:
-// qq = q_i q_j, (dx, dy, dz) = r_j - r_i, d2 its square norm.
-template <typename R, bool EFLAG, bool SPECIAL>
-__device__ __forceinline__ void eval_b(const PairConst<R>& pc, const R* s_lj,
-                                       int ti, int tj, R qq, R dx, R dy, R dz,
-                                       R d2, R w, Acc<R>& a, const SpecRow& sr,
-                                       int gj) {
-  R flj = 0, elj = 0;
-  if (d2 < pc.rclj2) {
-    const int tt = 4 * (ti * pc.ntypes + tj);
-    const R m24 = s_lj[tt], sig2 = s_lj[tt + 1], eps = s_lj[tt + 2];
-    const R inv = R(1) / d2;
-    const R s2 = sig2 * inv;
-    const R s6 = s2 * s2 * s2;
-    flj = m24 * (R(2) * s6 - R(1)) * s6 * inv;
-    elj = R(2) * eps * (s6 - R(1)) * s6;
-  }
-  R fc, ec;
-  if (pc.sog)
-    coul_sog<R, EFLAG>(pc, d2, qq, fc, ec);
-  else
-    coul<R>(pc, d2, qq, fc, ec);
-  if (!SPECIAL) {
-    const R f = w * (flj + fc);
-    a.fx += f * dx;
-    a.fy += f * dy;
-    a.fz += f * dz;
-    if (EFLAG) {
-      a.e_lj += (double)elj;
-      a.e_c += (double)ec;
-      add_virial<R, EFLAG>(a, dx, dy, dz, flj + fc);
-    }
-    return;
-  }
-  const R ws = special_w<R>(sr, gj, pc);
-  R ffull = R(0), efull = R(0);
-  if (ws != R(1)) {
-    const R rinv = R(1) / sqrt(d2);
-    ffull = -pc.qqr2e * qq * rinv * rinv * rinv;
-    efull = R(0.5) * pc.qqr2e * qq * rinv;
-  }
-  const R corr = (R(1) - ws) * ffull;
-  const R f = w * (ws * flj + fc) - (w != R(0) ? corr : R(0));
-  a.fx += f * dx;
-  a.fy += f * dy;
-  a.fz += f * dz;
-  if (EFLAG) {
-    a.e_lj += (double)(ws * elj);
-    a.e_c += (double)(ec - (R(1) - ws) * efull);
-    add_virial<R, EFLAG>(a, dx, dy, dz, ws * flj + fc - corr);
   }
 }
 

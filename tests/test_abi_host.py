@@ -178,13 +178,17 @@ def test_sog_sampler_stream_equals_oracle(level, P, seed):
 
 @pytest.mark.parametrize("edit,msg", [
     (lambda hp: hp.update(precision="fp64"), "FP32"),
-    (lambda hp: hp["coulomb"].update(type="RBSOG", rbsog_level=1), "RBE and EWALD"),
+    (lambda hp: (hp["neighbor"].update(type="VERLET-SOG"), hp["coulomb"].update(rbsog_level=1)),
+     "RBL and Verlet-List"),
+    (lambda hp: hp["coulomb"].update(type="RBSOG", rbsog_level=1, rbsog_near_field="sog"),
+     "erfc near field"),
+    (lambda hp: hp["coulomb"].update(type="RBSOG", rbsog_level=1, coulomb_sample_num=510),
+     "506"),
     (lambda hp: hp["coulomb"].update(type="EWALD"), "kmax"),
     (lambda hp: hp["coulomb"].update(type="EWALD", kmax=[5, 5, 5]), "512 k-vectors"),
     (lambda hp: (hp["coulomb"].pop("alpha"), hp["coulomb"].update(accuracy=1e-4)),
      "coulomb.alpha"),
     (lambda hp: hp["coulomb"].update(coulomb_sample_num=600), "sample_num"),
-    (lambda hp: hp["force_field"].update(type="CVFF"), "not CVFF"),
 ])
 def test_batch_config_validation(edit, msg):
     """The replica batch rejects what it does not implement, before touching

@@ -177,3 +177,111 @@ def test_batch_ewald_matches_fp64_oracle_per_replica(device):
         np.testing.assert_allclose(E[r, 2], ek, rtol=1e-5)
         w = np.asarray(vir) + np.asarray(vk)
         np.testing.assert_allclose(E[r, 3:], w, rtol=1e-5, atol=1e-5 * np.abs(w).max())
+
+
+def _sog_samples(reps, level, rc, P, seed, step):
+    """Per replica the RBSOG k-sample (m (P,3), idx (P,)) and the flat input
+    of rbmdx_batch_force: each replica's P k-vectors, then its P sources."""
+    from rbmd_amd import sog_sample_host
+
+    ms = [sog_sample_host(level, rc, s.box_len, P, seed + r, step) for r, s in enumerate(reps)]
+    flat = np.concatenate([np.concatenate([m.reshape(-1), idx]) for m, idx in ms])
+    return ms, flat.astype(np.int32)
+
+
+def _sog_kspace(s, pos32, m, idx, level, rc, qqr2e):
+    """FP64 oracle RBSOG k-space force (N,3), energy with the self term, virial."""
+    import oracle
+
+    sset = oracle.sog_setup(level, rc, s.box_len)
+    ci = np.nonzero(s.charges)[0]
+    fk, ek, vk = oracle.sog_kspace_f64(pos32.astype(np.float64)[ci], s.charges[ci],
+                                      s.box_len, m, idx, sset, qqr2e=qqr2e)
+    F = np.zeros((s.n, 3))
+    F[ci] = fk
+    return F, ek + oracle.sog_self((s.charges ** 2).sum(), sset, qqr2e), vk
+
+
+def test_batch_rbsog_matches_fp64_oracle_per_replica(device):
+    """RBSOG k-space (level 1, P = 64 sampled k's and the direct vectors) in the
+    batch: every replica of the electrolyte set against the FP64 oracle of that
+    box (its own SOG coefficients and normalisation, its own k-sample)."""
+    import torch
+
+    P = 64
+    cfg = systems.ljsalt_config(P=P)
+    cl = cfg["hyper_parameters"]["coulomb"]
+    cl.update({"type": "RBSOG", "rbsog_level": 1, "coulomb_sample_num": P})
+    cl.pop("energy_rbe_flag", None)
+    b, reps = _batch(cfg, SIZES, seed=31)
+    assert b.num_samples() == P
+    step = 2
+    px, py, pz, f, _, _ = _tensors(device, reps, P, step)
+    ms, flat = _sog_samples(reps, 1, 3.5, P, 77, step)
+    b.force(step, px, py, pz, *f, m_samples=torch.tensor(flat, device=device), energy=True,
+            sync=True)
+    F = _forces(f)
+    E = b.energies()
+    off = b.offsets
+    for r, s in enumerate(reps):
+        pos32 = s.pos.astype(np.float32)
+        Fp, en, vir, _, _ = reference_force(s, pos32, cfg, step, None, with_kspace=False)
+        Fk, ek, vk = _sog_kspace(s, pos32, ms[r][0], ms[r][1], 1, 3.5, 1.0)
+        err = rel_rms(F[off[r]:off[r + 1]], Fp + Fk)
+        print(f"RBSOG replica {r} N={s.n}: rel RMS {err:.2e}")
+        assert err < 2e-5
+        np.testing.assert_allclose(E[r, 0], en[0], rtol=1e-5)
+        np.testing.assert_allclose(E[r, 1], en[1], rtol=1e-5)
+        np.testing.assert_allclose(E[r, 2], ek, rtol=1e-5)
+        w = np.asarray(vir) + np.asarray(vk)
+        np.testing.assert_allclose(E[r, 3:], w, rtol=1e-5, atol=1e-5 * np.abs(w).max())
+
+
+@pytest.mark.parametrize("coulomb", ["RBE", "RBSOG"])
+def test_batch_cvff_water_matches_fp64_oracle_per_replica(device, coulomb):
+    """CVFF SPC/E water replicas (420 to 1,000 molecules, boxes of 23 to 31 A)
+    with special_bonds [0, 0.5, 0.25], RBL and RBE or RBSOG (level 1) k-space:
+    every replica against the FP64 CVFF oracle of that box.  Tolerances as the
+    single-system CVFF water tests (tests/test_gpu_cvff.py): forces 5e-5,
+    pair energies 2e-5; k-space energy 1e-5."""
+    import torch
+
+    from tests.refs import cfg_qqr2e
+
+    P = 64
+    cfg = systems.cvff_water_config(coulomb=coulomb, alpha=0.0784, P=P,
+                                    special_bonds=(0.0, 0.5, 0.25))
+    reps = [systems.spce_water(n, seed=40 + r) for r, n in enumerate([420, 600, 800, 1000])]
+    b = BatchContext(cfg, device=0)
+    b.set_systems(reps)
+    b.set_special(reps)
+    step = 4
+    pos = np.concatenate([s.pos for s in reps]).astype(np.float32)
+    p = torch.tensor(pos, device=device)
+    px, py, pz = (p[:, d].contiguous() for d in range(3))
+    f = [torch.zeros(len(pos), dtype=torch.float32, device=device) for _ in range(3)]
+    if coulomb == "RBE":
+        ms = [rbe_sample_host(0.0784, s.box_len, P, 50 + r, step) for r, s in enumerate(reps)]
+        flat = np.concatenate(ms).reshape(-1).astype(np.int32)
+    else:
+        ms, flat = _sog_samples(reps, 1, 10.0, P, 50, step)
+    b.force(step, px, py, pz, *f, m_samples=torch.tensor(flat, device=device), energy=True,
+            sync=True)
+    F = _forces(f)
+    E = b.energies()
+    off = b.offsets
+    qqr2e = cfg_qqr2e(cfg)
+    for r, s in enumerate(reps):
+        pos32 = s.pos.astype(np.float32)
+        if coulomb == "RBE":
+            Fo, en, vir, ek, vk = reference_force(s, pos32, cfg, step, ms[r])
+        else:
+            Fp, en, vir, _, _ = reference_force(s, pos32, cfg, step, None, with_kspace=False)
+            Fk, ek, vk = _sog_kspace(s, pos32, ms[r][0], ms[r][1], 1, 10.0, qqr2e)
+            Fo = Fp + Fk
+        err = rel_rms(F[off[r]:off[r + 1]], Fo)
+        print(f"CVFF water {coulomb} replica {r} N={s.n}: rel RMS {err:.2e}")
+        assert err < 5e-5
+        np.testing.assert_allclose(E[r, 0], en[0], rtol=2e-5)
+        np.testing.assert_allclose(E[r, 1], en[1], rtol=2e-5)
+        np.testing.assert_allclose(E[r, 2], ek, rtol=1e-5)
