@@ -225,7 +225,8 @@ __device__ __forceinline__ void batch_charged(const PairConst<float>& pc,
   const float d2 = pair_d2<float, MI>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
   if (!(d2 < pc.rc2)) return;
   const float w = pair_weight<float, RBL>(pc, d2, hi, [&] { return rbl_hj(gj); });
-  eval_b<float, EFLAG, SPECIAL>(pc, s_lj, ti, tj, xi.w * xj.w, dx, dy, dz, d2, w, a, sr, gj);
+  const float ws = SPECIAL ? special_w<float>(sr, gj, pc) : 1.f;
+  eval_b<float, EFLAG, SPECIAL>(pc, s_lj, ti, tj, xi.w * xj.w, dx, dy, dz, d2, w, a, ws);
 }
 
 // The CVFF special-bond row of sorted atom s (counts and partner ids in

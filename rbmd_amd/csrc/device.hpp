@@ -79,6 +79,23 @@ __device__ __forceinline__ R pair_d2(const Box<R>& b, R xi, R yi, R zi, R xj,
   return fma_r(dz, dz, fma_r(dy, dy, mul_rn(dx, dx)));
 }
 
+// pair_d2 for a list entry: the image shift as k = rint(d / L) in {-1, 0, 1}
+// and d - k L as one fma.  Every pair that can pass a cutoff (|d| <= cut +
+// skin < L/2, which the configuration enforces) gets the same d and d^2 bit
+// for bit as pair_d2 (no shift, or one subtraction of L); the two differ only
+// for |d| within rounding of L/2, far outside every cutoff under either.
+template <typename R>
+__device__ __forceinline__ R pair_d2_list(const Box<R>& b, R xi, R yi, R zi, R xj,
+                                          R yj, R zj, R& dx, R& dy, R& dz) {
+  dx = xj - xi;
+  dy = yj - yi;
+  dz = zj - zi;
+  dx = fma_r(-rint(dx * b.invL[0]), b.L[0], dx);
+  dy = fma_r(-rint(dy * b.invL[1]), b.L[1], dy);
+  dz = fma_r(-rint(dz * b.invL[2]), b.L[2], dz);
+  return fma_r(dz, dz, fma_r(dy, dy, mul_rn(dx, dx)));
+}
+
 // Fold a coordinate into [lo, hi).  A no-op for coordinates already inside.
 template <typename R>
 __device__ __forceinline__ R wrap(R x, R lo, R hi, R L, R invL) {
@@ -121,6 +138,24 @@ __device__ __forceinline__ bool rbl_keep(int f, unsigned fmask, unsigned c32,
   if (f <= 1) return true;
   const unsigned long long h = mix64((unsigned long long)epoch ^ hi ^ hj);
   if (fmask) return ((unsigned)h & fmask) == 0u;
+  const unsigned uf = (unsigned)f;
+  if (uf < 65536u) {
+    const unsigned r1 = (unsigned)(h >> 32) % uf;
+    unsigned r = (r1 * c32) % uf + (unsigned)h % uf;
+    if (r >= uf) r -= uf;
+    return r == 0u;
+  }
+  return h % (unsigned long long)f == 0ULL;
+}
+// rbl_keep with the hash formed ahead of every branch (f = 1 is the power of
+// two 2^0, whose mask 0 keeps every pair): the same decision, and no load of
+// hj sinks into a branch where it would wait for every load before it.
+__device__ __forceinline__ bool rbl_keep_flat(int f, unsigned fmask, unsigned c32,
+                                              long long epoch,
+                                              unsigned long long hi,
+                                              unsigned long long hj) {
+  const unsigned long long h = mix64((unsigned long long)epoch ^ hi ^ hj);
+  if (fmask != 0u || f <= 1) return ((unsigned)h & fmask) == 0u;
   const unsigned uf = (unsigned)f;
   if (uf < 65536u) {
     const unsigned r1 = (unsigned)(h >> 32) % uf;
@@ -257,6 +292,19 @@ __device__ __forceinline__ R pair_weight(const PairConst<R>& pc, R d2,
   return R(0);
 }
 
+// pair_weight as selects: the selection hash is formed for every candidate,
+// which on a wave costs less than the divergent branches around it.
+template <typename R, bool RBL>
+__device__ __forceinline__ R pair_weight_flat(const PairConst<R>& pc, R d2,
+                                              unsigned long long hi,
+                                              unsigned long long hj) {
+  if (!RBL) return R(1);
+  const bool sel = rbl_keep_flat(pc.ifreq, pc.fmask, pc.c32, pc.epoch, hi, hj);
+  const R wc = ((d2 < pc.rs2) & (d2 > pc.eps_min)) ? R(1) : R(0);
+  const R wsh = ((d2 < pc.t_shell) & (d2 > pc.rs2) & sel) ? pc.freq : R(0);
+  return d2 < pc.t_core ? wc : wsh;
+}
+
 template <typename R>
 struct Acc {
   R fx = 0, fy = 0, fz = 0;
@@ -276,15 +324,22 @@ __device__ __forceinline__ void add_virial(Acc<R>& a, R dx, R dy, R dz, R f) {
   }
 }
 
-// erfc-screened Coulomb scalar and half entry energy.
+// Reciprocal and reciprocal square root of d^2 in the pair terms: the
+// hardware instructions (1 ulp) in FP32, IEEE division and square root in FP64.
+__device__ __forceinline__ float pair_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double pair_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float pair_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double pair_rsqrt(double x) { return 1.0 / sqrt(x); }
+
+// erfc-screened Coulomb scalar and half entry energy, from d^2, 1/r and
+// 1/r^2.
 template <typename R>
-__device__ __forceinline__ void coul(const PairConst<R>& pc, R d2, R qq, R& f,
-                                     R& e) {
-  const R r = sqrt(d2);
-  const R ec = erfc(pc.g * r);
-  const R gn = ec / d2 + pc.two_g_over_sqrtpi * exp(-pc.alpha * d2) / r;
-  f = -pc.qqr2e * qq * gn / r;
-  e = R(0.5) * pc.qqr2e * qq * ec / r;
+__device__ __forceinline__ void coul(const PairConst<R>& pc, R d2, R rinv, R inv, R qq,
+                                     R& f, R& e) {
+  const R ec = erfc(pc.g * (d2 * rinv));
+  const R gn = ec * inv + pc.two_g_over_sqrtpi * exp(-pc.alpha * d2) * rinv;
+  f = -pc.qqr2e * qq * gn * rinv;
+  e = R(0.5) * pc.qqr2e * qq * ec * rinv;
 }
 
 // CVFF special-bond row of one atom: cumulative counts n12 <= n13 <= n14 of
@@ -313,14 +368,42 @@ __device__ __forceinline__ R special_w(const SpecRow& r, int gid,
   return w;
 }
 
+// A row of up to kSpecReg partners held in registers (one atom per wave:
+// scalar registers), so that the weight of a pair is a few compares and
+// selects with no load.  Entries past n14 repeat the last partner, which
+// leaves the first match unchanged.
+constexpr int kSpecReg = 8;
+struct SpecRowReg {
+  int n12 = 0, n13 = 0, n14 = 0;
+  int id[kSpecReg] = {};
+};
+__device__ __forceinline__ SpecRowReg spec_row_reg(const SpecRow& r) {
+  SpecRowReg g;
+  g.n12 = r.n12;
+  g.n13 = r.n13;
+  g.n14 = r.n14;
+#pragma unroll
+  for (int k = 0; k < kSpecReg; ++k) g.id[k] = r.n14 > 0 ? r.ids[min(k, r.n14 - 1)] : 0;
+  return g;
+}
+// special_w for a row of at most kSpecReg partners
+template <typename R>
+__device__ __forceinline__ R special_w_reg(const SpecRowReg& r, int gid,
+                                           const PairConst<R>& pc) {
+  int hit = kSpecReg;
+#pragma unroll
+  for (int k = kSpecReg - 1; k >= 0; --k) hit = r.id[k] == gid ? k : hit;
+  const R w12 = pc.w12, w13 = pc.w13, w14 = pc.w14;
+  return hit < r.n12 ? w12 : (hit < r.n13 ? w13 : (hit < r.n14 ? w14 : R(1)));
+}
+
 // SOG near-field Coulomb (VERLET-SOG, lj_cut_coul_kspace_op.hip.cu:196-252):
 // f = -qqr2e qq (1/r^3 + sum_n T_n r^{2n}) and, when energies are wanted,
 // e = 1/2 qqr2e qq (1/r - sum_l omega_l exp(-r^2 / (2 s_l^2))).
 template <typename R, bool EFLAG>
-__device__ __forceinline__ void coul_sog(const PairConst<R>& pc, R d2, R qq,
-                                         R& f, R& e) {
-  const R rinv = R(1) / sqrt(d2);
-  const R r3inv = rinv / d2;
+__device__ __forceinline__ void coul_sog(const PairConst<R>& pc, R d2, R rinv, R inv,
+                                         R qq, R& f, R& e) {
+  const R r3inv = rinv * inv;
   R poly = pc.T[5];
 #pragma unroll
   for (int n = 4; n >= 0; --n) poly = fma_r(poly, d2, pc.T[n]);
@@ -334,18 +417,19 @@ __device__ __forceinline__ void coul_sog(const PairConst<R>& pc, R d2, R qq,
   }
 }
 
-// The terms of a charged pair inside r_c once its RBL weight w is known:
-// qq = q_i q_j, (dx, dy, dz) = r_j - r_i, d2 its square norm.
+// The terms of a charged pair inside r_c once its RBL weight w and (with
+// SPECIAL) its special-bond weight ws are known: qq = q_i q_j,
+// (dx, dy, dz) = r_j - r_i, d2 its square norm.
 template <typename R, bool EFLAG, bool SPECIAL>
 __device__ __forceinline__ void eval_b(const PairConst<R>& pc, const R* s_lj,
                                        int ti, int tj, R qq, R dx, R dy, R dz,
-                                       R d2, R w, Acc<R>& a, const SpecRow& sr,
-                                       int gj) {
+                                       R d2, R w, Acc<R>& a, R ws) {
+  const R rinv = pair_rsqrt(d2);
+  const R inv = pair_rcp(d2);
   R flj = 0, elj = 0;
   if (d2 < pc.rclj2) {
     const int tt = 4 * (ti * pc.ntypes + tj);
     const R m24 = s_lj[tt], sig2 = s_lj[tt + 1], eps = s_lj[tt + 2];
-    const R inv = R(1) / d2;
     const R s2 = sig2 * inv;
     const R s6 = s2 * s2 * s2;
     flj = m24 * (R(2) * s6 - R(1)) * s6 * inv;
@@ -353,9 +437,9 @@ __device__ __forceinline__ void eval_b(const PairConst<R>& pc, const R* s_lj,
   }
   R fc, ec;
   if (pc.sog)
-    coul_sog<R, EFLAG>(pc, d2, qq, fc, ec);
+    coul_sog<R, EFLAG>(pc, d2, rinv, inv, qq, fc, ec);
   else
-    coul<R>(pc, d2, qq, fc, ec);
+    coul<R>(pc, d2, rinv, inv, qq, fc, ec);
   if (!SPECIAL) {
     const R f = w * (flj + fc);
     a.fx += f * dx;
@@ -368,10 +452,8 @@ __device__ __forceinline__ void eval_b(const PairConst<R>& pc, const R* s_lj,
     }
     return;
   }
-  const R ws = special_w<R>(sr, gj, pc);
   R ffull = R(0), efull = R(0);
   if (ws != R(1)) {
-    const R rinv = R(1) / sqrt(d2);
     ffull = -pc.qqr2e * qq * rinv * rinv * rinv;
     efull = R(0.5) * pc.qqr2e * qq * rinv;
   }

@@ -857,7 +857,7 @@ __device__ __forceinline__ void list_b_block(const Dev<R>& d, const Box<R>& b,
 // each of its pairs when the dimension has at least 4 cells (both are one
 // subtraction of L from r_j - r_i); with fewer the minimum image is applied
 // per pair.
-constexpr int kDenseCoop = 12;
+constexpr int kDenseCoop = 24;
 
 template <typename R, bool MI>
 __device__ __forceinline__ unsigned long long dense_mask(const Box<R>& b,
@@ -1154,15 +1154,9 @@ __device__ __forceinline__ void interact_b(const PairConst<R>& pc,
   const R d2 = pair_d2<R>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
   if (!(d2 < pc.rc2)) return;
   const R w = pair_weight<R, RBL>(pc, d2, hi, [&] { return *hjp; });
-  eval_b<R, EFLAG, SPECIAL>(pc, s_lj, ti, tj, xi.w * xj.w, dx, dy, dz, d2, w, a, sr, gj);
+  const R ws = SPECIAL ? special_w<R>(sr, gj, pc) : R(1);
+  eval_b<R, EFLAG, SPECIAL>(pc, s_lj, ti, tj, xi.w * xj.w, dx, dy, dz, d2, w, a, ws);
 }
-
-// Reciprocal of d^2 in the force terms: the hardware reciprocal (1 ulp) in
-// FP32, IEEE division in FP64.
-__device__ __forceinline__ float pair_rcp(float x) {
-  return __builtin_amdgcn_rcpf(x);
-}
-__device__ __forceinline__ double pair_rcp(double x) { return 1.0 / x; }
 
 template <typename R>
 struct R2Of {
@@ -1504,15 +1498,15 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
 // ---------------------------------------------------------------------------
 // Charged pairs of mostly charged systems (water, molten salts): one wave per
 // charged atom.  The wave reads the atom's list-B row and its type bytes 64
-// entries per lane step, kQBatch steps' loads in flight, applies the distance
-// test and the RBL class (the selection hash is read for shell pairs only)
-// and packs the pairs that add a force -- with energies, every pair inside
-// r_c -- into a per-wave LDS ring in list order: the pair vector and d^2,
-// q_j, j with a weight code and the type of j.  Every 64 packed pairs then
-// take LJ, the Coulomb term and the special-bond weight on a full wave
-// without further gathers (but j's global id for special bonds), so with RBL
-// at r_c + skin only the core and the selected shell (about a quarter of a
-// row) reach that part.  Per-atom sums: each lane's terms in ring order, then
+// entries per lane step, kQBatch steps' loads in flight (position and
+// selection hash of every candidate), applies the distance test and the RBL
+// class as selects, without branches, and packs the pairs that add a force
+// -- with energies, every pair inside r_c -- into a per-wave LDS ring in list
+// order: the pair vector and d^2, q_j, j with a weight code and the type of
+// j.  Every 64 packed pairs then take LJ, the Coulomb term and the
+// special-bond weight on a full wave without further gathers (but j's global
+// id for special bonds), so with RBL at r_c + skin only the core and the
+// selected shell (about a quarter of a row) reach that part.  Per-atom sums: each lane's terms in ring order, then
 // a fixed tree over the wave; block partials of the thermo sums in `part`
 // (k_reduce's second segment).  Each wave takes a contiguous run of atoms,
 // and the runs of one XCD are contiguous, so an XCD's L2 holds one region's
@@ -1555,6 +1549,9 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_pair_b(Dev<R> d, Box<R> b, Grid g
     const unsigned long long hi = RBL ? rbl_hi(d.gidc[c]) : 0ull;
     const int cnt = d.cnt_b[c];
     const SpecRow sr = SPECIAL ? spec_row<R>(d, pc.ms, d.c2s[c]) : SpecRow();
+    // rows of up to kSpecReg partners (water: 2) from registers
+    const bool sreg = SPECIAL && sr.n14 <= kSpecReg;
+    const SpecRowReg srr = sreg ? spec_row_reg(sr) : SpecRowReg();
     Acc<R> a;
     unsigned head = 0, tail = 0;  // ring positions (wave-uniform)
     // one candidate per lane, appended in lane order when kept; code 1:
@@ -1586,8 +1583,10 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_pair_b(Dev<R> d, Box<R> b, Grid g
           const int code = (ev >> 30) & 3;
           const int gj = SPECIAL ? d.gidc[ev & 0x3fffffff] : 0;
           const R w = code == 0 ? R(0) : (code == 1 ? R(1) : pc.freq);
+          const R ws = !SPECIAL ? R(1)
+                       : (sreg ? special_w_reg<R>(srr, gj, pc) : special_w<R>(sr, gj, pc));
           eval_b<R, EFLAG, SPECIAL>(pc, s_lj, ti, (int)rt[s], xi.w * rq[s], v.x, v.y, v.z,
-                                    v.w, w, a, sr, gj);
+                                    v.w, w, a, ws);
         }
         head += m;
         __builtin_amdgcn_wave_barrier();
@@ -1599,9 +1598,11 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_pair_b(Dev<R> d, Box<R> b, Grid g
       for (int k0 = 0; k0 < cnt; k0 += 64 * kQBatch) {
         int e[kQBatch], tj[kQBatch];
         R4T<R> xj[kQBatch];
-        R dx[kQBatch], dy[kQBatch], dz[kQBatch], d2[kQBatch];
         unsigned long long hj[kQBatch];
-        // indices past the count are clamped, so every load issues
+        // indices past the count are clamped, so every load issues; the
+        // selection hash of every candidate is gathered beside its position,
+        // so that no load waits for the distance and the filter below has no
+        // branches
 #pragma unroll
         for (int u = 0; u < kQBatch; ++u) {
           const int k = min(k0 + 64 * u + lane, cnt - 1);
@@ -1609,21 +1610,19 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_pair_b(Dev<R> d, Box<R> b, Grid g
           tj[u] = tp[k];
         }
 #pragma unroll
-        for (int u = 0; u < kQBatch; ++u) xj[u] = d.xqc[e[u]];
-#pragma unroll
         for (int u = 0; u < kQBatch; ++u) {
-          d2[u] = pair_d2<R>(b, xi.x, xi.y, xi.z, xj[u].x, xj[u].y, xj[u].z, dx[u], dy[u],
-                             dz[u]);
-          // the selection hash of j is read for RBL shell candidates only
-          hj[u] = (RBL && !(d2[u] < pc.t_core) && d2[u] < pc.t_shell && d2[u] > pc.rs2)
-                      ? d.hjc[e[u]]
-                      : 0ull;
+          xj[u] = d.xqc[e[u]];
+          hj[u] = RBL ? d.hjc[e[u]] : 0ull;
         }
 #pragma unroll
         for (int u = 0; u < kQBatch; ++u) {
-          const bool in = k0 + 64 * u + lane < cnt && d2[u] < pc.rc2;
-          const R w = in ? pair_weight<R, RBL>(pc, d2[u], hi, [&] { return hj[u]; }) : R(0);
-          offer(EFLAG ? in : w != R(0), e[u], tj[u], dx[u], dy[u], dz[u], d2[u], w, xj[u].w);
+          R dx, dy, dz;
+          const R d2 = pair_d2_list<R>(b, xi.x, xi.y, xi.z, xj[u].x, xj[u].y, xj[u].z, dx,
+                                       dy, dz);
+          const R wf = pair_weight_flat<R, RBL>(pc, d2, hi, hj[u]);
+          const bool in = (k0 + 64 * u + lane < cnt) & (d2 < pc.rc2);
+          const R w = in ? wf : R(0);
+          offer(EFLAG ? in : w != R(0), e[u], tj[u], dx, dy, dz, d2, w, xj[u].w);
         }
         drain(64);
       }
@@ -2221,7 +2220,7 @@ __device__ __forceinline__ R md_half_kick(R v, R f, R m, R dt, R ftm2v) {
 // v += 0.5 F / m dt ftm2v instead and writes the workgroup's FP64 sum of
 // m v^2 to d.md_part[blockIdx.x].
 template <typename R>
-__global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix,
+__global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix, int has_a,
                                                   int has_k, int has_b, R dt,
                                                   R dt_ftm, int md_kick, R ftm2v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2239,7 +2238,7 @@ __global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix,
   double ke = 0.0;
   if (i < d.n) {
     const int s = d.inv_perm[i];
-    R4T<R> f = d.fa[s];
+    R4T<R> f = has_a ? d.fa[s] : make4<R>(R(0), R(0), R(0), R(0));
     if (mean_fix) {
       f.x -= (R)d.mean[0];
       f.y -= (R)d.mean[1];
@@ -2466,7 +2465,12 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
                      a.lj_tab, p.split)
   // list A holds RBL shell pairs only when the LJ cutoff exceeds r_core
   const bool shell_a = a.pc.rclj2 > a.pc.rs2;
-  if (p.special) {
+  // every atom charged (water, molten salts): list A is empty and every pair
+  // is k_pair_b's, so the brick kernel is not launched at all
+  const bool has_a = !(p.split && d.nc == d.n && g.qblocks > 0);
+  if (!has_a) {
+    // nothing: k_assemble reads no list-A force
+  } else if (p.special) {
     // CVFF special bonds: the general (shell-capable) list-A loop
     if (p.rbl) {
       if (p.eflag)
@@ -2541,12 +2545,12 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
     for (int q = 0; q < 3; ++q) ksum.L[q] = (double)a.box.L[q];
   }
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, d.pair_part,
-                     g.nbrick * g.psub, (const double*)d.pairb_part, g.qblocks,
+                     has_a ? g.nbrick * g.psub : 0, (const double*)d.pairb_part, g.qblocks,
                      p.eflag ? 1 : 0, d.thermo, d.mean, p.mean_fix, p.n_global,
                      d.lista_part, g.nbrick, d.listb_part,
                      list_b_grid(p, d.nc, g), d.ctrl, a.rebuild_thr, ksum);
   hipLaunchKernelGGL((k_assemble<R>), dim3(blocks_for(d.n)), dim3(kBlock), 0,
-                     s, d, p.mean_fix, ks_on ? 1 : 0, p.split ? 1 : 0, (R)a.dt,
+                     s, d, p.mean_fix, has_a ? 1 : 0, ks_on ? 1 : 0, p.split ? 1 : 0, (R)a.dt,
                      (R)a.dt_ftm, a.md_kick, (R)a.ftm2v);
   if (a.ev[5]) hipEventRecord(a.ev[5], s);
   return hipGetLastError();
