@@ -274,7 +274,19 @@ def _iptr(a):
 def _tensor_ptr(t):
     if t is None:
         return None
-    return ctypes.c_void_p(t.data_ptr())
+    return t.data_ptr()
+
+
+def _current_stream(device):
+    """The raw HIP stream of torch's current stream on `device` (the int that
+    ``torch.cuda.current_stream(device).cuda_stream`` gives, without building
+    the Stream object on every call)."""
+    import torch
+
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:
+        return raw(device.index if device.index is not None else torch.cuda.current_device())
+    return torch.cuda.current_stream(device).cuda_stream
 
 
 class ForceContext:
@@ -349,8 +361,7 @@ class ForceContext:
                  | (RBMDX_FLAG_TIMING if timing else 0))
         nk = 0 if m_samples is None else int(num_k or m_samples.numel() // 3)
         if stream is None:
-            import torch
-            stream = torch.cuda.current_stream(px.device).cuda_stream
+            stream = _current_stream(px.device)
         self._check(self.lib.rbmdx_force(
             self._h, int(step), _tensor_ptr(px), _tensor_ptr(py), _tensor_ptr(pz),
             _tensor_ptr(fx), _tensor_ptr(fy), _tensor_ptr(fz),
@@ -366,8 +377,7 @@ class ForceContext:
                  | (RBMDX_FLAG_REBUILD if rebuild else 0))
         nk = 0 if m_samples is None else int(num_k or m_samples.numel() // 3)
         if stream is None:
-            import torch
-            stream = torch.cuda.current_stream(px.device).cuda_stream
+            stream = _current_stream(px.device)
         self._check(self.lib.rbmdx_force_noalloc(
             self._h, int(step), _tensor_ptr(px), _tensor_ptr(py), _tensor_ptr(pz),
             _tensor_ptr(fx), _tensor_ptr(fy), _tensor_ptr(fz),
@@ -393,8 +403,7 @@ class ForceContext:
         flags = ((RBMDX_FLAG_SYNC if sync else 0)
                  | (RBMDX_FLAG_VJP_NEUTRAL_Q if neutral_q else 0))
         if stream is None:
-            import torch
-            stream = torch.cuda.current_stream(px.device).cuda_stream
+            stream = _current_stream(px.device)
         args = [_tensor_ptr(t) for t in (px, py, pz, gx, gy, gz, dx, dy, dz)]
         self._check(self.lib.rbmdx_force_vjp(
             self._h, int(step), *args, ctypes.cast(_tensor_ptr(dparam),
@@ -408,8 +417,7 @@ class ForceContext:
         flags = ((RBMDX_FLAG_ENERGY if energy else 0) | (RBMDX_FLAG_SYNC if sync else 0)
                  | (RBMDX_FLAG_TIMING if timing else 0) | (RBMDX_FLAG_EAGER if eager else 0))
         if stream is None:
-            import torch
-            stream = torch.cuda.current_stream(px.device).cuda_stream
+            stream = _current_stream(px.device)
         args = [_tensor_ptr(t) for t in (px, py, pz, vx, vy, vz, fx, fy, fz)]
         self._check(self.lib.rbmdx_force_leapfrog(self._h, int(step), *args, float(dt), flags,
                                                   ctypes.c_void_p(stream)))
@@ -427,8 +435,7 @@ class ForceContext:
         step 0 into f and set the thermostat up.  Settings come from the
         configuration's execution section; masses from set_masses."""
         if stream is None:
-            import torch
-            stream = torch.cuda.current_stream(x[0].device).cuda_stream
+            stream = _current_stream(x[0].device)
         flags = RBMDX_FLAG_ENERGY if energy else 0
         self._check(self.lib.rbmdx_md_init(self._h, *self._md_args(x, v, image, f), flags,
                                            ctypes.c_void_p(stream)))
@@ -436,8 +443,7 @@ class ForceContext:
     def md_run(self, nsteps, x, v, f, image=None, energy=False, stream=None):
         """nsteps velocity-Verlet steps with the configured thermostat (in place)."""
         if stream is None:
-            import torch
-            stream = torch.cuda.current_stream(x[0].device).cuda_stream
+            stream = _current_stream(x[0].device)
         flags = RBMDX_FLAG_ENERGY if energy else 0
         self._check(self.lib.rbmdx_md_run(self._h, int(nsteps),
                                           *self._md_args(x, v, image, f), flags,
@@ -574,8 +580,7 @@ class BatchContext:
                  | (RBMDX_FLAG_EAGER if eager else 0)
                  | (RBMDX_FLAG_PER_REPLICA if per_replica else 0))
         if stream is None:
-            import torch
-            stream = torch.cuda.current_stream(px.device).cuda_stream
+            stream = _current_stream(px.device)
         self._check(self.lib.rbmdx_batch_force(
             self._h, int(step), _tensor_ptr(px), _tensor_ptr(py), _tensor_ptr(pz),
             _tensor_ptr(fx), _tensor_ptr(fy), _tensor_ptr(fz), _tensor_ptr(m_samples), flags,
@@ -591,8 +596,7 @@ class BatchContext:
                  stream=None):
         """Measure and keep the fastest launch shape; returns (shape, us)."""
         if stream is None:
-            import torch
-            stream = torch.cuda.current_stream(px.device).cuda_stream
+            stream = _current_stream(px.device)
         us = ctypes.c_double(0.0)
         self._check(self.lib.rbmdx_batch_autotune(
             self._h, int(step), _tensor_ptr(px), _tensor_ptr(py), _tensor_ptr(pz),

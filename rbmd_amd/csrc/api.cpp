@@ -1333,6 +1333,14 @@ class Engine : public EngineBase {
     d_.mean = alloc<double>(3);
     d_.ctrl = alloc<Ctrl>(1);
     d_.export_count = alloc<int>(1);
+    {
+      // a valid order before the first rebuild (k_skin gathers through it on
+      // every evaluation): the identity, with no charged copies
+      std::vector<int> iota(n_);
+      for (int i = 0; i < n_; ++i) iota[i] = i;
+      HIPCHK(hipMemcpy(d_.inv_perm, iota.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
+      HIPCHK(hipMemset(d_.c_of, 0xff, sizeof(int) * n_));
+    }
     HIPCHK(hipMemset(d_.key_count, 0, sizeof(int) * 2 * (size_t)grid_.ncell));
     HIPCHK(hipMemset(d_.ctrl, 0, sizeof(Ctrl)));
     HIPCHK(hipMemset(d_.thermo, 0, sizeof(double) * 32));

@@ -326,66 +326,63 @@ __device__ __forceinline__ void unpack_coord(int packed, int c[3]) {
 //   * xqs[qpos[i]] = (r, q): the charged atoms in a fixed order (ascending
 //     caller index) for the k-space kernels.
 // When k_bin then decides to rebuild, k_cellsort gathers again in the new
-// order.  A forced rebuild skips the gather here: the order of the last
-// rebuild may not exist (first evaluation, new system, new lists).
+// order.  The gather runs also when the step ring asks for a rebuild: the
+// order maps always hold a valid order (the identity, without charged
+// copies, until the first rebuild), and k_cellsort overwrites its result.
+// Only block 0 reads the step ring (host memory): it publishes the sampling
+// epoch and reports a requested rebuild as an infinite block maximum, so no
+// other block waits for that read.  force (an explicit rebuild of the list
+// builder alone) skips the sorted gather.
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force) {
   __shared__ float s_max[4];
-  __shared__ int s_force;
-  // this evaluation's parameters come from the step ring
-  if (threadIdx.x == 0) {
-    int f = force;
-    if (!force && d.hdr_ring) {
-      const StepHdr h = d.hdr_ring[d.ctrl->step_count % kSlots];
-      if (blockIdx.x == 0) d.ctrl->epoch = h.epoch;
-      f = h.force;
-    }
-    s_force = f;
-  }
-  __syncthreads();
   float mx = 0.f;
-  {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
-         i += gridDim.x * blockDim.x) {
-      const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
-      const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
-      const R z = wrap<R>(d.pz[i], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
-      // the charged atoms in their fixed order for the k-space kernels
-      // (independent of the sorted order, so also on a forced rebuild)
-      if (d.qpos) {
-        const int qc = d.qpos[i];
-        if (qc >= 0) d.xqs[qc] = make4<R>(x, y, z, d.charge[i]);
-      }
-      if (s_force) continue;
-      const R4T<R> r = d.ref[i];
-      // minimum image of r - r_ref as r_ref + (r - shift) - r_ref: the
-      // shift also undoes a wrap across the box edge since the rebuild
-      const R ex = x - r.x, ey = y - r.y, ez = z - r.z;
-      const R cx = fabs(ex) > b.half[0] ? copysign(b.L[0], ex) : R(0);
-      const R cy = fabs(ey) > b.half[1] ? copysign(b.L[1], ey) : R(0);
-      const R cz = fabs(ez) > b.half[2] ? copysign(b.L[2], ez) : R(0);
-      const R dx = ex - cx, dy = ey - cy, dz = ez - cz;
-      const float d2 = (float)(dx * dx + dy * dy + dz * dz);
-      mx = fmaxf(mx, d2 != d2 ? __int_as_float(0x7f800000) : d2);
-      const int s = d.inv_perm[i], c = d.c_of[i];
-      // Sorted tiles keep the frame of the last rebuild: an atom that has
-      // wrapped across the box edge since stays beside its cell (x - shift),
-      // so that stage coordinates remain continuous while lists are reused.
-      // .w: the type, plus one half for a charged atom (truncation gives the
-      // type back; the half marks the atom for the list builds)
-      const R xu = x - cx, yu = y - cy, zu = z - cz;
-      d.xs[s] = make4<R>(xu, yu, zu, (R)d.type[i] + (c >= 0 ? R(0.5) : R(0)));
-      if (c >= 0) d.xqc[c] = make4<R>(xu, yu, zu, d.charge[i]);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
+       i += gridDim.x * blockDim.x) {
+    const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
+    const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
+    const R z = wrap<R>(d.pz[i], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
+    // the charged atoms in their fixed order for the k-space kernels
+    // (independent of the sorted order, so also on a forced rebuild)
+    if (d.qpos) {
+      const int qc = d.qpos[i];
+      if (qc >= 0) d.xqs[qc] = make4<R>(x, y, z, d.charge[i]);
     }
+    if (force) continue;
+    const R4T<R> r = d.ref[i];
+    // minimum image of r - r_ref as r_ref + (r - shift) - r_ref: the
+    // shift also undoes a wrap across the box edge since the rebuild
+    const R ex = x - r.x, ey = y - r.y, ez = z - r.z;
+    const R cx = fabs(ex) > b.half[0] ? copysign(b.L[0], ex) : R(0);
+    const R cy = fabs(ey) > b.half[1] ? copysign(b.L[1], ey) : R(0);
+    const R cz = fabs(ez) > b.half[2] ? copysign(b.L[2], ez) : R(0);
+    const R dx = ex - cx, dy = ey - cy, dz = ez - cz;
+    const float d2 = (float)(dx * dx + dy * dy + dz * dz);
+    mx = fmaxf(mx, d2 != d2 ? __int_as_float(0x7f800000) : d2);
+    const int s = d.inv_perm[i], c = d.c_of[i];
+    // Sorted tiles keep the frame of the last rebuild: an atom that has
+    // wrapped across the box edge since stays beside its cell (x - shift),
+    // so that stage coordinates remain continuous while lists are reused.
+    // .w: the type, plus one half for a charged atom (truncation gives the
+    // type back; the half marks the atom for the list builds)
+    const R xu = x - cx, yu = y - cy, zu = z - cz;
+    d.xs[s] = make4<R>(xu, yu, zu, (R)d.type[i] + (c >= 0 ? R(0.5) : R(0)));
+    if (c >= 0) d.xqc[c] = make4<R>(xu, yu, zu, d.charge[i]);
   }
-  if (s_force) mx = __int_as_float(0x7f800000);
   mx = wave_max(mx);
   if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = mx;
   __syncthreads();
   if (threadIdx.x == 0) {
     float m = s_max[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, s_max[w]);
+    if (force) m = __int_as_float(0x7f800000);
+    // this evaluation's parameters come from the step ring
+    if (blockIdx.x == 0 && !force && d.hdr_ring) {
+      const StepHdr h = d.hdr_ring[d.ctrl->step_count % kSlots];
+      d.ctrl->epoch = h.epoch;
+      if (h.force) m = __int_as_float(0x7f800000);
+    }
     d.skin_part[blockIdx.x] = m;
   }
 }

@@ -1,5 +1,9 @@
 #include "sampler.hpp"
 
+#include <algorithm>
+#include <chrono>
+#include <thread>
+
 #include <cmath>
 
 namespace rbmdx {
@@ -106,52 +110,46 @@ uint32_t rbe_seed_of_call(uint32_t base_seed, int64_t call_index) {
 
 SamplePipeline::SamplePipeline(const KSampler* s, uint32_t base_seed,
                                int threads, int window)
-    : sampler_(s), base_(base_seed), window_(window) {
+    : sampler_(s), base_(base_seed), window_(window < 1 ? 1 : window),
+      slots_(new Slot[(size_t)(window < 1 ? 1 : window)]) {
+  for (int k = 0; k < window_; ++k) slots_[k].data.resize((size_t)sampler_->out_size());
   if (threads < 1) threads = 1;
   for (int t = 0; t < threads; ++t) threads_.emplace_back([this] { worker(); });
 }
 
 SamplePipeline::~SamplePipeline() {
-  {
-    std::lock_guard<std::mutex> g(mu_);
-    stop_ = true;
-  }
-  cv_work_.notify_all();
+  stop_.store(true);
   for (auto& t : threads_) t.join();
 }
 
 void SamplePipeline::worker() {
-  std::vector<int32_t> buf((size_t)sampler_->out_size());
   for (;;) {
     int64_t call;
     uint32_t seed;
     {
-      std::unique_lock<std::mutex> lk(mu_);
-      cv_work_.wait(lk, [this] {
-        return stop_ || next_seeded_ < consumer_ + window_;
-      });
-      if (stop_) return;
+      std::lock_guard<std::mutex> g(mu_);
+      if (stop_.load()) return;
       call = next_seeded_++;
       seed = (uint32_t)base_();
     }
-    sampler_->draw(seed, buf.data());
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      ready_[call] = buf;
+    // the slot is free once the consumer has taken call - window
+    while (call >= consumer_.load(std::memory_order_acquire) + window_) {
+      if (stop_.load()) return;
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
-    cv_ready_.notify_all();
+    Slot& sl = slots_[(size_t)(call % window_)];
+    sampler_->draw(seed, sl.data.data());
+    sl.tag.store(call, std::memory_order_release);
   }
 }
 
 void SamplePipeline::take(int64_t call, int32_t* m_out) {
-  std::unique_lock<std::mutex> lk(mu_);
-  cv_ready_.wait(lk, [&] { return ready_.count(call) > 0; });
-  auto it = ready_.find(call);
-  std::copy(it->second.begin(), it->second.end(), m_out);
-  ready_.erase(it);
-  consumer_ = call + 1;
-  lk.unlock();
-  cv_work_.notify_all();
+  Slot& sl = slots_[(size_t)(call % window_)];
+  for (int spin = 0; sl.tag.load(std::memory_order_acquire) != call; ++spin)
+    if (spin > 64) std::this_thread::yield();
+  std::copy(sl.data.begin(), sl.data.end(), m_out);
+  sl.tag.store(-1, std::memory_order_relaxed);
+  consumer_.store(call + 1, std::memory_order_release);
 }
 
 }  // namespace rbmdx

@@ -15,9 +15,8 @@
 // draws the samples of upcoming evaluations ahead of the GPU.
 #pragma once
 #include <atomic>
-#include <condition_variable>
 #include <cstdint>
-#include <map>
+#include <memory>
 #include <mutex>
 #include <random>
 #include <thread>
@@ -62,26 +61,36 @@ class RbeSampler : public KSampler {
 // Seeds of successive evaluations: outputs of std::mt19937(base_seed).
 uint32_t rbe_seed_of_call(uint32_t base_seed, int64_t call_index);
 
-// Draws samples for upcoming evaluations on worker threads.
+// Draws samples for upcoming evaluations on worker threads.  A ring of
+// `window` slots holds them: the worker that draws evaluation `call` (seeds
+// are drawn in call order under the workers' lock) fills slot call % window
+// once the consumer has taken call - window, then publishes the call index
+// in the slot's tag.  The consumer's take is lock-free and makes no system
+// call when the sample is ready, which it normally is: the workers run up to
+// `window` evaluations ahead and poll while the ring is full.
 class SamplePipeline {
  public:
   SamplePipeline(const KSampler* s, uint32_t base_seed, int threads,
                  int window);
   ~SamplePipeline();
-  // Blocks until the sample of evaluation `call` is ready, copies it out.
+  // Waits until the sample of evaluation `call` is ready and copies it out
+  // (calls are taken in order 0, 1, 2, ...).
   void take(int64_t call, int32_t* m_out);
 
  private:
+  struct Slot {
+    std::atomic<int64_t> tag{-1};  // the call whose sample the slot holds
+    std::vector<int32_t> data;
+  };
   void worker();
   const KSampler* sampler_;
   std::mt19937 base_;
-  int64_t next_seeded_ = 0;     // next call index whose seed is drawn
-  int64_t consumer_ = 0;        // next call the consumer will take
+  std::mutex mu_;                  // call claims and seed draws (workers)
+  int64_t next_seeded_ = 0;        // next call index whose seed is drawn
+  std::atomic<int64_t> consumer_{0};  // next call the consumer will take
   int window_;
-  bool stop_ = false;
-  std::map<int64_t, std::vector<int32_t>> ready_;
-  std::mutex mu_;
-  std::condition_variable cv_work_, cv_ready_;
+  std::atomic<bool> stop_{false};
+  std::unique_ptr<Slot[]> slots_;
   std::vector<std::thread> threads_;
 };
 
