@@ -1,29 +1,31 @@
 // HIP kernels of the RBMD force path for gfx950 (CDNA4, wave64).
 //
-// Per force evaluation (no host synchronisation; k-space on a second stream
-// beside the pair kernel):
+// Per force evaluation, one stream, no host synchronisation:
 //   k_skin        rebuild decision, part 1 (max displacement since the last
 //                 rebuild, per block) and the gather of caller-order
 //                 positions into the cell-sorted tiles
-//   k_bin         folds the block maxima into the decision; then, and in
-//                 k_scan, k_scatter, k_cellsort, k_lists only on a rebuild
-//                 (early exit otherwise): cell keys, counting sort with a
-//                 deterministic in-cell order (wave bitonic) and the gather
-//                 in the new order, brick-local all-atom candidate lists
-//                 built from LDS-staged neighbourhoods beside the
-//                 charged-pair lists
-//   k_pair        one workgroup per brick: the brick's 4x4x4-cell
-//                 neighbourhood is staged in LDS; LJ pairs gather from LDS
-//                 through 16-bit brick-local lists with the RBL
-//                 classification applied on the fly; charged atoms then add
-//                 LJ + Coulomb over their charged-pair lists, 16 lanes per
-//                 atom; FP64 per-brick partial sums
-//   k_reduce      one block: the per-brick partials in a fixed order (mean
-//                 pair force, energies, virial; list statistics)
-//   k_sk          S(k) partial sums over the charged atoms (atom chunks x k
-//                 tiles); k_sk_fin reduces them in FP64 to rho_k, the
-//                 k-space energy and virial and the per-k force coefficients
-//   k_rbe_force   per charged atom, k split over the waves of a block
+//   k_bin_sk      folds the block maxima into the decision; on a rebuild the
+//                 cell keys.  Beside them, the S(k) partials (atom chunks x
+//                 k tiles) of sampled k-space
+//   k_scan_fin    on a rebuild the scan of the cell counts; beside it, the
+//                 S(k) reduction per k tile (rho_k, per-k energy terms and
+//                 force coefficients)
+//   k_scatter_rbe on a rebuild the scatter into cell order; beside it, the
+//                 RBE / RBSOG force per charged atom
+//   k_cellsort,   on a rebuild only (early exit otherwise): deterministic
+//   k_lists,      in-cell order and the gather in it; brick-local all-atom
+//   k_lists_dense candidate lists from LDS-staged neighbourhoods and the
+//                 charged-pair lists (per cell in mostly charged systems)
+//   k_pair        one workgroup per brick: the brick's neighbourhood is
+//                 staged in LDS; LJ pairs gather from LDS through 16-bit
+//                 brick-local lists with the RBL classification applied on
+//                 the fly; in mostly neutral systems the charged atoms then
+//                 add LJ + Coulomb over their charged-pair lists
+//   k_pair_b      mostly charged systems: the charged pairs, one wave per
+//                 atom (k_pair is not launched when list A is empty)
+//   k_reduce      one block: the per-block partials in a fixed order (mean
+//                 pair force, energies, virial, k-space sums; list
+//                 statistics)
 //   k_assemble    F = F_pair - <F_pair> + F_kspace in caller order (and the
 //                 leapfrog update when asked)
 // Cross-block reductions happen at kernel boundaries only: no same-address
