@@ -577,6 +577,8 @@ class Batch {
   }
 
   void drop_graphs() {
+    // an exec may still run from the last call: wait for it before destroying
+    if (!graphs_.empty() && last_stream_set_) HIPCHK(hipStreamSynchronize(last_stream_));
     for (auto& kv : graphs_) hipGraphExecDestroy(kv.second);
     graphs_.clear();
   }

@@ -194,7 +194,7 @@ __device__ __forceinline__ void batch_lj(const PairConst<float>& pc, const Box<f
   if (!(d2 < pc.rclj2) || !(d2 < pc.rc2)) return;
   const int tt = 4 * (ti * pc.ntypes + tj);
   const float m24 = s_lj[tt], sig2 = s_lj[tt + 1], eps = s_lj[tt + 2];
-  const float inv = 1.f / d2;
+  const float inv = pair_rcp(d2);
   const float s2 = sig2 * inv;
   const float s6 = s2 * s2 * s2;
   // CVFF: the owner's special-bond weight scales the LJ term
