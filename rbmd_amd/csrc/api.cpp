@@ -568,7 +568,7 @@ class Engine : public EngineBase {
     };
     take(dd.pair_part, (size_t)11 * grid_.nbrick * grid_.psub);
     if (grid_.qblocks) take(dd.pairb_part, (size_t)11 * grid_.qblocks);
-    take(dd.skin_part, kSkinBlocksMax);
+    take(dd.skin_part, kSkinBlocksMax + 1);
     take(dd.lista_part, grid_.nbrick);
     take(dd.listb_part, std::max(1, list_b_count(grid_, dd.nc)));
     take(dd.mean, 3);
@@ -1326,7 +1326,7 @@ class Engine : public EngineBase {
     d_.pair_part_blocks = grid_.nbrick * grid_.psub;
     d_.pair_part = alloc<double>((size_t)11 * grid_.nbrick * grid_.psub);
     d_.pairb_part = grid_.qblocks ? alloc<double>((size_t)11 * grid_.qblocks) : nullptr;
-    d_.skin_part = alloc<float>(kSkinBlocksMax);
+    d_.skin_part = alloc<float>(kSkinBlocksMax + 1);  // + the control block
     d_.lista_part = alloc<int4>(grid_.nbrick);
     d_.listb_part = alloc<int2>(std::max(1, list_b_count(grid_, nc)));
     d_.thermo = alloc<double>(32);

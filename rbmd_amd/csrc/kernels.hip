@@ -331,17 +331,49 @@ __device__ __forceinline__ void unpack_coord(int packed, int c[3]) {
 // order.  The gather runs also when the step ring asks for a rebuild: the
 // order maps always hold a valid order (the identity, without charged
 // copies, until the first rebuild), and k_cellsort overwrites its result.
-// Only block 0 reads the step ring (host memory): it publishes the sampling
-// epoch and reports a requested rebuild as an infinite block maximum, so no
-// other block waits for that read.  force (an explicit rebuild of the list
-// builder alone) skips the sorted gather.
+// With ctl, the last block is the control block and reads no atoms: it
+// takes the step ring's header (host memory), publishes the sampling epoch,
+// reports a requested rebuild as an infinite block maximum and copies the
+// step's k-sample (kcopy vectors, then with_src their virial-chain sources)
+// from the host-mapped ring slot to HBM for the k-space kernels, all beside
+// the gather.  force (an explicit rebuild of the list builder alone) skips
+// the sorted gather.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force) {
+__global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force, int ctl,
+                                              int kcopy, int with_src) {
   __shared__ float s_max[4];
+  const int nab = (int)gridDim.x - ctl;  // blocks over the atoms
+  if ((int)blockIdx.x == nab) {
+    const int slot = (int)(d.ctrl->step_count % kSlots);
+    // every load of the slot is issued before the stores
+    const int* ms = d.stage_ring + (size_t)slot * d.stage_stride;
+    const int n3 = 3 * kcopy, nt = n3 + (with_src ? kcopy : 0);
+    for (int q0 = 0; q0 < nt; q0 += 8 * (int)blockDim.x) {
+      int v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = q0 + u * (int)blockDim.x + (int)threadIdx.x;
+        v[u] = q < nt ? ms[q] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = q0 + u * (int)blockDim.x + (int)threadIdx.x;
+        if (q < n3)
+          d.m_dev[q] = v[u];
+        else if (q < nt)
+          d.src_dev[q - n3] = v[u];
+      }
+    }
+    if (threadIdx.x == 0) {
+      const StepHdr h = d.hdr_ring[slot];
+      d.ctrl->epoch = h.epoch;
+      d.skin_part[blockIdx.x] = h.force ? __int_as_float(0x7f800000) : 0.f;
+    }
+    return;
+  }
   float mx = 0.f;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n;
-       i += gridDim.x * blockDim.x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n; i += nab * blockDim.x) {
     const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
     const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
     const R z = wrap<R>(d.pz[i], b.lo[2], b.hi[2], b.L[2], b.invL[2]);
@@ -378,14 +410,7 @@ __global__ __launch_bounds__(256) void k_skin(Dev<R> d, Box<R> b, int force) {
   if (threadIdx.x == 0) {
     float m = s_max[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, s_max[w]);
-    if (force) m = __int_as_float(0x7f800000);
-    // this evaluation's parameters come from the step ring
-    if (blockIdx.x == 0 && !force && d.hdr_ring) {
-      const StepHdr h = d.hdr_ring[d.ctrl->step_count % kSlots];
-      d.ctrl->epoch = h.epoch;
-      if (h.force) m = __int_as_float(0x7f800000);
-    }
-    d.skin_part[blockIdx.x] = m;
+    d.skin_part[blockIdx.x] = force ? __int_as_float(0x7f800000) : m;
   }
 }
 
@@ -1943,28 +1968,14 @@ __global__ __launch_bounds__(1024) void k_sk_exact(Dev<R> d, Box<R> b, int nchun
 // LDS as fractional coordinates and charge, each wave takes 64 of them with
 // broadcast reads, and lane l of every wave accumulates k = tile * kSkK + l
 // in FP32 over its atoms (two alternating accumulators); the four waves'
-// sums are added in FP64 in wave order into sk_part[chunk][k].  m_src holds
-// the K integer k-vectors, in device memory or, for the sampled stream, in
-// host-mapped staging memory; in that case m_copy (device) is given and the
-// chunk-0 block of every k tile copies its tile there for the reduction and
-// later calls (likewise src_src -> src_copy, the RBSOG virial-chain sources,
-// when given).
+// sums are added in FP64 in wave order into sk_part[chunk][k].  The K
+// integer k-vectors are m_arg (device memory) or, for the sampled stream,
+// the step's ring slot, which k_skin has copied to m_dev.
 template <typename R>
 __device__ __forceinline__ void sk2_block(const Dev<R>& d, const Box<R>& b,
                                           const int* __restrict__ m_arg, int from_ring,
-                                          int with_src, int K, int chunk, int tile) {
-  const int* m_src = m_arg;
-  int* m_copy = nullptr;
-  const int* src_src = nullptr;
-  int* src_copy = nullptr;
-  if (from_ring) {
-    m_src = d.stage_ring + (size_t)(d.ctrl->step_count % kSlots) * d.stage_stride;
-    m_copy = d.m_dev;
-    if (with_src) {
-      src_src = m_src + 3 * (size_t)K;
-      src_copy = d.src_dev;
-    }
-  }
+                                          int K, int chunk, int tile) {
+  const int* m_src = from_ring ? d.m_dev : m_arg;
   __shared__ float4 s_a[kSkAtoms];
   __shared__ double s_p[kSkAtoms / 64][2][kSkK];
   const int t = (int)threadIdx.x, lane = t & 63;
@@ -1979,10 +1990,6 @@ __device__ __forceinline__ void sk2_block(const Dev<R>& d, const Box<R>& b,
     s_a[t] = make_float4(sx, sy, sz, (float)pp.w);
   } else {
     s_a[t] = make_float4(0.f, 0.f, 0.f, 0.f);  // charge 0 adds nothing
-  }
-  if (m_copy && chunk == 0) {
-    for (int q = t; q < 3 * kn; q += blockDim.x) m_copy[3 * k0 + q] = m_src[3 * k0 + q];
-    if (src_copy && t < kn) src_copy[k0 + t] = src_src[k0 + t];
   }
   const int kl = k0 + min(lane, kn - 1);
   const float mx = (float)m_src[3 * kl], my = (float)m_src[3 * kl + 1],
@@ -2155,8 +2162,7 @@ __device__ __forceinline__ void rbe2_block(const Dev<R>& d, const Box<R>& b, int
 struct KsArgs {
   int K = 0;                    // k-vectors of the evaluation (0: none)
   const int* m_arg = nullptr;   // device k-vectors for the S(k) blocks, or
-  int from_ring = 0;            // the step's ring slot (copied to m_dev)
-  int with_src = 0;             // RBSOG virial-chain sources in the slot
+  int from_ring = 0;            // the step's ring slot (k_skin copies it to m_dev)
   const int* m_fin = nullptr;   // k-vectors as the reduction reads them
   const int* src = nullptr;     // RBSOG virial-chain sources (device)
   int vmode = 0;                // virial form (0 RBE / EWALD, 1 RBSOG)
@@ -2175,8 +2181,7 @@ __global__ __launch_bounds__(256) void k_bin_sk(Dev<R> d, Box<R> b, Grid g, int 
     return;
   }
   const int q = (int)blockIdx.x - nbin;
-  sk2_block<R>(d, b, ks.m_arg, ks.from_ring, ks.with_src, ks.K, q % ks.nchunks,
-               q / ks.nchunks);
+  sk2_block<R>(d, b, ks.m_arg, ks.from_ring, ks.K, q % ks.nchunks, q / ks.nchunks);
 }
 
 // The cell scans (block 0) and the FP64 S(k) reduction (blocks 1 .. ntiles,
@@ -2402,7 +2407,7 @@ hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, int nskin,
 
 template <typename R>
 hipError_t launch_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_skin<R>), dim3(1), dim3(kBlock), 0, s, d, a.box, 1);
+  hipLaunchKernelGGL((k_skin<R>), dim3(1), dim3(kBlock), 0, s, d, a.box, 1, 0, 0, 0);
   hipError_t e = build_part<R>(d, a, 1, s);
   if (e != hipSuccess) return e;
   // list statistics only (no pair partials)
@@ -2420,18 +2425,21 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   // (grid-stride beyond; their maxima are folded by every block of k_bin)
   const int sb = blocks_for(d.n, kBlock, kSkinBlocksMax);
   if (a.ev[0]) hipEventRecord(a.ev[0], s);
-  // (a forced rebuild comes through the step ring, like the epoch)
-  hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(kBlock), 0, s, d, a.box, 0);
-  // k-space blocks beside the rebuild kernels' blocks (one stream, no fork)
   const int K = p.kspace ? p.num_k : 0;
   const bool ks_on = K > 0 && d.nc > 0;
+  // (a forced rebuild comes through the step ring, like the epoch, and so
+  // does the sampled k-sample, which k_skin copies to HBM)
+  const int kcopy = (ks_on && a.m_from_ring) ? K : 0;
+  const int ctl = d.hdr_ring ? 1 : 0;
+  hipLaunchKernelGGL((k_skin<R>), dim3(sb + ctl), dim3(kBlock), 0, s, d, a.box, 0, ctl,
+                     kcopy, p.kspace == 3 ? 1 : 0);
+  // k-space blocks beside the rebuild kernels' blocks (one stream, no fork)
   KsArgs ks;
   if (ks_on) {
     const bool sog = p.kspace == 3;
     ks.K = K;
     ks.from_ring = a.m_from_ring;
     ks.m_arg = a.m_use;
-    ks.with_src = sog ? 1 : 0;
     ks.m_fin = a.m_from_ring ? (const int*)d.m_dev : a.m_use;
     ks.src = sog ? (const int*)d.src_dev : nullptr;
     ks.vmode = sog ? 1 : 0;
@@ -2441,7 +2449,7 @@ hipError_t launch_force(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
     ks.p = p;
   }
   if (a.ev[3]) hipEventRecord(a.ev[3], s);
-  hipError_t e = build_part<R>(d, a, sb, s, ks);
+  hipError_t e = build_part<R>(d, a, sb + ctl, s, ks);
   if (e != hipSuccess) return e;
   if (ks_on && p.eflag && d.kex > 0) {
     // energy_rbe_flag: the exact Ewald energy over the kmax grid
@@ -3345,8 +3353,9 @@ __global__ __launch_bounds__(256) void k_vjp_params(VjpDev<R> v, int nt, int K) 
 template <typename R>
 hipError_t launch_skin_build(const Dev<R>& d, const StepArgs<R>& a, hipStream_t s) {
   const int sb = blocks_for(d.n, kBlock, kSkinBlocksMax);
-  hipLaunchKernelGGL((k_skin<R>), dim3(sb), dim3(kBlock), 0, s, d, a.box, 0);
-  return build_part<R>(d, a, sb, s);
+  const int ctl = d.hdr_ring ? 1 : 0;
+  hipLaunchKernelGGL((k_skin<R>), dim3(sb + ctl), dim3(kBlock), 0, s, d, a.box, 0, ctl, 0, 0);
+  return build_part<R>(d, a, sb + ctl, s);
 }
 
 template <typename R>
