@@ -1309,6 +1309,7 @@ class Engine : public EngineBase {
       HIPCHK(hipMemcpy(d_.qpos, qp.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
     }
     d_.cnt_a = alloc<int>(n_);
+    d_.order_a = alloc<int>(n_);
     d_.cnt_b = alloc<int>(nc);
     d_.inv_mass = alloc<R>(ntypes_);
     md_mass_ = alloc<double>(ntypes_);
@@ -1339,6 +1340,7 @@ class Engine : public EngineBase {
       std::vector<int> iota(n_);
       for (int i = 0; i < n_; ++i) iota[i] = i;
       HIPCHK(hipMemcpy(d_.inv_perm, iota.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_.order_a, iota.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
       HIPCHK(hipMemset(d_.c_of, 0xff, sizeof(int) * n_));
     }
     HIPCHK(hipMemset(d_.key_count, 0, sizeof(int) * 2 * (size_t)grid_.ncell));

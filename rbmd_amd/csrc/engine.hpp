@@ -11,6 +11,7 @@ namespace rbmdx {
 
 constexpr int kBlock = 256;      // threads per block for per-atom kernels
 constexpr int kSkinBlocksMax = 1024;  // skin-check grid (grid-stride beyond)
+constexpr int kOrderMax = 512;        // bricks of more atoms keep the cell order
 constexpr int kMaxTypes = 32;    // LJ type table held in LDS
 constexpr int kMaxSpecial = 64;  // CVFF special partners per atom
 constexpr int kSkTile = 8;       // k-vectors per S(k) block
@@ -190,6 +191,7 @@ struct Dev {
   int* brick_tab = nullptr;    // [nbrick * kBrickTabInts] slot tables (list build)
   uint16_t* list_a = nullptr;  // [cap_a * n] groups of 8 (list_a_index), stage byte offsets
   int* cnt_a = nullptr;
+  int* order_a = nullptr;  // [n] per brick: its atoms by list-A length, longest first
   int* list_b = nullptr;       // [nc * cap_b] row-major, charged compact index
   int* cnt_b = nullptr;
   // list B of a mostly charged system: the type of each entry's atom

@@ -788,6 +788,45 @@ __device__ __forceinline__ void list_a_brick(const Dev<R>& d, const Box<R>& b,
     }
     d.lista_part[brick] = make_int4(m, su, total, staged ? 0 : 1);
   }
+  // The pair kernel's order of the brick's atoms: by list length, longest
+  // first (ties by position), so that each of its waves takes lists of about
+  // equal length and no wave idles most of its lanes behind one long list.
+  const int i0 = d.key_start[2 * h.cell0], na = d.key_start[2 * h.cell1] - i0;
+  // the keys reuse the stage's LDS (the build is done with it: the block
+  // statistics above passed a barrier)
+  unsigned* s_key = reinterpret_cast<unsigned*>(stage);
+  static_assert(sizeof(stage) >= kOrderMax * sizeof(unsigned), "order keys");
+  if (na <= kOrderMax) {
+    __threadfence_block();  // this block's counts are read back below
+    __syncthreads();
+    // keys (longer list first, then position) in a bitonic sort over the next
+    // power of two, padded with the largest key
+    int np = 1;
+    while (np < na) np <<= 1;
+    for (int k = threadIdx.x; k < np; k += blockDim.x)
+      s_key[k] = k < na ? ((unsigned)(0xffff - min(d.cnt_a[i0 + k], 0xffff)) << 16) |
+                              (unsigned)k
+                        : 0xffffffffu;
+    __syncthreads();
+    for (int kk = 2; kk <= np; kk <<= 1)
+      for (int j = kk >> 1; j > 0; j >>= 1) {
+        for (int t = threadIdx.x; t < np; t += blockDim.x) {
+          const int u = t ^ j;
+          if (u > t) {
+            const unsigned x = s_key[t], y = s_key[u];
+            if ((x > y) == ((t & kk) == 0)) {
+              s_key[t] = y;
+              s_key[u] = x;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    for (int k = threadIdx.x; k < na; k += blockDim.x)
+      d.order_a[i0 + k] = i0 + (int)(s_key[k] & 0xffffu);
+  } else {
+    for (int k = threadIdx.x; k < na; k += blockDim.x) d.order_a[i0 + k] = i0 + k;
+  }
 }
 
 // Charged-pair candidate list: kChargedLanes lanes per charged atom scan the
@@ -1378,8 +1417,10 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
       stage_atoms<R, true>(g, b, h, t, d.xs, stage, SPECIAL ? d.gid_s : nullptr,
                            SPECIAL ? sgid : nullptr);
     const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
-    // phase 1: LJ over list A
-    for (int i = i0 + (int)threadIdx.x; i < i1 && (int)threadIdx.x < TA; i += TA) {
+    // phase 1: LJ over list A, the atoms in the order of the last rebuild
+    // (longest list first)
+    for (int k = (int)threadIdx.x; k < i1 - i0 && (int)threadIdx.x < TA; k += TA) {
+      const int i = d.order_a[i0 + k];
       Acc<R> a;
       const R4T<R> xi = d.xs[i];
       const int ti = (int)xi.w;
