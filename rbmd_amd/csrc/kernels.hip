@@ -623,8 +623,8 @@ __device__ __forceinline__ bool charged_w(R w) {
 // All-atom candidate list of one brick: every j != i within r2 among the 27
 // cells around i, except charged-charged pairs when the work is split (those
 // go to list B).  Entries are 16-bit positions in the brick's staged
-// neighbourhood in the interleaved layout of list_a_index; each lane packs 8
-// entries in registers and writes them with one 16-byte store.  count_only
+// neighbourhood in the interleaved layout of list_a_index; each lane stores
+// its atom's entries as it accepts them (16-bit stores).  count_only
 // (cap 0): no writes.  Neighbourhood positions come from the LDS stage when
 // STAGED, else from the sorted arrays.
 template <typename R, bool STAGED, bool MI>
@@ -645,23 +645,13 @@ __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
     const R qf = qi ? R(1) : R(0);
     const R4T<R> xi = d.xs[i];
     int count = 0;
-    unsigned long long lo = 0, hi = 0;  // entries count&~7 .. count&~7 + 7
-    uint4* out = reinterpret_cast<uint4*>(d.list_a) + i;
+    // entry e of atom i: 16-bit slot e & 7 of group e >> 3 (groups of all
+    // atoms interleaved, n8 entries apart), stored as it is accepted
+    uint16_t* row = d.list_a + (size_t)i * 8;
+    const size_t n8 = (size_t)d.n * 8;
     constexpr int sh = StageShift<R>::value;
     auto accept = [&](int l) {
-      const int s8 = count & 7;
-      const unsigned long long v = (unsigned long long)(l << sh);
-      if (s8 < 4)
-        lo |= v << (16 * s8);
-      else
-        hi |= v << (16 * (s8 - 4));
-      if (s8 == 7) {
-        if (count < lcap)
-          out[(size_t)(count >> 3) * d.n] =
-              make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi,
-                         (unsigned)(hi >> 32));
-        lo = hi = 0;
-      }
+      if (count < lcap) row[(size_t)(count >> 3) * n8 + (count & 7)] = (uint16_t)(l << sh);
       ++count;
     };
     // Staged neighbourhoods scan a stencil row (its x cells are consecutive
@@ -693,9 +683,9 @@ __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
                                               xj[u].z, dx, dy, dz);
                   // charged-charged pairs belong to list B: the stage's .w
                   // (1e30 for a charged j) pushes them past the cutoff
-                  const bool keep = lu < ce && fma_r(qf, xj[u].w, d2) < r2 &&
-                                    lu != self;
-                  ok |= keep ? (1ull << (lu - cb)) : 0ull;
+                  const bool keep = (lu < ce) & (fma_r(qf, xj[u].w, d2) < r2) &
+                                    (lu != self);
+                  ok |= (unsigned long long)keep << (lu - cb);
                 }
               }
               while (ok) {
@@ -719,17 +709,9 @@ __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
       }
     if ((count & 7) && count <= lcap) {
       // the rest of the last group: the pair kernel's sentinel entry
-      const unsigned long long sent =
-          (unsigned long long)((StageCap<R>::value - 1) << sh);
-      for (int s8 = count & 7; s8 < 8; ++s8) {
-        if (s8 < 4)
-          lo |= sent << (16 * s8);
-        else
-          hi |= sent << (16 * (s8 - 4));
-      }
-      out[(size_t)(count >> 3) * d.n] =
-          make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi,
-                     (unsigned)(hi >> 32));
+      const uint16_t sent = (uint16_t)((StageCap<R>::value - 1) << sh);
+      uint16_t* gp = row + (size_t)(count >> 3) * n8;
+      for (int s8 = count & 7; s8 < 8; ++s8) gp[s8] = sent;
     }
     d.cnt_a[i] = (lcap == 0 && cap > 0) ? cap + 1 + count : count;
     mx = max(mx, count);
