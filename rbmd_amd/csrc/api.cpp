@@ -1293,8 +1293,10 @@ class Engine : public EngineBase {
     d_.hjc = alloc<unsigned long long>(nc);
     d_.typec = alloc<int>(nc);
     d_.xqc = alloc<R4>(nc);
-    d_.fk = alloc<R4>(nc);
-    d_.fb = alloc<R4>(nc);
+    d_.fk = alloc<R4>(n_);
+    d_.fb = alloc<R4>(n_);
+    d_.c2i = alloc<int>(nc);
+    d_.qcall = alloc<int>(nc);
     // the k-space kernels' fixed order of the charged atoms: ascending caller
     // index (zero charges stay zero, so the set is fixed for the system)
     d_.xqs = alloc<R4>(nc);
@@ -1307,6 +1309,11 @@ class Engine : public EngineBase {
           if ((R)q_[i] != R(0)) qp[i] = k++;
       if (k != nc) throw Error(RBMDX_E_STATE, "charged-atom count mismatch");
       HIPCHK(hipMemcpy(d_.qpos, qp.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
+      std::vector<int> qc(std::max(nc, 1), 0);
+      for (int i = 0; i < n_; ++i)
+        if (qp[i] >= 0) qc[qp[i]] = i;
+      if (nc > 0)
+        HIPCHK(hipMemcpy(d_.qcall, qc.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
     }
     d_.cnt_a = alloc<int>(n_);
     d_.order_a = alloc<int>(n_);
@@ -1348,7 +1355,9 @@ class Engine : public EngineBase {
     HIPCHK(hipMemset(d_.thermo, 0, sizeof(double) * 32));
     HIPCHK(hipMemset(d_.mean, 0, sizeof(double) * 3));
     HIPCHK(hipMemset(d_.ref, 0, sizeof(R4) * n_));
-    HIPCHK(hipMemset(d_.fk, 0, sizeof(R4) * std::max(nc, 1)));
+    // caller-order force rows: the neutral atoms' rows of fb and fk stay 0
+    HIPCHK(hipMemset(d_.fk, 0, sizeof(R4) * n_));
+    HIPCHK(hipMemset(d_.fb, 0, sizeof(R4) * n_));
     {
       void* hp = nullptr;
       HIPCHK(hipHostMalloc(&hp, sizeof(Ctrl), hipHostMallocMapped));

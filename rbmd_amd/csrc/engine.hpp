@@ -174,18 +174,20 @@ struct Dev {
   R4* xs = nullptr;        // [n] (x, y, z, type)
   int4* spn_s = nullptr;   // [n] spn in sorted order
   int* spid_s = nullptr;   // [n * ms] spid in sorted order
-  R4* fa = nullptr;        // [n] total pair force
+  R4* fa = nullptr;        // [n] list-A pair force, caller order
   // charged compact
   int* c2s = nullptr;      // [nc]
+  int* c2i = nullptr;      // [nc] caller index of charged atom c (rebuild order)
+  int* qcall = nullptr;    // [nc] caller index of the k-space order's atom
   int* gidc = nullptr;     // [nc]
   int* typec = nullptr;    // [nc]
   R4* xqc = nullptr;       // [nc] (x, y, z, q)
-  R4* fb = nullptr;        // [nc] charged-pair (list B) force
+  R4* fb = nullptr;        // [n] charged-pair (list B) force, caller order (0: neutral)
   // charged atoms in a fixed order (ascending caller index), independent of
   // the cell sort: the k-space kernels' input and output
   int* qpos = nullptr;     // [n] caller -> fixed charged index, -1 if neutral
   R4* xqs = nullptr;       // [nc] (x, y, z, q) in the fixed order
-  R4* fk = nullptr;        // [nc] k-space force, fixed order
+  R4* fk = nullptr;        // [n] k-space force, caller order (0: neutral)
   // lists
   int cap_a = 0, cap_b = 0;
   int* brick_tab = nullptr;    // [nbrick * kBrickTabInts] slot tables (list build)

@@ -597,6 +597,7 @@ __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
     if (s < aq) {
       c = cq + (s - a);
       d.c2s[c] = s;
+      d.c2i[c] = i;
       d.gidc[c] = gid;
       d.hjc[c] = hj;
       d.typec[c] = t;
@@ -1441,7 +1442,7 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
               }
             }
       }
-      d.fa[i] = make4<R>(a.fx, a.fy, a.fz, R(0));
+      d.fa[d.perm[i]] = make4<R>(a.fx, a.fy, a.fz, R(0));
       acc.fx += a.fx;
       acc.fy += a.fy;
       acc.fz += a.fz;
@@ -1528,7 +1529,7 @@ __attribute__((amdgpu_waves_per_eu(kPairWaves<R, EFLAG, SHELL || SPECIAL>::value
         fy += __shfl_xor(fy, o, QL);
         fz += __shfl_xor(fz, o, QL);
       }
-      if (gl == 0) d.fb[c] = make4<R>(fx, fy, fz, R(0));
+      if (gl == 0) d.fb[d.c2i[c]] = make4<R>(fx, fy, fz, R(0));
       acc.fx += a.fx;
       acc.fy += a.fy;
       acc.fz += a.fz;
@@ -1707,7 +1708,7 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_pair_b(Dev<R> d, Box<R> b, Grid g
       fy += __shfl_xor(fy, o, 64);
       fz += __shfl_xor(fz, o, 64);
     }
-    if (lane == 0) d.fb[c] = make4<R>(fx, fy, fz, R(0));
+    if (lane == 0) d.fb[d.c2i[c]] = make4<R>(fx, fy, fz, R(0));
     acc.fx += a.fx;
     acc.fy += a.fy;
     acc.fz += a.fz;
@@ -2172,7 +2173,7 @@ __device__ __forceinline__ void rbe2_block(const Dev<R>& d, const Box<R>& b, int
       ay += s_f[w][1][lane];
       az += s_f[w][2][lane];
     }
-    d.fk[c] = make4<R>((R)(q * ax), (R)(q * ay), (R)(q * az), R(0));
+    d.fk[d.qcall[c]] = make4<R>((R)(q * ax), (R)(q * ay), (R)(q * az), R(0));
   }
 }
 
@@ -2264,30 +2265,25 @@ __global__ __launch_bounds__(256) void k_assemble(Dev<R> d, int mean_fix, int ha
   }
   double ke = 0.0;
   if (i < d.n) {
-    const int s = d.inv_perm[i];
-    R4T<R> f = has_a ? d.fa[s] : make4<R>(R(0), R(0), R(0), R(0));
+    // the force kernels write in caller order (zero rows of fb and fk for
+    // the neutral atoms): coalesced reads, no gathers through the maps
+    R4T<R> f = has_a ? d.fa[i] : make4<R>(R(0), R(0), R(0), R(0));
     if (mean_fix) {
       f.x -= (R)d.mean[0];
       f.y -= (R)d.mean[1];
       f.z -= (R)d.mean[2];
     }
     if (has_b) {
-      const int c = d.c_of[i];
-      if (c >= 0) {
-        const R4T<R> g = d.fb[c];
-        f.x += g.x;
-        f.y += g.y;
-        f.z += g.z;
-      }
+      const R4T<R> g = d.fb[i];
+      f.x += g.x;
+      f.y += g.y;
+      f.z += g.z;
     }
     if (has_k) {
-      const int qc = d.qpos[i];
-      if (qc >= 0) {
-        const R4T<R> g = d.fk[qc];
-        f.x += g.x;
-        f.y += g.y;
-        f.z += g.z;
-      }
+      const R4T<R> g = d.fk[i];
+      f.x += g.x;
+      f.y += g.y;
+      f.z += g.z;
     }
     d.fx[i] = f.x;
     d.fy[i] = f.y;
