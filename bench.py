@@ -309,6 +309,23 @@ def bench_water(args):
     t_cpu = time.perf_counter() - tc
     ferr = rel_rms(got, ref + fk)
     th = ctx.thermo()
+    # the dominant kernel, k_pair_b (k_pair is not launched when every atom is
+    # charged), timed by HIP events around the pair phase over a repeat of the
+    # timed evaluations.  Algorithmic bytes per launch: 5 B per list-B entry
+    # (4 B index + 1 B type) + 36 B per atom (16 B position/charge, 4 B type,
+    # 16 B force); the kernel is bound by VALU issue, not by these bytes.
+    ctx.kernel_times()
+    for k in range(warm + steps, warm + 2 * steps):
+        ctx.force(k, px, py, pz, *f, rebuild=(k % 10 == 0), timing=True)
+    torch.cuda.synchronize()
+    pair_us = ctx.kernel_times()["pair"]
+    algo = 5 * int(th["pairs_b"]) + 36 * n
+    ach = algo / (pair_us * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_pair_b", "kernel_us": round(pair_us, 2),
+                "algorithmic_bytes": algo,
+                "limiter": "VALU issue (profiles/valu_counters_water73k.md)"}
     line = {
         "metric": "atom-steps/s, CVFF SPC/E water, RBL + RBSOG, 72,981 atoms",
         "value": round(n * steps / elapsed, 1), "unit": "atom-steps/s", "n_gpus": 1,
@@ -320,6 +337,7 @@ def bench_water(args):
                                "P=512, positions fixed, rebuild every 10th evaluation",
                    "atoms": n, "P": P, "rbl_frequency": th["rbl_frequency"],
                    "pairs_a": th["pairs_a"], "pairs_b": th["pairs_b"]},
+        "roofline": roofline,
         "cpu_baseline": {"value": round(n / t_cpu, 1), "unit": "atom-steps/s", "cores": 1,
                          "kind": "port",
                          "sample": "one evaluation (pair forces with the cell search, SOG "
