@@ -450,58 +450,60 @@ __device__ __forceinline__ void bin_block(const Dev<R>& d, const Box<R>& b,
 }
 
 // Exclusive scans in one block of 1024 threads: key_start over all 2 ncell
-// keys and cstart over the charged key (2 cell) of every cell.  Thread t owns
-// the consecutive cells [t per, (t + 1) per): it reads their two counts as
-// one int2, 8 cells per batch of loads (clamped addresses, so every load of
-// a batch issues at once), and one block scan of the thread totals of both
-// scans follows; the second read writes the starts and clears key_count for
-// the next rebuild.
+// keys and cstart over the charged key (2 cell) of every cell.  The cells go
+// in tiles of kScanPer * blockDim: the tile's counts (one int2 per cell) are
+// loaded coalesced into LDS, each thread scans its kScanPer consecutive cells
+// after one block scan of the thread totals (plus the earlier tiles' carry),
+// the starts go back through LDS and out coalesced, and key_count is cleared
+// for the next rebuild.  buf: LDS of kScanPer * blockDim * 12 bytes.
+constexpr int kScanPer = 4;
 __device__ __forceinline__ void scan_block(int* key_count, int* key_start,
                                            int* cstart, int ncell, Ctrl* ctrl,
-                                           unsigned thr) {
+                                           unsigned thr, char* buf) {
   if (!rebuild_now(ctrl, thr)) return;
   __shared__ int smem[33];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, nt = blockDim.x, tile = nt * kScanPer;
+  int2* s_io = reinterpret_cast<int2*>(buf);                 // counts in, starts out
+  int* s_cs = reinterpret_cast<int*>(buf + sizeof(int2) * tile);  // cstart out
   if (t == 0) ctrl->rebuild_count += 1;
   int2* kc = reinterpret_cast<int2*>(key_count);
-  const int per = (ncell + (int)blockDim.x - 1) / (int)blockDim.x;
-  const int c0 = min(ncell, t * per), c1 = min(ncell, c0 + per);
-  int sa = 0, sq = 0;
-  for (int c = c0; c < c1; c += 8) {
-    int2 v[8];
+  int2* ks2 = reinterpret_cast<int2*>(key_start);
+  int carry_a = 0, carry_q = 0;
+  for (int base = 0; base < ncell; base += tile) {
+    const int n = min(tile, ncell - base);
+    for (int k = t; k < tile; k += nt) s_io[k] = k < n ? kc[base + k] : make_int2(0, 0);
+    __syncthreads();
+    int2 v[kScanPer];
+    int sa = 0, sq = 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = kc[min(c + u, c1 - 1)];
+    for (int u = 0; u < kScanPer; ++u) {
+      v[u] = s_io[kScanPer * t + u];
+      sa += v[u].x + v[u].y;
+      sq += v[u].x;
+    }
+    int tot_a, tot_q;
+    int ra = block_excl_scan(sa, tot_a, smem) + carry_a;
+    int rq = block_excl_scan(sq, tot_q, smem) + carry_q;
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (c + u < c1) {
-        sa += v[u].x + v[u].y;
-        sq += v[u].x;
-      }
-  }
-  int tot_a, tot_q;
-  int ra = block_excl_scan(sa, tot_a, smem);
-  int rq = block_excl_scan(sq, tot_q, smem);
-  // the same batches again, loads first (the stores below may not be
-  // reordered before loads of a later cell)
-  for (int c = c0; c < c1; c += 8) {
-    int2 v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = kc[min(c + u, c1 - 1)];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (c + u < c1) {
-        key_start[2 * (c + u)] = ra;
-        ra += v[u].x;
-        key_start[2 * (c + u) + 1] = ra;
-        ra += v[u].y;
-        cstart[c + u] = rq;
-        rq += v[u].x;
-        kc[c + u] = make_int2(0, 0);
-      }
+    for (int u = 0; u < kScanPer; ++u) {
+      s_io[kScanPer * t + u] = make_int2(ra, ra + v[u].x);
+      ra += v[u].x + v[u].y;
+      s_cs[kScanPer * t + u] = rq;
+      rq += v[u].x;
+    }
+    __syncthreads();
+    for (int k = t; k < n; k += nt) {
+      ks2[base + k] = s_io[k];
+      cstart[base + k] = s_cs[k];
+      kc[base + k] = make_int2(0, 0);
+    }
+    carry_a += tot_a;
+    carry_q += tot_q;
+    __syncthreads();  // the tile's LDS is read before the next tile's loads
   }
   if (t == 0) {
-    key_start[2 * ncell] = tot_a;
-    cstart[ncell] = tot_q;
+    key_start[2 * ncell] = carry_a;
+    cstart[ncell] = carry_q;
   }
 }
 
@@ -2061,9 +2063,11 @@ __device__ __forceinline__ void sk2_block(const Dev<R>& d, const Box<R>& b,
 template <typename R>
 __device__ __forceinline__ void skfin2_block(const Dev<R>& d, const Box<R>& b,
                                              const int* __restrict__ m, int K, int nchunks,
-                                             const Params& p, int vmode, int tile) {
+                                             const Params& p, int vmode, int tile,
+                                             char* buf) {
   constexpr int kSub = 16;  // threads per k (blockDim = kSkK * kSub)
-  __shared__ double s_s[kSub][2][kSkK];
+  // buf: LDS of kSub * 2 * kSkK doubles
+  auto s_s = reinterpret_cast<double (*)[2][kSkK]>(buf);
   const int t = (int)threadIdx.x;
   const int kk = t % kSkK, r = t / kSkK;
   const int k = tile * kSkK + kk;
@@ -2213,10 +2217,15 @@ __global__ __launch_bounds__(256) void k_bin_sk(Dev<R> d, Box<R> b, Grid g, int 
 template <typename R>
 __global__ __launch_bounds__(1024) void k_scan_fin(Dev<R> d, Box<R> b, int ncell,
                                                    unsigned thr, KsArgs ks) {
+  // one LDS buffer for either role of the block
+  constexpr size_t kScanBytes = (size_t)kScanPer * 1024 * (sizeof(int2) + sizeof(int));
+  constexpr size_t kFinBytes = (size_t)16 * 2 * kSkK * sizeof(double);
+  __shared__ __align__(16) char buf[kScanBytes > kFinBytes ? kScanBytes : kFinBytes];
   if (blockIdx.x == 0)
-    scan_block(d.key_count, d.key_start, d.cstart, ncell, d.ctrl, thr);
+    scan_block(d.key_count, d.key_start, d.cstart, ncell, d.ctrl, thr, buf);
   else
-    skfin2_block<R>(d, b, ks.m_fin, ks.K, ks.nchunks, ks.p, ks.vmode, (int)blockIdx.x - 1);
+    skfin2_block<R>(d, b, ks.m_fin, ks.K, ks.nchunks, ks.p, ks.vmode, (int)blockIdx.x - 1,
+                    buf);
 }
 
 // The sorted-order scatter (blocks [0, nsc)) and the RBE force (the blocks
