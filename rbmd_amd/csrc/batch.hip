@@ -212,6 +212,48 @@ __device__ __forceinline__ void batch_lj(const PairConst<float>& pc, const Box<f
   }
 }
 
+// batch_lj over the staged candidates [l0, l1) of one stencil row without
+// energies or special bonds, with no branch per candidate: the owner itself
+// and (for a charged owner) charged candidates are masked, the cutoffs and
+// the RBL weight are selects, and only a shell candidate (rare at an LJ cutoff
+// at or below r_core) forms its selection hash.  Per pair the same terms in
+// the same order as batch_lj.
+template <bool RBL, bool MI>
+__device__ __forceinline__ void batch_lj_row(const PairConst<float>& pc, const Box<float>& b,
+                                             const float* s_lj, int ti, const float4& xi,
+                                             unsigned long long hi, bool qi, int li,
+                                             const float4* sx, const int2* stg, int l0,
+                                             int l1, float& fx, float& fy, float& fz) {
+  const int row = 4 * ti * pc.ntypes;
+  for (int l = l0; l < l1; ++l) {
+    const float4 xj = sx[l];
+    const int2 tgj = stg[l];
+    float dx, dy, dz;
+    const float d2 = pair_d2<float, MI>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+    const bool in = (l != li) & !(qi & (xj.w != 0.f)) & (d2 < pc.rclj2) & (d2 < pc.rc2);
+    float w = 1.f;
+    if (RBL) {
+      const bool core = d2 < pc.t_core;
+      const float wc = ((d2 < pc.rs2) & (d2 > pc.eps_min)) ? 1.f : 0.f;
+      float wsh = 0.f;
+      if (in & !core & (d2 < pc.t_shell) & (d2 > pc.rs2))
+        wsh = rbl_keep(pc.ifreq, pc.fmask, pc.c32, pc.epoch, hi, rbl_hj(tgj.y)) ? pc.freq
+                                                                               : 0.f;
+      w = core ? wc : wsh;
+    }
+    const int tt = row + 4 * tgj.x;
+    const float m24 = s_lj[tt], sig2 = s_lj[tt + 1];
+    const float inv = pair_rcp(d2);
+    const float s2 = sig2 * inv;
+    const float s6 = s2 * s2 * s2;
+    const float flj = m24 * (2.f * s6 - 1.f) * s6 * inv;
+    const float f = (in & (w != 0.f)) ? w * flj : 0.f;
+    fx += f * dx;
+    fy += f * dy;
+    fz += f * dz;
+  }
+}
+
 // LJ + erfc Coulomb of a charged pair (interact_b's terms: eval_b, with the
 // owner's special-bond weight under CVFF; minimum image).
 template <bool RBL, bool EFLAG, bool MI, bool SPECIAL>
@@ -380,16 +422,27 @@ __global__ __launch_bounds__(256) void kb_pair(BatchDev d, PairConst<float> pc_a
         const int srow = (kz * h.rn[1] + ky) * h.rn[0];
         if (staged) {
           const int l0 = soff[srow + lo3[0]], l1 = soff[srow + hi3[0] + 1];
-          for (int l = l0; l < l1; ++l) {
-            const float4 xj = sx[l];
-            if (l == li || (qi && xj.w != 0.f)) continue;
-            const int2 tgj = stg[l];
+          if (!EFLAG && !SPECIAL) {
+            // the force pass: every candidate's terms as selects (the same
+            // pairs, weights and arithmetic as batch_lj)
             if (h.mi)
-              batch_lj<RBL, EFLAG, true, SPECIAL>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x,
-                                                  tgj.y, fx, fy, fz, a, sr);
+              batch_lj_row<RBL, true>(pc, b, s_lj, tgi.x, xi, hi, qi, li, sx, stg, l0, l1, fx,
+                                      fy, fz);
             else
-              batch_lj<RBL, EFLAG, false, SPECIAL>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x,
-                                                   tgj.y, fx, fy, fz, a, sr);
+              batch_lj_row<RBL, false>(pc, b, s_lj, tgi.x, xi, hi, qi, li, sx, stg, l0, l1,
+                                       fx, fy, fz);
+          } else {
+            for (int l = l0; l < l1; ++l) {
+              const float4 xj = sx[l];
+              if (l == li || (qi && xj.w != 0.f)) continue;
+              const int2 tgj = stg[l];
+              if (h.mi)
+                batch_lj<RBL, EFLAG, true, SPECIAL>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x,
+                                                    tgj.y, fx, fy, fz, a, sr);
+              else
+                batch_lj<RBL, EFLAG, false, SPECIAL>(pc, b, s_lj, tgi.x, xi, hi, xj, tgj.x,
+                                                     tgj.y, fx, fy, fz, a, sr);
+            }
           }
         } else {
           // beyond the stage: the same cells from global memory, minimum image
