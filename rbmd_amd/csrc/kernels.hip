@@ -2632,7 +2632,7 @@ hipError_t launch_export(const Dev<R>& d, const StepArgs<R>& a, int candidates,
 // fs = df/d sigma_ij.
 template <typename R>
 __device__ __forceinline__ void lj_vjp(const R* row, R u, R& f, R& fp, R& fs) {
-  const R inv = R(1) / u;
+  const R inv = pair_rcp(u);
   const R s2 = row[1] * inv;
   const R s6 = s2 * s2 * s2;
   f = row[0] * (R(2) * s6 - R(1)) * s6 * inv;
@@ -2647,22 +2647,25 @@ __device__ __forceinline__ void lj_vjp(const R* row, R u, R& f, R& fp, R& fs) {
 template <typename R>
 __device__ __forceinline__ void coul_vjp(const PairConst<R>& pc, R u, R& h,
                                          R& hp) {
+  // 1/r and 1/u from the hardware reciprocal square root and reciprocal
+  // (1 ulp) in FP32, as in the forward's pair terms
+  const R rinv = pair_rsqrt(u);
+  const R inv = pair_rcp(u);
   if (pc.sog) {
-    const R r3inv = R(1) / (sqrt(u) * u);
+    const R r3inv = rinv * inv;
     R poly = pc.T[5], dpoly = R(5) * pc.T[5];
 #pragma unroll
     for (int n = 4; n >= 0; --n) poly = poly * u + pc.T[n];
 #pragma unroll
     for (int n = 4; n >= 1; --n) dpoly = dpoly * u + R(n) * pc.T[n];
     h = r3inv + poly;
-    hp = R(-1.5) * r3inv / u + dpoly;
+    hp = R(-1.5) * r3inv * inv + dpoly;
     return;
   }
-  const R r = sqrt(u);
+  const R r = u * rinv;
   const R er = erfc(pc.g * r), ex = exp(-pc.alpha * u);
-  const R inv = R(1) / u;
-  h = er * inv / r + pc.two_g_over_sqrtpi * ex * inv;
-  hp = R(-1.5) * er * inv * inv / r -
+  h = er * inv * rinv + pc.two_g_over_sqrtpi * ex * inv;
+  hp = R(-1.5) * er * inv * inv * rinv -
        R(1.5) * pc.two_g_over_sqrtpi * ex * inv * inv -
        pc.alpha * pc.two_g_over_sqrtpi * ex * inv;
 }
@@ -2776,9 +2779,9 @@ __device__ __forceinline__ void vjp_pair(R dx, R dy, R dz, R u, const R* row,
   }
   if (coul) acc.dq -= (float)(qj * gqb * hd);
   if (coul && ws != R(1)) {
-    const R rinv = R(1) / sqrt(u);
+    const R rinv = pair_rsqrt(u);
     const R gc = (R(1) - ws) * pc.qqr2e * rinv * rinv * rinv;  // f_c / (q_i q_j)
-    const R fc = qi * qj * gc, fcp = R(-1.5) * fc / u;
+    const R fc = qi * qj * gc, fcp = R(-1.5) * fc * pair_rcp(u);
     const R kx = cji * gj[0] - cij * gi[0], ky = cji * gj[1] - cij * gi[1],
             kz = cji * gj[2] - cij * gi[2];
     const R kd = kx * dx + ky * dy + kz * dz;
@@ -2837,7 +2840,9 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
   }
   constexpr int sh = StageShift<R>::value;
   const int i0 = d.key_start[2 * h.cell0], i1 = d.key_start[2 * h.cell1];
-  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+  // the brick's atoms in the pair kernel's order (longest list first)
+  for (int k = (int)threadIdx.x; k < i1 - i0; k += blockDim.x) {
+    const int i = d.order_a[i0 + k];
     const R4T<R> xi = d.xs[i];
     const int ti = (int)xi.w;
     const R4T<R> g4 = v.gs[i];
@@ -2865,7 +2870,39 @@ __global__ __launch_bounds__(512) void k_pair_vjp(Dev<R> d, VjpDev<R> v, Box<R> 
       vjp_pair<R>(dx, dy, dz, u, lrow + 4 * tj, false, pc, R(0), R(0), wij, wji, gi,
                   gj, r, acc, ws);
     };
-    if (staged && cnt <= d.cap_a) {
+    // Without special bonds or list-A shell pairs, an entry's weight both
+    // ways is the force window a_lo < u < a_hi: the entries (and the
+    // sentinels that pad the last group) go without branches, a masked one
+    // with finite stand-ins (u = 1, G'_j = 0, weights 0) that add exact zeros.
+    const bool flat = !SPECIAL && (!RBL || !(pc.rclj2 > pc.rs2));
+    if (flat && staged && cnt <= d.cap_a) {
+      auto run = [&](auto mi) {
+        constexpr bool MI = decltype(mi)::value;
+        const uint4* lp = reinterpret_cast<const uint4*>(d.list_a) + i;
+        for (int k0 = 0; k0 < cnt; k0 += 8) {
+          const uint4 q8 = lp[(size_t)(k0 >> 3) * d.n];
+          const unsigned wv[4] = {q8.x, q8.y, q8.z, q8.w};
+#pragma unroll
+          for (int kk = 0; kk < 8; ++kk) {
+            const int l = (int)((wv[kk >> 1] >> (16 * (kk & 1))) & 0xffffu) >> sh;
+            const R4T<R> xj = stage[l];
+            const R4T<R> gq = gstage[l];
+            R dx, dy, dz;
+            const R u = pair_d2<R, MI>(b, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, dx, dy, dz);
+            const bool in = (u > pc.a_lo) & (u < pc.a_hi);
+            const R w = in ? R(1) : R(0);
+            const R gj[3] = {in ? gq.x : R(0), in ? gq.y : R(0), in ? gq.z : R(0)};
+            const int tj = pair_woff(xj.w) / (4 * (int)sizeof(R));
+            vjp_pair<R>(dx, dy, dz, in ? u : R(1), lrow + 4 * tj, false, pc, R(0), R(0), w,
+                        w, gi, gj, r, acc, R(1));
+          }
+        }
+      };
+      if (h.mi)
+        run(std::true_type());
+      else
+        run(std::false_type());
+    } else if (staged && cnt <= d.cap_a) {
       const uint4* lp = reinterpret_cast<const uint4*>(d.list_a) + i;
       for (int k0 = 0; k0 < cnt; k0 += 8) {
         const uint4 q8 = lp[(size_t)(k0 >> 3) * d.n];
@@ -3098,7 +3135,7 @@ __global__ __launch_bounds__(256) void k_vjp_qnc(Dev<R> d, VjpDev<R> v, Box<R> b
         if (SPECIAL) {
           const R ws = special_w<R>(sr, gidj, pc);
           if (ws != R(1)) {
-            const R rinv = R(1) / sqrt(u);
+            const R rinv = pair_rsqrt(u);
             const R gc = (R(1) - ws) * pc.qqr2e * rinv * rinv * rinv;
             const R ms = (wij != R(0) ? R(1) : R(0)) * (gi[0] * dx + gi[1] * dy + gi[2] * dz) -
                          (wji != R(0) ? R(1) : R(0)) * (gj[0] * dx + gj[1] * dy + gj[2] * dz);
