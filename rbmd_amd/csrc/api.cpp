@@ -1140,9 +1140,15 @@ class Engine : public EngineBase {
       // round of 5 waves per SIMD (contiguous runs of atoms per wave)
       grid_.psub = 1;
       grid_.qblocks = 0;
-      if (split_ && 2 * (long)nc > (long)n_)
+      grid_.dsplit = 1;
+      if (split_ && 2 * (long)nc > (long)n_) {
         grid_.qblocks = std::max(1, std::min((nc + kQWaves - 1) / kQWaves,
                                              num_cu_ * 4 * 5 / kQWaves));
+        // the list build's rounds of 64 charged atoms per cell, with margin
+        // for cells above the mean
+        const double qpc = (double)nc / grid_.ncell;
+        grid_.dsplit = std::max(1, std::min(16, (int)std::ceil(1.25 * qpc / 64.0)));
+      }
       // mostly neutral: one more wave per brick block takes the charged
       // pairs beside the list-A waves
       grid_.qwave = (split_ && grid_.qblocks == 0 && grid_.block <= 448) ? 64 : 0;

@@ -90,15 +90,18 @@ struct Grid {
   // blocks of the dense charged-pair kernel (one wave per charged atom) that
   // takes list B in mostly charged systems; 0: list B inside the pair kernel
   int qblocks = 0;
+  // blocks per cell of the dense charged-pair list build (64 charged atoms
+  // of the cell per block and round)
+  int dsplit = 1;
   const int* cell_of_coord = nullptr;  // [ncell] (z*ny+y)*nx+x -> cell id
   const int* coord_of_cell = nullptr;  // [ncell] cell id -> packed x|y<<10|z<<20
   const int* brick_cell0 = nullptr;    // [nbrick+1] first cell id of a brick
 };
 
-// Blocks of the charged-pair list build: one per cell in mostly charged
+// Blocks of the charged-pair list build: dsplit per cell in mostly charged
 // systems, else kChargedLanes lanes per charged atom.
 inline int list_b_count(const Grid& g, int nc) {
-  return g.qblocks ? g.ncell : list_b_blocks(nc, g.block);
+  return g.qblocks ? g.ncell * g.dsplit : list_b_blocks(nc, g.block);
 }
 
 template <typename R>

@@ -890,46 +890,60 @@ __device__ __forceinline__ void list_b_block(const Dev<R>& d, const Box<R>& b,
   }
 }
 
-// Charged-pair candidate list of a mostly charged system: one block per
-// cell, one lane per charged atom of the cell (rounds of blockDim atoms).
-// The charged atoms of the stencil cells are staged in LDS, blockDim at a
-// time, double-buffered (the next chunk's loads are in flight while the
-// current one is scanned) and padded to whole groups of 64 with far-away
-// sentinels.  Every lane tests 64 staged atoms at a time with broadcast
-// reads into an acceptance mask; the accepted ones are appended in order
-// (stencil-cell order, ascending within a cell) with their types in list_t,
-// by the wave one atom at a time with coalesced stores when some lane of the
-// wave accepted more than kDenseCoop of the 64 (near cells), else by every
-// lane for itself.  Across a periodic edge the pair vector takes the
-// stencil cell's shift by L, which is exactly the minimum-image shift of
-// each of its pairs when the dimension has at least 4 cells (both are one
-// subtraction of L from r_j - r_i); with fewer the minimum image is applied
-// per pair.
-constexpr int kDenseCoop = 24;
+// Charged-pair candidate list of a mostly charged system.  A cell's charged
+// atoms are taken 64 at a time by blocks of 256 threads, four lanes per atom
+// (g.dsplit blocks per cell, the cell's rounds of 64 atoms dealt over them).
+// The charged atoms of the stencil cells are staged in LDS, 256 at a time,
+// double-buffered (the next chunk's loads are in flight while the current
+// one is scanned) and padded to whole groups of 64 with far-away sentinels.
+// In each group of 64 staged atoms, lane q of an atom tests the staged atoms
+// 4v + q (sixteen; a wave reads four consecutive entries at a time) and the
+// atom's four lanes OR their bits into its 64-candidate acceptance mask.
+// The accepted atoms are appended in order (stencil-cell order, ascending
+// within a cell) with their types in list_t: by the wave, one atom at a time
+// with coalesced stores, when some atom of the wave accepted more than
+// kDenseCoop of the 64 (near cells), else by each lane for its own
+// candidates at their ranks in the mask.  Across a periodic edge the pair
+// vector takes the stencil cell's shift by L, which is exactly the
+// minimum-image shift of each of its pairs when the dimension has at least 4
+// cells (both are one subtraction of L from r_j - r_i; a stencil cell
+// without a shift subtracts nothing); with fewer the minimum image is
+// applied per pair.
+constexpr int kDenseCoop = 40;
+constexpr int kDenseBlock = 256;
 
-template <typename R, bool MI>
+// Bits 4v (v < NV) of the acceptance mask of one lane: candidate st[4v].
+template <typename R, bool MI, bool SHIFT, int NV>
 __device__ __forceinline__ unsigned long long dense_mask(const Box<R>& b,
                                                          const R4T<R>& xi,
                                                          const R4T<R>* st, R r2,
                                                          R sx, R sy, R sz) {
   unsigned long long m = 0;
 #pragma unroll
-  for (int u = 0; u < 64; ++u) {
-    const R4T<R> xj = st[u];
+  for (int v = 0; v < NV; ++v) {
+    const R4T<R> xj = st[4 * v];
     R dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
     if (MI) {
       dx = min_image<R>(dx, b.L[0], b.half[0]);
       dy = min_image<R>(dy, b.L[1], b.half[1]);
       dz = min_image<R>(dz, b.L[2], b.half[2]);
-    } else {
+    } else if (SHIFT) {
       dx -= sx;
       dy -= sy;
       dz -= sz;
     }
     const R d2 = fma_r(dz, dz, fma_r(dy, dy, mul_rn(dx, dx)));
-    m |= (d2 < r2 ? 1ull : 0ull) << u;
+    m |= (d2 < r2 ? 1ull : 0ull) << (4 * v);
   }
   return m;
+}
+
+// OR over the four lanes of each quad (DPP quad permutations [1,0,3,2] and
+// [2,3,0,1]).
+__device__ __forceinline__ unsigned quad_or(unsigned x) {
+  x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xf, 0xf, false);
+  x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xf, 0xf, false);
+  return x;
 }
 
 // The stencil cell `st` of a dense list-B block: its charged range, its
@@ -966,10 +980,10 @@ __device__ __forceinline__ DenseCell<R> dense_cell(const Grid& g, const Box<R>& 
 
 template <typename R>
 __device__ __forceinline__ void list_b_dense(const Dev<R>& d, const Box<R>& b,
-                                             const Grid& g, int cell, R r2, int cap,
-                                             int ntypes) {
-  __shared__ R4T<R> st_x[2][512];
-  __shared__ unsigned char st_t[2][512];
+                                             const Grid& g, int cell, int part, int blk,
+                                             R r2, int cap) {
+  __shared__ R4T<R> st_x[2][kDenseBlock];
+  __shared__ unsigned char st_t[2][kDenseBlock];
   const int cs = d.cstart[cell], ce = d.cstart[cell + 1];
   int cc[3];
   unpack_coord(g.coord_of_cell[cell], cc);
@@ -978,12 +992,10 @@ __device__ __forceinline__ void list_b_dense(const Dev<R>& d, const Box<R>& b,
                       make_span(cc[2], g.ext_b[2], g.nc[2])};
   const int nst = sp[0].count * sp[1].count * sp[2].count;
   const R far = sizeof(R) == 4 ? R(1e30) : R(1e200);
-  const int bd = (int)blockDim.x, t = (int)threadIdx.x, lane = t & 63;
-  // type bits (types < 32)
-  const int nbt = ntypes > 1 ? 32 - __builtin_clz((unsigned)(ntypes - 1)) : 0;
+  const int t = (int)threadIdx.x, lane = t & 63, q = t & 3;
   int mx = 0, sum = 0;
-  for (int i0 = cs; i0 < ce; i0 += bd) {
-    const int c = i0 + t;
+  for (int i0 = cs + 64 * part; i0 < ce; i0 += 64 * g.dsplit) {
+    const int c = i0 + (t >> 2);
     const bool act = c < ce;
     const R4T<R> xi = d.xqc[act ? c : cs];
     int* row = d.list_b + (size_t)(act ? c : cs) * cap;
@@ -1012,7 +1024,7 @@ __device__ __forceinline__ void list_b_dense(const Dev<R>& d, const Box<R>& b,
     }
     int buf = 0;
     while (st < nst) {
-      const int nj = min(dc.j1 - jb, bd);
+      const int nj = min(dc.j1 - jb, kDenseBlock);
       const int npad = (nj + 63) & ~63;
       const int cjb = jb;
       const DenseCell<R> cdc = dc;
@@ -1024,8 +1036,8 @@ __device__ __forceinline__ void list_b_dense(const Dev<R>& d, const Box<R>& b,
       }
       __syncthreads();
       // the next chunk's loads, in flight during the scan
-      if (jb + bd < dc.j1) {
-        jb += bd;
+      if (jb + kDenseBlock < dc.j1) {
+        jb += kDenseBlock;
       } else {
         ++st;
         jb = 0;
@@ -1037,82 +1049,90 @@ __device__ __forceinline__ void list_b_dense(const Dev<R>& d, const Box<R>& b,
       } else {
         px = make4<R>(far, far, far, R(0));
       }
+      const bool shift = cdc.s[0] != R(0) || cdc.s[1] != R(0) || cdc.s[2] != R(0);
       for (int k0 = 0; k0 < npad; k0 += 64) {
         const int tyl = st_t[buf][k0 + lane];  // the type of staged atom k0 + lane
         unsigned long long m = 0;
         if (act) {
-          m = cdc.mi ? dense_mask<R, true>(b, xi, st_x[buf] + k0, r2, R(0), R(0), R(0))
-                     : dense_mask<R, false>(b, xi, st_x[buf] + k0, r2, cdc.s[0], cdc.s[1],
-                                            cdc.s[2]);
-          const int self = c - (cjb + k0);
-          if (self >= 0 && self < 64) m &= ~(1ull << self);
+          const R4T<R>* sq = st_x[buf] + k0 + q;
+          auto test = [&](auto nv) {
+            constexpr int NV = decltype(nv)::value;
+            return cdc.mi ? dense_mask<R, true, false, NV>(b, xi, sq, r2, R(0), R(0), R(0))
+                   : shift ? dense_mask<R, false, true, NV>(b, xi, sq, r2, cdc.s[0],
+                                                            cdc.s[1], cdc.s[2])
+                           : dense_mask<R, false, false, NV>(b, xi, sq, r2, R(0), R(0),
+                                                             R(0));
+          };
+          // a last group of at most 32 atoms: the sentinels after them fail
+          // every test, so only the first 32 are tested
+          m = nj - k0 <= 32 ? test(std::integral_constant<int, 8>())
+                            : test(std::integral_constant<int, 16>());
+          m <<= q;
         }
+        m = ((unsigned long long)quad_or((unsigned)(m >> 32)) << 32) |
+            quad_or((unsigned)m);
+        const int self = c - (cjb + k0);
+        if (self >= 0 && self < 64) m &= ~(1ull << self);
         const int na = __popcll(m);
-        if (wave_max_i(na) > kDenseCoop) {
-          // many accepted: the wave writes each atom's entries of the chunk
+        if (__ballot(na > kDenseCoop) != 0ull) {
+          // many accepted: the wave writes each atom's entries of the group
           // with one coalesced store, lane k taking staged atom k0 + k
-          const int abase = i0 + (t & ~63);
-          for (int l = 0; l < 64; ++l) {
-            const unsigned lo = __builtin_amdgcn_readlane((unsigned)m, l);
-            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(m >> 32), l);
+          const int abase = __builtin_amdgcn_readfirstlane(i0 + ((t & ~63) >> 2));
+          for (int l = 0; l < 16; ++l) {
+            const unsigned lo = __builtin_amdgcn_readlane((unsigned)m, 4 * l);
+            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(m >> 32), 4 * l);
             if ((lo | hi) == 0u) continue;
             const unsigned long long ml = ((unsigned long long)hi << 32) | lo;
             if ((ml >> lane) & 1ull) {
-              const int r = __builtin_amdgcn_readlane(pos, l) +
-                            (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
-              if (r < cap) {
-                const size_t o = (size_t)(abase + l) * cap + r;
-                d.list_b[o] = cjb + k0 + lane;
-                d.list_t[o] = (unsigned char)tyl;
+              const unsigned r =
+                  (unsigned)__builtin_amdgcn_readlane(pos, 4 * l) +
+                  __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+              if (r < (unsigned)cap) {
+                const size_t o = (size_t)(abase + l) * cap;
+                d.list_b[o + r] = cjb + k0 + lane;
+                d.list_t[o + r] = (unsigned char)tyl;
               }
             }
           }
         } else {
-          // few: each lane appends its own, the types from per-chunk bit
-          // planes (one ballot per type bit)
-          unsigned long long pl[5];
-#pragma unroll
-          for (int q = 0; q < 5; ++q) pl[q] = q < nbt ? __ballot((tyl >> q) & 1) : 0ull;
-          int p = pos;
-          while (m) {
-            const int u = __builtin_ctzll(m);
-            m &= m - 1;
-            unsigned ty = 0;
-#pragma unroll
-            for (int q = 0; q < 5; ++q)
-              if (q < nbt) ty |= (unsigned)((pl[q] >> u) & 1ull) << q;
-            if (p < cap) {
-              row[p] = cjb + k0 + u;
-              trow[p] = (unsigned char)ty;
+          // few: lane q of an atom appends its candidates 4v + q at their
+          // ranks in the mask, with their staged types
+          unsigned long long mq = m & (0x1111111111111111ull << q);
+          while (mq) {
+            const int u = __builtin_ctzll(mq);
+            mq &= mq - 1;
+            const int r = pos + __popcll(m & ((1ull << u) - 1ull));
+            if (r < cap) {
+              row[r] = cjb + k0 + u;
+              trow[r] = st_t[buf][k0 + u];
             }
-            ++p;
           }
         }
         pos += na;
       }
       buf ^= 1;
     }
-    if (act) {
+    if (act && q == 0) {
       d.cnt_b[c] = pos;
       mx = max(mx, pos);
       sum += pos;
     }
   }
   // per-block statistics (folded by k_reduce)
-  __shared__ int s_red[2 * 8];
+  __shared__ int s_red[2 * (kDenseBlock / 64)];
   mx = wave_max_i(mx);
   sum = wave_sum(sum);
-  if ((threadIdx.x & 63) == 0) {
-    s_red[2 * (threadIdx.x >> 6)] = mx;
-    s_red[2 * (threadIdx.x >> 6) + 1] = sum;
+  if (lane == 0) {
+    s_red[2 * (t >> 6)] = mx;
+    s_red[2 * (t >> 6) + 1] = sum;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+  if (t == 0) {
+    for (int w = 1; w < kDenseBlock / 64; ++w) {
       mx = max(mx, s_red[2 * w]);
       sum += s_red[2 * w + 1];
     }
-    d.listb_part[cell] = make_int2(mx, sum);
+    d.listb_part[blk] = make_int2(mx, sum);
   }
 }
 
@@ -1134,10 +1154,12 @@ __global__ __launch_bounds__(512) void k_lists(Dev<R> d, Box<R> b, Grid g,
 }
 
 template <typename R>
-__global__ __launch_bounds__(512) void k_lists_dense(Dev<R> d, Box<R> b, Grid g, R r2b,
-                                                     int cap_b, unsigned thr, int ntypes) {
+__global__ __launch_bounds__(kDenseBlock) void k_lists_dense(Dev<R> d, Box<R> b, Grid g,
+                                                             R r2b, int cap_b, unsigned thr) {
   if (!rebuild_now(d.ctrl, thr)) return;
-  list_b_dense<R>(d, b, g, blockIdx.x, r2b, cap_b, ntypes);
+  // g.dsplit consecutive blocks per cell, each XCD a contiguous range of cells
+  const int blk = xcd_brick(blockIdx.x, gridDim.x);
+  list_b_dense<R>(d, b, g, blk / g.dsplit, blk % g.dsplit, blk, r2b, cap_b);
 }
 
 template <typename R>
@@ -2426,8 +2448,9 @@ hipError_t build_part(const Dev<R>& d, const StepArgs<R>& a, int nskin,
                      dim3(g.block), 0, s, d, a.box, g, a.pc.r2_list_a,
                      a.pc.r2_list_b, a.p.split, cap_a, cap_b, thr);
   if (dense)
-    hipLaunchKernelGGL((k_lists_dense<R>), dim3(g.ncell), dim3(g.block), 0, s, d, a.box, g,
-                       a.pc.r2_list_b, cap_b, thr, a.pc.ntypes);
+    hipLaunchKernelGGL((k_lists_dense<R>), dim3(g.ncell * g.dsplit), dim3(kDenseBlock), 0, s,
+                       d, a.box, g,
+                       a.pc.r2_list_b, cap_b, thr);
   return hipGetLastError();
 }
 
