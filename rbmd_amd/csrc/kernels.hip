@@ -547,6 +547,7 @@ __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
     } else if (len <= 512) {
       // rank sort by the whole wave (the values are distinct caller
       // indices): every lane ranks its up to 8 values against the segment,
+      // whose values come from the lanes' registers (no load per value),
       // and all ranks are known before the first store
       int v[8], r[8];
 #pragma unroll
@@ -555,11 +556,22 @@ __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
         v[u] = p < e ? perm[p] : 0x7fffffff;
         r[u] = 0;
       }
-      for (int q = a; q < e; ++q) {
-        const int w = perm[q];
+      auto rank = [&](auto nu) {
+        constexpr int NU = decltype(nu)::value;  // 64-value groups to rank
 #pragma unroll
-        for (int u = 0; u < 8; ++u) r[u] += w < v[u] ? 1 : 0;
-      }
+        for (int u2 = 0; u2 < NU; ++u2) {
+          const int nl = min(64, len - 64 * u2);
+          for (int l = 0; l < nl; ++l) {
+            const int w = __builtin_amdgcn_readlane(v[u2], l);
+#pragma unroll
+            for (int u = 0; u < NU; ++u) r[u] += w < v[u] ? 1 : 0;
+          }
+        }
+      };
+      if (len <= 256)
+        rank(std::integral_constant<int, 4>());
+      else
+        rank(std::integral_constant<int, 8>());
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int u = 0; u < 8; ++u)

@@ -122,3 +122,36 @@ def test_dense_water_four_cells_special(device):
     err = rel_rms(got, ref)
     print(f"water 4-cell grid: force rel RMS vs FP64 = {err:.3e}")
     assert err < 5e-5  # the CVFF water bound of test_gpu_cvff.py
+
+
+@pytest.mark.parametrize("nmol,cut,lo,hi", [(7600, 13.0, 257, 512), (5000, 13.5, 513, 1 << 20)],
+                         ids=["cells_257_512", "cells_over_512"])
+def test_dense_water_large_cells(device, nmol, cut, lo, hi):
+    """SPC/E water whose cells hold more than 256 atoms: 7,600 molecules (box
+    61 A) at rc 13 A + skin 2 A give 4 cells of 15.25 A per dimension, about
+    360 atoms each (the cell sort ranks eight groups of 64 per wave); 5,000
+    molecules (box 53 A) at rc 13.5 A give 3 cells of 17.7 A, about 560 atoms
+    each (sorted by one lane).  CVFF special bonds [0, 0, 0], RBL + RBE."""
+    import torch
+
+    s = systems.spce_water(nmol, seed=8)
+    rb = cut + 2.0  # the cell side bound, rc + skin (the LJ and Coulomb cutoffs agree)
+    nc = np.floor(np.asarray(s.box_len) / (rb * (1.0 + 1e-5))).astype(int)
+    cell = np.floor((s.pos - np.asarray(s.box_lo)) / (np.asarray(s.box_len) / nc)).astype(int)
+    cell = np.clip(cell, 0, nc - 1)
+    occ = np.bincount((cell[:, 2] * nc[1] + cell[:, 1]) * nc[0] + cell[:, 0])
+    assert lo <= occ.max() <= hi, (nc, occ.max())
+    cfg = systems.cvff_water_config(cut_off=cut, r_core=6.5, alpha=0.0784, P=300)
+    ctx = _ctx(s, cfg)
+    ctx.set_special(s.nspecial, s.special)
+    px, py, pz = _pos(s, device)
+    f = [torch.zeros(s.n, dtype=torch.float32, device=device) for _ in range(3)]
+    m = rbe_sample_host(0.0784, s.box_len, 300, 20261203, 0)
+    mt = torch.tensor(m.reshape(-1), dtype=torch.int32, device=device)
+    ctx.force(1, px, py, pz, *f, m_samples=mt, sync=True)
+    got = np.stack([t.double().cpu().numpy() for t in f], axis=1)
+    ref = reference_force(s, s.pos.astype(np.float32), cfg, 1, m)[0]
+    err = rel_rms(got, ref)
+    print(f"water {nmol} molecules, rc {cut}: largest cell {occ.max()}, "
+          f"force rel RMS vs FP64 = {err:.3e}")
+    assert err < 5e-5  # the CVFF water bound of test_gpu_cvff.py
