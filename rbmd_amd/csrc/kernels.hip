@@ -665,9 +665,11 @@ __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
     uint16_t* row = d.list_a + (size_t)i * 8;
     const size_t n8 = (size_t)d.n * 8;
     constexpr int sh = StageShift<R>::value;
+    size_t wo = 0;  // (count >> 3) * n8 + (count & 7), kept as entries are added
     auto accept = [&](int l) {
-      if (count < lcap) row[(size_t)(count >> 3) * n8 + (count & 7)] = (uint16_t)(l << sh);
+      if (count < lcap) row[wo] = (uint16_t)(l << sh);
       ++count;
+      wo += (count & 7) ? 1 : n8 - 7;
     };
     // Staged neighbourhoods scan a stencil row (its x cells are consecutive
     // slots, hence one contiguous stage range) as one run, 4 candidates at a
@@ -682,27 +684,33 @@ __device__ __forceinline__ void list_a_atoms(const Dev<R>& d, const Box<R>& b,
           const int l0 = t.off[s0], l1 = t.off[s0 + runlen];
           if (STAGED) {
             // chunks of up to 64 candidates: first the accept mask (no
-            // branches), then the accepted entries in candidate order
+            // branches; candidate cb + 4k + u sets bit 4k + u), then the
+            // accepted entries in candidate order
             for (int cb = l0; cb < l1; cb += 64) {
-              const int ce = min(l1, cb + 64);
+              const int ce = min(l1, cb + 64), nk = ce - cb;
               unsigned long long ok = 0;
-              for (int l = cb; l < ce; l += 4) {
+#pragma unroll
+              for (int k = 0; k < 16; ++k) {
+                const int l = cb + 4 * k;
+                if (l >= ce) continue;
                 R4T<R> xj[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) xj[u] = stage[min(l + u, ce - 1)];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                  const int lu = l + u;
                   R dx, dy, dz;
                   const R d2 = pair_d2<R, MI>(b, xi.x, xi.y, xi.z, xj[u].x, xj[u].y,
                                               xj[u].z, dx, dy, dz);
                   // charged-charged pairs belong to list B: the stage's .w
                   // (1e30 for a charged j) pushes them past the cutoff
-                  const bool keep = (lu < ce) & (fma_r(qf, xj[u].w, d2) < r2) &
-                                    (lu != self);
-                  ok |= (unsigned long long)keep << (lu - cb);
+                  ok |= (unsigned long long)(fma_r(qf, xj[u].w, d2) < r2) << (4 * k + u);
                 }
               }
+              // the bits past the chunk's end (their loads repeat its last
+              // candidate) and i itself are not entries
+              if (nk < 64) ok &= (1ull << nk) - 1ull;
+              const int so = self - cb;
+              if (so >= 0 && so < nk) ok &= ~(1ull << so);
               while (ok) {
                 const int u = __builtin_ctzll(ok);
                 ok &= ok - 1;
