@@ -516,10 +516,12 @@ __device__ __forceinline__ void scatter_block(const Dev<R>& d, unsigned thr, int
 }
 
 // One wave per cell: both key segments are sorted by caller index (bitonic
-// across the wave; serial by lane 0 beyond 64 atoms), then the cell's atoms
-// are gathered into the sorted tiles: positions (and the charged copy),
-// reference positions for the skin check, sorted ids, RBL hashes and cells,
-// the charged map, inv_perm, and the CVFF special-bond rows (ms per atom).
+// across the wave up to 64 atoms, a rank sort up to 512, serial by lane 0
+// beyond), then the cell's atoms are gathered into the sorted tiles:
+// positions (and the charged copy), reference positions for the skin check,
+// sorted ids, RBL hashes and cells, the charged map, inv_perm, and the CVFF
+// special-bond rows (ms per atom).  A cell of at most 64 atoms is gathered in
+// one round with the caller indices taken from the sort's registers.
 template <typename R>
 __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
                                                   int ncell, unsigned thr,
@@ -530,21 +532,31 @@ __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
   const int lane = threadIdx.x & 63;
   const int cell = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (cell >= ncell) return;
+  // the key starts (charged segment, neutral segment, end) and, for segments
+  // of at most 64 atoms, the sorted caller indices in lane order
+  int bnd[3];
+  int sv[2] = {0x7fffffff, 0x7fffffff};
+  bool small = true;
   for (int seg = 0; seg < 2; ++seg) {
     const int a = key_start[2 * cell + seg], e = key_start[2 * cell + seg + 1];
+    bnd[seg] = a;
+    bnd[seg + 1] = e;
     const int len = e - a;
-    if (len <= 1) continue;
     if (len <= 64) {
       int v = lane < len ? perm[a + lane] : 0x7fffffff;
-      for (int k = 2; k <= 64; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-          const int p = __shfl_xor(v, j, 64);
-          const bool up = (lane & k) == 0;
-          const bool lower = (lane & j) == 0;
-          v = (lower == up) ? min(v, p) : max(v, p);
-        }
-      if (lane < len) perm[a + lane] = v;
+      if (len > 1) {
+        for (int k = 2; k <= 64; k <<= 1)
+          for (int j = k >> 1; j > 0; j >>= 1) {
+            const int p = __shfl_xor(v, j, 64);
+            const bool up = (lane & k) == 0;
+            const bool lower = (lane & j) == 0;
+            v = (lower == up) ? min(v, p) : max(v, p);
+          }
+        if (lane < len) perm[a + lane] = v;
+      }
+      sv[seg] = v;
     } else if (len <= 512) {
+      small = false;
       // rank sort by the whole wave (the values are distinct caller
       // indices): every lane ranks its up to 8 values against the segment,
       // whose values come from the lanes' registers (no load per value),
@@ -576,26 +588,27 @@ __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
 #pragma unroll
       for (int u = 0; u < 8; ++u)
         if (a + lane + 64 * u < e) perm[a + r[u]] = v[u];
-    } else if (lane == 0) {
-      for (int p = a + 1; p < e; ++p) {
-        const int v = perm[p];
-        int q = p - 1;
-        while (q >= a && perm[q] > v) {
-          perm[q + 1] = perm[q];
-          --q;
+    } else {
+      small = false;
+      if (lane == 0) {
+        for (int p = a + 1; p < e; ++p) {
+          const int v = perm[p];
+          int q = p - 1;
+          while (q >= a && perm[q] > v) {
+            perm[q + 1] = perm[q];
+            --q;
+          }
+          perm[q + 1] = v;
         }
-        perm[q + 1] = v;
       }
     }
   }
   __threadfence_block();
   __builtin_amdgcn_wave_barrier();
   // key 2 cell (charged when the work is split) precedes key 2 cell + 1
-  const int a = key_start[2 * cell], aq = key_start[2 * cell + 1],
-            e = key_start[2 * cell + 2];
+  const int a = bnd[0], aq = bnd[1], e = bnd[2];
   const int cq = d.cstart[cell];
-  for (int s = a + lane; s < e; s += 64) {
-    const int i = perm[s];
+  auto gather = [&](int s, int i) {
     d.inv_perm[i] = s;
     const R x = wrap<R>(d.px[i], b.lo[0], b.hi[0], b.L[0], b.invL[0]);
     const R y = wrap<R>(d.py[i], b.lo[1], b.hi[1], b.L[1], b.invL[1]);
@@ -626,6 +639,16 @@ __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
     }
     // .w: the type, plus one half for a charged atom (as in k_skin)
     d.xs[s] = make4<R>(x, y, z, (R)t + (c >= 0 ? R(0.5) : R(0)));
+  };
+  if (small && e - a <= 64) {
+    // one round, the caller indices from the sort's registers: lane k takes
+    // slot a + k, the charged segment's k-th atom or the neutral one's
+    // (k - len0)-th
+    const int len0 = aq - a;
+    const int i1 = __shfl(sv[1], max(lane - len0, 0), 64);
+    if (lane < e - a) gather(a + lane, lane < len0 ? sv[0] : i1);
+  } else {
+    for (int s = a + lane; s < e; s += 64) gather(s, perm[s]);
   }
 }
 
