@@ -1144,7 +1144,7 @@ class Engine : public EngineBase {
       if (split_ && 2 * (long)nc > (long)n_) {
         grid_.qblocks = std::max(1, std::min((nc + kQWaves - 1) / kQWaves,
                                              num_cu_ * 4 * 5 / kQWaves));
-        // the list build's rounds of 64 charged atoms per cell, with margin
+        // the list build's groups of 64 charged atoms per cell, with margin
         // for cells above the mean
         const double qpc = (double)nc / grid_.ncell;
         grid_.dsplit = std::max(1, std::min(16, (int)std::ceil(1.25 * qpc / 64.0)));

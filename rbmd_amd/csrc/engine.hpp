@@ -91,7 +91,7 @@ struct Grid {
   // takes list B in mostly charged systems; 0: list B inside the pair kernel
   int qblocks = 0;
   // blocks per cell of the dense charged-pair list build (64 charged atoms
-  // of the cell per block and round)
+  // of the cell per block and pass)
   int dsplit = 1;
   const int* cell_of_coord = nullptr;  // [ncell] (z*ny+y)*nx+x -> cell id
   const int* coord_of_cell = nullptr;  // [ncell] cell id -> packed x|y<<10|z<<20

@@ -521,7 +521,7 @@ __device__ __forceinline__ void scatter_block(const Dev<R>& d, unsigned thr, int
 // positions (and the charged copy), reference positions for the skin check,
 // sorted ids, RBL hashes and cells, the charged map, inv_perm, and the CVFF
 // special-bond rows (ms per atom).  A cell of at most 64 atoms is gathered in
-// one round with the caller indices taken from the sort's registers.
+// one pass with the caller indices taken from the sort's registers.
 template <typename R>
 __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
                                                   int ncell, unsigned thr,
@@ -641,7 +641,7 @@ __global__ __launch_bounds__(256) void k_cellsort(Dev<R> d, Box<R> b,
     d.xs[s] = make4<R>(x, y, z, (R)t + (c >= 0 ? R(0.5) : R(0)));
   };
   if (small && e - a <= 64) {
-    // one round, the caller indices from the sort's registers: lane k takes
+    // one pass, the caller indices from the sort's registers: lane k takes
     // slot a + k, the charged segment's k-th atom or the neutral one's
     // (k - len0)-th
     const int len0 = aq - a;
@@ -935,7 +935,7 @@ __device__ __forceinline__ void list_b_block(const Dev<R>& d, const Box<R>& b,
 
 // Charged-pair candidate list of a mostly charged system.  A cell's charged
 // atoms are taken 64 at a time by blocks of 256 threads, four lanes per atom
-// (g.dsplit blocks per cell, the cell's rounds of 64 atoms dealt over them).
+// (g.dsplit blocks per cell, the cell's groups of 64 atoms dealt over them).
 // The charged atoms of the stencil cells are staged in LDS, 256 at a time,
 // double-buffered (the next chunk's loads are in flight while the current
 // one is scanned) and padded to whole groups of 64 with far-away sentinels.
