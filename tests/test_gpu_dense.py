@@ -155,3 +155,46 @@ def test_dense_water_large_cells(device, nmol, cut, lo, hi):
     print(f"water {nmol} molecules, rc {cut}: largest cell {occ.max()}, "
           f"force rel RMS vs FP64 = {err:.3e}")
     assert err < 5e-5  # the CVFF water bound of test_gpu_cvff.py
+
+
+def _slab():
+    """The all-charged 4,000-atom electrolyte in a box twice as tall (vacuum
+    above the atoms): 4 x 4 x 8 cells with a mean of 31 atoms, so each cell
+    has one block of the list build, while 19 filled cells hold more than the
+    64 atoms of one pass and take a second."""
+    s = systems.electrolyte(4000, frac_ion=0.5, seed=17)
+    box = np.array(s.box_len, dtype=np.float64)
+    box[2] *= 2.0
+    return systems.System(s.pos, s.box_lo, box, s.types, s.ids, s.charges, s.eps, s.sigma,
+                          name="electrolyte_4000_slab")
+
+
+def test_dense_slab_lists_and_forces(device):
+    import torch
+
+    s = _slab()
+    nc = np.floor(s.box_len / (3.8 * (1.0 + 1e-5))).astype(int)
+    cell = np.clip(np.floor((s.pos - s.box_lo) / (s.box_len / nc)).astype(int), 0, nc - 1)
+    occ = np.bincount((cell[:, 2] * nc[1] + cell[:, 1]) * nc[0] + cell[:, 0])
+    assert 1.25 * s.n / nc.prod() <= 64 and occ.max() > 64  # one block per cell, two passes
+    # candidate lists (rc + skin = 3.8, the cell side bound)
+    ctx = _ctx(s, systems.ljsalt_config(lj_cut_off=3.5))
+    px, py, pz = _pos(s, device)
+    ctx.export_pairs(0, px, py, pz, cap=1 << 23)
+    gi, gj, _ = ctx.export_candidates(cap=1 << 23)
+    ref = oracle.pairs_f32(s.pos.astype(np.float32), s.box_len, float(np.float32(3.8 * 3.8)))
+    got = set(zip(gi.tolist(), gj.tolist()))
+    assert len(got) == len(gi)
+    assert got == set(zip(ref[:, 0].tolist(), ref[:, 1].tolist()))
+    # forces, RBL + RBE, against the FP64 oracle
+    cfg = systems.ljsalt_config(lj_cut_off=3.5, sample_num=30, P=200)
+    ctx = _ctx(s, cfg)
+    f = [torch.zeros(s.n, dtype=torch.float32, device=device) for _ in range(3)]
+    m = rbe_sample_host(0.09, s.box_len, 200, 20261203, 0)
+    mt = torch.tensor(m.reshape(-1), dtype=torch.int32, device=device)
+    ctx.force(3, px, py, pz, *f, m_samples=mt, sync=True)
+    got_f = np.stack([t.double().cpu().numpy() for t in f], axis=1)
+    ref_f = reference_force(s, s.pos.astype(np.float32), cfg, 3, m)[0]
+    err = rel_rms(got_f, ref_f)
+    print(f"slab: largest cell {occ.max()}, force rel RMS vs FP64 = {err:.3e}")
+    assert err < 2e-5
